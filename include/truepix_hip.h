@@ -1,0 +1,122 @@
+/*
+ * truepix_hip.h - C ABI of libtruepix_hip.so, the MI355X (gfx950) GKR prover
+ * library for TruePix's fft_gkr / native_fp2 path.
+ *
+ * Conventions (same as the reference ABI):
+ *   - Fp2 values over p = 2^61 - 1 with i^2 = -1, every limb canonical [0, p).
+ *   - The native_fp2_* symbols take SoA arrays (re[], im[]) in host memory,
+ *     are synchronous, and serve single-threaded callers.  Wire tables are
+ *     [wire][copy] row-major; variables bind LSB-first (pairs 2i, 2i+1).
+ *   - tp_* symbols are this library's own; device pointers are marked d_.
+ *   - No host fallback: a device failure makes void entry points leave their
+ *     outputs unwritten and record a message for tp_last_error(); int entry
+ *     points return a negative value.
+ */
+#ifndef TRUEPIX_HIP_H
+#define TRUEPIX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- drop-in: TP/native_fp2/native_fp2.cu:824-1022 (CUDA ABI) and
+ *      TP/native_fp2/native_fp2.cpp:11-160 (CPU ABI), bound by
+ *      TP/libTruePix/native_fp2_bridge.py:46-113 -------------------------- */
+
+/* native_fp2.cu:826-829: init, idempotent; 0 on success (here: -1 without a device). */
+int native_fp2_init(void);
+
+/* native_fp2.cu:831-834: 1 when the device path is active. */
+int native_fp2_cuda_available(void);
+
+/* native_fp2.cu:836-848 / core.hpp:53-64: out[i] = in[2i] + r (in[2i+1] - in[2i]). */
+void native_fp2_fold(const uint64_t* in_real, const uint64_t* in_imag, size_t n_out,
+                     uint64_t r_real, uint64_t r_imag, uint64_t* out_real, uint64_t* out_imag);
+
+/* native_fp2.cu:850-862 / core.hpp:66-80: out[i*ncopies + j] = in[i]. */
+void native_fp2_expand(const uint64_t* in_real, const uint64_t* in_imag, size_t n_src,
+                       size_t ncopies, uint64_t* out_real, uint64_t* out_imag);
+
+/* native_fp2.cu:864-893 / core.hpp:82-137: early-round claim with explicit
+ * evaluation tables at -1 (fact0) and 2 (fact1); out8 = F(0), F(1), F(-1), F(2). */
+void native_fp2_sumcheck_early(int n_gates, const int* gate_type, const int* in0, const int* in1,
+                               const uint64_t* z1_real, const uint64_t* z1_imag, int n_copies,
+                               int n_wires, const uint64_t* values_real,
+                               const uint64_t* values_imag, const uint64_t* fact0_real,
+                               const uint64_t* fact0_imag, const uint64_t* fact1_real,
+                               const uint64_t* fact1_imag, const uint64_t* beta_real,
+                               const uint64_t* beta_imag, const uint64_t* beta_f0_real,
+                               const uint64_t* beta_f0_imag, const uint64_t* beta_f1_real,
+                               const uint64_t* beta_f1_imag, uint64_t out8[8]);
+
+/* native_fp2.cu:895-907 / core.hpp:139-159: beta[i] = init prod_j (bit_j(i) ? z_j : 1 - z_j). */
+void native_fp2_compute_beta(const uint64_t* z_real, const uint64_t* z_imag, int n_z,
+                             uint64_t init_real, uint64_t init_imag, uint64_t* out_real,
+                             uint64_t* out_imag);
+
+/* native_fp2.cu:909-921 / core.hpp:300-364: MLE of 2^n_z base-field inputs at z. */
+void native_fp2_mle_eval_base(const uint64_t* inputs, const uint64_t* z_real,
+                              const uint64_t* z_imag, int n_z, uint64_t* out_real,
+                              uint64_t* out_imag);
+
+/* native_fp2.cu:923-952 / core.hpp:173-198: resident early-round session;
+ * NULL on a bad shape (n_gates <= 0, n_wires <= 0, n_copies < 2 or not a
+ * power of two) or on device failure. */
+void* native_fp2_early_create(int n_gates, const int* gate_type, const int* in0, const int* in1,
+                              const uint64_t* z1_real, const uint64_t* z1_imag, int n_copies,
+                              int n_wires, const uint64_t* values_real,
+                              const uint64_t* values_imag, const uint64_t* beta_real,
+                              const uint64_t* beta_imag);
+
+/* native_fp2.cu:954-963 / core.hpp:200-248: claim at t = 0, 1, -1, 2; 0 or -1. */
+int native_fp2_early_claim(void* handle, uint64_t out8[8]);
+
+/* native_fp2.cu:965-974 / core.hpp:250-279: fold every wire row and beta with r. */
+int native_fp2_early_fold(void* handle, uint64_t r_real, uint64_t r_imag);
+
+/* native_fp2.cu:976-989 / core.hpp:281-296: element 0 of each wire row and of beta. */
+int native_fp2_early_finish(void* handle, uint64_t* wire_real, uint64_t* wire_imag,
+                            uint64_t* beta_real, uint64_t* beta_imag);
+
+/* native_fp2.cu:991-1005: current copy count / wire count (0 for NULL). */
+int native_fp2_early_n_copies(void* handle);
+int native_fp2_early_n_wires(void* handle);
+
+/* native_fp2.cu:1007-1020: frees the session (NULL is a no-op). */
+void native_fp2_early_destroy(void* handle);
+
+/* ---- library status (no reference counterpart) -------------------------- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* tp_last_error(void);
+void tp_clear_error(void);
+/* Number of visible HIP devices (does not initialize the library). */
+int tp_device_count(void);
+
+/* ---- degree-2 sumcheck: OZ/src/VPD/fft_circuit_GKR.cpp:171-209
+ *      (sumcheck_phase1_update; the addV term is zero) --------------------- */
+
+/* Prover over two resident device tables of 2^nv Fp2 values (interleaved re,
+ * im; read only).  Challenges come from the TruePix transcript
+ * (TP/libTruePix/fiatshamir.py) after put(["c2", seed]); round k absorbs
+ * put([a, b, c]) and draws r_k = rand_ext("c2-<k>"), unless `fixed` (2*nv
+ * limbs) is given.  Outputs: polys (nv x 6 limbs, (a, b, c) of
+ * s_k(X) = a X^2 + b X + c as in the reference's quadratic_poly), chals
+ * (nv x 2), final4 = f~(r), g~(r).  stream: hipStream_t or NULL.  0 / -1. */
+int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed,
+                 const uint64_t* fixed, uint64_t* polys, uint64_t* chals, uint64_t* final4,
+                 void* stream);
+/* Per-launch HIP-event timing of the fused fold+round kernel (off by default). */
+void tp_sc2_set_timing(int on);
+/* Summed event milliseconds, algorithmic bytes and launch count of the fused
+ * fold+round kernel in the last tp_sc2_prove. */
+void tp_sc2_stats(double* fold_ms, double* fold_bytes, int* fold_launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRUEPIX_HIP_H */

@@ -1,0 +1,201 @@
+"""ctypes loader for the CPU checker.  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker or as the timed CPU baseline.  The product package
+(truepix_amd) never imports it.
+
+liboracle_fp2.so is the restated oracle (oracle/fp2_oracle.c).  The optional
+oracle/_ref/*.so are the reference's own sources compiled in place by
+`make -C oracle ref` (they exist only where /root/reference does).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+P = (1 << 61) - 1
+
+_U64P = ctypes.POINTER(ctypes.c_uint64)
+_I32P = ctypes.POINTER(ctypes.c_int)
+_U64 = ctypes.c_uint64
+_SZ = ctypes.c_size_t
+_INT = ctypes.c_int
+
+
+def _p(a: np.ndarray):
+    """Pointer to a contiguous uint64 / int32 numpy array."""
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(_I32P)
+    assert a.dtype == np.uint64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_U64P)
+
+
+def _bind(lib, name, argtypes, restype=None):
+    fn = getattr(lib, name)
+    fn.argtypes = argtypes
+    fn.restype = restype
+    return fn
+
+
+class Oracle:
+    """Restated oracle: oracle/fp2_oracle.c."""
+
+    def __init__(self, path: str | None = None):
+        path = path or os.path.join(HERE, "liboracle_fp2.so")
+        if not os.path.isfile(path):
+            raise FileNotFoundError("oracle library not built: %s (run make -C oracle)" % path)
+        lib = ctypes.CDLL(path)
+        self.lib = lib
+        self.fp2_mul = _bind(lib, "or_fp2_mul", [_U64, _U64, _U64, _U64, _U64P])
+        self.fp2_inv = _bind(lib, "or_fp2_inv", [_U64, _U64, _U64P])
+        self.gatefn = _bind(lib, "or_gatefn", [_INT, _U64, _U64, _U64, _U64, _U64P])
+        self.fold = _bind(lib, "or_fold", [_U64P, _U64P, _SZ, _U64, _U64, _U64P, _U64P])
+        self.expand = _bind(lib, "or_expand", [_U64P, _U64P, _SZ, _SZ, _U64P, _U64P])
+        self.sumcheck_early = _bind(lib, "or_sumcheck_early",
+                                    [_INT, _I32P, _I32P, _I32P, _U64P, _U64P, _INT, _INT]
+                                    + [_U64P] * 12 + [_U64P])
+        self.compute_beta = _bind(lib, "or_compute_beta", [_U64P, _U64P, _INT, _U64, _U64, _U64P, _U64P])
+        self.mle_eval_base = _bind(lib, "or_mle_eval_base", [_U64P, _U64P, _U64P, _INT, _U64P, _U64P])
+        self.early_claim = _bind(lib, "or_early_claim",
+                                 [_INT, _I32P, _I32P, _I32P, _U64P, _U64P, _INT,
+                                  _U64P, _U64P, _U64P, _U64P, _U64P])
+        self.early_fold = _bind(lib, "or_early_fold",
+                                [_INT, _INT, _U64P, _U64P, _U64P, _U64P, _U64, _U64,
+                                 _U64P, _U64P, _U64P, _U64P])
+        self.sc2_round = _bind(lib, "or_sc2_round", [_U64P, _U64P, _SZ, _INT, _U64P])
+        self.sc2_fold = _bind(lib, "or_sc2_fold", [_U64P, _U64P, _SZ, _U64, _U64, _INT])
+        self.root_of_unity = _bind(lib, "or_root_of_unity", [_INT, _U64P])
+        self.fft_raw = _bind(lib, "or_fft", [_U64P, _INT, _INT, _U64P])
+
+    # -- numpy conveniences -------------------------------------------------
+    def mul(self, a, b):
+        out = (ctypes.c_uint64 * 2)()
+        self.fp2_mul(a[0], a[1], b[0], b[1], out)
+        return (out[0], out[1])
+
+    def root(self, log_order):
+        out = (ctypes.c_uint64 * 2)()
+        self.root_of_unity(log_order, out)
+        return (out[0], out[1])
+
+    def fft(self, coef: np.ndarray, log_coef: int, log_order: int) -> np.ndarray:
+        coef = np.ascontiguousarray(coef, dtype=np.uint64)
+        assert coef.size == 2 << log_coef
+        out = np.zeros(2 << log_order, dtype=np.uint64)
+        self.fft_raw(_p(coef), log_coef, log_order, _p(out))
+        return out
+
+    def sc2_prove(self, f: np.ndarray, g: np.ndarray, challenge_fn, nthreads: int = 1):
+        """Degree-2 sumcheck over interleaved tables; challenge_fn(round, poly6) -> (re, im).
+
+        Returns (polys [nv x 6], challenges [nv x 2], final [4])."""
+        f = np.array(f, dtype=np.uint64, copy=True)
+        g = np.array(g, dtype=np.uint64, copy=True)
+        n = f.size // 2
+        nv = n.bit_length() - 1
+        polys = np.zeros((nv, 6), dtype=np.uint64)
+        chals = np.zeros((nv, 2), dtype=np.uint64)
+        out6 = (ctypes.c_uint64 * 6)()
+        for r in range(nv):
+            self.sc2_round(_p(f), _p(g), n, nthreads, out6)
+            polys[r] = list(out6)
+            cr, ci = challenge_fn(r, tuple(out6))
+            chals[r] = (cr, ci)
+            self.sc2_fold(_p(f), _p(g), n, cr, ci, nthreads)
+            n //= 2
+        return polys, chals, np.array([f[0], f[1], g[0], g[1]], dtype=np.uint64)
+
+
+class RefNative:
+    """TP/native_fp2/native_fp2.cpp compiled from source into oracle/_ref."""
+
+    PATH = os.path.join(HERE, "_ref", "libnative_fp2_ref.so")
+
+    @classmethod
+    def available(cls) -> bool:
+        return os.path.isfile(cls.PATH)
+
+    def __init__(self):
+        lib = ctypes.CDLL(self.PATH)
+        self.lib = lib
+        self.fold = _bind(lib, "native_fp2_fold", [_U64P, _U64P, _SZ, _U64, _U64, _U64P, _U64P])
+        self.expand = _bind(lib, "native_fp2_expand", [_U64P, _U64P, _SZ, _SZ, _U64P, _U64P])
+        self.sumcheck_early = _bind(lib, "native_fp2_sumcheck_early",
+                                    [_INT, _I32P, _I32P, _I32P, _U64P, _U64P, _INT, _INT]
+                                    + [_U64P] * 12 + [_U64P])
+        self.compute_beta = _bind(lib, "native_fp2_compute_beta", [_U64P, _U64P, _INT, _U64, _U64, _U64P, _U64P])
+        self.mle_eval_base = _bind(lib, "native_fp2_mle_eval_base", [_U64P, _U64P, _U64P, _INT, _U64P, _U64P])
+        self.early_create = _bind(lib, "native_fp2_early_create",
+                                  [_INT, _I32P, _I32P, _I32P, _U64P, _U64P, _INT, _INT,
+                                   _U64P, _U64P, _U64P, _U64P], ctypes.c_void_p)
+        self.early_claim = _bind(lib, "native_fp2_early_claim", [ctypes.c_void_p, _U64P], _INT)
+        self.early_fold = _bind(lib, "native_fp2_early_fold", [ctypes.c_void_p, _U64, _U64], _INT)
+        self.early_finish = _bind(lib, "native_fp2_early_finish",
+                                  [ctypes.c_void_p, _U64P, _U64P, _U64P, _U64P], _INT)
+        self.early_destroy = _bind(lib, "native_fp2_early_destroy", [ctypes.c_void_p])
+
+
+class RefSumcheck:
+    """sumcheck_phase1_update from OZ/src/VPD/fft_circuit_GKR.cpp, compiled in place."""
+
+    PATH = os.path.join(HERE, "_ref", "libsumcheck_ref.so")
+
+    @classmethod
+    def available(cls) -> bool:
+        return os.path.isfile(cls.PATH)
+
+    def __init__(self):
+        lib = ctypes.CDLL(self.PATH)
+        self.sc2 = _bind(lib, "ref_sc2", [_U64P, _U64P, _INT, _U64P, _U64P, _U64P])
+
+    def prove(self, f, g, chals):
+        f = np.ascontiguousarray(f, dtype=np.uint64)
+        g = np.ascontiguousarray(g, dtype=np.uint64)
+        nv = (f.size // 2).bit_length() - 1
+        ch = np.ascontiguousarray(chals, dtype=np.uint64).reshape(-1)
+        polys = np.zeros(nv * 6, dtype=np.uint64)
+        fin = np.zeros(4, dtype=np.uint64)
+        self.sc2(_p(f), _p(g), nv, _p(ch), _p(polys), _p(fin))
+        return polys.reshape(nv, 6), fin
+
+
+class RefRS:
+    """fast_fourier_transform / get_root_of_unity from Orion, compiled in place."""
+
+    PATH = os.path.join(HERE, "_ref", "librs_ref.so")
+
+    @classmethod
+    def available(cls) -> bool:
+        return os.path.isfile(cls.PATH)
+
+    def __init__(self):
+        lib = ctypes.CDLL(self.PATH)
+        self.root_raw = _bind(lib, "ref_root_of_unity", [_INT, _U64P])
+        self.fft_raw = _bind(lib, "ref_fft", [_U64P, _INT, _INT, _U64P])
+
+    def root(self, log_order):
+        out = (ctypes.c_uint64 * 2)()
+        self.root_raw(log_order, out)
+        return (out[0], out[1])
+
+    def fft(self, coef, log_coef, log_order):
+        coef = np.ascontiguousarray(coef, dtype=np.uint64)
+        out = np.zeros(2 << log_order, dtype=np.uint64)
+        self.fft_raw(_p(coef), log_coef, log_order, _p(out))
+        return out
+
+
+def rand_fp2(rng: np.random.Generator, n: int) -> np.ndarray:
+    """n uniform canonical Fp2 elements, interleaved (re, im), by rejection."""
+    out = np.empty(2 * n, dtype=np.uint64)
+    filled = 0
+    while filled < 2 * n:
+        x = rng.integers(0, 1 << 61, size=2 * n - filled + 64, dtype=np.uint64)
+        x = x[x < P]
+        take = min(x.size, 2 * n - filled)
+        out[filled:filled + take] = x[:take]
+        filled += take
+    return out

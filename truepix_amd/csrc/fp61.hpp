@@ -1,0 +1,96 @@
+// fp61.hpp - arithmetic in Fp (p = 2^61 - 1) and Fp2 = Fp[i]/(i^2 + 1) for
+// CDNA4 (gfx950) device code and the host side of the library.
+//
+// Semantics follow the reference field (TP/native_fp2/fp61.cuh:14-166): every
+// public input and output limb is canonical in [0, p); i^2 = -1; products are
+// exact.  The multiply is written for a GPU with 32x32->64 multiply-add
+// (v_mad_u64_u32) and no 64x64 multiplier: operands are split into a 32-bit
+// low limb and a high limb of at most 30 bits, and the three partial products
+// are folded with 2^61 = 1 and 2^64 = 8 (mod p), so no 128-bit value is formed.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TP_HD __host__ __device__ __forceinline__
+
+namespace tp {
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+static constexpr u64 P = (1ULL << 61) - 1;
+
+TP_HD u64 canon(u64 x) { return x >= P ? x - P : x; }           // x < 2p
+TP_HD u64 fadd(u64 a, u64 b) { return canon(a + b); }
+TP_HD u64 fsub(u64 a, u64 b) { return canon(a + P - b); }
+TP_HD u64 fneg(u64 a) { return a ? P - a : 0; }
+TP_HD u64 fred64(u64 x) { return canon((x & P) + (x >> 61)); }  // any u64
+
+// a, b < 2^62 (so a canonical value plus another canonical value is allowed);
+// result canonical.
+TP_HD u64 fmul(u64 a, u64 b)
+{
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32);   // a1 < 2^30
+    const u32 b0 = (u32)b, b1 = (u32)(b >> 32);
+    const u64 lo = (u64)a0 * b0;                  // < 2^64
+    const u64 mid = (u64)a0 * b1 + (u64)a1 * b0;  // < 2^63
+    const u64 hi = (u64)a1 * b1;                  // < 2^60
+    // a*b = hi*2^64 + mid*2^32 + lo
+    //     = 8 hi + (mid >> 29) + ((mid mod 2^29) << 32) + (lo >> 61) + (lo mod 2^61)
+    const u64 s = (lo & P) + (lo >> 61) + ((mid & ((1ULL << 29) - 1)) << 32)
+                + (mid >> 29) + (hi << 3);        // < 2^64
+    return canon((s & P) + (s >> 61));
+}
+
+struct fp2 { u64 re, im; };
+
+TP_HD fp2 mk2(u64 r, u64 i) { fp2 x; x.re = r; x.im = i; return x; }
+TP_HD fp2 zero2() { return mk2(0, 0); }
+TP_HD fp2 one2() { return mk2(1, 0); }
+TP_HD fp2 add2(fp2 a, fp2 b) { return mk2(fadd(a.re, b.re), fadd(a.im, b.im)); }
+TP_HD fp2 sub2(fp2 a, fp2 b) { return mk2(fsub(a.re, b.re), fsub(a.im, b.im)); }
+TP_HD fp2 dbl2(fp2 a) { return add2(a, a); }
+TP_HD bool eq2(fp2 a, fp2 b) { return a.re == b.re && a.im == b.im; }
+
+// Karatsuba: re = ac - bd, im = (a + b)(c + d) - ac - bd.
+TP_HD fp2 mul2(fp2 x, fp2 y)
+{
+    const u64 ac = fmul(x.re, y.re);
+    const u64 bd = fmul(x.im, y.im);
+    const u64 k = fmul(x.re + x.im, y.re + y.im);   // operands < 2^62
+    return mk2(fsub(ac, bd), fsub(k, fadd(ac, bd)));
+}
+
+// Base-field scalar times Fp2.
+TP_HD fp2 smul2(u64 s, fp2 y) { return mk2(fmul(s, y.re), fmul(s, y.im)); }
+
+// a + r (b - a): the sumcheck fold (TP/native_fp2/native_fp2_core.hpp:53-64).
+TP_HD fp2 fold2(fp2 a, fp2 b, fp2 r) { return add2(a, mul2(r, sub2(b, a))); }
+
+// Gate functions, type indices as in TP/libTruePix/native_fp2_bridge.py:450-472
+// and TP/native_fp2/native_fp2_core.hpp:32-51.
+enum GateType {
+    G_MUL = 0, G_ADD = 1, G_SUB = 2, G_MUXL = 3, G_MUXR = 4, G_OR = 5, G_XOR = 6,
+    G_NOT = 7, G_NAND = 8, G_NOR = 9, G_NXOR = 10, G_NAAB = 11, G_PASS = 12
+};
+
+TP_HD fp2 gatefn(int tp, fp2 x, fp2 y)
+{
+    switch (tp) {
+    case G_MUL: return mul2(x, y);
+    case G_ADD: return add2(x, y);
+    case G_SUB: return sub2(x, y);
+    case G_MUXL: return x;
+    case G_MUXR: return y;
+    case G_OR: return sub2(add2(x, y), mul2(x, y));
+    case G_XOR: return sub2(add2(x, y), dbl2(mul2(x, y)));
+    case G_NOT: return sub2(one2(), x);
+    case G_NAND: return sub2(one2(), mul2(x, y));
+    case G_NOR: return sub2(add2(one2(), mul2(x, y)), add2(x, y));
+    case G_NXOR: return sub2(add2(one2(), dbl2(mul2(x, y))), add2(x, y));
+    case G_NAAB: return mul2(sub2(one2(), x), y);
+    default: return x;   // G_PASS and unknown types pass x, as the reference does
+    }
+}
+
+}  // namespace tp

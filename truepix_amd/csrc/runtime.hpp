@@ -1,0 +1,55 @@
+// runtime.hpp - device bring-up, error reporting and grow-only device buffers
+// shared by every translation unit of libtruepix_hip.so.
+//
+// Error model: the library never falls back to host arithmetic.  Entry points
+// that return int report failure with a negative value; entry points whose
+// reference signature returns void record the failure in a thread-local
+// message that tp_last_error() returns (the Python bridge checks it after
+// every call and raises).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace tp {
+
+// Records a formatted error for tp_last_error(); returns false for chaining.
+bool fail(const char* fmt, ...);
+void clear_error();
+
+// One-time device bring-up on device 0 (the reference uses device 0 only,
+// TP/native_fp2/native_fp2.cu:88-93).  Returns false if no HIP device.
+bool device_ready();
+
+// Library stream (default priority).  Every drop-in entry point enqueues on
+// it and synchronizes before returning, as the reference ABI is synchronous.
+hipStream_t stream();
+
+// Grow-only device allocation (malloc on grow, never shrinks).
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    bool ensure(size_t need);
+    void release();
+    template <typename T> T* as() const { return reinterpret_cast<T*>(ptr); }
+};
+
+// Kernel launch shapes: 256-thread workgroups (four wave64s per workgroup).
+static constexpr int kBlock = 256;
+inline unsigned grid_for(size_t n, size_t per_block = kBlock, unsigned cap = 1u << 20)
+{
+    size_t g = (n + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+}  // namespace tp
+
+#define TP_HIP_OK(call)                                                        \
+    do {                                                                       \
+        hipError_t tp_err_ = (call);                                           \
+        if (tp_err_ != hipSuccess)                                             \
+            return tp::fail("%s failed: %s (%s:%d)", #call,                    \
+                            hipGetErrorString(tp_err_), __FILE__, __LINE__);   \
+    } while (0)

@@ -1,0 +1,91 @@
+"""Host wrapper of the degree-2 sumcheck prover (tp_sc2_prove).
+
+The prover follows OZ/src/VPD/fft_circuit_GKR.cpp:171-209 (round polynomial
+(a, b, c) of sum_i A_i(X) V_i(X), fold with the round challenge) over two
+resident device tables; challenges come from the TruePix transcript seeded by
+put(["c2", seed]).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from truepix_amd import native_fp2_bridge as nb
+
+P = (1 << 61) - 1
+
+
+@dataclass
+class Deg2Proof:
+    polys: np.ndarray    # [nv, 6]: a.re a.im b.re b.im c.re c.im
+    chals: np.ndarray    # [nv, 2]
+    final: np.ndarray    # [4]: f~(r), g~(r)
+
+
+def synthetic_tables(nv: int, seed: int):
+    """Two tables of 2^nv uniform canonical Fp2 values (interleaved re, im),
+    drawn from numpy PCG64(seed) with rejection of the single value 2^61 - 1."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(2):
+        n = 2 << nv
+        x = rng.integers(0, 1 << 61, size=n, dtype=np.uint64)
+        bad = x == P
+        while bad.any():
+            x[bad] = rng.integers(0, 1 << 61, size=int(bad.sum()), dtype=np.uint64)
+            bad = x == P
+        out.append(x)
+    return out[0], out[1]
+
+
+def _u64p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+def prove_deg2_device(d_f: int, d_g: int, nv: int, seed: int = 0, fixed=None,
+                      stream: int = 0) -> Deg2Proof:
+    """Prove over device tables given as raw device addresses (16-B aligned)."""
+    nb.require()
+    polys = np.zeros((nv, 6), dtype=np.uint64)
+    chals = np.zeros((nv, 2), dtype=np.uint64)
+    final = np.zeros(4, dtype=np.uint64)
+    fx = None
+    if fixed is not None:
+        fx = np.ascontiguousarray(fixed, dtype=np.uint64).reshape(-1)
+        if fx.size != 2 * nv:
+            raise ValueError("fixed challenges need 2*nv limbs")
+    rc = nb.LIB.tp_sc2_prove(ctypes.c_void_p(d_f), ctypes.c_void_p(d_g), nv, seed,
+                             _u64p(fx) if fx is not None else None, _u64p(polys), _u64p(chals),
+                             _u64p(final), ctypes.c_void_p(stream) if stream else None)
+    if rc != 0:
+        raise RuntimeError("tp_sc2_prove failed: %s" % nb.last_error())
+    return Deg2Proof(polys, chals, final)
+
+
+def to_device(table: np.ndarray):
+    """Copy an interleaved uint64 table into a torch device tensor (int64 view)."""
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(table).view(np.int64)).to("cuda")
+    torch.cuda.synchronize()
+    return t
+
+
+def prove_deg2_host(f: np.ndarray, g: np.ndarray, seed: int = 0, fixed=None) -> Deg2Proof:
+    """Upload host tables, prove, return the proof (tables are not modified)."""
+    nv = (f.size // 2).bit_length() - 1
+    if f.size != 2 << nv or g.size != f.size:
+        raise ValueError("tables must hold 2^nv Fp2 values each")
+    df, dg = to_device(f), to_device(g)
+    return prove_deg2_device(df.data_ptr(), dg.data_ptr(), nv, seed, fixed)
+
+
+def set_timing(on: bool) -> None:
+    nb.LIB.tp_sc2_set_timing(1 if on else 0)
+
+
+def stats():
+    ms, by, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    nb.LIB.tp_sc2_stats(ctypes.byref(ms), ctypes.byref(by), ctypes.byref(n))
+    return ms.value, by.value, n.value
