@@ -115,6 +115,18 @@ void tp_sc2_set_timing(int on);
  * fold+round kernel in the last tp_sc2_prove. */
 void tp_sc2_stats(double* fold_ms, double* fold_bytes, int* fold_launches);
 
+/* ---- Reed-Solomon encoding: OZ/src/infrastructure/RS_polynomial.cpp:46-130
+ *      (fast_fourier_transform, with Orion's root of unity,
+ *      OZ/src/linear_gkr/prime_field.cpp:60-73) ------------------------------ */
+
+/* n_cols columns of 2^log_coef coefficients (device, interleaved re/im, column
+ * k at k*2^log_coef) -> 2^log_order natural-order evaluations per column
+ * (column k at k*2^log_order): out[k][i] = sum_j coef[k][j] w^(i j).
+ * cols_per_chunk: columns per pass pair (0 = 2).  log_coef <= 20,
+ * log_coef <= log_order <= 30.  stream: hipStream_t or NULL.  0 / -1. */
+int tp_rs_encode(const uint64_t* d_coef, int n_cols, int log_coef, int log_order,
+                 uint64_t* d_out, int cols_per_chunk, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

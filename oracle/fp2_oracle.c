@@ -361,3 +361,20 @@ void or_fft(const u64* coef, int log_coef, int log_order, u64* result)
     }
     free(buf);
 }
+
+/* Horner evaluation of sum_j c_j x^j (c interleaved, n coefficients): used by
+ * the tests to check single NTT outputs at full size. */
+void or_poly_eval(const u64* coef, size_t n, u64 xr, u64 xi, u64 out[2])
+{
+    fp2 x = mk(xr, xi), acc = {0, 0};
+    for (size_t j = n; j-- > 0;) acc = add2(mul2(acc, x), mk(coef[2 * j], coef[2 * j + 1]));
+    out[0] = acc.re; out[1] = acc.im;
+}
+
+/* x^e in Fp2 */
+void or_fp2_pow(u64 xr, u64 xi, u64 e, u64 out[2])
+{
+    fp2 r = {1, 0}, b = mk(xr, xi);
+    while (e) { if (e & 1) r = mul2(r, b); b = mul2(b, b); e >>= 1; }
+    out[0] = r.re; out[1] = r.im;
+}

@@ -69,6 +69,8 @@ class Oracle:
         self.sc2_fold = _bind(lib, "or_sc2_fold", [_U64P, _U64P, _SZ, _U64, _U64, _INT])
         self.root_of_unity = _bind(lib, "or_root_of_unity", [_INT, _U64P])
         self.fft_raw = _bind(lib, "or_fft", [_U64P, _INT, _INT, _U64P])
+        self.poly_eval_raw = _bind(lib, "or_poly_eval", [_U64P, _SZ, _U64, _U64, _U64P])
+        self.pow_raw = _bind(lib, "or_fp2_pow", [_U64, _U64, _U64, _U64P])
 
     # -- numpy conveniences -------------------------------------------------
     def mul(self, a, b):
@@ -79,6 +81,17 @@ class Oracle:
     def root(self, log_order):
         out = (ctypes.c_uint64 * 2)()
         self.root_of_unity(log_order, out)
+        return (out[0], out[1])
+
+    def pow(self, x, e):
+        out = (ctypes.c_uint64 * 2)()
+        self.pow_raw(x[0], x[1], e, out)
+        return (out[0], out[1])
+
+    def poly_eval(self, coef: np.ndarray, x):
+        coef = np.ascontiguousarray(coef, dtype=np.uint64)
+        out = (ctypes.c_uint64 * 2)()
+        self.poly_eval_raw(_p(coef), coef.size // 2, x[0], x[1], out)
         return (out[0], out[1])
 
     def fft(self, coef: np.ndarray, log_coef: int, log_order: int) -> np.ndarray:

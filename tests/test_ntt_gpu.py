@@ -1,0 +1,52 @@
+"""GPU parity of the Reed-Solomon NTT (tp_rs_encode) against the oracle:
+bit-exact full transforms at small and medium sizes, and at the benchmark
+size (64 x 2^20 -> 2^22) exact evaluations at sampled points (Horner) plus a
+full oracle comparison of two columns."""
+import numpy as np
+import pytest
+
+from oracle.oracle import rand_fp2
+from truepix_amd import ntt
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("log_coef,log_order,cols", [
+    (0, 0, 1), (0, 3, 2), (1, 1, 1), (2, 4, 3), (3, 3, 1), (5, 7, 2), (7, 7, 1),
+    (8, 10, 2), (10, 12, 2), (11, 13, 1), (12, 12, 1), (14, 16, 3), (17, 17, 1)])
+def test_rs_encode_vs_oracle(bridge, oracle, torch_cuda, log_coef, log_order, cols):
+    rng = np.random.default_rng(log_coef * 100 + log_order)
+    coef = rand_fp2(rng, cols << log_coef)
+    got = ntt.rs_encode_host(coef, cols, log_coef, log_order, cols_per_chunk=2)
+    m = 2 << log_coef
+    n = 2 << log_order
+    for c in range(cols):
+        want = oracle.fft(coef[c * m:(c + 1) * m], log_coef, log_order)
+        assert np.array_equal(got[c * n:(c + 1) * n], want), c
+
+
+def test_rs_encode_edge_coefficients(bridge, oracle, torch_cuda):
+    P = (1 << 61) - 1
+    coef = np.full(2 << 10, P - 1, dtype=np.uint64)
+    coef[::7] = 0
+    got = ntt.rs_encode_host(coef, 1, 10, 12)
+    assert np.array_equal(got, oracle.fft(coef, 10, 12))
+
+
+def test_rs_encode_benchmark_shape(bridge, oracle, torch_cuda):
+    """64 columns x 2^20 -> 2^22: sampled outputs of every column equal the
+    polynomial evaluated at w^i; two whole columns equal the oracle."""
+    cols, lc, lo = 64, 20, 22
+    rng = np.random.default_rng(2022)
+    coef = rand_fp2(rng, cols << lc)
+    got = ntt.rs_encode_host(coef, cols, lc, lo)
+    w = oracle.root(lo)
+    m, n = 2 << lc, 2 << lo
+    for c in range(cols):
+        col = coef[c * m:(c + 1) * m]
+        for i in [0, 1, (1 << lo) - 1, int(rng.integers(0, 1 << lo))]:
+            want = oracle.poly_eval(col, oracle.pow(w, i))
+            assert (int(got[c * n + 2 * i]), int(got[c * n + 2 * i + 1])) == want, (c, i)
+    for c in (0, cols - 1):
+        want = oracle.fft(coef[c * m:(c + 1) * m], lc, lo)
+        assert np.array_equal(got[c * n:(c + 1) * n], want), c

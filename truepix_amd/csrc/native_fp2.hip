@@ -7,8 +7,8 @@
 //  * every size runs on the GPU; there is no host fallback.  A device failure
 //    is reported through tp_last_error() (void entry points) or a negative
 //    return (int entry points) instead of silently recomputing on the host;
-//  * the early-round claim sums all four evaluation points in one launch with
-//    a grid-wide last-workgroup reduction (no second reduce kernel);
+//  * the early-round claim sums all four evaluation points in one launch
+//    (per-workgroup partials, then a one-workgroup finalize launch);
 //  * beta and the input MLE are built from LDS-resident half tables in one
 //    launch instead of n_z doubling launches.
 #include "fp61.hpp"
@@ -57,7 +57,7 @@ __global__ void k_early_claim(int n_gates, int gpt, const int* __restrict__ gt,
                               const u64* __restrict__ z1r, const u64* __restrict__ z1i, int nc,
                               const u64* __restrict__ vr, const u64* __restrict__ vi,
                               const u64* __restrict__ br, const u64* __restrict__ bi,
-                              fp2* partials, unsigned* counter, fp2* result)
+                              fp2* partials)
 {
     const int n_pairs = nc >> 1;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -84,7 +84,7 @@ __global__ void k_early_claim(int n_gates, int gpt, const int* __restrict__ gt,
         acc[2] = mul2(v2, sub2(dbl2(be0), be1));
         acc[3] = mul2(v3, sub2(dbl2(be1), be0));
     }
-    grid_sum<4>(acc, partials, counter, result);
+    block_partial<4>(acc, partials);
 }
 
 // Non-session early claim with explicit -1 / 2 tables
@@ -98,7 +98,7 @@ __global__ void k_sumcheck_early(int n_gates, int gpt, const int* __restrict__ g
                                  const u64* __restrict__ br, const u64* __restrict__ bi,
                                  const u64* __restrict__ bf0r, const u64* __restrict__ bf0i,
                                  const u64* __restrict__ bf1r, const u64* __restrict__ bf1i,
-                                 fp2* partials, unsigned* counter, fp2* result)
+                                 fp2* partials)
 {
     const int hn = nc >> 1;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -124,7 +124,7 @@ __global__ void k_sumcheck_early(int n_gates, int gpt, const int* __restrict__ g
         acc[2] = mul2(v2, mk2(bf0r[h], bf0i[h]));
         acc[3] = mul2(v3, mk2(bf1r[h], bf1i[h]));
     }
-    grid_sum<4>(acc, partials, counter, result);
+    block_partial<4>(acc, partials);
 }
 
 // Fold every wire row [n_wires][n_in] -> [n_wires][n_in/2] and beta in one
@@ -209,8 +209,7 @@ __global__ void k_beta(const u64* __restrict__ zr, const u64* __restrict__ zi, i
 // (TP/native_fp2/native_fp2_core.hpp:300-364): sum_x beta_z(x) * from_base(in[x]).
 // Rows of 2^lb inputs share the LDS low table; workgroups stride over rows.
 __global__ void k_mle_base(const u64* __restrict__ in, const u64* __restrict__ zr,
-                           const u64* __restrict__ zi, int n_z, fp2* partials,
-                           unsigned* counter, fp2* result)
+                           const u64* __restrict__ zi, int n_z, fp2* partials)
 {
     __shared__ fp2 lo[1024];
     __shared__ fp2 hrow;
@@ -229,7 +228,7 @@ __global__ void k_mle_base(const u64* __restrict__ in, const u64* __restrict__ z
         acc[0] = add2(acc[0], mul2(dot, hrow));
         __syncthreads();
     }
-    grid_sum<1>(acc, partials, counter, result);
+    block_partial<1>(acc, partials);
 }
 
 // ------------------------------------------------------------ host state --
@@ -239,28 +238,28 @@ struct Pool {
     DevBuf gt, i0, i1, z1r, z1i;                // gate tables
     DevBuf v_r, v_i, f0_r, f0_i, f1_r, f1_i;    // non-session sumcheck tables
     DevBuf be_r, be_i, bf0_r, bf0_i, bf1_r, bf1_i;
-    DevBuf red;                                 // partials + counter + result
+    DevBuf red;                                 // result + partials
     DevBuf mle_in, z_r, z_i;
 };
 Pool g_pool;
 
-// Reduction scratch: [counter (256 B)] [result (256 B)] [partials ...].
-struct RedView { unsigned* counter; fp2* result; fp2* partials; };
+// Reduction scratch: [result (256 B)] [partials ...].
+struct RedView { fp2* result; fp2* partials; };
 bool red_view(DevBuf& buf, size_t n_blocks, int n_acc, RedView& v)
 {
-    const bool fresh = buf.ptr == nullptr;
-    const size_t need = 512 + n_blocks * n_acc * sizeof(fp2);
-    if (need > buf.bytes) {
-        // Grow keeps the counter at zero: zero the whole new allocation.
-        if (!buf.ensure(need)) return false;
-        TP_HIP_OK(hipMemsetAsync(buf.ptr, 0, need, stream()));
-    } else if (fresh) {
-        TP_HIP_OK(hipMemsetAsync(buf.ptr, 0, need, stream()));
-    }
+    if (!buf.ensure(256 + n_blocks * n_acc * sizeof(fp2))) return false;
     char* base = buf.as<char>();
-    v.counter = reinterpret_cast<unsigned*>(base);
-    v.result = reinterpret_cast<fp2*>(base + 256);
-    v.partials = reinterpret_cast<fp2*>(base + 512);
+    v.result = reinterpret_cast<fp2*>(base);
+    v.partials = reinterpret_cast<fp2*>(base + 256);
+    return true;
+}
+
+// Finalize launch of a grid reduction with N accumulators.
+template <int N>
+bool finalize(const RedView& rv, size_t n_blocks)
+{
+    k_finalize_sum<N><<<1, kBlock, 0, stream()>>>(rv.partials, (unsigned)n_blocks, rv.result);
+    TP_HIP_OK(hipGetLastError());
     return true;
 }
 
@@ -344,9 +343,8 @@ bool gpu_mle(const u64* inputs, const u64* z_r, const u64* z_i, int n_z, u64* ou
     RedView rv;
     if (!red_view(P_.red, blocks, 1, rv)) return false;
     k_mle_base<<<blocks, kBlock, 0, stream()>>>(P_.mle_in.as<u64>(), P_.z_r.as<u64>(),
-                                                P_.z_i.as<u64>(), n_z, rv.partials, rv.counter,
-                                                rv.result);
-    if (!launch_ok()) return false;
+                                                P_.z_i.as<u64>(), n_z, rv.partials);
+    if (!launch_ok() || !finalize<1>(rv, blocks)) return false;
     fp2 res;
     TP_HIP_OK(hipMemcpyAsync(&res, rv.result, sizeof(fp2), hipMemcpyDeviceToHost, stream()));
     TP_HIP_OK(hipStreamSynchronize(stream()));
@@ -399,8 +397,8 @@ bool gpu_sumcheck_early(int n_gates, const int* gate_type, const int* in0, const
         P_.z1i.as<u64>(), n_copies, P_.v_r.as<u64>(), P_.v_i.as<u64>(), P_.f0_r.as<u64>(),
         P_.f0_i.as<u64>(), P_.f1_r.as<u64>(), P_.f1_i.as<u64>(), P_.be_r.as<u64>(),
         P_.be_i.as<u64>(), P_.bf0_r.as<u64>(), P_.bf0_i.as<u64>(), P_.bf1_r.as<u64>(),
-        P_.bf1_i.as<u64>(), rv.partials, rv.counter, rv.result);
-    if (!launch_ok()) return false;
+        P_.bf1_i.as<u64>(), rv.partials);
+    if (!launch_ok() || !finalize<4>(rv, (size_t)grid.x * grid.y)) return false;
     fp2 res[4];
     TP_HIP_OK(hipMemcpyAsync(res, rv.result, sizeof(res), hipMemcpyDeviceToHost, stream()));
     TP_HIP_OK(hipStreamSynchronize(stream()));
@@ -444,8 +442,8 @@ bool session_claim(EarlySession* s, u64 out8[8])
     k_early_claim<<<grid, kBlock, 0, stream()>>>(
         s->n_gates, gpt, s->gt.as<int>(), s->i0.as<int>(), s->i1.as<int>(), s->z1r.as<u64>(),
         s->z1i.as<u64>(), s->n_copies, s->cur_vr(), s->cur_vi(), s->cur_br(), s->cur_bi(),
-        rv.partials, rv.counter, rv.result);
-    if (!launch_ok()) return false;
+        rv.partials);
+    if (!launch_ok() || !finalize<4>(rv, (size_t)grid.x * grid.y)) return false;
     fp2 res[4];
     TP_HIP_OK(hipMemcpyAsync(res, rv.result, sizeof(res), hipMemcpyDeviceToHost, stream()));
     TP_HIP_OK(hipStreamSynchronize(stream()));

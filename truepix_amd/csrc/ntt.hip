@@ -1,0 +1,334 @@
+// ntt.hip - Reed-Solomon encoding by a two-pass, LDS-staged Fp2 NTT on CDNA4.
+//
+// Computes what OZ/src/infrastructure/RS_polynomial.cpp:46-130
+// (fast_fourier_transform) computes: for each column of M = 2^c coefficients,
+// the N = 2^o evaluations result[i] = sum_{j<M} c_j w^(i j), natural order,
+// w = the root of unity of order N from Orion's generator
+// g = 2^31 + 1033321771269002680 i (OZ/src/linear_gkr/prime_field.cpp:60-73),
+// w = g^(2^(62 - o)).  The reference replicates the coefficients N/M times
+// and runs c radix-2 Stockham stages over all N points; here the zero-padded
+// transform is split four-step style, N = (S M2) x M1 with S = N/M,
+// M = M1 M2, j = j1 + M1 j2, i = u + (S M2) q1:
+//   pass A (per column j1): Y[u][j1] = w^(j1 u) * sum_{j2<M2} c[j1 + M1 j2] w_{S M2}^(j2 u)
+//          computed as S cosets (u = s + S q2): twist by w_{S M2}^(s j2), then
+//          an M2-point DFT over j2 giving q2;
+//   pass B (per row u):    result[u + S M2 q1] = sum_{j1<M1} Y[u][j1] w_{M1}^(j1 q1).
+// Each pass stages 8 rows of at most 1024 points in LDS (8 x 1025 x 16 B) and
+// runs radix-4 (plus one radix-2) Stockham stages; the 8-row tiles make every
+// global load and store a 128-byte segment (8 consecutive Fp2).  Columns are
+// processed a few at a time so the intermediate stays in the 256 MB MALL.
+// Roots of order 4 are +-i in Fp2, so radix-4 butterflies multiply by w4
+// with a coordinate swap and one negation.
+#include "fp61.hpp"
+#include "runtime.hpp"
+
+#include <stdint.h>
+
+using namespace tp;
+
+namespace {
+
+struct alignas(16) V2 { u64 re, im; };
+
+__device__ __forceinline__ fp2 ldv(const V2* p) { const V2 v = *p; return mk2(v.re, v.im); }
+__device__ __forceinline__ void stv(V2* p, fp2 x) { V2 v; v.re = x.re; v.im = x.im; *p = v; }
+
+constexpr int kRows = 8;          // rows per LDS tile
+constexpr int kThreads = 512;     // threads per NTT workgroup
+constexpr int kMaxLog = 10;       // LDS DFT length up to 2^10
+constexpr int kRowStride = (1 << kMaxLog) + 1;   // +1 element pads LDS banks
+
+// Powers of the order-N root: w^e = TL[e & (2^h - 1)] * TH[e >> h].
+struct Tw {
+    const V2* t1024;   // w_1024^k, k < 1024
+    const V2* tl;      // w^e, e < 2^h
+    const V2* th;      // w^(e 2^h), e < 2^(o - h)
+    int h;
+    int w4_is_i;       // w_4 == +i (else -i)
+};
+
+__device__ __forceinline__ fp2 wpow(const Tw& t, u64 e)
+{
+    return mul2(ldv(t.tl + (e & ((1ULL << t.h) - 1))), ldv(t.th + (e >> t.h)));
+}
+
+__device__ __forceinline__ fp2 mul_w4(fp2 v, int is_i)
+{
+    // (a + b i) * i = -b + a i;  (a + b i) * (-i) = b - a i
+    return is_i ? mk2(fneg(v.im), v.re) : mk2(v.im, fneg(v.re));
+}
+
+// Forward DFT of `rows` rows of length 2^l held in LDS (row stride
+// kRowStride), Stockham auto-sort: stage with span Ns reads x[j + i L/R],
+// multiplies by w_{Ns R}^((j mod Ns) i), applies DFT_R and writes
+// y[(j / Ns) Ns R + (j mod Ns) + i Ns].  Values pass through registers with a
+// barrier between the read and write halves, so one LDS buffer suffices.
+__device__ void lds_dft(fp2* lds, int rows, int l, const Tw& tw)
+{
+    const int L = 1 << l;
+    int Ns = 1;
+    int log_ns = 0;
+    if (l & 1) {   // one radix-2 stage (twiddles of a span-1 stage are all 1)
+        const int nb = L >> 1, total = rows * nb;
+        constexpr int IT = (kRows << (kMaxLog - 1)) / kThreads;
+        fp2 v[IT][2];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int b = threadIdx.x + it * kThreads;
+            if (b < total) {
+                const int row = b >> (l - 1), j = b & (nb - 1);
+                const fp2* x = lds + row * kRowStride;
+                v[it][0] = x[j];
+                v[it][1] = x[j + nb];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int b = threadIdx.x + it * kThreads;
+            if (b < total) {
+                const int row = b >> (l - 1), j = b & (nb - 1);
+                fp2* y = lds + row * kRowStride;
+                y[2 * j] = add2(v[it][0], v[it][1]);
+                y[2 * j + 1] = sub2(v[it][0], v[it][1]);
+            }
+        }
+        __syncthreads();
+        Ns = 2;
+        log_ns = 1;
+    }
+    while (Ns < L) {
+        const int nb = L >> 2, total = rows * nb;
+        const int lnb = l - 2;
+        const int tw_shift = kMaxLog - (log_ns + 2);   // w_{4 Ns}^t = w_1024^(t << shift)
+        constexpr int IT = (kRows << (kMaxLog - 2)) / kThreads;
+        fp2 v[IT][4];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int b = threadIdx.x + it * kThreads;
+            if (b < total) {
+                const int row = b >> lnb, j = b & (nb - 1);
+                const fp2* x = lds + row * kRowStride;
+                fp2 a0 = x[j], a1 = x[j + nb], a2 = x[j + 2 * nb], a3 = x[j + 3 * nb];
+                const int k = j & (Ns - 1);
+                if (k) {
+                    a1 = mul2(a1, ldv(tw.t1024 + (k << tw_shift)));
+                    a2 = mul2(a2, ldv(tw.t1024 + ((2 * k) << tw_shift)));
+                    a3 = mul2(a3, ldv(tw.t1024 + ((3 * k) << tw_shift)));
+                }
+                const fp2 s02 = add2(a0, a2), d02 = sub2(a0, a2);
+                const fp2 s13 = add2(a1, a3), d13 = mul_w4(sub2(a1, a3), tw.w4_is_i);
+                v[it][0] = add2(s02, s13);
+                v[it][1] = add2(d02, d13);
+                v[it][2] = sub2(s02, s13);
+                v[it][3] = sub2(d02, d13);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int b = threadIdx.x + it * kThreads;
+            if (b < total) {
+                const int row = b >> lnb, j = b & (nb - 1);
+                const int k = j & (Ns - 1);
+                fp2* y = lds + row * kRowStride + ((j >> log_ns) << (log_ns + 2)) + k;
+                y[0] = v[it][0];
+                y[Ns] = v[it][1];
+                y[2 * Ns] = v[it][2];
+                y[3 * Ns] = v[it][3];
+            }
+        }
+        __syncthreads();
+        Ns <<= 2;
+        log_ns += 2;
+    }
+}
+
+// Pass A.  grid = (M1 / rows, S, columns); rows consecutive j1 per tile.
+__global__ void __launch_bounds__(kThreads) k_ntt_pass_a(const V2* __restrict__ coef, size_t coef_stride,
+                                                         V2* __restrict__ ybuf, size_t y_stride,
+                                                         int a, int b, int log_s, int log_rows, Tw tw)
+{
+    extern __shared__ fp2 lds[];
+    const int M1 = 1 << a, M2 = 1 << b, S = 1 << log_s;
+    const int rows = 1 << log_rows;
+    const int s = blockIdx.y;
+    const int j1_0 = blockIdx.x * rows;
+    const V2* src = coef + blockIdx.z * coef_stride;
+    V2* dst = ybuf + blockIdx.z * y_stride;
+    const int total = rows * M2;
+    // load + coset twist w_{S M2}^(s j2) = w_N^(M1 s j2)
+    for (int idx = threadIdx.x; idx < total; idx += kThreads) {
+        const int r = idx & (rows - 1), j2 = idx >> log_rows;
+        fp2 x = ldv(src + (size_t)(j1_0 + r) + (size_t)M1 * j2);
+        if (s && j2) x = mul2(x, wpow(tw, (u64)M1 * s * j2));
+        lds[r * kRowStride + j2] = x;
+    }
+    __syncthreads();
+    lds_dft(lds, rows, b, tw);
+    // twiddle w_N^(j1 u), u = s + S q2; store Y[u][j1]
+    for (int idx = threadIdx.x; idx < total; idx += kThreads) {
+        const int r = idx & (rows - 1), q2 = idx >> log_rows;
+        const u64 u = (u64)s + (u64)S * q2;
+        const u64 j1 = (u64)(j1_0 + r);
+        fp2 v = lds[r * kRowStride + q2];
+        if (j1 && u) v = mul2(v, wpow(tw, j1 * u));
+        stv(dst + u * M1 + j1, v);
+    }
+}
+
+// Pass B.  grid = (S M2 / rows, columns); rows consecutive u per tile.
+__global__ void __launch_bounds__(kThreads) k_ntt_pass_b(const V2* __restrict__ ybuf, size_t y_stride,
+                                                         V2* __restrict__ out, size_t out_stride,
+                                                         int a, int log_sm2, int log_rows, Tw tw)
+{
+    extern __shared__ fp2 lds[];
+    const int M1 = 1 << a;
+    const int rows = 1 << log_rows;
+    const size_t SM2 = (size_t)1 << log_sm2;
+    const size_t u0 = (size_t)blockIdx.x * rows;
+    const V2* src = ybuf + blockIdx.y * y_stride;
+    V2* dst = out + blockIdx.y * out_stride;
+    const int total = rows * M1;
+    for (int idx = threadIdx.x; idx < total; idx += kThreads) {
+        const int r = idx >> a, j1 = idx & (M1 - 1);
+        lds[r * kRowStride + j1] = ldv(src + (u0 + r) * M1 + j1);
+    }
+    __syncthreads();
+    lds_dft(lds, rows, a, tw);
+    for (int idx = threadIdx.x; idx < total; idx += kThreads) {
+        const int r = idx & (rows - 1), q1 = idx >> log_rows;
+        stv(dst + (u0 + r) + SM2 * q1, lds[r * kRowStride + q1]);
+    }
+}
+
+// Twiddle tables: thread e computes its entries by square-and-multiply.
+__global__ void k_ntt_tables(fp2 w1024, fp2 w, fp2 wh, int h, int log_hi, V2* t1024, V2* tl,
+                             V2* th)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    auto pw = [](fp2 base, u64 ex) {
+        fp2 r = one2();
+        while (ex) {
+            if (ex & 1) r = mul2(r, base);
+            base = mul2(base, base);
+            ex >>= 1;
+        }
+        return r;
+    };
+    if (e < 1024) stv(t1024 + e, pw(w1024, e));
+    if (e < (1 << h)) stv(tl + e, pw(w, e));
+    if (e < (1 << log_hi)) stv(th + e, pw(wh, e));
+}
+
+struct Plan {
+    int log_order = -1;
+    int h = 0;
+    int w4_is_i = 0;
+    DevBuf t1024, tl, th, ybuf;
+};
+Plan g_plan;
+
+fp2 host_pow2k(fp2 x, int k)   // x^(2^k)
+{
+    for (int i = 0; i < k; ++i) x = mul2(x, x);
+    return x;
+}
+
+bool plan_for(int log_order, hipStream_t st)
+{
+    Plan& P_ = g_plan;
+    if (P_.log_order == log_order) return true;
+    const fp2 g = mk2(2147483648ULL, 1033321771269002680ULL);   // order 2^62
+    const fp2 w = host_pow2k(g, 62 - log_order);
+    const int h = (log_order + 1) / 2, log_hi = log_order - h;
+    const fp2 wh = host_pow2k(w, h);
+    const fp2 w1024 = host_pow2k(g, 62 - 10);
+    const fp2 w4 = host_pow2k(g, 60);
+    P_.w4_is_i = (w4.re == 0 && w4.im == 1) ? 1 : 0;
+    if (!P_.w4_is_i && !(w4.re == 0 && w4.im == P - 1))
+        return fail("ntt: fourth root of unity is not +-i");
+    if (!P_.t1024.ensure(1024 * sizeof(V2)) || !P_.tl.ensure(((size_t)1 << h) * sizeof(V2)) ||
+        !P_.th.ensure(((size_t)1 << log_hi) * sizeof(V2)))
+        return false;
+    const int n = 1 << (h > 10 ? h : 10);
+    k_ntt_tables<<<(n + 255) / 256, 256, 0, st>>>(w1024, w, wh, h, log_hi, P_.t1024.as<V2>(),
+                                                  P_.tl.as<V2>(), P_.th.as<V2>());
+    TP_HIP_OK(hipGetLastError());
+    P_.log_order = log_order;
+    P_.h = h;
+    return true;
+}
+
+size_t lds_bytes(int rows) { return (size_t)rows * kRowStride * sizeof(fp2); }
+
+}  // namespace
+
+extern "C" {
+
+// Reed-Solomon encode n_cols columns on the device.
+//   d_coef : n_cols x 2^log_coef Fp2 (interleaved re, im), column-major
+//            (column k starts at k * 2^log_coef).
+//   d_out  : n_cols x 2^log_order Fp2, column k at k * 2^log_order;
+//            out[k][i] = sum_j coef[k][j] w^(i j), w of order 2^log_order.
+//   cols_per_chunk : columns per pass-A/pass-B pair (0 = 2); the intermediate
+//            holds cols_per_chunk x 2^log_order Fp2.
+// 0 <= log_coef <= 20, log_coef <= log_order <= 30.  Returns 0 / -1.
+int tp_rs_encode(const uint64_t* d_coef, int n_cols, int log_coef, int log_order,
+                 uint64_t* d_out, int cols_per_chunk, void* stream_handle)
+{
+    tp::clear_error();
+    if (n_cols < 1 || log_coef < 0 || log_coef > 2 * kMaxLog || log_order < log_coef ||
+        log_order > 30)
+        return tp::fail("tp_rs_encode: unsupported shape (cols=%d log_coef=%d log_order=%d)",
+                        n_cols, log_coef, log_order), -1;
+    if (!tp::device_ready()) return -1;
+    hipStream_t st = stream_handle ? (hipStream_t)stream_handle : tp::stream();
+    if (!plan_for(log_order, st)) return -1;
+    Plan& P_ = g_plan;
+    const int a = log_coef / 2, b = log_coef - a, log_s = log_order - log_coef;
+    if (b > kMaxLog || a > kMaxLog)
+        return tp::fail("tp_rs_encode: log_coef=%d too large", log_coef), -1;
+    const int chunk = cols_per_chunk > 0 ? cols_per_chunk : 2;
+    const size_t N = (size_t)1 << log_order, M = (size_t)1 << log_coef;
+    if (!P_.ybuf.ensure((size_t)chunk * N * sizeof(V2))) return -1;
+    static bool attr_done = false;
+    if (!attr_done) {
+        TP_HIP_OK(hipFuncSetAttribute((const void*)k_ntt_pass_a,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds_bytes(kRows)));
+        TP_HIP_OK(hipFuncSetAttribute((const void*)k_ntt_pass_b,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds_bytes(kRows)));
+        attr_done = true;
+    }
+    Tw tw;
+    tw.t1024 = P_.t1024.as<V2>();
+    tw.tl = P_.tl.as<V2>();
+    tw.th = P_.th.as<V2>();
+    tw.h = P_.h;
+    tw.w4_is_i = P_.w4_is_i;
+    const int M1 = 1 << a;
+    const size_t SM2 = (size_t)1 << (log_s + b);
+    const int rows_a = M1 < kRows ? M1 : kRows;
+    const int rows_b = SM2 < (size_t)kRows ? (int)SM2 : kRows;
+    const int lra = __builtin_ctz(rows_a), lrb = __builtin_ctz(rows_b);
+    const V2* coef = reinterpret_cast<const V2*>(d_coef);
+    V2* out = reinterpret_cast<V2*>(d_out);
+    for (int c0 = 0; c0 < n_cols; c0 += chunk) {
+        const int nc = (n_cols - c0) < chunk ? (n_cols - c0) : chunk;
+        dim3 ga((unsigned)(M1 / rows_a), 1u << log_s, (unsigned)nc);
+        k_ntt_pass_a<<<ga, kThreads, lds_bytes(rows_a), st>>>(coef + (size_t)c0 * M, M,
+                                                               P_.ybuf.as<V2>(), N, a, b, log_s,
+                                                               lra, tw);
+        TP_HIP_OK(hipGetLastError());
+        dim3 gb((unsigned)(SM2 / rows_b), (unsigned)nc);
+        k_ntt_pass_b<<<gb, kThreads, lds_bytes(rows_b), st>>>(P_.ybuf.as<V2>(), N,
+                                                               out + (size_t)c0 * N, N, a,
+                                                               log_s + b, lrb, tw);
+        TP_HIP_OK(hipGetLastError());
+    }
+    return 0;
+}
+
+}  // extern "C"

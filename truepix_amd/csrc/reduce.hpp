@@ -1,10 +1,14 @@
-// reduce.hpp - wave64 / workgroup / grid reductions of Fp2 accumulators.
+// reduce.hpp - wave64 / workgroup reductions of Fp2 accumulators and the
+// two-launch grid reduction.
 //
 // Fp2 addition of canonical values is exact and associative, so every
 // reduction order yields the same bits; the order here is chosen for CDNA4:
-// xor-shuffles inside each wave64, one LDS slot per wave, then a grid-wide
-// "last workgroup finishes" step so a round sum needs one launch and no
-// separate reduce kernel.
+// xor-shuffles inside each wave64, one LDS slot per wave, one partial per
+// workgroup, then a one-workgroup finalize launch.  Cross-workgroup
+// visibility comes from the kernel boundary: an in-kernel "last workgroup"
+// scheme needs agent-scope release/acquire fences, which on gfx950 write back
+// and invalidate the whole XCD L2 (buffer_wbl2 / buffer_inv) in every
+// workgroup and throttle a streaming kernel.
 #pragma once
 #include "fp61.hpp"
 #include "runtime.hpp"
@@ -25,13 +29,15 @@ __device__ __forceinline__ void wave_sum(fp2 (&v)[N])
     }
 }
 
-// Sums v over the workgroup (blockDim.x == kBlock); the total lands in thread 0.
+// Sums v over the workgroup (blockDim.x a multiple of 64, at most 1024); the
+// total lands in thread 0.
 template <int N>
 __device__ __forceinline__ void block_sum(fp2 (&v)[N])
 {
-    __shared__ fp2 sh[N][kBlock / 64];
+    __shared__ fp2 sh[N][16];
     wave_sum<N>(v);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
     __syncthreads();
     if (lane == 0) {
 #pragma unroll
@@ -42,50 +48,55 @@ __device__ __forceinline__ void block_sum(fp2 (&v)[N])
 #pragma unroll
         for (int k = 0; k < N; ++k) {
             fp2 s = sh[k][0];
-            for (int w = 1; w < kBlock / 64; ++w) s = add2(s, sh[k][w]);
+            for (int w = 1; w < nw; ++w) s = add2(s, sh[k][w]);
             v[k] = s;
         }
     }
 }
 
-// Grid reduction in one launch.  Every workgroup stores its sum in
-// partials[block * N + k]; the workgroup that arrives last (agent-scope
-// fences around a device-wide counter make every XCD's stores visible) sums
-// all partials, writes result[0..N), and re-arms the counter for the next
-// launch.  Returns true in the thread that wrote the result.
-template <int N>
-__device__ __forceinline__ bool grid_sum(fp2 (&v)[N], fp2* partials, unsigned* counter,
-                                         fp2* result)
+__device__ __forceinline__ unsigned linear_block_id()
 {
-    __shared__ bool last;
-    const unsigned nblk = gridDim.x * gridDim.y * gridDim.z;
-    const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+}
+
+// First half of a grid reduction: the workgroup total goes to
+// partials[block * N + k].
+template <int N>
+__device__ __forceinline__ void block_partial(fp2 (&v)[N], fp2* partials)
+{
     block_sum<N>(v);
     if (threadIdx.x == 0) {
+        const size_t b = linear_block_id();
 #pragma unroll
-        for (int k = 0; k < N; ++k) partials[(size_t)bid * N + k] = v[k];
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        last = (prev == nblk - 1);
+        for (int k = 0; k < N; ++k) partials[b * N + k] = v[k];
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    fp2 acc[N];
+}
+
+// Sums n_blocks partials of N accumulators inside one workgroup; the total
+// lands in thread 0's acc.
+template <int N>
+__device__ __forceinline__ void sum_partials(const fp2* partials, unsigned n_blocks, fp2 (&acc)[N])
+{
 #pragma unroll
     for (int k = 0; k < N; ++k) acc[k] = zero2();
-    for (unsigned b = threadIdx.x; b < nblk; b += blockDim.x) {
+    for (unsigned b = threadIdx.x; b < n_blocks; b += blockDim.x) {
 #pragma unroll
         for (int k = 0; k < N; ++k) acc[k] = add2(acc[k], partials[(size_t)b * N + k]);
     }
     block_sum<N>(acc);
+}
+
+// Second half: one workgroup of kBlock threads writes result[0..N).
+template <int N>
+__global__ void __launch_bounds__(kBlock) k_finalize_sum(const fp2* partials, unsigned n_blocks,
+                                                         fp2* result)
+{
+    fp2 acc[N];
+    sum_partials<N>(partials, n_blocks, acc);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < N; ++k) result[k] = acc[k];
-        atomicExch(counter, 0u);
-        return true;
     }
-    return false;
 }
 
 }  // namespace tp

@@ -12,12 +12,20 @@
 // Bytes per fused launch: 32 m read + 16 m written (both tables), the first
 // launch reads 32 n and writes nothing.  Tables are AoS {re, im} so every
 // element is one 128-bit load; a thread owns one output pair (64 B per table).
+//
+// Round trip per round: fused launch -> one-workgroup finalize launch that
+// writes (E0, E1, EA) and a sequence number straight into mapped pinned host
+// memory -> the host polls the sequence number, runs the transcript, and
+// launches the next round with the challenge as a kernel argument.
 #include "fp61.hpp"
 #include "reduce.hpp"
 #include "runtime.hpp"
 #include "transcript.hpp"
 
+#include <atomic>
+#include <chrono>
 #include <stdint.h>
+#include <thread>
 
 using namespace tp;
 
@@ -38,8 +46,7 @@ __device__ __forceinline__ void accum_pair(fp2 (&e)[3], fp2 f0, fp2 f1, fp2 g0, 
 // Round 0: no fold, sum over the n/2 pairs of the input tables.
 __global__ void __launch_bounds__(256) k_sc2_first(const V2* __restrict__ f,
                                                    const V2* __restrict__ g, size_t n_pairs,
-                                                   fp2* partials, unsigned* counter,
-                                                   fp2* result)
+                                                   fp2* partials)
 {
     fp2 e[3] = {zero2(), zero2(), zero2()};
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n_pairs;
@@ -48,7 +55,7 @@ __global__ void __launch_bounds__(256) k_sc2_first(const V2* __restrict__ f,
         const fp2 g0 = ldv(g + 2 * p), g1 = ldv(g + 2 * p + 1);
         accum_pair(e, f0, f1, g0, g1);
     }
-    grid_sum<3>(e, partials, counter, result);
+    block_partial<3>(e, partials);
 }
 
 // Round k >= 1: fold T_{k-1} (length 4 * n_quads) with r into T_k and sum
@@ -57,8 +64,7 @@ __global__ void __launch_bounds__(256) k_sc2_fold_round(const V2* __restrict__ f
                                                         const V2* __restrict__ gin,
                                                         size_t n_quads, u64 rr, u64 ri,
                                                         V2* __restrict__ fout,
-                                                        V2* __restrict__ gout, fp2* partials,
-                                                        unsigned* counter, fp2* result)
+                                                        V2* __restrict__ gout, fp2* partials)
 {
     const fp2 r = mk2(rr, ri);
     fp2 e[3] = {zero2(), zero2(), zero2()};
@@ -76,48 +82,75 @@ __global__ void __launch_bounds__(256) k_sc2_fold_round(const V2* __restrict__ f
         stv(gout + 2 * q + 1, g1);
         accum_pair(e, f0, f1, g0, g1);
     }
-    grid_sum<3>(e, partials, counter, result);
+    block_partial<3>(e, partials);
 }
 
-// Final fold of the last pair: f~(r), g~(r).
+// One workgroup: sum the partials and publish (E0, E1, EA) plus the sequence
+// number `seq` in host-mapped memory (values first, then a system-scope fence,
+// then the sequence number).
+__global__ void __launch_bounds__(256) k_sc2_publish(const fp2* partials, unsigned n_blocks,
+                                                     fp2* host_vals, volatile unsigned* host_seq,
+                                                     unsigned seq)
+{
+    fp2 acc[3];
+    sum_partials<3>(partials, n_blocks, acc);
+    if (threadIdx.x == 0) {
+        host_vals[0] = acc[0];
+        host_vals[1] = acc[1];
+        host_vals[2] = acc[2];
+        __threadfence_system();
+        *host_seq = seq;
+    }
+}
+
+// Final fold of the last pair: f~(r), g~(r), published like the round sums.
 __global__ void k_sc2_final(const V2* __restrict__ f, const V2* __restrict__ g, u64 rr, u64 ri,
-                            fp2* out)
+                            fp2* host_vals, volatile unsigned* host_seq, unsigned seq)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         const fp2 r = mk2(rr, ri);
-        out[0] = fold2(ldv(f), ldv(f + 1), r);
-        out[1] = fold2(ldv(g), ldv(g + 1), r);
+        host_vals[0] = fold2(ldv(f), ldv(f + 1), r);
+        host_vals[1] = fold2(ldv(g), ldv(g + 1), r);
+        __threadfence_system();
+        *host_seq = seq;
     }
 }
 
 struct Sc2State {
-    DevBuf work;          // T_k ping-pong: A (n/2) and B (n/4), for f and g
-    DevBuf red;           // counter | result | partials
-    fp2* host = nullptr;  // pinned: 3 round sums (+ 2 final values)
+    DevBuf work;                 // T_k ping-pong: A (n/2) and B (n/4), for f and g
+    DevBuf partials;             // 3 fp2 per workgroup
+    fp2* host_vals = nullptr;    // mapped pinned: 3 values
+    unsigned* host_seq = nullptr;
+    fp2* dev_vals = nullptr;     // device aliases of the mapped memory
+    unsigned* dev_seq = nullptr;
+    unsigned seq = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    // timing of the fused fold+round launches, for roofline reporting
-    bool timed = false;
+    bool timed = false;          // per-launch event timing of the fused kernel
     double fold_ms = 0, fold_bytes = 0;
     int fold_launches = 0;
-    double total_ms = 0;
 };
 Sc2State g_sc2;
 
-bool sc2_setup(size_t n, unsigned max_blocks)
+bool sc2_setup(size_t n)
 {
     Sc2State& S = g_sc2;
-    if (!S.host) TP_HIP_OK(hipHostMalloc((void**)&S.host, 8 * sizeof(fp2), hipHostMallocDefault));
+    if (!S.host_vals) {
+        void* hv = nullptr;
+        TP_HIP_OK(hipHostMalloc(&hv, 256, hipHostMallocMapped | hipHostMallocCoherent));
+        S.host_vals = static_cast<fp2*>(hv);
+        S.host_seq = reinterpret_cast<unsigned*>(static_cast<char*>(hv) + 128);
+        *S.host_seq = 0;
+        void* dv = nullptr;
+        TP_HIP_OK(hipHostGetDevicePointer(&dv, hv, 0));
+        S.dev_vals = static_cast<fp2*>(dv);
+        S.dev_seq = reinterpret_cast<unsigned*>(static_cast<char*>(dv) + 128);
+    }
     if (!S.ev[0]) {
         TP_HIP_OK(hipEventCreate(&S.ev[0]));
         TP_HIP_OK(hipEventCreate(&S.ev[1]));
     }
-    const size_t work = 2 * (n / 2 + n / 4 + 2) * sizeof(V2);
-    if (!S.work.ensure(work)) return false;
-    const size_t red = 512 + (size_t)max_blocks * 3 * sizeof(fp2);
-    if (S.red.bytes < red) {
-        if (!S.red.ensure(red)) return false;
-        TP_HIP_OK(hipMemset(S.red.ptr, 0, red));
-    }
+    if (!S.work.ensure(2 * (n / 2 + n / 4 + 2) * sizeof(V2))) return false;
+    if (!S.partials.ensure(2048 * 3 * sizeof(fp2))) return false;
     return true;
 }
 
@@ -127,6 +160,28 @@ unsigned blocks_for(size_t items)
     size_t b = (items + kBlock - 1) / kBlock;
     if (b > 2048) b = 2048;
     return (unsigned)(b ? b : 1);
+}
+
+// Waits until the device has published sequence number `seq`; checks the
+// stream for errors while waiting and gives up after a generous timeout.
+bool wait_seq(hipStream_t st, unsigned seq)
+{
+    volatile unsigned* hs = g_sc2.host_seq;
+    const auto t0 = std::chrono::steady_clock::now();
+    long spins = 0;
+    while (*hs != seq) {
+        if (++spins % 4096 == 0) {
+            const hipError_t q = hipStreamQuery(st);
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return fail("sumcheck stream error: %s", hipGetErrorString(q));
+            if (q == hipSuccess && *hs != seq)
+                return fail("sumcheck: stream idle but round %u not published", seq);
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(30))
+                return fail("sumcheck: timed out waiting for round %u", seq);
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return true;
 }
 
 }  // namespace
@@ -152,42 +207,38 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
     if (!tp::device_ready()) return -1;
     hipStream_t st = stream_handle ? (hipStream_t)stream_handle : tp::stream();
     const size_t n = (size_t)1 << nv;
-    if (!sc2_setup(n, 2048)) return -1;
+    if (!sc2_setup(n)) return -1;
     Sc2State& S = g_sc2;
-    S.fold_ms = 0; S.fold_bytes = 0; S.fold_launches = 0;
+    S.fold_ms = 0;
+    S.fold_bytes = 0;
+    S.fold_launches = 0;
 
-    char* red = S.red.as<char>();
-    unsigned* counter = reinterpret_cast<unsigned*>(red);
-    fp2* result = reinterpret_cast<fp2*>(red + 256);
-    fp2* partials = reinterpret_cast<fp2*>(red + 512);
+    fp2* partials = S.partials.as<fp2>();
     V2* A = S.work.as<V2>();             // f: n/2, then g: n/2
     V2* B = A + 2 * (n / 2 + 1);         // f: n/4, then g: n/4
     const size_t capA = n / 2 + 1, capB = n / 4 + 1;
 
     FsTranscript fs;
     fs.init("truepix");
-    {   // put(["c2", seed])
-        fs.item_str("c2", 2);
-        fs.item_int(seed);
-    }
+    fs.item_str("c2", 2);   // put(["c2", seed])
+    fs.item_int(seed);
 
     const V2* cf = reinterpret_cast<const V2*>(d_f);
     const V2* cg = reinterpret_cast<const V2*>(d_g);
     size_t m = n;   // length of the current table T_k
     u64 rr = 0, ri = 0;
     for (int k = 0; k < nv; ++k) {
+        unsigned nb;
         if (k == 0) {
-            const size_t pairs = m / 2;
-            k_sc2_first<<<blocks_for(pairs), kBlock, 0, st>>>(cf, cg, pairs, partials, counter,
-                                                              result);
+            nb = blocks_for(m / 2);
+            k_sc2_first<<<nb, kBlock, 0, st>>>(cf, cg, m / 2, partials);
         } else {
             const size_t quads = m / 2;   // input T_{k-1} has length 2m
             V2* of = (k & 1) ? A : B;
             V2* og = of + ((k & 1) ? capA : capB);
-            const unsigned nb = blocks_for(quads);
+            nb = blocks_for(quads);
             if (S.timed) TP_HIP_OK(hipEventRecord(S.ev[0], st));
-            k_sc2_fold_round<<<nb, kBlock, 0, st>>>(cf, cg, quads, rr, ri, of, og, partials,
-                                                    counter, result);
+            k_sc2_fold_round<<<nb, kBlock, 0, st>>>(cf, cg, quads, rr, ri, of, og, partials);
             if (S.timed) TP_HIP_OK(hipEventRecord(S.ev[1], st));
             cf = of;
             cg = og;
@@ -195,14 +246,17 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             S.fold_launches += 1;
         }
         TP_HIP_OK(hipGetLastError());
-        TP_HIP_OK(hipMemcpyAsync(S.host, result, 3 * sizeof(fp2), hipMemcpyDeviceToHost, st));
-        TP_HIP_OK(hipStreamSynchronize(st));
+        const unsigned seq = ++S.seq;
+        k_sc2_publish<<<1, kBlock, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
+        TP_HIP_OK(hipGetLastError());
+        if (!wait_seq(st, seq)) return -1;
         if (S.timed && k > 0) {
+            TP_HIP_OK(hipEventSynchronize(S.ev[1]));
             float ms = 0;
             TP_HIP_OK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
             S.fold_ms += ms;
         }
-        const fp2 E0 = S.host[0], E1 = S.host[1], EA = S.host[2];
+        const fp2 E0 = S.host_vals[0], E1 = S.host_vals[1], EA = S.host_vals[2];
         const fp2 a = EA, c = E0, b = sub2(sub2(E1, EA), E0);
         polys[6 * k + 0] = a.re; polys[6 * k + 1] = a.im;
         polys[6 * k + 2] = b.re; polys[6 * k + 3] = b.im;
@@ -225,12 +279,12 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         m /= 2;
     }
     // T_{nv-1} has two entries: fold them with r_{nv-1}.
-    k_sc2_final<<<1, 64, 0, st>>>(cf, cg, rr, ri, result);
+    const unsigned seq = ++S.seq;
+    k_sc2_final<<<1, 64, 0, st>>>(cf, cg, rr, ri, S.dev_vals, S.dev_seq, seq);
     TP_HIP_OK(hipGetLastError());
-    TP_HIP_OK(hipMemcpyAsync(S.host + 3, result, 2 * sizeof(fp2), hipMemcpyDeviceToHost, st));
-    TP_HIP_OK(hipStreamSynchronize(st));
-    final4[0] = S.host[3].re; final4[1] = S.host[3].im;
-    final4[2] = S.host[4].re; final4[3] = S.host[4].im;
+    if (!wait_seq(st, seq)) return -1;
+    final4[0] = S.host_vals[0].re; final4[1] = S.host_vals[0].im;
+    final4[2] = S.host_vals[1].re; final4[3] = S.host_vals[1].im;
     return 0;
 }
 

@@ -74,6 +74,7 @@ def bind(lib) -> None:
     _sig(lib, "tp_device_count", [], _I32)
     _sig(lib, "tp_sc2_prove", [_VP, _VP, _I32, _U64, _U64P, _U64P, _U64P, _U64P, _VP], _I32)
     _sig(lib, "tp_sc2_set_timing", [_I32])
+    _sig(lib, "tp_rs_encode", [_VP, _I32, _I32, _I32, _VP, _I32, _VP], _I32)
     _sig(lib, "tp_sc2_stats", [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                _I32P])
 
