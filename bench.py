@@ -50,14 +50,14 @@ def bench_rs_encode(steps: int = 5, cols: int = 64, lc: int = 20, lo: int = 22):
     coef, _ = synthetic_tables(lc + 6, seed=33)          # 2^26 = 64 x 2^20 Fp2
     dc = torch.from_numpy(coef.view("int64")).to("cuda")
     do = torch.empty(cols * (2 << lo), dtype=torch.int64, device="cuda")
-    stream = torch.cuda.current_stream().cuda_stream
-    ntt.rs_encode_device(dc.data_ptr(), cols, lc, lo, do.data_ptr(), stream=stream)
+    st = torch.cuda.Stream()                             # a real stream: events and kernels share it
+    ntt.rs_encode_device(dc.data_ptr(), cols, lc, lo, do.data_ptr(), stream=st.cuda_stream)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    ev0.record()
+    ev0.record(st)
     for _ in range(steps):
-        ntt.rs_encode_device(dc.data_ptr(), cols, lc, lo, do.data_ptr(), stream=stream)
-    ev1.record()
+        ntt.rs_encode_device(dc.data_ptr(), cols, lc, lo, do.data_ptr(), stream=st.cuda_stream)
+    ev1.record(st)
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / steps
     alg = cols * ((16 << lc) + (16 << lo))
@@ -117,7 +117,7 @@ def main(argv=None):
     f, g = sc.synthetic_tables(NV, seed=2024)
     df, dg = sc.to_device(f), sc.to_device(g)
     del f, g
-    stream = torch.cuda.current_stream().cuda_stream
+    stream = torch.cuda.Stream().cuda_stream            # the kernels' stream (not the null stream)
     for _ in range(a.warmup):
         sc.prove_deg2_device(df.data_ptr(), dg.data_ptr(), NV, seed=2024, stream=stream)
 

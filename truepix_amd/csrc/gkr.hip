@@ -1,0 +1,596 @@
+// gkr.hip - device side of one GKR layer proof beyond the early rounds:
+//   A3 layer evaluation     TP/libTruePix/arithcircuit.py:20-145 (C*Gate.run,
+//                           ArithCircuitLayer.run): base-field gates over copies
+//   A4 wire table           TP/libTruePix/layerprover.py:113-120 (transpose
+//                           [copy][wire] -> [wire][copy], zero-padded copies)
+//   A9 late rounds          TP/libTruePix/gateprover.py:102-176 and
+//                           layerprover.py:225-244,282-285: the quadratic rounds
+//                           over w1 then w2 with the per-gate wiring predicate
+//   A10 line projection     TP/libTruePix/layercompute.py:169-243: V at points
+//   A11 wiring MLE          TP/libTruePix/circuitverifier.py:224-290 (eval_mlext)
+//   A12 input MLE           native_fp2_mle_eval_base on a device-resident vector
+//
+// Layouts: circuit values are u64 base-field rows [copy][row_stride] with
+// row_stride = 2^bits (copy-major, as the proof's input vector is laid out);
+// Fp2 tables are SoA (re[], im[]) like the early-round session.
+#include "fp61.hpp"
+#include "reduce.hpp"
+#include "runtime.hpp"
+
+#include <stdint.h>
+#include <string.h>
+#include <vector>
+
+using namespace tp;
+
+namespace {
+
+// Base-field gate evaluation (ArithCircuit semantics: a MUX gate yields in0
+// when its mux bit is set and in1 otherwise, as CMuxGate.run does).
+__device__ __forceinline__ u64 eval_gate(int op, u64 x, u64 y, int muxbit)
+{
+    switch (op) {
+    case G_MUL: return fmul(x, y);
+    case G_ADD: return fadd(x, y);
+    case G_SUB: return fsub(x, y);
+    case G_MUXL: return muxbit ? x : y;
+    case G_OR: return fsub(fadd(x, y), fmul(x, y));
+    case G_XOR: { const u64 xy = fmul(x, y); return fsub(fadd(x, y), fadd(xy, xy)); }
+    case G_NOT: return fsub(1, x);
+    case G_NAND: return fsub(1, fmul(x, y));
+    case G_NOR: return fsub(fadd(1, fmul(x, y)), fadd(x, y));
+    case G_NXOR: { const u64 xy = fmul(x, y); return fsub(fadd(1, fadd(xy, xy)), fadd(x, y)); }
+    case G_NAAB: return fmul(fsub(1, x), y);
+    default: return x;   // PASS
+    }
+}
+
+// A3: out[c][g] = gate_g(in[c][in0_g], in[c][in1_g]) for c < n_copies and
+// g < n_gates; zero for g >= n_gates and for padded copies < n_copies_pad.
+__global__ void k_eval_layer(const u64* __restrict__ in, int in_stride, u64* __restrict__ out,
+                             int out_log, int n_copies, int n_copies_pad,
+                             const int* __restrict__ op, const int* __restrict__ in0,
+                             const int* __restrict__ in1, const int* __restrict__ mux, int n_gates)
+{
+    const size_t total = (size_t)n_copies_pad << out_log;
+    const int out_stride = 1 << out_log;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < total;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = k >> out_log;
+        const int g = (int)(k & (out_stride - 1));
+        u64 v = 0;
+        if (g < n_gates && c < (size_t)n_copies) {
+            const u64* row = in + c * in_stride;
+            v = eval_gate(op[g], row[in0[g]], row[in1[g]], mux[g]);
+        }
+        out[k] = v;
+    }
+}
+
+// A4: Fp2 SoA wire table [wire][copy] from base-field rows [copy][row_stride]
+// through a 64 x 64 LDS tile (reads coalesced along wires, writes along copies).
+__global__ void k_wire_table(const u64* __restrict__ rows, int row_stride, int n_copies,
+                             int n_copies_pad, int n_wires, u64* __restrict__ tre,
+                             u64* __restrict__ tim)
+{
+    __shared__ u64 tile[64][65];
+    const int c0 = blockIdx.x * 64, w0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 64 x 4 threads
+    for (int i = ty; i < 64; i += 4) {
+        const int c = c0 + i, w = w0 + tx;
+        tile[i][tx] = (c < n_copies && w < n_wires) ? rows[(size_t)c * row_stride + w] : 0;
+    }
+    __syncthreads();
+    for (int i = ty; i < 64; i += 4) {
+        const int w = w0 + i, c = c0 + tx;
+        if (w < n_wires && c < n_copies_pad) {
+            tre[(size_t)w * n_copies_pad + c] = tile[tx][i];
+            tim[(size_t)w * n_copies_pad + c] = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- LDS folds
+
+constexpr int kOneWg = 512;              // threads of the one-workgroup kernels
+constexpr int kMaxNin = 11;              // 2^11 wires: two 32 KiB LDS buffers
+
+__device__ __forceinline__ fp2 ld2(const u64* r, const u64* i, size_t k) { return mk2(r[k], i[k]); }
+
+// Loads the 2^nb table into T and folds it with cs[0..nf) (interleaved re,
+// im), ping-ponging between T and U; returns the buffer holding the result.
+__device__ fp2* lds_fold_chain(fp2* T, fp2* U, const u64* vr, const u64* vi, int nb,
+                               const u64* cs, int nf)
+{
+    const int W = 1 << nb;
+    for (int w = threadIdx.x; w < W; w += blockDim.x) T[w] = ld2(vr, vi, w);
+    __syncthreads();
+    fp2* src = T;
+    fp2* dst = U;
+    for (int j = 0; j < nf; ++j) {
+        const fp2 r = mk2(cs[2 * j], cs[2 * j + 1]);
+        const int n_out = W >> (j + 1);
+        for (int i = threadIdx.x; i < n_out; i += blockDim.x)
+            dst[i] = fold2(src[2 * i], src[2 * i + 1], r);
+        __syncthreads();
+        fp2* t = src;
+        src = dst;
+        dst = t;
+    }
+    return src;
+}
+
+// ---------------------------------------------------------------- late rounds
+// Round rd < nin binds bit rd of in0 (w1), round nin + k binds bit k of in1
+// (w2).  For a gate with e = the bound bit of its input w and T the table
+// folded with this pass's earlier challenges (kk of them):
+//   t = e  : z1 * gate(cur, other)          cur = T[w >> kk]
+//   t = -1 : z1 * (e ? -1 : 2) * gate(m1, other),  m1 = 2 T[2j] - T[2j+1],
+//            j = w >> (kk + 1)
+// (argument order swapped in the w2 pass), summed over gates and multiplied
+// by beta* (TP/libTruePix/gateprover.py:102-132, layerprover.py:225-244).
+// Before that, the previous round's challenge updates each gate's
+// coefficient: z1 *= (bit ? r : 1 - r) (gateprover.py:135-176).
+__global__ void __launch_bounds__(kOneWg) k_late_round(
+    const int* __restrict__ gt, const int* __restrict__ in0, const int* __restrict__ in1,
+    u64* __restrict__ z1r, u64* __restrict__ z1i, const u64* __restrict__ oth_r,
+    const u64* __restrict__ oth_i, const u64* __restrict__ vr, const u64* __restrict__ vi,
+    const u64* __restrict__ chal, int nin, int n_gates, int rd, int apply, u64 beta_re,
+    u64 beta_im, fp2* host_vals, volatile unsigned* host_seq, unsigned seq)
+{
+    extern __shared__ fp2 lds[];
+    fp2* T = lds;
+    fp2* U = lds + (1 << nin);
+    if (apply) {
+        const int k = rd - 1;
+        const fp2 r = mk2(chal[2 * k], chal[2 * k + 1]);
+        const fp2 omr = sub2(one2(), r);
+        const bool w1pass = k < nin;
+        const int bit = w1pass ? k : k - nin;
+        for (int g = threadIdx.x; g < n_gates; g += blockDim.x) {
+            const int w = w1pass ? in0[g] : in1[g];
+            const fp2 nz = mul2(mk2(z1r[g], z1i[g]), ((w >> bit) & 1) ? r : omr);
+            z1r[g] = nz.re;
+            z1i[g] = nz.im;
+        }
+    }
+    const bool w1pass = rd < nin;
+    const int kk = w1pass ? rd : rd - nin;
+    const fp2* Tk = lds_fold_chain(T, U, vr, vi, nin, chal + (w1pass ? 0 : 2 * nin), kk);
+    fp2 acc[3] = {zero2(), zero2(), zero2()};
+    const fp2 m_one = mk2(P - 1, 0), two = mk2(2, 0);
+    for (int g = threadIdx.x; g < n_gates; g += blockDim.x) {
+        const int w = w1pass ? in0[g] : in1[g];
+        const int e = (w >> kk) & 1;
+        const fp2 z = mk2(z1r[g], z1i[g]);
+        const fp2 other = mk2(oth_r[g], oth_i[g]);
+        const fp2 cur = Tk[w >> kk];
+        const int j = w >> (kk + 1);
+        const fp2 m1 = sub2(dbl2(Tk[2 * j]), Tk[2 * j + 1]);
+        const int t = gt[g];
+        const fp2 v01 = mul2(z, w1pass ? gatefn(t, cur, other) : gatefn(t, other, cur));
+        const fp2 vm1 = mul2(mul2(z, e ? m_one : two),
+                             w1pass ? gatefn(t, m1, other) : gatefn(t, other, m1));
+        if (e) acc[1] = add2(acc[1], v01);
+        else acc[0] = add2(acc[0], v01);
+        acc[2] = add2(acc[2], vm1);
+    }
+    block_sum<3>(acc);
+    if (threadIdx.x == 0) {
+        const fp2 beta = mk2(beta_re, beta_im);
+        for (int t = 0; t < 3; ++t) host_vals[t] = mul2(acc[t], beta);
+        __threadfence_system();
+        *host_seq = seq;
+    }
+}
+
+// Start of the w2 pass: every gate's fixed side becomes V(w1) (the table
+// folded with all w1 challenges; gateprover.py:150-158).
+__global__ void __launch_bounds__(kOneWg) k_late_switch(const u64* __restrict__ vr,
+                                                        const u64* __restrict__ vi, int nin,
+                                                        const u64* __restrict__ chal, int n_gates,
+                                                        u64* __restrict__ oth_r,
+                                                        u64* __restrict__ oth_i)
+{
+    extern __shared__ fp2 lds[];
+    const fp2* R = lds_fold_chain(lds, lds + (1 << nin), vr, vi, nin, chal, nin);
+    const fp2 vw1 = R[0];
+    for (int g = threadIdx.x; g < n_gates; g += blockDim.x) {
+        oth_r[g] = vw1.re;
+        oth_i[g] = vw1.im;
+    }
+}
+
+// A10: MLE of a 2^nb Fp2 table at K points (one workgroup per point).
+__global__ void __launch_bounds__(kOneWg) k_mle_points(const u64* __restrict__ vr,
+                                                       const u64* __restrict__ vi, int nb,
+                                                       const u64* __restrict__ pts,
+                                                       fp2* __restrict__ out)
+{
+    extern __shared__ fp2 lds[];
+    const fp2* R = lds_fold_chain(lds, lds + (1 << nb), vr, vi, nb,
+                                  pts + (size_t)blockIdx.x * 2 * nb, nb);
+    if (threadIdx.x == 0) out[blockIdx.x] = R[0];
+}
+
+// A11: per gate type, sum_g beta_z1[g] * beta_w1[in0_g] * beta_w2[in1_g]
+// (with use_z1_2: mul0 beta_z1[g] + mul1 beta_z1_2[g] in place of
+// beta_z1[g]); eq tables built in LDS by doubling; one workgroup.
+__device__ void lds_eq(fp2* tab, const u64* z, int nb)
+{
+    if (threadIdx.x == 0) tab[0] = one2();
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        const int cur = 1 << j;
+        const fp2 zz = mk2(z[2 * j], z[2 * j + 1]);
+        const fp2 omz = sub2(one2(), zz);
+        for (int t = threadIdx.x; t < cur; t += blockDim.x) {
+            const fp2 b = tab[t];
+            tab[t + cur] = mul2(b, zz);
+            tab[t] = mul2(b, omz);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kOneWg) k_wiring_mlext(
+    const int* __restrict__ type_idx, const int* __restrict__ in0, const int* __restrict__ in1,
+    int n_gates, const u64* __restrict__ z1, const u64* __restrict__ z1_2, int n_out_bits,
+    const u64* __restrict__ w1, const u64* __restrict__ w2, int n_in_bits, u64 m0r, u64 m0i,
+    u64 m1r, u64 m1i, int use_z1_2, fp2* __restrict__ out13)
+{
+    extern __shared__ fp2 sm[];
+    const int WO = 1 << n_out_bits, WI = 1 << n_in_bits;
+    fp2* bz = sm;
+    fp2* bz2 = bz + WO;
+    fp2* b1 = bz2 + (use_z1_2 ? WO : 0);
+    fp2* b2 = b1 + WI;
+    lds_eq(bz, z1, n_out_bits);
+    if (use_z1_2) lds_eq(bz2, z1_2, n_out_bits);
+    lds_eq(b1, w1, n_in_bits);
+    lds_eq(b2, w2, n_in_bits);
+    fp2 acc[13];
+#pragma unroll
+    for (int t = 0; t < 13; ++t) acc[t] = zero2();
+    const fp2 m0 = mk2(m0r, m0i), m1 = mk2(m1r, m1i);
+    for (int g = threadIdx.x; g < n_gates; g += blockDim.x) {
+        fp2 tv = use_z1_2 ? add2(mul2(m0, bz[g]), mul2(m1, bz2[g])) : bz[g];
+        tv = mul2(tv, mul2(b1[in0[g]], b2[in1[g]]));
+        const int t = type_idx[g];
+#pragma unroll
+        for (int q = 0; q < 13; ++q)
+            if (q == t) acc[q] = add2(acc[q], tv);
+    }
+    block_sum<13>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int t = 0; t < 13; ++t) out13[t] = acc[t];
+    }
+}
+
+// A12 on a device-resident base-field vector of 2^n_z entries: one partial
+// per workgroup over rows of 2^lb inputs sharing an LDS low-half eq table.
+__global__ void k_mle_base_dev(const u64* __restrict__ in, const u64* __restrict__ z, int n_z,
+                               fp2* partials)
+{
+    __shared__ fp2 lo[1024];
+    __shared__ fp2 hrow;
+    const int lb = n_z < 10 ? n_z : 10;
+    const int row_len = 1 << lb;
+    const size_t n_rows = (size_t)1 << (n_z - lb);
+    lds_eq(lo, z, lb);
+    fp2 acc[1] = {zero2()};
+    for (size_t row = blockIdx.x; row < n_rows; row += gridDim.x) {
+        if (threadIdx.x == 0) {
+            fp2 h = one2();
+            for (int j = lb; j < n_z; ++j) {
+                const fp2 zz = mk2(z[2 * j], z[2 * j + 1]);
+                h = mul2(h, ((row >> (j - lb)) & 1) ? zz : sub2(one2(), zz));
+            }
+            hrow = h;
+        }
+        __syncthreads();
+        fp2 dot = zero2();
+        const u64* src = in + row * row_len;
+        for (int t = threadIdx.x; t < row_len; t += blockDim.x)
+            dot = add2(dot, smul2(fred64(src[t]), lo[t]));
+        acc[0] = add2(acc[0], mul2(dot, hrow));
+        __syncthreads();
+    }
+    block_partial<1>(acc, partials);
+}
+
+// ------------------------------------------------------------------ host side
+
+struct Mapped {
+    fp2* vals = nullptr;
+    unsigned* seq = nullptr;
+    fp2* dvals = nullptr;
+    unsigned* dseq = nullptr;
+    unsigned next = 0;
+};
+Mapped g_map;
+
+bool mapped_ready()
+{
+    if (g_map.vals) return true;
+    void* hv = nullptr;
+    TP_HIP_OK(hipHostMalloc(&hv, 512, hipHostMallocMapped | hipHostMallocCoherent));
+    g_map.vals = static_cast<fp2*>(hv);
+    g_map.seq = reinterpret_cast<unsigned*>(static_cast<char*>(hv) + 256);
+    *g_map.seq = 0;
+    void* dv = nullptr;
+    TP_HIP_OK(hipHostGetDevicePointer(&dv, hv, 0));
+    g_map.dvals = static_cast<fp2*>(dv);
+    g_map.dseq = reinterpret_cast<unsigned*>(static_cast<char*>(dv) + 256);
+    return true;
+}
+
+bool wait_mapped(hipStream_t st, unsigned seq)
+{
+    volatile unsigned* hs = g_map.seq;
+    long spins = 0;
+    while (*hs != seq) {
+        if (++spins % 4096 == 0) {
+            const hipError_t q = hipStreamQuery(st);
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return fail("late round: stream error %s", hipGetErrorString(q));
+            if (q == hipSuccess && *hs != seq) return fail("late round: result not published");
+        }
+    }
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    return true;
+}
+
+bool h2d(DevBuf& b, const void* src, size_t bytes, hipStream_t st)
+{
+    if (!b.ensure(bytes ? bytes : 8)) return false;
+    if (bytes) TP_HIP_OK(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, st));
+    return true;
+}
+
+size_t fold_lds(int nb) { return ((size_t)3 << nb) / 2 * sizeof(fp2); }
+
+bool set_lds_limits()
+{
+    static bool done = false;
+    if (done) return true;
+    const int need = (int)(((size_t)4 << kMaxNin) * sizeof(fp2) / 2);
+    TP_HIP_OK(hipFuncSetAttribute((const void*)k_late_round,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, need));
+    TP_HIP_OK(hipFuncSetAttribute((const void*)k_late_switch,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, need));
+    TP_HIP_OK(hipFuncSetAttribute((const void*)k_mle_points,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, need));
+    TP_HIP_OK(hipFuncSetAttribute((const void*)k_wiring_mlext,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 2 * need));
+    done = true;
+    return true;
+}
+
+struct LateSession {
+    int n_gates = 0, nin = 0, next_round = 0;
+    u64 beta_re = 0, beta_im = 0;
+    DevBuf gt, i0, i1, z1r, z1i, othr, othi, vr, vi, chal;
+    ~LateSession()
+    {
+        DevBuf* all[] = {&gt, &i0, &i1, &z1r, &z1i, &othr, &othi, &vr, &vi, &chal};
+        for (DevBuf* b : all) b->release();
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+// A3: evaluate one layer.  d_in: [n_copies_pad][in_stride] u64 (canonical);
+// d_out: [n_copies_pad][2^out_log] u64.  op: gate type index per gate
+// (0 MUL, 1 ADD, 2 SUB, 3 MUX, 5 OR, 6 XOR, 7 NOT, 8 NAND, 9 NOR, 10 NXOR,
+// 11 NAAB, 12 PASS); muxbit: the mux bit value of each MUX gate.
+int tp_eval_layer(const uint64_t* d_in, int in_stride, uint64_t* d_out, int out_log,
+                  int n_copies, int n_copies_pad, const int* op, const int* in0, const int* in1,
+                  const int* muxbit, int n_gates, void* stream_handle)
+{
+    tp::clear_error();
+    if (!tp::device_ready()) return -1;
+    if (n_gates < 0 || out_log < 0 || out_log > 24 || n_gates > (1 << out_log) ||
+        n_copies > n_copies_pad)
+        return tp::fail("tp_eval_layer: bad shape"), -1;
+    for (int g = 0; g < n_gates; ++g)
+        if (in0[g] < 0 || in0[g] >= in_stride || in1[g] < 0 || in1[g] >= in_stride)
+            return tp::fail("tp_eval_layer: gate %d reads outside the row", g), -1;
+    hipStream_t st = stream_handle ? (hipStream_t)stream_handle : tp::stream();
+    static DevBuf b_op, b_i0, b_i1, b_mx;
+    const size_t gb = (size_t)n_gates * 4;
+    if (!h2d(b_op, op, gb, st) || !h2d(b_i0, in0, gb, st) || !h2d(b_i1, in1, gb, st) ||
+        !h2d(b_mx, muxbit, gb, st))
+        return -1;
+    const size_t total = (size_t)n_copies_pad << out_log;
+    k_eval_layer<<<grid_for(total, kBlock, 8192), kBlock, 0, st>>>(
+        (const u64*)d_in, in_stride, (u64*)d_out, out_log, n_copies, n_copies_pad,
+        b_op.as<int>(), b_i0.as<int>(), b_i1.as<int>(), b_mx.as<int>(), n_gates);
+    TP_HIP_OK(hipGetLastError());
+    TP_HIP_OK(hipStreamSynchronize(st));   // gate arrays are reused by the next call
+    return 0;
+}
+
+// A4: SoA Fp2 wire table [n_wires][n_copies_pad] from rows [n_copies][row_stride].
+int tp_wire_table(const uint64_t* d_rows, int row_stride, int n_copies, int n_copies_pad,
+                  int n_wires, uint64_t* d_re, uint64_t* d_im, void* stream_handle)
+{
+    tp::clear_error();
+    if (!tp::device_ready()) return -1;
+    if (n_wires > row_stride || n_copies > n_copies_pad || n_wires <= 0 || n_copies_pad <= 0)
+        return tp::fail("tp_wire_table: bad shape"), -1;
+    hipStream_t st = stream_handle ? (hipStream_t)stream_handle : tp::stream();
+    dim3 grid((unsigned)((n_copies_pad + 63) / 64), (unsigned)((n_wires + 63) / 64));
+    k_wire_table<<<grid, 256, 0, st>>>((const u64*)d_rows, row_stride, n_copies, n_copies_pad,
+                                       n_wires, (u64*)d_re, (u64*)d_im);
+    TP_HIP_OK(hipGetLastError());
+    return 0;
+}
+
+// A9: late-phase session over the 2^nin wire values V (host SoA, the early
+// rounds' result), per-gate type index, inputs, wiring coefficient z1 and
+// beta*.  NULL on a bad shape or device failure.
+void* tp_late_create(int n_gates, const int* type_idx, const int* in0, const int* in1,
+                     const uint64_t* z1_re, const uint64_t* z1_im, int nin,
+                     const uint64_t* v_re, const uint64_t* v_im, uint64_t beta_re,
+                     uint64_t beta_im)
+{
+    tp::clear_error();
+    if (!tp::device_ready() || !mapped_ready() || !set_lds_limits()) return nullptr;
+    if (n_gates <= 0 || nin < 1 || nin > kMaxNin) {
+        tp::fail("tp_late_create: bad shape (n_gates=%d nin=%d)", n_gates, nin);
+        return nullptr;
+    }
+    for (int g = 0; g < n_gates; ++g)
+        if (in0[g] < 0 || in0[g] >= (1 << nin) || in1[g] < 0 || in1[g] >= (1 << nin)) {
+            tp::fail("tp_late_create: gate %d input outside 2^nin", g);
+            return nullptr;
+        }
+    hipStream_t st = stream();
+    auto* s = new LateSession();
+    s->n_gates = n_gates;
+    s->nin = nin;
+    s->beta_re = beta_re;
+    s->beta_im = beta_im;
+    const size_t W = (size_t)1 << nin;
+    // fixed side during the w1 pass: V[in1] (gateprover.py:51-62 reset())
+    std::vector<u64> oth_r(n_gates), oth_i(n_gates);
+    for (int g = 0; g < n_gates; ++g) { oth_r[g] = v_re[in1[g]]; oth_i[g] = v_im[in1[g]]; }
+    bool ok = h2d(s->gt, type_idx, n_gates * 4, st) && h2d(s->i0, in0, n_gates * 4, st) &&
+              h2d(s->i1, in1, n_gates * 4, st) && h2d(s->z1r, z1_re, n_gates * 8, st) &&
+              h2d(s->z1i, z1_im, n_gates * 8, st) && h2d(s->othr, oth_r.data(), n_gates * 8, st) &&
+              h2d(s->othi, oth_i.data(), n_gates * 8, st) && h2d(s->vr, v_re, W * 8, st) &&
+              h2d(s->vi, v_im, W * 8, st) && s->chal.ensure(4 * nin * 8);
+    if (ok && hipStreamSynchronize(st) != hipSuccess) ok = tp::fail("tp_late_create: upload");
+    if (!ok) {
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
+// Output of late round `round` (0 .. 2 nin - 1, in order): out6 =
+// (f(0), f(1), f(-1)) times beta*.  r_prev (2 limbs) is the previous round's
+// challenge, required for round > 0.  0 / -1.
+int tp_late_round(void* handle, int round, const uint64_t* r_prev, uint64_t out6[6])
+{
+    tp::clear_error();
+    auto* s = static_cast<LateSession*>(handle);
+    if (!s) return -1;
+    if (round != s->next_round || round >= 2 * s->nin)
+        return tp::fail("tp_late_round: round %d out of order", round), -1;
+    if (round > 0 && !r_prev) return tp::fail("tp_late_round: missing challenge"), -1;
+    hipStream_t st = stream();
+    if (round > 0)
+        TP_HIP_OK(hipMemcpyAsync(s->chal.as<u64>() + 2 * (round - 1), r_prev, 16,
+                                 hipMemcpyHostToDevice, st));
+    const size_t lds = fold_lds(s->nin);
+    if (round == s->nin) {
+        k_late_switch<<<1, kOneWg, lds, st>>>(s->vr.as<u64>(), s->vi.as<u64>(), s->nin,
+                                              s->chal.as<u64>(), s->n_gates, s->othr.as<u64>(),
+                                              s->othi.as<u64>());
+        TP_HIP_OK(hipGetLastError());
+    }
+    const unsigned seq = ++g_map.next;
+    k_late_round<<<1, kOneWg, lds, st>>>(
+        s->gt.as<int>(), s->i0.as<int>(), s->i1.as<int>(), s->z1r.as<u64>(), s->z1i.as<u64>(),
+        s->othr.as<u64>(), s->othi.as<u64>(), s->vr.as<u64>(), s->vi.as<u64>(), s->chal.as<u64>(),
+        s->nin, s->n_gates, round, round > 0 ? 1 : 0, s->beta_re, s->beta_im, g_map.dvals,
+        g_map.dseq, seq);
+    TP_HIP_OK(hipGetLastError());
+    if (!wait_mapped(st, seq)) return -1;
+    for (int t = 0; t < 3; ++t) {
+        out6[2 * t] = g_map.vals[t].re;
+        out6[2 * t + 1] = g_map.vals[t].im;
+    }
+    s->next_round = round + 1;
+    return 0;
+}
+
+// A10: MLE of the session's V table at K points (K x nin x 2 limbs) -> out (K x 2).
+int tp_late_values(void* handle, const uint64_t* pts, int K, uint64_t* out)
+{
+    tp::clear_error();
+    auto* s = static_cast<LateSession*>(handle);
+    if (!s || K <= 0) return tp::fail("tp_late_values: bad arguments"), -1;
+    hipStream_t st = stream();
+    static DevBuf b_pts, b_out;
+    if (!h2d(b_pts, pts, (size_t)K * s->nin * 16, st) || !b_out.ensure((size_t)K * 16)) return -1;
+    k_mle_points<<<K, kOneWg, fold_lds(s->nin), st>>>(s->vr.as<u64>(), s->vi.as<u64>(), s->nin,
+                                                       b_pts.as<u64>(), b_out.as<fp2>());
+    TP_HIP_OK(hipGetLastError());
+    TP_HIP_OK(hipMemcpyAsync(out, b_out.ptr, (size_t)K * 16, hipMemcpyDeviceToHost, st));
+    TP_HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
+void tp_late_destroy(void* handle) { delete static_cast<LateSession*>(handle); }
+
+// A11: wiring-predicate MLE per gate type.  z1 (n_out_bits x 2 limbs),
+// optional z1_2 with mul0 / mul1 (use_z1_2), w1 / w2 (n_in_bits x 2 limbs);
+// out26 = 13 Fp2 sums indexed by gate type.  0 / -1.
+int tp_wiring_mlext(int n_gates, const int* type_idx, const int* in0, const int* in1,
+                    const uint64_t* z1, const uint64_t* z1_2, int n_out_bits, const uint64_t* w1,
+                    const uint64_t* w2, int n_in_bits, const uint64_t* mul0, const uint64_t* mul1,
+                    int use_z1_2, uint64_t out26[26])
+{
+    tp::clear_error();
+    if (!tp::device_ready() || !set_lds_limits()) return -1;
+    if (n_gates <= 0 || n_gates > (1 << n_out_bits) || n_out_bits > kMaxNin ||
+        n_in_bits > kMaxNin || n_in_bits < 0)
+        return tp::fail("tp_wiring_mlext: bad shape"), -1;
+    for (int g = 0; g < n_gates; ++g)
+        if (in0[g] < 0 || in0[g] >= (1 << n_in_bits) || in1[g] < 0 || in1[g] >= (1 << n_in_bits) ||
+            type_idx[g] < 0 || type_idx[g] > 12)
+            return tp::fail("tp_wiring_mlext: gate %d out of range", g), -1;
+    hipStream_t st = stream();
+    static DevBuf b_t, b_i0, b_i1, b_z, b_z2, b_w1, b_w2, b_out;
+    const uint64_t zero2v[2] = {0, 0};
+    if (!h2d(b_t, type_idx, n_gates * 4, st) || !h2d(b_i0, in0, n_gates * 4, st) ||
+        !h2d(b_i1, in1, n_gates * 4, st) || !h2d(b_z, z1, (size_t)n_out_bits * 16, st) ||
+        !h2d(b_z2, use_z1_2 ? z1_2 : z1, (size_t)n_out_bits * 16, st) ||
+        !h2d(b_w1, w1, (size_t)n_in_bits * 16, st) || !h2d(b_w2, w2, (size_t)n_in_bits * 16, st) ||
+        !b_out.ensure(13 * 16))
+        return -1;
+    const uint64_t* m0 = use_z1_2 ? mul0 : zero2v;
+    const uint64_t* m1 = use_z1_2 ? mul1 : zero2v;
+    const size_t lds = (((size_t)(use_z1_2 ? 2 : 1) << n_out_bits) + ((size_t)2 << n_in_bits)) * 16;
+    k_wiring_mlext<<<1, kOneWg, lds, st>>>(b_t.as<int>(), b_i0.as<int>(), b_i1.as<int>(), n_gates,
+                                           b_z.as<u64>(), b_z2.as<u64>(), n_out_bits, b_w1.as<u64>(),
+                                           b_w2.as<u64>(), n_in_bits, m0[0], m0[1], m1[0], m1[1],
+                                           use_z1_2, b_out.as<fp2>());
+    TP_HIP_OK(hipGetLastError());
+    TP_HIP_OK(hipMemcpyAsync(out26, b_out.ptr, 13 * 16, hipMemcpyDeviceToHost, st));
+    TP_HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// A12: MLE of a device-resident base-field vector of 2^n_z entries at z
+// (n_z x 2 limbs, LSB-first); out2 = the Fp2 value.  0 / -1.
+int tp_mle_eval_base_dev(const uint64_t* d_inputs, int n_z, const uint64_t* z, uint64_t out2[2],
+                         void* stream_handle)
+{
+    tp::clear_error();
+    if (!tp::device_ready()) return -1;
+    if (n_z < 1 || n_z > 40) return tp::fail("tp_mle_eval_base_dev: n_z=%d", n_z), -1;
+    hipStream_t st = stream_handle ? (hipStream_t)stream_handle : tp::stream();
+    static DevBuf b_z, b_part;
+    const int lb = n_z < 10 ? n_z : 10;
+    const size_t rows = (size_t)1 << (n_z - lb);
+    const unsigned blocks = (unsigned)(rows < 1024 ? rows : 1024);
+    if (!h2d(b_z, z, (size_t)n_z * 16, st) || !b_part.ensure(256 + (size_t)blocks * 16)) return -1;
+    fp2* result = b_part.as<fp2>();
+    fp2* partials = reinterpret_cast<fp2*>(b_part.as<char>() + 256);
+    k_mle_base_dev<<<blocks, kBlock, 0, st>>>((const u64*)d_inputs, b_z.as<u64>(), n_z, partials);
+    TP_HIP_OK(hipGetLastError());
+    k_finalize_sum<1><<<1, kBlock, 0, st>>>(partials, blocks, result);
+    TP_HIP_OK(hipGetLastError());
+    TP_HIP_OK(hipMemcpyAsync(out2, result, 16, hipMemcpyDeviceToHost, st));
+    TP_HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
+}  // extern "C"
