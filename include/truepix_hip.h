@@ -127,6 +127,57 @@ void tp_sc2_stats(double* fold_ms, double* fold_bytes, int* fold_launches);
 int tp_rs_encode(const uint64_t* d_coef, int n_cols, int log_coef, int log_order,
                  uint64_t* d_out, int cols_per_chunk, void* stream);
 
+/* ---- device-resident GKR layer (the engine under the NIZK prover of
+ *      TP/libTruePix/circuitnizkvec_vole.py:265-501) --------------------------- */
+
+/* A3, TP/libTruePix/arithcircuit.py:20-145: one layer over copies.  d_in
+ * [n_copies_pad][in_stride], d_out [n_copies_pad][2^out_log] (u64 base field,
+ * zero padding).  op: 0 MUL 1 ADD 2 SUB 3 MUX 5 OR 6 XOR 7 NOT 8 NAND 9 NOR
+ * 10 NXOR 11 NAAB 12 PASS; muxbit: each gate's mux bit value.  0 / -1. */
+int tp_eval_layer(const uint64_t* d_in, int in_stride, uint64_t* d_out, int out_log,
+                  int n_copies, int n_copies_pad, const int* op, const int* in0, const int* in1,
+                  const int* muxbit, int n_gates, void* stream);
+
+/* A4, TP/libTruePix/layerprover.py:113-120 + layercompute.py:37-56: SoA Fp2
+ * table [n_wires][n_copies_pad] (im = 0, padded copies zero) from rows. */
+int tp_wire_table(const uint64_t* d_rows, int row_stride, int n_copies, int n_copies_pad,
+                  int n_wires, uint64_t* d_re, uint64_t* d_im, void* stream);
+
+/* A6-A8 on device tables: native_fp2_early_create (native_fp2.cu:923-952) with
+ * the [wire][copy] SoA tables already in device memory. */
+void* tp_early_create_dev(int n_gates, const int* gate_type, const int* in0, const int* in1,
+                          const uint64_t* z1_real, const uint64_t* z1_imag, int n_copies,
+                          int n_wires, const uint64_t* d_values_real,
+                          const uint64_t* d_values_imag, const uint64_t* beta_real,
+                          const uint64_t* beta_imag);
+
+/* A9, TP/libTruePix/gateprover.py:102-176 + layerprover.py:225-244,282-285:
+ * late-phase session over the 2^nin folded wire values (host SoA), gate type
+ * indices, inputs, wiring coefficients z1 and beta*.  NULL on failure. */
+void* tp_late_create(int n_gates, const int* type_idx, const int* in0, const int* in1,
+                     const uint64_t* z1_re, const uint64_t* z1_im, int nin,
+                     const uint64_t* v_re, const uint64_t* v_im, uint64_t beta_re,
+                     uint64_t beta_im);
+/* Round `round` (0 .. 2 nin - 1, in order); r_prev = previous challenge (2
+ * limbs, NULL for round 0); out6 = (f(0), f(1), f(-1)) x beta*.  0 / -1. */
+int tp_late_round(void* handle, int round, const uint64_t* r_prev, uint64_t out6[6]);
+/* A10, TP/libTruePix/layercompute.py:169-243: V at K points (K x nin x 2). */
+int tp_late_values(void* handle, const uint64_t* pts, int K, uint64_t* out);
+void tp_late_destroy(void* handle);
+
+/* A11, TP/libTruePix/circuitverifier.py:224-290 (eval_mlext): per gate type,
+ * sum_g eq(z1, g) eq(w1, in0_g) eq(w2, in1_g) (with use_z1_2: mul0 eq(z1, g) +
+ * mul1 eq(z1_2, g)); out26 = 13 Fp2 sums.  Points are interleaved limbs. */
+int tp_wiring_mlext(int n_gates, const int* type_idx, const int* in0, const int* in1,
+                    const uint64_t* z1, const uint64_t* z1_2, int n_out_bits, const uint64_t* w1,
+                    const uint64_t* w2, int n_in_bits, const uint64_t* mul0, const uint64_t* mul1,
+                    int use_z1_2, uint64_t out26[26]);
+
+/* A12, native_fp2_mle_eval_base (native_fp2.cu:909-921) on a device-resident
+ * base-field vector of 2^n_z entries; z: n_z x 2 limbs; out2 = the value. */
+int tp_mle_eval_base_dev(const uint64_t* d_inputs, int n_z, const uint64_t* z, uint64_t out2[2],
+                         void* stream);
+
 #ifdef __cplusplus
 }
 #endif

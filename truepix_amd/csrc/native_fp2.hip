@@ -517,6 +517,45 @@ EarlySession* session_create(int n_gates, const int* gate_type, const int* in0, 
     return s;
 }
 
+// Session over wire tables already on the device (SoA [wire][copy]): the
+// tables are copied device-to-device, beta and the gate arrays come from the host.
+EarlySession* session_create_dev(int n_gates, const int* gate_type, const int* in0,
+                                 const int* in1, const u64* z1_r, const u64* z1_i, int n_copies,
+                                 int n_wires, const u64* d_v_r, const u64* d_v_i, const u64* b_r,
+                                 const u64* b_i)
+{
+    if (n_gates <= 0 || n_wires <= 0 || n_copies < 2 || (n_copies & (n_copies - 1)) != 0) {
+        fail("early_create_dev: bad shape (n_gates=%d n_wires=%d n_copies=%d)", n_gates, n_wires,
+             n_copies);
+        return nullptr;
+    }
+    for (int g = 0; g < n_gates; ++g)
+        if (in0[g] < 0 || in0[g] >= n_wires || in1[g] < 0 || in1[g] >= n_wires) {
+            fail("early_create_dev: gate %d reads wire outside [0, %d)", g, n_wires);
+            return nullptr;
+        }
+    auto* s = new EarlySession();
+    s->n_gates = n_gates;
+    s->n_wires = n_wires;
+    s->n_copies = n_copies;
+    const size_t vn = (size_t)n_wires * n_copies;
+    bool ok = upload_gates(s->gt, s->i0, s->i1, s->z1r, s->z1i, n_gates, gate_type, in0, in1,
+                           z1_r, z1_i) &&
+              s->val_r.ensure(vn * 8) && s->val_i.ensure(vn * 8) &&
+              s->val2_r.ensure(vn * 8 / 2) && s->val2_i.ensure(vn * 8 / 2) &&
+              h2d(s->beta_r, b_r, n_copies * 8) && h2d(s->beta_i, b_i, n_copies * 8) &&
+              s->beta2_r.ensure(n_copies * 8 / 2) && s->beta2_i.ensure(n_copies * 8 / 2);
+    ok = ok && hipMemcpyAsync(s->val_r.ptr, d_v_r, vn * 8, hipMemcpyDeviceToDevice, stream()) == hipSuccess &&
+         hipMemcpyAsync(s->val_i.ptr, d_v_i, vn * 8, hipMemcpyDeviceToDevice, stream()) == hipSuccess;
+    if (ok && hipStreamSynchronize(stream()) != hipSuccess) ok = fail("early_create_dev: copy failed");
+    if (!ok) {
+        if (!tp_last_error_set()) fail("early_create_dev: device copy failed");
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
 }  // namespace
 
 // ===================================================================== ABI ==
@@ -641,5 +680,18 @@ int native_fp2_early_n_wires(void* handle)
 }
 
 void native_fp2_early_destroy(void* handle) { delete reinterpret_cast<EarlySession*>(handle); }
+
+// Early session over device-resident SoA wire tables [n_wires][n_copies]
+// (e.g. from tp_wire_table); otherwise as native_fp2_early_create.
+void* tp_early_create_dev(int n_gates, const int* gate_type, const int* in0, const int* in1,
+                          const uint64_t* z1_real, const uint64_t* z1_imag, int n_copies,
+                          int n_wires, const uint64_t* d_values_real, const uint64_t* d_values_imag,
+                          const uint64_t* beta_real, const uint64_t* beta_imag)
+{
+    tp::clear_error();
+    if (!tp::device_ready()) return nullptr;
+    return session_create_dev(n_gates, gate_type, in0, in1, z1_real, z1_imag, n_copies, n_wires,
+                              d_values_real, d_values_imag, beta_real, beta_imag);
+}
 
 }  // extern "C"

@@ -23,6 +23,8 @@ bool fail(const char* fmt, ...)
 
 void clear_error() { g_err[0] = 0; }
 
+bool tp_last_error_set() { return g_err[0] != 0; }
+
 static void init_once()
 {
     int n = 0;

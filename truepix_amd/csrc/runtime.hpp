@@ -16,6 +16,8 @@ namespace tp {
 // Records a formatted error for tp_last_error(); returns false for chaining.
 bool fail(const char* fmt, ...);
 void clear_error();
+// True if an error message is recorded on this thread.
+bool tp_last_error_set();
 
 // One-time device bring-up on device 0 (the reference uses device 0 only,
 // TP/native_fp2/native_fp2.cu:88-93).  Returns false if no HIP device.

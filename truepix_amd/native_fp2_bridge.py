@@ -75,6 +75,19 @@ def bind(lib) -> None:
     _sig(lib, "tp_sc2_prove", [_VP, _VP, _I32, _U64, _U64P, _U64P, _U64P, _U64P, _VP], _I32)
     _sig(lib, "tp_sc2_set_timing", [_I32])
     _sig(lib, "tp_rs_encode", [_VP, _I32, _I32, _I32, _VP, _I32, _VP], _I32)
+    _sig(lib, "tp_early_create_dev",
+         [_I32, _I32P, _I32P, _I32P, _U64P, _U64P, _I32, _I32, _VP, _VP, _U64P, _U64P], _VP)
+    _sig(lib, "tp_eval_layer", [_VP, _I32, _VP, _I32, _I32, _I32, _I32P, _I32P, _I32P, _I32P, _I32,
+                                _VP], _I32)
+    _sig(lib, "tp_wire_table", [_VP, _I32, _I32, _I32, _I32, _VP, _VP, _VP], _I32)
+    _sig(lib, "tp_late_create", [_I32, _I32P, _I32P, _I32P, _U64P, _U64P, _I32, _U64P, _U64P,
+                                 _U64, _U64], _VP)
+    _sig(lib, "tp_late_round", [_VP, _I32, _U64P, _U64P], _I32)
+    _sig(lib, "tp_late_values", [_VP, _U64P, _I32, _U64P], _I32)
+    _sig(lib, "tp_late_destroy", [_VP])
+    _sig(lib, "tp_wiring_mlext", [_I32, _I32P, _I32P, _I32P, _U64P, _U64P, _I32, _U64P, _U64P, _I32,
+                                  _U64P, _U64P, _I32, _U64P], _I32)
+    _sig(lib, "tp_mle_eval_base_dev", [_VP, _I32, _U64P, _U64P, _VP], _I32)
     _sig(lib, "tp_sc2_stats", [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                _I32P])
 
