@@ -68,6 +68,43 @@ def bench_rs_encode(steps: int = 5, cols: int = 64, lc: int = 20, lo: int = 22):
             "alg_bytes": alg}
 
 
+def bench_gkr_frame(frames: int = 5, circuit: str = "gray", n_copies: int = 1350):
+    """Plain GKR prove of one frame (C1 shape: gray, 1350 macroblocks, 384
+    samples each) with this package's prover; proof bytes as the reference's."""
+    import tempfile
+    from truepix_amd import circuits, gkr
+    from truepix_amd.fp2_link import full_input_rows, make_manifest, random_private_inputs, write_manifest
+    from truepix_amd.vole import ExtVOLEGenerator
+    pid = 4242
+    work = tempfile.mkdtemp(prefix="tpbench_")
+    gen = ExtVOLEGenerator(seed=777)
+    gen.generate_deltas()
+    gen.generate_vole_instances(500)
+    gen.save_to_files(os.path.join(work, "vole_ext_prover_%d.json" % pid),
+                      os.path.join(work, "vole_ext_verifier_%d.json" % pid))
+    write_manifest(os.path.join(work, "orion_commit.json"), make_manifest(n_copies, seed=1))
+    parsed = circuits.circuit(circuit)
+    rows = full_input_rows(parsed, random_private_inputs(circuit, parsed, n_copies, seed=105))
+    times = []
+    proof = None
+    for i in range(frames + 1):
+        t0 = time.perf_counter()
+        _, prv = gkr.ProverClass.from_pws(parsed, n_copies)
+        prv.pid = pid
+        prv.vole_dir = work
+        prv.manifest_path = os.path.join(work, "orion_commit.json")
+        proof = prv.run(rows)
+        if i:                                  # the first frame is warm-up
+            times.append(time.perf_counter() - t0)
+    times.sort()
+    med = times[len(times) // 2]
+    return {"workload": "plain GKR prove, %s circuit, %d copies x 384 samples, 1 frame "
+                        "(proof bytes identical to the reference's)" % (circuit, n_copies),
+            "frames_per_s": round(1.0 / med, 3), "ms_per_frame_median": round(med * 1e3, 2),
+            "ms_per_frame_min": round(times[0] * 1e3, 2), "proof_bytes": len(proof),
+            "phases_ms": {k: round(v * 1e3, 2) for k, v in getattr(prv, "timings", {}).items()}}
+
+
 def cpu_baseline(nv_sample: int = 24, nthreads: int = 1, min_seconds: float = 10.0):
     from oracle.oracle import Oracle
     from truepix_amd.sumcheck import synthetic_tables
@@ -170,6 +207,10 @@ def main(argv=None):
                      "alg_bytes_per_launch": round(k_bytes / k_n, 1)},
         "check": {"round0_poly": [int(x) for x in proof.polys[0][:2]]},
     }
+    try:
+        line["gkr_frame"] = bench_gkr_frame()
+    except Exception as exc:
+        line["gkr_frame"] = {"error": str(exc)}
     try:
         line["rs_encode"] = bench_rs_encode()
     except Exception as exc:

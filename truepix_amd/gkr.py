@@ -234,11 +234,13 @@ class CircuitProver:
         assert len(inputs) == self.nCopies
         cp = 1 << self.nCopyBits
         rows = np.zeros((cp, 1 << self.nInBits), dtype=np.uint64)
-        arr = np.asarray(inputs, dtype=object)
-        if arr.ndim != 2:
-            raise ValueError("inputs must be nCopies rows of equal length")
-        rows[:self.nCopies, :arr.shape[1]] = np.array(
-            [[int(v) % P for v in row] for row in inputs], dtype=np.uint64)
+        if isinstance(inputs, np.ndarray) and inputs.dtype.kind in "iu":
+            arr = inputs.astype(np.uint64) % np.uint64(P)
+        else:
+            arr = np.array([[int(v) % P for v in row] for row in inputs], dtype=np.uint64)
+        if arr.ndim != 2 or arr.shape[1] > rows.shape[1]:
+            raise ValueError("inputs must be nCopies rows of at most 2^nInBits values")
+        rows[:self.nCopies, :arr.shape[1]] = arr
         cur = torch.from_numpy(rows.view(np.int64)).to("cuda")
         self.values = [cur]
         stride = rows.shape[1]
@@ -520,6 +522,11 @@ class CircuitProverNIZK(_NIZKBase):
         self.fs.put(self.com.est_val(tag, chal))
 
     def run(self, inputs, muxbits=None):
+        import time as _time
+        marks = [("start", _time.perf_counter())]
+
+        def mark(name):
+            marks.append((name, _time.perf_counter()))
         if self.prover is None:
             self.build_prover()
         pid = os.getpid() if self.pid is None else self.pid
@@ -531,10 +538,13 @@ class CircuitProverNIZK(_NIZKBase):
         self.fs.put(muxbits, True)
         n_inv = 1 << (self.nInBits + self.nCopyBits)
         self.fs.put(["private_input_omitted", n_inv], True)
+        mark("setup")
         prover.set_inputs(inputs)
+        mark("circuit_eval")
         outvals = prover.ckt_outputs.reshape(-1)
         self.fs.put_io_int_array(outvals)
         self.fs.put(commit_binding(read_commit_manifest(self.manifest_path)))
+        mark("absorb_outputs")
 
         nOutBits = self.layOutBits[0]
         z1 = [self._challenge("z1-%d" % i) for i in range(nOutBits)]
@@ -542,6 +552,7 @@ class CircuitProverNIZK(_NIZKBase):
         z2 = [self._challenge("z2-%d" % i) for i in range(self.nCopyBits)]
         project_line = len(self.layers) == 1
         prover.set_z(z1, z2, None, None, project_line)
+        mark("tables")
         prev_rval = None
         for lay in range(len(self.layers)):
             nInBits = self.layInBits[lay]
@@ -592,6 +603,7 @@ class CircuitProverNIZK(_NIZKBase):
                 z1_2 = w2
             project_line = project_next
             z2 = w3
+        mark("sumcheck")
         self.create_pok_final()
         self.final_input_point = list(z1) + list(z2)
         self.final_input_tag = prev_rval
@@ -599,7 +611,11 @@ class CircuitProverNIZK(_NIZKBase):
         self.final_input_value = mle_device(prover.input_vector_device(), self.final_input_point)
         self.create_input_value_proof(prev_rval, self.final_input_value)
         prover.close()
-        return self.fs.to_string()
+        mark("input_mle")
+        out = self.fs.to_string()
+        mark("serialize")
+        self.timings = {marks[i][0]: marks[i][1] - marks[i - 1][1] for i in range(1, len(marks))}
+        return out
 
 
 class CircuitVerifierNIZK(_NIZKBase):
