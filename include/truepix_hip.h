@@ -178,6 +178,83 @@ int tp_wiring_mlext(int n_gates, const int* type_idx, const int* in0, const int*
 int tp_mle_eval_base_dev(const uint64_t* d_inputs, int n_z, const uint64_t* z, uint64_t out2[2],
                          void* stream);
 
+/* ---- whole frames: CircuitProverNIZK.run (TP/libTruePix/circuitnizkvec_vole.py:265-501)
+ *      for a one-layer circuit, as run_truepix_prover.run_truepix calls it
+ *      (TP/run_truepix_prover.py:218-423), device-resident, batched and served --- */
+
+/* A one-layer circuit (the TP/pws files via TP/libTruePix/parse_pws.py:8-64): per
+ * gate the evaluation op and mux bit (as tp_eval_layer), the sumcheck type
+ * index, in0 / in1; input_slot[w] (w < n_inputs) = the private input feeding
+ * input wire w, or -1 for a constant wire whose value is input_const[w].
+ * NULL on a bad shape.  Circuits are shared by any number of jobs. */
+void* tp_circuit_create(int n_gates, const int* op, const int* muxbit, const int* type_idx,
+                        const int* in0, const int* in1, int n_inputs, const int* input_slot,
+                        const uint64_t* input_const);
+void tp_circuit_destroy(void* circuit);
+
+/* One frame to prove.  Inputs [n_copies][n_private] (uint8 or uint64);
+ * pid is the first transcript value (os.getpid() in the reference);
+ * the prover's VOLE correlations U, V (TP/libTruePix/ExtVOLEGen.py:130-187,
+ * one security repetition) and the commitment binding list
+ * (TP/libTruePix/fp2_link.py:487-504) as typed items (kind 0 int, 1 str). */
+typedef struct tp_frame_job {
+    void* circuit;
+    int n_copies;
+    int n_private;
+    int input_bytes;                /* 1 or 8 */
+    const void* inputs;
+    uint64_t pid;
+    int n_vole;
+    const uint64_t* vole_u_re;
+    const uint64_t* vole_u_im;
+    const uint64_t* vole_v_re;
+    const uint64_t* vole_v_im;
+    int n_binding;
+    const int* binding_kind;
+    const uint64_t* binding_int;    /* read where kind == 0 */
+    const char* const* binding_str; /* read where kind == 1 */
+    int keep_outputs;               /* keep the circuit outputs for tp_pool_outputs */
+} tp_frame_job;
+
+/* A pool of n_workers provers (host threads, each with its own HIP stream and
+ * buffers) taking frames from one FIFO queue.  NULL on failure. */
+void* tp_pool_create(int n_workers);
+void tp_pool_destroy(void* pool);
+/* Seconds since the pool was created (the clock of tp_pool_info's times). */
+double tp_pool_now(void* pool);
+/* Queues a frame (everything it reads is copied); job id > 0, or -1. */
+int64_t tp_pool_submit(void* pool, const tp_frame_job* job);
+/* 1 done, 0 not done within timeout_s (< 0 waits forever), -1 failed / unknown. */
+int tp_pool_wait(void* pool, int64_t job, double timeout_s);
+/* Message of a failed job. */
+const char* tp_pool_error(void* pool, int64_t job);
+/* Proof bytes (pickle of the transcript, fiatshamir.py:234-237): *len = size,
+ * copied to buf when cap suffices.  0 / -1. */
+int tp_pool_proof(void* pool, int64_t job, uint8_t* buf, size_t cap, size_t* len);
+/* Final GKR input point (nInBits + nCopyBits variables, 2 limbs each), the
+ * input MLE there, times (submit, start, end) and phase times in ms
+ * (eval+download, output absorb, tables, early, late, tail, input MLE, proof
+ * bytes).  0 / -1. */
+int tp_pool_info(void* pool, int64_t job, uint64_t* point, int point_cap, int* n_point,
+                 uint64_t value[2], double times[3], double phases[8]);
+/* Circuit outputs [copies_pad][2^out_bits] of a job submitted with keep_outputs. */
+int tp_pool_outputs(void* pool, int64_t job, uint64_t* out, size_t cap, size_t* n);
+/* Forgets a finished job. */
+void tp_pool_release(void* pool, int64_t job);
+
+/* ---- host pieces of the frame prover, exported for their CPU tests ------- */
+
+/* A15, TP/libTruePix/fiatshamir.py:234-237: pickle.dumps(tree, 5) of a
+ * token-encoded tree ('i' u64 | 's' u32 len bytes | 'n' | 'f' u64 u64 |
+ * 'l' u32 n items | 't' u32 n items | 'a' u32 n u64[n]); *out is freed with
+ * tp_free.  0 / -1. */
+int tp_pickle_tree(const uint8_t* tokens, size_t n, uint8_t** out, size_t* out_len);
+void tp_free(void* p);
+/* A14: SHA-256 of the transcript's host hash (SHA-NI when present), fed in
+ * pieces of `chunk` bytes (0 = one piece). */
+void tp_sha256(const uint8_t* data, size_t n, size_t chunk, uint8_t out[32]);
+int tp_sha256_has_ni(void);
+
 #ifdef __cplusplus
 }
 #endif

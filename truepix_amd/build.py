@@ -32,8 +32,13 @@ CFLAGS = [
 ]
 
 
+CXX = shutil.which("g++") or "g++"
+# host-only sources (.cpp): the transcript hash and the proof writer
+CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wno-unused-function"]
+
+
 def _sources():
-    return sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
+    return sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
 def _headers_mtime():
@@ -43,11 +48,14 @@ def _headers_mtime():
 
 def _compile(src: str, force: bool) -> str:
     s = os.path.join(CSRC, src)
-    o = os.path.join(OBJ, src[:-4] + ".o")
+    o = os.path.join(OBJ, src.rsplit(".", 1)[0] + (".o" if src.endswith(".hip") else ".host.o"))
     if (not force and os.path.isfile(o)
             and os.path.getmtime(o) >= max(os.path.getmtime(s), _headers_mtime())):
         return o
-    cmd = [HIPCC] + CFLAGS + ["-c", s, "-o", o]
+    if src.endswith(".cpp"):
+        cmd = [CXX] + CXXFLAGS + ["-c", s, "-o", o]
+    else:
+        cmd = [HIPCC] + CFLAGS + ["-c", s, "-o", o]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in r.stdout.splitlines()
                     if "packed-fp32-ops" not in l and "not a recognized feature" not in l)

@@ -1,0 +1,1079 @@
+// frame.hip - device-resident prover of whole TruePix frames, and a pool of
+// such provers for batched and served operation.
+//
+// One frame is one run of CircuitProverNIZK.run
+// (TP/libTruePix/circuitnizkvec_vole.py:265-501) over a one-layer circuit,
+// such as the four edit circuits (TP/pws/{gray,inv,Crop,mask}.pws): circuit
+// evaluation, the nCopyBits early (cubic) rounds, the 2 nInBits late
+// (quadratic) rounds, the line projection, the final product proof, the
+// wiring predicate, the input MLE and the proof bytes.  Every protocol step,
+// transcript absorb and challenge draw happens in the reference's order, so
+// the proof bytes equal the reference prover's for the same inputs, VOLE
+// correlations, commitment binding and pid.
+//
+// Device residency: inputs go up once (the private bytes), outputs come down
+// once (they are absorbed into the transcript and pickled into the proof);
+// between those, tables stay in HBM, challenges travel as kernel arguments
+// and each round's sums come back through a mapped host slot that the host
+// polls (no stream synchronization per round).
+//
+// The pool runs one host thread per worker, each with its own HIP stream,
+// mapped slot and grow-only buffers.  Frames are independent (each has its
+// own transcript), so workers take frames from one FIFO queue as they free
+// up: a batch is a queue filled at once, a served stream is a queue filled as
+// requests arrive.  Kernels of different workers overlap on the GPU; host
+// transcript work (SHA-256 of the outputs, commitments, the proof writer)
+// overlaps across CPU cores.
+#include "fp61.hpp"
+#include "host_sha.hpp"
+#include "kernels.hpp"
+#include "proof_pickle.hpp"
+#include "reduce.hpp"
+#include "runtime.hpp"
+#include "transcript.hpp"
+#include "../../include/truepix_hip.h"
+
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+using namespace tp;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// ------------------------------------------------------------------ kernels
+
+// Input rows [cp][2^row_log] of the circuit's input layer: private inputs
+// where slot[w] >= 0, the constant cval[w] elsewhere, zero rows for padded
+// copies (TP/libTruePix/util.py get_inputs + layerprover.py:113-120 padding).
+__global__ void k_fe_rows(const uint8_t* __restrict__ src8, const u64* __restrict__ src64,
+                          int n_priv, int n_copies, int cp, int row_log,
+                          const int* __restrict__ slot, const u64* __restrict__ cval,
+                          u64* __restrict__ rows)
+{
+    const size_t total = (size_t)cp << row_log;
+    const int mask = (1 << row_log) - 1;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < total;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = k >> row_log;
+        const int w = (int)(k & mask);
+        u64 v = 0;
+        if (c < (size_t)n_copies) {
+            const int s = slot[w];
+            if (s >= 0) {
+                const size_t at = c * n_priv + s;
+                v = src8 ? (u64)src8[at] : fred64(src64[at]);
+            } else {
+                v = cval[w];
+            }
+        }
+        rows[k] = v;
+    }
+}
+
+// eq table: out[k] = prod_j (bit_j(k) ? z_j : 1 - z_j) for k < n_out
+// (TP/native_fp2/native_fp2_core.hpp:139-159, beta with init 1).
+__global__ void k_fe_eq(PointArg z, int n, u64* __restrict__ outr, u64* __restrict__ outi,
+                        int n_out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_out) return;
+    fp2 b = one2();
+    for (int j = 0; j < n; ++j) b = mul2(b, ((k >> j) & 1) ? z.z[j] : sub2(one2(), z.z[j]));
+    outr[k] = b.re;
+    outi[k] = b.im;
+}
+
+// Start of the late rounds (TP/libTruePix/gateprover.py:51-62): each gate's
+// fixed side is V[in1]; its coefficient starts at the z1 eq value; beta* is
+// packed (re, im) for the late-round kernel.
+__global__ void k_fe_late_init(const u64* __restrict__ vr, const u64* __restrict__ vi,
+                               const int* __restrict__ in1, int n_gates,
+                               const u64* __restrict__ z1cr, const u64* __restrict__ z1ci,
+                               u64* __restrict__ z1lr, u64* __restrict__ z1li,
+                               u64* __restrict__ othr, u64* __restrict__ othi,
+                               const u64* __restrict__ br, const u64* __restrict__ bi,
+                               u64* __restrict__ beta2)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0) {
+        beta2[0] = br[0];
+        beta2[1] = bi[0];
+    }
+    if (g >= n_gates) return;
+    othr[g] = vr[in1[g]];
+    othi[g] = vi[in1[g]];
+    z1lr[g] = z1cr[g];
+    z1li[g] = z1ci[g];
+}
+
+// Copies n <= 16 device values to the mapped slot, then the sequence number.
+__global__ void k_publish_copy(const fp2* __restrict__ src, int n, fp2* host_vals,
+                               volatile unsigned* host_seq, unsigned seq)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        for (int k = 0; k < n; ++k) host_vals[k] = src[k];
+        __threadfence_system();
+        *host_seq = seq;
+    }
+}
+
+// --------------------------------------------------------------- host math
+
+const u64 kHalf = 1ULL << 60;                  // 1/2 mod p
+const u64 kThird = (2 * P + 1) / 3;            // 1/3 mod p
+
+fp2 sc(u64 s, fp2 x) { return smul2(s, x); }
+
+u64 powmod(u64 a, u64 e)
+{
+    u64 r = 1;
+    while (e) {
+        if (e & 1) r = fmul(r, a);
+        a = fmul(a, a);
+        e >>= 1;
+    }
+    return r;
+}
+
+// TP/libTruePix/util.py:117-137: coefficients of the cubic through
+// (0, f0), (1, f1), (-1, fm1), (2, f2).
+void interp_cubic(const fp2 f[4], fp2 c[4])
+{
+    const fp2 f0 = f[0], f1 = f[1], fm1 = f[2], f2 = f[3];
+    const fp2 b = sc(kHalf, add2(f1, fm1));
+    const fp2 cc = sc(kHalf, sub2(f1, fm1));
+    const fp2 d = sc(kThird, sub2(f2, fm1));
+    const fp2 k2 = sub2(b, f0);
+    const fp2 k3 = sc(kHalf, sub2(sub2(d, cc), k2));
+    c[0] = f0;
+    c[1] = sub2(cc, k3);
+    c[2] = k2;
+    c[3] = k3;
+}
+
+// TP/libTruePix/util.py:91-115: the quadratic through (0, f0), (1, f1), (-1, fm1).
+void interp_quadratic(const fp2 f[3], fp2 c[3])
+{
+    const fp2 k2 = sub2(sc(kHalf, add2(f[1], f[2])), f[0]);
+    c[0] = f[0];
+    c[1] = sub2(sub2(f[1], k2), f[0]);
+    c[2] = k2;
+}
+
+// TP/libTruePix/util.py:279-312: monomial coefficients of the interpolant
+// through (t, y_t), t = 0..n-1, as out[i] = sum_j L[j][i] y_j with the
+// base-field Lagrange basis coefficients L (cached per n).
+const std::vector<u64>& lagrange_basis(int n)
+{
+    static std::mutex mu;
+    static std::unordered_map<int, std::vector<u64>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(n);
+    if (it != cache.end()) return it->second;
+    std::vector<u64> L((size_t)n * n, 0);
+    for (int j = 0; j < n; ++j) {
+        std::vector<u64> num(1, 1);
+        u64 den = 1;
+        for (int m = 0; m < n; ++m) {
+            if (m == j) continue;
+            std::vector<u64> nxt(num.size() + 1, 0);
+            for (size_t i = 0; i < num.size(); ++i) {   // num * (x - m)
+                nxt[i] = fsub(nxt[i], fmul((u64)m, num[i]));
+                nxt[i + 1] = fadd(nxt[i + 1], num[i]);
+            }
+            num.swap(nxt);
+            den = fmul(den, j >= m ? (u64)(j - m) : P - (u64)(m - j));
+        }
+        const u64 inv = powmod(den, P - 2);
+        for (int i = 0; i < n; ++i) L[(size_t)j * n + i] = fmul(num[i], inv);
+    }
+    return cache.emplace(n, std::move(L)).first->second;
+}
+
+fp2 horner(const std::vector<fp2>& c, fp2 x)
+{
+    fp2 out = c.back();
+    for (size_t k = c.size() - 1; k-- > 0;) out = add2(mul2(out, x), c[k]);
+    return out;
+}
+
+// Commitment-side gate forms f(x, y, xy) of the prover
+// (TP/libTruePix/gateprover_vole.py:706-720, GateFunctionsPC).
+fp2 gate_com(int t, fp2 x, fp2 y, fp2 xy)
+{
+    const fp2 one = one2();
+    switch (t) {
+    case G_MUL: return xy;
+    case G_ADD: return add2(x, y);
+    case G_SUB: return sub2(x, y);
+    case G_MUXL: return x;
+    case G_MUXR: return y;
+    case G_OR: return sub2(add2(x, y), xy);
+    case G_XOR: return sub2(add2(x, y), dbl2(xy));
+    case G_NOT: return sub2(one, x);
+    case G_NAND: return sub2(one, xy);
+    case G_NOR: return sub2(add2(one, xy), add2(x, y));
+    case G_NXOR: return sub2(add2(one, dbl2(xy)), add2(x, y));
+    case G_NAAB: return sub2(y, xy);
+    default: return x;
+    }
+}
+
+int clog2(long long v)
+{
+    int b = 0;
+    while ((1LL << b) < v) ++b;
+    return b;
+}
+
+// ------------------------------------------------------------ circuit, jobs
+
+struct Circuit {
+    int n_gates = 0, in_bits = 0, out_bits = 0, n_inputs = 0, n_priv = 0;
+    std::vector<int> type, in0, in1;
+    DevBuf d_op, d_mux, d_type, d_in0, d_in1, d_slot, d_cval;
+};
+
+struct Job {
+    int64_t id = 0;
+    const Circuit* circ = nullptr;
+    int n_copies = 0, n_priv = 0, in_bytes = 1;
+    std::vector<uint8_t> in8;
+    std::vector<u64> in64;
+    u64 pid = 0;
+    std::vector<u64> ur, ui, vr, vi;
+    std::vector<int> bkind;
+    std::vector<u64> bint;
+    std::vector<std::string> bstr;
+    bool keep_outputs = false;
+    // results
+    std::atomic<int> state{0};   // 0 queued, 1 running, 2 done, 3 failed
+    std::string err;
+    std::vector<uint8_t> proof;
+    std::vector<u64> point;      // (re, im) per variable: nInBits line point, then w3
+    fp2 value = {0, 0};
+    std::vector<u64> outputs;    // [cp][2^out_bits] when keep_outputs
+    double t_submit = 0, t_start = 0, t_end = 0;
+    double phase[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// Per-worker device and host resources (grow-only, reused across frames).
+struct Worker {
+    hipStream_t st = nullptr;
+    MappedSlot slot;
+    DevBuf raw_in, rows_in, rows_out, vr[2], vi[2], br[2], bi[2];
+    DevBuf z1cr, z1ci, z1lr, z1li, othr, othi, partials, small;
+    uint8_t* h_in = nullptr;
+    size_t h_in_bytes = 0;
+    u64* h_out = nullptr;
+    size_t h_out_bytes = 0;
+
+    bool init()
+    {
+        if (hipSetDevice(0) != hipSuccess) return fail("hipSetDevice(0) failed");
+        TP_HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        return slot.ready() && gkr_lds_limits();
+    }
+    bool host_in(size_t n)
+    {
+        if (n <= h_in_bytes) return true;
+        if (h_in) (void)hipHostFree(h_in);
+        h_in = nullptr;
+        TP_HIP_OK(hipHostMalloc((void**)&h_in, n));
+        h_in_bytes = n;
+        return true;
+    }
+    bool host_out(size_t n)
+    {
+        if (n <= h_out_bytes) return true;
+        if (h_out) (void)hipHostFree(h_out);
+        h_out = nullptr;
+        TP_HIP_OK(hipHostMalloc((void**)&h_out, n));
+        h_out_bytes = n;
+        return true;
+    }
+    void release()
+    {
+        DevBuf* all[] = {&raw_in, &rows_in, &rows_out, &vr[0], &vr[1], &vi[0], &vi[1], &br[0],
+                         &br[1], &bi[0], &bi[1], &z1cr, &z1ci, &z1lr, &z1li, &othr, &othi,
+                         &partials, &small};
+        for (DevBuf* b : all) b->release();
+        if (h_in) (void)hipHostFree(h_in);
+        if (h_out) (void)hipHostFree(h_out);
+        h_in = nullptr;
+        h_out = nullptr;
+        slot.release();
+        if (st) (void)hipStreamDestroy(st);
+        st = nullptr;
+    }
+};
+
+// The transcript side of one frame: Fiat-Shamir state, VOLE commitments
+// (TP/libTruePix/ExtVOLECommit.py:180-298, one security repetition) and the
+// proof's transcript entries in order.
+struct Proto {
+    FsTranscriptT<HostSha256> fs;
+    std::vector<PNode> trans;
+    const Job* job = nullptr;
+    int used = 0;
+
+    fp2 chal(const char* label)
+    {
+        return fs.rand_ext(label, (uint32_t)strlen(label));
+    }
+    fp2 chal_num(const char* prefix, int k)
+    {
+        char buf[64];
+        const uint32_t n = label_num(buf, prefix, k);
+        return fs.rand_ext(buf, n);
+    }
+    // put(x) of an Fp2: the entry [x]
+    void put(fp2 x)
+    {
+        fs.item_fp2(x);
+        PNode l = PNode::list();
+        l.kids.push_back(PNode::fp2v(x.re, x.im));
+        trans.push_back(std::move(l));
+    }
+    // put((a, b)): the entry [(a, b)]
+    void put2(fp2 a, fp2 b)
+    {
+        fs.item_fp2(a);
+        fs.item_fp2(b);
+        PNode t = PNode::tuple();
+        t.kids.push_back(PNode::fp2v(a.re, a.im));
+        t.kids.push_back(PNode::fp2v(b.re, b.im));
+        PNode l = PNode::list();
+        l.kids.push_back(std::move(t));
+        trans.push_back(std::move(l));
+    }
+    // commit(m): the next correlation; tag = V, correction d = m - U
+    bool commit(fp2 m, fp2& tag, fp2& d)
+    {
+        if (used >= (int)job->ur.size()) return fail("VOLE instance index out of range");
+        tag = mk2(job->vr[used], job->vi[used]);
+        d = sub2(m, mk2(job->ur[used], job->ui[used]));
+        ++used;
+        return true;
+    }
+};
+
+double now_s(Clock::time_point t0)
+{
+    return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+// ------------------------------------------------------------- one frame
+
+bool prove_frame(Worker& W, Job& J)
+{
+    using std::vector;
+    const Circuit& C = *J.circ;
+    const int nin = C.in_bits, nout = C.out_bits;
+    const int ncb = clog2(J.n_copies);
+    const int cp = 1 << ncb;
+    const int NW = 1 << nin;
+    const int n_z = nin + ncb;
+    if (J.n_copies < 2) return fail("frame: nCopies must be at least 2");
+    if (ncb > 16 || n_z > kMaxZ) return fail("frame: %d copies is beyond this build's limit", J.n_copies);
+    hipStream_t st = W.st;
+    auto t_mark = Clock::now();
+    auto lap = [&](int k) {
+        const auto t = Clock::now();
+        J.phase[k] = std::chrono::duration<double, std::milli>(t - t_mark).count();
+        t_mark = t;
+    };
+
+    // 1. inputs -> rows, evaluate the layer, outputs to the host (A3, A4)
+    const size_t in_bytes = (size_t)J.n_copies * J.n_priv * (J.in_bytes == 1 ? 1 : 8);
+    const size_t rows_n = (size_t)cp << nin;
+    const size_t out_n = (size_t)cp << nout;
+    if (!W.host_in(in_bytes ? in_bytes : 8) || !W.host_out(out_n * 8)) return false;
+    memcpy(W.h_in, J.in_bytes == 1 ? (const void*)J.in8.data() : (const void*)J.in64.data(), in_bytes);
+    if (!W.raw_in.ensure(in_bytes ? in_bytes : 8) || !W.rows_in.ensure(rows_n * 8) ||
+        !W.rows_out.ensure(out_n * 8))
+        return false;
+    if (in_bytes)
+        TP_HIP_OK(hipMemcpyAsync(W.raw_in.ptr, W.h_in, in_bytes, hipMemcpyHostToDevice, st));
+    k_fe_rows<<<grid_for(rows_n, kBlock, 8192), kBlock, 0, st>>>(
+        J.in_bytes == 1 ? W.raw_in.as<uint8_t>() : nullptr,
+        J.in_bytes == 1 ? nullptr : W.raw_in.as<u64>(), J.n_priv, J.n_copies, cp, nin,
+        C.d_slot.as<int>(), C.d_cval.as<u64>(), W.rows_in.as<u64>());
+    TP_HIP_OK(hipGetLastError());
+    kern::k_eval_layer<<<grid_for(out_n, kBlock, 8192), kBlock, 0, st>>>(
+        W.rows_in.as<u64>(), NW, W.rows_out.as<u64>(), nout, J.n_copies, cp, C.d_op.as<int>(),
+        C.d_in0.as<int>(), C.d_in1.as<int>(), C.d_mux.as<int>(), C.n_gates);
+    TP_HIP_OK(hipGetLastError());
+    // wire table [wire][copy] for the early rounds, while the outputs travel
+    const size_t tab_n = (size_t)NW * cp;
+    if (!W.vr[0].ensure(tab_n * 8) || !W.vi[0].ensure(tab_n * 8) || !W.vr[1].ensure(tab_n * 4) ||
+        !W.vi[1].ensure(tab_n * 4) || !W.br[0].ensure((size_t)cp * 8) ||
+        !W.bi[0].ensure((size_t)cp * 8) || !W.br[1].ensure((size_t)cp * 4 + 8) ||
+        !W.bi[1].ensure((size_t)cp * 4 + 8))
+        return false;
+    {
+        dim3 grid((unsigned)((cp + 63) / 64), (unsigned)((NW + 63) / 64));
+        kern::k_wire_table<<<grid, 256, 0, st>>>(W.rows_in.as<u64>(), NW, J.n_copies, cp, NW,
+                                                W.vr[0].as<u64>(), W.vi[0].as<u64>());
+        TP_HIP_OK(hipGetLastError());
+    }
+    TP_HIP_OK(hipMemcpyAsync(W.h_out, W.rows_out.ptr, out_n * 8, hipMemcpyDeviceToHost, st));
+    TP_HIP_OK(hipStreamSynchronize(st));
+    lap(0);
+
+    // 2. transcript head (circuitnizkvec_vole.py:265-300)
+    Proto pr;
+    pr.job = &J;
+    pr.fs.init("truepix");
+    PNode io = PNode::list();
+    pr.fs.int_list(&J.pid, 1);                                   // put(pid)
+    {
+        PNode l = PNode::list();
+        l.kids.push_back(PNode::integer(J.pid));
+        pr.trans.push_back(std::move(l));
+    }
+    pr.fs.item_none();                                           // put(muxbits=None, io)
+    {
+        PNode l = PNode::list();
+        l.kids.push_back(PNode::none());
+        io.kids.push_back(std::move(l));
+    }
+    const u64 n_inv = (u64)1 << n_z;                             // put([..omitted, n], io)
+    pr.fs.item_str("private_input_omitted");
+    pr.fs.item_int(n_inv);
+    {
+        PNode l = PNode::list();
+        l.kids.push_back(PNode::str("private_input_omitted"));
+        l.kids.push_back(PNode::integer(n_inv));
+        io.kids.push_back(std::move(l));
+    }
+    {                                                            // put(outvals, io)
+        const uint8_t tag = 0x01;
+        pr.fs.raw(&tag, 1);
+        pr.fs.le32((uint32_t)out_n);
+        pr.fs.raw(W.h_out, out_n * 8);
+        io.kids.push_back(PNode::int_array(W.h_out, out_n));
+    }
+    {                                                            // put(commit_binding)
+        PNode l = PNode::list();
+        size_t si = 0, ii = 0;
+        for (int kind : J.bkind) {
+            if (kind == 1) {
+                const std::string& s = J.bstr[si++];
+                pr.fs.item_str(s.data(), (uint32_t)s.size());
+                l.kids.push_back(PNode::str(s));
+            } else {
+                const u64 v = J.bint[ii++];
+                pr.fs.item_int(v);
+                l.kids.push_back(PNode::integer(v));
+            }
+        }
+        pr.trans.push_back(std::move(l));
+    }
+    if (J.keep_outputs) J.outputs.assign(W.h_out, W.h_out + out_n);
+    lap(1);
+
+    // 3. z challenges and the early tables (A5)
+    PointArg z1a, z2a;
+    memset(&z1a, 0, sizeof(z1a));
+    memset(&z2a, 0, sizeof(z2a));
+    vector<fp2> z1(nout), z2(ncb);
+    for (int i = 0; i < nout; ++i) z1a.z[i] = z1[i] = pr.chal_num("z1-", i);
+    for (int i = 0; i < ncb; ++i) z2a.z[i] = z2[i] = pr.chal_num("z2-", i);
+    if (!W.z1cr.ensure((size_t)C.n_gates * 8) || !W.z1ci.ensure((size_t)C.n_gates * 8) ||
+        !W.z1lr.ensure((size_t)C.n_gates * 8) || !W.z1li.ensure((size_t)C.n_gates * 8) ||
+        !W.othr.ensure((size_t)C.n_gates * 8) || !W.othi.ensure((size_t)C.n_gates * 8) ||
+        !W.small.ensure(64 * sizeof(fp2)))
+        return false;
+    k_fe_eq<<<(cp + kBlock - 1) / kBlock, kBlock, 0, st>>>(z2a, ncb, W.br[0].as<u64>(),
+                                                          W.bi[0].as<u64>(), cp);
+    k_fe_eq<<<(C.n_gates + kBlock - 1) / kBlock, kBlock, 0, st>>>(z1a, nout, W.z1cr.as<u64>(),
+                                                                 W.z1ci.as<u64>(), C.n_gates);
+    TP_HIP_OK(hipGetLastError());
+    lap(2);
+
+    // 4. the rounds (layer loop of circuitnizkvec_vole.py:318-372, one layer)
+    vector<fp2> v_elem, out_elem, w1, w2, w3;
+    fp2 prev = zero2();
+    bool have_prev = false;
+    fp2* host = static_cast<fp2*>(W.slot.vals);
+    auto pok_temp = [&](const fp2* outs, int n, vector<fp2>& rv) -> bool {
+        rv.resize(n);
+        for (int k = 0; k < n; ++k) {
+            fp2 d;
+            if (!pr.commit(outs[k], rv[k], d)) return false;
+            pr.put(d);
+            if ((int)v_elem.size() < n) {
+                v_elem.push_back(zero2());
+                out_elem.push_back(zero2());
+            }
+        }
+        for (int k = 0; k < n; ++k) {
+            v_elem[k] = add2(v_elem[k], rv[k]);
+            out_elem[k] = add2(out_elem[k], outs[k]);
+        }
+        return true;
+    };
+    auto eq_proof = [&](fp2 v2) {
+        const fp2 c = pr.chal("eq");
+        pr.put(have_prev ? mul2(sub2(prev, v2), c) : mul2(v2, c));
+    };
+    auto round_tail = [&](const fp2* outs, int n, int rd, fp2& nrand) -> bool {
+        vector<fp2> rv;
+        if (!pok_temp(outs, n, rv)) return false;
+        fp2 zp1 = rv[0];
+        for (int k = 0; k < n; ++k) zp1 = add2(zp1, rv[k]);
+        eq_proof(zp1);
+        nrand = pr.chal_num("sc-0-", rd);
+        prev = horner(rv, nrand);
+        have_prev = true;
+        return true;
+    };
+
+    int cur = 0, nc = cp;
+    fp2* parts = nullptr;
+    for (int rd = 0; rd < ncb; ++rd) {                           // early rounds (A6, A7)
+        const int n_pairs = nc >> 1;
+        const int gpt = gates_per_thread(C.n_gates, n_pairs);
+        dim3 grid(grid_for((size_t)n_pairs), (unsigned)((C.n_gates + gpt - 1) / gpt));
+        const unsigned nblk = grid.x * grid.y;
+        if (!W.partials.ensure((size_t)nblk * 4 * sizeof(fp2))) return false;
+        parts = W.partials.as<fp2>();
+        kern::k_early_claim<<<grid, kBlock, 0, st>>>(
+            C.n_gates, gpt, C.d_type.as<int>(), C.d_in0.as<int>(), C.d_in1.as<int>(),
+            W.z1cr.as<u64>(), W.z1ci.as<u64>(), nc, W.vr[cur].as<u64>(), W.vi[cur].as<u64>(),
+            W.br[cur].as<u64>(), W.bi[cur].as<u64>(), parts);
+        const unsigned seq = W.slot.advance();
+        k_publish_sum<4><<<1, kBlock, 0, st>>>(parts, nblk, static_cast<fp2*>(W.slot.dvals),
+                                                W.slot.dseq, seq);
+        TP_HIP_OK(hipGetLastError());
+        if (!W.slot.wait(st, seq, "early claim")) return false;
+        fp2 f[4], c[4];
+        for (int t = 0; t < 4; ++t) f[t] = host[t];
+        interp_cubic(f, c);
+        fp2 r;
+        if (!round_tail(c, 4, rd, r)) return false;
+        w3.push_back(r);
+        const int n_out = nc >> 1;
+        const int log_out = clog2(n_out);
+        const size_t total = ((size_t)NW + 1) * n_out;
+        kern::k_fold_rows<<<grid_for(total), kBlock, 0, st>>>(
+            W.vr[cur].as<u64>(), W.vi[cur].as<u64>(), NW, log_out, r.re, r.im,
+            W.vr[cur ^ 1].as<u64>(), W.vi[cur ^ 1].as<u64>(), W.br[cur].as<u64>(),
+            W.bi[cur].as<u64>(), W.br[cur ^ 1].as<u64>(), W.bi[cur ^ 1].as<u64>());
+        TP_HIP_OK(hipGetLastError());
+        cur ^= 1;
+        nc = n_out;
+    }
+    lap(3);
+
+    // late rounds over V = the folded wire values, [wire][1] (A8, A9)
+    const u64* Vr = W.vr[cur].as<u64>();
+    const u64* Vi = W.vi[cur].as<u64>();
+    u64* beta2 = W.small.as<u64>();
+    fp2* dev_vals = W.small.as<fp2>() + 4;
+    k_fe_late_init<<<(C.n_gates + kBlock - 1) / kBlock, kBlock, 0, st>>>(
+        Vr, Vi, C.d_in1.as<int>(), C.n_gates, W.z1cr.as<u64>(), W.z1ci.as<u64>(),
+        W.z1lr.as<u64>(), W.z1li.as<u64>(), W.othr.as<u64>(), W.othi.as<u64>(),
+        W.br[cur].as<u64>(), W.bi[cur].as<u64>(), beta2);
+    TP_HIP_OK(hipGetLastError());
+    ChalSet ch;
+    memset(&ch, 0, sizeof(ch));
+    const size_t lds = fold_lds(nin);
+    for (int k = 0; k < 2 * nin; ++k) {
+        if (k == nin) {
+            kern::k_late_switch<<<1, kOneWg, lds, st>>>(Vr, Vi, nin, ch, C.n_gates,
+                                                        W.othr.as<u64>(), W.othi.as<u64>());
+            TP_HIP_OK(hipGetLastError());
+        }
+        const unsigned seq = W.slot.advance();
+        kern::k_late_round<<<1, kOneWg, lds, st>>>(
+            C.d_type.as<int>(), C.d_in0.as<int>(), C.d_in1.as<int>(), W.z1lr.as<u64>(),
+            W.z1li.as<u64>(), W.othr.as<u64>(), W.othi.as<u64>(), Vr, Vi, ch, nin, C.n_gates, k,
+            k > 0 ? 1 : 0, beta2, static_cast<fp2*>(W.slot.dvals), W.slot.dseq, seq);
+        TP_HIP_OK(hipGetLastError());
+        if (!W.slot.wait(st, seq, "late round")) return false;
+        fp2 f[3], c[3];
+        for (int t = 0; t < 3; ++t) f[t] = host[t];
+        interp_quadratic(f, c);
+        fp2 r;
+        if (!round_tail(c, 3, ncb + k, r)) return false;
+        ch.c[k] = r;
+        (k < nin ? w1 : w2).push_back(r);
+    }
+    lap(4);
+
+    // 5. line projection H (A10) and the final product proof
+    {
+        const int K = nin + 1;
+        kern::k_line_values<<<K, kOneWg, lds + (size_t)nin * sizeof(fp2), st>>>(Vr, Vi, nin, ch,
+                                                                               dev_vals);
+        const unsigned seq = W.slot.advance();
+        k_publish_copy<<<1, 64, 0, st>>>(dev_vals, K, static_cast<fp2*>(W.slot.dvals), W.slot.dseq,
+                                         seq);
+        TP_HIP_OK(hipGetLastError());
+        if (!W.slot.wait(st, seq, "line values")) return false;
+    }
+    vector<fp2> H(nin + 1, zero2());
+    {
+        const int n = nin + 1;
+        const vector<u64>& L = lagrange_basis(n);
+        for (int j = 0; j < n; ++j) {
+            const fp2 y = host[j];
+            for (int i = 0; i < n; ++i) H[i] = add2(H[i], smul2(L[(size_t)j * n + i], y));
+        }
+    }
+    // launch the wiring predicate now; its result is read after the product proof
+    {
+        WiringArgs a;
+        memset(&a, 0, sizeof(a));
+        for (int j = 0; j < nout; ++j) a.z1[j] = z1[j];
+        for (int j = 0; j < nin; ++j) {
+            a.w1[j] = w1[j];
+            a.w2[j] = w2[j];
+        }
+        const size_t wl = (((size_t)1 << nout) + ((size_t)2 << nin)) * sizeof(fp2);
+        kern::k_wiring_mlext<<<1, kOneWg, wl, st>>>(C.d_type.as<int>(), C.d_in0.as<int>(),
+                                                    C.d_in1.as<int>(), C.n_gates, nout, nin, 0, a,
+                                                    dev_vals + 16);
+        TP_HIP_OK(hipGetLastError());
+    }
+    // create_final_prod_pok (circuitnizkvec_vole.py:175-218)
+    vector<fp2> vvals;
+    fp2 prv[3];
+    {
+        fp2 v1, d1, v3, d3;
+        fp2 sval = zero2();
+        for (const fp2& h : H) sval = add2(sval, h);
+        const fp2 prod = mul2(H[0], sval);
+        if (!pr.commit(H[0], v1, d1) || !pr.commit(prod, v3, d3)) return false;
+        pr.put2(d1, d3);
+        vector<fp2> rest_v;
+        fp2 rest_m = zero2(), rest_t = zero2();
+        for (size_t k = 1; k < H.size(); ++k) {
+            fp2 v, d;
+            if (!pr.commit(H[k], v, d)) return false;
+            rest_v.push_back(v);
+            pr.put(d);
+            rest_m = add2(rest_m, H[k]);
+            rest_t = add2(rest_t, v);
+        }
+        const fp2 c = pr.chal("pok-vec");
+        pr.put2(mul2(rest_m, c), mul2(rest_t, c));
+        vvals.push_back(v1);
+        vvals.insert(vvals.end(), rest_v.begin(), rest_v.end());
+        fp2 vs = zero2();
+        for (const fp2& v : vvals) vs = add2(vs, v);
+        prv[0] = v1;
+        prv[1] = vs;
+        prv[2] = v3;
+        // prod_finish((outs[0], sval, prod), (v1, vs, v3))
+        pr.put2(mul2(prv[0], prv[1]), sub2(add2(mul2(H[0], prv[1]), mul2(sval, prv[0])), prv[2]));
+    }
+    // wiring predicate tV (A11) and its equality proof
+    {
+        const unsigned seq = W.slot.advance();
+        k_publish_copy<<<1, 64, 0, st>>>(dev_vals + 16, 13, static_cast<fp2*>(W.slot.dvals),
+                                         W.slot.dseq, seq);
+        TP_HIP_OK(hipGetLastError());
+        if (!W.slot.wait(st, seq, "wiring predicate")) return false;
+        fp2 mlx = one2();
+        for (int j = 0; j < ncb; ++j) {
+            const fp2 w = w3[j], z = z2[j];
+            mlx = mul2(mlx, sub2(add2(dbl2(mul2(w, z)), one2()), add2(w, z)));
+        }
+        fp2 tv = zero2();
+        const fp2 xy = prv[2];
+        for (int t = 0; t < 13; ++t)
+            tv = add2(tv, mul2(host[t], gate_com(t, prv[0], prv[1], xy)));
+        tv = mul2(tv, mlx);
+        eq_proof(tv);
+    }
+    const fp2 tau = pr.chal("tau-0");
+    prev = horner(vvals, tau);
+    vector<fp2> zline(nin);
+    for (int j = 0; j < nin; ++j) zline[j] = add2(w1[j], mul2(sub2(w2[j], w1[j]), tau));
+    {                                                            // create_pok_final
+        const fp2 c = pr.chal("pok-final");
+        fp2 so = zero2(), sv = zero2();
+        for (const fp2& x : out_elem) so = add2(so, x);
+        for (const fp2& x : v_elem) sv = add2(sv, x);
+        pr.put2(mul2(so, c), mul2(sv, c));
+    }
+    // final input point = line point (nInBits) then w3 (nCopyBits)
+    J.point.clear();
+    PointArg pz;
+    memset(&pz, 0, sizeof(pz));
+    {
+        PNode l = PNode::list();
+        l.kids.push_back(PNode::str("gkr_input_point"));
+        pr.fs.item_str("gkr_input_point");
+        PNode pts = PNode::list();
+        for (int j = 0; j < n_z; ++j) {
+            const fp2 e = j < nin ? zline[j] : w3[j - nin];
+            pz.z[j] = e;
+            J.point.push_back(e.re);
+            J.point.push_back(e.im);
+            pr.fs.item_int(e.re);
+            pr.fs.item_int(e.im);
+            PNode xy = PNode::list();
+            xy.kids.push_back(PNode::integer(e.re));
+            xy.kids.push_back(PNode::integer(e.im));
+            pts.kids.push_back(std::move(xy));
+        }
+        l.kids.push_back(std::move(pts));
+        pr.trans.push_back(std::move(l));
+    }
+    lap(5);
+
+    // 6. input MLE at the point (A12) and its value proof
+    {
+        const int lb = n_z < 10 ? n_z : 10;
+        const size_t nrows = (size_t)1 << (n_z - lb);
+        const unsigned blocks = (unsigned)(nrows < 1024 ? nrows : 1024);
+        if (!W.partials.ensure((size_t)blocks * sizeof(fp2) * 4)) return false;
+        kern::k_mle_base_dev<<<blocks, kBlock, 0, st>>>(W.rows_in.as<u64>(), pz, n_z,
+                                                       W.partials.as<fp2>());
+        const unsigned seq = W.slot.advance();
+        k_publish_sum<1><<<1, kBlock, 0, st>>>(W.partials.as<fp2>(), blocks,
+                                                static_cast<fp2*>(W.slot.dvals), W.slot.dseq, seq);
+        TP_HIP_OK(hipGetLastError());
+        if (!W.slot.wait(st, seq, "input MLE")) return false;
+        J.value = host[0];
+    }
+    {
+        const fp2 c = pr.chal("input-val");
+        pr.put(mul2(prev, c));
+    }
+    lap(6);
+
+    // 7. proof bytes: pickle((q, io, trans, ndb, rvstart, rvend, wDiv, 4), 5)
+    PNode root = PNode::tuple();
+    root.kids.push_back(PNode::integer(P));
+    root.kids.push_back(std::move(io));
+    PNode tr = PNode::list();
+    tr.kids = std::move(pr.trans);
+    root.kids.push_back(std::move(tr));
+    for (int k = 0; k < 4; ++k) root.kids.push_back(PNode::none());
+    root.kids.push_back(PNode::integer(4));
+    pickle_dumps(root, J.proof);
+    lap(7);
+    return true;
+}
+
+// ------------------------------------------------------------------- pool
+
+struct Pool {
+    std::vector<std::thread> threads;
+    std::mutex mu;
+    std::condition_variable cv_job, cv_done;
+    std::deque<Job*> queue;
+    std::unordered_map<int64_t, std::unique_ptr<Job>> jobs;
+    int64_t next_id = 1;
+    bool stop = false;
+    Clock::time_point t0 = Clock::now();
+    std::string init_err;
+};
+
+void worker_main(Pool* pool)
+{
+    Worker W;
+    const bool ok = device_ready() && W.init();
+    if (!ok) {
+        std::lock_guard<std::mutex> lk(pool->mu);
+        pool->init_err = std::string("worker init failed: ") + tp_last_error();
+    }
+    for (;;) {
+        Job* J = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(pool->mu);
+            pool->cv_job.wait(lk, [&] { return pool->stop || !pool->queue.empty(); });
+            if (pool->queue.empty()) break;
+            J = pool->queue.front();
+            pool->queue.pop_front();
+        }
+        J->t_start = now_s(pool->t0);
+        J->state.store(1);
+        clear_error();
+        bool done = ok && prove_frame(W, *J);
+        if (!done) {
+            // drain the stream so no kernel of this frame still runs
+            if (W.st) (void)hipStreamSynchronize(W.st);
+        }
+        J->t_end = now_s(pool->t0);
+        {
+            std::lock_guard<std::mutex> lk(pool->mu);
+            if (!done) J->err = ok ? std::string(tp_last_error()) : pool->init_err;
+            J->state.store(done ? 2 : 3);
+        }
+        pool->cv_done.notify_all();
+    }
+    W.release();
+}
+
+Job* find(Pool* p, int64_t id)
+{
+    auto it = p->jobs.find(id);
+    return it == p->jobs.end() ? nullptr : it->second.get();
+}
+
+}  // namespace
+
+extern "C" {
+
+// A one-layer circuit for the frame prover: per gate the evaluation op and mux
+// bit (as tp_eval_layer takes them), the sumcheck type index, in0 / in1; the
+// input layer of n_inputs wires, input_slot[w] = index of the private input
+// feeding wire w or -1, input_const[w] the constant of a non-private wire.
+void* tp_circuit_create(int n_gates, const int* op, const int* muxbit, const int* type_idx,
+                        const int* in0, const int* in1, int n_inputs, const int* input_slot,
+                        const uint64_t* input_const)
+{
+    clear_error();
+    if (!device_ready()) return nullptr;
+    const int in_bits = clog2(n_inputs), out_bits = clog2(n_gates);
+    if (n_gates < 1 || n_inputs < 1 || in_bits > kMaxNin || out_bits > kMaxNin || in_bits < 1) {
+        fail("tp_circuit_create: bad shape (n_gates=%d n_inputs=%d)", n_gates, n_inputs);
+        return nullptr;
+    }
+    int n_priv = 0;
+    for (int w = 0; w < n_inputs; ++w)
+        if (input_slot[w] >= 0) n_priv = input_slot[w] + 1 > n_priv ? input_slot[w] + 1 : n_priv;
+    for (int g = 0; g < n_gates; ++g)
+        if (in0[g] < 0 || in0[g] >= n_inputs || in1[g] < 0 || in1[g] >= n_inputs ||
+            type_idx[g] < 0 || type_idx[g] > 12 || op[g] < 0 || op[g] > 12) {
+            fail("tp_circuit_create: gate %d out of range", g);
+            return nullptr;
+        }
+    auto* C = new Circuit();
+    C->n_gates = n_gates;
+    C->in_bits = in_bits;
+    C->out_bits = out_bits;
+    C->n_inputs = n_inputs;
+    C->n_priv = n_priv;
+    C->type.assign(type_idx, type_idx + n_gates);
+    C->in0.assign(in0, in0 + n_gates);
+    C->in1.assign(in1, in1 + n_gates);
+    const int NW = 1 << in_bits;
+    std::vector<int> slot(NW, -1);
+    std::vector<u64> cval(NW, 0);
+    for (int w = 0; w < n_inputs; ++w) {
+        slot[w] = input_slot[w];
+        cval[w] = input_slot[w] >= 0 ? 0 : input_const[w] % P;
+    }
+    hipStream_t st = stream();
+    auto up = [&](DevBuf& b, const void* src, size_t n) {
+        return b.ensure(n) && hipMemcpyAsync(b.ptr, src, n, hipMemcpyHostToDevice, st) == hipSuccess;
+    };
+    const size_t gb = (size_t)n_gates * 4;
+    bool ok = up(C->d_op, op, gb) && up(C->d_mux, muxbit, gb) && up(C->d_type, type_idx, gb) &&
+              up(C->d_in0, in0, gb) && up(C->d_in1, in1, gb) &&
+              up(C->d_slot, slot.data(), (size_t)NW * 4) && up(C->d_cval, cval.data(), (size_t)NW * 8) &&
+              hipStreamSynchronize(st) == hipSuccess;
+    if (!ok) {
+        if (!tp_last_error_set()) fail("tp_circuit_create: upload failed");
+        delete C;
+        return nullptr;
+    }
+    return C;
+}
+
+void tp_circuit_destroy(void* c)
+{
+    auto* C = static_cast<Circuit*>(c);
+    if (!C) return;
+    DevBuf* all[] = {&C->d_op, &C->d_mux, &C->d_type, &C->d_in0, &C->d_in1, &C->d_slot, &C->d_cval};
+    for (DevBuf* b : all) b->release();
+    delete C;
+}
+
+void* tp_pool_create(int n_workers)
+{
+    clear_error();
+    if (!device_ready()) return nullptr;
+    if (n_workers < 1 || n_workers > 64) {
+        fail("tp_pool_create: n_workers=%d outside [1, 64]", n_workers);
+        return nullptr;
+    }
+    auto* p = new Pool();
+    for (int i = 0; i < n_workers; ++i) p->threads.emplace_back(worker_main, p);
+    return p;
+}
+
+void tp_pool_destroy(void* pool)
+{
+    auto* p = static_cast<Pool*>(pool);
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(p->mu);
+        p->stop = true;
+    }
+    p->cv_job.notify_all();
+    for (auto& t : p->threads) t.join();
+    delete p;
+}
+
+// Seconds since the pool was created (the clock of the job times).
+double tp_pool_now(void* pool)
+{
+    auto* p = static_cast<Pool*>(pool);
+    return p ? now_s(p->t0) : 0.0;
+}
+
+// Queues one frame; the job copies everything it reads.  Returns the job id
+// (> 0) or -1.
+int64_t tp_pool_submit(void* pool, const tp_frame_job* d)
+{
+    clear_error();
+    auto* p = static_cast<Pool*>(pool);
+    if (!p || !d || !d->circuit) return fail("tp_pool_submit: null argument"), -1;
+    const auto* C = static_cast<const Circuit*>(d->circuit);
+    if (d->n_copies < 2 || d->n_private != C->n_priv || (d->input_bytes != 1 && d->input_bytes != 8))
+        return fail("tp_pool_submit: bad frame shape (copies=%d private=%d bytes=%d)", d->n_copies,
+                    d->n_private, d->input_bytes), -1;
+    if (d->n_vole < 0 || d->n_binding < 0) return fail("tp_pool_submit: bad counts"), -1;
+    auto J = std::make_unique<Job>();
+    J->circ = C;
+    J->n_copies = d->n_copies;
+    J->n_priv = d->n_private;
+    J->in_bytes = d->input_bytes;
+    const size_t n = (size_t)d->n_copies * d->n_private;
+    if (d->input_bytes == 1) {
+        const auto* s = static_cast<const uint8_t*>(d->inputs);
+        J->in8.assign(s, s + n);
+    } else {
+        const auto* s = static_cast<const u64*>(d->inputs);
+        J->in64.assign(s, s + n);
+    }
+    J->pid = d->pid;
+    J->ur.assign(d->vole_u_re, d->vole_u_re + d->n_vole);
+    J->ui.assign(d->vole_u_im, d->vole_u_im + d->n_vole);
+    J->vr.assign(d->vole_v_re, d->vole_v_re + d->n_vole);
+    J->vi.assign(d->vole_v_im, d->vole_v_im + d->n_vole);
+    for (int k = 0; k < d->n_binding; ++k) {
+        J->bkind.push_back(d->binding_kind[k]);
+        if (d->binding_kind[k] == 1) J->bstr.emplace_back(d->binding_str[k]);
+        else J->bint.push_back(d->binding_int[k]);
+    }
+    J->keep_outputs = d->keep_outputs != 0;
+    Job* raw = J.get();
+    {
+        std::lock_guard<std::mutex> lk(p->mu);
+        raw->id = p->next_id++;
+        raw->t_submit = now_s(p->t0);
+        p->jobs.emplace(raw->id, std::move(J));
+        p->queue.push_back(raw);
+    }
+    p->cv_job.notify_one();
+    return raw->id;
+}
+
+// 1 = done, 0 = still running after timeout_s (< 0: wait forever), -1 = failed
+// or unknown job.
+int tp_pool_wait(void* pool, int64_t job, double timeout_s)
+{
+    auto* p = static_cast<Pool*>(pool);
+    if (!p) return -1;
+    std::unique_lock<std::mutex> lk(p->mu);
+    Job* J = find(p, job);
+    if (!J) return -1;
+    auto finished = [&] { return J->state.load() >= 2; };
+    if (timeout_s < 0) p->cv_done.wait(lk, finished);
+    else if (!p->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), finished)) return 0;
+    return J->state.load() == 2 ? 1 : -1;
+}
+
+// Message of a failed job ("" otherwise).
+const char* tp_pool_error(void* pool, int64_t job)
+{
+    auto* p = static_cast<Pool*>(pool);
+    if (!p) return "null pool";
+    std::lock_guard<std::mutex> lk(p->mu);
+    Job* J = find(p, job);
+    return J ? J->err.c_str() : "unknown job";
+}
+
+// Proof bytes of a finished job: *len = size; copies when cap suffices.
+// 0 / -1.
+int tp_pool_proof(void* pool, int64_t job, uint8_t* buf, size_t cap, size_t* len)
+{
+    auto* p = static_cast<Pool*>(pool);
+    if (!p) return -1;
+    std::lock_guard<std::mutex> lk(p->mu);
+    Job* J = find(p, job);
+    if (!J || J->state.load() != 2) return -1;
+    if (len) *len = J->proof.size();
+    if (buf && cap >= J->proof.size()) memcpy(buf, J->proof.data(), J->proof.size());
+    return 0;
+}
+
+// Final input point (2 limbs per variable, n_point variables written), input
+// MLE value, times (submit, start, end in pool seconds) and phase times (ms).
+int tp_pool_info(void* pool, int64_t job, uint64_t* point, int point_cap, int* n_point,
+                 uint64_t value[2], double times[3], double phases[8])
+{
+    auto* p = static_cast<Pool*>(pool);
+    if (!p) return -1;
+    std::lock_guard<std::mutex> lk(p->mu);
+    Job* J = find(p, job);
+    if (!J || J->state.load() < 2) return -1;
+    const int np = (int)J->point.size() / 2;
+    if (n_point) *n_point = np;
+    if (point)
+        for (int k = 0; k < np && k < point_cap; ++k) {
+            point[2 * k] = J->point[2 * k];
+            point[2 * k + 1] = J->point[2 * k + 1];
+        }
+    if (value) {
+        value[0] = J->value.re;
+        value[1] = J->value.im;
+    }
+    if (times) {
+        times[0] = J->t_submit;
+        times[1] = J->t_start;
+        times[2] = J->t_end;
+    }
+    if (phases)
+        for (int k = 0; k < 8; ++k) phases[k] = J->phase[k];
+    return 0;
+}
+
+// Circuit outputs [cp][2^out_bits] of a finished job submitted with
+// keep_outputs; *n = count, copied when cap suffices.  0 / -1.
+int tp_pool_outputs(void* pool, int64_t job, uint64_t* out, size_t cap, size_t* n)
+{
+    auto* p = static_cast<Pool*>(pool);
+    if (!p) return -1;
+    std::lock_guard<std::mutex> lk(p->mu);
+    Job* J = find(p, job);
+    if (!J || J->state.load() != 2) return -1;
+    if (n) *n = J->outputs.size();
+    if (out && cap >= J->outputs.size()) memcpy(out, J->outputs.data(), J->outputs.size() * 8);
+    return 0;
+}
+
+// Forgets a finished job (a queued or running job is kept).
+void tp_pool_release(void* pool, int64_t job)
+{
+    auto* p = static_cast<Pool*>(pool);
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(p->mu);
+    Job* J = find(p, job);
+    if (J && J->state.load() >= 2) p->jobs.erase(job);
+}
+
+}  // extern "C"

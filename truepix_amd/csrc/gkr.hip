@@ -14,11 +14,14 @@
 // row_stride = 2^bits (copy-major, as the proof's input vector is laid out);
 // Fp2 tables are SoA (re[], im[]) like the early-round session.
 #include "fp61.hpp"
+#include "kernels.hpp"
 #include "reduce.hpp"
 #include "runtime.hpp"
 
 #include <stdint.h>
 #include <string.h>
+
+#include <mutex>
 #include <vector>
 
 using namespace tp;
@@ -44,6 +47,54 @@ __device__ __forceinline__ u64 eval_gate(int op, u64 x, u64 y, int muxbit)
     default: return x;   // PASS
     }
 }
+
+__device__ __forceinline__ fp2 ld2(const u64* r, const u64* i, size_t k) { return mk2(r[k], i[k]); }
+
+// Loads the 2^nb table into T and folds it with cs[0..nf), ping-ponging
+// between T and U; returns the buffer holding the result.
+__device__ fp2* lds_fold_chain(fp2* T, fp2* U, const u64* vr, const u64* vi, int nb,
+                               const fp2* cs, int nf)
+{
+    const int W = 1 << nb;
+    for (int w = threadIdx.x; w < W; w += blockDim.x) T[w] = ld2(vr, vi, w);
+    __syncthreads();
+    fp2* src = T;
+    fp2* dst = U;
+    for (int j = 0; j < nf; ++j) {
+        const fp2 r = cs[j];
+        const int n_out = W >> (j + 1);
+        for (int i = threadIdx.x; i < n_out; i += blockDim.x)
+            dst[i] = fold2(src[2 * i], src[2 * i + 1], r);
+        __syncthreads();
+        fp2* t = src;
+        src = dst;
+        dst = t;
+    }
+    return src;
+}
+
+// eq table in LDS by doubling: tab[t] = prod_j (bit_j(t) ? z_j : 1 - z_j).
+__device__ void lds_eq(fp2* tab, const fp2* z, int nb)
+{
+    if (threadIdx.x == 0) tab[0] = one2();
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        const int cur = 1 << j;
+        const fp2 zz = z[j];
+        const fp2 omz = sub2(one2(), zz);
+        for (int t = threadIdx.x; t < cur; t += blockDim.x) {
+            const fp2 b = tab[t];
+            tab[t + cur] = mul2(b, zz);
+            tab[t] = mul2(b, omz);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+namespace tp {
+namespace kern {
 
 // A3: out[c][g] = gate_g(in[c][in0_g], in[c][in1_g]) for c < n_copies and
 // g < n_gates; zero for g >= n_gates and for padded copies < n_copies_pad.
@@ -90,36 +141,6 @@ __global__ void k_wire_table(const u64* __restrict__ rows, int row_stride, int n
     }
 }
 
-// ---------------------------------------------------------------- LDS folds
-
-constexpr int kOneWg = 512;              // threads of the one-workgroup kernels
-constexpr int kMaxNin = 11;              // 2^11 wires: two 32 KiB LDS buffers
-
-__device__ __forceinline__ fp2 ld2(const u64* r, const u64* i, size_t k) { return mk2(r[k], i[k]); }
-
-// Loads the 2^nb table into T and folds it with cs[0..nf) (interleaved re,
-// im), ping-ponging between T and U; returns the buffer holding the result.
-__device__ fp2* lds_fold_chain(fp2* T, fp2* U, const u64* vr, const u64* vi, int nb,
-                               const u64* cs, int nf)
-{
-    const int W = 1 << nb;
-    for (int w = threadIdx.x; w < W; w += blockDim.x) T[w] = ld2(vr, vi, w);
-    __syncthreads();
-    fp2* src = T;
-    fp2* dst = U;
-    for (int j = 0; j < nf; ++j) {
-        const fp2 r = mk2(cs[2 * j], cs[2 * j + 1]);
-        const int n_out = W >> (j + 1);
-        for (int i = threadIdx.x; i < n_out; i += blockDim.x)
-            dst[i] = fold2(src[2 * i], src[2 * i + 1], r);
-        __syncthreads();
-        fp2* t = src;
-        src = dst;
-        dst = t;
-    }
-    return src;
-}
-
 // ---------------------------------------------------------------- late rounds
 // Round rd < nin binds bit rd of in0 (w1), round nin + k binds bit k of in1
 // (w2).  For a gate with e = the bound bit of its input w and T the table
@@ -135,15 +156,15 @@ __global__ void __launch_bounds__(kOneWg) k_late_round(
     const int* __restrict__ gt, const int* __restrict__ in0, const int* __restrict__ in1,
     u64* __restrict__ z1r, u64* __restrict__ z1i, const u64* __restrict__ oth_r,
     const u64* __restrict__ oth_i, const u64* __restrict__ vr, const u64* __restrict__ vi,
-    const u64* __restrict__ chal, int nin, int n_gates, int rd, int apply, u64 beta_re,
-    u64 beta_im, fp2* host_vals, volatile unsigned* host_seq, unsigned seq)
+    ChalSet chal, int nin, int n_gates, int rd, int apply, const u64* __restrict__ beta,
+    fp2* host_vals, volatile unsigned* host_seq, unsigned seq)
 {
     extern __shared__ fp2 lds[];
     fp2* T = lds;
     fp2* U = lds + (1 << nin);
     if (apply) {
         const int k = rd - 1;
-        const fp2 r = mk2(chal[2 * k], chal[2 * k + 1]);
+        const fp2 r = chal.c[k];
         const fp2 omr = sub2(one2(), r);
         const bool w1pass = k < nin;
         const int bit = w1pass ? k : k - nin;
@@ -156,7 +177,7 @@ __global__ void __launch_bounds__(kOneWg) k_late_round(
     }
     const bool w1pass = rd < nin;
     const int kk = w1pass ? rd : rd - nin;
-    const fp2* Tk = lds_fold_chain(T, U, vr, vi, nin, chal + (w1pass ? 0 : 2 * nin), kk);
+    const fp2* Tk = lds_fold_chain(T, U, vr, vi, nin, chal.c + (w1pass ? 0 : nin), kk);
     fp2 acc[3] = {zero2(), zero2(), zero2()};
     const fp2 m_one = mk2(P - 1, 0), two = mk2(2, 0);
     for (int g = threadIdx.x; g < n_gates; g += blockDim.x) {
@@ -177,8 +198,8 @@ __global__ void __launch_bounds__(kOneWg) k_late_round(
     }
     block_sum<3>(acc);
     if (threadIdx.x == 0) {
-        const fp2 beta = mk2(beta_re, beta_im);
-        for (int t = 0; t < 3; ++t) host_vals[t] = mul2(acc[t], beta);
+        const fp2 b = mk2(beta[0], beta[1]);
+        for (int t = 0; t < 3; ++t) host_vals[t] = mul2(acc[t], b);
         __threadfence_system();
         *host_seq = seq;
     }
@@ -188,12 +209,12 @@ __global__ void __launch_bounds__(kOneWg) k_late_round(
 // folded with all w1 challenges; gateprover.py:150-158).
 __global__ void __launch_bounds__(kOneWg) k_late_switch(const u64* __restrict__ vr,
                                                         const u64* __restrict__ vi, int nin,
-                                                        const u64* __restrict__ chal, int n_gates,
+                                                        ChalSet chal, int n_gates,
                                                         u64* __restrict__ oth_r,
                                                         u64* __restrict__ oth_i)
 {
     extern __shared__ fp2 lds[];
-    const fp2* R = lds_fold_chain(lds, lds + (1 << nin), vr, vi, nin, chal, nin);
+    const fp2* R = lds_fold_chain(lds, lds + (1 << nin), vr, vi, nin, chal.c, nin);
     const fp2 vw1 = R[0];
     for (int g = threadIdx.x; g < n_gates; g += blockDim.x) {
         oth_r[g] = vw1.re;
@@ -204,40 +225,38 @@ __global__ void __launch_bounds__(kOneWg) k_late_switch(const u64* __restrict__ 
 // A10: MLE of a 2^nb Fp2 table at K points (one workgroup per point).
 __global__ void __launch_bounds__(kOneWg) k_mle_points(const u64* __restrict__ vr,
                                                        const u64* __restrict__ vi, int nb,
-                                                       const u64* __restrict__ pts,
+                                                       const fp2* __restrict__ pts,
                                                        fp2* __restrict__ out)
 {
     extern __shared__ fp2 lds[];
     const fp2* R = lds_fold_chain(lds, lds + (1 << nb), vr, vi, nb,
-                                  pts + (size_t)blockIdx.x * 2 * nb, nb);
+                                  pts + (size_t)blockIdx.x * nb, nb);
+    if (threadIdx.x == 0) out[blockIdx.x] = R[0];
+}
+
+// A10 on the line of the late rounds: workgroup t evaluates V at
+// w1 + t (w2 - w1) (TP/libTruePix/layercompute.py:169-243); the point is
+// formed in LDS past the two fold buffers; dynamic LDS fold_lds(nb) + nb Fp2.
+__global__ void __launch_bounds__(kOneWg) k_line_values(const u64* __restrict__ vr,
+                                                        const u64* __restrict__ vi, int nb,
+                                                        ChalSet w12, fp2* __restrict__ out)
+{
+    extern __shared__ fp2 lds[];
+    fp2* pt = lds + ((3 << nb) >> 1);   // past the fold buffers (fold_lds)
+    const fp2 t = mk2(blockIdx.x, 0);
+    for (int j = threadIdx.x; j < nb; j += blockDim.x)
+        pt[j] = add2(w12.c[j], mul2(t, sub2(w12.c[nb + j], w12.c[j])));
+    __syncthreads();
+    const fp2* R = lds_fold_chain(lds, lds + (1 << nb), vr, vi, nb, pt, nb);
     if (threadIdx.x == 0) out[blockIdx.x] = R[0];
 }
 
 // A11: per gate type, sum_g beta_z1[g] * beta_w1[in0_g] * beta_w2[in1_g]
 // (with use_z1_2: mul0 beta_z1[g] + mul1 beta_z1_2[g] in place of
 // beta_z1[g]); eq tables built in LDS by doubling; one workgroup.
-__device__ void lds_eq(fp2* tab, const u64* z, int nb)
-{
-    if (threadIdx.x == 0) tab[0] = one2();
-    __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        const int cur = 1 << j;
-        const fp2 zz = mk2(z[2 * j], z[2 * j + 1]);
-        const fp2 omz = sub2(one2(), zz);
-        for (int t = threadIdx.x; t < cur; t += blockDim.x) {
-            const fp2 b = tab[t];
-            tab[t + cur] = mul2(b, zz);
-            tab[t] = mul2(b, omz);
-        }
-        __syncthreads();
-    }
-}
-
 __global__ void __launch_bounds__(kOneWg) k_wiring_mlext(
     const int* __restrict__ type_idx, const int* __restrict__ in0, const int* __restrict__ in1,
-    int n_gates, const u64* __restrict__ z1, const u64* __restrict__ z1_2, int n_out_bits,
-    const u64* __restrict__ w1, const u64* __restrict__ w2, int n_in_bits, u64 m0r, u64 m0i,
-    u64 m1r, u64 m1i, int use_z1_2, fp2* __restrict__ out13)
+    int n_gates, int n_out_bits, int n_in_bits, int use_z1_2, WiringArgs a, fp2* __restrict__ out13)
 {
     extern __shared__ fp2 sm[];
     const int WO = 1 << n_out_bits, WI = 1 << n_in_bits;
@@ -245,16 +264,15 @@ __global__ void __launch_bounds__(kOneWg) k_wiring_mlext(
     fp2* bz2 = bz + WO;
     fp2* b1 = bz2 + (use_z1_2 ? WO : 0);
     fp2* b2 = b1 + WI;
-    lds_eq(bz, z1, n_out_bits);
-    if (use_z1_2) lds_eq(bz2, z1_2, n_out_bits);
-    lds_eq(b1, w1, n_in_bits);
-    lds_eq(b2, w2, n_in_bits);
+    lds_eq(bz, a.z1, n_out_bits);
+    if (use_z1_2) lds_eq(bz2, a.z1b, n_out_bits);
+    lds_eq(b1, a.w1, n_in_bits);
+    lds_eq(b2, a.w2, n_in_bits);
     fp2 acc[13];
 #pragma unroll
     for (int t = 0; t < 13; ++t) acc[t] = zero2();
-    const fp2 m0 = mk2(m0r, m0i), m1 = mk2(m1r, m1i);
     for (int g = threadIdx.x; g < n_gates; g += blockDim.x) {
-        fp2 tv = use_z1_2 ? add2(mul2(m0, bz[g]), mul2(m1, bz2[g])) : bz[g];
+        fp2 tv = use_z1_2 ? add2(mul2(a.m0, bz[g]), mul2(a.m1, bz2[g])) : bz[g];
         tv = mul2(tv, mul2(b1[in0[g]], b2[in1[g]]));
         const int t = type_idx[g];
 #pragma unroll
@@ -270,21 +288,20 @@ __global__ void __launch_bounds__(kOneWg) k_wiring_mlext(
 
 // A12 on a device-resident base-field vector of 2^n_z entries: one partial
 // per workgroup over rows of 2^lb inputs sharing an LDS low-half eq table.
-__global__ void k_mle_base_dev(const u64* __restrict__ in, const u64* __restrict__ z, int n_z,
-                               fp2* partials)
+__global__ void k_mle_base_dev(const u64* __restrict__ in, PointArg z, int n_z, fp2* partials)
 {
     __shared__ fp2 lo[1024];
     __shared__ fp2 hrow;
     const int lb = n_z < 10 ? n_z : 10;
     const int row_len = 1 << lb;
     const size_t n_rows = (size_t)1 << (n_z - lb);
-    lds_eq(lo, z, lb);
+    lds_eq(lo, z.z, lb);
     fp2 acc[1] = {zero2()};
     for (size_t row = blockIdx.x; row < n_rows; row += gridDim.x) {
         if (threadIdx.x == 0) {
             fp2 h = one2();
             for (int j = lb; j < n_z; ++j) {
-                const fp2 zz = mk2(z[2 * j], z[2 * j + 1]);
+                const fp2 zz = z.z[j];
                 h = mul2(h, ((row >> (j - lb)) & 1) ? zz : sub2(one2(), zz));
             }
             hrow = h;
@@ -300,47 +317,38 @@ __global__ void k_mle_base_dev(const u64* __restrict__ in, const u64* __restrict
     block_partial<1>(acc, partials);
 }
 
+}  // namespace kern
+
+size_t fold_lds(int nb) { return ((size_t)3 << nb) / 2 * sizeof(fp2); }
+
+bool gkr_lds_limits()
+{
+    static std::once_flag once;
+    static bool ok = false;
+    std::call_once(once, [] {
+        const int need = (int)(((size_t)4 << kMaxNin) * sizeof(fp2) / 2);
+        ok = hipFuncSetAttribute((const void*)kern::k_late_round,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, need) == hipSuccess &&
+             hipFuncSetAttribute((const void*)kern::k_late_switch,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, need) == hipSuccess &&
+             hipFuncSetAttribute((const void*)kern::k_mle_points,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, need) == hipSuccess &&
+             hipFuncSetAttribute((const void*)kern::k_line_values,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 need + kMaxNin * (int)sizeof(fp2)) == hipSuccess &&
+             hipFuncSetAttribute((const void*)kern::k_wiring_mlext,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 2 * need) == hipSuccess;
+    });
+    return ok ? true : fail("hipFuncSetAttribute (dynamic LDS) failed");
+}
+
+}  // namespace tp
+
+namespace {
+
 // ------------------------------------------------------------------ host side
 
-struct Mapped {
-    fp2* vals = nullptr;
-    unsigned* seq = nullptr;
-    fp2* dvals = nullptr;
-    unsigned* dseq = nullptr;
-    unsigned next = 0;
-};
-Mapped g_map;
-
-bool mapped_ready()
-{
-    if (g_map.vals) return true;
-    void* hv = nullptr;
-    TP_HIP_OK(hipHostMalloc(&hv, 512, hipHostMallocMapped | hipHostMallocCoherent));
-    g_map.vals = static_cast<fp2*>(hv);
-    g_map.seq = reinterpret_cast<unsigned*>(static_cast<char*>(hv) + 256);
-    *g_map.seq = 0;
-    void* dv = nullptr;
-    TP_HIP_OK(hipHostGetDevicePointer(&dv, hv, 0));
-    g_map.dvals = static_cast<fp2*>(dv);
-    g_map.dseq = reinterpret_cast<unsigned*>(static_cast<char*>(dv) + 256);
-    return true;
-}
-
-bool wait_mapped(hipStream_t st, unsigned seq)
-{
-    volatile unsigned* hs = g_map.seq;
-    long spins = 0;
-    while (*hs != seq) {
-        if (++spins % 4096 == 0) {
-            const hipError_t q = hipStreamQuery(st);
-            if (q != hipSuccess && q != hipErrorNotReady)
-                return fail("late round: stream error %s", hipGetErrorString(q));
-            if (q == hipSuccess && *hs != seq) return fail("late round: result not published");
-        }
-    }
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
-    return true;
-}
+MappedSlot g_map;   // results of the ABI's late rounds
 
 bool h2d(DevBuf& b, const void* src, size_t bytes, hipStream_t st)
 {
@@ -349,32 +357,13 @@ bool h2d(DevBuf& b, const void* src, size_t bytes, hipStream_t st)
     return true;
 }
 
-size_t fold_lds(int nb) { return ((size_t)3 << nb) / 2 * sizeof(fp2); }
-
-bool set_lds_limits()
-{
-    static bool done = false;
-    if (done) return true;
-    const int need = (int)(((size_t)4 << kMaxNin) * sizeof(fp2) / 2);
-    TP_HIP_OK(hipFuncSetAttribute((const void*)k_late_round,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, need));
-    TP_HIP_OK(hipFuncSetAttribute((const void*)k_late_switch,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, need));
-    TP_HIP_OK(hipFuncSetAttribute((const void*)k_mle_points,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, need));
-    TP_HIP_OK(hipFuncSetAttribute((const void*)k_wiring_mlext,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 2 * need));
-    done = true;
-    return true;
-}
-
 struct LateSession {
     int n_gates = 0, nin = 0, next_round = 0;
-    u64 beta_re = 0, beta_im = 0;
-    DevBuf gt, i0, i1, z1r, z1i, othr, othi, vr, vi, chal;
+    ChalSet chal;                         // challenges received so far
+    DevBuf gt, i0, i1, z1r, z1i, othr, othi, vr, vi, beta;
     ~LateSession()
     {
-        DevBuf* all[] = {&gt, &i0, &i1, &z1r, &z1i, &othr, &othi, &vr, &vi, &chal};
+        DevBuf* all[] = {&gt, &i0, &i1, &z1r, &z1i, &othr, &othi, &vr, &vi, &beta};
         for (DevBuf* b : all) b->release();
     }
 };
@@ -406,7 +395,7 @@ int tp_eval_layer(const uint64_t* d_in, int in_stride, uint64_t* d_out, int out_
         !h2d(b_mx, muxbit, gb, st))
         return -1;
     const size_t total = (size_t)n_copies_pad << out_log;
-    k_eval_layer<<<grid_for(total, kBlock, 8192), kBlock, 0, st>>>(
+    kern::k_eval_layer<<<grid_for(total, kBlock, 8192), kBlock, 0, st>>>(
         (const u64*)d_in, in_stride, (u64*)d_out, out_log, n_copies, n_copies_pad,
         b_op.as<int>(), b_i0.as<int>(), b_i1.as<int>(), b_mx.as<int>(), n_gates);
     TP_HIP_OK(hipGetLastError());
@@ -424,7 +413,7 @@ int tp_wire_table(const uint64_t* d_rows, int row_stride, int n_copies, int n_co
         return tp::fail("tp_wire_table: bad shape"), -1;
     hipStream_t st = stream_handle ? (hipStream_t)stream_handle : tp::stream();
     dim3 grid((unsigned)((n_copies_pad + 63) / 64), (unsigned)((n_wires + 63) / 64));
-    k_wire_table<<<grid, 256, 0, st>>>((const u64*)d_rows, row_stride, n_copies, n_copies_pad,
+    kern::k_wire_table<<<grid, 256, 0, st>>>((const u64*)d_rows, row_stride, n_copies, n_copies_pad,
                                        n_wires, (u64*)d_re, (u64*)d_im);
     TP_HIP_OK(hipGetLastError());
     return 0;
@@ -439,7 +428,7 @@ void* tp_late_create(int n_gates, const int* type_idx, const int* in0, const int
                      uint64_t beta_im)
 {
     tp::clear_error();
-    if (!tp::device_ready() || !mapped_ready() || !set_lds_limits()) return nullptr;
+    if (!tp::device_ready() || !g_map.ready() || !gkr_lds_limits()) return nullptr;
     if (n_gates <= 0 || nin < 1 || nin > kMaxNin) {
         tp::fail("tp_late_create: bad shape (n_gates=%d nin=%d)", n_gates, nin);
         return nullptr;
@@ -453,8 +442,7 @@ void* tp_late_create(int n_gates, const int* type_idx, const int* in0, const int
     auto* s = new LateSession();
     s->n_gates = n_gates;
     s->nin = nin;
-    s->beta_re = beta_re;
-    s->beta_im = beta_im;
+    const u64 beta2[2] = {beta_re, beta_im};
     const size_t W = (size_t)1 << nin;
     // fixed side during the w1 pass: V[in1] (gateprover.py:51-62 reset())
     std::vector<u64> oth_r(n_gates), oth_i(n_gates);
@@ -463,7 +451,7 @@ void* tp_late_create(int n_gates, const int* type_idx, const int* in0, const int
               h2d(s->i1, in1, n_gates * 4, st) && h2d(s->z1r, z1_re, n_gates * 8, st) &&
               h2d(s->z1i, z1_im, n_gates * 8, st) && h2d(s->othr, oth_r.data(), n_gates * 8, st) &&
               h2d(s->othi, oth_i.data(), n_gates * 8, st) && h2d(s->vr, v_re, W * 8, st) &&
-              h2d(s->vi, v_im, W * 8, st) && s->chal.ensure(4 * nin * 8);
+              h2d(s->vi, v_im, W * 8, st) && h2d(s->beta, beta2, 16, st);
     if (ok && hipStreamSynchronize(st) != hipSuccess) ok = tp::fail("tp_late_create: upload");
     if (!ok) {
         delete s;
@@ -484,27 +472,26 @@ int tp_late_round(void* handle, int round, const uint64_t* r_prev, uint64_t out6
         return tp::fail("tp_late_round: round %d out of order", round), -1;
     if (round > 0 && !r_prev) return tp::fail("tp_late_round: missing challenge"), -1;
     hipStream_t st = stream();
-    if (round > 0)
-        TP_HIP_OK(hipMemcpyAsync(s->chal.as<u64>() + 2 * (round - 1), r_prev, 16,
-                                 hipMemcpyHostToDevice, st));
+    if (round > 0) s->chal.c[round - 1] = mk2(r_prev[0], r_prev[1]);
     const size_t lds = fold_lds(s->nin);
     if (round == s->nin) {
-        k_late_switch<<<1, kOneWg, lds, st>>>(s->vr.as<u64>(), s->vi.as<u64>(), s->nin,
-                                              s->chal.as<u64>(), s->n_gates, s->othr.as<u64>(),
-                                              s->othi.as<u64>());
+        kern::k_late_switch<<<1, kOneWg, lds, st>>>(s->vr.as<u64>(), s->vi.as<u64>(), s->nin,
+                                                    s->chal, s->n_gates, s->othr.as<u64>(),
+                                                    s->othi.as<u64>());
         TP_HIP_OK(hipGetLastError());
     }
-    const unsigned seq = ++g_map.next;
-    k_late_round<<<1, kOneWg, lds, st>>>(
+    const unsigned seq = g_map.advance();
+    kern::k_late_round<<<1, kOneWg, lds, st>>>(
         s->gt.as<int>(), s->i0.as<int>(), s->i1.as<int>(), s->z1r.as<u64>(), s->z1i.as<u64>(),
-        s->othr.as<u64>(), s->othi.as<u64>(), s->vr.as<u64>(), s->vi.as<u64>(), s->chal.as<u64>(),
-        s->nin, s->n_gates, round, round > 0 ? 1 : 0, s->beta_re, s->beta_im, g_map.dvals,
+        s->othr.as<u64>(), s->othi.as<u64>(), s->vr.as<u64>(), s->vi.as<u64>(), s->chal, s->nin,
+        s->n_gates, round, round > 0 ? 1 : 0, s->beta.as<u64>(), static_cast<fp2*>(g_map.dvals),
         g_map.dseq, seq);
     TP_HIP_OK(hipGetLastError());
-    if (!wait_mapped(st, seq)) return -1;
+    if (!g_map.wait(st, seq, "tp_late_round")) return -1;
+    const fp2* vals = static_cast<const fp2*>(g_map.vals);
     for (int t = 0; t < 3; ++t) {
-        out6[2 * t] = g_map.vals[t].re;
-        out6[2 * t + 1] = g_map.vals[t].im;
+        out6[2 * t] = vals[t].re;
+        out6[2 * t + 1] = vals[t].im;
     }
     s->next_round = round + 1;
     return 0;
@@ -519,8 +506,8 @@ int tp_late_values(void* handle, const uint64_t* pts, int K, uint64_t* out)
     hipStream_t st = stream();
     static DevBuf b_pts, b_out;
     if (!h2d(b_pts, pts, (size_t)K * s->nin * 16, st) || !b_out.ensure((size_t)K * 16)) return -1;
-    k_mle_points<<<K, kOneWg, fold_lds(s->nin), st>>>(s->vr.as<u64>(), s->vi.as<u64>(), s->nin,
-                                                       b_pts.as<u64>(), b_out.as<fp2>());
+    kern::k_mle_points<<<K, kOneWg, fold_lds(s->nin), st>>>(s->vr.as<u64>(), s->vi.as<u64>(),
+                                                             s->nin, b_pts.as<fp2>(), b_out.as<fp2>());
     TP_HIP_OK(hipGetLastError());
     TP_HIP_OK(hipMemcpyAsync(out, b_out.ptr, (size_t)K * 16, hipMemcpyDeviceToHost, st));
     TP_HIP_OK(hipStreamSynchronize(st));
@@ -538,8 +525,8 @@ int tp_wiring_mlext(int n_gates, const int* type_idx, const int* in0, const int*
                     int use_z1_2, uint64_t out26[26])
 {
     tp::clear_error();
-    if (!tp::device_ready() || !set_lds_limits()) return -1;
-    if (n_gates <= 0 || n_gates > (1 << n_out_bits) || n_out_bits > kMaxNin ||
+    if (!tp::device_ready() || !gkr_lds_limits()) return -1;
+    if (n_gates <= 0 || n_gates > (1 << n_out_bits) || n_out_bits > kMaxNin || n_out_bits < 0 ||
         n_in_bits > kMaxNin || n_in_bits < 0)
         return tp::fail("tp_wiring_mlext: bad shape"), -1;
     for (int g = 0; g < n_gates; ++g)
@@ -547,21 +534,28 @@ int tp_wiring_mlext(int n_gates, const int* type_idx, const int* in0, const int*
             type_idx[g] < 0 || type_idx[g] > 12)
             return tp::fail("tp_wiring_mlext: gate %d out of range", g), -1;
     hipStream_t st = stream();
-    static DevBuf b_t, b_i0, b_i1, b_z, b_z2, b_w1, b_w2, b_out;
-    const uint64_t zero2v[2] = {0, 0};
+    static DevBuf b_t, b_i0, b_i1, b_out;
+    WiringArgs a;
+    memset(&a, 0, sizeof(a));
+    for (int j = 0; j < n_out_bits; ++j) {
+        a.z1[j] = mk2(z1[2 * j], z1[2 * j + 1]);
+        if (use_z1_2) a.z1b[j] = mk2(z1_2[2 * j], z1_2[2 * j + 1]);
+    }
+    for (int j = 0; j < n_in_bits; ++j) {
+        a.w1[j] = mk2(w1[2 * j], w1[2 * j + 1]);
+        a.w2[j] = mk2(w2[2 * j], w2[2 * j + 1]);
+    }
+    if (use_z1_2) {
+        a.m0 = mk2(mul0[0], mul0[1]);
+        a.m1 = mk2(mul1[0], mul1[1]);
+    }
     if (!h2d(b_t, type_idx, n_gates * 4, st) || !h2d(b_i0, in0, n_gates * 4, st) ||
-        !h2d(b_i1, in1, n_gates * 4, st) || !h2d(b_z, z1, (size_t)n_out_bits * 16, st) ||
-        !h2d(b_z2, use_z1_2 ? z1_2 : z1, (size_t)n_out_bits * 16, st) ||
-        !h2d(b_w1, w1, (size_t)n_in_bits * 16, st) || !h2d(b_w2, w2, (size_t)n_in_bits * 16, st) ||
-        !b_out.ensure(13 * 16))
+        !h2d(b_i1, in1, n_gates * 4, st) || !b_out.ensure(13 * 16))
         return -1;
-    const uint64_t* m0 = use_z1_2 ? mul0 : zero2v;
-    const uint64_t* m1 = use_z1_2 ? mul1 : zero2v;
     const size_t lds = (((size_t)(use_z1_2 ? 2 : 1) << n_out_bits) + ((size_t)2 << n_in_bits)) * 16;
-    k_wiring_mlext<<<1, kOneWg, lds, st>>>(b_t.as<int>(), b_i0.as<int>(), b_i1.as<int>(), n_gates,
-                                           b_z.as<u64>(), b_z2.as<u64>(), n_out_bits, b_w1.as<u64>(),
-                                           b_w2.as<u64>(), n_in_bits, m0[0], m0[1], m1[0], m1[1],
-                                           use_z1_2, b_out.as<fp2>());
+    kern::k_wiring_mlext<<<1, kOneWg, lds, st>>>(b_t.as<int>(), b_i0.as<int>(), b_i1.as<int>(),
+                                                 n_gates, n_out_bits, n_in_bits, use_z1_2, a,
+                                                 b_out.as<fp2>());
     TP_HIP_OK(hipGetLastError());
     TP_HIP_OK(hipMemcpyAsync(out26, b_out.ptr, 13 * 16, hipMemcpyDeviceToHost, st));
     TP_HIP_OK(hipStreamSynchronize(st));
@@ -577,14 +571,17 @@ int tp_mle_eval_base_dev(const uint64_t* d_inputs, int n_z, const uint64_t* z, u
     if (!tp::device_ready()) return -1;
     if (n_z < 1 || n_z > 40) return tp::fail("tp_mle_eval_base_dev: n_z=%d", n_z), -1;
     hipStream_t st = stream_handle ? (hipStream_t)stream_handle : tp::stream();
-    static DevBuf b_z, b_part;
+    static DevBuf b_part;
+    PointArg pz;
+    memset(&pz, 0, sizeof(pz));
+    for (int j = 0; j < n_z; ++j) pz.z[j] = mk2(z[2 * j], z[2 * j + 1]);
     const int lb = n_z < 10 ? n_z : 10;
     const size_t rows = (size_t)1 << (n_z - lb);
     const unsigned blocks = (unsigned)(rows < 1024 ? rows : 1024);
-    if (!h2d(b_z, z, (size_t)n_z * 16, st) || !b_part.ensure(256 + (size_t)blocks * 16)) return -1;
+    if (!b_part.ensure(256 + (size_t)blocks * 16)) return -1;
     fp2* result = b_part.as<fp2>();
     fp2* partials = reinterpret_cast<fp2*>(b_part.as<char>() + 256);
-    k_mle_base_dev<<<blocks, kBlock, 0, st>>>((const u64*)d_inputs, b_z.as<u64>(), n_z, partials);
+    kern::k_mle_base_dev<<<blocks, kBlock, 0, st>>>((const u64*)d_inputs, pz, n_z, partials);
     TP_HIP_OK(hipGetLastError());
     k_finalize_sum<1><<<1, kBlock, 0, st>>>(partials, blocks, result);
     TP_HIP_OK(hipGetLastError());

@@ -12,6 +12,7 @@
 //  * beta and the input MLE are built from LDS-resident half tables in one
 //    launch instead of n_z doubling launches.
 #include "fp61.hpp"
+#include "kernels.hpp"
 #include "reduce.hpp"
 #include "runtime.hpp"
 
@@ -20,33 +21,8 @@
 
 using namespace tp;
 
-namespace {
-
-// ---------------------------------------------------------------- kernels --
-
-__global__ void k_fold(const u64* __restrict__ ir, const u64* __restrict__ ii, size_t n_out,
-                       u64 rr, u64 ri, u64* __restrict__ orr, u64* __restrict__ oi)
-{
-    const fp2 r = mk2(rr, ri);
-    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n_out;
-         k += (size_t)gridDim.x * blockDim.x) {
-        const fp2 v = fold2(mk2(ir[2 * k], ii[2 * k]), mk2(ir[2 * k + 1], ii[2 * k + 1]), r);
-        orr[k] = v.re;
-        oi[k] = v.im;
-    }
-}
-
-__global__ void k_expand(const u64* __restrict__ ir, const u64* __restrict__ ii, size_t n_src,
-                         size_t nc, u64* __restrict__ orr, u64* __restrict__ oi)
-{
-    const size_t total = n_src * nc;
-    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < total;
-         k += (size_t)gridDim.x * blockDim.x) {
-        const size_t s = k / nc;
-        orr[k] = ir[s];
-        oi[k] = ii[s];
-    }
-}
+namespace tp {
+namespace kern {
 
 // Early-round claim over resident [wire][nc] tables (session form,
 // TP/native_fp2/native_fp2_core.hpp:200-248).  Thread (pair p, gate chunk
@@ -87,6 +63,75 @@ __global__ void k_early_claim(int n_gates, int gpt, const int* __restrict__ gt,
     block_partial<4>(acc, partials);
 }
 
+// Fold every wire row [n_wires][n_in] -> [n_wires][n_in/2] and beta in one
+// launch (TP/native_fp2/native_fp2_core.hpp:250-279).
+__global__ void k_fold_rows(const u64* __restrict__ ir, const u64* __restrict__ ii, int n_wires,
+                            int log_out, u64 rr, u64 ri, u64* __restrict__ orr,
+                            u64* __restrict__ oi, const u64* __restrict__ bir,
+                            const u64* __restrict__ bii, u64* __restrict__ bor,
+                            u64* __restrict__ boi)
+{
+    const fp2 r = mk2(rr, ri);
+    const size_t n_out = (size_t)1 << log_out;
+    const size_t total = ((size_t)n_wires + 1) << log_out;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < total;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t row = k >> log_out, i = k & (n_out - 1);
+        if (row < (size_t)n_wires) {
+            const size_t src = row * 2 * n_out + 2 * i;
+            const fp2 v = fold2(mk2(ir[src], ii[src]), mk2(ir[src + 1], ii[src + 1]), r);
+            orr[k] = v.re;
+            oi[k] = v.im;
+        } else {
+            const fp2 v = fold2(mk2(bir[2 * i], bii[2 * i]), mk2(bir[2 * i + 1], bii[2 * i + 1]), r);
+            bor[i] = v.re;
+            boi[i] = v.im;
+        }
+    }
+}
+
+}  // namespace kern
+
+int gates_per_thread(int n_gates, int n_pairs)
+{
+    const long long target = 1LL << 16;
+    long long chunks = target / (n_pairs > 0 ? n_pairs : 1);
+    if (chunks < 1) chunks = 1;
+    if (chunks > n_gates) chunks = n_gates;
+    long long gpt = (n_gates + chunks - 1) / chunks;
+    return (int)(gpt < 1 ? 1 : gpt);
+}
+
+}  // namespace tp
+
+namespace {
+
+// ---------------------------------------------------------------- kernels --
+
+__global__ void k_fold(const u64* __restrict__ ir, const u64* __restrict__ ii, size_t n_out,
+                       u64 rr, u64 ri, u64* __restrict__ orr, u64* __restrict__ oi)
+{
+    const fp2 r = mk2(rr, ri);
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n_out;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const fp2 v = fold2(mk2(ir[2 * k], ii[2 * k]), mk2(ir[2 * k + 1], ii[2 * k + 1]), r);
+        orr[k] = v.re;
+        oi[k] = v.im;
+    }
+}
+
+__global__ void k_expand(const u64* __restrict__ ir, const u64* __restrict__ ii, size_t n_src,
+                         size_t nc, u64* __restrict__ orr, u64* __restrict__ oi)
+{
+    const size_t total = n_src * nc;
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < total;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t s = k / nc;
+        orr[k] = ir[s];
+        oi[k] = ii[s];
+    }
+}
+
 // Non-session early claim with explicit -1 / 2 tables
 // (TP/native_fp2/native_fp2_core.hpp:82-137); fact tables are [wire][nc/2].
 __global__ void k_sumcheck_early(int n_gates, int gpt, const int* __restrict__ gt,
@@ -125,33 +170,6 @@ __global__ void k_sumcheck_early(int n_gates, int gpt, const int* __restrict__ g
         acc[3] = mul2(v3, mk2(bf1r[h], bf1i[h]));
     }
     block_partial<4>(acc, partials);
-}
-
-// Fold every wire row [n_wires][n_in] -> [n_wires][n_in/2] and beta in one
-// launch (TP/native_fp2/native_fp2_core.hpp:250-279).
-__global__ void k_fold_rows(const u64* __restrict__ ir, const u64* __restrict__ ii, int n_wires,
-                            int log_out, u64 rr, u64 ri, u64* __restrict__ orr,
-                            u64* __restrict__ oi, const u64* __restrict__ bir,
-                            const u64* __restrict__ bii, u64* __restrict__ bor,
-                            u64* __restrict__ boi)
-{
-    const fp2 r = mk2(rr, ri);
-    const size_t n_out = (size_t)1 << log_out;
-    const size_t total = ((size_t)n_wires + 1) << log_out;
-    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < total;
-         k += (size_t)gridDim.x * blockDim.x) {
-        const size_t row = k >> log_out, i = k & (n_out - 1);
-        if (row < (size_t)n_wires) {
-            const size_t src = row * 2 * n_out + 2 * i;
-            const fp2 v = fold2(mk2(ir[src], ii[src]), mk2(ir[src + 1], ii[src + 1]), r);
-            orr[k] = v.re;
-            oi[k] = v.im;
-        } else {
-            const fp2 v = fold2(mk2(bir[2 * i], bii[2 * i]), mk2(bir[2 * i + 1], bii[2 * i + 1]), r);
-            bor[i] = v.re;
-            boi[i] = v.im;
-        }
-    }
 }
 
 // Low-bit half of an eq/beta table in LDS: tab[t] = init * prod_{j<lb}
@@ -268,17 +286,6 @@ bool h2d(DevBuf& b, const void* src, size_t bytes)
     if (!b.ensure(bytes)) return false;
     if (bytes) TP_HIP_OK(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, stream()));
     return true;
-}
-
-// Gate-chunk size so that pairs x chunks gives about 64K threads.
-int gates_per_thread(int n_gates, int n_pairs)
-{
-    const long long target = 1LL << 16;
-    long long chunks = target / (n_pairs > 0 ? n_pairs : 1);
-    if (chunks < 1) chunks = 1;
-    if (chunks > n_gates) chunks = n_gates;
-    long long gpt = (n_gates + chunks - 1) / chunks;
-    return (int)(gpt < 1 ? 1 : gpt);
 }
 
 bool launch_ok() { TP_HIP_OK(hipGetLastError()); return true; }
@@ -439,7 +446,7 @@ bool session_claim(EarlySession* s, u64 out8[8])
     dim3 grid(grid_for((size_t)n_pairs), (unsigned)((s->n_gates + gpt - 1) / gpt));
     RedView rv;
     if (!red_view(s->red, (size_t)grid.x * grid.y, 4, rv)) return false;
-    k_early_claim<<<grid, kBlock, 0, stream()>>>(
+    kern::k_early_claim<<<grid, kBlock, 0, stream()>>>(
         s->n_gates, gpt, s->gt.as<int>(), s->i0.as<int>(), s->i1.as<int>(), s->z1r.as<u64>(),
         s->z1i.as<u64>(), s->n_copies, s->cur_vr(), s->cur_vi(), s->cur_br(), s->cur_bi(),
         rv.partials);
@@ -458,7 +465,7 @@ bool session_fold(EarlySession* s, u64 rr, u64 ri)
     int log_out = 0;
     while ((1 << log_out) < n_out) ++log_out;
     const size_t total = ((size_t)s->n_wires + 1) * n_out;
-    k_fold_rows<<<grid_for(total), kBlock, 0, stream()>>>(
+    kern::k_fold_rows<<<grid_for(total), kBlock, 0, stream()>>>(
         s->cur_vr(), s->cur_vi(), s->n_wires, log_out, rr, ri, s->nxt_vr(), s->nxt_vi(),
         s->cur_br(), s->cur_bi(), s->nxt_br(), s->nxt_bi());
     if (!launch_ok()) return false;

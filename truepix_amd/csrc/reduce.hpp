@@ -99,4 +99,21 @@ __global__ void __launch_bounds__(kBlock) k_finalize_sum(const fp2* partials, un
     }
 }
 
+// One workgroup sums n_blocks partials and publishes the N totals to mapped
+// host memory, then the sequence number (MappedSlot protocol).
+template <int N>
+__global__ void __launch_bounds__(kBlock) k_publish_sum(const fp2* partials, unsigned n_blocks,
+                                                        fp2* host_vals, volatile unsigned* host_seq,
+                                                        unsigned seq)
+{
+    fp2 acc[N];
+    sum_partials<N>(partials, n_blocks, acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) host_vals[k] = acc[k];
+        __threadfence_system();
+        *host_seq = seq;
+    }
+}
+
 }  // namespace tp

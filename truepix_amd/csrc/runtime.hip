@@ -68,6 +68,50 @@ void DevBuf::release()
     bytes = 0;
 }
 
+bool MappedSlot::ready()
+{
+    if (vals) return true;
+    void* hv = nullptr;
+    TP_HIP_OK(hipHostMalloc(&hv, 512, hipHostMallocMapped | hipHostMallocCoherent));
+    void* dv = nullptr;
+    if (hipHostGetDevicePointer(&dv, hv, 0) != hipSuccess) {
+        (void)hipHostFree(hv);
+        return fail("hipHostGetDevicePointer failed");
+    }
+    vals = hv;
+    seq = reinterpret_cast<volatile unsigned*>(static_cast<char*>(hv) + 256);
+    *seq = 0;
+    dvals = dv;
+    dseq = reinterpret_cast<unsigned*>(static_cast<char*>(dv) + 256);
+    next = 0;
+    return true;
+}
+
+bool MappedSlot::wait(hipStream_t st, unsigned s, const char* what)
+{
+    long spins = 0;
+    while (*seq != s) {
+        if (++spins % 2048 == 0) {
+            const hipError_t q = hipStreamQuery(st);
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return fail("%s: stream error %s", what, hipGetErrorString(q));
+            if (q == hipSuccess && *seq != s) return fail("%s: result not published", what);
+        } else {
+            __builtin_ia32_pause();
+        }
+    }
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    return true;
+}
+
+void MappedSlot::release()
+{
+    if (vals) (void)hipHostFree(vals);
+    vals = dvals = nullptr;
+    seq = nullptr;
+    dseq = nullptr;
+}
+
 }  // namespace tp
 
 extern "C" {

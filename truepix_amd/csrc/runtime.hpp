@@ -36,6 +36,24 @@ struct DevBuf {
     template <typename T> T* as() const { return reinterpret_cast<T*>(ptr); }
 };
 
+// A result slot in mapped, coherent host memory: a kernel writes up to 16
+// Fp2 values, fences system scope and then stores the sequence number; the
+// host spins on the sequence number instead of synchronizing the stream.
+struct MappedSlot {
+    void* vals = nullptr;                 // host view of the values
+    volatile unsigned* seq = nullptr;     // host view of the sequence number
+    void* dvals = nullptr;                // device views of the same memory
+    unsigned* dseq = nullptr;
+    unsigned next = 0;
+
+    bool ready();                         // allocates on first use
+    unsigned advance() { return ++next; }
+    // Waits until *seq == s; fails if the stream reports an error or has
+    // drained without publishing.
+    bool wait(hipStream_t st, unsigned s, const char* what);
+    void release();
+};
+
 // Kernel launch shapes: 256-thread workgroups (four wave64s per workgroup).
 static constexpr int kBlock = 256;
 inline unsigned grid_for(size_t n, size_t per_block = kBlock, unsigned cap = 1u << 20)

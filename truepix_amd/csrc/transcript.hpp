@@ -19,8 +19,11 @@
 
 namespace tp {
 
-struct FsTranscript {
-    Sha256 sha;
+// H is the running hash: tp::Sha256 (host and device) or tp::HostSha256
+// (host only, SHA-NI).
+template <class H>
+struct FsTranscriptT {
+    H sha;
     u64 rnum;
 
     TP_HD void raw(const void* p, size_t n) { sha.update((const uint8_t*)p, n); }
@@ -114,6 +117,8 @@ struct FsTranscript {
         return mk2(re, im);
     }
 };
+
+using FsTranscript = FsTranscriptT<Sha256>;
 
 // Writes "<prefix><k>" into out (no terminator) and returns its length.
 TP_HD uint32_t label_num(char* out, const char* prefix, int k)

@@ -67,6 +67,52 @@ def encode_items(vals) -> bytes:
     return bytes(out)
 
 
+def tree_tokens(obj) -> bytes:
+    """Token form of a proof object for the library's proof writer
+    (tp_pickle_tree): ints, strs, None, Fp2, lists and tuples; a list of
+    non-negative u64 ints (two or more) becomes one packed int array."""
+    out = bytearray()
+
+    def enc(x):
+        t = type(x)
+        if t is int:
+            if not 0 <= x <= _MASK64:
+                raise ValueError("proof ints are non-negative u64")
+            out.extend(b"i" + _PACK_Q(x))
+        elif x is None:
+            out.extend(b"n")
+        elif t is str:
+            b = x.encode("utf-8")
+            out.extend(b"s" + _PACK_I(len(b)) + b)
+        elif t is Fp2:
+            out.extend(b"f" + _PACK_Q(x.real) + _PACK_Q(x.imag))
+        elif t is list or t is tuple:
+            if (t is list and len(x) > 1 and all(type(v) is int and 0 <= v <= _MASK64 for v in x)):
+                out.extend(b"a" + _PACK_I(len(x)) + struct.pack("<%dQ" % len(x), *x))
+                return
+            out.extend((b"l" if t is list else b"t") + _PACK_I(len(x)))
+            for v in x:
+                enc(v)
+        else:
+            raise TypeError("unsupported proof element %r" % (t,))
+    enc(obj)
+    return bytes(out)
+
+
+def native_dumps(obj, lib) -> bytes:
+    """pickle.dumps(obj, 5) through the library's proof writer."""
+    import ctypes
+    tok = tree_tokens(obj)
+    ptr = ctypes.POINTER(ctypes.c_uint8)()
+    n = ctypes.c_size_t()
+    if lib.tp_pickle_tree(tok, len(tok), ctypes.byref(ptr), ctypes.byref(n)) != 0:
+        raise ValueError("tp_pickle_tree rejected the tree")
+    try:
+        return ctypes.string_at(ptr, n.value)
+    finally:
+        lib.tp_free(ptr)
+
+
 class _ProofUnpickler(pickle.Unpickler):
     """Admits only the Fp2 class; any other global in a proof is rejected."""
 

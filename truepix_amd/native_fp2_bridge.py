@@ -51,6 +51,17 @@ def _sig(lib, name, argtypes, restype=None):
     fn.restype = restype
 
 
+class FrameJob(ctypes.Structure):
+    """tp_frame_job (include/truepix_hip.h)."""
+    _fields_ = [
+        ("circuit", _VP), ("n_copies", _I32), ("n_private", _I32), ("input_bytes", _I32),
+        ("inputs", _VP), ("pid", _U64), ("n_vole", _I32),
+        ("vole_u_re", _U64P), ("vole_u_im", _U64P), ("vole_v_re", _U64P), ("vole_v_im", _U64P),
+        ("n_binding", _I32), ("binding_kind", _I32P), ("binding_int", _U64P),
+        ("binding_str", ctypes.POINTER(ctypes.c_char_p)), ("keep_outputs", _I32),
+    ]
+
+
 def bind(lib) -> None:
     """Declare every exported symbol's signature (include/truepix_hip.h)."""
     _sig(lib, "native_fp2_init", [], _I32)
@@ -90,6 +101,26 @@ def bind(lib) -> None:
     _sig(lib, "tp_mle_eval_base_dev", [_VP, _I32, _U64P, _U64P, _VP], _I32)
     _sig(lib, "tp_sc2_stats", [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                _I32P])
+    _U8PP = ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8))
+    _sig(lib, "tp_pickle_tree", [ctypes.c_char_p, _SIZE, _U8PP, ctypes.POINTER(_SIZE)], _I32)
+    _sig(lib, "tp_free", [_VP])
+    _sig(lib, "tp_sha256", [ctypes.c_char_p, _SIZE, _SIZE, ctypes.c_char_p])
+    _sig(lib, "tp_sha256_has_ni", [], _I32)
+    _D = ctypes.c_double
+    _sig(lib, "tp_circuit_create", [_I32, _I32P, _I32P, _I32P, _I32P, _I32P, _I32, _I32P, _U64P],
+         _VP)
+    _sig(lib, "tp_circuit_destroy", [_VP])
+    _sig(lib, "tp_pool_create", [_I32], _VP)
+    _sig(lib, "tp_pool_destroy", [_VP])
+    _sig(lib, "tp_pool_now", [_VP], _D)
+    _sig(lib, "tp_pool_submit", [_VP, ctypes.POINTER(FrameJob)], ctypes.c_int64)
+    _sig(lib, "tp_pool_wait", [_VP, ctypes.c_int64, _D], _I32)
+    _sig(lib, "tp_pool_error", [_VP, ctypes.c_int64], ctypes.c_char_p)
+    _sig(lib, "tp_pool_proof", [_VP, ctypes.c_int64, _VP, _SIZE, ctypes.POINTER(_SIZE)], _I32)
+    _sig(lib, "tp_pool_info", [_VP, ctypes.c_int64, _U64P, _I32, _I32P, _U64P,
+                               ctypes.POINTER(_D), ctypes.POINTER(_D)], _I32)
+    _sig(lib, "tp_pool_outputs", [_VP, ctypes.c_int64, _U64P, _SIZE, ctypes.POINTER(_SIZE)], _I32)
+    _sig(lib, "tp_pool_release", [_VP, ctypes.c_int64])
 
 
 def load_library(path: str = LIB_PATH):
