@@ -30,6 +30,19 @@ PEAK_HBM_GBS = 8000.0
 NV = 24
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sc2_fold_round_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per k_sc2_fold_round launch from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (each scaled by the calibration copy), or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            return json.load(fh)["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def alg_bytes(nv: int) -> float:
     """Algorithmic HBM bytes of one nv-round prove (both tables, 16 B per Fp2)."""
     n = 1 << nv
@@ -70,7 +83,9 @@ def bench_rs_encode(steps: int = 5, cols: int = 64, lc: int = 20, lo: int = 22):
 
 def bench_gkr_frame(frames: int = 5, circuit: str = "gray", n_copies: int = 1350):
     """Plain GKR prove of one frame (C1 shape: gray, 1350 macroblocks, 384
-    samples each) with this package's prover; proof bytes as the reference's."""
+    samples each) with the Python-driven device prover (gkr.CircuitProverNIZK:
+    protocol steps in Python, kernels through the ABI); proof bytes as the
+    reference's."""
     import tempfile
     from truepix_amd import circuits, gkr
     from truepix_amd.fp2_link import full_input_rows, make_manifest, random_private_inputs, write_manifest
@@ -103,6 +118,160 @@ def bench_gkr_frame(frames: int = 5, circuit: str = "gray", n_copies: int = 1350
             "frames_per_s": round(1.0 / med, 3), "ms_per_frame_median": round(med * 1e3, 2),
             "ms_per_frame_min": round(times[0] * 1e3, 2), "proof_bytes": len(proof),
             "phases_ms": {k: round(v * 1e3, 2) for k, v in getattr(prv, "timings", {}).items()}}
+
+
+# --- whole frames (C1 plain, C4 batched, C5 served) through the device frame prover
+
+FRAME_SIZES = {"64x64": 16, "240p": 300, "480p": 1350, "720p": 3600, "1080p": 8040}
+CIRCUITS = ("gray", "crop", "mask", "invert")
+
+
+class _FrameFactory:
+    """Synthetic frames: uint8 macroblock samples from PCG64(seed) (crop adds
+    its keep bit), a fresh pid, VOLE correlations and commitment binding per
+    frame (VOLE U, V uniform nonzero from PCG64; the prover only reads U, V)."""
+
+    def __init__(self, seed: int):
+        import numpy as np
+        from truepix_amd import circuits, frames
+        self.np = np
+        self.rng = np.random.Generator(np.random.PCG64(seed))
+        self.circ = {n: frames.FrameCircuit(circuits.circuit(n)) for n in CIRCUITS}
+        self.inputs = {}
+
+    def inputs_for(self, name: str, n_copies: int, variant: int = 0):
+        key = (name, n_copies, variant)
+        if key not in self.inputs:
+            from truepix_amd.fp2_link import crop_bits
+            c = self.circ[name]
+            arr = self.rng.integers(0, 256, size=(n_copies, c.n_private), dtype=self.np.uint8)
+            if name == "crop":
+                arr[:, -1] = crop_bits(n_copies, 0.2, 0.8, 0.2, 0.8).astype(self.np.uint8)
+            self.inputs[key] = arr
+        return self.inputs[key]
+
+    def job(self, name: str, n_copies: int, k: int, variant: int = 0):
+        from truepix_amd.fp2_link import commit_binding, make_manifest
+        P = (1 << 61) - 1
+        vole = tuple(self.rng.integers(1, P, size=500, dtype=self.np.int64).astype(self.np.uint64)
+                     for _ in range(4))
+        binding = commit_binding(make_manifest(n_copies, seed=k))
+        return (self.circ[name], self.inputs_for(name, n_copies, variant), 20000 + k, vole, binding)
+
+
+def bench_frames_plain(frames_n: int = 12, circuit: str = "gray", n_copies: int = 1350):
+    """C1: plain prove of one frame at a time (gray, 1350 macroblocks x 384
+    samples) by the device frame prover, one worker."""
+    from truepix_amd import frames
+    fac = _FrameFactory(105)
+    times, phases = [], []
+    with frames.FramePool(1) as pool:
+        for i in range(frames_n + 2):
+            spec = fac.job(circuit, n_copies, i)
+            t0 = time.perf_counter()
+            res = pool.result(pool.submit(*spec))
+            if i >= 2:                            # two warm-up frames
+                times.append(time.perf_counter() - t0)
+                phases.append(res.phases_ms)
+    times.sort()
+    med = times[len(times) // 2]
+    ph = {k: round(sorted(p[k] for p in phases)[len(phases) // 2], 3) for k in phases[0]}
+    return {"workload": "plain GKR prove, %s circuit, %d copies x 384 samples, one frame at a time "
+                        "(device frame prover, proof bytes as the reference's)" % (circuit, n_copies),
+            "frames_per_s": round(1.0 / med, 2), "ms_per_frame_median": round(med * 1e3, 3),
+            "ms_per_frame_min": round(times[0] * 1e3, 3), "proof_bytes": len(res.proof),
+            "phases_ms_median": ph}
+
+
+def bench_batched(n_frames: int = 256, workers=(1, 4, 8, 16), seed: int = 4):
+    """C4: 256 frames, circuit uniform over gray/crop/mask/invert, size uniform
+    over 64x64 / 240p / 480p / 720p; all queued at once to a pool of W workers.
+    W = 1 is the plain prover one frame after another."""
+    from truepix_amd import frames
+    import numpy as np
+    fac = _FrameFactory(seed)
+    rng = np.random.Generator(np.random.PCG64(seed + 1))
+    sizes = ["64x64", "240p", "480p", "720p"]
+    plan = [(CIRCUITS[rng.integers(0, 4)], FRAME_SIZES[sizes[rng.integers(0, 4)]]) for _ in range(n_frames)]
+    specs = [fac.job(c, n, k, variant=k % 4) for k, (c, n) in enumerate(plan)]
+    out = {"workload": "%d frames, circuits uniform over %s, sizes uniform over %s macroblocks"
+                       % (n_frames, "/".join(CIRCUITS), "/".join(str(FRAME_SIZES[s]) for s in sizes)),
+           "total_copies": int(sum(n for _, n in plan)), "by_workers": {}}
+    ref_proofs = None
+    for w in workers:
+        with frames.FramePool(w) as pool:
+            warm = [pool.submit(*specs[k]) for k in range(min(2 * w, n_frames))]
+            for j in warm:
+                pool.result(j)
+            t0 = time.perf_counter()
+            jobs = [pool.submit(*sp) for sp in specs]
+            proofs = [pool.result(j).proof for j in jobs]
+            dt = time.perf_counter() - t0
+        if ref_proofs is None:
+            ref_proofs = proofs
+        same = all(a == b for a, b in zip(ref_proofs, proofs))
+        out["by_workers"][str(w)] = {"frames_per_s": round(n_frames / dt, 2),
+                                     "seconds": round(dt, 4), "proofs_equal_plain": same}
+    base = out["by_workers"][str(workers[0])]["frames_per_s"]
+    best = max(workers, key=lambda w: out["by_workers"][str(w)]["frames_per_s"])
+    out["best_workers"] = best
+    out["frames_per_s"] = out["by_workers"][str(best)]["frames_per_s"]
+    out["vs_plain"] = round(out["frames_per_s"] / base, 2)
+    return out
+
+
+def bench_served(n_jobs: int = 2000, rate: float = 200.0, workers: int = 8, seed: int = 5):
+    """C5: Poisson arrivals (rate jobs/s, PCG64(seed) exponential gaps) of
+    gray/crop/mask/invert jobs over 240p / 480p / 720p / 1080p frames; each job
+    is submitted at its arrival time to a pool of `workers` provers that admit
+    it as soon as one frees up (continuous batching).  Latency = proof done -
+    arrival.  The plain figure is each job's latency alone on an idle prover."""
+    from truepix_amd import frames
+    import numpy as np
+    fac = _FrameFactory(seed)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = ["240p", "480p", "720p", "1080p"]
+    gaps = rng.exponential(1.0 / rate, size=n_jobs)
+    arrive = np.cumsum(gaps)
+    plan = [(CIRCUITS[rng.integers(0, 4)], FRAME_SIZES[sizes[rng.integers(0, 4)]]) for _ in range(n_jobs)]
+    specs = [fac.job(c, n, k, variant=k % 2) for k, (c, n) in enumerate(plan)]
+    # plain: every (circuit, size) alone on an idle prover
+    plain = {}
+    with frames.FramePool(1) as pool:
+        for c in CIRCUITS:
+            for sz in sizes:
+                n = FRAME_SIZES[sz]
+                sp = fac.job(c, n, 99999)
+                pool.result(pool.submit(*sp))
+                lat = []
+                for _ in range(3):
+                    r = pool.result(pool.submit(*sp))
+                    lat.append(r.t_end - r.t_submit)
+                plain[(c, n)] = sorted(lat)[1]
+    plain_lat = np.array([plain[p] for p in plan])
+    with frames.FramePool(workers) as pool:
+        for k in range(2 * workers):                       # warm every worker's buffers
+            pool.result(pool.submit(*fac.job("gray", FRAME_SIZES["1080p"], 90000 + k)))
+        t0 = time.perf_counter()
+        p0 = pool.now()
+        jobs = []
+        for k, sp in enumerate(specs):
+            while time.perf_counter() - t0 < arrive[k]:
+                pass
+            jobs.append(pool.submit(*sp))
+        res = [pool.result(j) for j in jobs]
+    lat = np.array([r.t_end - r.t_submit for r in res])
+    span = max(r.t_end for r in res) - p0
+    return {"workload": "%d jobs, Poisson %.0f jobs/s, circuits uniform over %s, sizes uniform over "
+                        "%s macroblocks, %d provers" % (n_jobs, rate, "/".join(CIRCUITS),
+                                                        "/".join(str(FRAME_SIZES[s]) for s in sizes),
+                                                        workers),
+            "frames_per_s": round(n_jobs / span, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "plain_p50_ms": round(float(np.percentile(plain_lat, 50)) * 1e3, 3),
+            "plain_p99_ms": round(float(np.percentile(plain_lat, 99)) * 1e3, 3),
+            "offered_jobs_per_s": rate}
 
 
 def cpu_baseline(nv_sample: int = 24, nthreads: int = 1, min_seconds: float = 10.0):
@@ -139,7 +308,14 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only run the two kernels whose HBM counters the traffic figure "
+                         "compares: a 256 MiB elementwise copy (x + 0, read 256 MiB, write "
+                         "256 MiB) and one 24-round prove")
+    ap.add_argument("--skip", default="", help="comma list of side measurements to skip "
+                    "(gkr_frame, batched, served, rs_encode)")
     a = ap.parse_args(argv)
+    a.skip = set(x for x in a.skip.split(",") if x)
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the proofs here are single-GPU work "
               "(frames and tables shard by frame, with no cross-GPU exchange); "
@@ -155,6 +331,14 @@ def main(argv=None):
     df, dg = sc.to_device(f), sc.to_device(g)
     del f, g
     stream = torch.cuda.Stream().cuda_stream            # the kernels' stream (not the null stream)
+    if a.traffic_probe:
+        x = torch.zeros(1 << 25, dtype=torch.int64, device="cuda")
+        y = torch.add(x, 0)                              # calibration: 256 MiB in, 256 MiB out
+        torch.cuda.synchronize()
+        sc.prove_deg2_device(df.data_ptr(), dg.data_ptr(), NV, seed=2024, stream=stream)
+        torch.cuda.synchronize()
+        print(json.dumps({"traffic_probe": True, "copy_bytes": int(y.numel() * 8)}))
+        return 0
     for _ in range(a.warmup):
         sc.prove_deg2_device(df.data_ptr(), dg.data_ptr(), NV, seed=2024, stream=stream)
 
@@ -201,20 +385,26 @@ def main(argv=None):
                    "alg_bytes_per_prove": alg_bytes(NV)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": None,
+                     "traffic": measured_traffic(),
                      "kernel": "k_sc2_fold_round",
                      "avg_launch_us": round(k_ms / k_n * 1e3, 2),
                      "alg_bytes_per_launch": round(k_bytes / k_n, 1)},
         "check": {"round0_poly": [int(x) for x in proof.polys[0][:2]]},
     }
-    try:
-        line["gkr_frame"] = bench_gkr_frame()
-    except Exception as exc:
-        line["gkr_frame"] = {"error": str(exc)}
-    try:
-        line["rs_encode"] = bench_rs_encode()
-    except Exception as exc:
-        line["rs_encode"] = {"error": str(exc)}
+    for key, fn in (("gkr_frame", bench_frames_plain), ("gkr_frame_python_driver", bench_gkr_frame),
+                    ("batched", bench_batched), ("served", bench_served)):
+        if key in a.skip:
+            continue
+        try:
+            line[key] = fn()
+        except Exception as exc:
+            line[key] = {"error": "%s: %s" % (type(exc).__name__, exc)}
+        print("bench: %s done" % key, file=sys.stderr, flush=True)
+    if "rs_encode" not in a.skip:
+        try:
+            line["rs_encode"] = bench_rs_encode()
+        except Exception as exc:
+            line["rs_encode"] = {"error": str(exc)}
     if not a.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline()

@@ -372,6 +372,17 @@ struct Proto {
     }
 };
 
+// Joins its thread when it goes out of scope (error returns included).
+struct Joiner {
+    std::thread t;
+    explicit Joiner(std::thread&& th) : t(std::move(th)) {}
+    void join()
+    {
+        if (t.joinable()) t.join();
+    }
+    ~Joiner() { join(); }
+};
+
 double now_s(Clock::time_point t0)
 {
     return std::chrono::duration<double>(Clock::now() - t0).count();
@@ -435,11 +446,34 @@ bool prove_frame(Worker& W, Job& J)
     TP_HIP_OK(hipStreamSynchronize(st));
     lap(0);
 
-    // 2. transcript head (circuitnizkvec_vole.py:265-300)
+    // 2. transcript head (circuitnizkvec_vole.py:265-300).  The proof's
+    // leading items (q, io) are known now, so their bytes are written on a
+    // helper thread while this one hashes the outputs and runs the rounds.
+    const u64 n_inv = (u64)1 << n_z;
+    PNode io = PNode::list();
+    {
+        PNode l = PNode::list();
+        l.kids.push_back(PNode::none());
+        io.kids.push_back(std::move(l));
+        PNode m = PNode::list();
+        m.kids.push_back(PNode::str("private_input_omitted"));
+        m.kids.push_back(PNode::integer(n_inv));
+        io.kids.push_back(std::move(m));
+        io.kids.push_back(PNode::int_array(W.h_out, out_n));
+    }
+    J.proof.clear();
+    J.proof.reserve(out_n * 2 + (out_n / 1000 + 64) * 2 + (size_t)(out_n / 32768 + 8) * 9 + 65536);
+    PickleWriter pw(J.proof);
+    Joiner prefix(std::thread([&pw, &io] {
+        pw.start();
+        pw.open_tuple();
+        pw.save(PNode::integer(P));
+        pw.save(io);
+    }));
+
     Proto pr;
     pr.job = &J;
     pr.fs.init("truepix");
-    PNode io = PNode::list();
     pr.fs.int_list(&J.pid, 1);                                   // put(pid)
     {
         PNode l = PNode::list();
@@ -447,26 +481,13 @@ bool prove_frame(Worker& W, Job& J)
         pr.trans.push_back(std::move(l));
     }
     pr.fs.item_none();                                           // put(muxbits=None, io)
-    {
-        PNode l = PNode::list();
-        l.kids.push_back(PNode::none());
-        io.kids.push_back(std::move(l));
-    }
-    const u64 n_inv = (u64)1 << n_z;                             // put([..omitted, n], io)
-    pr.fs.item_str("private_input_omitted");
+    pr.fs.item_str("private_input_omitted");                     // put([..omitted, n], io)
     pr.fs.item_int(n_inv);
-    {
-        PNode l = PNode::list();
-        l.kids.push_back(PNode::str("private_input_omitted"));
-        l.kids.push_back(PNode::integer(n_inv));
-        io.kids.push_back(std::move(l));
-    }
     {                                                            // put(outvals, io)
         const uint8_t tag = 0x01;
         pr.fs.raw(&tag, 1);
         pr.fs.le32((uint32_t)out_n);
         pr.fs.raw(W.h_out, out_n * 8);
-        io.kids.push_back(PNode::int_array(W.h_out, out_n));
     }
     {                                                            // put(commit_binding)
         PNode l = PNode::list();
@@ -761,16 +782,16 @@ bool prove_frame(Worker& W, Job& J)
     }
     lap(6);
 
-    // 7. proof bytes: pickle((q, io, trans, ndb, rvstart, rvend, wDiv, 4), 5)
-    PNode root = PNode::tuple();
-    root.kids.push_back(PNode::integer(P));
-    root.kids.push_back(std::move(io));
+    // 7. proof bytes: pickle((q, io, trans, ndb, rvstart, rvend, wDiv, 4), 5);
+    // (q, io) were written by the helper thread
+    prefix.join();
     PNode tr = PNode::list();
     tr.kids = std::move(pr.trans);
-    root.kids.push_back(std::move(tr));
-    for (int k = 0; k < 4; ++k) root.kids.push_back(PNode::none());
-    root.kids.push_back(PNode::integer(4));
-    pickle_dumps(root, J.proof);
+    pw.save(tr);
+    for (int k = 0; k < 4; ++k) pw.save(PNode::none());
+    pw.save(PNode::integer(4));
+    pw.close_tuple();
+    pw.finish();
     lap(7);
     return true;
 }

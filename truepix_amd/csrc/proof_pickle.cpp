@@ -20,251 +20,276 @@ constexpr uint8_t OP_STACK_GLOBAL = 0x93, OP_NEWOBJ = 0x81, OP_BUILD = 'b';
 constexpr size_t kFrameTarget = 64 * 1024, kFrameMin = 4, kFrameHeader = 9, kBatch = 1000;
 constexpr uint64_t kP = (1ULL << 61) - 1;
 
-class Writer {
-public:
-    explicit Writer(std::vector<uint8_t>& o) : out(o) { out.clear(); }
+// Encoded size of one int (opcode included) and the encoder itself.
+size_t int_size(uint64_t v)
+{
+    if (v <= 0xff) return 2;
+    if (v <= 0xffff) return 3;
+    if (v <= 0x7fffffffULL) return 5;
+    const int bits = 64 - __builtin_clzll(v);
+    return 2 + (size_t)(bits / 8 + 1);
+}
 
-    void start()
-    {
-        put1(OP_PROTO);
-        put1(5);
-        framing = true;
+uint8_t* enc_int(uint8_t* d, uint64_t v)
+{
+    if (v <= 0xff) {
+        d[0] = OP_BININT1;
+        d[1] = (uint8_t)v;
+        return d + 2;
     }
-    void finish()
-    {
-        put1(OP_STOP);
-        commit();
+    if (v <= 0xffff) {
+        d[0] = OP_BININT2;
+        d[1] = (uint8_t)v;
+        d[2] = (uint8_t)(v >> 8);
+        return d + 3;
     }
-
-    void save(const PNode& x)
-    {
-        boundary();
-        switch (x.kind) {
-        case PNode::INT: save_int(x.a); return;
-        case PNode::NONE: put1(OP_NONE); return;
-        case PNode::STR: save_str(x.s); return;
-        case PNode::FP2: save_fp2(x.a, x.b); return;
-        case PNode::LIST: save_list(x); return;
-        case PNode::TUPLE: save_tuple(x); return;
-        case PNode::INT_ARRAY: save_int_array(x.arr, x.n); return;
-        }
+    if (v <= 0x7fffffffULL) {
+        d[0] = OP_BININT;
+        for (int i = 0; i < 4; ++i) d[1 + i] = (uint8_t)(v >> (8 * i));
+        return d + 5;
     }
-
-private:
-    std::vector<uint8_t>& out;
-    bool framing = false;
-    long frame_start = -1;
-    uint32_t memo = 0;
-    long fp2_class = -1;
-
-    uint8_t* grab(size_t n)
-    {
-        if (framing && frame_start < 0) {
-            frame_start = (long)out.size();
-            out.resize(out.size() + kFrameHeader, 0xfe);
-        }
-        const size_t at = out.size();
-        out.resize(at + n);
-        return out.data() + at;
-    }
-    void put1(uint8_t c) { *grab(1) = c; }
-    void put(const void* p, size_t n) { memcpy(grab(n), p, n); }
-
-    void commit()
-    {
-        if (frame_start < 0) return;
-        const size_t len = out.size() - (size_t)frame_start - kFrameHeader;
-        if (len >= kFrameMin) {
-            uint8_t* h = out.data() + frame_start;
-            h[0] = OP_FRAME;
-            for (int i = 0; i < 8; ++i) h[1 + i] = (uint8_t)((uint64_t)len >> (8 * i));
-        } else {
-            out.erase(out.begin() + frame_start, out.begin() + frame_start + kFrameHeader);
-        }
-        frame_start = -1;
-    }
-    void boundary()
-    {
-        if (frame_start >= 0 && out.size() - (size_t)frame_start - kFrameHeader >= kFrameTarget)
-            commit();
-    }
-    void memoize()
-    {
-        put1(OP_MEMOIZE);
-        ++memo;
-    }
-    void get(uint32_t idx)
-    {
-        if (idx < 256) {
-            uint8_t b[2] = {OP_BINGET, (uint8_t)idx};
-            put(b, 2);
-        } else {
-            uint8_t b[5] = {OP_LONG_BINGET, (uint8_t)idx, (uint8_t)(idx >> 8), (uint8_t)(idx >> 16),
-                            (uint8_t)(idx >> 24)};
-            put(b, 5);
-        }
-    }
-
-    // Encoded size of one int (opcode included) and the encoder itself.
-    static size_t int_size(uint64_t v)
-    {
-        if (v <= 0xff) return 2;
-        if (v <= 0xffff) return 3;
-        if (v <= 0x7fffffffULL) return 5;
-        int bits = 64 - __builtin_clzll(v);
-        return 2 + (size_t)(bits / 8 + 1);
-    }
-    static uint8_t* enc_int(uint8_t* d, uint64_t v)
-    {
-        if (v <= 0xff) {
-            d[0] = OP_BININT1;
-            d[1] = (uint8_t)v;
-            return d + 2;
-        }
-        if (v <= 0xffff) {
-            d[0] = OP_BININT2;
-            d[1] = (uint8_t)v;
-            d[2] = (uint8_t)(v >> 8);
-            return d + 3;
-        }
-        if (v <= 0x7fffffffULL) {
-            d[0] = OP_BININT;
-            for (int i = 0; i < 4; ++i) d[1 + i] = (uint8_t)(v >> (8 * i));
-            return d + 5;
-        }
-        // LONG1: minimal two's-complement little-endian, bit_length / 8 + 1 bytes
-        const int bits = 64 - __builtin_clzll(v);
-        const int nb = bits / 8 + 1;
-        d[0] = OP_LONG1;
-        d[1] = (uint8_t)nb;
-        for (int i = 0; i < nb; ++i) d[2 + i] = i < 8 ? (uint8_t)(v >> (8 * i)) : 0;
-        return d + 2 + nb;
-    }
-    void save_int(uint64_t v)
-    {
-        uint8_t tmp[12];
-        put(tmp, (size_t)(enc_int(tmp, v) - tmp));
-    }
-    void save_str(const std::string& s)
-    {
-        if (s.size() <= 0xff) {
-            put1(OP_SHORT_BINUNICODE);
-            put1((uint8_t)s.size());
-        } else {
-            put1(OP_BINUNICODE);
-            uint8_t l[4];
-            for (int i = 0; i < 4; ++i) l[i] = (uint8_t)(s.size() >> (8 * i));
-            put(l, 4);
-        }
-        put(s.data(), s.size());
-        memoize();
-    }
-    void save_fp2(uint64_t re, uint64_t im)
-    {
-        boundary();                       // save(cls)
-        if (fp2_class < 0) {
-            boundary();
-            save_str("libTruePix.extfield");
-            boundary();
-            save_str("Fp2");
-            put1(OP_STACK_GLOBAL);
-            fp2_class = memo;
-            memoize();
-        } else {
-            get((uint32_t)fp2_class);
-        }
-        boundary();                       // save(()) -> EMPTY_TUPLE, not memoized
-        put1(OP_EMPTY_TUPLE);
-        put1(OP_NEWOBJ);
-        memoize();
-        boundary();                       // save(state tuple)
-        boundary();
-        save_int(re);
-        boundary();
-        save_int(im);
-        boundary();
-        save_int(kP);
-        put1(OP_TUPLE3);
-        memoize();
-        put1(OP_BUILD);
-    }
-    void save_list(const PNode& x)
-    {
-        put1(OP_EMPTY_LIST);
-        memoize();
-        const size_t n = x.kids.size();
-        if (n == 0) return;
-        if (n == 1) {
-            save(x.kids[0]);
-            put1(OP_APPEND);
-            return;
-        }
-        size_t i = 0;
-        do {
-            put1(OP_MARK);
-            for (size_t b = 0; b < kBatch && i < n; ++b, ++i) save(x.kids[i]);
-            put1(OP_APPENDS);
-        } while (i < n);
-    }
-    // A list of ints from an array, batched and framed exactly like save_list.
-    void save_int_array(const uint64_t* v, size_t n)
-    {
-        put1(OP_EMPTY_LIST);
-        memoize();
-        if (n == 0) return;
-        if (n == 1) {
-            boundary();
-            save_int(v[0]);
-            put1(OP_APPEND);
-            return;
-        }
-        size_t i = 0;
-        do {
-            put1(OP_MARK);
-            const size_t end = i + kBatch < n ? i + kBatch : n;
-            size_t worst = 0;
-            for (size_t k = i; k < end; ++k) worst += int_size(v[k]);
-            // No frame boundary falls inside this batch if the open frame
-            // cannot reach the target before its last item starts.
-            const size_t open = frame_start >= 0 ? out.size() - (size_t)frame_start - kFrameHeader : 0;
-            if (frame_start >= 0 && open + worst < kFrameTarget) {
-                uint8_t* d = grab(worst);
-                for (size_t k = i; k < end; ++k) d = enc_int(d, v[k]);
-                i = end;
-            } else {
-                for (; i < end; ++i) {
-                    boundary();
-                    save_int(v[i]);
-                }
-            }
-            put1(OP_APPENDS);
-        } while (i < n);
-    }
-    void save_tuple(const PNode& x)
-    {
-        const size_t n = x.kids.size();
-        if (n == 0) {
-            put1(OP_EMPTY_TUPLE);
-            return;
-        }
-        if (n <= 3) {
-            for (const PNode& k : x.kids) save(k);
-            put1(n == 1 ? OP_TUPLE1 : n == 2 ? OP_TUPLE2 : OP_TUPLE3);
-            memoize();
-            return;
-        }
-        put1(OP_MARK);
-        for (const PNode& k : x.kids) save(k);
-        put1(OP_TUPLE);
-        memoize();
-    }
-};
+    // LONG1: minimal two's-complement little-endian, bit_length / 8 + 1 bytes
+    const int bits = 64 - __builtin_clzll(v);
+    const int nb = bits / 8 + 1;
+    d[0] = OP_LONG1;
+    d[1] = (uint8_t)nb;
+    for (int i = 0; i < nb; ++i) d[2 + i] = i < 8 ? (uint8_t)(v >> (8 * i)) : 0;
+    return d + 2 + nb;
+}
 
 }  // namespace
 
+PickleWriter::PickleWriter(std::vector<uint8_t>& o) : out(o) { out.clear(); }
+
+void PickleWriter::start()
+{
+    put1(OP_PROTO);
+    put1(5);
+    framing = true;
+}
+
+void PickleWriter::finish()
+{
+    put1(OP_STOP);
+    commit();
+}
+
+void PickleWriter::open_tuple()
+{
+    boundary();
+    put1(OP_MARK);
+}
+
+void PickleWriter::close_tuple()
+{
+    put1(OP_TUPLE);
+    memoize();
+}
+
+void PickleWriter::save(const PNode& x)
+{
+    boundary();
+    switch (x.kind) {
+    case PNode::INT: save_int(x.a); return;
+    case PNode::NONE: put1(OP_NONE); return;
+    case PNode::STR: save_str(x.s); return;
+    case PNode::FP2: save_fp2(x.a, x.b); return;
+    case PNode::LIST: save_list(x); return;
+    case PNode::TUPLE: save_tuple(x); return;
+    case PNode::INT_ARRAY: save_int_array(x.arr, x.n); return;
+    }
+}
+
+uint8_t* PickleWriter::grab(size_t n)
+{
+    if (framing && frame_start < 0) {
+        frame_start = (long)out.size();
+        out.resize(out.size() + kFrameHeader, 0xfe);
+    }
+    const size_t at = out.size();
+    out.resize(at + n);
+    return out.data() + at;
+}
+
+void PickleWriter::put(const void* p, size_t n) { memcpy(grab(n), p, n); }
+
+void PickleWriter::commit()
+{
+    if (frame_start < 0) return;
+    const size_t len = out.size() - (size_t)frame_start - kFrameHeader;
+    if (len >= kFrameMin) {
+        uint8_t* h = out.data() + frame_start;
+        h[0] = OP_FRAME;
+        for (int i = 0; i < 8; ++i) h[1 + i] = (uint8_t)((uint64_t)len >> (8 * i));
+    } else {
+        out.erase(out.begin() + frame_start, out.begin() + frame_start + kFrameHeader);
+    }
+    frame_start = -1;
+}
+
+void PickleWriter::boundary()
+{
+    if (frame_start >= 0 && out.size() - (size_t)frame_start - kFrameHeader >= kFrameTarget)
+        commit();
+}
+
+void PickleWriter::memoize()
+{
+    put1(OP_MEMOIZE);
+    ++memo;
+}
+
+void PickleWriter::get(uint32_t idx)
+{
+    if (idx < 256) {
+        uint8_t b[2] = {OP_BINGET, (uint8_t)idx};
+        put(b, 2);
+    } else {
+        uint8_t b[5] = {OP_LONG_BINGET, (uint8_t)idx, (uint8_t)(idx >> 8), (uint8_t)(idx >> 16),
+                        (uint8_t)(idx >> 24)};
+        put(b, 5);
+    }
+}
+
+void PickleWriter::save_int(uint64_t v)
+{
+    uint8_t tmp[12];
+    put(tmp, (size_t)(enc_int(tmp, v) - tmp));
+}
+
+void PickleWriter::save_str(const std::string& s)
+{
+    if (s.size() <= 0xff) {
+        put1(OP_SHORT_BINUNICODE);
+        put1((uint8_t)s.size());
+    } else {
+        put1(OP_BINUNICODE);
+        uint8_t l[4];
+        for (int i = 0; i < 4; ++i) l[i] = (uint8_t)(s.size() >> (8 * i));
+        put(l, 4);
+    }
+    put(s.data(), s.size());
+    memoize();
+}
+
+void PickleWriter::save_fp2(uint64_t re, uint64_t im)
+{
+    boundary();                       // save(cls)
+    if (fp2_class < 0) {
+        boundary();
+        save_str("libTruePix.extfield");
+        boundary();
+        save_str("Fp2");
+        put1(OP_STACK_GLOBAL);
+        fp2_class = memo;
+        memoize();
+    } else {
+        get((uint32_t)fp2_class);
+    }
+    boundary();                       // save(()) -> EMPTY_TUPLE, not memoized
+    put1(OP_EMPTY_TUPLE);
+    put1(OP_NEWOBJ);
+    memoize();
+    boundary();                       // save(state tuple)
+    boundary();
+    save_int(re);
+    boundary();
+    save_int(im);
+    boundary();
+    save_int(kP);
+    put1(OP_TUPLE3);
+    memoize();
+    put1(OP_BUILD);
+}
+
+void PickleWriter::save_list(const PNode& x)
+{
+    put1(OP_EMPTY_LIST);
+    memoize();
+    const size_t n = x.kids.size();
+    if (n == 0) return;
+    if (n == 1) {
+        save(x.kids[0]);
+        put1(OP_APPEND);
+        return;
+    }
+    size_t i = 0;
+    do {
+        put1(OP_MARK);
+        for (size_t b = 0; b < kBatch && i < n; ++b, ++i) save(x.kids[i]);
+        put1(OP_APPENDS);
+    } while (i < n);
+}
+
+// A list of ints from an array, batched and framed exactly like save_list.
+void PickleWriter::save_int_array(const uint64_t* v, size_t n)
+{
+    put1(OP_EMPTY_LIST);
+    memoize();
+    if (n == 0) return;
+    if (n == 1) {
+        boundary();
+        save_int(v[0]);
+        put1(OP_APPEND);
+        return;
+    }
+    size_t i = 0;
+    do {
+        put1(OP_MARK);
+        const size_t end = i + kBatch < n ? i + kBatch : n;
+        uint64_t big = 0;
+        for (size_t k = i; k < end; ++k) big |= v[k];
+        const size_t worst = big <= 0xff ? 2 * (end - i) : [&] {
+            size_t w = 0;
+            for (size_t k = i; k < end; ++k) w += int_size(v[k]);
+            return w;
+        }();
+        // No frame boundary falls inside this batch if the open frame cannot
+        // reach the target before its last item starts.
+        const size_t open = frame_start >= 0 ? out.size() - (size_t)frame_start - kFrameHeader : 0;
+        if (frame_start >= 0 && open + worst < kFrameTarget) {
+            uint8_t* d = grab(worst);
+            if (big <= 0xff) {
+                for (size_t k = i; k < end; ++k, d += 2) {
+                    d[0] = OP_BININT1;
+                    d[1] = (uint8_t)v[k];
+                }
+            } else {
+                for (size_t k = i; k < end; ++k) d = enc_int(d, v[k]);
+            }
+            i = end;
+        } else {
+            for (; i < end; ++i) {
+                boundary();
+                save_int(v[i]);
+            }
+        }
+        put1(OP_APPENDS);
+    } while (i < n);
+}
+
+void PickleWriter::save_tuple(const PNode& x)
+{
+    const size_t n = x.kids.size();
+    if (n == 0) {
+        put1(OP_EMPTY_TUPLE);
+        return;
+    }
+    if (n <= 3) {
+        for (const PNode& k : x.kids) save(k);
+        put1(n == 1 ? OP_TUPLE1 : n == 2 ? OP_TUPLE2 : OP_TUPLE3);
+        memoize();
+        return;
+    }
+    put1(OP_MARK);
+    for (const PNode& k : x.kids) save(k);
+    put1(OP_TUPLE);
+    memoize();
+}
+
 void pickle_dumps(const PNode& root, std::vector<uint8_t>& out)
 {
-    Writer w(out);
+    PickleWriter w(out);
     w.start();
     w.save(root);
     w.finish();

@@ -52,4 +52,39 @@ struct PNode {
 // p = 2^61 - 1.
 void pickle_dumps(const PNode& root, std::vector<uint8_t>& out);
 
+// The same writer, driven step by step, so a prefix of the proof (the
+// top-level tuple's leading items) can be written before the rest is known:
+//   start(); open_tuple(); save(a); save(b); ... close_tuple(); finish();
+// produces pickle.dumps((a, b, ...), 5) for a tuple of more than 3 items.
+class PickleWriter {
+public:
+    explicit PickleWriter(std::vector<uint8_t>& out);
+    void start();
+    void open_tuple();
+    void close_tuple();
+    void save(const PNode& x);
+    void finish();
+
+private:
+    std::vector<uint8_t>& out;
+    bool framing = false;
+    long frame_start = -1;
+    uint32_t memo = 0;
+    long fp2_class = -1;
+
+    uint8_t* grab(size_t n);
+    void put1(uint8_t c) { *grab(1) = c; }
+    void put(const void* p, size_t n);
+    void commit();
+    void boundary();
+    void memoize();
+    void get(uint32_t idx);
+    void save_int(uint64_t v);
+    void save_str(const std::string& s);
+    void save_fp2(uint64_t re, uint64_t im);
+    void save_list(const PNode& x);
+    void save_int_array(const uint64_t* v, size_t n);
+    void save_tuple(const PNode& x);
+};
+
 }  // namespace tp
