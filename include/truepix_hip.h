@@ -242,6 +242,25 @@ int tp_pool_outputs(void* pool, int64_t job, uint64_t* out, size_t cap, size_t* 
 /* Forgets a finished job. */
 void tp_pool_release(void* pool, int64_t job);
 
+/* ---- fft_gkr: OZ/src/VPD/fft_circuit_GKR.cpp (main :827-845), the program
+ *      vpd_verifier.cpp:80 runs as "./fft_gkr <lg> <logfile>" ------------------ */
+
+/* A16 + A18: builds the circuit for lg_size and runs every layer protocol on the
+ * device with the reference's challenges (glibc rand(), seed 1), then the
+ * verifier's checks on the host.  Returns 1 when every check passes (the
+ * reference prints "Error, fft gkr failed" otherwise), 0 on a failed check,
+ * -1 on error.  v_time / p_time in seconds, proof_size in bytes as the
+ * reference counts them (what it writes to its log file). */
+int tp_fft_gkr(int lg_size, double* v_time, int* proof_size, double* p_time);
+/* The same run, exposing what it computes: polys (cap_polys x (a, b, c) Fp2,
+ * interleaved limbs) and *n_polys = (lg+6) + lg + 2 lg^2 round polynomials in
+ * protocol order; claims (7 Fp2) = alpha_beta_sum after V_output, addition,
+ * mult, intermediate and all IFFT layers, then the final alpha and beta;
+ * layers (optional): last IFFT layer (2^lg Fp2), the 1/n-scaled layer (2^lg)
+ * and the 64 sums.  Return as tp_fft_gkr. */
+int tp_fft_gkr_trace(int lg_size, uint64_t* polys, int cap_polys, int* n_polys, uint64_t* claims,
+                     uint64_t* layers, int* proof_size);
+
 /* ---- host pieces of the frame prover, exported for their CPU tests ------- */
 
 /* A15, TP/libTruePix/fiatshamir.py:234-237: pickle.dumps(tree, 5) of a

@@ -71,6 +71,10 @@ class Oracle:
         self.fft_raw = _bind(lib, "or_fft", [_U64P, _INT, _INT, _U64P])
         self.poly_eval_raw = _bind(lib, "or_poly_eval", [_U64P, _SZ, _U64, _U64, _U64P])
         self.pow_raw = _bind(lib, "or_fp2_pow", [_U64, _U64, _U64, _U64P])
+        self.glibc_rand_raw = _bind(lib, "or_glibc_rand",
+                                    [ctypes.c_uint32, _INT, ctypes.POINTER(ctypes.c_int32)])
+        self.fft_gkr_raw = _bind(lib, "or_fft_gkr", [_INT, _U64P, _INT, _I32P, _U64P, _U64P, _I32P],
+                                 _INT)
 
     # -- numpy conveniences -------------------------------------------------
     def mul(self, a, b):
@@ -120,6 +124,60 @@ class Oracle:
             self.sc2_fold(_p(f), _p(g), n, cr, ci, nthreads)
             n //= 2
         return polys, chals, np.array([f[0], f[1], g[0], g[1]], dtype=np.uint64)
+
+
+def fft_gkr_result(fn, lg: int, with_polys: bool = True):
+    """Runs an or_fft_gkr-shaped function; returns ok, polys (n x 6 limbs),
+    claims (7 x 2), layers (2^lg last IFFT, 2^lg scaled, 64 sums; x 2 limbs)
+    and proof_size."""
+    n = 1 << lg
+    n_rounds = (lg + 6) + lg + 2 * lg * lg
+    polys = np.zeros(6 * n_rounds, np.uint64) if with_polys else None
+    claims = np.zeros(14, np.uint64)
+    layers = np.zeros(4 * n + 128, np.uint64)
+    npol = ctypes.c_int(0)
+    psize = ctypes.c_int(0)
+    ok = fn(lg, _p(polys) if with_polys else None, n_rounds if with_polys else 0,
+            ctypes.byref(npol), _p(claims), _p(layers), ctypes.byref(psize))
+    return {"ok": ok, "polys": polys, "n_polys": npol.value, "claims": claims, "layers": layers,
+            "proof_size": psize.value}
+
+
+def _oracle_fft_gkr(self, lg: int, with_polys: bool = True):
+    return fft_gkr_result(self.fft_gkr_raw, lg, with_polys)
+
+
+def _oracle_glibc_rand(self, seed: int, n: int):
+    out = (ctypes.c_int32 * n)()
+    self.glibc_rand_raw(seed, n, out)
+    return list(out)
+
+
+Oracle.fft_gkr = _oracle_fft_gkr
+Oracle.glibc_rand = _oracle_glibc_rand
+
+
+class RefFftGkr:
+    """fft_circuit_GKR.cpp compiled in place, driven through its own layer
+    functions (oracle/ref_harness_fftgkr.cpp)."""
+
+    PATH = os.path.join(HERE, "_ref", "libfftgkr_ref.so")
+
+    @classmethod
+    def available(cls) -> bool:
+        return os.path.isfile(cls.PATH)
+
+    def __init__(self):
+        lib = ctypes.CDLL(self.PATH)
+        self.raw = _bind(lib, "ref_fft_gkr", [_INT, _U64P, _U64P, _I32P], _INT)
+
+    def run(self, lg: int):
+        n = 1 << lg
+        claims = np.zeros(14, np.uint64)
+        layers = np.zeros(4 * n + 128, np.uint64)
+        psize = ctypes.c_int(0)
+        ok = self.raw(lg, _p(claims), _p(layers), ctypes.byref(psize))
+        return {"ok": ok, "claims": claims, "layers": layers, "proof_size": psize.value}
 
 
 class RefNative:

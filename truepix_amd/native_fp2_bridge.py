@@ -121,6 +121,8 @@ def bind(lib) -> None:
                                ctypes.POINTER(_D), ctypes.POINTER(_D)], _I32)
     _sig(lib, "tp_pool_outputs", [_VP, ctypes.c_int64, _U64P, _SIZE, ctypes.POINTER(_SIZE)], _I32)
     _sig(lib, "tp_pool_release", [_VP, ctypes.c_int64])
+    _sig(lib, "tp_fft_gkr", [_I32, ctypes.POINTER(_D), _I32P, ctypes.POINTER(_D)], _I32)
+    _sig(lib, "tp_fft_gkr_trace", [_I32, _U64P, _I32, _I32P, _U64P, _U64P, _I32P], _I32)
 
 
 def load_library(path: str = LIB_PATH):
