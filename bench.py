@@ -274,6 +274,44 @@ def bench_served(n_jobs: int = 2000, rate: float = 200.0, workers: int = 8, seed
             "offered_jobs_per_s": rate}
 
 
+def bench_fft_gkr(sizes=(7, 17, 20), reps: int = 3, cpu_lg: int = 17, cpu: bool = True):
+    """fft_gkr (OZ/src/VPD/fft_circuit_GKR.cpp): circuit build + every layer
+    protocol + the verifier's checks, per lg_size (2^lg coefficients; the
+    TruePix pipeline runs lg 7).  Wall time per run and the device span
+    (HIP events from the first to the last kernel).  CPU: the oracle's
+    restatement, single-threaded, at cpu_lg."""
+    from truepix_amd import fft_gkr
+    out = {"workload": "fft_gkr: IFFT + 64-point evaluation circuit over 2^lg Fp2 coefficients, "
+                       "addition / mult / intermediate / lg IFFT layer sumchecks, reference "
+                       "rand() challenges, verifier checks", "by_lg": {}}
+    for lg in sizes:
+        fft_gkr.run(lg)
+        walls, devs = [], []
+        ok = True
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            okk, v_time, psize, p_time = fft_gkr.run(lg)
+            walls.append(time.perf_counter() - t0)
+            devs.append(p_time)
+            ok = ok and okk
+        walls.sort()
+        devs.sort()
+        out["by_lg"][str(lg)] = {"ms_wall": round(walls[len(walls) // 2] * 1e3, 3),
+                                 "ms_device": round(devs[len(devs) // 2] * 1e3, 3),
+                                 "verified": ok, "proof_size": psize}
+    if cpu:
+        from oracle.oracle import Oracle
+        t0 = time.perf_counter()
+        r = Oracle().fft_gkr(cpu_lg, with_polys=False)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"lg": cpu_lg, "ms": round(dt * 1e3, 1), "verified": bool(r["ok"]),
+                               "cores": 1, "kind": "port",
+                               "sample": "one full run at lg %d, oracle/fp2_oracle.c or_fft_gkr" % cpu_lg}
+        if str(cpu_lg) in out["by_lg"]:
+            out["speedup_vs_cpu_at_lg%d" % cpu_lg] = round(dt * 1e3 / out["by_lg"][str(cpu_lg)]["ms_wall"], 1)
+    return out
+
+
 def cpu_baseline(nv_sample: int = 24, nthreads: int = 1, min_seconds: float = 10.0):
     from oracle.oracle import Oracle
     from truepix_amd.sumcheck import synthetic_tables
@@ -313,7 +351,7 @@ def main(argv=None):
                          "compares: a 256 MiB elementwise copy (x + 0, read 256 MiB, write "
                          "256 MiB) and one 24-round prove")
     ap.add_argument("--skip", default="", help="comma list of side measurements to skip "
-                    "(gkr_frame, batched, served, rs_encode)")
+                    "(gkr_frame, gkr_frame_python_driver, batched, served, fft_gkr, rs_encode)")
     a = ap.parse_args(argv)
     a.skip = set(x for x in a.skip.split(",") if x)
     if a.gpus != 1:
@@ -400,6 +438,12 @@ def main(argv=None):
         except Exception as exc:
             line[key] = {"error": "%s: %s" % (type(exc).__name__, exc)}
         print("bench: %s done" % key, file=sys.stderr, flush=True)
+    if "fft_gkr" not in a.skip:
+        try:
+            line["fft_gkr"] = bench_fft_gkr(cpu=not a.no_cpu_baseline)
+        except Exception as exc:
+            line["fft_gkr"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
+        print("bench: fft_gkr done", file=sys.stderr, flush=True)
     if "rs_encode" not in a.skip:
         try:
             line["rs_encode"] = bench_rs_encode()
