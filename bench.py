@@ -395,12 +395,16 @@ def main(argv=None):
     sc.set_timing(True)
     k_ms = k_bytes = 0.0
     k_n = 0
+    t_ms, t_n = 0.0, 0
     for _ in range(max(3, a.steps // 4)):
         sc.prove_deg2_device(df.data_ptr(), dg.data_ptr(), NV, seed=2024, stream=stream)
         ms, by, n = sc.stats()
         k_ms += ms
         k_bytes += by
         k_n += n
+        tm, tn = sc.tail_stats()
+        t_ms += tm
+        t_n += tn
     sc.set_timing(False)
     achieved = k_bytes / (k_ms / 1e3) / 1e9
 
@@ -426,7 +430,13 @@ def main(argv=None):
                      "traffic": measured_traffic(),
                      "kernel": "k_sc2_fold_round",
                      "avg_launch_us": round(k_ms / k_n * 1e3, 2),
-                     "alg_bytes_per_launch": round(k_bytes / k_n, 1)},
+                     "launches_per_prove": k_n // max(1, max(3, a.steps // 4)),
+                     "alg_bytes_per_launch": round(k_bytes / k_n, 1),
+                     "tail_kernel": {"name": "k_sc2_tail_round",
+                                     "launches_per_prove": t_n // max(1, max(3, a.steps // 4)),
+                                     "avg_launch_us": round(t_ms / max(1, t_n) * 1e3, 2),
+                                     "note": "rounds with <= 1024 input entries per table: fold, "
+                                             "round sum and publish in one workgroup (latency-bound)"}},
         "check": {"round0_poly": [int(x) for x in proof.polys[0][:2]]},
     }
     for key, fn in (("gkr_frame", bench_frames_plain), ("gkr_frame_python_driver", bench_gkr_frame),

@@ -111,9 +111,13 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
                  void* stream);
 /* Per-launch HIP-event timing of the fused fold+round kernel (off by default). */
 void tp_sc2_set_timing(int on);
-/* Summed event milliseconds, algorithmic bytes and launch count of the fused
- * fold+round kernel in the last tp_sc2_prove. */
+/* Summed event milliseconds, algorithmic bytes and launch count of the
+ * streaming fused fold+round kernel in the last tp_sc2_prove (rounds whose
+ * input exceeds 1024 entries). */
 void tp_sc2_stats(double* fold_ms, double* fold_bytes, int* fold_launches);
+/* Summed event milliseconds and count of the one-workgroup tail rounds
+ * (fold + round sum + publish in one launch) of the last tp_sc2_prove. */
+void tp_sc2_tail_stats(double* tail_ms, int* tail_launches);
 
 /* ---- Reed-Solomon encoding: OZ/src/infrastructure/RS_polynomial.cpp:46-130
  *      (fast_fourier_transform, with Orion's root of unity,
@@ -260,6 +264,10 @@ int tp_fft_gkr(int lg_size, double* v_time, int* proof_size, double* p_time);
  * and the 64 sums.  Return as tp_fft_gkr. */
 int tp_fft_gkr_trace(int lg_size, uint64_t* polys, int cap_polys, int* n_polys, uint64_t* claims,
                      uint64_t* layers, int* proof_size);
+/* On (default): a run's launch sequence, which depends only on lg_size and
+ * the buffer addresses, is captured once into a hipGraph and replayed; off:
+ * every launch is enqueued directly. */
+void tp_fft_gkr_set_graph(int on);
 
 /* ---- host pieces of the frame prover, exported for their CPU tests ------- */
 

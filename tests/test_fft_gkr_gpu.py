@@ -70,3 +70,23 @@ def test_fft_gkr_cli_writes_reference_log_line(bridge, tmp_path):
     v, size, p = out.read_text().split()
     assert int(size) == 7008          # the reference's proof size at lg 7
     float(v), float(p)
+
+
+def test_graph_replays_equal_eager_runs(bridge):
+    """The captured launch sequence, replayed several times with eager runs of
+    other sizes in between, gives the eager result every time."""
+    try:
+        fft_gkr.set_graph(False)
+        eager = fft_gkr.trace(9, layers=True)
+        fft_gkr.set_graph(True)
+        for rep in range(3):
+            got = fft_gkr.trace(9, layers=True)
+            for key in ("polys", "claims", "layers"):
+                assert np.array_equal(got[key], eager[key]), (rep, key)
+            assert got["ok"] == 1
+            fft_gkr.set_graph(False)
+            other = fft_gkr.trace(5 + rep)
+            assert other["ok"] == 1
+            fft_gkr.set_graph(True)
+    finally:
+        fft_gkr.set_graph(True)

@@ -275,6 +275,34 @@ __device__ __forceinline__ void acc_pair(fp2 (&e)[5], fp2 f0, fp2 f1, fp2 g0, fp
     }
 }
 
+// The previous round's polynomial (a, b, c) from its partials: summed by
+// workgroup 0 of the next launch in the same stream (which starts after that
+// launch has finished), so no separate finalize launch is needed.
+struct Pending {
+    const fp2* partials;   // null: nothing pending
+    unsigned nb;
+    int has_h;
+    fp2* dst;
+};
+__device__ __forceinline__ void finish_pending(const Pending& pd)
+{
+    if (!pd.partials || blockIdx.x != 0) return;
+    fp2 acc[5];
+    sum_partials<5>(pd.partials, pd.nb, acc);
+    if (threadIdx.x == 0) {
+        const fp2 E0 = acc[0], E1 = acc[1], EA = acc[2];
+        fp2 b = sub2(sub2(E1, EA), E0), c = E0;
+        if (pd.has_h) {
+            b = add2(b, sub2(acc[4], acc[3]));
+            c = add2(c, acc[3]);
+        }
+        pd.dst[0] = EA;
+        pd.dst[1] = b;
+        pd.dst[2] = c;
+    }
+    __syncthreads();
+}
+
 // Round 0 (current_bit == 0): sums over pairs of the initial tables.
 template <bool H>
 __global__ void __launch_bounds__(256) k_fg_first(const V2* __restrict__ f,
@@ -298,8 +326,9 @@ template <bool H>
 __global__ void __launch_bounds__(256) k_fg_fold_round(
     const V2* __restrict__ fin, const V2* __restrict__ gin, const V2* __restrict__ hin,
     size_t n_quads, const fp2* __restrict__ chal, V2* __restrict__ fout, V2* __restrict__ gout,
-    V2* __restrict__ hout, fp2* partials)
+    V2* __restrict__ hout, fp2* partials, Pending pd)
 {
+    finish_pending(pd);
     const fp2 r = chal[0];
     fp2 e[5] = {zero2(), zero2(), zero2(), zero2(), zero2()};
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n_quads;
@@ -325,28 +354,78 @@ __global__ void __launch_bounds__(256) k_fg_fold_round(
     block_partial<5>(e, partials);
 }
 
-// Round polynomial (a, b, c) from the partials (one workgroup).
+// The remaining rounds k0..nv-1 of a sumcheck whose current input table (m_in
+// entries: T_{k0-1}, or T_0 when k0 = 0) fits in LDS, all in one workgroup:
+// the challenges are known in advance, so no round leaves the device.  Tables
+// ping-pong between two LDS regions (m_in and m_in / 2 entries per table).
+constexpr int kRestThreads = 256;
+constexpr size_t kRestMax = 512;
 template <bool H>
-__global__ void __launch_bounds__(kBlock) k_fg_poly(const fp2* partials, unsigned nb, fp2* dst)
+__global__ void __launch_bounds__(kRestThreads) k_fg_rest(
+    const V2* __restrict__ fin, const V2* __restrict__ gin, const V2* __restrict__ hin,
+    size_t m_in, int k0, int nv, const fp2* __restrict__ chal, fp2* polys, fp2* final_out,
+    Pending pd)
 {
-    fp2 acc[5];
-    sum_partials<5>(partials, nb, acc);
-    if (threadIdx.x == 0) {
-        const fp2 E0 = acc[0], E1 = acc[1], EA = acc[2];
-        fp2 b = sub2(sub2(E1, EA), E0), c = E0;
-        if (H) {
-            b = add2(b, sub2(acc[4], acc[3]));
-            c = add2(c, acc[3]);
-        }
-        dst[0] = EA;
-        dst[1] = b;
-        dst[2] = c;
+    finish_pending(pd);
+    extern __shared__ fp2 lds[];
+    const int nt = H ? 3 : 2;
+    fp2* A = lds;                       // nt x m_in
+    fp2* B = lds + (size_t)nt * m_in;   // nt x m_in / 2
+    for (size_t i = threadIdx.x; i < m_in; i += blockDim.x) {
+        A[i] = ldv(fin + i);
+        A[m_in + i] = ldv(gin + i);
+        if (H) A[2 * m_in + i] = ldv(hin + i);
     }
+    __syncthreads();
+    fp2* cur = A;
+    fp2* nxt = B;
+    size_t cm = m_in, stride_cur = m_in, stride_nxt = m_in / 2;
+    for (int k = k0; k < nv; ++k) {
+        if (k > 0) {                     // fold T_{k-1} with chal[k-1]
+            const fp2 r = chal[k - 1];
+            for (size_t i = threadIdx.x; i < cm / 2; i += blockDim.x)
+                for (int t = 0; t < nt; ++t)
+                    nxt[t * stride_nxt + i] =
+                        fold2(cur[t * stride_cur + 2 * i], cur[t * stride_cur + 2 * i + 1], r);
+            __syncthreads();
+            fp2* tmp = cur;
+            cur = nxt;
+            nxt = tmp;
+            const size_t s = stride_cur;
+            stride_cur = stride_nxt;
+            stride_nxt = s;
+            cm /= 2;
+        }
+        fp2 e[5] = {zero2(), zero2(), zero2(), zero2(), zero2()};
+        for (size_t p = threadIdx.x; p < cm / 2; p += blockDim.x) {
+            const fp2 h0 = H ? cur[2 * stride_cur + 2 * p] : zero2();
+            const fp2 h1 = H ? cur[2 * stride_cur + 2 * p + 1] : zero2();
+            acc_pair<H>(e, cur[2 * p], cur[2 * p + 1], cur[stride_cur + 2 * p],
+                        cur[stride_cur + 2 * p + 1], h0, h1);
+        }
+        block_sum<5>(e);
+        if (threadIdx.x == 0) {
+            fp2 b = sub2(sub2(e[1], e[2]), e[0]), c = e[0];
+            if (H) {
+                b = add2(b, sub2(e[4], e[3]));
+                c = add2(c, e[3]);
+            }
+            fp2* dst = polys + 3 * (size_t)(k - k0);
+            dst[0] = e[2];
+            dst[1] = b;
+            dst[2] = c;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) final_out[0] = fold2(cur[0], cur[1], chal[nv - 1]);
 }
 
 // v = the last pair of f folded with chal[0] (V_mult_add[0].eval(previous_random)).
-__global__ void k_fg_final(const V2* __restrict__ f, const fp2* __restrict__ chal, fp2* dst)
+__global__ void __launch_bounds__(kBlock) k_fg_final(const V2* __restrict__ f,
+                                                     const fp2* __restrict__ chal, fp2* dst,
+                                                     Pending pd)
 {
+    finish_pending(pd);
     if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = fold2(ldv(f), ldv(f + 1), chal[0]);
 }
 
@@ -395,14 +474,29 @@ fp2 eq_at(const fp2* z, int n, size_t idx, fp2 init)
     return b;
 }
 
+// eq table over nb variables by doubling (the reference's beta_g_*half loops,
+// :265-285): tab[t] = init prod_j (bit_j(t) ? z_j : 1 - z_j)
+void eq_table(const fp2* z, int nb, fp2 init, std::vector<fp2>& tab)
+{
+    tab.assign((size_t)1 << nb, zero2());
+    tab[0] = init;
+    for (int j = 0; j < nb; ++j) {
+        const size_t cur = (size_t)1 << j;
+        const fp2 omz = one_minus(z[j]);
+        for (size_t t = 0; t < cur; ++t) {
+            const fp2 b = tab[t];
+            tab[t + cur] = mul2(b, z[j]);
+            tab[t] = mul2(b, omz);
+        }
+    }
+}
+
 // host tables of an EqTab: lo/hi halves with alpha / beta folded into lo
 void eq_halves(const fp2* z, int nb, fp2 init, std::vector<fp2>& lo, std::vector<fp2>& hi,
                int lo_bits)
 {
-    lo.assign((size_t)1 << lo_bits, zero2());
-    hi.assign((size_t)1 << (nb - lo_bits), zero2());
-    for (size_t t = 0; t < lo.size(); ++t) lo[t] = eq_at(z, lo_bits, t, init);
-    for (size_t t = 0; t < hi.size(); ++t) hi[t] = eq_at(z + lo_bits, nb - lo_bits, t, one2());
+    eq_table(z, lo_bits, init, lo);
+    eq_table(z + lo_bits, nb - lo_bits, one2(), hi);
 }
 
 TP_HD fp2 qeval(const fp2* q, fp2 x) { return add2(mul2(add2(mul2(q[0], x), q[1]), x), q[2]); }
@@ -450,6 +544,12 @@ struct FgBuffers {
 };
 FgBuffers g_fb;
 std::mutex g_mu;
+bool g_use_graph = true;
+struct {
+    int lg = -1;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void*> ptrs;
+} g_graph;
 
 unsigned blocks_for(size_t n, size_t per = kBlock, unsigned cap = 2048)
 {
@@ -513,48 +613,69 @@ struct Run {
     // chal offset co: nv round polynomials into polys, the final fold into finals.
     bool phase(const V2* f, const V2* g, const V2* h, int nv, size_t co)
     {
-        fp2* partials = g_fb.partials.as<fp2>();
         fp2* polys = g_fb.polys.as<fp2>();
         const fp2* chal = g_fb.chal.as<fp2>() + co;
         size_t m = (size_t)1 << nv;
-        {
-            const unsigned nb = blocks_for(m / 2);
-            if (h) k_fg_first<true><<<nb, kBlock, 0, st>>>(f, g, h, m / 2, partials);
-            else k_fg_first<false><<<nb, kBlock, 0, st>>>(f, g, nullptr, m / 2, partials);
-            if (h) k_fg_poly<true><<<1, kBlock, 0, st>>>(partials, nb, polys + 3 * poly_idx);
-            else k_fg_poly<false><<<1, kBlock, 0, st>>>(partials, nb, polys + 3 * poly_idx);
-            TP_HIP_OK(hipGetLastError());
-            ++poly_idx;
-        }
         const V2* fi = f;
         const V2* gi = g;
         const V2* hi = h;
-        int ping = 0;
-        for (int k = 1; k < nv; ++k) {
-            const size_t quads = m / 4;
-            V2* fo = g_fb.pf[ping].as<V2>();
-            V2* go = g_fb.pg[ping].as<V2>();
-            V2* ho = g_fb.ph[ping].as<V2>();
-            const unsigned nb = blocks_for(quads);
-            if (h)
-                k_fg_fold_round<true><<<nb, kBlock, 0, st>>>(fi, gi, hi, quads, chal + (k - 1), fo,
-                                                             go, ho, partials);
-            else
-                k_fg_fold_round<false><<<nb, kBlock, 0, st>>>(fi, gi, nullptr, quads,
-                                                              chal + (k - 1), fo, go, nullptr,
-                                                              partials);
-            if (h) k_fg_poly<true><<<1, kBlock, 0, st>>>(partials, nb, polys + 3 * poly_idx);
-            else k_fg_poly<false><<<1, kBlock, 0, st>>>(partials, nb, polys + 3 * poly_idx);
+        int ping = 0, pp = 0;
+        int k = 0;
+        Pending pd = {nullptr, 0, h ? 1 : 0, nullptr};
+        // rounds over tables too large for one workgroup; each launch also
+        // finishes the previous round's polynomial
+        while (k < nv && (k == 0 ? m : 2 * m) > kRestMax) {
+            fp2* partials = g_fb.partials.as<fp2>() + (size_t)pp * 2048 * 5;
+            const unsigned nb = blocks_for(m / 2);
+            if (k == 0) {
+                if (h) k_fg_first<true><<<nb, kBlock, 0, st>>>(f, g, h, m / 2, partials);
+                else k_fg_first<false><<<nb, kBlock, 0, st>>>(f, g, nullptr, m / 2, partials);
+            } else {
+                const size_t quads = m / 2;      // input T_{k-1} has 2m entries
+                V2* fo = g_fb.pf[ping].as<V2>();
+                V2* go = g_fb.pg[ping].as<V2>();
+                V2* ho = g_fb.ph[ping].as<V2>();
+                if (h)
+                    k_fg_fold_round<true><<<nb, kBlock, 0, st>>>(fi, gi, hi, quads, chal + (k - 1),
+                                                                 fo, go, ho, partials, pd);
+                else
+                    k_fg_fold_round<false><<<nb, kBlock, 0, st>>>(fi, gi, nullptr, quads,
+                                                                  chal + (k - 1), fo, go, nullptr,
+                                                                  partials, pd);
+                fi = fo;
+                gi = go;
+                hi = h ? ho : nullptr;
+                ping ^= 1;
+            }
             TP_HIP_OK(hipGetLastError());
+            pd.partials = partials;
+            pd.nb = nb;
+            pd.dst = polys + 3 * poly_idx;
             ++poly_idx;
-            fi = fo;
-            gi = go;
-            hi = h ? ho : nullptr;
-            ping ^= 1;
-            m /= 2;
+            pp ^= 1;
+            ++k;
+            if (k < nv) m /= 2;
         }
-        k_fg_final<<<1, 64, 0, st>>>(fi, chal + (nv - 1), g_fb.finals.as<fp2>() + final_idx);
-        TP_HIP_OK(hipGetLastError());
+        // the rest in one workgroup: input table = T_{k-1} (2m entries) or T_0 (m)
+        if (k < nv) {
+            const size_t m_in = k == 0 ? m : 2 * m;
+            const size_t lds = (size_t)(h ? 3 : 2) * (m_in + m_in / 2) * sizeof(fp2);
+            if (h)
+                k_fg_rest<true><<<1, kRestThreads, lds, st>>>(fi, gi, hi, m_in, k, nv, chal,
+                                                              polys + 3 * poly_idx,
+                                                              g_fb.finals.as<fp2>() + final_idx, pd);
+            else
+                k_fg_rest<false><<<1, kRestThreads, lds, st>>>(fi, gi, nullptr, m_in, k, nv, chal,
+                                                               polys + 3 * poly_idx,
+                                                               g_fb.finals.as<fp2>() + final_idx,
+                                                               pd);
+            TP_HIP_OK(hipGetLastError());
+            poly_idx += nv - k;
+        } else {
+            k_fg_final<<<1, kBlock, 0, st>>>(fi, chal + (nv - 1), g_fb.finals.as<fp2>() + final_idx,
+                                            pd);
+            TP_HIP_OK(hipGetLastError());
+        }
         ++final_idx;
         return true;
     }
@@ -569,13 +690,28 @@ struct FgResult {
     double p_ms = 0, v_ms = 0;
 };
 
+bool rest_lds_limits()
+{
+    static bool done = false;
+    if (done) return true;
+    const int need = (int)(3 * (kRestMax + kRestMax / 2) * sizeof(fp2));
+    TP_HIP_OK(hipFuncSetAttribute((const void*)k_fg_rest<true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, need));
+    TP_HIP_OK(hipFuncSetAttribute((const void*)k_fg_rest<false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, need));
+    done = true;
+    return true;
+}
+
 bool run_fft_gkr(int lg, bool want_layers, FgResult& R)
 {
     std::lock_guard<std::mutex> lk(g_mu);
+    if (!rest_lds_limits()) return false;
     if (lg < 1 || lg > 22) return fail("fft_gkr: lg_size=%d outside [1, 22]", lg);
     const auto t0 = std::chrono::steady_clock::now();
     const Challenges C(lg);
-    hipStream_t st = stream();
+    static hipStream_t st = nullptr;        // own stream: a graph capture takes nothing else
+    if (!st) TP_HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     Run E(lg, st, C);
     const size_t n = E.n;
     const int n_rounds = (lg + 6) + lg + lg * 2 * lg;
@@ -606,7 +742,7 @@ bool run_fft_gkr(int lg, bool want_layers, FgResult& R)
         !g_fb.hA.ensure(n * 16) || !g_fb.pf[0].ensure(big / 2 * 16) ||
         !g_fb.pg[0].ensure(big / 2 * 16) || !g_fb.ph[0].ensure(n / 2 * 16 + 16) ||
         !g_fb.pf[1].ensure(big / 4 * 16 + 16) || !g_fb.pg[1].ensure(big / 4 * 16 + 16) ||
-        !g_fb.ph[1].ensure(n / 4 * 16 + 16) || !g_fb.partials.ensure(2048 * 5 * 16) ||
+        !g_fb.ph[1].ensure(n / 4 * 16 + 16) || !g_fb.partials.ensure(2 * 2048 * 5 * 16) ||
         !g_fb.polys.ensure((size_t)n_rounds * 3 * 16) || !g_fb.finals.ensure((2 + 2 * lg) * 16) ||
         !g_fb.sums.ensure(64 * 16) || !g_fb.rowp.ensure(64 * 64 * 16) ||
         !g_fb.eqt.ensure(E.eqhost.size() * 16) || !g_fb.chal.ensure(E.chalhost.size() * 16))
@@ -618,73 +754,112 @@ bool run_fft_gkr(int lg, bool want_layers, FgResult& R)
     hipEvent_t ev0, ev1;
     TP_HIP_OK(hipEventCreate(&ev0));
     TP_HIP_OK(hipEventCreate(&ev1));
-    TP_HIP_OK(hipEventRecord(ev0, st));
-
-    // A16: build the circuit
+    // host-side constants of the circuit
     PointArg rr;
     memset(&rr, 0, sizeof(rr));
     for (int i = 0; i < lg; ++i) rr.z[i] = C.r[i];
-    k_fg_expand<<<blocks_for(n, kBlock, 8192), kBlock, 0, st>>>(rr, lg, E.L(0));
     fp2 rou = mk2(2147483648ULL, 1033321771269002680ULL);      // order 2^62 (prime_field.cpp:60-73)
     for (int i = 0; i < 62 - lg; ++i) rou = mul2(rou, rou);
-    const fp2 inv_rou = powe(mul2(rou, mk2(rou.re, fneg(rou.im))), P - 2);   // 1/(a+bi) = (a-bi)/N
-    const fp2 irou = mul2(mk2(rou.re, fneg(rou.im)), inv_rou);
-    k_fg_pow_table<<<blocks_for(n / 2 + 1, kBlock, 8192), kBlock, 0, st>>>(irou, n / 2 + 1,
-                                                                         g_fb.tw.as<V2>());
-    for (int s = 1; s <= lg; ++s)
-        k_fg_butterfly<<<blocks_for(n / 2, kBlock, 8192), kBlock, 0, st>>>(
-            E.L(s - 1), E.L(s), lg, lg - s, g_fb.tw.as<V2>());
+    const fp2 conj = mk2(rou.re, fneg(rou.im));                 // 1/(a+bi) = (a-bi)/(a^2+b^2)
+    const fp2 irou = mul2(conj, powe(mul2(rou, conj), P - 2));
     u64 inv_n = 1, base = n % P, e = P - 2;
     while (e) {
         if (e & 1) inv_n = fmul(inv_n, base);
         base = fmul(base, base);
         e >>= 1;
     }
-    k_fg_scale<<<blocks_for(n, kBlock, 8192), kBlock, 0, st>>>(E.L(lg), inv_n, n,
-                                                               g_fb.scaled.as<V2>());
     Pts64 ep;
     for (int i = 0; i < 64; ++i) ep.p[i] = C.ep[i];
-    {
-        const size_t ch = n < (size_t)kEvalChunk ? n : (size_t)kEvalChunk;
-        k_fg_eval<<<blocks_for(64 * (n / ch), kBlock, 8192), kBlock, 0, st>>>(
-            g_fb.scaled.as<V2>(), lg, ep, g_fb.ev.as<V2>());
-    }
-    {
-        const unsigned bx = blocks_for(n, kBlock, 64);
-        k_fg_rowsum<<<dim3(bx, 64), kBlock, 0, st>>>(g_fb.ev.as<V2>(), n, g_fb.rowp.as<fp2>());
-        k_fg_rowsum_final<<<64, kBlock, 0, st>>>(g_fb.rowp.as<fp2>(), bx, g_fb.sums.as<fp2>());
-    }
-    TP_HIP_OK(hipGetLastError());
-
-    // A18: addition layer
     Pts64 tmp;
     for (int i = 0; i < 64; ++i)
         tmp.p[i] = add2(eq_at(C.r0.data(), 6, i, one2()), eq_at(C.r1.data(), 6, i, zero2()));
-    k_fg_fill_blocks<<<blocks_for(big, kBlock, 8192), kBlock, 0, st>>>(g_fb.gA.as<V2>(), lg, tmp);
-    if (!E.phase(g_fb.ev.as<V2>(), g_fb.gA.as<V2>(), nullptr, lg + 6, co_add)) return false;
-    // mult layer
-    k_fg_mult_init<<<blocks_for((n + kMultChunk - 1) / kMultChunk, kBlock, 8192), kBlock, 0, st>>>(
-        g_fb.gA.as<V2>(), lg, E.tab(eq_mult), ep);
-    if (!E.phase(g_fb.scaled.as<V2>(), g_fb.gA.as<V2>(), nullptr, lg, co_mult)) return false;
-    // IFFT layers
-    for (int d = 0; d < lg; ++d) {
-        const V2* pre = E.L(lg - d - 1);
-        const EqTab et = E.tab(eq_ifft[d]);
-        k_fg_ifft_init1<<<blocks_for(n / 2, kBlock, 8192), kBlock, 0, st>>>(
-            pre, g_fb.gA.as<V2>(), g_fb.hA.as<V2>(), lg, d, g_fb.tw.as<V2>(), et);
-        if (!E.phase(pre, g_fb.gA.as<V2>(), g_fb.hA.as<V2>(), lg, co_u[d])) return false;
-        const EqTab bt = E.tab(eq_bu[d]);
-        EqOne bu;
-        bu.lo = bt.lo0;
-        bu.hi = bt.hi0;
-        bu.lo_bits = bt.lo_bits;
-        k_fg_ifft_init2<<<blocks_for(n / 2, kBlock, 8192), kBlock, 0, st>>>(
-            g_fb.gA.as<V2>(), g_fb.hA.as<V2>(), lg, d, g_fb.tw.as<V2>(), et, bu,
-            g_fb.finals.as<fp2>() + E.final_idx - 1);
-        if (!E.phase(pre, g_fb.gA.as<V2>(), g_fb.hA.as<V2>(), lg, co_v[d])) return false;
+
+    // every launch of the run, in order, on st
+    auto enqueue = [&]() -> bool {
+        // A16: build the circuit
+        k_fg_expand<<<blocks_for(n, kBlock, 8192), kBlock, 0, st>>>(rr, lg, E.L(0));
+        k_fg_pow_table<<<blocks_for(n / 2 + 1, kBlock, 8192), kBlock, 0, st>>>(irou, n / 2 + 1,
+                                                                             g_fb.tw.as<V2>());
+        for (int s = 1; s <= lg; ++s)
+            k_fg_butterfly<<<blocks_for(n / 2, kBlock, 8192), kBlock, 0, st>>>(
+                E.L(s - 1), E.L(s), lg, lg - s, g_fb.tw.as<V2>());
+        k_fg_scale<<<blocks_for(n, kBlock, 8192), kBlock, 0, st>>>(E.L(lg), inv_n, n,
+                                                                   g_fb.scaled.as<V2>());
+        {
+            const size_t ch = n < (size_t)kEvalChunk ? n : (size_t)kEvalChunk;
+            k_fg_eval<<<blocks_for(64 * (n / ch), kBlock, 8192), kBlock, 0, st>>>(
+                g_fb.scaled.as<V2>(), lg, ep, g_fb.ev.as<V2>());
+        }
+        {
+            const unsigned bx = blocks_for(n, kBlock, 64);
+            k_fg_rowsum<<<dim3(bx, 64), kBlock, 0, st>>>(g_fb.ev.as<V2>(), n, g_fb.rowp.as<fp2>());
+            k_fg_rowsum_final<<<64, kBlock, 0, st>>>(g_fb.rowp.as<fp2>(), bx, g_fb.sums.as<fp2>());
+        }
+        TP_HIP_OK(hipGetLastError());
+        // A18: addition layer
+        k_fg_fill_blocks<<<blocks_for(big, kBlock, 8192), kBlock, 0, st>>>(g_fb.gA.as<V2>(), lg,
+                                                                          tmp);
+        if (!E.phase(g_fb.ev.as<V2>(), g_fb.gA.as<V2>(), nullptr, lg + 6, co_add)) return false;
+        // mult layer
+        k_fg_mult_init<<<blocks_for((n + kMultChunk - 1) / kMultChunk, kBlock, 8192), kBlock, 0,
+                         st>>>(g_fb.gA.as<V2>(), lg, E.tab(eq_mult), ep);
+        if (!E.phase(g_fb.scaled.as<V2>(), g_fb.gA.as<V2>(), nullptr, lg, co_mult)) return false;
+        // IFFT layers
+        for (int d = 0; d < lg; ++d) {
+            const V2* pre = E.L(lg - d - 1);
+            const EqTab et = E.tab(eq_ifft[d]);
+            k_fg_ifft_init1<<<blocks_for(n / 2, kBlock, 8192), kBlock, 0, st>>>(
+                pre, g_fb.gA.as<V2>(), g_fb.hA.as<V2>(), lg, d, g_fb.tw.as<V2>(), et);
+            if (!E.phase(pre, g_fb.gA.as<V2>(), g_fb.hA.as<V2>(), lg, co_u[d])) return false;
+            const EqTab bt = E.tab(eq_bu[d]);
+            EqOne bu;
+            bu.lo = bt.lo0;
+            bu.hi = bt.hi0;
+            bu.lo_bits = bt.lo_bits;
+            k_fg_ifft_init2<<<blocks_for(n / 2, kBlock, 8192), kBlock, 0, st>>>(
+                g_fb.gA.as<V2>(), g_fb.hA.as<V2>(), lg, d, g_fb.tw.as<V2>(), et, bu,
+                g_fb.finals.as<fp2>() + E.final_idx - 1);
+            if (!E.phase(pre, g_fb.gA.as<V2>(), g_fb.hA.as<V2>(), lg, co_v[d])) return false;
+        }
+        TP_HIP_OK(hipGetLastError());
+        return true;
+    };
+
+    // The launch sequence depends only on lg and the buffer addresses, so it is
+    // captured once into a graph and replayed (unless graphs are switched off).
+    std::vector<void*> ptrs;
+    for (DevBuf* b : {&g_fb.layers, &g_fb.tw, &g_fb.scaled, &g_fb.ev, &g_fb.gA, &g_fb.hA,
+                      &g_fb.pf[0], &g_fb.pf[1], &g_fb.pg[0], &g_fb.pg[1], &g_fb.ph[0],
+                      &g_fb.ph[1], &g_fb.partials, &g_fb.polys, &g_fb.finals, &g_fb.sums,
+                      &g_fb.rowp, &g_fb.eqt, &g_fb.chal})
+        ptrs.push_back(b->ptr);
+    if (g_use_graph) {
+        if (!g_graph.exec || g_graph.lg != lg || g_graph.ptrs != ptrs) {
+            if (g_graph.exec) (void)hipGraphExecDestroy(g_graph.exec);
+            g_graph.exec = nullptr;
+            TP_HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const bool ok_enq = enqueue();
+            hipGraph_t graph = nullptr;
+            const hipError_t ec = hipStreamEndCapture(st, &graph);
+            if (!ok_enq) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return false;
+            }
+            if (ec != hipSuccess) return fail("fft_gkr: graph capture failed: %s", hipGetErrorString(ec));
+            const hipError_t ei = hipGraphInstantiate(&g_graph.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ei != hipSuccess) return fail("fft_gkr: graph instantiate failed: %s", hipGetErrorString(ei));
+            g_graph.lg = lg;
+            g_graph.ptrs = ptrs;
+        }
+        TP_HIP_OK(hipEventRecord(ev0, st));
+        TP_HIP_OK(hipGraphLaunch(g_graph.exec, st));
+        TP_HIP_OK(hipEventRecord(ev1, st));
+    } else {
+        TP_HIP_OK(hipEventRecord(ev0, st));
+        if (!enqueue()) return false;
+        TP_HIP_OK(hipEventRecord(ev1, st));
     }
-    TP_HIP_OK(hipGetLastError());
-    TP_HIP_OK(hipEventRecord(ev1, st));
 
     std::vector<fp2> polys((size_t)n_rounds * 3), finals(2 + 2 * lg), sums(64);
     TP_HIP_OK(hipMemcpyAsync(polys.data(), g_fb.polys.ptr, polys.size() * 16,
@@ -836,6 +1011,12 @@ bool run_fft_gkr(int lg, bool want_layers, FgResult& R)
 }  // namespace
 
 extern "C" {
+
+void tp_fft_gkr_set_graph(int on)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_use_graph = on != 0;
+}
 
 int tp_fft_gkr(int lg_size, double* v_time, int* proof_size, double* p_time)
 {

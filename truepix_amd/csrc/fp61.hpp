@@ -27,9 +27,11 @@ TP_HD u64 fneg(u64 a) { return a ? P - a : 0; }
 TP_HD u64 fred64(u64 x) { return canon((x & P) + (x >> 61)); }  // any u64
 
 // a, b < 2^62 (so a canonical value plus another canonical value is allowed);
-// result canonical.
+// result canonical.  Device code has no 64 x 64 multiplier: it splits the
+// operands into 32-bit limbs (below); host code uses the 128-bit product.
 TP_HD u64 fmul(u64 a, u64 b)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
     const u32 a0 = (u32)a, a1 = (u32)(a >> 32);   // a1 < 2^30
     const u32 b0 = (u32)b, b1 = (u32)(b >> 32);
     const u64 lo = (u64)a0 * b0;                  // < 2^64
@@ -40,6 +42,11 @@ TP_HD u64 fmul(u64 a, u64 b)
     const u64 s = (lo & P) + (lo >> 61) + ((mid & ((1ULL << 29) - 1)) << 32)
                 + (mid >> 29) + (hi << 3);        // < 2^64
     return canon((s & P) + (s >> 61));
+#else
+    const unsigned __int128 t = (unsigned __int128)a * b;   // < 2^124
+    const u64 s = ((u64)t & P) + (u64)(t >> 61);            // < 2^63 + 2^61
+    return canon((s & P) + (s >> 61));
+#endif
 }
 
 struct fp2 { u64 re, im; };

@@ -18,6 +18,7 @@
 // memory -> the host polls the sequence number, runs the transcript, and
 // launches the next round with the challenge as a kernel argument.
 #include "fp61.hpp"
+#include "host_sha.hpp"
 #include "reduce.hpp"
 #include "runtime.hpp"
 #include "transcript.hpp"
@@ -43,17 +44,28 @@ __device__ __forceinline__ void accum_pair(fp2 (&e)[3], fp2 f0, fp2 f1, fp2 g0, 
     e[2] = add2(e[2], mul2(sub2(f1, f0), sub2(g1, g0)));
 }
 
-// Round 0: no fold, sum over the n/2 pairs of the input tables.
+// Round 0: no fold, sum over the pairs of the input tables; a thread owns two
+// adjacent pairs (64 contiguous bytes per table), as the fused rounds do.
 __global__ void __launch_bounds__(256) k_sc2_first(const V2* __restrict__ f,
                                                    const V2* __restrict__ g, size_t n_pairs,
                                                    fp2* partials)
 {
     fp2 e[3] = {zero2(), zero2(), zero2()};
-    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n_pairs;
-         p += (size_t)gridDim.x * blockDim.x) {
-        const fp2 f0 = ldv(f + 2 * p), f1 = ldv(f + 2 * p + 1);
-        const fp2 g0 = ldv(g + 2 * p), g1 = ldv(g + 2 * p + 1);
-        accum_pair(e, f0, f1, g0, g1);
+    const size_t n_quads = n_pairs / 2;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n_quads;
+         q += (size_t)gridDim.x * blockDim.x) {
+        const V2* fp = f + 4 * q;
+        const V2* gp = g + 4 * q;
+        const fp2 fa = ldv(fp), fb = ldv(fp + 1), fc = ldv(fp + 2), fd = ldv(fp + 3);
+        const fp2 ga = ldv(gp), gb = ldv(gp + 1), gc = ldv(gp + 2), gd = ldv(gp + 3);
+        accum_pair(e, fa, fb, ga, gb);
+        accum_pair(e, fc, fd, gc, gd);
+    }
+    if (n_pairs & 1) {   // one pair (n = 2) left over
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            const size_t p = n_pairs - 1;
+            accum_pair(e, ldv(f + 2 * p), ldv(f + 2 * p + 1), ldv(g + 2 * p), ldv(g + 2 * p + 1));
+        }
     }
     block_partial<3>(e, partials);
 }
@@ -83,6 +95,38 @@ __global__ void __launch_bounds__(256) k_sc2_fold_round(const V2* __restrict__ f
         accum_pair(e, f0, f1, g0, g1);
     }
     block_partial<3>(e, partials);
+}
+
+// Rounds whose input table is at most one workgroup's quads (kTailQuads): fold,
+// round sum and publish in one launch, without the partials round trip.
+constexpr int kTailThreads = 256;
+constexpr size_t kTailQuads = 256;
+__global__ void __launch_bounds__(kTailThreads) k_sc2_tail_round(
+    const V2* __restrict__ fin, const V2* __restrict__ gin, size_t n_quads, u64 rr, u64 ri,
+    V2* __restrict__ fout, V2* __restrict__ gout, fp2* host_vals, volatile unsigned* host_seq,
+    unsigned seq)
+{
+    const fp2 r = mk2(rr, ri);
+    fp2 e[3] = {zero2(), zero2(), zero2()};
+    for (size_t q = threadIdx.x; q < n_quads; q += blockDim.x) {
+        const V2* fp = fin + 4 * q;
+        const V2* gp = gin + 4 * q;
+        const fp2 f0 = fold2(ldv(fp), ldv(fp + 1), r), f1 = fold2(ldv(fp + 2), ldv(fp + 3), r);
+        const fp2 g0 = fold2(ldv(gp), ldv(gp + 1), r), g1 = fold2(ldv(gp + 2), ldv(gp + 3), r);
+        stv(fout + 2 * q, f0);
+        stv(fout + 2 * q + 1, f1);
+        stv(gout + 2 * q, g0);
+        stv(gout + 2 * q + 1, g1);
+        accum_pair(e, f0, f1, g0, g1);
+    }
+    block_sum<3>(e);
+    if (threadIdx.x == 0) {
+        host_vals[0] = e[0];
+        host_vals[1] = e[1];
+        host_vals[2] = e[2];
+        __threadfence_system();
+        *host_seq = seq;
+    }
 }
 
 // One workgroup: sum the partials and publish (E0, E1, EA) plus the sequence
@@ -128,6 +172,8 @@ struct Sc2State {
     bool timed = false;          // per-launch event timing of the fused kernel
     double fold_ms = 0, fold_bytes = 0;
     int fold_launches = 0;
+    double tail_ms = 0;          // the one-workgroup tail rounds
+    int tail_launches = 0;
 };
 Sc2State g_sc2;
 
@@ -212,13 +258,15 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
     S.fold_ms = 0;
     S.fold_bytes = 0;
     S.fold_launches = 0;
+    S.tail_ms = 0;
+    S.tail_launches = 0;
 
     fp2* partials = S.partials.as<fp2>();
     V2* A = S.work.as<V2>();             // f: n/2, then g: n/2
     V2* B = A + 2 * (n / 2 + 1);         // f: n/4, then g: n/4
     const size_t capA = n / 2 + 1, capB = n / 4 + 1;
 
-    FsTranscript fs;
+    FsTranscriptT<HostSha256> fs;
     fs.init("truepix");
     fs.item_str("c2", 2);   // put(["c2", seed])
     fs.item_int(seed);
@@ -229,9 +277,23 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
     u64 rr = 0, ri = 0;
     for (int k = 0; k < nv; ++k) {
         unsigned nb;
+        bool published = false;
+        const unsigned seq = ++S.seq;
         if (k == 0) {
-            nb = blocks_for(m / 2);
+            nb = blocks_for((m / 2 + 1) / 2);
             k_sc2_first<<<nb, kBlock, 0, st>>>(cf, cg, m / 2, partials);
+        } else if (m / 2 <= kTailQuads) {
+            const size_t quads = m / 2;
+            V2* of = (k & 1) ? A : B;
+            V2* og = of + ((k & 1) ? capA : capB);
+            nb = 1;
+            if (S.timed) TP_HIP_OK(hipEventRecord(S.ev[0], st));
+            k_sc2_tail_round<<<1, kTailThreads, 0, st>>>(cf, cg, quads, rr, ri, of, og, S.dev_vals,
+                                                         S.dev_seq, seq);
+            if (S.timed) TP_HIP_OK(hipEventRecord(S.ev[1], st));
+            cf = of;
+            cg = og;
+            published = true;
         } else {
             const size_t quads = m / 2;   // input T_{k-1} has length 2m
             V2* of = (k & 1) ? A : B;
@@ -246,15 +308,21 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             S.fold_launches += 1;
         }
         TP_HIP_OK(hipGetLastError());
-        const unsigned seq = ++S.seq;
-        k_sc2_publish<<<1, kBlock, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
-        TP_HIP_OK(hipGetLastError());
+        if (!published) {
+            k_sc2_publish<<<1, kBlock, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
+            TP_HIP_OK(hipGetLastError());
+        }
         if (!wait_seq(st, seq)) return -1;
         if (S.timed && k > 0) {
             TP_HIP_OK(hipEventSynchronize(S.ev[1]));
             float ms = 0;
             TP_HIP_OK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
-            S.fold_ms += ms;
+            if (published) {
+                S.tail_ms += ms;
+                S.tail_launches += 1;
+            } else {
+                S.fold_ms += ms;
+            }
         }
         const fp2 E0 = S.host_vals[0], E1 = S.host_vals[1], EA = S.host_vals[2];
         const fp2 a = EA, c = E0, b = sub2(sub2(E1, EA), E0);
@@ -292,12 +360,21 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
 void tp_sc2_set_timing(int on) { g_sc2.timed = on != 0; }
 
 // Statistics of the last tp_sc2_prove: summed event time and algorithmic
-// bytes of the fused fold+round launches, and their count.
+// bytes of the streaming fused fold+round launches (k_sc2_fold_round), and
+// their count.
 void tp_sc2_stats(double* fold_ms, double* fold_bytes, int* fold_launches)
 {
     *fold_ms = g_sc2.fold_ms;
     *fold_bytes = g_sc2.fold_bytes;
     *fold_launches = g_sc2.fold_launches;
+}
+
+// Summed event time and count of the one-workgroup tail rounds of the last
+// tp_sc2_prove (timing on).
+void tp_sc2_tail_stats(double* tail_ms, int* tail_launches)
+{
+    *tail_ms = g_sc2.tail_ms;
+    *tail_launches = g_sc2.tail_launches;
 }
 
 }  // extern "C"

@@ -33,6 +33,13 @@ def run(lg_size: int):
     return bool(rc), v.value, ps.value, p.value
 
 
+def set_graph(on: bool) -> None:
+    """Replay each size's launch sequence from a captured hipGraph (default),
+    or enqueue every launch directly."""
+    nb.require()
+    nb.LIB.tp_fft_gkr_set_graph(1 if on else 0)
+
+
 def trace(lg_size: int, layers: bool = False) -> dict:
     """Everything one run computes: round polynomials (n x 6 limbs), the claim
     after every layer with the final alpha / beta (7 x 2 limbs), optionally

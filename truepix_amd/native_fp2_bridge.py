@@ -101,6 +101,7 @@ def bind(lib) -> None:
     _sig(lib, "tp_mle_eval_base_dev", [_VP, _I32, _U64P, _U64P, _VP], _I32)
     _sig(lib, "tp_sc2_stats", [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                _I32P])
+    _sig(lib, "tp_sc2_tail_stats", [ctypes.POINTER(ctypes.c_double), _I32P])
     _U8PP = ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8))
     _sig(lib, "tp_pickle_tree", [ctypes.c_char_p, _SIZE, _U8PP, ctypes.POINTER(_SIZE)], _I32)
     _sig(lib, "tp_free", [_VP])
@@ -123,6 +124,7 @@ def bind(lib) -> None:
     _sig(lib, "tp_pool_release", [_VP, ctypes.c_int64])
     _sig(lib, "tp_fft_gkr", [_I32, ctypes.POINTER(_D), _I32P, ctypes.POINTER(_D)], _I32)
     _sig(lib, "tp_fft_gkr_trace", [_I32, _U64P, _I32, _I32P, _U64P, _U64P, _I32P], _I32)
+    _sig(lib, "tp_fft_gkr_set_graph", [_I32])
 
 
 def load_library(path: str = LIB_PATH):

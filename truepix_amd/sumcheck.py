@@ -89,3 +89,10 @@ def stats():
     ms, by, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
     nb.LIB.tp_sc2_stats(ctypes.byref(ms), ctypes.byref(by), ctypes.byref(n))
     return ms.value, by.value, n.value
+
+
+def tail_stats():
+    """(ms, launches) of the one-workgroup tail rounds of the last prove."""
+    ms, n = ctypes.c_double(), ctypes.c_int()
+    nb.LIB.tp_sc2_tail_stats(ctypes.byref(ms), ctypes.byref(n))
+    return ms.value, n.value
