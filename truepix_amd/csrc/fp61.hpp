@@ -26,10 +26,11 @@ TP_HD u64 fsub(u64 a, u64 b) { return canon(a + P - b); }
 TP_HD u64 fneg(u64 a) { return a ? P - a : 0; }
 TP_HD u64 fred64(u64 x) { return canon((x & P) + (x >> 61)); }  // any u64
 
-// a, b < 2^62 (so a canonical value plus another canonical value is allowed);
-// result canonical.  Device code has no 64 x 64 multiplier: it splits the
-// operands into 32-bit limbs (below); host code uses the 128-bit product.
-TP_HD u64 fmul(u64 a, u64 b)
+// a * b mod p as a value in [0, 2^61 + 7] (congruent, not yet canonical);
+// a, b < 2^62 (so a canonical value plus another canonical value is allowed).
+// Device code has no 64 x 64 multiplier: it splits the operands into 32-bit
+// limbs (below); host code uses the 128-bit product.
+TP_HD u64 fmul_lazy(u64 a, u64 b)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     const u32 a0 = (u32)a, a1 = (u32)(a >> 32);   // a1 < 2^30
@@ -41,13 +42,18 @@ TP_HD u64 fmul(u64 a, u64 b)
     //     = 8 hi + (mid >> 29) + ((mid mod 2^29) << 32) + (lo >> 61) + (lo mod 2^61)
     const u64 s = (lo & P) + (lo >> 61) + ((mid & ((1ULL << 29) - 1)) << 32)
                 + (mid >> 29) + (hi << 3);        // < 2^64
-    return canon((s & P) + (s >> 61));
 #else
     const unsigned __int128 t = (unsigned __int128)a * b;   // < 2^124
     const u64 s = ((u64)t & P) + (u64)(t >> 61);            // < 2^63 + 2^61
-    return canon((s & P) + (s >> 61));
 #endif
+    return (s & P) + (s >> 61);
 }
+
+// Any u64 to its canonical residue (x mod 2^61 folded once, then < 2p).
+TP_HD u64 fold61(u64 x) { return canon((x & P) + (x >> 61)); }
+
+// a, b < 2^62; result canonical.
+TP_HD u64 fmul(u64 a, u64 b) { return canon(fmul_lazy(a, b)); }
 
 struct fp2 { u64 re, im; };
 
@@ -59,13 +65,15 @@ TP_HD fp2 sub2(fp2 a, fp2 b) { return mk2(fsub(a.re, b.re), fsub(a.im, b.im)); }
 TP_HD fp2 dbl2(fp2 a) { return add2(a, a); }
 TP_HD bool eq2(fp2 a, fp2 b) { return a.re == b.re && a.im == b.im; }
 
-// Karatsuba: re = ac - bd, im = (a + b)(c + d) - ac - bd.
+// Karatsuba: re = ac - bd, im = (a + b)(c + d) - ac - bd, with the three
+// products left unreduced (each < 2^61 + 8) and one fold per coordinate:
+// re = ac + (2p - bd) < 2^63, im = k + (4p - ac - bd) < 2^64.
 TP_HD fp2 mul2(fp2 x, fp2 y)
 {
-    const u64 ac = fmul(x.re, y.re);
-    const u64 bd = fmul(x.im, y.im);
-    const u64 k = fmul(x.re + x.im, y.re + y.im);   // operands < 2^62
-    return mk2(fsub(ac, bd), fsub(k, fadd(ac, bd)));
+    const u64 ac = fmul_lazy(x.re, y.re);
+    const u64 bd = fmul_lazy(x.im, y.im);
+    const u64 k = fmul_lazy(x.re + x.im, y.re + y.im);   // operands < 2^62
+    return mk2(fold61(ac + (2 * P - bd)), fold61(k + (4 * P - ac - bd)));
 }
 
 // Base-field scalar times Fp2.

@@ -14,11 +14,13 @@
 //          an M2-point DFT over j2 giving q2;
 //   pass B (per row u):    result[u + S M2 q1] = sum_{j1<M1} Y[u][j1] w_{M1}^(j1 q1).
 // Each pass stages 8 rows of at most 1024 points in LDS (8 x 1025 x 16 B) and
-// runs radix-4 (plus one radix-2) Stockham stages; the 8-row tiles make every
+// runs radix-8 Stockham stages (after one radix-2 or radix-4 stage when the
+// length is not a power of 8); the 8-row tiles make every
 // global load and store a 128-byte segment (8 consecutive Fp2).  Columns are
 // processed a few at a time so the intermediate stays in the 256 MB MALL.
-// Roots of order 4 are +-i in Fp2, so radix-4 butterflies multiply by w4
-// with a coordinate swap and one negation.
+// Roots of order 4 are +-i in Fp2, so multiplying by w4 is a coordinate swap
+// and one negation; roots of order 8 are 2^30 (+-1 +-i), so multiplying by w8
+// is a coordinate mix and a 30-bit rotation of each 61-bit coordinate.
 #include "fp61.hpp"
 #include "runtime.hpp"
 
@@ -34,7 +36,7 @@ __device__ __forceinline__ fp2 ldv(const V2* p) { const V2 v = *p; return mk2(v.
 __device__ __forceinline__ void stv(V2* p, fp2 x) { V2 v; v.re = x.re; v.im = x.im; *p = v; }
 
 constexpr int kRows = 8;          // rows per LDS tile
-constexpr int kThreads = 512;     // threads per NTT workgroup
+constexpr int kThreads = 1024;    // threads per NTT workgroup (16 waves: 4 per SIMD)
 constexpr int kMaxLog = 10;       // LDS DFT length up to 2^10
 constexpr int kRowStride = (1 << kMaxLog) + 1;   // +1 element pads LDS banks
 
@@ -45,6 +47,7 @@ struct Tw {
     const V2* th;      // w^(e 2^h), e < 2^(o - h)
     int h;
     int w4_is_i;       // w_4 == +i (else -i)
+    int w8_s1, w8_s2;  // w_8 = 2^30 (s1 + s2 i), s1, s2 in {+1, -1}
 };
 
 __device__ __forceinline__ fp2 wpow(const Tw& t, u64 e)
@@ -58,6 +61,20 @@ __device__ __forceinline__ fp2 mul_w4(fp2 v, int is_i)
     return is_i ? mk2(fneg(v.im), v.re) : mk2(v.im, fneg(v.re));
 }
 
+// x 2^30 mod p for a canonical x: a 30-bit rotation of the 61-bit word
+// (2^61 = 1), so the result is canonical again.
+__device__ __forceinline__ u64 rot30(u64 x) { return ((x << 30) & P) | (x >> 31); }
+
+// v * w_8 with w_8 = 2^30 (s1 + s2 i) (2^30 (1 + i) squares to 2^61 i = i):
+// a coordinate mix by signs and two rotations, no field multiply.
+__device__ __forceinline__ fp2 mul_w8(fp2 v, int s1, int s2)
+{
+    const u64 a = v.re, b = v.im;
+    const u64 re = fsub(s1 > 0 ? a : fneg(a), s2 > 0 ? b : fneg(b));   // s1 a - s2 b
+    const u64 im = fadd(s2 > 0 ? a : fneg(a), s1 > 0 ? b : fneg(b));   // s2 a + s1 b
+    return mk2(rot30(re), rot30(im));
+}
+
 // Forward DFT of `rows` rows of length 2^l held in LDS (row stride
 // kRowStride), Stockham auto-sort: stage with span Ns reads x[j + i L/R],
 // multiplies by w_{Ns R}^((j mod Ns) i), applies DFT_R and writes
@@ -68,7 +85,8 @@ __device__ void lds_dft(fp2* lds, int rows, int l, const Tw& tw)
     const int L = 1 << l;
     int Ns = 1;
     int log_ns = 0;
-    if (l & 1) {   // one radix-2 stage (twiddles of a span-1 stage are all 1)
+    // l = 3 q + rem: one radix-2 (rem 1) or radix-4 (rem 2) stage, then radix 8
+    if (l % 3 == 1) {   // radix-2 stage (twiddles of a span-1 stage are all 1)
         const int nb = L >> 1, total = rows * nb;
         constexpr int IT = (kRows << (kMaxLog - 1)) / kThreads;
         fp2 v[IT][2];
@@ -96,11 +114,9 @@ __device__ void lds_dft(fp2* lds, int rows, int l, const Tw& tw)
         __syncthreads();
         Ns = 2;
         log_ns = 1;
-    }
-    while (Ns < L) {
+    } else if (l % 3 == 2) {   // radix-4 stage, span 1 (twiddles all 1)
         const int nb = L >> 2, total = rows * nb;
         const int lnb = l - 2;
-        const int tw_shift = kMaxLog - (log_ns + 2);   // w_{4 Ns}^t = w_1024^(t << shift)
         constexpr int IT = (kRows << (kMaxLog - 2)) / kThreads;
         fp2 v[IT][4];
 #pragma unroll
@@ -109,13 +125,7 @@ __device__ void lds_dft(fp2* lds, int rows, int l, const Tw& tw)
             if (b < total) {
                 const int row = b >> lnb, j = b & (nb - 1);
                 const fp2* x = lds + row * kRowStride;
-                fp2 a0 = x[j], a1 = x[j + nb], a2 = x[j + 2 * nb], a3 = x[j + 3 * nb];
-                const int k = j & (Ns - 1);
-                if (k) {
-                    a1 = mul2(a1, ldv(tw.t1024 + (k << tw_shift)));
-                    a2 = mul2(a2, ldv(tw.t1024 + ((2 * k) << tw_shift)));
-                    a3 = mul2(a3, ldv(tw.t1024 + ((3 * k) << tw_shift)));
-                }
+                const fp2 a0 = x[j], a1 = x[j + nb], a2 = x[j + 2 * nb], a3 = x[j + 3 * nb];
                 const fp2 s02 = add2(a0, a2), d02 = sub2(a0, a2);
                 const fp2 s13 = add2(a1, a3), d13 = mul_w4(sub2(a1, a3), tw.w4_is_i);
                 v[it][0] = add2(s02, s13);
@@ -130,17 +140,77 @@ __device__ void lds_dft(fp2* lds, int rows, int l, const Tw& tw)
             const int b = threadIdx.x + it * kThreads;
             if (b < total) {
                 const int row = b >> lnb, j = b & (nb - 1);
-                const int k = j & (Ns - 1);
-                fp2* y = lds + row * kRowStride + ((j >> log_ns) << (log_ns + 2)) + k;
+                fp2* y = lds + row * kRowStride + 4 * j;
                 y[0] = v[it][0];
-                y[Ns] = v[it][1];
-                y[2 * Ns] = v[it][2];
-                y[3 * Ns] = v[it][3];
+                y[1] = v[it][1];
+                y[2] = v[it][2];
+                y[3] = v[it][3];
             }
         }
         __syncthreads();
-        Ns <<= 2;
-        log_ns += 2;
+        Ns = 4;
+        log_ns = 2;
+    }
+    while (Ns < L) {   // radix-8 stages: DFT_8 = two DFT_4 and w_8 rotations
+        const int nb = L >> 3, total = rows * nb;
+        const int lnb = l - 3;
+        const int tw_shift = kMaxLog - (log_ns + 3);   // w_{8 Ns}^t = w_1024^(t << shift)
+        constexpr int IT = (kRows << (kMaxLog - 3)) / kThreads;
+        fp2 v[IT][8];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int b = threadIdx.x + it * kThreads;
+            if (b < total) {
+                const int row = b >> lnb, j = b & (nb - 1);
+                const fp2* x = lds + row * kRowStride;
+                fp2 a[8];
+#pragma unroll
+                for (int m = 0; m < 8; ++m) a[m] = x[j + m * nb];
+                const int k = j & (Ns - 1);
+                if (k) {
+#pragma unroll
+                    for (int m = 1; m < 8; ++m)
+                        a[m] = mul2(a[m], ldv(tw.t1024 + ((m * k) << tw_shift)));
+                }
+                fp2 E[4], O[4];
+                {
+                    const fp2 s02 = add2(a[0], a[4]), d02 = sub2(a[0], a[4]);
+                    const fp2 s13 = add2(a[2], a[6]), d13 = mul_w4(sub2(a[2], a[6]), tw.w4_is_i);
+                    E[0] = add2(s02, s13);
+                    E[1] = add2(d02, d13);
+                    E[2] = sub2(s02, s13);
+                    E[3] = sub2(d02, d13);
+                }
+                {
+                    const fp2 s02 = add2(a[1], a[5]), d02 = sub2(a[1], a[5]);
+                    const fp2 s13 = add2(a[3], a[7]), d13 = mul_w4(sub2(a[3], a[7]), tw.w4_is_i);
+                    O[0] = add2(s02, s13);
+                    O[1] = mul_w8(add2(d02, d13), tw.w8_s1, tw.w8_s2);
+                    O[2] = mul_w4(sub2(s02, s13), tw.w4_is_i);
+                    O[3] = mul_w4(mul_w8(sub2(d02, d13), tw.w8_s1, tw.w8_s2), tw.w4_is_i);
+                }
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    v[it][m] = add2(E[m], O[m]);
+                    v[it][m + 4] = sub2(E[m], O[m]);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int b = threadIdx.x + it * kThreads;
+            if (b < total) {
+                const int row = b >> lnb, j = b & (nb - 1);
+                const int k = j & (Ns - 1);
+                fp2* y = lds + row * kRowStride + ((j >> log_ns) << (log_ns + 3)) + k;
+#pragma unroll
+                for (int m = 0; m < 8; ++m) y[m * Ns] = v[it][m];
+            }
+        }
+        __syncthreads();
+        Ns <<= 3;
+        log_ns += 3;
     }
 }
 
@@ -157,23 +227,45 @@ __global__ void __launch_bounds__(kThreads) k_ntt_pass_a(const V2* __restrict__ 
     const V2* src = coef + blockIdx.z * coef_stride;
     V2* dst = ybuf + blockIdx.z * y_stride;
     const int total = rows * M2;
+    const u64 nmask = ((u64)1 << (a + b + log_s)) - 1;   // exponents mod N (w^N = 1)
+    // Each thread keeps one row r (kThreads is a multiple of rows) and steps
+    // j2 (and q2) by kThreads / rows, so both twiddles advance by a constant
+    // ratio: one multiply per element instead of two table loads and a multiply.
+    const int step = kThreads >> log_rows;
     // load + coset twist w_{S M2}^(s j2) = w_N^(M1 s j2)
-    for (int idx = threadIdx.x; idx < total; idx += kThreads) {
-        const int r = idx & (rows - 1), j2 = idx >> log_rows;
-        fp2 x = ldv(src + (size_t)(j1_0 + r) + (size_t)M1 * j2);
-        if (s && j2) x = mul2(x, wpow(tw, (u64)M1 * s * j2));
-        lds[r * kRowStride + j2] = x;
+    {
+        const int r = threadIdx.x & (rows - 1);
+        fp2 cur = s ? wpow(tw, ((u64)M1 * s * (threadIdx.x >> log_rows)) & nmask) : one2();
+        const fp2 ratio = s ? wpow(tw, ((u64)M1 * s * step) & nmask) : one2();
+        for (int idx = threadIdx.x; idx < total; idx += kThreads) {
+            const int j2 = idx >> log_rows;
+            fp2 x = ldv(src + (size_t)(j1_0 + r) + (size_t)M1 * j2);
+            if (s) {
+                x = mul2(x, cur);
+                cur = mul2(cur, ratio);
+            }
+            lds[r * kRowStride + j2] = x;
+        }
     }
     __syncthreads();
     lds_dft(lds, rows, b, tw);
     // twiddle w_N^(j1 u), u = s + S q2; store Y[u][j1]
-    for (int idx = threadIdx.x; idx < total; idx += kThreads) {
-        const int r = idx & (rows - 1), q2 = idx >> log_rows;
-        const u64 u = (u64)s + (u64)S * q2;
+    {
+        const int r = threadIdx.x & (rows - 1);
         const u64 j1 = (u64)(j1_0 + r);
-        fp2 v = lds[r * kRowStride + q2];
-        if (j1 && u) v = mul2(v, wpow(tw, j1 * u));
-        stv(dst + u * M1 + j1, v);
+        const u64 u0 = (u64)s + (u64)S * (threadIdx.x >> log_rows);
+        fp2 cur = j1 ? wpow(tw, (j1 * u0) & nmask) : one2();
+        const fp2 ratio = j1 ? wpow(tw, (j1 * (u64)S * step) & nmask) : one2();
+        for (int idx = threadIdx.x; idx < total; idx += kThreads) {
+            const int q2 = idx >> log_rows;
+            const u64 u = (u64)s + (u64)S * q2;
+            fp2 v = lds[r * kRowStride + q2];
+            if (j1) {
+                v = mul2(v, cur);
+                cur = mul2(cur, ratio);
+            }
+            stv(dst + u * M1 + j1, v);
+        }
     }
 }
 
@@ -225,6 +317,7 @@ struct Plan {
     int log_order = -1;
     int h = 0;
     int w4_is_i = 0;
+    int w8_s1 = 1, w8_s2 = 1;
     DevBuf t1024, tl, th, ybuf;
 };
 Plan g_plan;
@@ -248,6 +341,13 @@ bool plan_for(int log_order, hipStream_t st)
     P_.w4_is_i = (w4.re == 0 && w4.im == 1) ? 1 : 0;
     if (!P_.w4_is_i && !(w4.re == 0 && w4.im == P - 1))
         return fail("ntt: fourth root of unity is not +-i");
+    // the generator's 8th root is 2^30 (s1 + s2 i)
+    const fp2 w8 = host_pow2k(g, 59);
+    const u64 c = 1ULL << 30, mc = P - c;
+    if (!((w8.re == c || w8.re == mc) && (w8.im == c || w8.im == mc)))
+        return fail("ntt: eighth root of unity is not 2^30 (+-1 +-i)");
+    P_.w8_s1 = w8.re == c ? 1 : -1;
+    P_.w8_s2 = w8.im == c ? 1 : -1;
     if (!P_.t1024.ensure(1024 * sizeof(V2)) || !P_.tl.ensure(((size_t)1 << h) * sizeof(V2)) ||
         !P_.th.ensure(((size_t)1 << log_hi) * sizeof(V2)))
         return false;
@@ -308,6 +408,8 @@ int tp_rs_encode(const uint64_t* d_coef, int n_cols, int log_coef, int log_order
     tw.th = P_.th.as<V2>();
     tw.h = P_.h;
     tw.w4_is_i = P_.w4_is_i;
+    tw.w8_s1 = P_.w8_s1;
+    tw.w8_s2 = P_.w8_s2;
     const int M1 = 1 << a;
     const size_t SM2 = (size_t)1 << (log_s + b);
     const int rows_a = M1 < kRows ? M1 : kRows;
