@@ -221,13 +221,28 @@ typedef struct tp_frame_job {
 } tp_frame_job;
 
 /* A pool of n_workers provers (host threads, each with its own HIP stream and
- * buffers) taking frames from one FIFO queue.  NULL on failure. */
+ * buffers) taking frames from one FIFO queue.  NULL on failure.  The
+ * reference runs frames as separate prover processes (TP/TruePix.sh --threads
+ * with TP/run_truepix_prover.py); a pool is that fan-out inside one process. */
 void* tp_pool_create(int n_workers);
+/* A pool of n_drivers lockstep batch drivers: each takes up to max_batch queued
+ * frames in arrival order whose device table bytes fit budget_bytes beside the
+ * other drivers' batches (0 = three quarters of the device memory free now),
+ * and proves them together with one launch per protocol step across the batch
+ * (per-frame descriptor and item tables), the frames' transcript work spread
+ * over host_threads threads.  Frames arriving while a batch runs form the
+ * next one.  Same job API and the same proof bytes as tp_pool_create.  No
+ * reference counterpart beyond the process fan-out named above. */
+void* tp_pool_create_batched(int n_drivers, int max_batch, uint64_t budget_bytes,
+                             int host_threads);
 void tp_pool_destroy(void* pool);
 /* Seconds since the pool was created (the clock of tp_pool_info's times). */
 double tp_pool_now(void* pool);
 /* Queues a frame (everything it reads is copied); job id > 0, or -1. */
 int64_t tp_pool_submit(void* pool, const tp_frame_job* job);
+/* Queues n frames at once and in order, all or none (a batch driver admits
+ * them together); ids[k] = the job id of jobs[k].  0 / -1. */
+int tp_pool_submit_many(void* pool, const tp_frame_job* jobs, int n, int64_t* ids);
 /* 1 done, 0 not done within timeout_s (< 0 waits forever), -1 failed / unknown. */
 int tp_pool_wait(void* pool, int64_t job, double timeout_s);
 /* Message of a failed job. */
@@ -235,12 +250,27 @@ const char* tp_pool_error(void* pool, int64_t job);
 /* Proof bytes (pickle of the transcript, fiatshamir.py:234-237): *len = size,
  * copied to buf when cap suffices.  0 / -1. */
 int tp_pool_proof(void* pool, int64_t job, uint8_t* buf, size_t cap, size_t* len);
+/* The same bytes in place (no copy), valid until tp_pool_release.  0 / -1. */
+int tp_pool_proof_view(void* pool, int64_t job, const uint8_t** data, size_t* len);
 /* Final GKR input point (nInBits + nCopyBits variables, 2 limbs each), the
  * input MLE there, times (submit, start, end) and phase times in ms
  * (eval+download, output absorb, tables, early, late, tail, input MLE, proof
  * bytes).  0 / -1. */
 int tp_pool_info(void* pool, int64_t job, uint64_t* point, int point_cap, int* n_point,
                  uint64_t value[2], double times[3], double phases[8]);
+/* Frames in the launch set that proved a finished job (1 for a stream worker)
+ * and the job's device table bytes (what admission counts).  0 / -1. */
+int tp_pool_job_stats(void* pool, int64_t job, int* batch, uint64_t* footprint);
+/* Admission state of a batched pool: bytes admitted now, the budget, the most
+ * admitted at once, batches run and the frames they held.  0 / -1. */
+int tp_pool_memory(void* pool, uint64_t* in_use, uint64_t* budget, uint64_t* peak,
+                   uint64_t* batches, uint64_t* frames);
+/* Where a batched pool's batches spent their time since it was created: batches,
+ * frames, then ms: input staging, launch queueing, transcript heads (wall),
+ * output waits (sum over frames), heads (sum), proof prefixes (sum), rounds
+ * (wall), step waits, step transcript work (wall), proof ends (wall).  Writes
+ * min(n, count) values and returns the count (12), -1 on a null pool. */
+int tp_pool_batch_profile(void* pool, double* out, int n);
 /* Circuit outputs [copies_pad][2^out_bits] of a job submitted with keep_outputs. */
 int tp_pool_outputs(void* pool, int64_t job, uint64_t* out, size_t cap, size_t* n);
 /* Forgets a finished job. */

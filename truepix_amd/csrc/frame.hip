@@ -7,9 +7,9 @@
 // evaluation, the nCopyBits early (cubic) rounds, the 2 nInBits late
 // (quadratic) rounds, the line projection, the final product proof, the
 // wiring predicate, the input MLE and the proof bytes.  Every protocol step,
-// transcript absorb and challenge draw happens in the reference's order, so
-// the proof bytes equal the reference prover's for the same inputs, VOLE
-// correlations, commitment binding and pid.
+// transcript absorb and challenge draw happens in the reference's order
+// (FrameProto below), so the proof bytes equal the reference prover's for the
+// same inputs, VOLE correlations, commitment binding and pid.
 //
 // Device residency: inputs go up once (the private bytes), outputs come down
 // once (they are absorbed into the transcript and pickled into the proof);
@@ -17,89 +17,53 @@
 // and each round's sums come back through a mapped host slot that the host
 // polls (no stream synchronization per round).
 //
-// The pool runs one host thread per worker, each with its own HIP stream,
-// mapped slot and grow-only buffers.  Frames are independent (each has its
-// own transcript), so workers take frames from one FIFO queue as they free
-// up: a batch is a queue filled at once, a served stream is a queue filled as
-// requests arrive.  Kernels of different workers overlap on the GPU; host
-// transcript work (SHA-256 of the outputs, commitments, the proof writer)
-// overlaps across CPU cores.
-#include "fp61.hpp"
-#include "host_sha.hpp"
-#include "kernels.hpp"
-#include "proof_pickle.hpp"
+// The pool has two kinds of drivers taking frames from one FIFO queue:
+//  - stream workers: one host thread per worker, each with its own HIP
+//    stream, mapped slot and grow-only buffers, proving one frame at a time
+//    (prove_frame); kernels of different workers overlap on the GPU;
+//  - lockstep batch drivers (batch.hip): each takes every queued frame that
+//    fits its batch size and the pool's device-memory budget, and proves them
+//    together, one launch per protocol step across the whole batch, with the
+//    per-frame transcript work spread over a shared pool of host threads.
+// A batch is a queue filled at once; a served stream is a queue filled as
+// requests arrive (continuous batching: frames that arrive while a batch runs
+// form the next one).
+#include "frame.hpp"
+#include "layer_bodies.hpp"
 #include "reduce.hpp"
-#include "runtime.hpp"
-#include "transcript.hpp"
 #include "../../include/truepix_hip.h"
 
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
 
-#include <atomic>
-#include <chrono>
-#include <condition_variable>
-#include <deque>
-#include <memory>
-#include <mutex>
-#include <string>
-#include <thread>
 #include <unordered_map>
-#include <vector>
 
 using namespace tp;
+using namespace tp::frame;
 
 namespace {
 
-using Clock = std::chrono::steady_clock;
-
 // ------------------------------------------------------------------ kernels
 
-// Input rows [cp][2^row_log] of the circuit's input layer: private inputs
-// where slot[w] >= 0, the constant cval[w] elsewhere, zero rows for padded
-// copies (TP/libTruePix/util.py get_inputs + layerprover.py:113-120 padding).
 __global__ void k_fe_rows(const uint8_t* __restrict__ src8, const u64* __restrict__ src64,
                           int n_priv, int n_copies, int cp, int row_log,
                           const int* __restrict__ slot, const u64* __restrict__ cval,
                           u64* __restrict__ rows)
 {
-    const size_t total = (size_t)cp << row_log;
-    const int mask = (1 << row_log) - 1;
-    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < total;
-         k += (size_t)gridDim.x * blockDim.x) {
-        const size_t c = k >> row_log;
-        const int w = (int)(k & mask);
-        u64 v = 0;
-        if (c < (size_t)n_copies) {
-            const int s = slot[w];
-            if (s >= 0) {
-                const size_t at = c * n_priv + s;
-                v = src8 ? (u64)src8[at] : fred64(src64[at]);
-            } else {
-                v = cval[w];
-            }
-        }
-        rows[k] = v;
-    }
+    body::input_rows(src8, src64, n_priv, n_copies, cp, row_log, slot, cval, rows, blockIdx.x,
+                     gridDim.x);
 }
 
-// eq table: out[k] = prod_j (bit_j(k) ? z_j : 1 - z_j) for k < n_out
-// (TP/native_fp2/native_fp2_core.hpp:139-159, beta with init 1).
 __global__ void k_fe_eq(PointArg z, int n, u64* __restrict__ outr, u64* __restrict__ outi,
                         int n_out)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_out) return;
-    fp2 b = one2();
-    for (int j = 0; j < n; ++j) b = mul2(b, ((k >> j) & 1) ? z.z[j] : sub2(one2(), z.z[j]));
-    outr[k] = b.re;
-    outi[k] = b.im;
+    __shared__ fp2 zs[kMaxZ];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) zs[j] = z.z[j];
+    __syncthreads();
+    body::eq_table(zs, n, outr, outi, n_out, blockIdx.x);
 }
 
-// Start of the late rounds (TP/libTruePix/gateprover.py:51-62): each gate's
-// fixed side is V[in1]; its coefficient starts at the z1 eq value; beta* is
-// packed (re, im) for the late-round kernel.
 __global__ void k_fe_late_init(const u64* __restrict__ vr, const u64* __restrict__ vi,
                                const int* __restrict__ in1, int n_gates,
                                const u64* __restrict__ z1cr, const u64* __restrict__ z1ci,
@@ -108,19 +72,11 @@ __global__ void k_fe_late_init(const u64* __restrict__ vr, const u64* __restrict
                                const u64* __restrict__ br, const u64* __restrict__ bi,
                                u64* __restrict__ beta2)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g == 0) {
-        beta2[0] = br[0];
-        beta2[1] = bi[0];
-    }
-    if (g >= n_gates) return;
-    othr[g] = vr[in1[g]];
-    othi[g] = vi[in1[g]];
-    z1lr[g] = z1cr[g];
-    z1li[g] = z1ci[g];
+    body::late_init(vr, vi, in1, n_gates, z1cr, z1ci, z1lr, z1li, othr, othi, br, bi, beta2,
+                    blockIdx.x);
 }
 
-// Copies n <= 16 device values to the mapped slot, then the sequence number.
+// Copies n <= 64 device values to the mapped slot, then the sequence number.
 __global__ void k_publish_copy(const fp2* __restrict__ src, int n, fp2* host_vals,
                                volatile unsigned* host_seq, unsigned seq)
 {
@@ -233,6 +189,22 @@ fp2 gate_com(int t, fp2 x, fp2 y, fp2 xy)
     }
 }
 
+// Joins its thread when it goes out of scope (error returns included).
+struct Joiner {
+    std::thread t;
+    explicit Joiner(std::thread&& th) : t(std::move(th)) {}
+    void join()
+    {
+        if (t.joinable()) t.join();
+    }
+    ~Joiner() { join(); }
+};
+
+}  // namespace
+
+namespace tp {
+namespace frame {
+
 int clog2(long long v)
 {
     int b = 0;
@@ -240,36 +212,482 @@ int clog2(long long v)
     return b;
 }
 
-// ------------------------------------------------------------ circuit, jobs
+double now_s(Clock::time_point t0)
+{
+    return std::chrono::duration<double>(Clock::now() - t0).count();
+}
 
-struct Circuit {
-    int n_gates = 0, in_bits = 0, out_bits = 0, n_inputs = 0, n_priv = 0;
-    std::vector<int> type, in0, in1;
-    DevBuf d_op, d_mux, d_type, d_in0, d_in1, d_slot, d_cval;
-};
+bool Shape::set(const Job& J)
+{
+    const Circuit& C = *J.circ;
+    nin = C.in_bits;
+    nout = C.out_bits;
+    n_gates = C.n_gates;
+    if (J.n_copies < 2) return fail("frame: nCopies must be at least 2");
+    ncb = clog2(J.n_copies);
+    cp = 1 << ncb;
+    NW = 1 << nin;
+    NO = 1 << nout;
+    n_z = nin + ncb;
+    if (ncb > 16 || n_z > kMaxZ) return fail("frame: %d copies is beyond this build's limit", J.n_copies);
+    rows_n = (size_t)cp << nin;
+    out_n = (size_t)cp << nout;
+    in_bytes = (size_t)J.n_copies * J.n_priv * (J.in_bytes == 1 ? 1 : 8);
+    return true;
+}
 
-struct Job {
-    int64_t id = 0;
-    const Circuit* circ = nullptr;
-    int n_copies = 0, n_priv = 0, in_bytes = 1;
-    std::vector<uint8_t> in8;
-    std::vector<u64> in64;
-    u64 pid = 0;
-    std::vector<u64> ur, ui, vr, vi;
-    std::vector<int> bkind;
-    std::vector<u64> bint;
-    std::vector<std::string> bstr;
-    bool keep_outputs = false;
-    // results
-    std::atomic<int> state{0};   // 0 queued, 1 running, 2 done, 3 failed
-    std::string err;
-    std::vector<uint8_t> proof;
-    std::vector<u64> point;      // (re, im) per variable: nInBits line point, then w3
-    fp2 value = {0, 0};
-    std::vector<u64> outputs;    // [cp][2^out_bits] when keep_outputs
-    double t_submit = 0, t_start = 0, t_end = 0;
-    double phase[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+bool job_fail(Job& J)
+{
+    J.err = tp_last_error();
+    if (J.err.empty()) J.err = "frame failed";
+    return false;
+}
+
+// ------------------------------------------------------------ FrameProto
+
+FrameProto::FrameProto(Job& job, const Shape& shape) : J(job), S(shape), pw(job.proof)
+{
+    memset(&point, 0, sizeof(point));
+}
+
+fp2 FrameProto::chal(const char* label) { return fs.rand_ext(label, (uint32_t)strlen(label)); }
+
+fp2 FrameProto::chal_num(const char* prefix, int k)
+{
+    char buf[64];
+    const uint32_t n = label_num(buf, prefix, k);
+    return fs.rand_ext(buf, n);
+}
+
+// put(x) of an Fp2: the entry [x]
+void FrameProto::put(fp2 x)
+{
+    fs.item_fp2(x);
+    PNode l = PNode::list();
+    l.kids.push_back(PNode::fp2v(x.re, x.im));
+    trans.push_back(std::move(l));
+}
+
+// put((a, b)): the entry [(a, b)]
+void FrameProto::put2(fp2 a, fp2 b)
+{
+    fs.item_fp2(a);
+    fs.item_fp2(b);
+    PNode t = PNode::tuple();
+    t.kids.push_back(PNode::fp2v(a.re, a.im));
+    t.kids.push_back(PNode::fp2v(b.re, b.im));
+    PNode l = PNode::list();
+    l.kids.push_back(std::move(t));
+    trans.push_back(std::move(l));
+}
+
+// commit(m): the next VOLE correlation (TP/libTruePix/ExtVOLECommit.py:180-298,
+// one security repetition); tag = V, correction d = m - U.
+bool FrameProto::commit(fp2 m, fp2& tag, fp2& d)
+{
+    if (used >= (int)J.ur.size()) return fail("VOLE instance index out of range");
+    tag = mk2(J.vr[used], J.vi[used]);
+    d = sub2(m, mk2(J.ur[used], J.ui[used]));
+    ++used;
+    return true;
+}
+
+// Commit to a round polynomial's coefficients (create_pok_temp,
+// circuitnizkvec_vole.py:112-133) and accumulate them for the final pok.
+bool FrameProto::pok_temp(const fp2* outs, int n, std::vector<fp2>& rv)
+{
+    rv.resize(n);
+    for (int k = 0; k < n; ++k) {
+        fp2 d;
+        if (!commit(outs[k], rv[k], d)) return false;
+        put(d);
+        if ((int)v_elem.size() < n) {
+            v_elem.push_back(zero2());
+            out_elem.push_back(zero2());
+        }
+    }
+    for (int k = 0; k < n; ++k) {
+        v_elem[k] = add2(v_elem[k], rv[k]);
+        out_elem[k] = add2(out_elem[k], outs[k]);
+    }
+    return true;
+}
+
+// create_eq_proof (circuitnizkvec_vole.py:220-233)
+void FrameProto::eq_proof(fp2 v2)
+{
+    const fp2 c = chal("eq");
+    put(have_prev ? mul2(sub2(prev, v2), c) : mul2(v2, c));
+}
+
+bool FrameProto::round_tail(const fp2* outs, int n, int rd, fp2& nrand)
+{
+    std::vector<fp2> rv;
+    if (!pok_temp(outs, n, rv)) return false;
+    fp2 zp1 = rv[0];
+    for (int k = 0; k < n; ++k) zp1 = add2(zp1, rv[k]);
+    eq_proof(zp1);
+    nrand = chal_num("sc-0-", rd);
+    prev = horner(rv, nrand);
+    have_prev = true;
+    return true;
+}
+
+// Proof head (circuitnizkvec_vole.py:265-363): the io list of the proof
+// refers to the outputs buffer, which must outlive write_prefix.
+void FrameProto::prepare(const u64* outputs)
+{
+    outs = outputs;
+    outs8 = nullptr;
+    begin_io(PNode::int_array(outputs, S.out_n));
+}
+
+// The same with the outputs as bytes (every output below 256): the
+// transcript absorbs them widened to u64, the proof lists the same ints.
+void FrameProto::prepare8(const uint8_t* outputs)
+{
+    outs = nullptr;
+    outs8 = outputs;
+    begin_io(PNode::int_array8(outputs, S.out_n));
+}
+
+void FrameProto::begin_io(PNode&& values)
+{
+    const u64 n_inv = (u64)1 << S.n_z;
+    io = PNode::list();
+    PNode l = PNode::list();
+    l.kids.push_back(PNode::none());
+    io.kids.push_back(std::move(l));
+    PNode m = PNode::list();
+    m.kids.push_back(PNode::str("private_input_omitted"));
+    m.kids.push_back(PNode::integer(n_inv));
+    io.kids.push_back(std::move(m));
+    io.kids.push_back(std::move(values));
+    J.proof.clear();
+    J.proof.reserve(S.out_n * 2 + (S.out_n / 1000 + 64) * 2 + (size_t)(S.out_n / 32768 + 8) * 9 +
+                    65536);
+}
+
+void FrameProto::write_prefix()
+{
+    pw.start();
+    pw.open_tuple();
+    pw.save(PNode::integer(P));
+    pw.save(io);
+}
+
+void FrameProto::head()
+{
+    const u64 n_inv = (u64)1 << S.n_z;
+    fs.init("truepix");
+    fs.int_list(&J.pid, 1);                                      // put(pid)
+    {
+        PNode l = PNode::list();
+        l.kids.push_back(PNode::integer(J.pid));
+        trans.push_back(std::move(l));
+    }
+    fs.item_none();                                              // put(muxbits=None, io)
+    fs.item_str("private_input_omitted");                        // put([..omitted, n], io)
+    fs.item_int(n_inv);
+    {                                                            // put(outvals, io)
+        const uint8_t tag = 0x01;
+        fs.raw(&tag, 1);
+        fs.le32((uint32_t)S.out_n);
+        if (outs8) {
+            u64 wide[512];   // little-endian host: the u64 array is the absorbed bytes
+            for (size_t i = 0; i < S.out_n; i += 512) {
+                const size_t m = S.out_n - i < 512 ? S.out_n - i : 512;
+                for (size_t k = 0; k < m; ++k) wide[k] = outs8[i + k];
+                fs.raw(wide, m * 8);
+            }
+        } else {
+            fs.raw(outs, S.out_n * 8);
+        }
+    }
+    {                                                            // put(commit_binding)
+        PNode l = PNode::list();
+        size_t si = 0, ii = 0;
+        for (int kind : J.bkind) {
+            if (kind == 1) {
+                const std::string& s = J.bstr[si++];
+                fs.item_str(s.data(), (uint32_t)s.size());
+                l.kids.push_back(PNode::str(s));
+            } else {
+                const u64 v = J.bint[ii++];
+                fs.item_int(v);
+                l.kids.push_back(PNode::integer(v));
+            }
+        }
+        trans.push_back(std::move(l));
+    }
+    if (J.keep_outputs) {
+        if (outs8) J.outputs.assign(outs8, outs8 + S.out_n);
+        else J.outputs.assign(outs, outs + S.out_n);
+    }
+    z1.resize(S.nout);
+    z2.resize(S.ncb);
+    for (int i = 0; i < S.nout; ++i) z1[i] = chal_num("z1-", i);
+    for (int i = 0; i < S.ncb; ++i) z2[i] = chal_num("z2-", i);
+}
+
+// Early round rd from F(0), F(1), F(-1), F(2) (layerprover.py:195-224).
+bool FrameProto::early(int rd, const fp2 f[4], fp2& r)
+{
+    fp2 c[4];
+    interp_cubic(f, c);
+    if (!round_tail(c, 4, rd, r)) return false;
+    w3.push_back(r);
+    return true;
+}
+
+// Late round k from f(0), f(1), f(-1) (layerprover.py:225-244).
+bool FrameProto::late(int k, const fp2 f[3], fp2& r)
+{
+    fp2 c[3];
+    interp_quadratic(f, c);
+    if (!round_tail(c, 3, S.ncb + k, r)) return false;
+    (k < S.nin ? w1 : w2).push_back(r);
+    return true;
+}
+
+WiringArgs FrameProto::wiring_args() const
+{
+    WiringArgs a;
+    memset(&a, 0, sizeof(a));
+    for (int j = 0; j < S.nout; ++j) a.z1[j] = z1[j];
+    for (int j = 0; j < S.nin; ++j) {
+        a.w1[j] = w1[j];
+        a.w2[j] = w2[j];
+    }
+    return a;
+}
+
+// The end of the layer: line projection H from V on the line
+// (layercompute.py:169-243), create_final_prod_pok
+// (circuitnizkvec_vole.py:175-218, called at :435), the wiring-predicate
+// equality proof (:441-449), tau (:456-460), create_pok_final (:135-147,
+// called at :485) and the input point (:496).
+bool FrameProto::finish_layer(const fp2* line_vals, const fp2* wiring13)
+{
+    const int nin = S.nin;
+    std::vector<fp2> H(nin + 1, zero2());
+    {
+        const int n = nin + 1;
+        const std::vector<u64>& L = lagrange_basis(n);
+        for (int j = 0; j < n; ++j) {
+            const fp2 y = line_vals[j];
+            for (int i = 0; i < n; ++i) H[i] = add2(H[i], smul2(L[(size_t)j * n + i], y));
+        }
+    }
+    std::vector<fp2> vvals;
+    fp2 prv[3];
+    {
+        fp2 v1, d1, v3, d3;
+        fp2 sval = zero2();
+        for (const fp2& h : H) sval = add2(sval, h);
+        const fp2 prod = mul2(H[0], sval);
+        if (!commit(H[0], v1, d1) || !commit(prod, v3, d3)) return false;
+        put2(d1, d3);
+        std::vector<fp2> rest_v;
+        fp2 rest_m = zero2(), rest_t = zero2();
+        for (size_t k = 1; k < H.size(); ++k) {
+            fp2 v, d;
+            if (!commit(H[k], v, d)) return false;
+            rest_v.push_back(v);
+            put(d);
+            rest_m = add2(rest_m, H[k]);
+            rest_t = add2(rest_t, v);
+        }
+        const fp2 c = chal("pok-vec");
+        put2(mul2(rest_m, c), mul2(rest_t, c));
+        vvals.push_back(v1);
+        vvals.insert(vvals.end(), rest_v.begin(), rest_v.end());
+        fp2 vs = zero2();
+        for (const fp2& v : vvals) vs = add2(vs, v);
+        prv[0] = v1;
+        prv[1] = vs;
+        prv[2] = v3;
+        // prod_finish((outs[0], sval, prod), (v1, vs, v3))
+        put2(mul2(prv[0], prv[1]), sub2(add2(mul2(H[0], prv[1]), mul2(sval, prv[0])), prv[2]));
+    }
+    {                                                            // wiring predicate tV (A11)
+        fp2 mlx = one2();
+        for (int j = 0; j < S.ncb; ++j) {
+            const fp2 w = w3[j], z = z2[j];
+            mlx = mul2(mlx, sub2(add2(dbl2(mul2(w, z)), one2()), add2(w, z)));
+        }
+        fp2 tv = zero2();
+        const fp2 xy = prv[2];
+        for (int t = 0; t < 13; ++t) tv = add2(tv, mul2(wiring13[t], gate_com(t, prv[0], prv[1], xy)));
+        tv = mul2(tv, mlx);
+        eq_proof(tv);
+    }
+    const fp2 tau = chal("tau-0");
+    prev = horner(vvals, tau);
+    std::vector<fp2> zline(nin);
+    for (int j = 0; j < nin; ++j) zline[j] = add2(w1[j], mul2(sub2(w2[j], w1[j]), tau));
+    {                                                            // create_pok_final
+        const fp2 c = chal("pok-final");
+        fp2 so = zero2(), sv = zero2();
+        for (const fp2& x : out_elem) so = add2(so, x);
+        for (const fp2& x : v_elem) sv = add2(sv, x);
+        put2(mul2(so, c), mul2(sv, c));
+    }
+    // final input point = line point (nInBits) then w3 (nCopyBits)
+    J.point.clear();
+    memset(&point, 0, sizeof(point));
+    PNode l = PNode::list();
+    l.kids.push_back(PNode::str("gkr_input_point"));
+    fs.item_str("gkr_input_point");
+    PNode pts = PNode::list();
+    for (int j = 0; j < S.n_z; ++j) {
+        const fp2 e = j < nin ? zline[j] : w3[j - nin];
+        point.z[j] = e;
+        J.point.push_back(e.re);
+        J.point.push_back(e.im);
+        fs.item_int(e.re);
+        fs.item_int(e.im);
+        PNode xy = PNode::list();
+        xy.kids.push_back(PNode::integer(e.re));
+        xy.kids.push_back(PNode::integer(e.im));
+        pts.kids.push_back(std::move(xy));
+    }
+    l.kids.push_back(std::move(pts));
+    trans.push_back(std::move(l));
+    return true;
+}
+
+// The input MLE value proof (create_input_value_proof,
+// circuitnizkvec_vole.py:235-242, called at :498).
+void FrameProto::input_value(fp2 v)
+{
+    J.value = v;
+    const fp2 c = chal("input-val");
+    put(mul2(prev, c));
+}
+
+// pickle((q, io, trans, ndb, rvstart, rvend, wDiv, 4), 5) after (q, io).
+void FrameProto::write_rest()
+{
+    PNode tr = PNode::list();
+    tr.kids = std::move(trans);
+    pw.save(tr);
+    for (int k = 0; k < 4; ++k) pw.save(PNode::none());
+    pw.save(PNode::integer(4));
+    pw.close_tuple();
+    pw.finish();
+}
+
+// ------------------------------------------------------------ TaskPool
+
+TaskPool::TaskPool(int n_threads)
+{
+    if (n_threads < 1) n_threads = 1;
+    for (int i = 0; i < n_threads; ++i) th.emplace_back([this] { run(); });
+}
+
+TaskPool::~TaskPool()
+{
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        stop = true;
+    }
+    cv.notify_all();
+    for (auto& t : th) t.join();
+}
+
+void TaskPool::run()
+{
+    for (;;) {
+        std::function<void()> f;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return stop || !hi.empty() || !lo.empty(); });
+            if (!hi.empty()) {
+                f = std::move(hi.front());
+                hi.pop_front();
+            } else if (!lo.empty()) {
+                f = std::move(lo.front());
+                lo.pop_front();
+            } else {
+                return;
+            }
+        }
+        f();
+    }
+}
+
+namespace {
+struct ForState {
+    std::atomic<int> next{0}, done{0};
+    int n = 0;
+    std::function<void(int)> fn;
+    std::mutex mu;
+    std::condition_variable cv;
 };
+}  // namespace
+
+void TaskPool::parallel_for(int n, const std::function<void(int)>& fn)
+{
+    if (n <= 0) return;
+    auto st = std::make_shared<ForState>();
+    st->n = n;
+    st->fn = fn;
+    auto work = [st] {
+        int k;
+        while ((k = st->next.fetch_add(1)) < st->n) {
+            st->fn(k);
+            if (st->done.fetch_add(1) + 1 == st->n) {
+                std::lock_guard<std::mutex> lk(st->mu);
+                st->cv.notify_all();
+            }
+        }
+    };
+    const int helpers = std::min(n - 1, (int)th.size());
+    if (helpers > 0) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (int i = 0; i < helpers; ++i) hi.push_back(work);
+        }
+        if (helpers == 1) cv.notify_one();
+        else cv.notify_all();
+    }
+    work();
+    std::unique_lock<std::mutex> lk(st->mu);
+    st->cv.wait(lk, [&] { return st->done.load() == n; });
+}
+
+void TaskPool::spawn(Group& g, std::function<void()> fn)
+{
+    {
+        std::lock_guard<std::mutex> lk(g.mu);
+        ++g.pending;
+    }
+    Group* gp = &g;
+    auto task = [gp, fn = std::move(fn)] {
+        fn();
+        std::lock_guard<std::mutex> lk(gp->mu);
+        if (--gp->pending == 0) gp->cv.notify_all();
+    };
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        lo.push_back(std::move(task));
+    }
+    cv.notify_one();
+}
+
+void TaskPool::wait(Group& g)
+{
+    std::unique_lock<std::mutex> lk(g.mu);
+    g.cv.wait(lk, [&] { return g.pending == 0; });
+}
+
+}  // namespace frame
+}  // namespace tp
+
+namespace {
+
+// ----------------------------------------------------------- stream worker
 
 // Per-worker device and host resources (grow-only, reused across frames).
 struct Worker {
@@ -322,85 +740,13 @@ struct Worker {
     }
 };
 
-// The transcript side of one frame: Fiat-Shamir state, VOLE commitments
-// (TP/libTruePix/ExtVOLECommit.py:180-298, one security repetition) and the
-// proof's transcript entries in order.
-struct Proto {
-    FsTranscriptT<HostSha256> fs;
-    std::vector<PNode> trans;
-    const Job* job = nullptr;
-    int used = 0;
-
-    fp2 chal(const char* label)
-    {
-        return fs.rand_ext(label, (uint32_t)strlen(label));
-    }
-    fp2 chal_num(const char* prefix, int k)
-    {
-        char buf[64];
-        const uint32_t n = label_num(buf, prefix, k);
-        return fs.rand_ext(buf, n);
-    }
-    // put(x) of an Fp2: the entry [x]
-    void put(fp2 x)
-    {
-        fs.item_fp2(x);
-        PNode l = PNode::list();
-        l.kids.push_back(PNode::fp2v(x.re, x.im));
-        trans.push_back(std::move(l));
-    }
-    // put((a, b)): the entry [(a, b)]
-    void put2(fp2 a, fp2 b)
-    {
-        fs.item_fp2(a);
-        fs.item_fp2(b);
-        PNode t = PNode::tuple();
-        t.kids.push_back(PNode::fp2v(a.re, a.im));
-        t.kids.push_back(PNode::fp2v(b.re, b.im));
-        PNode l = PNode::list();
-        l.kids.push_back(std::move(t));
-        trans.push_back(std::move(l));
-    }
-    // commit(m): the next correlation; tag = V, correction d = m - U
-    bool commit(fp2 m, fp2& tag, fp2& d)
-    {
-        if (used >= (int)job->ur.size()) return fail("VOLE instance index out of range");
-        tag = mk2(job->vr[used], job->vi[used]);
-        d = sub2(m, mk2(job->ur[used], job->ui[used]));
-        ++used;
-        return true;
-    }
-};
-
-// Joins its thread when it goes out of scope (error returns included).
-struct Joiner {
-    std::thread t;
-    explicit Joiner(std::thread&& th) : t(std::move(th)) {}
-    void join()
-    {
-        if (t.joinable()) t.join();
-    }
-    ~Joiner() { join(); }
-};
-
-double now_s(Clock::time_point t0)
-{
-    return std::chrono::duration<double>(Clock::now() - t0).count();
-}
-
-// ------------------------------------------------------------- one frame
-
+// One frame on the worker's stream.
 bool prove_frame(Worker& W, Job& J)
 {
-    using std::vector;
+    Shape S;
+    if (!S.set(J)) return false;
     const Circuit& C = *J.circ;
-    const int nin = C.in_bits, nout = C.out_bits;
-    const int ncb = clog2(J.n_copies);
-    const int cp = 1 << ncb;
-    const int NW = 1 << nin;
-    const int n_z = nin + ncb;
-    if (J.n_copies < 2) return fail("frame: nCopies must be at least 2");
-    if (ncb > 16 || n_z > kMaxZ) return fail("frame: %d copies is beyond this build's limit", J.n_copies);
+    const int nin = S.nin, nout = S.nout, ncb = S.ncb, cp = S.cp, NW = S.NW;
     hipStream_t st = W.st;
     auto t_mark = Clock::now();
     auto lap = [&](int k) {
@@ -410,9 +756,7 @@ bool prove_frame(Worker& W, Job& J)
     };
 
     // 1. inputs -> rows, evaluate the layer, outputs to the host (A3, A4)
-    const size_t in_bytes = (size_t)J.n_copies * J.n_priv * (J.in_bytes == 1 ? 1 : 8);
-    const size_t rows_n = (size_t)cp << nin;
-    const size_t out_n = (size_t)cp << nout;
+    const size_t in_bytes = S.in_bytes, rows_n = S.rows_n, out_n = S.out_n;
     if (!W.host_in(in_bytes ? in_bytes : 8) || !W.host_out(out_n * 8)) return false;
     memcpy(W.h_in, J.in_bytes == 1 ? (const void*)J.in8.data() : (const void*)J.in64.data(), in_bytes);
     if (!W.raw_in.ensure(in_bytes ? in_bytes : 8) || !W.rows_in.ensure(rows_n * 8) ||
@@ -446,75 +790,21 @@ bool prove_frame(Worker& W, Job& J)
     TP_HIP_OK(hipStreamSynchronize(st));
     lap(0);
 
-    // 2. transcript head (circuitnizkvec_vole.py:265-300).  The proof's
-    // leading items (q, io) are known now, so their bytes are written on a
-    // helper thread while this one hashes the outputs and runs the rounds.
-    const u64 n_inv = (u64)1 << n_z;
-    PNode io = PNode::list();
-    {
-        PNode l = PNode::list();
-        l.kids.push_back(PNode::none());
-        io.kids.push_back(std::move(l));
-        PNode m = PNode::list();
-        m.kids.push_back(PNode::str("private_input_omitted"));
-        m.kids.push_back(PNode::integer(n_inv));
-        io.kids.push_back(std::move(m));
-        io.kids.push_back(PNode::int_array(W.h_out, out_n));
-    }
-    J.proof.clear();
-    J.proof.reserve(out_n * 2 + (out_n / 1000 + 64) * 2 + (size_t)(out_n / 32768 + 8) * 9 + 65536);
-    PickleWriter pw(J.proof);
-    Joiner prefix(std::thread([&pw, &io] {
-        pw.start();
-        pw.open_tuple();
-        pw.save(PNode::integer(P));
-        pw.save(io);
-    }));
-
-    Proto pr;
-    pr.job = &J;
-    pr.fs.init("truepix");
-    pr.fs.int_list(&J.pid, 1);                                   // put(pid)
-    {
-        PNode l = PNode::list();
-        l.kids.push_back(PNode::integer(J.pid));
-        pr.trans.push_back(std::move(l));
-    }
-    pr.fs.item_none();                                           // put(muxbits=None, io)
-    pr.fs.item_str("private_input_omitted");                     // put([..omitted, n], io)
-    pr.fs.item_int(n_inv);
-    {                                                            // put(outvals, io)
-        const uint8_t tag = 0x01;
-        pr.fs.raw(&tag, 1);
-        pr.fs.le32((uint32_t)out_n);
-        pr.fs.raw(W.h_out, out_n * 8);
-    }
-    {                                                            // put(commit_binding)
-        PNode l = PNode::list();
-        size_t si = 0, ii = 0;
-        for (int kind : J.bkind) {
-            if (kind == 1) {
-                const std::string& s = J.bstr[si++];
-                pr.fs.item_str(s.data(), (uint32_t)s.size());
-                l.kids.push_back(PNode::str(s));
-            } else {
-                const u64 v = J.bint[ii++];
-                pr.fs.item_int(v);
-                l.kids.push_back(PNode::integer(v));
-            }
-        }
-        pr.trans.push_back(std::move(l));
-    }
-    if (J.keep_outputs) J.outputs.assign(W.h_out, W.h_out + out_n);
+    // 2. transcript head.  The proof's leading items (q, io) are known now, so
+    // their bytes are written on a helper thread while this one hashes the
+    // outputs and runs the rounds.
+    FrameProto pr(J, S);
+    pr.prepare(W.h_out);
+    Joiner prefix(std::thread([&pr] { pr.write_prefix(); }));
+    pr.head();
     lap(1);
 
-    // 3. z challenges and the early tables (A5)
+    // 3. eq tables of z2 (beta over copies) and z1 (gate coefficients) (A5)
     PointArg z1a, z2a;
     memset(&z1a, 0, sizeof(z1a));
     memset(&z2a, 0, sizeof(z2a));
-    vector<fp2> z1(nout), z2(ncb);
-    for (int i = 0; i < nout; ++i) z1a.z[i] = z1[i] = pr.chal_num("z1-", i);
-    for (int i = 0; i < ncb; ++i) z2a.z[i] = z2[i] = pr.chal_num("z2-", i);
+    for (int i = 0; i < nout; ++i) z1a.z[i] = pr.z1[i];
+    for (int i = 0; i < ncb; ++i) z2a.z[i] = pr.z2[i];
     if (!W.z1cr.ensure((size_t)C.n_gates * 8) || !W.z1ci.ensure((size_t)C.n_gates * 8) ||
         !W.z1lr.ensure((size_t)C.n_gates * 8) || !W.z1li.ensure((size_t)C.n_gates * 8) ||
         !W.othr.ensure((size_t)C.n_gates * 8) || !W.othi.ensure((size_t)C.n_gates * 8) ||
@@ -528,52 +818,15 @@ bool prove_frame(Worker& W, Job& J)
     lap(2);
 
     // 4. the rounds (layer loop of circuitnizkvec_vole.py:318-372, one layer)
-    vector<fp2> v_elem, out_elem, w1, w2, w3;
-    fp2 prev = zero2();
-    bool have_prev = false;
-    fp2* host = static_cast<fp2*>(W.slot.vals);
-    auto pok_temp = [&](const fp2* outs, int n, vector<fp2>& rv) -> bool {
-        rv.resize(n);
-        for (int k = 0; k < n; ++k) {
-            fp2 d;
-            if (!pr.commit(outs[k], rv[k], d)) return false;
-            pr.put(d);
-            if ((int)v_elem.size() < n) {
-                v_elem.push_back(zero2());
-                out_elem.push_back(zero2());
-            }
-        }
-        for (int k = 0; k < n; ++k) {
-            v_elem[k] = add2(v_elem[k], rv[k]);
-            out_elem[k] = add2(out_elem[k], outs[k]);
-        }
-        return true;
-    };
-    auto eq_proof = [&](fp2 v2) {
-        const fp2 c = pr.chal("eq");
-        pr.put(have_prev ? mul2(sub2(prev, v2), c) : mul2(v2, c));
-    };
-    auto round_tail = [&](const fp2* outs, int n, int rd, fp2& nrand) -> bool {
-        vector<fp2> rv;
-        if (!pok_temp(outs, n, rv)) return false;
-        fp2 zp1 = rv[0];
-        for (int k = 0; k < n; ++k) zp1 = add2(zp1, rv[k]);
-        eq_proof(zp1);
-        nrand = pr.chal_num("sc-0-", rd);
-        prev = horner(rv, nrand);
-        have_prev = true;
-        return true;
-    };
-
+    const fp2* host = static_cast<const fp2*>(W.slot.vals);
     int cur = 0, nc = cp;
-    fp2* parts = nullptr;
     for (int rd = 0; rd < ncb; ++rd) {                           // early rounds (A6, A7)
         const int n_pairs = nc >> 1;
         const int gpt = gates_per_thread(C.n_gates, n_pairs);
         dim3 grid(grid_for((size_t)n_pairs), (unsigned)((C.n_gates + gpt - 1) / gpt));
         const unsigned nblk = grid.x * grid.y;
         if (!W.partials.ensure((size_t)nblk * 4 * sizeof(fp2))) return false;
-        parts = W.partials.as<fp2>();
+        fp2* parts = W.partials.as<fp2>();
         kern::k_early_claim<<<grid, kBlock, 0, st>>>(
             C.n_gates, gpt, C.d_type.as<int>(), C.d_in0.as<int>(), C.d_in1.as<int>(),
             W.z1cr.as<u64>(), W.z1ci.as<u64>(), nc, W.vr[cur].as<u64>(), W.vi[cur].as<u64>(),
@@ -583,12 +836,10 @@ bool prove_frame(Worker& W, Job& J)
                                                 W.slot.dseq, seq);
         TP_HIP_OK(hipGetLastError());
         if (!W.slot.wait(st, seq, "early claim")) return false;
-        fp2 f[4], c[4];
+        fp2 f[4];
         for (int t = 0; t < 4; ++t) f[t] = host[t];
-        interp_cubic(f, c);
         fp2 r;
-        if (!round_tail(c, 4, rd, r)) return false;
-        w3.push_back(r);
+        if (!pr.early(rd, f, r)) return false;
         const int n_out = nc >> 1;
         const int log_out = clog2(n_out);
         const size_t total = ((size_t)NW + 1) * n_out;
@@ -628,170 +879,53 @@ bool prove_frame(Worker& W, Job& J)
             k > 0 ? 1 : 0, beta2, static_cast<fp2*>(W.slot.dvals), W.slot.dseq, seq);
         TP_HIP_OK(hipGetLastError());
         if (!W.slot.wait(st, seq, "late round")) return false;
-        fp2 f[3], c[3];
+        fp2 f[3];
         for (int t = 0; t < 3; ++t) f[t] = host[t];
-        interp_quadratic(f, c);
         fp2 r;
-        if (!round_tail(c, 3, ncb + k, r)) return false;
+        if (!pr.late(k, f, r)) return false;
         ch.c[k] = r;
-        (k < nin ? w1 : w2).push_back(r);
     }
     lap(4);
 
-    // 5. line projection H (A10) and the final product proof
+    // 5. line values V(w1 + t (w2 - w1)) (A10) and the wiring predicate sums
+    // (A11) in one publish: both need only z1, w1 and w2
     {
         const int K = nin + 1;
         kern::k_line_values<<<K, kOneWg, lds + (size_t)nin * sizeof(fp2), st>>>(Vr, Vi, nin, ch,
                                                                                dev_vals);
-        const unsigned seq = W.slot.advance();
-        k_publish_copy<<<1, 64, 0, st>>>(dev_vals, K, static_cast<fp2*>(W.slot.dvals), W.slot.dseq,
-                                         seq);
-        TP_HIP_OK(hipGetLastError());
-        if (!W.slot.wait(st, seq, "line values")) return false;
-    }
-    vector<fp2> H(nin + 1, zero2());
-    {
-        const int n = nin + 1;
-        const vector<u64>& L = lagrange_basis(n);
-        for (int j = 0; j < n; ++j) {
-            const fp2 y = host[j];
-            for (int i = 0; i < n; ++i) H[i] = add2(H[i], smul2(L[(size_t)j * n + i], y));
-        }
-    }
-    // launch the wiring predicate now; its result is read after the product proof
-    {
-        WiringArgs a;
-        memset(&a, 0, sizeof(a));
-        for (int j = 0; j < nout; ++j) a.z1[j] = z1[j];
-        for (int j = 0; j < nin; ++j) {
-            a.w1[j] = w1[j];
-            a.w2[j] = w2[j];
-        }
         const size_t wl = (((size_t)1 << nout) + ((size_t)2 << nin)) * sizeof(fp2);
         kern::k_wiring_mlext<<<1, kOneWg, wl, st>>>(C.d_type.as<int>(), C.d_in0.as<int>(),
-                                                    C.d_in1.as<int>(), C.n_gates, nout, nin, 0, a,
-                                                    dev_vals + 16);
-        TP_HIP_OK(hipGetLastError());
-    }
-    // create_final_prod_pok (circuitnizkvec_vole.py:175-218)
-    vector<fp2> vvals;
-    fp2 prv[3];
-    {
-        fp2 v1, d1, v3, d3;
-        fp2 sval = zero2();
-        for (const fp2& h : H) sval = add2(sval, h);
-        const fp2 prod = mul2(H[0], sval);
-        if (!pr.commit(H[0], v1, d1) || !pr.commit(prod, v3, d3)) return false;
-        pr.put2(d1, d3);
-        vector<fp2> rest_v;
-        fp2 rest_m = zero2(), rest_t = zero2();
-        for (size_t k = 1; k < H.size(); ++k) {
-            fp2 v, d;
-            if (!pr.commit(H[k], v, d)) return false;
-            rest_v.push_back(v);
-            pr.put(d);
-            rest_m = add2(rest_m, H[k]);
-            rest_t = add2(rest_t, v);
-        }
-        const fp2 c = pr.chal("pok-vec");
-        pr.put2(mul2(rest_m, c), mul2(rest_t, c));
-        vvals.push_back(v1);
-        vvals.insert(vvals.end(), rest_v.begin(), rest_v.end());
-        fp2 vs = zero2();
-        for (const fp2& v : vvals) vs = add2(vs, v);
-        prv[0] = v1;
-        prv[1] = vs;
-        prv[2] = v3;
-        // prod_finish((outs[0], sval, prod), (v1, vs, v3))
-        pr.put2(mul2(prv[0], prv[1]), sub2(add2(mul2(H[0], prv[1]), mul2(sval, prv[0])), prv[2]));
-    }
-    // wiring predicate tV (A11) and its equality proof
-    {
+                                                    C.d_in1.as<int>(), C.n_gates, nout, nin, 0,
+                                                    pr.wiring_args(), dev_vals + 16);
         const unsigned seq = W.slot.advance();
-        k_publish_copy<<<1, 64, 0, st>>>(dev_vals + 16, 13, static_cast<fp2*>(W.slot.dvals),
+        k_publish_copy<<<1, 64, 0, st>>>(dev_vals, 16 + 13, static_cast<fp2*>(W.slot.dvals),
                                          W.slot.dseq, seq);
         TP_HIP_OK(hipGetLastError());
-        if (!W.slot.wait(st, seq, "wiring predicate")) return false;
-        fp2 mlx = one2();
-        for (int j = 0; j < ncb; ++j) {
-            const fp2 w = w3[j], z = z2[j];
-            mlx = mul2(mlx, sub2(add2(dbl2(mul2(w, z)), one2()), add2(w, z)));
-        }
-        fp2 tv = zero2();
-        const fp2 xy = prv[2];
-        for (int t = 0; t < 13; ++t)
-            tv = add2(tv, mul2(host[t], gate_com(t, prv[0], prv[1], xy)));
-        tv = mul2(tv, mlx);
-        eq_proof(tv);
+        if (!W.slot.wait(st, seq, "line values and wiring predicate")) return false;
     }
-    const fp2 tau = pr.chal("tau-0");
-    prev = horner(vvals, tau);
-    vector<fp2> zline(nin);
-    for (int j = 0; j < nin; ++j) zline[j] = add2(w1[j], mul2(sub2(w2[j], w1[j]), tau));
-    {                                                            // create_pok_final
-        const fp2 c = pr.chal("pok-final");
-        fp2 so = zero2(), sv = zero2();
-        for (const fp2& x : out_elem) so = add2(so, x);
-        for (const fp2& x : v_elem) sv = add2(sv, x);
-        pr.put2(mul2(so, c), mul2(sv, c));
-    }
-    // final input point = line point (nInBits) then w3 (nCopyBits)
-    J.point.clear();
-    PointArg pz;
-    memset(&pz, 0, sizeof(pz));
-    {
-        PNode l = PNode::list();
-        l.kids.push_back(PNode::str("gkr_input_point"));
-        pr.fs.item_str("gkr_input_point");
-        PNode pts = PNode::list();
-        for (int j = 0; j < n_z; ++j) {
-            const fp2 e = j < nin ? zline[j] : w3[j - nin];
-            pz.z[j] = e;
-            J.point.push_back(e.re);
-            J.point.push_back(e.im);
-            pr.fs.item_int(e.re);
-            pr.fs.item_int(e.im);
-            PNode xy = PNode::list();
-            xy.kids.push_back(PNode::integer(e.re));
-            xy.kids.push_back(PNode::integer(e.im));
-            pts.kids.push_back(std::move(xy));
-        }
-        l.kids.push_back(std::move(pts));
-        pr.trans.push_back(std::move(l));
-    }
+    if (!pr.finish_layer(host, host + 16)) return false;
     lap(5);
 
     // 6. input MLE at the point (A12) and its value proof
     {
-        const int lb = n_z < 10 ? n_z : 10;
-        const size_t nrows = (size_t)1 << (n_z - lb);
+        const int lb = S.n_z < 10 ? S.n_z : 10;
+        const size_t nrows = (size_t)1 << (S.n_z - lb);
         const unsigned blocks = (unsigned)(nrows < 1024 ? nrows : 1024);
         if (!W.partials.ensure((size_t)blocks * sizeof(fp2) * 4)) return false;
-        kern::k_mle_base_dev<<<blocks, kBlock, 0, st>>>(W.rows_in.as<u64>(), pz, n_z,
+        kern::k_mle_base_dev<<<blocks, kBlock, 0, st>>>(W.rows_in.as<u64>(), pr.point, S.n_z,
                                                        W.partials.as<fp2>());
         const unsigned seq = W.slot.advance();
         k_publish_sum<1><<<1, kBlock, 0, st>>>(W.partials.as<fp2>(), blocks,
                                                 static_cast<fp2*>(W.slot.dvals), W.slot.dseq, seq);
         TP_HIP_OK(hipGetLastError());
         if (!W.slot.wait(st, seq, "input MLE")) return false;
-        J.value = host[0];
-    }
-    {
-        const fp2 c = pr.chal("input-val");
-        pr.put(mul2(prev, c));
+        pr.input_value(host[0]);
     }
     lap(6);
 
-    // 7. proof bytes: pickle((q, io, trans, ndb, rvstart, rvend, wDiv, 4), 5);
-    // (q, io) were written by the helper thread
+    // 7. proof bytes; (q, io) were written by the helper thread
     prefix.join();
-    PNode tr = PNode::list();
-    tr.kids = std::move(pr.trans);
-    pw.save(tr);
-    for (int k = 0; k < 4; ++k) pw.save(PNode::none());
-    pw.save(PNode::integer(4));
-    pw.close_tuple();
-    pw.finish();
+    pr.write_rest();
     lap(7);
     return true;
 }
@@ -808,6 +942,13 @@ struct Pool {
     bool stop = false;
     Clock::time_point t0 = Clock::now();
     std::string init_err;
+    // lockstep batch drivers
+    bool batched = false;
+    int max_batch = 1;
+    size_t budget = 0, in_use = 0, peak = 0;
+    uint64_t batches = 0, batched_frames = 0;
+    double prof[BP_COUNT] = {};
+    std::unique_ptr<TaskPool> tasks;
 };
 
 void worker_main(Pool* pool)
@@ -828,6 +969,7 @@ void worker_main(Pool* pool)
             pool->queue.pop_front();
         }
         J->t_start = now_s(pool->t0);
+        J->batch = 1;
         J->state.store(1);
         clear_error();
         bool done = ok && prove_frame(W, *J);
@@ -844,6 +986,83 @@ void worker_main(Pool* pool)
         pool->cv_done.notify_all();
     }
     W.release();
+}
+
+// A lockstep batch driver: admits queued frames in arrival order while the
+// batch has room and their table bytes fit the pool's budget beside the other
+// drivers' batches, then proves them together.
+void batch_main(Pool* pool)
+{
+    BatchCtx* B = device_ready() ? batch_ctx_create() : nullptr;
+    if (!B) {
+        std::lock_guard<std::mutex> lk(pool->mu);
+        pool->init_err = std::string("batch driver init failed: ") + tp_last_error();
+    }
+    for (;;) {
+        std::vector<Job*> batch;
+        size_t bytes = 0;
+        {
+            std::unique_lock<std::mutex> lk(pool->mu);
+            pool->cv_job.wait(lk, [&] {
+                if (pool->stop || pool->queue.empty()) return pool->stop;
+                const Job* h = pool->queue.front();
+                return h->footprint > pool->budget || pool->in_use + h->footprint <= pool->budget;
+            });
+            if (pool->queue.empty()) break;
+            while (!pool->queue.empty() && (int)batch.size() < pool->max_batch) {
+                Job* J = pool->queue.front();
+                if (J->footprint > pool->budget) {
+                    if (!batch.empty()) break;   // fail it when it reaches the head alone
+                    pool->queue.pop_front();
+                    char msg[160];
+                    snprintf(msg, sizeof(msg),
+                             "frame needs %zu device bytes, over the pool's budget of %zu",
+                             J->footprint, pool->budget);
+                    J->err = msg;
+                    J->t_start = J->t_end = now_s(pool->t0);
+                    J->state.store(3);
+                    pool->cv_done.notify_all();
+                    continue;
+                }
+                if (pool->in_use + bytes + J->footprint > pool->budget) break;
+                bytes += J->footprint;
+                batch.push_back(J);
+                pool->queue.pop_front();
+            }
+            pool->in_use += bytes;
+            if (pool->in_use > pool->peak) pool->peak = pool->in_use;
+            if (!batch.empty()) {
+                ++pool->batches;
+                pool->batched_frames += batch.size();
+            }
+        }
+        if (batch.empty()) continue;
+        const double t = now_s(pool->t0);
+        for (Job* J : batch) {
+            J->t_start = t;
+            J->batch = (int)batch.size();
+            J->state.store(1);
+        }
+        clear_error();
+        double prof[BP_COUNT] = {};
+        const bool dev_ok = B && prove_batch(*B, batch, *pool->tasks, prof);
+        const std::string dev_err = dev_ok ? std::string() : (B ? std::string(tp_last_error())
+                                                                : pool->init_err);
+        const double t1 = now_s(pool->t0);
+        {
+            std::lock_guard<std::mutex> lk(pool->mu);
+            pool->in_use -= bytes;
+            for (int k = 0; k < BP_COUNT; ++k) pool->prof[k] += prof[k];
+            for (Job* J : batch) {
+                J->t_end = t1;
+                if (!dev_ok && J->err.empty()) J->err = dev_err;
+                J->state.store(J->err.empty() ? 2 : 3);
+            }
+        }
+        pool->cv_done.notify_all();
+        pool->cv_job.notify_all();
+    }
+    if (B) batch_ctx_destroy(B);
 }
 
 Job* find(Pool* p, int64_t id)
@@ -931,7 +1150,41 @@ void* tp_pool_create(int n_workers)
         return nullptr;
     }
     auto* p = new Pool();
+    p->budget = SIZE_MAX;
     for (int i = 0; i < n_workers; ++i) p->threads.emplace_back(worker_main, p);
+    return p;
+}
+
+// A pool of n_drivers lockstep batch drivers sharing host_threads transcript
+// threads.  Each driver takes up to max_batch queued frames (in arrival
+// order) whose table bytes fit budget_bytes beside the other drivers'
+// batches (0: three quarters of the device memory free now) and proves them
+// with one launch per protocol step across the batch.
+void* tp_pool_create_batched(int n_drivers, int max_batch, uint64_t budget_bytes, int host_threads)
+{
+    clear_error();
+    if (!device_ready()) return nullptr;
+    if (n_drivers < 1 || n_drivers > 8 || max_batch < 1 || max_batch > 1024 || host_threads < 1 ||
+        host_threads > 256) {
+        fail("tp_pool_create_batched: drivers=%d max_batch=%d host_threads=%d out of range",
+             n_drivers, max_batch, host_threads);
+        return nullptr;
+    }
+    size_t budget = (size_t)budget_bytes;
+    if (!budget) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+            fail("tp_pool_create_batched: hipMemGetInfo failed");
+            return nullptr;
+        }
+        budget = free_b / 4 * 3;
+    }
+    auto* p = new Pool();
+    p->batched = true;
+    p->max_batch = max_batch;
+    p->budget = budget;
+    p->tasks.reset(new TaskPool(host_threads));
+    for (int i = 0; i < n_drivers; ++i) p->threads.emplace_back(batch_main, p);
     return p;
 }
 
@@ -955,23 +1208,28 @@ double tp_pool_now(void* pool)
     return p ? now_s(p->t0) : 0.0;
 }
 
-// Queues one frame; the job copies everything it reads.  Returns the job id
-// (> 0) or -1.
-int64_t tp_pool_submit(void* pool, const tp_frame_job* d)
+}  // extern "C"
+
+namespace {
+
+// A job from its description (everything it reads is copied); null with the
+// error recorded on a bad shape.
+std::unique_ptr<Job> make_job(const tp_frame_job* d)
 {
-    clear_error();
-    auto* p = static_cast<Pool*>(pool);
-    if (!p || !d || !d->circuit) return fail("tp_pool_submit: null argument"), -1;
+    if (!d || !d->circuit) return fail("tp_pool_submit: null argument"), nullptr;
     const auto* C = static_cast<const Circuit*>(d->circuit);
     if (d->n_copies < 2 || d->n_private != C->n_priv || (d->input_bytes != 1 && d->input_bytes != 8))
         return fail("tp_pool_submit: bad frame shape (copies=%d private=%d bytes=%d)", d->n_copies,
-                    d->n_private, d->input_bytes), -1;
-    if (d->n_vole < 0 || d->n_binding < 0) return fail("tp_pool_submit: bad counts"), -1;
+                    d->n_private, d->input_bytes), nullptr;
+    if (d->n_vole < 0 || d->n_binding < 0) return fail("tp_pool_submit: bad counts"), nullptr;
     auto J = std::make_unique<Job>();
     J->circ = C;
     J->n_copies = d->n_copies;
     J->n_priv = d->n_private;
     J->in_bytes = d->input_bytes;
+    Shape S;
+    if (!S.set(*J)) return nullptr;
+    J->footprint = S.footprint();
     const size_t n = (size_t)d->n_copies * d->n_private;
     if (d->input_bytes == 1) {
         const auto* s = static_cast<const uint8_t*>(d->inputs);
@@ -991,16 +1249,62 @@ int64_t tp_pool_submit(void* pool, const tp_frame_job* d)
         else J->bint.push_back(d->binding_int[k]);
     }
     J->keep_outputs = d->keep_outputs != 0;
-    Job* raw = J.get();
+    return J;
+}
+
+// Queues jobs in order under one lock (a batch driver sees them together).
+void enqueue(Pool* p, std::vector<std::unique_ptr<Job>>& js, int64_t* ids)
+{
     {
         std::lock_guard<std::mutex> lk(p->mu);
-        raw->id = p->next_id++;
-        raw->t_submit = now_s(p->t0);
-        p->jobs.emplace(raw->id, std::move(J));
-        p->queue.push_back(raw);
+        const double t = now_s(p->t0);
+        for (size_t k = 0; k < js.size(); ++k) {
+            Job* raw = js[k].get();
+            raw->id = p->next_id++;
+            raw->t_submit = t;
+            ids[k] = raw->id;
+            p->queue.push_back(raw);
+            p->jobs.emplace(raw->id, std::move(js[k]));
+        }
     }
-    p->cv_job.notify_one();
-    return raw->id;
+    if (p->batched || js.size() > 1) p->cv_job.notify_all();
+    else p->cv_job.notify_one();
+}
+
+}  // namespace
+
+extern "C" {
+
+// Queues one frame; the job copies everything it reads.  Returns the job id
+// (> 0) or -1.
+int64_t tp_pool_submit(void* pool, const tp_frame_job* d)
+{
+    clear_error();
+    auto* p = static_cast<Pool*>(pool);
+    if (!p) return fail("tp_pool_submit: null pool"), -1;
+    std::vector<std::unique_ptr<Job>> js;
+    js.push_back(make_job(d));
+    if (!js[0]) return -1;
+    int64_t id = -1;
+    enqueue(p, js, &id);
+    return id;
+}
+
+// Queues n frames at once, in order (all or none): ids[k] receives the job id
+// of jobs[k].  0 / -1.
+int tp_pool_submit_many(void* pool, const tp_frame_job* jobs, int n, int64_t* ids)
+{
+    clear_error();
+    auto* p = static_cast<Pool*>(pool);
+    if (!p || n < 0 || (n && (!jobs || !ids))) return fail("tp_pool_submit_many: bad arguments"), -1;
+    std::vector<std::unique_ptr<Job>> js;
+    js.reserve(n);
+    for (int k = 0; k < n; ++k) {
+        js.push_back(make_job(&jobs[k]));
+        if (!js.back()) return -1;
+    }
+    enqueue(p, js, ids);
+    return 0;
 }
 
 // 1 = done, 0 = still running after timeout_s (< 0: wait forever), -1 = failed
@@ -1042,6 +1346,20 @@ int tp_pool_proof(void* pool, int64_t job, uint8_t* buf, size_t cap, size_t* len
     return 0;
 }
 
+// Proof bytes of a finished job in place: valid until the job is released.
+// 0 / -1.
+int tp_pool_proof_view(void* pool, int64_t job, const uint8_t** data, size_t* len)
+{
+    auto* p = static_cast<Pool*>(pool);
+    if (!p || !data || !len) return -1;
+    std::lock_guard<std::mutex> lk(p->mu);
+    Job* J = find(p, job);
+    if (!J || J->state.load() != 2) return -1;
+    *data = J->proof.data();
+    *len = J->proof.size();
+    return 0;
+}
+
 // Final input point (2 limbs per variable, n_point variables written), input
 // MLE value, times (submit, start, end in pool seconds) and phase times (ms).
 int tp_pool_info(void* pool, int64_t job, uint64_t* point, int point_cap, int* n_point,
@@ -1071,6 +1389,48 @@ int tp_pool_info(void* pool, int64_t job, uint64_t* point, int point_cap, int* n
     if (phases)
         for (int k = 0; k < 8; ++k) phases[k] = J->phase[k];
     return 0;
+}
+
+// Size of the launch set that proved a finished job (1 for a stream worker)
+// and its table bytes.  0 / -1.
+int tp_pool_job_stats(void* pool, int64_t job, int* batch, uint64_t* footprint)
+{
+    auto* p = static_cast<Pool*>(pool);
+    if (!p) return -1;
+    std::lock_guard<std::mutex> lk(p->mu);
+    Job* J = find(p, job);
+    if (!J || J->state.load() < 2) return -1;
+    if (batch) *batch = J->batch;
+    if (footprint) *footprint = J->footprint;
+    return 0;
+}
+
+// Device-memory admission of a batched pool: bytes admitted now, the budget,
+// the most admitted at once; batches run and frames they held.  0 / -1.
+int tp_pool_memory(void* pool, uint64_t* in_use, uint64_t* budget, uint64_t* peak,
+                   uint64_t* batches, uint64_t* frames)
+{
+    auto* p = static_cast<Pool*>(pool);
+    if (!p) return -1;
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (in_use) *in_use = p->in_use;
+    if (budget) *budget = p->budget;
+    if (peak) *peak = p->peak;
+    if (batches) *batches = p->batches;
+    if (frames) *frames = p->batched_frames;
+    return 0;
+}
+
+// Where a batched pool's batches spent their time, accumulated since the
+// pool was created (batches, frames, then milliseconds in the order of
+// tp::frame::BatchProf); writes min(n, count) values, returns count.
+int tp_pool_batch_profile(void* pool, double* out, int n)
+{
+    auto* p = static_cast<Pool*>(pool);
+    if (!p) return -1;
+    std::lock_guard<std::mutex> lk(p->mu);
+    for (int k = 0; k < n && k < BP_COUNT; ++k) out[k] = p->prof[k];
+    return BP_COUNT;
 }
 
 // Circuit outputs [cp][2^out_bits] of a finished job submitted with

@@ -15,6 +15,7 @@
 // Fp2 tables are SoA (re[], im[]) like the early-round session.
 #include "fp61.hpp"
 #include "kernels.hpp"
+#include "layer_bodies.hpp"
 #include "reduce.hpp"
 #include "runtime.hpp"
 
@@ -26,132 +27,31 @@
 
 using namespace tp;
 
-namespace {
-
-// Base-field gate evaluation (ArithCircuit semantics: a MUX gate yields in0
-// when its mux bit is set and in1 otherwise, as CMuxGate.run does).
-__device__ __forceinline__ u64 eval_gate(int op, u64 x, u64 y, int muxbit)
-{
-    switch (op) {
-    case G_MUL: return fmul(x, y);
-    case G_ADD: return fadd(x, y);
-    case G_SUB: return fsub(x, y);
-    case G_MUXL: return muxbit ? x : y;
-    case G_OR: return fsub(fadd(x, y), fmul(x, y));
-    case G_XOR: { const u64 xy = fmul(x, y); return fsub(fadd(x, y), fadd(xy, xy)); }
-    case G_NOT: return fsub(1, x);
-    case G_NAND: return fsub(1, fmul(x, y));
-    case G_NOR: return fsub(fadd(1, fmul(x, y)), fadd(x, y));
-    case G_NXOR: { const u64 xy = fmul(x, y); return fsub(fadd(1, fadd(xy, xy)), fadd(x, y)); }
-    case G_NAAB: return fmul(fsub(1, x), y);
-    default: return x;   // PASS
-    }
-}
-
-__device__ __forceinline__ fp2 ld2(const u64* r, const u64* i, size_t k) { return mk2(r[k], i[k]); }
-
-// Loads the 2^nb table into T and folds it with cs[0..nf), ping-ponging
-// between T and U; returns the buffer holding the result.
-__device__ fp2* lds_fold_chain(fp2* T, fp2* U, const u64* vr, const u64* vi, int nb,
-                               const fp2* cs, int nf)
-{
-    const int W = 1 << nb;
-    for (int w = threadIdx.x; w < W; w += blockDim.x) T[w] = ld2(vr, vi, w);
-    __syncthreads();
-    fp2* src = T;
-    fp2* dst = U;
-    for (int j = 0; j < nf; ++j) {
-        const fp2 r = cs[j];
-        const int n_out = W >> (j + 1);
-        for (int i = threadIdx.x; i < n_out; i += blockDim.x)
-            dst[i] = fold2(src[2 * i], src[2 * i + 1], r);
-        __syncthreads();
-        fp2* t = src;
-        src = dst;
-        dst = t;
-    }
-    return src;
-}
-
-// eq table in LDS by doubling: tab[t] = prod_j (bit_j(t) ? z_j : 1 - z_j).
-__device__ void lds_eq(fp2* tab, const fp2* z, int nb)
-{
-    if (threadIdx.x == 0) tab[0] = one2();
-    __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        const int cur = 1 << j;
-        const fp2 zz = z[j];
-        const fp2 omz = sub2(one2(), zz);
-        for (int t = threadIdx.x; t < cur; t += blockDim.x) {
-            const fp2 b = tab[t];
-            tab[t + cur] = mul2(b, zz);
-            tab[t] = mul2(b, omz);
-        }
-        __syncthreads();
-    }
-}
-
-}  // namespace
-
 namespace tp {
 namespace kern {
 
-// A3: out[c][g] = gate_g(in[c][in0_g], in[c][in1_g]) for c < n_copies and
-// g < n_gates; zero for g >= n_gates and for padded copies < n_copies_pad.
+// A3: one layer over copies (body::eval_layer).
 __global__ void k_eval_layer(const u64* __restrict__ in, int in_stride, u64* __restrict__ out,
                              int out_log, int n_copies, int n_copies_pad,
                              const int* __restrict__ op, const int* __restrict__ in0,
                              const int* __restrict__ in1, const int* __restrict__ mux, int n_gates)
 {
-    const size_t total = (size_t)n_copies_pad << out_log;
-    const int out_stride = 1 << out_log;
-    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < total;
-         k += (size_t)gridDim.x * blockDim.x) {
-        const size_t c = k >> out_log;
-        const int g = (int)(k & (out_stride - 1));
-        u64 v = 0;
-        if (g < n_gates && c < (size_t)n_copies) {
-            const u64* row = in + c * in_stride;
-            v = eval_gate(op[g], row[in0[g]], row[in1[g]], mux[g]);
-        }
-        out[k] = v;
-    }
+    body::eval_layer(in, in_stride, out, out_log, n_copies, n_copies_pad, op, in0, in1, mux,
+                     n_gates, blockIdx.x, gridDim.x);
 }
 
-// A4: Fp2 SoA wire table [wire][copy] from base-field rows [copy][row_stride]
-// through a 64 x 64 LDS tile (reads coalesced along wires, writes along copies).
+// A4: [wire][copy] table, one 64 x 64 tile per workgroup (body::wire_table).
 __global__ void k_wire_table(const u64* __restrict__ rows, int row_stride, int n_copies,
                              int n_copies_pad, int n_wires, u64* __restrict__ tre,
                              u64* __restrict__ tim)
 {
-    __shared__ u64 tile[64][65];
-    const int c0 = blockIdx.x * 64, w0 = blockIdx.y * 64;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // 64 x 4 threads
-    for (int i = ty; i < 64; i += 4) {
-        const int c = c0 + i, w = w0 + tx;
-        tile[i][tx] = (c < n_copies && w < n_wires) ? rows[(size_t)c * row_stride + w] : 0;
-    }
-    __syncthreads();
-    for (int i = ty; i < 64; i += 4) {
-        const int w = w0 + i, c = c0 + tx;
-        if (w < n_wires && c < n_copies_pad) {
-            tre[(size_t)w * n_copies_pad + c] = tile[tx][i];
-            tim[(size_t)w * n_copies_pad + c] = 0;
-        }
-    }
+    body::wire_table(rows, row_stride, n_copies, n_copies_pad, n_wires, tre, tim, blockIdx.x,
+                     blockIdx.y);
 }
 
 // ---------------------------------------------------------------- late rounds
-// Round rd < nin binds bit rd of in0 (w1), round nin + k binds bit k of in1
-// (w2).  For a gate with e = the bound bit of its input w and T the table
-// folded with this pass's earlier challenges (kk of them):
-//   t = e  : z1 * gate(cur, other)          cur = T[w >> kk]
-//   t = -1 : z1 * (e ? -1 : 2) * gate(m1, other),  m1 = 2 T[2j] - T[2j+1],
-//            j = w >> (kk + 1)
-// (argument order swapped in the w2 pass), summed over gates and multiplied
-// by beta* (TP/libTruePix/gateprover.py:102-132, layerprover.py:225-244).
-// Before that, the previous round's challenge updates each gate's
-// coefficient: z1 *= (bit ? r : 1 - r) (gateprover.py:135-176).
+// One workgroup per round (body::late_round); the challenges are staged from
+// the kernel argument into LDS, the result goes to mapped host memory.
 __global__ void __launch_bounds__(kOneWg) k_late_round(
     const int* __restrict__ gt, const int* __restrict__ in0, const int* __restrict__ in1,
     u64* __restrict__ z1r, u64* __restrict__ z1i, const u64* __restrict__ oth_r,
@@ -160,53 +60,20 @@ __global__ void __launch_bounds__(kOneWg) k_late_round(
     fp2* host_vals, volatile unsigned* host_seq, unsigned seq)
 {
     extern __shared__ fp2 lds[];
-    fp2* T = lds;
-    fp2* U = lds + (1 << nin);
-    if (apply) {
-        const int k = rd - 1;
-        const fp2 r = chal.c[k];
-        const fp2 omr = sub2(one2(), r);
-        const bool w1pass = k < nin;
-        const int bit = w1pass ? k : k - nin;
-        for (int g = threadIdx.x; g < n_gates; g += blockDim.x) {
-            const int w = w1pass ? in0[g] : in1[g];
-            const fp2 nz = mul2(mk2(z1r[g], z1i[g]), ((w >> bit) & 1) ? r : omr);
-            z1r[g] = nz.re;
-            z1i[g] = nz.im;
-        }
-    }
-    const bool w1pass = rd < nin;
-    const int kk = w1pass ? rd : rd - nin;
-    const fp2* Tk = lds_fold_chain(T, U, vr, vi, nin, chal.c + (w1pass ? 0 : nin), kk);
-    fp2 acc[3] = {zero2(), zero2(), zero2()};
-    const fp2 m_one = mk2(P - 1, 0), two = mk2(2, 0);
-    for (int g = threadIdx.x; g < n_gates; g += blockDim.x) {
-        const int w = w1pass ? in0[g] : in1[g];
-        const int e = (w >> kk) & 1;
-        const fp2 z = mk2(z1r[g], z1i[g]);
-        const fp2 other = mk2(oth_r[g], oth_i[g]);
-        const fp2 cur = Tk[w >> kk];
-        const int j = w >> (kk + 1);
-        const fp2 m1 = sub2(dbl2(Tk[2 * j]), Tk[2 * j + 1]);
-        const int t = gt[g];
-        const fp2 v01 = mul2(z, w1pass ? gatefn(t, cur, other) : gatefn(t, other, cur));
-        const fp2 vm1 = mul2(mul2(z, e ? m_one : two),
-                             w1pass ? gatefn(t, m1, other) : gatefn(t, other, m1));
-        if (e) acc[1] = add2(acc[1], v01);
-        else acc[0] = add2(acc[0], v01);
-        acc[2] = add2(acc[2], vm1);
-    }
-    block_sum<3>(acc);
+    __shared__ fp2 cs[2 * kMaxNin];
+    for (int j = threadIdx.x; j < 2 * nin; j += blockDim.x) cs[j] = chal.c[j];
+    __syncthreads();
+    fp2 out[3];
+    body::late_round(gt, in0, in1, z1r, z1i, oth_r, oth_i, vr, vi, cs, nin, n_gates, rd, apply,
+                     beta, lds, out);
     if (threadIdx.x == 0) {
-        const fp2 b = mk2(beta[0], beta[1]);
-        for (int t = 0; t < 3; ++t) host_vals[t] = mul2(acc[t], b);
+        for (int t = 0; t < 3; ++t) host_vals[t] = out[t];
         __threadfence_system();
         *host_seq = seq;
     }
 }
 
-// Start of the w2 pass: every gate's fixed side becomes V(w1) (the table
-// folded with all w1 challenges; gateprover.py:150-158).
+// Start of the w2 pass (body::late_switch).
 __global__ void __launch_bounds__(kOneWg) k_late_switch(const u64* __restrict__ vr,
                                                         const u64* __restrict__ vi, int nin,
                                                         ChalSet chal, int n_gates,
@@ -214,12 +81,10 @@ __global__ void __launch_bounds__(kOneWg) k_late_switch(const u64* __restrict__ 
                                                         u64* __restrict__ oth_i)
 {
     extern __shared__ fp2 lds[];
-    const fp2* R = lds_fold_chain(lds, lds + (1 << nin), vr, vi, nin, chal.c, nin);
-    const fp2 vw1 = R[0];
-    for (int g = threadIdx.x; g < n_gates; g += blockDim.x) {
-        oth_r[g] = vw1.re;
-        oth_i[g] = vw1.im;
-    }
+    __shared__ fp2 cs[kMaxNin];
+    for (int j = threadIdx.x; j < nin; j += blockDim.x) cs[j] = chal.c[j];
+    __syncthreads();
+    body::late_switch(vr, vi, nin, cs, n_gates, oth_r, oth_i, lds);
 }
 
 // A10: MLE of a 2^nb Fp2 table at K points (one workgroup per point).
@@ -235,86 +100,46 @@ __global__ void __launch_bounds__(kOneWg) k_mle_points(const u64* __restrict__ v
 }
 
 // A10 on the line of the late rounds: workgroup t evaluates V at
-// w1 + t (w2 - w1) (TP/libTruePix/layercompute.py:169-243); the point is
-// formed in LDS past the two fold buffers; dynamic LDS fold_lds(nb) + nb Fp2.
+// w1 + t (w2 - w1) (body::line_value); dynamic LDS fold_lds(nb) + nb Fp2.
 __global__ void __launch_bounds__(kOneWg) k_line_values(const u64* __restrict__ vr,
                                                         const u64* __restrict__ vi, int nb,
                                                         ChalSet w12, fp2* __restrict__ out)
 {
     extern __shared__ fp2 lds[];
-    fp2* pt = lds + ((3 << nb) >> 1);   // past the fold buffers (fold_lds)
-    const fp2 t = mk2(blockIdx.x, 0);
-    for (int j = threadIdx.x; j < nb; j += blockDim.x)
-        pt[j] = add2(w12.c[j], mul2(t, sub2(w12.c[nb + j], w12.c[j])));
+    __shared__ fp2 cs[2 * kMaxNin];
+    for (int j = threadIdx.x; j < 2 * nb; j += blockDim.x) cs[j] = w12.c[j];
     __syncthreads();
-    const fp2* R = lds_fold_chain(lds, lds + (1 << nb), vr, vi, nb, pt, nb);
-    if (threadIdx.x == 0) out[blockIdx.x] = R[0];
+    const fp2 v = body::line_value(vr, vi, nb, cs, (int)blockIdx.x, lds);
+    if (threadIdx.x == 0) out[blockIdx.x] = v;
 }
 
-// A11: per gate type, sum_g beta_z1[g] * beta_w1[in0_g] * beta_w2[in1_g]
-// (with use_z1_2: mul0 beta_z1[g] + mul1 beta_z1_2[g] in place of
-// beta_z1[g]); eq tables built in LDS by doubling; one workgroup.
+// A11: 13 per-type sums of the wiring predicate (body::wiring_mlext), one
+// workgroup; the arguments are staged into LDS past the eq tables.
 __global__ void __launch_bounds__(kOneWg) k_wiring_mlext(
     const int* __restrict__ type_idx, const int* __restrict__ in0, const int* __restrict__ in1,
     int n_gates, int n_out_bits, int n_in_bits, int use_z1_2, WiringArgs a, fp2* __restrict__ out13)
 {
     extern __shared__ fp2 sm[];
-    const int WO = 1 << n_out_bits, WI = 1 << n_in_bits;
-    fp2* bz = sm;
-    fp2* bz2 = bz + WO;
-    fp2* b1 = bz2 + (use_z1_2 ? WO : 0);
-    fp2* b2 = b1 + WI;
-    lds_eq(bz, a.z1, n_out_bits);
-    if (use_z1_2) lds_eq(bz2, a.z1b, n_out_bits);
-    lds_eq(b1, a.w1, n_in_bits);
-    lds_eq(b2, a.w2, n_in_bits);
-    fp2 acc[13];
-#pragma unroll
-    for (int t = 0; t < 13; ++t) acc[t] = zero2();
-    for (int g = threadIdx.x; g < n_gates; g += blockDim.x) {
-        fp2 tv = use_z1_2 ? add2(mul2(a.m0, bz[g]), mul2(a.m1, bz2[g])) : bz[g];
-        tv = mul2(tv, mul2(b1[in0[g]], b2[in1[g]]));
-        const int t = type_idx[g];
-#pragma unroll
-        for (int q = 0; q < 13; ++q)
-            if (q == t) acc[q] = add2(acc[q], tv);
+    __shared__ fp2 args[4 * kMaxNin];
+    for (int j = threadIdx.x; j < kMaxNin; j += blockDim.x) {
+        args[j] = a.z1[j];
+        args[kMaxNin + j] = a.z1b[j];
+        args[2 * kMaxNin + j] = a.w1[j];
+        args[3 * kMaxNin + j] = a.w2[j];
     }
-    block_sum<13>(acc);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int t = 0; t < 13; ++t) out13[t] = acc[t];
-    }
+    __syncthreads();
+    body::wiring_mlext(type_idx, in0, in1, n_gates, n_out_bits, n_in_bits, args,
+                       use_z1_2 ? args + kMaxNin : nullptr, args + 2 * kMaxNin,
+                       args + 3 * kMaxNin, a.m0, a.m1, sm, out13);
 }
 
-// A12 on a device-resident base-field vector of 2^n_z entries: one partial
-// per workgroup over rows of 2^lb inputs sharing an LDS low-half eq table.
+// A12 on a device-resident base-field vector of 2^n_z entries (body::mle_base).
 __global__ void k_mle_base_dev(const u64* __restrict__ in, PointArg z, int n_z, fp2* partials)
 {
-    __shared__ fp2 lo[1024];
-    __shared__ fp2 hrow;
-    const int lb = n_z < 10 ? n_z : 10;
-    const int row_len = 1 << lb;
-    const size_t n_rows = (size_t)1 << (n_z - lb);
-    lds_eq(lo, z.z, lb);
-    fp2 acc[1] = {zero2()};
-    for (size_t row = blockIdx.x; row < n_rows; row += gridDim.x) {
-        if (threadIdx.x == 0) {
-            fp2 h = one2();
-            for (int j = lb; j < n_z; ++j) {
-                const fp2 zz = z.z[j];
-                h = mul2(h, ((row >> (j - lb)) & 1) ? zz : sub2(one2(), zz));
-            }
-            hrow = h;
-        }
-        __syncthreads();
-        fp2 dot = zero2();
-        const u64* src = in + row * row_len;
-        for (int t = threadIdx.x; t < row_len; t += blockDim.x)
-            dot = add2(dot, smul2(fred64(src[t]), lo[t]));
-        acc[0] = add2(acc[0], mul2(dot, hrow));
-        __syncthreads();
-    }
-    block_partial<1>(acc, partials);
+    __shared__ fp2 zs[kMaxZ];
+    for (int j = threadIdx.x; j < n_z; j += blockDim.x) zs[j] = z.z[j];
+    __syncthreads();
+    body::mle_base(in, zs, n_z, partials, blockIdx.x, gridDim.x, blockIdx.x);
 }
 
 }  // namespace kern

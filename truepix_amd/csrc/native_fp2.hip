@@ -13,6 +13,7 @@
 //    launch instead of n_z doubling launches.
 #include "fp61.hpp"
 #include "kernels.hpp"
+#include "layer_bodies.hpp"
 #include "reduce.hpp"
 #include "runtime.hpp"
 
@@ -35,32 +36,8 @@ __global__ void k_early_claim(int n_gates, int gpt, const int* __restrict__ gt,
                               const u64* __restrict__ br, const u64* __restrict__ bi,
                               fp2* partials)
 {
-    const int n_pairs = nc >> 1;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const int g0 = blockIdx.y * gpt;
-    const int g1 = min(n_gates, g0 + gpt);
-    fp2 acc[4] = {zero2(), zero2(), zero2(), zero2()};
-    if (p < n_pairs) {
-        const size_t c = 2 * (size_t)p;
-        fp2 v0 = zero2(), v1 = zero2(), v2 = zero2(), v3 = zero2();
-        for (int g = g0; g < g1; ++g) {
-            const int tpg = gt[g];
-            const size_t a = (size_t)in0[g] * nc + c, b = (size_t)in1[g] * nc + c;
-            const fp2 z = mk2(z1r[g], z1i[g]);
-            const fp2 a0 = mk2(vr[a], vi[a]), a1 = mk2(vr[a + 1], vi[a + 1]);
-            const fp2 b0 = mk2(vr[b], vi[b]), b1 = mk2(vr[b + 1], vi[b + 1]);
-            v0 = add2(v0, mul2(gatefn(tpg, a0, b0), z));
-            v1 = add2(v1, mul2(gatefn(tpg, a1, b1), z));
-            v2 = add2(v2, mul2(gatefn(tpg, sub2(dbl2(a0), a1), sub2(dbl2(b0), b1)), z));
-            v3 = add2(v3, mul2(gatefn(tpg, sub2(dbl2(a1), a0), sub2(dbl2(b1), b0)), z));
-        }
-        const fp2 be0 = mk2(br[c], bi[c]), be1 = mk2(br[c + 1], bi[c + 1]);
-        acc[0] = mul2(v0, be0);
-        acc[1] = mul2(v1, be1);
-        acc[2] = mul2(v2, sub2(dbl2(be0), be1));
-        acc[3] = mul2(v3, sub2(dbl2(be1), be0));
-    }
-    block_partial<4>(acc, partials);
+    body::early_claim(n_gates, gpt, gt, in0, in1, z1r, z1i, nc, vr, vi, br, bi, partials,
+                      blockIdx.x, blockIdx.y, linear_block_id());
 }
 
 // Fold every wire row [n_wires][n_in] -> [n_wires][n_in/2] and beta in one
@@ -71,23 +48,8 @@ __global__ void k_fold_rows(const u64* __restrict__ ir, const u64* __restrict__ 
                             const u64* __restrict__ bii, u64* __restrict__ bor,
                             u64* __restrict__ boi)
 {
-    const fp2 r = mk2(rr, ri);
-    const size_t n_out = (size_t)1 << log_out;
-    const size_t total = ((size_t)n_wires + 1) << log_out;
-    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < total;
-         k += (size_t)gridDim.x * blockDim.x) {
-        const size_t row = k >> log_out, i = k & (n_out - 1);
-        if (row < (size_t)n_wires) {
-            const size_t src = row * 2 * n_out + 2 * i;
-            const fp2 v = fold2(mk2(ir[src], ii[src]), mk2(ir[src + 1], ii[src + 1]), r);
-            orr[k] = v.re;
-            oi[k] = v.im;
-        } else {
-            const fp2 v = fold2(mk2(bir[2 * i], bii[2 * i]), mk2(bir[2 * i + 1], bii[2 * i + 1]), r);
-            bor[i] = v.re;
-            boi[i] = v.im;
-        }
-    }
+    body::fold_rows(ir, ii, n_wires, log_out, mk2(rr, ri), orr, oi, bir, bii, bor, boi, blockIdx.x,
+                    gridDim.x);
 }
 
 }  // namespace kern

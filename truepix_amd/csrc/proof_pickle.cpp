@@ -97,6 +97,7 @@ void PickleWriter::save(const PNode& x)
     case PNode::LIST: save_list(x); return;
     case PNode::TUPLE: save_tuple(x); return;
     case PNode::INT_ARRAY: save_int_array(x.arr, x.n); return;
+    case PNode::INT_ARRAY8: save_int_array(x.arr8, x.n); return;
     }
 }
 
@@ -221,8 +222,10 @@ void PickleWriter::save_list(const PNode& x)
     } while (i < n);
 }
 
-// A list of ints from an array, batched and framed exactly like save_list.
-void PickleWriter::save_int_array(const uint64_t* v, size_t n)
+// A list of ints from an array (u64, or u8 for values below 256), batched
+// and framed exactly like save_list.
+template <class T>
+void PickleWriter::save_int_array(const T* v, size_t n)
 {
     put1(OP_EMPTY_LIST);
     memoize();
@@ -238,7 +241,8 @@ void PickleWriter::save_int_array(const uint64_t* v, size_t n)
         put1(OP_MARK);
         const size_t end = i + kBatch < n ? i + kBatch : n;
         uint64_t big = 0;
-        for (size_t k = i; k < end; ++k) big |= v[k];
+        if (sizeof(T) > 1)
+            for (size_t k = i; k < end; ++k) big |= v[k];
         const size_t worst = big <= 0xff ? 2 * (end - i) : [&] {
             size_t w = 0;
             for (size_t k = i; k < end; ++k) w += int_size(v[k]);
@@ -298,10 +302,10 @@ void pickle_dumps(const PNode& root, std::vector<uint8_t>& out)
 namespace {
 
 // Token form of a tree: 'i' u64 | 's' u32 len bytes | 'n' | 'f' u64 u64 |
-// 'l' u32 count items | 't' u32 count items | 'a' u32 count u64 x count
-// (little-endian).
+// 'l' u32 count items | 't' u32 count items | 'a' u32 count u64 x count |
+// 'b' u32 count u8 x count (a list of ints below 256) (little-endian).
 bool decode(const uint8_t*& p, const uint8_t* end, PNode& x, int depth,
-            std::vector<std::vector<uint64_t>>& store)
+            std::vector<std::vector<uint64_t>>& store, std::vector<std::vector<uint8_t>>& store8)
 {
     auto need = [&](size_t k) { return (size_t)(end - p) >= k; };
     auto rd32 = [&]() { uint32_t v; memcpy(&v, p, 4); p += 4; return v; };
@@ -338,6 +342,15 @@ bool decode(const uint8_t*& p, const uint8_t* end, PNode& x, int depth,
         p += (size_t)n * 8;
         return true;
     }
+    case 'b': {
+        if (!need(4)) return false;
+        const uint32_t n = rd32();
+        if (!need(n)) return false;
+        store8.emplace_back(p, p + n);
+        x = PNode::int_array8(store8.back().data(), n);
+        p += n;
+        return true;
+    }
     case 'l':
     case 't': {
         if (!need(4)) return false;
@@ -345,7 +358,7 @@ bool decode(const uint8_t*& p, const uint8_t* end, PNode& x, int depth,
         x = t == 'l' ? PNode::list() : PNode::tuple();
         x.kids.resize(n);
         for (uint32_t k = 0; k < n; ++k)
-            if (!decode(p, end, x.kids[k], depth + 1, store)) return false;
+            if (!decode(p, end, x.kids[k], depth + 1, store, store8)) return false;
         return true;
     }
     default: return false;
@@ -365,7 +378,8 @@ int tp_pickle_tree(const uint8_t* tokens, size_t n, uint8_t** out, size_t* out_l
     const uint8_t* p = tokens;
     tp::PNode root;
     std::vector<std::vector<uint64_t>> store;
-    if (!out || !out_len || !tp::decode(p, tokens + n, root, 0, store) || p != tokens + n)
+    std::vector<std::vector<uint8_t>> store8;
+    if (!out || !out_len || !tp::decode(p, tokens + n, root, 0, store, store8) || p != tokens + n)
         return -1;
     std::vector<uint8_t> bytes;
     tp::pickle_dumps(root, bytes);

@@ -24,12 +24,13 @@
 namespace tp {
 
 struct PNode {
-    enum Kind : uint8_t { INT, STR, NONE, FP2, LIST, TUPLE, INT_ARRAY };
+    enum Kind : uint8_t { INT, STR, NONE, FP2, LIST, TUPLE, INT_ARRAY, INT_ARRAY8 };
     Kind kind = NONE;
     uint64_t a = 0, b = 0;          // INT value; FP2 (real, imag)
     std::string s;                  // STR (utf-8)
     std::vector<PNode> kids;        // LIST / TUPLE items
     const uint64_t* arr = nullptr;  // INT_ARRAY: a list of ints held elsewhere
+    const uint8_t* arr8 = nullptr;  // INT_ARRAY8: the same for ints below 256
     size_t n = 0;
 
     static PNode integer(uint64_t v) { PNode p; p.kind = INT; p.a = v; return p; }
@@ -43,6 +44,14 @@ struct PNode {
         PNode p;
         p.kind = INT_ARRAY;
         p.arr = v;
+        p.n = n;
+        return p;
+    }
+    static PNode int_array8(const uint8_t* v, size_t n)
+    {
+        PNode p;
+        p.kind = INT_ARRAY8;
+        p.arr8 = v;
         p.n = n;
         return p;
     }
@@ -83,7 +92,7 @@ private:
     void save_str(const std::string& s);
     void save_fp2(uint64_t re, uint64_t im);
     void save_list(const PNode& x);
-    void save_int_array(const uint64_t* v, size_t n);
+    template <class T> void save_int_array(const T* v, size_t n);
     void save_tuple(const PNode& x);
 };
 

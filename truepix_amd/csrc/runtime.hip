@@ -72,17 +72,17 @@ bool MappedSlot::ready()
 {
     if (vals) return true;
     void* hv = nullptr;
-    TP_HIP_OK(hipHostMalloc(&hv, 512, hipHostMallocMapped | hipHostMallocCoherent));
+    TP_HIP_OK(hipHostMalloc(&hv, 2048, hipHostMallocMapped | hipHostMallocCoherent));
     void* dv = nullptr;
     if (hipHostGetDevicePointer(&dv, hv, 0) != hipSuccess) {
         (void)hipHostFree(hv);
         return fail("hipHostGetDevicePointer failed");
     }
     vals = hv;
-    seq = reinterpret_cast<volatile unsigned*>(static_cast<char*>(hv) + 256);
+    seq = reinterpret_cast<volatile unsigned*>(static_cast<char*>(hv) + 1024);
     *seq = 0;
     dvals = dv;
-    dseq = reinterpret_cast<unsigned*>(static_cast<char*>(dv) + 256);
+    dseq = reinterpret_cast<unsigned*>(static_cast<char*>(dv) + 1024);
     next = 0;
     return true;
 }

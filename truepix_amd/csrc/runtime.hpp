@@ -36,7 +36,7 @@ struct DevBuf {
     template <typename T> T* as() const { return reinterpret_cast<T*>(ptr); }
 };
 
-// A result slot in mapped, coherent host memory: a kernel writes up to 16
+// A result slot in mapped, coherent host memory: a kernel writes up to 64
 // Fp2 values, fences system scope and then stores the sequence number; the
 // host spins on the sequence number instead of synchronizing the stream.
 struct MappedSlot {

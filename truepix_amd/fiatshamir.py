@@ -70,7 +70,8 @@ def encode_items(vals) -> bytes:
 def tree_tokens(obj) -> bytes:
     """Token form of a proof object for the library's proof writer
     (tp_pickle_tree): ints, strs, None, Fp2, lists and tuples; a list of
-    non-negative u64 ints (two or more) becomes one packed int array."""
+    non-negative u64 ints (two or more) becomes one packed int array (of
+    bytes when every int is below 256)."""
     out = bytearray()
 
     def enc(x):
@@ -88,7 +89,10 @@ def tree_tokens(obj) -> bytes:
             out.extend(b"f" + _PACK_Q(x.real) + _PACK_Q(x.imag))
         elif t is list or t is tuple:
             if (t is list and len(x) > 1 and all(type(v) is int and 0 <= v <= _MASK64 for v in x)):
-                out.extend(b"a" + _PACK_I(len(x)) + struct.pack("<%dQ" % len(x), *x))
+                if max(x) < 256:
+                    out.extend(b"b" + _PACK_I(len(x)) + bytes(x))
+                else:
+                    out.extend(b"a" + _PACK_I(len(x)) + struct.pack("<%dQ" % len(x), *x))
                 return
             out.extend((b"l" if t is list else b"t") + _PACK_I(len(x)))
             for v in x:

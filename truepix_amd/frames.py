@@ -11,8 +11,13 @@ circuit, inputs, VOLE correlations and commitment binding for it.
 
 ``FramePool(n)`` keeps n provers (host threads with their own HIP streams)
 taking frames from one FIFO queue: ``submit`` as frames arrive, ``wait`` /
-``result`` per frame.  The proof of a frame does not depend on which worker
-proves it or on what else is in flight (each frame has its own transcript).
+``result`` per frame.  ``FramePool(n, batched=True)`` keeps n lockstep batch
+drivers instead: each takes every queued frame that fits its batch size and
+the pool's device-memory budget and proves them together, one launch per
+protocol step across the batch (frames arriving meanwhile form the next
+batch).  The proof of a frame does not depend on which worker proves it, on
+the batch it is in or on what else is in flight (each frame has its own
+transcript).
 """
 from __future__ import annotations
 
@@ -115,6 +120,8 @@ class FrameResult:
     t_end: float
     phases_ms: dict = field(default_factory=dict)
     outputs: Optional[np.ndarray] = None    # [copies_pad][2^out_bits] when kept
+    batch: int = 1                          # frames in the launch set that proved it
+    footprint: int = 0                      # device table bytes admission counted
 
     @property
     def latency(self) -> float:
@@ -126,14 +133,45 @@ PHASES = ("eval_download", "absorb_outputs", "tables", "early_rounds", "late_rou
 
 
 class FramePool:
-    """n_workers device provers fed from one queue (tp_pool_*)."""
+    """n_workers device provers fed from one queue (tp_pool_*).
 
-    def __init__(self, n_workers: int = 1):
+    batched=True: n_workers lockstep batch drivers (tp_pool_create_batched),
+    each proving up to max_batch frames per launch set, admitted in arrival
+    order while their table bytes fit budget_bytes (0: three quarters of the
+    free device memory), with the frames' transcript work on host_threads
+    threads."""
+
+    def __init__(self, n_workers: int = 1, batched: bool = False, max_batch: int = 256,
+                 budget_bytes: int = 0, host_threads: int = 16):
         self._lib = _lib()
-        self.handle = self._lib.tp_pool_create(int(n_workers))
+        if batched:
+            self.handle = self._lib.tp_pool_create_batched(int(n_workers), int(max_batch),
+                                                           int(budget_bytes), int(host_threads))
+        else:
+            self.handle = self._lib.tp_pool_create(int(n_workers))
         if not self.handle:
             raise RuntimeError("tp_pool_create failed: %s" % nb.last_error())
         self.n_workers = int(n_workers)
+        self.batched = bool(batched)
+
+    BATCH_PROFILE = ("batches", "frames", "stage_wall_ms", "launch_wall_ms", "head_wall_ms",
+                     "head_wait_sum_ms", "head_cpu_sum_ms", "prefix_cpu_sum_ms", "rounds_wall_ms",
+                     "step_wait_ms", "step_host_wall_ms", "rest_wall_ms")
+
+    def batch_profile(self) -> dict:
+        """Where the batches of a batched pool spent their time (accumulated)."""
+        buf = (ctypes.c_double * 32)()
+        n = self._lib.tp_pool_batch_profile(self.handle, buf, 32)
+        if n < 0:
+            raise RuntimeError("tp_pool_batch_profile failed")
+        return {k: buf[i] for i, k in enumerate(self.BATCH_PROFILE[:n])}
+
+    def memory(self) -> dict:
+        """Admission state: bytes admitted now, budget, peak; batches and frames."""
+        v = [ctypes.c_uint64() for _ in range(5)]
+        if self._lib.tp_pool_memory(self.handle, *[ctypes.byref(x) for x in v]) != 0:
+            raise RuntimeError("tp_pool_memory failed")
+        return dict(zip(("in_use", "budget", "peak", "batches", "frames"), (x.value for x in v)))
 
     def now(self) -> float:
         return self._lib.tp_pool_now(self.handle)
@@ -143,6 +181,25 @@ class FramePool:
         """Queue one frame.  inputs: [n_copies][n_private] ints (uint8 values
         are sent as bytes); vole: (U.re, U.im, V.re, V.im) arrays; binding: the
         commit_binding list (str / int items)."""
+        job, _keep = self._job(circuit, inputs, pid, vole, binding, keep_outputs)
+        jid = self._lib.tp_pool_submit(self.handle, ctypes.byref(job))
+        if jid <= 0:
+            raise RuntimeError("tp_pool_submit failed: %s" % nb.last_error())
+        return int(jid)
+
+    def submit_many(self, specs) -> List[int]:
+        """Queue frames at once, in order: specs are submit()'s argument
+        tuples.  A batch driver admits them together."""
+        built = [self._job(*sp) for sp in specs]
+        arr = (nb.FrameJob * max(1, len(built)))(*[b[0] for b in built])
+        ids = (ctypes.c_int64 * max(1, len(built)))()
+        if self._lib.tp_pool_submit_many(self.handle, arr, len(built), ids) != 0:
+            raise RuntimeError("tp_pool_submit_many failed: %s" % nb.last_error())
+        return [int(ids[k]) for k in range(len(built))]
+
+    def _job(self, circuit: FrameCircuit, inputs, pid: int, vole, binding: list,
+             keep_outputs: bool = False):
+        """The C job description and the arrays it points into."""
         arr = np.asarray(inputs)
         if arr.ndim != 2 or arr.shape[1] != circuit.n_private:
             raise ValueError("inputs must be [n_copies][%d]" % circuit.n_private)
@@ -161,10 +218,7 @@ class FramePool:
                         data.ctypes.data, int(pid), ur.size, _p(ur, _U64P), _p(ui, _U64P),
                         _p(vr, _U64P), _p(vi, _U64P), len(binding), _p(kinds, _I32P),
                         _p(ints, _U64P), strs, 1 if keep_outputs else 0)
-        jid = self._lib.tp_pool_submit(self.handle, ctypes.byref(job))
-        if jid <= 0:
-            raise RuntimeError("tp_pool_submit failed: %s" % nb.last_error())
-        return int(jid)
+        return job, (data, ur, ui, vr, vi, kinds, ints, strs)
 
     def wait(self, job: int, timeout: Optional[float] = None) -> bool:
         rc = self._lib.tp_pool_wait(self.handle, job, -1.0 if timeout is None else float(timeout))
@@ -177,10 +231,10 @@ class FramePool:
         self.wait(job)
         lib, h = self._lib, self.handle
         n = ctypes.c_size_t()
-        if lib.tp_pool_proof(h, job, None, 0, ctypes.byref(n)) != 0:
-            raise RuntimeError("tp_pool_proof failed")
-        buf = ctypes.create_string_buffer(n.value)
-        lib.tp_pool_proof(h, job, buf, n.value, ctypes.byref(n))
+        ptr = ctypes.c_void_p()
+        if lib.tp_pool_proof_view(h, job, ctypes.byref(ptr), ctypes.byref(n)) != 0:
+            raise RuntimeError("tp_pool_proof_view failed")
+        proof = ctypes.string_at(ptr.value, n.value) if n.value else b""
         pt = np.zeros(2 * 64, np.uint64)
         npt = ctypes.c_int()
         val = np.zeros(2, np.uint64)
@@ -189,18 +243,21 @@ class FramePool:
         if lib.tp_pool_info(h, job, _p(pt, _U64P), 64, ctypes.byref(npt), _p(val, _U64P), times,
                             ph) != 0:
             raise RuntimeError("tp_pool_info failed")
+        bsz = ctypes.c_int()
+        fp = ctypes.c_uint64()
+        lib.tp_pool_job_stats(h, job, ctypes.byref(bsz), ctypes.byref(fp))
         outs = None
         cnt = ctypes.c_size_t()
         if lib.tp_pool_outputs(h, job, None, 0, ctypes.byref(cnt)) == 0 and cnt.value:
             outs = np.zeros(cnt.value, np.uint64)
             lib.tp_pool_outputs(h, job, _p(outs, _U64P), cnt.value, ctypes.byref(cnt))
         res = FrameResult(
-            proof=buf.raw[:n.value],
+            proof=proof,
             input_point=[_raw(int(pt[2 * k]), int(pt[2 * k + 1])) for k in range(npt.value)],
             input_value=_raw(int(val[0]), int(val[1])),
             t_submit=times[0], t_start=times[1], t_end=times[2],
             phases_ms={k: ph[i] for i, k in enumerate(PHASES)},
-            outputs=outs)
+            outputs=outs, batch=bsz.value, footprint=fp.value)
         if release:
             lib.tp_pool_release(h, job)
         return res

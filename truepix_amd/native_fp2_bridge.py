@@ -112,9 +112,17 @@ def bind(lib) -> None:
          _VP)
     _sig(lib, "tp_circuit_destroy", [_VP])
     _sig(lib, "tp_pool_create", [_I32], _VP)
+    _sig(lib, "tp_pool_create_batched", [_I32, _I32, ctypes.c_uint64, _I32], _VP)
+    _sig(lib, "tp_pool_job_stats", [_VP, ctypes.c_int64, _I32P, _U64P], _I32)
+    _sig(lib, "tp_pool_proof_view", [_VP, ctypes.c_int64, ctypes.POINTER(_VP),
+                                     ctypes.POINTER(_SIZE)], _I32)
+    _sig(lib, "tp_pool_memory", [_VP, _U64P, _U64P, _U64P, _U64P, _U64P], _I32)
+    _sig(lib, "tp_pool_batch_profile", [_VP, ctypes.POINTER(_D), _I32], _I32)
     _sig(lib, "tp_pool_destroy", [_VP])
     _sig(lib, "tp_pool_now", [_VP], _D)
     _sig(lib, "tp_pool_submit", [_VP, ctypes.POINTER(FrameJob)], ctypes.c_int64)
+    _sig(lib, "tp_pool_submit_many", [_VP, ctypes.POINTER(FrameJob), _I32,
+                                      ctypes.POINTER(ctypes.c_int64)], _I32)
     _sig(lib, "tp_pool_wait", [_VP, ctypes.c_int64, _D], _I32)
     _sig(lib, "tp_pool_error", [_VP, ctypes.c_int64], ctypes.c_char_p)
     _sig(lib, "tp_pool_proof", [_VP, ctypes.c_int64, _VP, _SIZE, ctypes.POINTER(_SIZE)], _I32)
