@@ -183,6 +183,15 @@ def bench_frames_plain(frames_n: int = 12, circuit: str = "gray", n_copies: int 
             "phases_ms_median": ph}
 
 
+# C4 pools: stream-W = W stream workers (stream-1 is the plain prover, one
+# frame after another); lockstep-DxB = D lockstep batch drivers of up to B
+# frames per launch set.
+BATCHED_CONFIGS = [("stream-1", dict(n_workers=1)), ("stream-8", dict(n_workers=8)),
+                   ("stream-16", dict(n_workers=16)),
+                   ("lockstep-1x256", dict(n_workers=1, batched=True, max_batch=256)),
+                   ("lockstep-2x128", dict(n_workers=2, batched=True, max_batch=128))]
+
+
 def bench_batched(n_frames: int = 256, seed: int = 4, configs=None):
     """C4: 256 frames, circuit uniform over gray/crop/mask/invert, size uniform
     over 64x64 / 240p / 480p / 720p; all queued at once.  Pools: stream workers
@@ -193,11 +202,7 @@ def bench_batched(n_frames: int = 256, seed: int = 4, configs=None):
     from truepix_amd import frames
     import numpy as np
     if configs is None:
-        configs = [("stream-1", dict(n_workers=1)), ("stream-8", dict(n_workers=8)),
-                   ("stream-16", dict(n_workers=16)),
-                   ("lockstep-1x256", dict(n_workers=1, batched=True, max_batch=256)),
-                   ("lockstep-2x128", dict(n_workers=2, batched=True, max_batch=128)),
-                   ("lockstep-4x64", dict(n_workers=4, batched=True, max_batch=64))]
+        configs = BATCHED_CONFIGS
     fac = _FrameFactory(seed)
     rng = np.random.Generator(np.random.PCG64(seed + 1))
     sizes = ["64x64", "240p", "480p", "720p"]
@@ -403,6 +408,9 @@ def main(argv=None):
                     help="only run the two kernels whose HBM counters the traffic figure "
                          "compares: a 256 MiB elementwise copy (x + 0, read 256 MiB, write "
                          "256 MiB) and one 24-round prove")
+    ap.add_argument("--batched-configs", default="",
+                    help="comma list of batched pool labels to run (default: all), e.g. "
+                         "lockstep-1x256")
     ap.add_argument("--skip", default="", help="comma list of side measurements to skip "
                     "(gkr_frame, gkr_frame_python_driver, batched, served, fft_gkr, rs_encode)")
     a = ap.parse_args(argv)
@@ -492,8 +500,14 @@ def main(argv=None):
                                              "round sum and publish in one workgroup (latency-bound)"}},
         "check": {"round0_poly": [int(x) for x in proof.polys[0][:2]]},
     }
+    def batched():
+        if not a.batched_configs:
+            return bench_batched()
+        want = a.batched_configs.split(",")
+        return bench_batched(configs=[c for c in BATCHED_CONFIGS if c[0] in want])
+
     for key, fn in (("gkr_frame", bench_frames_plain), ("gkr_frame_python_driver", bench_gkr_frame),
-                    ("batched", bench_batched), ("served", bench_served)):
+                    ("batched", batched), ("served", bench_served)):
         if key in a.skip:
             continue
         try:

@@ -184,8 +184,8 @@ __global__ void __launch_bounds__(kBlock) k_b_eq(const FDesc* __restrict__ D,
     else body::eq_table(zs, d.nout, d.z1cr, d.z1ci, d.n_gates, local - m.a);
 }
 
-// A6, early round rd for every frame that has it; a = pair blocks, b = gates
-// per thread (body::early_claim).
+// A6, early round rd for every frame that has it; a = gate chunks per pair,
+// b = gates per chunk (body::early_claim).
 __global__ void __launch_bounds__(kBlock) k_b_claim(const FDesc* __restrict__ D,
                                                     const Item* __restrict__ it, int n, int rd)
 {
@@ -194,9 +194,8 @@ __global__ void __launch_bounds__(kBlock) k_b_claim(const FDesc* __restrict__ D,
     const FDesc& d = D[m.f];
     const int cur = rd & 1;
     const int local = blockIdx.x - m.b0;
-    body::early_claim(d.n_gates, m.b, d.gt, d.in0, d.in1, d.z1cr, d.z1ci, d.cp >> rd, d.vr[cur],
-                      d.vi[cur], d.br[cur], d.bi[cur], d.partials, local % m.a, local / m.a,
-                      local);
+    body::early_claim(d.n_gates, m.b, m.a, d.gt, d.in0, d.in1, d.z1cr, d.z1ci, d.cp >> rd,
+                      d.vr[cur], d.vi[cur], d.br[cur], d.bi[cur], d.partials, local, local);
 }
 
 // One workgroup per item: the frame's a partials of N sums to its published
@@ -337,13 +336,10 @@ __global__ void __launch_bounds__(kBlock) k_b_mle(const FDesc* __restrict__ D,
 
 size_t al(size_t n) { return (n + 255) & ~(size_t)255; }
 
-// Early claim grid of round rd: pair blocks gx, gate chunks gy of gpt gates.
-void claim_grid(const Shape& S, int rd, int& gx, int& gy, int& gpt)
+// Early claim work of round rd.
+ClaimShape claim_of(const Shape& S, int rd)
 {
-    const int n_pairs = (S.cp >> rd) >> 1;
-    gpt = gates_per_thread(S.n_gates, n_pairs);
-    gx = (int)grid_for((size_t)n_pairs);
-    gy = (S.n_gates + gpt - 1) / gpt;
+    return claim_shape(S.n_gates, (S.cp >> rd) >> 1, kClaimItems);
 }
 
 int mle_blocks(const Shape& S)
@@ -356,11 +352,7 @@ int mle_blocks(const Shape& S)
 size_t partial_fp2(const Shape& S)
 {
     size_t n = (size_t)mle_blocks(S);
-    for (int rd = 0; rd < S.ncb; ++rd) {
-        int gx, gy, gpt;
-        claim_grid(S, rd, gx, gy, gpt);
-        n = std::max(n, (size_t)gx * gy * 4);
-    }
+    for (int rd = 0; rd < S.ncb; ++rd) n = std::max(n, (size_t)claim_of(S, rd).blocks * 4);
     return n;
 }
 
@@ -683,17 +675,14 @@ bool prove_batch(BatchCtx& B, std::vector<Job*>& jobs_in, TaskPool& tasks, doubl
         L_claim[rd] = sc.begin();
         for (int f = 0; f < nf; ++f) {
             if (rd >= S[f].ncb) continue;
-            int gx, gy, gpt;
-            claim_grid(S[f], rd, gx, gy, gpt);
-            sc.add(L_claim[rd], f, gx * gy, gx, gpt);
+            const ClaimShape c = claim_of(S[f], rd);
+            sc.add(L_claim[rd], f, (int)c.blocks, c.n_chunks, c.gpt);
         }
         sc.end(L_claim[rd]);
         L_pub4[rd] = sc.begin();
         for (int f = 0; f < nf; ++f) {
             if (rd >= S[f].ncb) continue;
-            int gx, gy, gpt;
-            claim_grid(S[f], rd, gx, gy, gpt);
-            sc.add(L_pub4[rd], f, 1, gx * gy);
+            sc.add(L_pub4[rd], f, 1, (int)claim_of(S[f], rd).blocks);
         }
         sc.end(L_pub4[rd]);
         L_fold[rd] = sc.begin();

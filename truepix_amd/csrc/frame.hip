@@ -821,16 +821,14 @@ bool prove_frame(Worker& W, Job& J)
     const fp2* host = static_cast<const fp2*>(W.slot.vals);
     int cur = 0, nc = cp;
     for (int rd = 0; rd < ncb; ++rd) {                           // early rounds (A6, A7)
-        const int n_pairs = nc >> 1;
-        const int gpt = gates_per_thread(C.n_gates, n_pairs);
-        dim3 grid(grid_for((size_t)n_pairs), (unsigned)((C.n_gates + gpt - 1) / gpt));
-        const unsigned nblk = grid.x * grid.y;
+        const ClaimShape cs = claim_shape(C.n_gates, nc >> 1, kClaimItems);
+        const unsigned nblk = cs.blocks;
         if (!W.partials.ensure((size_t)nblk * 4 * sizeof(fp2))) return false;
         fp2* parts = W.partials.as<fp2>();
-        kern::k_early_claim<<<grid, kBlock, 0, st>>>(
-            C.n_gates, gpt, C.d_type.as<int>(), C.d_in0.as<int>(), C.d_in1.as<int>(),
-            W.z1cr.as<u64>(), W.z1ci.as<u64>(), nc, W.vr[cur].as<u64>(), W.vi[cur].as<u64>(),
-            W.br[cur].as<u64>(), W.bi[cur].as<u64>(), parts);
+        kern::k_early_claim<<<nblk, kBlock, 0, st>>>(
+            C.n_gates, cs.gpt, cs.n_chunks, C.d_type.as<int>(), C.d_in0.as<int>(),
+            C.d_in1.as<int>(), W.z1cr.as<u64>(), W.z1ci.as<u64>(), nc, W.vr[cur].as<u64>(),
+            W.vi[cur].as<u64>(), W.br[cur].as<u64>(), W.bi[cur].as<u64>(), parts);
         const unsigned seq = W.slot.advance();
         k_publish_sum<4><<<1, kBlock, 0, st>>>(parts, nblk, static_cast<fp2*>(W.slot.dvals),
                                                 W.slot.dseq, seq);

@@ -55,10 +55,12 @@ __global__ void k_wiring_mlext(const int* type_idx, const int* in0, const int* i
 __global__ void k_mle_base_dev(const u64* in, PointArg z, int n_z, fp2* partials);
 
 // native_fp2.hip --------------------------------------------------------------
-// A6: early-round claim partials (4 accumulators per workgroup).
-__global__ void k_early_claim(int n_gates, int gpt, const int* gt, const int* in0, const int* in1,
-                              const u64* z1r, const u64* z1i, int nc, const u64* vr, const u64* vi,
-                              const u64* br, const u64* bi, fp2* partials);
+// A6: early-round claim partials (4 accumulators per workgroup), 1-D grid of
+// ClaimShape::blocks workgroups.
+__global__ void k_early_claim(int n_gates, int gpt, int n_chunks, const int* gt, const int* in0,
+                              const int* in1, const u64* z1r, const u64* z1i, int nc,
+                              const u64* vr, const u64* vi, const u64* br, const u64* bi,
+                              fp2* partials);
 // A7: fold every wire row and beta with r.
 __global__ void k_fold_rows(const u64* ir, const u64* ii, int n_wires, int log_out, u64 rr, u64 ri,
                             u64* orr, u64* oi, const u64* bir, const u64* bii, u64* bor, u64* boi);
@@ -73,5 +75,14 @@ bool gkr_lds_limits();
 // Gate-chunk size for the early claim so that pairs x chunks gives about 64K
 // threads (native_fp2.hip).
 int gates_per_thread(int n_gates, int n_pairs);
+
+// Work of one early claim: gate chunks of gpt gates, n_chunks per pair, about
+// `target` (pair, chunk) items in `blocks` workgroups of kBlock threads.
+struct ClaimShape {
+    int gpt, n_chunks;
+    unsigned blocks;
+};
+ClaimShape claim_shape(int n_gates, int n_pairs, long long target);
+constexpr long long kClaimItems = 1 << 16;   // default target
 
 }  // namespace tp

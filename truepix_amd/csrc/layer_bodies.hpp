@@ -175,23 +175,29 @@ __device__ __forceinline__ void eq_table(const fp2* z, int n, u64* outr, u64* ou
     outi[k] = b.im;
 }
 
-// A6: early-round claim of pair block bx and gate chunk by; the workgroup
-// total of (F(0), F(1), F(-1), F(2)) goes to partials[pidx]
-// (TP/native_fp2/native_fp2.cu:232-281 semantics, SoA [wire][copy] tables).
-__device__ __forceinline__ void early_claim(int n_gates, int gpt, const int* gt, const int* in0,
-                                            const int* in1, const u64* z1r, const u64* z1i, int nc,
-                                            const u64* vr, const u64* vi, const u64* br,
-                                            const u64* bi, fp2* partials, int bx, int by,
-                                            size_t pidx)
+// A6: early-round claim.  Work items (pair p, gate chunk q) with p fastest:
+// item = bid * blockDim + tid < n_pairs * n_chunks, chunk q covering gates
+// [q gpt, min(n_gates, (q + 1) gpt)); small rounds (few pairs) thus still fill
+// whole workgroups.  The workgroup total of (F(0), F(1), F(-1), F(2)) goes to
+// partials[pidx] (TP/native_fp2/native_fp2.cu:232-281 semantics, SoA
+// [wire][copy] tables; nc a power of two).
+__device__ __forceinline__ void early_claim(int n_gates, int gpt, int n_chunks, const int* gt,
+                                            const int* in0, const int* in1, const u64* z1r,
+                                            const u64* z1i, int nc, const u64* vr, const u64* vi,
+                                            const u64* br, const u64* bi, fp2* partials,
+                                            size_t bid, size_t pidx)
 {
     const int n_pairs = nc >> 1;
-    const int p = bx * blockDim.x + threadIdx.x;
-    const int g0 = by * gpt;
-    const int g1 = min(n_gates, g0 + gpt);
+    const int lp = __ffs(n_pairs) - 1;
+    const size_t item = bid * blockDim.x + threadIdx.x;
     fp2 acc[4] = {zero2(), zero2(), zero2(), zero2()};
-    if (p < n_pairs) {
+    if (item < ((size_t)n_chunks << lp)) {
+        const int p = (int)(item & (size_t)(n_pairs - 1));
+        const int g0 = (int)(item >> lp) * gpt;
+        const int g1 = min(n_gates, g0 + gpt);
         const size_t c = 2 * (size_t)p;
         fp2 v0 = zero2(), v1 = zero2(), v2 = zero2(), v3 = zero2();
+#pragma unroll 2
         for (int g = g0; g < g1; ++g) {
             const int tpg = gt[g];
             const size_t a = (size_t)in0[g] * nc + c, b = (size_t)in1[g] * nc + c;

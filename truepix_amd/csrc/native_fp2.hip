@@ -29,15 +29,15 @@ namespace kern {
 // TP/native_fp2/native_fp2_core.hpp:200-248).  Thread (pair p, gate chunk
 // blockIdx.y) sums z1[g] * gate_g at t = 0, 1, -1, 2 over its gates, scales by
 // beta_t(p) and joins the grid sum.  V(-1) = 2a - b and V(2) = 2b - a.
-__global__ void k_early_claim(int n_gates, int gpt, const int* __restrict__ gt,
+__global__ void k_early_claim(int n_gates, int gpt, int n_chunks, const int* __restrict__ gt,
                               const int* __restrict__ in0, const int* __restrict__ in1,
                               const u64* __restrict__ z1r, const u64* __restrict__ z1i, int nc,
                               const u64* __restrict__ vr, const u64* __restrict__ vi,
                               const u64* __restrict__ br, const u64* __restrict__ bi,
                               fp2* partials)
 {
-    body::early_claim(n_gates, gpt, gt, in0, in1, z1r, z1i, nc, vr, vi, br, bi, partials,
-                      blockIdx.x, blockIdx.y, linear_block_id());
+    body::early_claim(n_gates, gpt, n_chunks, gt, in0, in1, z1r, z1i, nc, vr, vi, br, bi, partials,
+                      blockIdx.x, blockIdx.x);
 }
 
 // Fold every wire row [n_wires][n_in] -> [n_wires][n_in/2] and beta in one
@@ -53,6 +53,21 @@ __global__ void k_fold_rows(const u64* __restrict__ ir, const u64* __restrict__ 
 }
 
 }  // namespace kern
+
+ClaimShape claim_shape(int n_gates, int n_pairs, long long target)
+{
+    ClaimShape c;
+    long long chunks = target / (n_pairs > 0 ? n_pairs : 1);
+    if (chunks < 1) chunks = 1;
+    if (chunks > n_gates) chunks = n_gates;
+    c.gpt = (int)((n_gates + chunks - 1) / chunks);
+    if (c.gpt < 1) c.gpt = 1;
+    c.n_chunks = (n_gates + c.gpt - 1) / c.gpt;
+    const long long items = (long long)c.n_chunks * n_pairs;
+    c.blocks = (unsigned)((items + kBlock - 1) / kBlock);
+    if (c.blocks < 1) c.blocks = 1;
+    return c;
+}
 
 int gates_per_thread(int n_gates, int n_pairs)
 {
@@ -404,15 +419,14 @@ bool session_claim(EarlySession* s, u64 out8[8])
 {
     const int n_pairs = s->n_copies >> 1;
     if (n_pairs < 1) return fail("early_claim: session already folded to one copy");
-    const int gpt = gates_per_thread(s->n_gates, n_pairs);
-    dim3 grid(grid_for((size_t)n_pairs), (unsigned)((s->n_gates + gpt - 1) / gpt));
+    const ClaimShape cs = claim_shape(s->n_gates, n_pairs, kClaimItems);
     RedView rv;
-    if (!red_view(s->red, (size_t)grid.x * grid.y, 4, rv)) return false;
-    kern::k_early_claim<<<grid, kBlock, 0, stream()>>>(
-        s->n_gates, gpt, s->gt.as<int>(), s->i0.as<int>(), s->i1.as<int>(), s->z1r.as<u64>(),
-        s->z1i.as<u64>(), s->n_copies, s->cur_vr(), s->cur_vi(), s->cur_br(), s->cur_bi(),
-        rv.partials);
-    if (!launch_ok() || !finalize<4>(rv, (size_t)grid.x * grid.y)) return false;
+    if (!red_view(s->red, cs.blocks, 4, rv)) return false;
+    kern::k_early_claim<<<cs.blocks, kBlock, 0, stream()>>>(
+        s->n_gates, cs.gpt, cs.n_chunks, s->gt.as<int>(), s->i0.as<int>(), s->i1.as<int>(),
+        s->z1r.as<u64>(), s->z1i.as<u64>(), s->n_copies, s->cur_vr(), s->cur_vi(), s->cur_br(),
+        s->cur_bi(), rv.partials);
+    if (!launch_ok() || !finalize<4>(rv, cs.blocks)) return false;
     fp2 res[4];
     TP_HIP_OK(hipMemcpyAsync(res, rv.result, sizeof(res), hipMemcpyDeviceToHost, stream()));
     TP_HIP_OK(hipStreamSynchronize(stream()));
