@@ -332,6 +332,50 @@ def bench_served(n_jobs: int = 2000, rate: float = 200.0, seed: int = 5, configs
     return out
 
 
+def bench_orion_commit(copies: int = 1350, reps: int = 5, seed: int = 777, cpu: bool = True):
+    """The signer's Orion commitment of one SD frame (1350 macroblocks: n = 2^20
+    coefficients per polynomial, 128 columns of 8192, codewords of 14,090):
+    encode both polynomials, hash every row's 64 column pairs, build both
+    Merkle trees.  CPU: the reference's own layout, mask, encode and Merkle
+    code compiled from source (oracle/_ref), single-threaded, one commit."""
+    import numpy as np
+    from truepix_amd import orion
+    x = np.random.Generator(np.random.PCG64(7)).integers(0, 256, size=(copies, 384), dtype=np.uint8)
+    roots = orion.commit(x, seed)                       # builds the code's graphs once
+    walls, devs = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        m, k, ms = orion.commit(x, seed, timing=True)
+        walls.append(time.perf_counter() - t0)
+        devs.append(ms)
+    walls.sort()
+    devs.sort()
+    out = {"workload": "Orion commit of one frame: %d copies, n = %d per polynomial, masked + mask"
+                       % (copies, (1 << max(0, (copies - 1).bit_length())) * 512),
+           "ms_wall": round(walls[len(walls) // 2] * 1e3, 3),
+           "ms_device": round(devs[len(devs) // 2], 3), "roots": list(roots)}
+    if cpu:
+        try:
+            sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+            from oracle.oracle import RefOrion, Oracle
+            if RefOrion.available():
+                impl, kind, what = RefOrion(), "reference", "oracle/_ref (the reference's code)"
+                t0 = time.perf_counter()
+                rm, rk, _ = impl.commit(x, seed)
+            else:
+                impl, kind, what = Oracle(), "port", "oracle/fp2_oracle.c or_orion_commit"
+                t0 = time.perf_counter()
+                rm, rk, _ = impl.orion_commit(x, seed)
+            dt = time.perf_counter() - t0
+            out["cpu_baseline"] = {"ms": round(dt * 1e3, 1), "cores": 1, "kind": kind,
+                                   "sample": "one commit at %d copies, %s" % (copies, what),
+                                   "roots_equal": [rm, rk] == list(roots)}
+            out["speedup_vs_cpu"] = round(dt * 1e3 / out["ms_wall"], 1)
+        except Exception as exc:  # the baseline must not hide the device result
+            out["cpu_baseline"] = {"error": str(exc)}
+    return out
+
+
 def bench_fft_gkr(sizes=(7, 17, 20), reps: int = 3, cpu_lg: int = 17, cpu: bool = True):
     """fft_gkr (OZ/src/VPD/fft_circuit_GKR.cpp): circuit build + every layer
     protocol + the verifier's checks, per lg_size (2^lg coefficients; the
@@ -521,6 +565,12 @@ def main(argv=None):
         except Exception as exc:
             line["fft_gkr"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
         print("bench: fft_gkr done", file=sys.stderr, flush=True)
+    if "orion" not in a.skip:
+        try:
+            line["orion_commit"] = bench_orion_commit(cpu=not a.no_cpu_baseline)
+        except Exception as exc:
+            line["orion_commit"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
+        print("bench: orion_commit done", file=sys.stderr, flush=True)
     if "rs_encode" not in a.skip:
         try:
             line["rs_encode"] = bench_rs_encode()

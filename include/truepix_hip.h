@@ -276,6 +276,22 @@ int tp_pool_outputs(void* pool, int64_t job, uint64_t* out, size_t cap, size_t* 
 /* Forgets a finished job. */
 void tp_pool_release(void* pool, int64_t job);
 
+/* ---- the signer's Orion commitment of a frame: commit_pair
+ *      (OZ/src/VPD/linearPC.cpp:253-459) as OZ/examples/multivariate_PC_test_commit.cpp:265-331
+ *      runs it (layout and mask of OZ/include/truepix/truepix_gkr.h) ------------ */
+
+/* inputs: [copies][384] sample values (the lines of input.txt); seed: the
+ * mask seed (orion_mask_seed.txt).  Writes the masked and mask Merkle roots
+ * as orion_commit.json carries them (64 hex digits + NUL); ms (optional):
+ * device time.  copies <= 65536.  0 / -1. */
+int tp_orion_commit(int copies, const uint8_t* inputs, uint64_t seed, char masked_root[65],
+                    char mask_root[65], double* ms);
+/* The expander code alone (encode, OZ/include/linear_code/linear_code_encode.h:35-110,
+ * graphs from glibc's initial rand() state as expander_init draws them):
+ * n_cols <= 256 columns of n <= 65536 Fp2, src [n_cols][n] and dst
+ * [n_cols][cw] as (re, im) pairs.  Returns the codeword length cw, or -1. */
+long long tp_orion_encode(long long n, int n_cols, const uint64_t* src, uint64_t* dst);
+
 /* ---- fft_gkr: OZ/src/VPD/fft_circuit_GKR.cpp (main :827-845), the program
  *      vpd_verifier.cpp:80 runs as "./fft_gkr <lg> <logfile>" ------------------ */
 

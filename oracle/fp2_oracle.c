@@ -30,6 +30,14 @@
  *                           source, driven through its own layer functions):
  *                           circuit layers, the claim after every layer, the
  *                           final alpha/beta, pass flag and proof size
+ *   or_sha256           FIPS 180-4 known answers and Python's hashlib
+ *   or_orion_encode     the expander code: oracle/_ref (encode of
+ *                           linear_code_encode.h compiled from source) on
+ *                           random columns, plus linearity and the systematic
+ *                           prefix (codeword starts with the message)
+ *   or_orion_commit     the signer's commit roots: oracle/_ref (the
+ *                           reference's layout, mask, encode, row hash and
+ *                           create_tree compiled from source)
  */
 #include <stddef.h>
 #include <stdint.h>
@@ -793,4 +801,249 @@ int or_fft_gkr(int lg, u64* polys, int cap_polys, int* n_polys, u64* claims, u64
     for (int s = 0; s <= lg; ++s) free(L[s]);
     free(L); free(r); free(scaled); free(ev); free(r0); free(r1); free(ru); free(rvv);
     return S.ok;
+}
+
+/* ---------------------------------------------------------------- SHA-256 --
+ * FIPS 180-4, plain C (the oracle's own; the product uses SHA-NI on the host
+ * and its own device code). */
+static const uint32_t SHA_K[64] = {
+    0x428a2f98u, 0x71374491u, 0xb5c0fbcfu, 0xe9b5dba5u, 0x3956c25bu, 0x59f111f1u, 0x923f82a4u,
+    0xab1c5ed5u, 0xd807aa98u, 0x12835b01u, 0x243185beu, 0x550c7dc3u, 0x72be5d74u, 0x80deb1feu,
+    0x9bdc06a7u, 0xc19bf174u, 0xe49b69c1u, 0xefbe4786u, 0x0fc19dc6u, 0x240ca1ccu, 0x2de92c6fu,
+    0x4a7484aau, 0x5cb0a9dcu, 0x76f988dau, 0x983e5152u, 0xa831c66du, 0xb00327c8u, 0xbf597fc7u,
+    0xc6e00bf3u, 0xd5a79147u, 0x06ca6351u, 0x14292967u, 0x27b70a85u, 0x2e1b2138u, 0x4d2c6dfcu,
+    0x53380d13u, 0x650a7354u, 0x766a0abbu, 0x81c2c92eu, 0x92722c85u, 0xa2bfe8a1u, 0xa81a664bu,
+    0xc24b8b70u, 0xc76c51a3u, 0xd192e819u, 0xd6990624u, 0xf40e3585u, 0x106aa070u, 0x19a4c116u,
+    0x1e376c08u, 0x2748774cu, 0x34b0bcb5u, 0x391c0cb3u, 0x4ed8aa4au, 0x5b9cca4fu, 0x682e6ff3u,
+    0x748f82eeu, 0x78a5636fu, 0x84c87814u, 0x8cc70208u, 0x90befffau, 0xa4506cebu, 0xbef9a3f7u,
+    0xc67178f2u};
+
+static uint32_t ror32(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+
+static void sha_block(uint32_t h[8], const unsigned char* p)
+{
+    uint32_t w[64], v[8];
+    for (int t = 0; t < 16; ++t)
+        w[t] = (uint32_t)p[4 * t] << 24 | (uint32_t)p[4 * t + 1] << 16 | (uint32_t)p[4 * t + 2] << 8 |
+               p[4 * t + 3];
+    for (int t = 16; t < 64; ++t)
+        w[t] = (ror32(w[t - 2], 17) ^ ror32(w[t - 2], 19) ^ (w[t - 2] >> 10)) + w[t - 7] +
+               (ror32(w[t - 15], 7) ^ ror32(w[t - 15], 18) ^ (w[t - 15] >> 3)) + w[t - 16];
+    memcpy(v, h, sizeof(v));
+    for (int t = 0; t < 64; ++t) {
+        const uint32_t t1 = v[7] + (ror32(v[4], 6) ^ ror32(v[4], 11) ^ ror32(v[4], 25)) +
+                            ((v[4] & v[5]) ^ (~v[4] & v[6])) + SHA_K[t] + w[t];
+        const uint32_t t2 = (ror32(v[0], 2) ^ ror32(v[0], 13) ^ ror32(v[0], 22)) +
+                            ((v[0] & v[1]) ^ (v[0] & v[2]) ^ (v[1] & v[2]));
+        memmove(v + 1, v, 7 * sizeof(uint32_t));
+        v[4] += t1;
+        v[0] = t1 + t2;
+    }
+    for (int i = 0; i < 8; ++i) h[i] += v[i];
+}
+
+void or_sha256(const unsigned char* msg, size_t n, unsigned char out[32])
+{
+    uint32_t h[8] = {0x6a09e667u, 0xbb67ae85u, 0x3c6ef372u, 0xa54ff53au,
+                     0x510e527fu, 0x9b05688cu, 0x1f83d9abu, 0x5be0cd19u};
+    size_t i = 0;
+    for (; i + 64 <= n; i += 64) sha_block(h, msg + i);
+    unsigned char tail[128];
+    size_t m = n - i;
+    memcpy(tail, msg + i, m);
+    tail[m++] = 0x80;
+    const size_t len = m + 8 <= 64 ? 64 : 128;
+    memset(tail + m, 0, len - m);
+    for (int k = 0; k < 8; ++k) tail[len - 1 - k] = (unsigned char)((u64)n * 8 >> (8 * k));
+    for (size_t b = 0; b < len; b += 64) sha_block(h, tail + b);
+    for (int k = 0; k < 8; ++k) {
+        out[4 * k] = (unsigned char)(h[k] >> 24);
+        out[4 * k + 1] = (unsigned char)(h[k] >> 16);
+        out[4 * k + 2] = (unsigned char)(h[k] >> 8);
+        out[4 * k + 3] = (unsigned char)h[k];
+    }
+}
+
+/* ----------------------------------------------------------- Orion commit --
+ * The signer's commit of a frame's inputs
+ * (OZ/examples/multivariate_PC_test_commit.cpp:265-331 -> commit_pair,
+ * OZ/src/VPD/linearPC.cpp:253-459):
+ *   - layout copy-major-384-const128-pad512 (OZ/include/truepix/truepix_gkr.h:107-210):
+ *     n = next_pow2(copies) * 512, samples at copy*512 + w, 128 at copy*512 + 384;
+ *   - mask (truepix_gkr.h:331-386): real part splitmix64(seed) % p at every
+ *     sample position in order, zero elsewhere; masked = coefficients + mask;
+ *   - 128 columns of n/128 (column c = flat indices c*col .. c*col + col - 1),
+ *     each encoded by the expander code (OZ/include/linear_code/expanders.h:19-60,
+ *     linear_code_encode.h:35-110): graphs drawn from glibc rand() in
+ *     expander_init's order (C at each depth on the way down, D on the way
+ *     up), weights (rand() % p, rand() % p) for (real, imag);
+ *     codeword(x) = x | codeword(C x) | D codeword(C x), inputs of length <= 13
+ *     kept as they are;
+ *   - per row r < 2 col: 64 chained SHA-256(state | col[2j][r] | col[2j+1][r])
+ *     from a zero state, an element as (imag, real) little-endian
+ *     (OZ/src/infrastructure/merkle_tree.cpp:15-27), zero past the codeword;
+ *   - a heap-ordered binary Merkle tree over the rows, node = SHA-256(left |
+ *     right) (merkle_tree.cpp:29-79), the root printed as root_hex does
+ *     (truepix_gkr.h:389-402).
+ */
+typedef struct { long long L, R; int deg; long long* to; fp2* w; } xgraph;
+static xgraph XG_C[64], XG_D[64];
+static const double X_ALPHA = 0.238, X_RATE = 1.72;
+static const int X_CN = 10, X_DN = 20, X_SMALL = 13, X_COLS = 128;
+
+static void xg_make(xgraph* g, grand* rg, long long L, long long R, int deg)
+{
+    free(g->to);
+    free(g->w);
+    g->L = L;
+    g->R = R;
+    g->deg = deg;
+    g->to = (long long*)malloc(sizeof(long long) * (size_t)(L * deg + 1));
+    g->w = (fp2*)malloc(sizeof(fp2) * (size_t)(L * deg + 1));
+    for (long long i = 0; i < L * deg; ++i) {
+        g->to[i] = (long long)(grand_next(rg) % (uint32_t)R);
+        const u64 re = grand_next(rg) % P61;
+        g->w[i] = mk(re, grand_next(rg) % P61);
+    }
+}
+
+static long long xg_init(grand* rg, long long n, int dep)
+{
+    if (n <= X_SMALL) return n;
+    const long long R = (long long)(X_ALPHA * n);
+    xg_make(&XG_C[dep], rg, n, R, X_CN);
+    const long long L = xg_init(rg, R, dep + 1);
+    const long long R2 = (long long)(n * (X_RATE - 1) - L);
+    xg_make(&XG_D[dep], rg, L, R2, X_DN);
+    return n + L + R2;
+}
+
+static long long x_len(long long n, int dep)
+{
+    if (n <= X_SMALL) return n;
+    return n + x_len(XG_C[dep].R, dep + 1) + XG_D[dep].R;
+}
+
+static long long x_encode(const fp2* src, long long n, int dep, fp2* dst)
+{
+    if (n <= X_SMALL) {
+        memcpy(dst, src, sizeof(fp2) * (size_t)n);
+        return n;
+    }
+    const xgraph* C = &XG_C[dep];
+    fp2* y = (fp2*)calloc((size_t)C->R + 1, sizeof(fp2));
+    for (long long i = 0; i < n; ++i)
+        for (int d = 0; d < C->deg; ++d) {
+            const long long k = i * C->deg + d;
+            y[C->to[k]] = add2(y[C->to[k]], mul2(C->w[k], src[i]));
+        }
+    const long long L = x_encode(y, C->R, dep + 1, dst + n);
+    free(y);
+    memcpy(dst, src, sizeof(fp2) * (size_t)n);
+    const xgraph* D = &XG_D[dep];
+    fp2* par = dst + n + L;
+    memset(par, 0, sizeof(fp2) * (size_t)D->R);
+    for (long long i = 0; i < L; ++i)
+        for (int d = 0; d < D->deg; ++d) {
+            const long long k = i * D->deg + d;
+            par[D->to[k]] = add2(par[D->to[k]], mul2(D->w[k], dst[n + i]));
+        }
+    return n + L + D->R;
+}
+
+/* codeword of n Fp2 (re, im pairs) with graphs for n from glibc rand(1) */
+long long or_orion_encode(long long n, const u64* src, u64* dst)
+{
+    grand rg;
+    grand_init(&rg, 1);
+    xg_init(&rg, n, 0);
+    const long long len = x_len(n, 0);
+    fp2* s = (fp2*)malloc(sizeof(fp2) * (size_t)n);
+    fp2* d = (fp2*)malloc(sizeof(fp2) * (size_t)len);
+    for (long long i = 0; i < n; ++i) s[i] = mk(src[2 * i], src[2 * i + 1]);
+    x_encode(s, n, 0, d);
+    for (long long i = 0; i < len; ++i) {
+        dst[2 * i] = d[i].re;
+        dst[2 * i + 1] = d[i].im;
+    }
+    free(s);
+    free(d);
+    return len;
+}
+
+static u64 smix(u64 z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+static void put_le64(unsigned char* p, u64 v)
+{
+    for (int k = 0; k < 8; ++k) p[k] = (unsigned char)(v >> (8 * k));
+}
+
+static void root_hex(const unsigned char d[32], char out[65])
+{
+    static const char hx[] = "0123456789abcdef";
+    const int order[4] = {8, 0, 24, 16};   /* root_hex prints h0[1], h0[0], h1[1], h1[0] */
+    for (int q = 0; q < 4; ++q) {
+        u64 v = 0;
+        for (int k = 7; k >= 0; --k) v = (v << 8) | d[order[q] + k];
+        for (int k = 0; k < 16; ++k) out[16 * q + k] = hx[(v >> (60 - 4 * k)) & 15];
+    }
+    out[64] = 0;
+}
+
+/* inputs [copies][384]; 0 / -1 */
+int or_orion_commit(int copies, const unsigned char* inputs, u64 seed, char masked_root[65],
+                    char mask_root[65], long long* cw_len_out)
+{
+    if (copies < 1) return -1;
+    long long pc = 1;
+    while (pc < copies) pc <<= 1;
+    const long long n = pc * 512, col = n / X_COLS, rows = 2 * col;
+    grand rg;
+    grand_init(&rg, 1);
+    xg_init(&rg, col, 0);
+    const long long cw = x_len(col, 0);
+    fp2* poly[2];
+    poly[0] = (fp2*)calloc((size_t)n, sizeof(fp2));   /* masked */
+    poly[1] = (fp2*)calloc((size_t)n, sizeof(fp2));   /* mask */
+    u64 state = seed;
+    for (long long c = 0; c < copies; ++c)
+        for (int w = 0; w < 384; ++w) {
+            state += 0x9E3779B97F4A7C15ULL;
+            const u64 m = smix(state) % P61;
+            poly[1][c * 512 + w] = mk(m, 0);
+            poly[0][c * 512 + w] = mk(fadd(inputs[c * 384 + w], m), 0);
+        }
+    for (long long c = 0; c < copies; ++c) poly[0][c * 512 + 384] = mk(128, 0);
+    fp2* cwb = (fp2*)malloc(sizeof(fp2) * (size_t)X_COLS * cw);
+    unsigned char* tree = (unsigned char*)malloc((size_t)(2 * rows) * 32);
+    char* roots[2] = {masked_root, mask_root};
+    for (int k = 0; k < 2; ++k) {
+        for (int c = 0; c < X_COLS; ++c) x_encode(poly[k] + c * col, col, 0, cwb + (long long)c * cw);
+        for (long long r = 0; r < rows; ++r) {
+            unsigned char msg[64];
+            memset(msg, 0, 32);
+            for (int j = 0; j < X_COLS / 2; ++j) {
+                for (int h = 0; h < 2; ++h) {
+                    const fp2 e = r < cw ? cwb[(long long)(2 * j + h) * cw + r] : mk(0, 0);
+                    put_le64(msg + 32 + 16 * h, e.im);
+                    put_le64(msg + 40 + 16 * h, e.re);
+                }
+                or_sha256(msg, 64, msg);
+            }
+            memcpy(tree + (rows + r) * 32, msg, 32);
+        }
+        for (long long v = rows - 1; v >= 1; --v) or_sha256(tree + 2 * v * 32, 64, tree + v * 32);
+        root_hex(tree + 32, roots[k]);
+    }
+    free(cwb);
+    free(tree);
+    free(poly[0]);
+    free(poly[1]);
+    if (cw_len_out) *cw_len_out = cw;
+    return 0;
 }

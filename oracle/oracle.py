@@ -157,6 +157,79 @@ Oracle.fft_gkr = _oracle_fft_gkr
 Oracle.glibc_rand = _oracle_glibc_rand
 
 
+def _orion_commit(fn, inputs, seed: int):
+    arr = np.ascontiguousarray(inputs, dtype=np.uint8)
+    a = ctypes.create_string_buffer(65)
+    b = ctypes.create_string_buffer(65)
+    cw = ctypes.c_longlong()
+    rc = fn(int(arr.shape[0]), arr.tobytes(), int(seed) & (2 ** 64 - 1), a, b, ctypes.byref(cw))
+    if rc != 0:
+        raise ValueError("orion commit rejected copies=%d" % arr.shape[0])
+    return a.value.decode(), b.value.decode(), cw.value
+
+
+def _orion_encode(fn, col):
+    """Codeword of one column [n][2] (re, im) -> [cw][2]."""
+    src = np.ascontiguousarray(col, dtype=np.uint64)
+    n = src.shape[0]
+    dst = np.zeros((int(n * 1.72) + 16) * 2, np.uint64)
+    cw = fn(n, _p(src), _p(dst))
+    return dst[:2 * cw].reshape(cw, 2).copy()
+
+
+_COMMIT_ARGS = [_INT, ctypes.c_char_p, _U64, ctypes.c_char_p, ctypes.c_char_p,
+                ctypes.POINTER(ctypes.c_longlong)]
+
+
+def _oracle_orion_commit(self, inputs, seed: int):
+    """(masked_root, mask_root, cw_len) of the signer's commitment."""
+    if not hasattr(self, "_orc"):
+        self._orc = _bind(self.lib, "or_orion_commit", _COMMIT_ARGS, _INT)
+    return _orion_commit(self._orc, inputs, seed)
+
+
+def _oracle_orion_encode(self, col):
+    if not hasattr(self, "_ore"):
+        self._ore = _bind(self.lib, "or_orion_encode", [ctypes.c_longlong, _U64P, _U64P],
+                          ctypes.c_longlong)
+    return _orion_encode(self._ore, col)
+
+
+def _oracle_sha256(self, data: bytes) -> bytes:
+    if not hasattr(self, "_sha"):
+        self._sha = _bind(self.lib, "or_sha256", [ctypes.c_char_p, _SZ, ctypes.c_char_p])
+    out = ctypes.create_string_buffer(32)
+    self._sha(data, len(data), out)
+    return out.raw
+
+
+Oracle.orion_commit = _oracle_orion_commit
+Oracle.orion_encode = _oracle_orion_encode
+Oracle.sha256 = _oracle_sha256
+
+
+class RefOrion:
+    """The signer's commitment computed with the reference's own layout, mask,
+    expander code, row hash and Merkle tree (oracle/ref_harness_orion.cpp)."""
+
+    PATH = os.path.join(HERE, "_ref", "liborion_ref.so")
+
+    @classmethod
+    def available(cls) -> bool:
+        return os.path.isfile(cls.PATH)
+
+    def __init__(self):
+        lib = ctypes.CDLL(self.PATH)
+        self.c = _bind(lib, "ref_orion_commit", _COMMIT_ARGS, _INT)
+        self.e = _bind(lib, "ref_orion_encode", [ctypes.c_longlong, _U64P, _U64P], ctypes.c_longlong)
+
+    def commit(self, inputs, seed: int):
+        return _orion_commit(self.c, inputs, seed)
+
+    def encode(self, col):
+        return _orion_encode(self.e, col)
+
+
 class RefFftGkr:
     """fft_circuit_GKR.cpp compiled in place, driven through its own layer
     functions (oracle/ref_harness_fftgkr.cpp)."""

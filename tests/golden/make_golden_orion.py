@@ -1,0 +1,67 @@
+#!/usr/bin/env python3
+"""Writes tests/golden/orion_ref.json from the reference's own commitment
+pieces (the layout and mask of OZ/include/truepix/truepix_gkr.h, expander_init
+and encode of OZ/include/linear_code/, the row hash and create_tree of
+OZ/src/infrastructure/merkle_tree.cpp over flo-shani's SHA-256, compiled in
+place by `make -C oracle ref` and driven in commit_pair's order by
+oracle/ref_harness_orion.cpp): the masked and mask roots and the codeword
+length for seeded inputs, and SHA-256 digests of single-column codewords.
+Run where /root/reference exists:
+
+    make -C oracle ref && python tests/golden/make_golden_orion.py
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+# (copies, mask seed, input seed): 1 and 3 copies keep every column below the
+# code's distance threshold; the rest cover 1 .. 5 expander levels, up to the
+# 1350-copy SD frame of the signer (n = 2^20).
+COMMIT_CASES = [(1, 5, 1), (3, 7, 2), (5, 11, 3), (16, 2024, 4), (33, 99, 5), (300, 12345, 6),
+                (1350, 777, 7)]
+ENCODE_SIZES = [5, 13, 14, 31, 64, 100, 1024, 8192]
+
+
+def inputs_for(copies, seed):
+    return np.random.Generator(np.random.PCG64(seed)).integers(0, 256, size=(copies, 384),
+                                                               dtype=np.uint8)
+
+
+def column_for(n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return rng.integers(0, (1 << 61) - 1, size=(n, 2), dtype=np.uint64)
+
+
+def main():
+    from oracle.oracle import RefOrion
+    if not RefOrion.available():
+        raise SystemExit("build the reference harness first: make -C oracle ref")
+    ref = RefOrion()
+    out = {"source": "truepix_gkr.h layout + mask, expander_init/encode, merkle_tree.cpp "
+                     "(commit_pair order), glibc rand() initial state",
+           "inputs": "PCG64(input_seed).integers(0, 256, (copies, 384), uint8)",
+           "columns": "PCG64(n).integers(0, 2^61 - 1, (n, 2), uint64) as (re, im)",
+           "commit": [], "encode": []}
+    for copies, seed, iseed in COMMIT_CASES:
+        m, k, cw = ref.commit(inputs_for(copies, iseed), seed)
+        out["commit"].append({"copies": copies, "seed": seed, "input_seed": iseed,
+                              "masked_root": m, "mask_root": k, "cw_len": cw})
+    for n in ENCODE_SIZES:
+        cwd = ref.encode(column_for(n, n))
+        out["encode"].append({"n": n, "cw_len": int(cwd.shape[0]),
+                              "sha256": hashlib.sha256(cwd.tobytes()).hexdigest(),
+                              "head": [int(x) for x in cwd[:4].reshape(-1)]})
+    with open(os.path.join(HERE, "orion_ref.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+    print("wrote", len(out["commit"]), "commit cases,", len(out["encode"]), "encode cases")
+
+
+if __name__ == "__main__":
+    main()

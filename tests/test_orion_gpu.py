@@ -1,0 +1,72 @@
+"""GPU parity of the signer's Orion commitment (tp_orion_commit,
+tp_orion_encode): the roots and codewords equal the reference's
+(tests/golden/orion_ref.json, from the oracle/_ref build of the reference's
+layout, mask, expander code and Merkle tree) and the oracle's on fresh
+inputs, seeds and columns."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from truepix_amd import orion
+from truepix_amd.fp2_link import commit_binding
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+P = (1 << 61) - 1
+
+
+@pytest.fixture(scope="module")
+def gold():
+    with open(os.path.join(GOLD, "orion_ref.json")) as fh:
+        return json.load(fh)
+
+
+def inputs_for(copies, seed):
+    return np.random.Generator(np.random.PCG64(seed)).integers(0, 256, size=(copies, 384),
+                                                               dtype=np.uint8)
+
+
+def test_commit_matches_reference_roots(bridge, gold):
+    for case in gold["commit"]:
+        print("orion commit copies", case["copies"], flush=True)
+        m, k = orion.commit(inputs_for(case["copies"], case["input_seed"]), case["seed"])
+        assert (m, k) == (case["masked_root"], case["mask_root"]), case["copies"]
+
+
+def test_encode_matches_reference_codewords(bridge, gold):
+    for case in gold["encode"]:
+        n = case["n"]
+        col = np.random.Generator(np.random.PCG64(n)).integers(0, P, size=(n, 2), dtype=np.uint64)
+        cw = orion.encode(col[None])[0]
+        assert cw.shape[0] == case["cw_len"], n
+        assert hashlib.sha256(cw.tobytes()).hexdigest() == case["sha256"], n
+
+
+def test_encode_many_columns_equals_oracle(bridge, oracle):
+    """256 different columns in one launch set: each equals its own oracle
+    codeword."""
+    for n in (64, 1000):
+        cols = np.random.Generator(np.random.PCG64(5 + n)).integers(0, P, size=(256, n, 2),
+                                                                    dtype=np.uint64)
+        got = orion.encode(cols)
+        for c in (0, 1, 127, 128, 200, 255):
+            assert np.array_equal(got[c], oracle.orion_encode(cols[c])), (n, c)
+
+
+def test_commit_equals_oracle_on_new_inputs(bridge, oracle):
+    for copies, seed in ((2, 1), (9, 2 ** 64 - 3), (100, 31337), (257, 8)):
+        x = inputs_for(copies, copies)
+        m, k, _ = oracle.orion_commit(x, seed)
+        assert orion.commit(x, seed) == (m, k), copies
+
+
+def test_manifest_binds_the_roots(bridge):
+    x = inputs_for(37, 1)
+    man = orion.manifest(x, 99)
+    m, k = orion.commit(x, 99)
+    assert (man["masked_root"], man["mask_root"]) == (m, k)
+    assert man["n"] == 64 * 512 and man["padded_copies"] == 64
+    assert commit_binding(man)[-2:] == [m, k]

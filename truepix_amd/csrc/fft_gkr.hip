@@ -19,6 +19,7 @@
 //   s(X) = sum_i (g0 + X dg)(f0 + X df) + (h0 + X dh)
 //        -> a = sum dg df, b = sum (dg f0 + g0 df + dh), c = sum (g0 f0 + h0).
 #include "fp61.hpp"
+#include "glibc_rand.hpp"
 #include "kernels.hpp"
 #include "reduce.hpp"
 #include "runtime.hpp"
@@ -431,41 +432,6 @@ __global__ void __launch_bounds__(kBlock) k_fg_final(const V2* __restrict__ f,
 
 // ------------------------------------------------------------------- host
 
-// glibc rand() (TYPE_3, seed 1), restated from its published definition:
-// r[0] = seed, r[i] = 16807 r[i-1] mod (2^31 - 1) (i < 31), r[i] = r[i-31]
-// (i < 34), then r[k] = r[k-31] + r[k-3] mod 2^32; output r[k] >> 1 from
-// k = 344 on.
-struct GlibcRand {
-    uint32_t r[34];
-    long k = 34;
-    explicit GlibcRand(uint32_t seed)
-    {
-        int32_t s[34];
-        s[0] = (int32_t)(seed ? seed : 1);
-        for (int i = 1; i < 31; ++i) {
-            long long t = (16807LL * s[i - 1]) % 2147483647LL;
-            if (t < 0) t += 2147483647LL;
-            s[i] = (int32_t)t;
-        }
-        for (int i = 31; i < 34; ++i) s[i] = s[i - 31];
-        for (int i = 0; i < 34; ++i) r[i] = (uint32_t)s[i];
-        for (int i = 0; i < 310; ++i) step();
-    }
-    uint32_t step()
-    {
-        const uint32_t v = r[(k - 31) % 34] + r[(k - 3) % 34];
-        r[k % 34] = v;
-        ++k;
-        return v;
-    }
-    uint32_t next() { return step() >> 1; }
-    fp2 fe()                               // prime_field::random()
-    {
-        const u64 re = next() % P;
-        const u64 im = next() % P;
-        return mk2(re, im);
-    }
-};
 
 fp2 eq_at(const fp2* z, int n, size_t idx, fp2 init)
 {
