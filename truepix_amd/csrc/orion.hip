@@ -35,7 +35,9 @@
 //   - the row hash is one thread per (row, polynomial), 128 compressions in
 //     sequence (independent across rows), the padding block of every 64-byte
 //     message taken from a precomputed schedule;
-//   - the tree is one launch per level (one workgroup for the last levels).
+//   - the all-zero rows past the codeword share one leaf;
+//   - the tree is one launch per level, one workgroup per tree for the last
+//     eleven levels.
 #include "fp61.hpp"
 #include "glibc_rand.hpp"
 #include "runtime.hpp"
@@ -84,12 +86,17 @@ __constant__ uint32_t c_K[64] = {
 __constant__ uint32_t c_pad_kw[64];
 
 __device__ __forceinline__ uint32_t rotr(uint32_t x, int n) { return (x >> n) | (x << (32 - n)); }
+// x ^ y ^ z in one v_bitop3_b32 (truth table 0x96, the 3-input parity)
+__device__ __forceinline__ uint32_t xor3(uint32_t x, uint32_t y, uint32_t z)
+{
+    return __builtin_amdgcn_bitop3_b32(x, y, z, 0x96);
+}
 
 #define TP_SHA_ROUND(KW)                                                                   \
     do {                                                                                   \
-        const uint32_t t1 = h + (rotr(e, 6) ^ rotr(e, 11) ^ rotr(e, 25)) +                 \
+        const uint32_t t1 = h + xor3(rotr(e, 6), rotr(e, 11), rotr(e, 25)) +               \
                             ((e & f) ^ (~e & g)) + (KW);                                   \
-        const uint32_t t2 = (rotr(a, 2) ^ rotr(a, 13) ^ rotr(a, 22)) +                     \
+        const uint32_t t2 = xor3(rotr(a, 2), rotr(a, 13), rotr(a, 22)) +                   \
                             ((a & b) ^ (a & c) ^ (b & c));                                 \
         h = g; g = f; f = e; e = d + t1; d = c; c = b; b = a; a = t1 + t2;                 \
     } while (0)
@@ -107,8 +114,8 @@ __device__ __forceinline__ void sha64(uint32_t w[16], uint32_t out[8])
             wt = w[t];
         } else {
             const uint32_t w15 = w[(t - 15) & 15], w2 = w[(t - 2) & 15];
-            wt = w[t & 15] + (rotr(w15, 7) ^ rotr(w15, 18) ^ (w15 >> 3)) + w[(t - 7) & 15] +
-                 (rotr(w2, 17) ^ rotr(w2, 19) ^ (w2 >> 10));
+            wt = w[t & 15] + xor3(rotr(w15, 7), rotr(w15, 18), w15 >> 3) + w[(t - 7) & 15] +
+                 xor3(rotr(w2, 17), rotr(w2, 19), w2 >> 10);
             w[t & 15] = wt;
         }
         TP_SHA_ROUND(c_K[t] + wt);
@@ -195,7 +202,21 @@ __global__ void __launch_bounds__(kLanes) k_or_step(V2* __restrict__ X, long lon
     const int lane = threadIdx.x;
     fp2 acc = zero2();
     const int e1 = rp[t + 1];
-    for (int e = rp[t]; e < e1; ++e) {
+    int e = rp[t];
+    // eight edges' loads in flight before their products (the gathers are
+    // independent; one at a time each would wait out the L2 latency)
+    for (; e + 8 <= e1; e += 8) {
+        V2 x[8], we[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            we[q] = w[e + q];
+            x[q] = X[(size_t)(in_base + src[e + q]) * kLanes + lane];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            acc = add2(acc, mul2(mk2(we[q].re, we[q].im), mk2(x[q].re, x[q].im)));
+    }
+    for (; e < e1; ++e) {
         const V2 we = w[e];
         const V2 x = X[(size_t)(in_base + src[e]) * kLanes + lane];
         acc = add2(acc, mul2(mk2(we.re, we.im), mk2(x.re, x.im)));
@@ -206,23 +227,32 @@ __global__ void __launch_bounds__(kLanes) k_or_step(V2* __restrict__ X, long lon
     X[(size_t)(out_base + t) * kLanes + lane] = o;
 }
 
-// Row leaves: one thread per (row, polynomial); leaves land at the tree's
-// bottom level, tree[poly][rows + r] (8 words each).
+// Row leaves: one thread per (row, polynomial) for rows 0 .. cw, row cw
+// standing for every all-zero row past the codeword; leaves land at the
+// tree's bottom level, tree[poly][rows + r] (8 words each).
 __global__ void __launch_bounds__(256) k_or_rows(const V2* __restrict__ X, long long cw,
                                                  long long rows, uint32_t* __restrict__ tree)
 {
+    const long long span = cw < rows ? cw + 1 : cw;
     const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (id >= 2 * rows) return;
-    const int poly = (int)(id / rows);
-    const long long r = id - poly * rows;
+    if (id >= 2 * span) return;
+    const int poly = (int)(id / span);
+    const long long r = id - poly * span;
+    const bool data = r < cw;
     uint32_t st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const V2* row = X + (size_t)r * kLanes + poly * kCols;
+    const V2* row = X + (size_t)(data ? r : 0) * kLanes + poly * kCols;
+    V2 na, nb;
+    na.re = na.im = nb.re = nb.im = 0;
+    if (data) {
+        na = row[0];
+        nb = row[1];
+    }
     for (int j = 0; j < kCols / 2; ++j) {
-        V2 a, b;
-        a.re = a.im = b.re = b.im = 0;
-        if (r < cw) {
-            a = row[2 * j];
-            b = row[2 * j + 1];
+        // this pair, and the next one's loads in flight behind the hash
+        const V2 a = na, b = nb;
+        if (data && j + 1 < kCols / 2) {
+            na = row[2 * j + 2];
+            nb = row[2 * j + 3];
         }
         uint32_t m[16];
 #pragma unroll
@@ -253,6 +283,19 @@ __device__ __forceinline__ void tree_node(uint32_t* base, long long v)
     for (int k = 0; k < 8; ++k) base[(size_t)v * 8 + k] = out[k];
 }
 
+// The all-zero rows past the codeword share row cw's leaf.
+__global__ void __launch_bounds__(256) k_or_zero_rows(long long cw, long long rows,
+                                                      uint32_t* __restrict__ tree)
+{
+    const long long n = rows - cw - 1;
+    const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (id >= 2 * n * 8) return;
+    const int poly = (int)(id / (n * 8));
+    const long long k = id - poly * n * 8;
+    uint32_t* base = tree + (size_t)poly * 2 * rows * 8;
+    base[(size_t)(rows + cw + 1) * 8 + k] = base[(size_t)(rows + cw) * 8 + (k & 7)];
+}
+
 // One tree level, nodes lo .. 2 lo - 1 of both trees.
 __global__ void __launch_bounds__(256) k_or_level(uint32_t* __restrict__ tree, long long rows,
                                                   long long lo)
@@ -264,8 +307,8 @@ __global__ void __launch_bounds__(256) k_or_level(uint32_t* __restrict__ tree, l
 }
 
 // The levels below and including lo (nodes < 2 lo) of one tree per workgroup.
-__global__ void __launch_bounds__(256) k_or_top(uint32_t* __restrict__ tree, long long rows,
-                                                long long lo)
+__global__ void __launch_bounds__(1024) k_or_top(uint32_t* __restrict__ tree, long long rows,
+                                                 long long lo)
 {
     uint32_t* base = tree + (size_t)blockIdx.x * 2 * rows * 8;
     for (long long l = lo; l >= 1; l >>= 1) {
@@ -484,16 +527,23 @@ int tp_orion_commit(int copies, const uint8_t* inputs, uint64_t seed, char maske
         g_in.as<uint8_t>(), copies, seed, col, g_X.as<V2>());
     TP_HIP_OK(hipGetLastError());
     if (!encode_all(*P, st)) return -1;
-    k_or_rows<<<(unsigned)((2 * rows + 255) / 256), 256, 0, st>>>(g_X.as<V2>(), P->cw, rows,
+    const long long span = P->cw < rows ? P->cw + 1 : P->cw;
+    k_or_rows<<<(unsigned)((2 * span + 255) / 256), 256, 0, st>>>(g_X.as<V2>(), P->cw, rows,
                                                                   g_tree.as<uint32_t>());
     TP_HIP_OK(hipGetLastError());
+    if (rows - P->cw - 1 > 0) {
+        const long long words = 2 * (rows - P->cw - 1) * 8;
+        k_or_zero_rows<<<(unsigned)((words + 255) / 256), 256, 0, st>>>(P->cw, rows,
+                                                                       g_tree.as<uint32_t>());
+        TP_HIP_OK(hipGetLastError());
+    }
     long long lo = rows / 2;
-    for (; lo > 128; lo >>= 1) {
+    for (; lo > 1024; lo >>= 1) {
         k_or_level<<<(unsigned)((2 * lo + 255) / 256), 256, 0, st>>>(g_tree.as<uint32_t>(), rows, lo);
         TP_HIP_OK(hipGetLastError());
     }
     if (lo >= 1) {
-        k_or_top<<<2, 256, 0, st>>>(g_tree.as<uint32_t>(), rows, lo);
+        k_or_top<<<2, 1024, 0, st>>>(g_tree.as<uint32_t>(), rows, lo);
         TP_HIP_OK(hipGetLastError());
     }
     TP_HIP_OK(hipEventRecord(g_ev[1], st));
