@@ -215,7 +215,12 @@ def bench_batched(n_frames: int = 256, seed: int = 4, configs=None):
     for label, kw in configs:
         with frames.FramePool(**kw) as pool:
             if kw.get("batched"):
-                warm = pool.submit_many(specs)                  # sizes every driver's arena
+                # sizes every driver's arena and lets the schedule tuner settle
+                # (three claim sizes, then three fold sizes, one batch each)
+                for _ in range(6):
+                    for j in pool.submit_many(specs):
+                        pool.result(j)
+                warm = []
             else:
                 warm = [pool.submit(*specs[k]) for k in range(min(2 * kw["n_workers"], n_frames))]
             for j in warm:
@@ -228,6 +233,7 @@ def bench_batched(n_frames: int = 256, seed: int = 4, configs=None):
             wall = time.perf_counter() - t0
             m1 = pool.memory() if kw.get("batched") else None
             pf1 = pool.batch_profile() if kw.get("batched") else None
+            pool_tuning = pool.tuning() if kw.get("batched") else None
         proofs = [r.proof for r in res]
         if ref_proofs is None:
             ref_proofs = proofs
@@ -248,6 +254,7 @@ def bench_batched(n_frames: int = 256, seed: int = 4, configs=None):
                     ph[k] = max(ph.get(k, 0.0), v)
             rec["batch_phases_ms_max"] = {k: round(v, 2) for k, v in ph.items()}
             rec["batch_profile"] = {k: round(pf1[k] - pf0[k], 2) for k in pf1}
+            rec["schedule_tuner"] = pool_tuning
         out["by_config"][label] = rec
         print("bench: batched %s %.1f frames/s" % (label, rec["frames_per_s"]), file=sys.stderr,
               flush=True)

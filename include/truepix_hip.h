@@ -268,9 +268,20 @@ int tp_pool_memory(void* pool, uint64_t* in_use, uint64_t* budget, uint64_t* pea
 /* Where a batched pool's batches spent their time since it was created: batches,
  * frames, then ms: input staging, launch queueing, transcript heads (wall),
  * output waits (sum over frames), heads (sum), proof prefixes (sum), rounds
- * (wall), step waits, step transcript work (wall), proof ends (wall).  Writes
- * min(n, count) values and returns the count (12), -1 on a null pool. */
+ * (wall), step waits, step transcript work (wall), proof ends (wall), device
+ * time of the early-claim and early-fold launches.  Writes min(n, count)
+ * values and returns the count (14), -1 on a null pool. */
 int tp_pool_batch_profile(void* pool, double* out, int n);
+/* The lockstep schedule autotuner of a batched pool: phase (0 timing the
+ * early-claim sizes, 1 timing the fold sizes, 2 settled), the claim items per
+ * frame and fold workgroups per frame in use, then the three claim costs (ns
+ * per pair-gate term) and three fold costs (ns per algorithmic byte) measured
+ * with events on real batches.  Writes min(n, 9) values; returns 9. */
+int tp_pool_tuning(void* pool, double* out, int n);
+/* Fixes the lockstep schedule (claim items per frame in [1024, 2^18], fold
+ * workgroups per frame in [16, 65536]) from the next batch on; the tuner
+ * stops.  Every schedule gives the same proof bytes.  0 / -1. */
+int tp_pool_fix_schedule(void* pool, long long claim_items, int fold_cap);
 /* Circuit outputs [copies_pad][2^out_bits] of a job submitted with keep_outputs. */
 int tp_pool_outputs(void* pool, int64_t job, uint64_t* out, size_t cap, size_t* n);
 /* Forgets a finished job. */

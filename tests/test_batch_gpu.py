@@ -175,3 +175,29 @@ def test_batch_wide_outputs_take_the_u64_path(circs):
         assert a.proof == b.proof, k
         assert np.array_equal(a.outputs, b.outputs), k
     wide.close()
+
+
+def test_schedule_tuner_settles_and_keeps_proofs(circs):
+    """The lockstep schedule's autotuner tries three early-claim sizes, then
+    three fold sizes, on real batches and settles; every schedule it tries,
+    and the extremes fixed by hand, give the frames' solo proofs."""
+    specs = synth_specs(circs, [1350] * 6 + [300, 16], seed=17)
+    with frames.FramePool(1) as solo:
+        want = [solo.result(solo.submit(*sp)).proof for sp in specs]
+    with frames.FramePool(1, batched=True, max_batch=8, host_threads=8) as pool:
+        for rep in range(7):
+            ids = pool.submit_many(specs)
+            assert [pool.result(j).proof for j in ids] == want, rep
+        t = pool.tuning()
+    print("tuning", t, flush=True)
+    assert t["phase"] == 2
+    assert all(c > 0 for c in t["claim_ns_per_term"]) and all(c > 0 for c in t["fold_ns_per_byte"])
+    assert t["claim_items"] in (1 << 14, 1 << 16, 1 << 18) and t["fold_cap"] in (256, 1024, 4096)
+    with frames.FramePool(1, batched=True, max_batch=8, host_threads=8) as pool:
+        for claim, fold in ((1024, 16), (1 << 18, 65536)):
+            pool.fix_schedule(claim, fold)
+            ids = pool.submit_many(specs)
+            assert [pool.result(j).proof for j in ids] == want, (claim, fold)
+        assert pool.tuning()["claim_items"] == 1 << 18
+        with pytest.raises(RuntimeError):
+            pool.fix_schedule(1 << 20, 1024)

@@ -63,6 +63,12 @@ static_assert(SZ2 - SZ1 >= kMaxNin && SER - SZ2 >= 16 && SLC - SER >= 16 &&
 constexpr int kVals = 32;        // published Fp2 values per frame
 constexpr int kSmall = 64;       // Fp2 of each frame's small device buffer
 constexpr int kElemCap = 1024;   // workgroups per frame of an elementwise launch
+// Autotuned schedule candidates (the sums are exact, so every choice gives the
+// same bits): the early claim's (pair, gate chunk) items per frame, and the
+// workgroups per frame of the fold launches.
+constexpr long long kClaimCand[3] = {1 << 14, 1 << 16, 1 << 18};
+constexpr int kFoldCand[3] = {256, 1024, 4096};
+constexpr long long kTuneMinItems = 1 << 22;   // batches smaller than this are not trials
 constexpr int kMleCap = 256;     // workgroups per frame of the input MLE
 
 struct FDesc {
@@ -336,10 +342,10 @@ __global__ void __launch_bounds__(kBlock) k_b_mle(const FDesc* __restrict__ D,
 
 size_t al(size_t n) { return (n + 255) & ~(size_t)255; }
 
-// Early claim work of round rd.
-ClaimShape claim_of(const Shape& S, int rd)
+// Early claim work of round rd with `items` (pair, chunk) items per frame.
+ClaimShape claim_of(const Shape& S, int rd, long long items)
 {
-    return claim_shape(S.n_gates, (S.cp >> rd) >> 1, kClaimItems);
+    return claim_shape(S.n_gates, (S.cp >> rd) >> 1, items);
 }
 
 int mle_blocks(const Shape& S)
@@ -352,7 +358,10 @@ int mle_blocks(const Shape& S)
 size_t partial_fp2(const Shape& S)
 {
     size_t n = (size_t)mle_blocks(S);
-    for (int rd = 0; rd < S.ncb; ++rd) n = std::max(n, (size_t)claim_of(S, rd).blocks * 4);
+    // sized for the largest claim candidate, so a frame's footprint does not
+    // depend on the schedule the tuner picks
+    for (int rd = 0; rd < S.ncb; ++rd)
+        n = std::max(n, (size_t)claim_of(S, rd, kClaimCand[2]).blocks * 4);
     return n;
 }
 
@@ -430,10 +439,10 @@ struct Sched {
     void end(Launch& l) { items.push_back({-1, l.blocks, 0, 0}); }
 };
 
-int elem_blocks(size_t n)
+int elem_blocks(size_t n, int cap = kElemCap)
 {
     const size_t b = (n + kBlock - 1) / kBlock;
-    return (int)(b < 1 ? 1 : (b > (size_t)kElemCap ? (size_t)kElemCap : b));
+    return (int)(b < 1 ? 1 : (b > (size_t)cap ? (size_t)cap : b));
 }
 
 }  // namespace
@@ -444,8 +453,57 @@ size_t Shape::footprint() const
            tables_layout(*this, nullptr, nullptr);
 }
 
+// The schedule's two knobs and their autotuner: coordinate descent over the
+// candidates on real batches, cost = device time of the claim launches per
+// (pair, gate) term and of the fold launches per algorithmic byte, each
+// measured with events around those launches; first the claim candidates
+// (fold at its default), then the fold candidates with the best claim.
+struct Knobs {
+    long long claim_items = kClaimItems;
+    int fold_cap = kElemCap;
+};
+
+struct Tuner {
+    int phase = 0;          // 0: claim candidates, 1: fold candidates, 2: settled
+    int trial = 0;          // candidate under trial in this phase
+    int best_c = 1, best_f = 1;
+    double cost_c[3] = {0, 0, 0}, cost_f[3] = {0, 0, 0};
+    bool fixed = false;     // knobs set by the caller
+    Knobs fixed_knobs;
+
+    Knobs pick() const
+    {
+        if (fixed) return fixed_knobs;
+        Knobs k;
+        k.claim_items = kClaimCand[phase == 0 ? trial : best_c];
+        k.fold_cap = kFoldCand[phase == 1 ? trial : best_f];
+        return k;
+    }
+    // one batch's costs under knobs k (ns per claim term, ns per fold byte)
+    void record(double claim_cost, double fold_cost)
+    {
+        if (fixed || phase == 2) return;
+        if (phase == 0) {
+            cost_c[trial] = claim_cost;
+            if (++trial == 3) {
+                best_c = (int)(std::min_element(cost_c, cost_c + 3) - cost_c);
+                phase = 1;
+                trial = 0;
+            }
+        } else {
+            cost_f[trial] = fold_cost;
+            if (++trial == 3) {
+                best_f = (int)(std::min_element(cost_f, cost_f + 3) - cost_f);
+                phase = 2;
+            }
+        }
+    }
+};
+
 struct BatchCtx {
     hipStream_t st = nullptr;
+    Tuner tuner;
+    std::vector<hipEvent_t> tev;   // timing events around claim and fold launches
     DevBuf arena;
     uint8_t* h_in = nullptr;
     size_t h_in_bytes = 0;
@@ -494,6 +552,15 @@ struct BatchCtx {
         map_frames = nf;
         return true;
     }
+    bool timing_events(int n)
+    {
+        while ((int)tev.size() < n) {
+            hipEvent_t e;
+            TP_HIP_OK(hipEventCreate(&e));
+            tev.push_back(e);
+        }
+        return true;
+    }
     bool events(int n)
     {
         while ((int)ev.size() < n) {
@@ -504,6 +571,26 @@ struct BatchCtx {
         return true;
     }
 };
+
+void batch_tuning(const BatchCtx& B, double state[9])
+{
+    const Tuner& t = B.tuner;
+    const Knobs k = t.pick();
+    state[0] = t.fixed ? 2 : t.phase;
+    state[1] = (double)k.claim_items;
+    state[2] = (double)k.fold_cap;
+    for (int i = 0; i < 3; ++i) {
+        state[3 + i] = t.cost_c[i];
+        state[6 + i] = t.cost_f[i];
+    }
+}
+
+void batch_fix_knobs(BatchCtx& B, long long claim_items, int fold_cap)
+{
+    B.tuner.fixed = true;
+    B.tuner.fixed_knobs.claim_items = claim_items;
+    B.tuner.fixed_knobs.fold_cap = fold_cap;
+}
 
 namespace {
 
@@ -579,6 +666,7 @@ void batch_ctx_destroy(BatchCtx* B)
     if (B->h_stage) (void)hipHostFree(B->h_stage);
     if (B->h_map) (void)hipHostFree(B->h_map);
     for (hipEvent_t e : B->ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : B->tev) (void)hipEventDestroy(e);
     if (B->st) (void)hipStreamDestroy(B->st);
     delete B;
 }
@@ -647,7 +735,8 @@ bool prove_batch(BatchCtx& B, std::vector<Job*>& jobs_in, TaskPool& tasks, doubl
         max_nz = std::max(max_nz, S[f].n_z);
     }
 
-    // 2. the schedule (every launch's items) from the shapes alone
+    // 2. the schedule (every launch's items) from the shapes and the knobs
+    const Knobs kn = B.tuner.pick();
     Sched sc;
     Launch L_rows = sc.begin();
     for (int f = 0; f < nf; ++f) sc.add(L_rows, f, elem_blocks(S[f].rows_n));
@@ -671,25 +760,28 @@ bool prove_batch(BatchCtx& B, std::vector<Job*>& jobs_in, TaskPool& tasks, doubl
     }
     sc.end(L_eq);
     std::vector<Launch> L_claim(max_ncb), L_pub4(max_ncb), L_fold(max_ncb);
+    double claim_terms = 0, fold_bytes = 0;
     for (int rd = 0; rd < max_ncb; ++rd) {
         L_claim[rd] = sc.begin();
         for (int f = 0; f < nf; ++f) {
             if (rd >= S[f].ncb) continue;
-            const ClaimShape c = claim_of(S[f], rd);
+            const ClaimShape c = claim_of(S[f], rd, kn.claim_items);
             sc.add(L_claim[rd], f, (int)c.blocks, c.n_chunks, c.gpt);
+            claim_terms += (double)((S[f].cp >> rd) >> 1) * S[f].n_gates;
         }
         sc.end(L_claim[rd]);
         L_pub4[rd] = sc.begin();
         for (int f = 0; f < nf; ++f) {
             if (rd >= S[f].ncb) continue;
-            sc.add(L_pub4[rd], f, 1, (int)claim_of(S[f], rd).blocks);
+            sc.add(L_pub4[rd], f, 1, (int)claim_of(S[f], rd, kn.claim_items).blocks);
         }
         sc.end(L_pub4[rd]);
         L_fold[rd] = sc.begin();
         for (int f = 0; f < nf; ++f) {
             if (rd >= S[f].ncb) continue;
             const size_t total = ((size_t)S[f].NW + 1) * (size_t)((S[f].cp >> rd) >> 1);
-            sc.add(L_fold[rd], f, elem_blocks(total));
+            sc.add(L_fold[rd], f, elem_blocks(total, kn.fold_cap));
+            fold_bytes += 48.0 * (double)total;
         }
         sc.end(L_fold[rd]);
     }
@@ -897,12 +989,16 @@ bool prove_batch(BatchCtx& B, std::vector<Job*>& jobs_in, TaskPool& tasks, doubl
     TP_HIP_OK(hipGetLastError());
     lap(2);
 
-    // 7. early rounds in lockstep (A6, A7)
+    // 7. early rounds in lockstep (A6, A7); the claim and fold launches are
+    // timed for the schedule tuner
+    if (!B.timing_events(4 * max_ncb)) return false;
     for (int rd = 0; rd < max_ncb; ++rd) {
         const Launch& lc = L_claim[rd];
         const Launch& lp = L_pub4[rd];
         const Launch& lf = L_fold[rd];
+        TP_HIP_OK(hipEventRecord(B.tev[4 * rd], st));
         k_b_claim<<<lc.blocks, kBlock, 0, st>>>(dD, dI + lc.first, lc.n, rd);
+        TP_HIP_OK(hipEventRecord(B.tev[4 * rd + 1], st));
         const unsigned s = ++B.seq;
         k_b_pub<4><<<lp.n, kBlock, 0, st>>>(dD, dI + lp.first, dv, dseq, s);
         TP_HIP_OK(hipGetLastError());
@@ -918,7 +1014,9 @@ bool prove_batch(BatchCtx& B, std::vector<Job*>& jobs_in, TaskPool& tasks, doubl
             hch[(size_t)(SER + rd) * nf + f] = r;
         });
         if (!pull(SER + rd, 1)) return false;
+        TP_HIP_OK(hipEventRecord(B.tev[4 * rd + 2], st));
         k_b_fold<<<lf.blocks, kBlock, 0, st>>>(dD, dI + lf.first, lf.n, rd, dch, nf);
+        TP_HIP_OK(hipEventRecord(B.tev[4 * rd + 3], st));
         TP_HIP_OK(hipGetLastError());
     }
     lap(3);
@@ -984,6 +1082,20 @@ bool prove_batch(BatchCtx& B, std::vector<Job*>& jobs_in, TaskPool& tasks, doubl
     }
     lap(6);
     prof[BP_ROUNDS_WALL] += ms_since(t_rounds);
+    {   // the schedule tuner's costs of this batch (its early rounds are done)
+        double claim_ms = 0, fold_ms = 0;
+        for (int rd = 0; rd < max_ncb; ++rd) {
+            float a = 0, b = 0;
+            TP_HIP_OK(hipEventElapsedTime(&a, B.tev[4 * rd], B.tev[4 * rd + 1]));
+            TP_HIP_OK(hipEventElapsedTime(&b, B.tev[4 * rd + 2], B.tev[4 * rd + 3]));
+            claim_ms += a;
+            fold_ms += b;
+        }
+        prof[BP_CLAIM_DEV] += claim_ms;
+        prof[BP_FOLD_DEV] += fold_ms;
+        if (claim_terms >= (double)kTuneMinItems)
+            B.tuner.record(claim_ms * 1e6 / claim_terms, fold_ms * 1e6 / fold_bytes);
+    }
     prof[BP_STEP_WAIT] += step_wait;
     prof[BP_STEP_HOST_WALL] += step_host;
     auto t_rest = Clock::now();

@@ -946,6 +946,9 @@ struct Pool {
     size_t budget = 0, in_use = 0, peak = 0;
     uint64_t batches = 0, batched_frames = 0;
     double prof[BP_COUNT] = {};
+    double tuning[9] = {};
+    long long fix_claim = 0;   // > 0: the schedule is fixed by the caller
+    int fix_fold = 0;
     std::unique_ptr<TaskPool> tasks;
 };
 
@@ -1042,6 +1045,10 @@ void batch_main(Pool* pool)
             J->state.store(1);
         }
         clear_error();
+        if (B) {
+            std::lock_guard<std::mutex> lk(pool->mu);
+            if (pool->fix_claim > 0) batch_fix_knobs(*B, pool->fix_claim, pool->fix_fold);
+        }
         double prof[BP_COUNT] = {};
         const bool dev_ok = B && prove_batch(*B, batch, *pool->tasks, prof);
         const std::string dev_err = dev_ok ? std::string() : (B ? std::string(tp_last_error())
@@ -1051,6 +1058,7 @@ void batch_main(Pool* pool)
             std::lock_guard<std::mutex> lk(pool->mu);
             pool->in_use -= bytes;
             for (int k = 0; k < BP_COUNT; ++k) pool->prof[k] += prof[k];
+            if (B) batch_tuning(*B, pool->tuning);
             for (Job* J : batch) {
                 J->t_end = t1;
                 if (!dev_ok && J->err.empty()) J->err = dev_err;
@@ -1429,6 +1437,37 @@ int tp_pool_batch_profile(void* pool, double* out, int n)
     std::lock_guard<std::mutex> lk(p->mu);
     for (int k = 0; k < n && k < BP_COUNT; ++k) out[k] = p->prof[k];
     return BP_COUNT;
+}
+
+// The lockstep schedule tuner of a batched pool (the driver that ran the
+// last batch): phase (0 trying claim sizes, 1 trying fold sizes, 2 settled),
+// the claim items per frame and fold workgroups per frame in use, the claim
+// costs (ns per term) and fold costs (ns per byte) of the three candidates
+// each.  Writes min(n, 9) values; returns 9, -1 on a null pool.
+int tp_pool_tuning(void* pool, double* out, int n)
+{
+    auto* p = static_cast<Pool*>(pool);
+    if (!p) return -1;
+    std::lock_guard<std::mutex> lk(p->mu);
+    for (int k = 0; k < n && k < 9; ++k) out[k] = p->tuning[k];
+    return 9;
+}
+
+// Fixes the lockstep schedule of a batched pool from the next batch on
+// (claim items per frame in [1024, 2^18], fold workgroups per frame in
+// [16, 65536]); the tuner stops.  0 / -1.
+int tp_pool_fix_schedule(void* pool, long long claim_items, int fold_cap)
+{
+    clear_error();
+    auto* p = static_cast<Pool*>(pool);
+    if (!p || !p->batched) return fail("tp_pool_fix_schedule: not a batched pool"), -1;
+    if (claim_items < 1024 || claim_items > (1 << 18) || fold_cap < 16 || fold_cap > 65536)
+        return fail("tp_pool_fix_schedule: claim_items=%lld fold_cap=%d out of range",
+                    claim_items, fold_cap), -1;
+    std::lock_guard<std::mutex> lk(p->mu);
+    p->fix_claim = claim_items;
+    p->fix_fold = fold_cap;
+    return 0;
 }
 
 // Circuit outputs [cp][2^out_bits] of a finished job submitted with

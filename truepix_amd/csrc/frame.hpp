@@ -185,6 +185,8 @@ enum BatchProf {
     BP_STEP_WAIT,         // driver waiting for published sums in those steps
     BP_STEP_HOST_WALL,    // per-step transcript work (parallel) in those steps
     BP_REST_WALL,         // waiting for the prefixes and writing the proof ends
+    BP_CLAIM_DEV,         // device time of the early-claim launches (events)
+    BP_FOLD_DEV,          // device time of the early-fold launches (events)
     BP_COUNT
 };
 
@@ -193,6 +195,12 @@ enum BatchProf {
 // after which every frame of the batch is failed.  prof (BP_COUNT doubles)
 // accumulates the batch's timings.
 bool prove_batch(BatchCtx& B, std::vector<Job*>& jobs, TaskPool& tasks, double* prof);
+// The schedule tuner of a driver: state[0] phase (2 = settled), [1] claim
+// items per frame, [2] fold workgroups per frame (the knobs in use next),
+// [3..5] claim costs (ns per term) and [6..8] fold costs (ns per byte) of the
+// candidates tried.  Fixing the knobs (claim_items > 0) stops the tuning.
+void batch_tuning(const BatchCtx& B, double state[9]);
+void batch_fix_knobs(BatchCtx& B, long long claim_items, int fold_cap);
 
 }  // namespace frame
 }  // namespace tp

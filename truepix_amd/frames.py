@@ -156,7 +156,23 @@ class FramePool:
 
     BATCH_PROFILE = ("batches", "frames", "stage_wall_ms", "launch_wall_ms", "head_wall_ms",
                      "head_wait_sum_ms", "head_cpu_sum_ms", "prefix_cpu_sum_ms", "rounds_wall_ms",
-                     "step_wait_ms", "step_host_wall_ms", "rest_wall_ms")
+                     "step_wait_ms", "step_host_wall_ms", "rest_wall_ms", "claim_device_ms",
+                     "fold_device_ms")
+
+    def tuning(self) -> dict:
+        """The lockstep schedule tuner: phase (2 = settled), the knobs in use
+        and the measured cost of every candidate."""
+        buf = (ctypes.c_double * 9)()
+        if self._lib.tp_pool_tuning(self.handle, buf, 9) < 0:
+            raise RuntimeError("tp_pool_tuning failed")
+        return {"phase": int(buf[0]), "claim_items": int(buf[1]), "fold_cap": int(buf[2]),
+                "claim_ns_per_term": [round(buf[3 + i], 4) for i in range(3)],
+                "fold_ns_per_byte": [round(buf[6 + i], 6) for i in range(3)]}
+
+    def fix_schedule(self, claim_items: int, fold_cap: int) -> None:
+        """Fix the lockstep schedule (the tuner stops); proofs do not change."""
+        if self._lib.tp_pool_fix_schedule(self.handle, int(claim_items), int(fold_cap)) != 0:
+            raise RuntimeError("tp_pool_fix_schedule failed: %s" % nb.last_error())
 
     def batch_profile(self) -> dict:
         """Where the batches of a batched pool spent their time (accumulated)."""

@@ -121,6 +121,8 @@ def bind(lib) -> None:
                                      ctypes.POINTER(_SIZE)], _I32)
     _sig(lib, "tp_pool_memory", [_VP, _U64P, _U64P, _U64P, _U64P, _U64P], _I32)
     _sig(lib, "tp_pool_batch_profile", [_VP, ctypes.POINTER(_D), _I32], _I32)
+    _sig(lib, "tp_pool_tuning", [_VP, ctypes.POINTER(_D), _I32], _I32)
+    _sig(lib, "tp_pool_fix_schedule", [_VP, ctypes.c_longlong, _I32], _I32)
     _sig(lib, "tp_pool_destroy", [_VP])
     _sig(lib, "tp_pool_now", [_VP], _D)
     _sig(lib, "tp_pool_submit", [_VP, ctypes.POINTER(FrameJob)], ctypes.c_int64)
