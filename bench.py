@@ -4,11 +4,14 @@
 
 One step = one full 24-round prove (round sums, fused folds, host transcript)
 from tables already resident in HBM.  Prints ONE JSON line:
-  value      whole-job fold HBM throughput: algorithmic bytes of all rounds
-             (32 n for round 0, then 48 x the input length of every fused
-             fold+round launch, plus the final fold) / wall time per step;
-  roofline   the fused fold+round kernel: its algorithmic bytes / its summed
-             HIP-event time (events on the stream it runs on), against 8 TB/s;
+  value      whole-job fold HBM throughput: the algorithmic bytes of a prove,
+             48 (2^25 - 2) B (round r reads 2 x 2^(24-r) x 16 B and writes
+             half of that, whatever the schedule), / wall time per step;
+  roofline   the launch class with the most device time (the first pass over
+             the input, the round-2 fold from the input, or the streaming
+             fold+round launches): the bytes its launches read and write / its
+             summed HIP-event time (events on the stream it runs on), against
+             8 TB/s;
   cpu_baseline  the CPU oracle (oracle/fp2_oracle.c, a port) proving the same
              2 x 2^24 workload single-threaded on this host, repeated for at
              least 10 s, in GB/s of the same kind;
@@ -33,26 +36,29 @@ NV = 24
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sc2_fold_round_traffic.json")
 
 
-def measured_traffic():
-    """HBM bytes per k_sc2_fold_round launch from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (each scaled by the calibration copy), or None."""
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (each scaled by the calibration copy), or
+    None when the summary covers another kernel."""
     try:
         with open(TRAFFIC_FILE) as fh:
-            return json.load(fh)["bytes_per_launch"]
-    except (OSError, KeyError, ValueError):
+            rec = json.load(fh)
+    except (OSError, ValueError):
         return None
+    if rec.get("kernel", "k_sc2_fold_round") != kernel:
+        return None
+    return rec.get("bytes_per_launch")
 
 
 def alg_bytes(nv: int) -> float:
-    """Algorithmic HBM bytes of one nv-round prove (both tables, 16 B per Fp2)."""
-    n = 1 << nv
-    total = 32.0 * n                       # round 0 reads both tables
-    m = n
-    for _ in range(1, nv):                 # fused launches read T_{k-1} (m) and write m/2
-        total += 48.0 * m
-        m //= 2
-    total += 32.0 * 2                      # final fold reads the last pair
-    return total
+    """Algorithmic HBM bytes of one nv-round prove (both tables, 16 B per Fp2):
+    round r reads the 2 x 2^(nv-r) entries of its tables and writes the
+    2 x 2^(nv-r-1) folded ones, 48 x 2^(nv-r) B, summed over the rounds."""
+    return 48.0 * ((1 << (nv + 1)) - 2)
+
+
+SC2_KERNELS = {"first": "k_sc2_first01", "fold01": "k_sc2_fold01_round",
+               "fold": "k_sc2_fold_round", "tail": "k_sc2_tail_host"}
 
 
 def bench_rs_encode(steps: int = 5, cols: int = 64, lc: int = 20, lo: int = 22):
@@ -502,23 +508,30 @@ def main(argv=None):
     ms_per_step = dt / a.steps * 1e3
     value = alg_bytes(NV) / (dt / a.steps) / 1e9
 
-    # roofline of the fused fold+round kernel: events around each launch,
-    # in separate (untimed) runs so event recording does not perturb `value`
+    # roofline per launch class: events around each launch, in separate
+    # (untimed) runs so event recording does not perturb `value`
     sc.set_timing(True)
-    k_ms = k_bytes = 0.0
-    k_n = 0
-    t_ms, t_n = 0.0, 0
-    for _ in range(max(3, a.steps // 4)):
+    reps = max(3, a.steps // 4)
+    acc = {c: {"ms": 0.0, "bytes": 0.0, "launches": 0} for c in sc.CLASSES}
+    for _ in range(reps):
         sc.prove_deg2_device(df.data_ptr(), dg.data_ptr(), NV, seed=2024, stream=stream)
-        ms, by, n = sc.stats()
-        k_ms += ms
-        k_bytes += by
-        k_n += n
-        tm, tn = sc.tail_stats()
-        t_ms += tm
-        t_n += tn
+        for c, v in sc.class_stats().items():
+            for key in v:
+                acc[c][key] += v[key]
+    tail_prof = sc.tail_profile()
     sc.set_timing(False)
-    achieved = k_bytes / (k_ms / 1e3) / 1e9
+    classes = {}
+    for c, v in acc.items():
+        if v["launches"]:
+            classes[c] = {"kernel": SC2_KERNELS[c], "launches_per_prove": v["launches"] // reps,
+                          "avg_launch_us": round(v["ms"] / v["launches"] * 1e3, 2),
+                          "bytes_per_launch": round(v["bytes"] / v["launches"], 1),
+                          "ms_per_prove": round(v["ms"] / reps, 4),
+                          "achieved_gbs": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1)}
+    if "tail" in classes:
+        classes["tail"]["per_round"] = tail_prof
+    dom = max((c for c in classes if c != "tail"), key=lambda c: acc[c]["ms"])
+    achieved = acc[dom]["bytes"] / (acc[dom]["ms"] / 1e3) / 1e9
 
     line = {
         "metric": METRIC,
@@ -539,16 +552,17 @@ def main(argv=None):
                    "alg_bytes_per_prove": alg_bytes(NV)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": measured_traffic(),
-                     "kernel": "k_sc2_fold_round",
-                     "avg_launch_us": round(k_ms / k_n * 1e3, 2),
-                     "launches_per_prove": k_n // max(1, max(3, a.steps // 4)),
-                     "alg_bytes_per_launch": round(k_bytes / k_n, 1),
-                     "tail_kernel": {"name": "k_sc2_tail_round",
-                                     "launches_per_prove": t_n // max(1, max(3, a.steps // 4)),
-                                     "avg_launch_us": round(t_ms / max(1, t_n) * 1e3, 2),
-                                     "note": "rounds with <= 1024 input entries per table: fold, "
-                                             "round sum and publish in one workgroup (latency-bound)"}},
+                     "traffic": measured_traffic(SC2_KERNELS[dom]),
+                     "kernel": SC2_KERNELS[dom],
+                     "avg_launch_us": classes[dom]["avg_launch_us"],
+                     "launches_per_prove": classes[dom]["launches_per_prove"],
+                     "alg_bytes_per_launch": classes[dom]["bytes_per_launch"],
+                     "classes": classes,
+                     "note": "first: rounds 0 and 1 from one read of the input; fold01: round 2 "
+                             "folds the input by r_0 and r_1 into T_2; fold: streaming "
+                             "fold+round launches; tail: one workgroup proving every round "
+                             "whose input has <= 8192 entries per table, the host answering "
+                             "each challenge through mapped memory (latency-bound)"},
         "check": {"round0_poly": [int(x) for x in proof.polys[0][:2]]},
     }
     def batched():

@@ -24,7 +24,7 @@ def transcript_chal(seed):
     return fn
 
 
-@pytest.mark.parametrize("nv", [1, 2, 3, 5, 9, 12, 16])
+@pytest.mark.parametrize("nv", [1, 2, 3, 4, 5, 9, 10, 11, 12, 16])
 def test_sc2_fixed_challenges(bridge, oracle, torch_cuda, nv):
     rng = np.random.default_rng(nv)
     f, g = rand_fp2(rng, 1 << nv), rand_fp2(rng, 1 << nv)
@@ -36,7 +36,7 @@ def test_sc2_fixed_challenges(bridge, oracle, torch_cuda, nv):
     assert np.array_equal(got.final, want[2])
 
 
-@pytest.mark.parametrize("nv,seed", [(1, 0), (4, 7), (13, 123456789)])
+@pytest.mark.parametrize("nv,seed", [(1, 0), (2, 3), (3, 99), (4, 7), (13, 123456789)])
 def test_sc2_transcript(bridge, oracle, torch_cuda, nv, seed):
     rng = np.random.default_rng(seed)
     f, g = rand_fp2(rng, 1 << nv), rand_fp2(rng, 1 << nv)
@@ -67,3 +67,28 @@ def test_sc2_full_size(bridge, oracle, torch_cuda):
     assert np.array_equal(got.polys, want[0])
     assert np.array_equal(got.chals, want[1])
     assert np.array_equal(got.final, want[2])
+
+
+def test_sc2_launch_classes(bridge, torch_cuda):
+    """Rounds 0 and 1 share one pass over the input and round 2 folds the
+    input straight into T_2: per class the bytes the launches move."""
+    nv = 16
+    rng = np.random.default_rng(1)
+    f, g = rand_fp2(rng, 1 << nv), rand_fp2(rng, 1 << nv)
+    sc.set_timing(True)
+    try:
+        sc.prove_deg2_host(f, g, seed=1)
+        cls = sc.class_stats()
+    finally:
+        sc.set_timing(False)
+    n = 1 << nv
+    assert cls["first"]["launches"] == 1 and cls["first"]["bytes"] == 32 * n
+    assert cls["fold01"]["launches"] == 1 and cls["fold01"]["bytes"] == 40 * n
+    # rounds 3.. : streaming launches while the input T_{k-1} has more than
+    # 8192 entries, then one workgroup proves the rest from its input
+    big = [k for k in range(3, nv) if (n >> (k - 1)) > 8192]
+    assert cls["fold"]["launches"] == len(big)
+    assert cls["fold"]["bytes"] == sum(48 * (n >> (k - 1)) for k in big)
+    k0 = big[-1] + 1 if big else 3
+    assert cls["tail"]["launches"] == 1 and cls["tail"]["bytes"] == 32 * (n >> (k0 - 1))
+    assert all(c["ms"] > 0 for c in cls.values() if c["launches"])

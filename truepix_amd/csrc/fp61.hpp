@@ -80,7 +80,36 @@ TP_HD fp2 mul2(fp2 x, fp2 y)
 TP_HD fp2 smul2(u64 s, fp2 y) { return mk2(fmul(s, y.re), fmul(s, y.im)); }
 
 // a + r (b - a): the sumcheck fold (TP/native_fp2/native_fp2_core.hpp:53-64).
-TP_HD fp2 fold2(fp2 a, fp2 b, fp2 r) { return add2(a, mul2(r, sub2(b, a))); }
+// The add of a joins the Karatsuba fold, one reduction per coordinate:
+// re = a.re + ac + (2p - bd) < 2^64, im = a.im + k + (4p - ac - bd) < 2^64.
+TP_HD fp2 fold2(fp2 a, fp2 b, fp2 r)
+{
+    const fp2 d = sub2(b, a);
+    const u64 ac = fmul_lazy(r.re, d.re);
+    const u64 bd = fmul_lazy(r.im, d.im);
+    const u64 k = fmul_lazy(r.re + r.im, d.re + d.im);
+    return mk2(fold61(a.re + ac + (2 * P - bd)), fold61(a.im + k + (4 * P - ac - bd)));
+}
+
+// Sum of Fp2 products kept lazily: the Karatsuba products ac, bd and
+// k = (a + b)(c + d) summed separately, each sum folded once per add so it
+// stays below 2^61 + 8; value() reduces once at the end.
+struct LazyProdSum {
+    u64 ac, bd, k;
+    TP_HD void zero() { ac = bd = k = 0; }
+    TP_HD static u64 acc(u64 s, u64 t) { s += t; return (s & P) + (s >> 61); }
+    TP_HD void mac(fp2 x, fp2 y)
+    {
+        ac = acc(ac, fmul_lazy(x.re, y.re));
+        bd = acc(bd, fmul_lazy(x.im, y.im));
+        k = acc(k, fmul_lazy(x.re + x.im, y.re + y.im));
+    }
+    TP_HD fp2 value() const
+    {
+        const u64 a = fold61(ac), b = fold61(bd), kk = fold61(k);
+        return mk2(fsub(a, b), fsub(fsub(kk, a), b));
+    }
+};
 
 // Gate functions, type indices as in TP/libTruePix/native_fp2_bridge.py:450-472
 // and TP/native_fp2/native_fp2_core.hpp:32-51.

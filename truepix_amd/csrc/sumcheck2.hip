@@ -9,12 +9,26 @@
 //   read T_{k-1} (m values per table), write T_k (m/2), accumulate
 //   E0 = sum F0 G0, E1 = sum F1 G1, EA = sum dF dG  ->  a = EA, c = E0,
 //   b = E1 - EA - E0.
-// Bytes per fused launch: 32 m read + 16 m written (both tables), the first
-// launch reads 32 n and writes nothing.  Tables are AoS {re, im} so every
-// element is one 128-bit load; a thread owns one output pair (64 B per table).
+// Bytes per fused launch: 32 m read + 16 m written (both tables).  Tables are
+// AoS {re, im} so every element is one 128-bit load; a thread owns one output
+// pair (64 B per table).
+//
+// Rounds 0 and 1 share one pass over the input (nv >= 3).  T_1 = A + r_0 B is
+// linear in r_0, so round 1's sums are quadratics in r_0 whose coefficients
+// are sums over the input:
+//   E0_1(r) = S00 + r (S11 - S00 - SAA) + r^2 SAA   (even round-0 pairs)
+//   E1_1(r) = S22 + r (S33 - S22 - SBB) + r^2 SBB   (odd round-0 pairs)
+//   EA_1(r) = X0  + r (X1  - X0  - X2 ) + r^2 X2
+// with, per input quad f0..f3 / g0..g3 and d = f1 - f0, d' = f3 - f2 (e, e'
+// for g): S00 = sum f0 g0, S11 = sum f1 g1, SAA = sum d e, S22, S33, SBB the
+// same over (f2, f3), X0 = sum (f2 - f0)(g2 - g0), X1 = sum (f3 - f1)(g3 - g1),
+// X2 = sum (d' - d)(e' - e).  Round 0's sums are S00 + S22, S11 + S33 and
+// SAA + SBB.  The first pass reads the input once (32 n bytes) and yields both
+// polynomials; the second pass folds the input by r_0 and r_1 straight into
+// T_2 (read 32 n, write 8 n), so T_1 is never written or read.
 //
 // Round trip per round: fused launch -> one-workgroup finalize launch that
-// writes (E0, E1, EA) and a sequence number straight into mapped pinned host
+// writes the sums and a sequence number straight into mapped pinned host
 // memory -> the host polls the sequence number, runs the transcript, and
 // launches the next round with the challenge as a kernel argument.
 #include "fp61.hpp"
@@ -25,6 +39,7 @@
 
 #include <atomic>
 #include <chrono>
+#include <cstring>
 #include <stdint.h>
 #include <thread>
 
@@ -37,11 +52,30 @@ struct alignas(16) V2 { u64 re, im; };
 __device__ __forceinline__ fp2 ldv(const V2* p) { const V2 v = *p; return mk2(v.re, v.im); }
 __device__ __forceinline__ void stv(V2* p, fp2 x) { V2 v; v.re = x.re; v.im = x.im; *p = v; }
 
-__device__ __forceinline__ void accum_pair(fp2 (&e)[3], fp2 f0, fp2 f1, fp2 g0, fp2 g1)
+// Round sums of one pair of T_k, lazily: E0 += f0 g0, E1 += f1 g1,
+// EA += (f1 - f0)(g1 - g0).
+__device__ __forceinline__ void accum_pair(LazyProdSum (&z)[3], fp2 f0, fp2 f1, fp2 g0, fp2 g1)
 {
-    e[0] = add2(e[0], mul2(f0, g0));
-    e[1] = add2(e[1], mul2(f1, g1));
-    e[2] = add2(e[2], mul2(sub2(f1, f0), sub2(g1, g0)));
+    z[0].mac(f0, g0);
+    z[1].mac(f1, g1);
+    z[2].mac(sub2(f1, f0), sub2(g1, g0));
+}
+
+template <int N>
+__device__ __forceinline__ void lazy_zero(LazyProdSum (&z)[N])
+{
+#pragma unroll
+    for (int k = 0; k < N; ++k) z[k].zero();
+}
+
+// The workgroup's totals of N lazy sums into partials[block * N + k].
+template <int N>
+__device__ __forceinline__ void lazy_partial(const LazyProdSum (&z)[N], fp2* partials)
+{
+    fp2 e[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) e[k] = z[k].value();
+    block_partial<N>(e, partials);
 }
 
 // Round 0: no fold, sum over the pairs of the input tables; a thread owns two
@@ -50,7 +84,8 @@ __global__ void __launch_bounds__(256) k_sc2_first(const V2* __restrict__ f,
                                                    const V2* __restrict__ g, size_t n_pairs,
                                                    fp2* partials)
 {
-    fp2 e[3] = {zero2(), zero2(), zero2()};
+    LazyProdSum z[3];
+    lazy_zero(z);
     const size_t n_quads = n_pairs / 2;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n_quads;
          q += (size_t)gridDim.x * blockDim.x) {
@@ -58,16 +93,81 @@ __global__ void __launch_bounds__(256) k_sc2_first(const V2* __restrict__ f,
         const V2* gp = g + 4 * q;
         const fp2 fa = ldv(fp), fb = ldv(fp + 1), fc = ldv(fp + 2), fd = ldv(fp + 3);
         const fp2 ga = ldv(gp), gb = ldv(gp + 1), gc = ldv(gp + 2), gd = ldv(gp + 3);
-        accum_pair(e, fa, fb, ga, gb);
-        accum_pair(e, fc, fd, gc, gd);
+        accum_pair(z, fa, fb, ga, gb);
+        accum_pair(z, fc, fd, gc, gd);
     }
     if (n_pairs & 1) {   // one pair (n = 2) left over
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             const size_t p = n_pairs - 1;
-            accum_pair(e, ldv(f + 2 * p), ldv(f + 2 * p + 1), ldv(g + 2 * p), ldv(g + 2 * p + 1));
+            accum_pair(z, ldv(f + 2 * p), ldv(f + 2 * p + 1), ldv(g + 2 * p), ldv(g + 2 * p + 1));
         }
     }
-    block_partial<3>(e, partials);
+    lazy_partial<3>(z, partials);
+}
+
+// Rounds 0 and 1 in one pass (see the header): per quad the 9 sums S00, S11,
+// SAA, S22, S33, SBB, X0, X1, X2 (9 products per quad against 6 for round 0
+// alone), each kept lazily.
+__device__ __forceinline__ void accum_quad01(LazyProdSum (&z)[9], const V2* fp, const V2* gp)
+{
+    const fp2 f0 = ldv(fp), f1 = ldv(fp + 1), f2 = ldv(fp + 2), f3 = ldv(fp + 3);
+    const fp2 g0 = ldv(gp), g1 = ldv(gp + 1), g2 = ldv(gp + 2), g3 = ldv(gp + 3);
+    const fp2 d = sub2(f1, f0), dd = sub2(f3, f2), c = sub2(g1, g0), cc = sub2(g3, g2);
+    z[0].mac(f0, g0);
+    z[1].mac(f1, g1);
+    z[2].mac(d, c);
+    z[3].mac(f2, g2);
+    z[4].mac(f3, g3);
+    z[5].mac(dd, cc);
+    z[6].mac(sub2(f2, f0), sub2(g2, g0));
+    z[7].mac(sub2(f3, f1), sub2(g3, g1));
+    z[8].mac(sub2(dd, d), sub2(cc, c));
+}
+
+__global__ void __launch_bounds__(256) k_sc2_first01(const V2* __restrict__ f,
+                                                     const V2* __restrict__ g, size_t n_quads,
+                                                     fp2* partials)
+{
+    LazyProdSum z[9];
+    lazy_zero(z);
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n_quads; q += stride)
+        accum_quad01(z, f + 4 * q, g + 4 * q);
+    lazy_partial<9>(z, partials);
+}
+
+// Round 2 from the input: fold T_0 by r_0 and r_1 into T_2 and sum the
+// round-2 pairs.  Oct o = inputs 8o..8o+7 -> outputs 2o, 2o+1.
+__device__ __forceinline__ void fold_oct(const V2* p, fp2 r0, fp2 r1, fp2& o0, fp2& o1)
+{
+    const fp2 a0 = fold2(ldv(p), ldv(p + 1), r0), a1 = fold2(ldv(p + 2), ldv(p + 3), r0);
+    const fp2 a2 = fold2(ldv(p + 4), ldv(p + 5), r0), a3 = fold2(ldv(p + 6), ldv(p + 7), r0);
+    o0 = fold2(a0, a1, r1);
+    o1 = fold2(a2, a3, r1);
+}
+
+__global__ void __launch_bounds__(256) k_sc2_fold01_round(const V2* __restrict__ fin,
+                                                          const V2* __restrict__ gin,
+                                                          size_t n_octs, u64 r0r, u64 r0i,
+                                                          u64 r1r, u64 r1i,
+                                                          V2* __restrict__ fout,
+                                                          V2* __restrict__ gout, fp2* partials)
+{
+    const fp2 r0 = mk2(r0r, r0i), r1 = mk2(r1r, r1i);
+    LazyProdSum z[3];
+    lazy_zero(z);
+    for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < n_octs;
+         o += (size_t)gridDim.x * blockDim.x) {
+        fp2 f0, f1, g0, g1;
+        fold_oct(fin + 8 * o, r0, r1, f0, f1);
+        fold_oct(gin + 8 * o, r0, r1, g0, g1);
+        stv(fout + 2 * o, f0);
+        stv(fout + 2 * o + 1, f1);
+        stv(gout + 2 * o, g0);
+        stv(gout + 2 * o + 1, g1);
+        accum_pair(z, f0, f1, g0, g1);
+    }
+    lazy_partial<3>(z, partials);
 }
 
 // Round k >= 1: fold T_{k-1} (length 4 * n_quads) with r into T_k and sum
@@ -79,7 +179,8 @@ __global__ void __launch_bounds__(256) k_sc2_fold_round(const V2* __restrict__ f
                                                         V2* __restrict__ gout, fp2* partials)
 {
     const fp2 r = mk2(rr, ri);
-    fp2 e[3] = {zero2(), zero2(), zero2()};
+    LazyProdSum z[3];
+    lazy_zero(z);
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n_quads;
          q += (size_t)gridDim.x * blockDim.x) {
         const V2* fp = fin + 4 * q;
@@ -92,88 +193,201 @@ __global__ void __launch_bounds__(256) k_sc2_fold_round(const V2* __restrict__ f
         stv(fout + 2 * q + 1, f1);
         stv(gout + 2 * q, g0);
         stv(gout + 2 * q + 1, g1);
-        accum_pair(e, f0, f1, g0, g1);
+        accum_pair(z, f0, f1, g0, g1);
     }
-    block_partial<3>(e, partials);
+    lazy_partial<3>(z, partials);
 }
 
-// Rounds whose input table is at most one workgroup's quads (kTailQuads): fold,
-// round sum and publish in one launch, without the partials round trip.
-constexpr int kTailThreads = 256;
-constexpr size_t kTailQuads = 256;
-__global__ void __launch_bounds__(kTailThreads) k_sc2_tail_round(
-    const V2* __restrict__ fin, const V2* __restrict__ gin, size_t n_quads, u64 rr, u64 ri,
-    V2* __restrict__ fout, V2* __restrict__ gout, fp2* host_vals, volatile unsigned* host_seq,
-    unsigned seq)
+// The last rounds in one workgroup, the host answering each round's
+// challenge through mapped memory.  Entered at round k0 with T_{k0-1}
+// (len_in <= kTailIn entries per table) in global memory and r_{k0-1}: the
+// workgroup folds it into LDS and proves rounds k0..nv-1 there (T_k in place:
+// f at [0, kTailIn/2), g after it), then folds the last pair.  Per round,
+// lane 0 publishes (E0, E1, EA) and the sequence number seq0 + i like the
+// other launches, then waits for the host to write the challenge and the
+// same number into the answer slot; the final values go out as seq0 + nr.
+// The wait is bounded by the wall clock (spin_ticks): on expiry the kernel
+// publishes kSeqAbort and exits, so it ends whatever the host does.
+constexpr int kTailThreads = 1024;
+constexpr unsigned kTailIn = 8192;
+constexpr size_t kTailLds = (size_t)kTailIn * sizeof(V2);   // both tables, T_{k0}
+constexpr unsigned kSeqAbort = 0xffffffffu;
+
+// Lane 0 publishes a round's sums under sequence number `want` and waits for
+// the host's answer; returns 0 (and publishes kSeqAbort) if the wait expires.
+__device__ __forceinline__ int tail_exchange(const fp2 (&e)[3], unsigned want, fp2* host_vals,
+                                             unsigned* host_seq, const u64* answer,
+                                             unsigned* answer_seq, long long spin_ticks,
+                                             long long* trace, int i, u64 (&rv)[2])
 {
-    const fp2 r = mk2(rr, ri);
-    fp2 e[3] = {zero2(), zero2(), zero2()};
-    for (size_t q = threadIdx.x; q < n_quads; q += blockDim.x) {
-        const V2* fp = fin + 4 * q;
-        const V2* gp = gin + 4 * q;
-        const fp2 f0 = fold2(ldv(fp), ldv(fp + 1), r), f1 = fold2(ldv(fp + 2), ldv(fp + 3), r);
-        const fp2 g0 = fold2(ldv(gp), ldv(gp + 1), r), g1 = fold2(ldv(gp + 2), ldv(gp + 3), r);
-        stv(fout + 2 * q, f0);
-        stv(fout + 2 * q + 1, f1);
-        stv(gout + 2 * q, g0);
-        stv(gout + 2 * q + 1, g1);
-        accum_pair(e, f0, f1, g0, g1);
+    if (trace) trace[3 * i + 1] = wall_clock64();
+    host_vals[0] = e[0];
+    host_vals[1] = e[1];
+    host_vals[2] = e[2];
+    __threadfence_system();
+    __hip_atomic_store(host_seq, want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const long long t0 = wall_clock64();
+    while (__hip_atomic_load(answer_seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != want) {
+        if (wall_clock64() - t0 > spin_ticks) {
+            __hip_atomic_store(host_seq, kSeqAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            return 0;
+        }
+        __builtin_amdgcn_s_sleep(1);
     }
-    block_sum<3>(e);
-    if (threadIdx.x == 0) {
-        host_vals[0] = e[0];
-        host_vals[1] = e[1];
-        host_vals[2] = e[2];
+    if (trace) trace[3 * i + 2] = wall_clock64();
+    rv[0] = __hip_atomic_load(answer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    rv[1] = __hip_atomic_load(answer + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return 1;
+}
+
+// Fold of quad q of T_{k-1} (src) by r: the two T_k entries per table.
+__device__ __forceinline__ void tail_fold(const V2* sf, const V2* sg, unsigned q, fp2 r, fp2& f0,
+                                          fp2& f1, fp2& g0, fp2& g1)
+{
+    f0 = fold2(ldv(sf + 4 * q), ldv(sf + 4 * q + 1), r);
+    f1 = fold2(ldv(sf + 4 * q + 2), ldv(sf + 4 * q + 3), r);
+    g0 = fold2(ldv(sg + 4 * q), ldv(sg + 4 * q + 1), r);
+    g1 = fold2(ldv(sg + 4 * q + 2), ldv(sg + 4 * q + 3), r);
+}
+
+__global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
+    const V2* __restrict__ fin, const V2* __restrict__ gin, unsigned len_in, int k0, int nv,
+    u64 rr, u64 ri, fp2* host_vals, unsigned* host_seq, unsigned seq0, const u64* answer,
+    unsigned* answer_seq, long long spin_ticks, long long* trace)
+{
+    extern __shared__ V2 tl[];
+    __shared__ u64 s_r[2];
+    __shared__ int s_ok;
+    V2* tf = tl;
+    V2* tg = tl + kTailIn / 2;
+    const int t = threadIdx.x;
+    const int nr = nv - k0;
+    fp2 r = mk2(rr, ri);
+    unsigned len = len_in;   // entries of the input table T_{k-1}
+    int k = k0;
+    // Rounds with more than one wave's quads: the whole workgroup.
+    for (; k < nv && len / 4 > 64; ++k) {
+        const unsigned nq = len / 4;
+        if (trace && t == 0) trace[3 * (k - k0)] = wall_clock64();
+        LazyProdSum z[3];
+        lazy_zero(z);
+        if (k == k0) {
+            // from global memory straight into LDS (up to two quads a thread)
+            for (unsigned q = t; q < nq; q += kTailThreads) {
+                fp2 f0, f1, g0, g1;
+                tail_fold(fin, gin, q, r, f0, f1, g0, g1);
+                stv(tf + 2 * q, f0);
+                stv(tf + 2 * q + 1, f1);
+                stv(tg + 2 * q, g0);
+                stv(tg + 2 * q + 1, g1);
+                accum_pair(z, f0, f1, g0, g1);
+            }
+        } else {
+            // in place in LDS: at most one quad a thread (len <= kTailIn / 2)
+            const unsigned q = t;
+            fp2 f0, f1, g0, g1;
+            if (q < nq) {
+                tail_fold(tf, tg, q, r, f0, f1, g0, g1);
+                accum_pair(z, f0, f1, g0, g1);
+            }
+            __syncthreads();   // every read of T_{k-1} is done before T_k overwrites it
+            if (q < nq) {
+                stv(tf + 2 * q, f0);
+                stv(tf + 2 * q + 1, f1);
+                stv(tg + 2 * q, g0);
+                stv(tg + 2 * q + 1, g1);
+            }
+        }
+        fp2 e[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) e[i] = z[i].value();
+        block_sum<3>(e);
+        if (t == 0) {
+            u64 rv[2] = {0, 0};
+            s_ok = tail_exchange(e, seq0 + (unsigned)(k - k0), host_vals, host_seq, answer,
+                                 answer_seq, spin_ticks, trace, k - k0, rv);
+            s_r[0] = rv[0];
+            s_r[1] = rv[1];
+        }
+        __syncthreads();   // r_k and T_k visible to every thread
+        if (!s_ok) return;
+        r = mk2(s_r[0], s_r[1]);
+        len /= 2;
+    }
+    // The rest in wave 0 alone: its LDS reads of T_{k-1} precede its writes
+    // of T_k in program order, so no barrier is needed; r goes lane to lane.
+    if (t >= 64) return;
+    const V2* sf = k == k0 ? fin : tf;
+    const V2* sg = k == k0 ? gin : tg;
+    for (; k < nv; ++k) {
+        const unsigned nq = len / 4;
+        if (trace && t == 0) trace[3 * (k - k0)] = wall_clock64();
+        LazyProdSum z[3];
+        lazy_zero(z);
+        const unsigned q = t;
+        if (q < nq) {
+            fp2 f0, f1, g0, g1;
+            tail_fold(sf, sg, q, r, f0, f1, g0, g1);
+            stv(tf + 2 * q, f0);
+            stv(tf + 2 * q + 1, f1);
+            stv(tg + 2 * q, g0);
+            stv(tg + 2 * q + 1, g1);
+            accum_pair(z, f0, f1, g0, g1);
+        }
+        sf = tf;
+        sg = tg;
+        fp2 e[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) e[i] = z[i].value();
+        wave_sum<3>(e);
+        u64 rv[2] = {0, 0};
+        int ok = 1;
+        if (t == 0)
+            ok = tail_exchange(e, seq0 + (unsigned)(k - k0), host_vals, host_seq, answer, answer_seq,
+                               spin_ticks, trace, k - k0, rv);
+        if (!__shfl(ok, 0, 64)) return;
+        r = mk2(__shfl(rv[0], 0, 64), __shfl(rv[1], 0, 64));
+        len /= 2;
+    }
+    if (t == 0) {
+        // T_{nv-1}: two entries (in LDS, or still the input when no round ran here)
+        host_vals[0] = fold2(ldv(sf), ldv(sf + 1), r);
+        host_vals[1] = fold2(ldv(sg), ldv(sg + 1), r);
         __threadfence_system();
-        *host_seq = seq;
+        __hip_atomic_store(host_seq, seq0 + (unsigned)nr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
-// One workgroup: sum the partials and publish (E0, E1, EA) plus the sequence
-// number `seq` in host-mapped memory (values first, then a system-scope fence,
-// then the sequence number).
-__global__ void __launch_bounds__(256) k_sc2_publish(const fp2* partials, unsigned n_blocks,
-                                                     fp2* host_vals, volatile unsigned* host_seq,
-                                                     unsigned seq)
-{
-    fp2 acc[3];
-    sum_partials<3>(partials, n_blocks, acc);
-    if (threadIdx.x == 0) {
-        host_vals[0] = acc[0];
-        host_vals[1] = acc[1];
-        host_vals[2] = acc[2];
-        __threadfence_system();
-        *host_seq = seq;
-    }
-}
 
-// Final fold of the last pair: f~(r), g~(r), published like the round sums.
-__global__ void k_sc2_final(const V2* __restrict__ f, const V2* __restrict__ g, u64 rr, u64 ri,
-                            fp2* host_vals, volatile unsigned* host_seq, unsigned seq)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const fp2 r = mk2(rr, ri);
-        host_vals[0] = fold2(ldv(f), ldv(f + 1), r);
-        host_vals[1] = fold2(ldv(g), ldv(g + 1), r);
-        __threadfence_system();
-        *host_seq = seq;
-    }
-}
+// Event-timed launch classes of one prove (timing on).
+enum Sc2Class { SC_FIRST, SC_FOLD01, SC_FOLD, SC_TAIL, SC_NCLASS };
 
 struct Sc2State {
     DevBuf work;                 // T_k ping-pong: A (n/2) and B (n/4), for f and g
-    DevBuf partials;             // 3 fp2 per workgroup
-    fp2* host_vals = nullptr;    // mapped pinned: 3 values
+    DevBuf partials;             // up to 9 fp2 per workgroup
+    // mapped pinned memory: up to 16 values at 0, the sequence number at 256,
+    // the tail's answer (a challenge) at 512 and its sequence number at 528
+    fp2* host_vals = nullptr;
     unsigned* host_seq = nullptr;
+    u64* host_answer = nullptr;
+    unsigned* host_answer_seq = nullptr;
     fp2* dev_vals = nullptr;     // device aliases of the mapped memory
     unsigned* dev_seq = nullptr;
+    u64* dev_answer = nullptr;
+    unsigned* dev_answer_seq = nullptr;
+    long long spin_ticks = 0;    // the tail's wait bound in wall-clock ticks
+    double clock_mhz = 100;      // wall-clock rate
+    long long* host_trace = nullptr;   // mapped: 3 stamps per tail round (timing on)
+    long long* dev_trace = nullptr;
+    int tail_rounds = 0;         // tail profile of the last timed prove
+    double tail_work_us = 0, tail_wait_us = 0, tail_host_us = 0;
+    double tail_round_us[40] = {};   // device us of each tail round
     unsigned seq = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    bool timed = false;          // per-launch event timing of the fused kernel
-    double fold_ms = 0, fold_bytes = 0;
-    int fold_launches = 0;
-    double tail_ms = 0;          // the one-workgroup tail rounds
-    int tail_launches = 0;
+    bool timed = false;          // per-launch event timing
+    double ms[SC_NCLASS] = {0, 0, 0, 0};
+    double bytes[SC_NCLASS] = {0, 0, 0, 0};
+    int launches[SC_NCLASS] = {0, 0, 0, 0};
 };
 Sc2State g_sc2;
 
@@ -182,21 +396,37 @@ bool sc2_setup(size_t n)
     Sc2State& S = g_sc2;
     if (!S.host_vals) {
         void* hv = nullptr;
-        TP_HIP_OK(hipHostMalloc(&hv, 256, hipHostMallocMapped | hipHostMallocCoherent));
-        S.host_vals = static_cast<fp2*>(hv);
-        S.host_seq = reinterpret_cast<unsigned*>(static_cast<char*>(hv) + 128);
+        TP_HIP_OK(hipHostMalloc(&hv, 4096, hipHostMallocMapped | hipHostMallocCoherent));
+        char* hc = static_cast<char*>(hv);
+        S.host_vals = reinterpret_cast<fp2*>(hc);
+        S.host_seq = reinterpret_cast<unsigned*>(hc + 256);
+        S.host_answer = reinterpret_cast<u64*>(hc + 512);
+        S.host_answer_seq = reinterpret_cast<unsigned*>(hc + 528);
+        S.host_trace = reinterpret_cast<long long*>(hc + 1024);   // 3 x 40 stamps
         *S.host_seq = 0;
+        *S.host_answer_seq = 0;
         void* dv = nullptr;
         TP_HIP_OK(hipHostGetDevicePointer(&dv, hv, 0));
-        S.dev_vals = static_cast<fp2*>(dv);
-        S.dev_seq = reinterpret_cast<unsigned*>(static_cast<char*>(dv) + 128);
+        char* dc = static_cast<char*>(dv);
+        S.dev_vals = reinterpret_cast<fp2*>(dc);
+        S.dev_seq = reinterpret_cast<unsigned*>(dc + 256);
+        S.dev_answer = reinterpret_cast<u64*>(dc + 512);
+        S.dev_answer_seq = reinterpret_cast<unsigned*>(dc + 528);
+        S.dev_trace = reinterpret_cast<long long*>(dc + 1024);
+        TP_HIP_OK(hipFuncSetAttribute((const void*)k_sc2_tail_host,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLds));
+        int khz = 0, dev = 0;
+        TP_HIP_OK(hipGetDevice(&dev));
+        TP_HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        S.spin_ticks = (long long)(khz > 0 ? khz : 100000) * 1000LL * 10;   // 10 s
+        S.clock_mhz = (khz > 0 ? khz : 100000) / 1000.0;
     }
     if (!S.ev[0]) {
         TP_HIP_OK(hipEventCreate(&S.ev[0]));
         TP_HIP_OK(hipEventCreate(&S.ev[1]));
     }
     if (!S.work.ensure(2 * (n / 2 + n / 4 + 2) * sizeof(V2))) return false;
-    if (!S.partials.ensure(2048 * 3 * sizeof(fp2))) return false;
+    if (!S.partials.ensure(2048 * 9 * sizeof(fp2))) return false;
     return true;
 }
 
@@ -216,6 +446,7 @@ bool wait_seq(hipStream_t st, unsigned seq)
     const auto t0 = std::chrono::steady_clock::now();
     long spins = 0;
     while (*hs != seq) {
+        if (*hs == kSeqAbort) return fail("sumcheck: the tail kernel gave up waiting for round %u", seq);
         if (++spins % 4096 == 0) {
             const hipError_t q = hipStreamQuery(st);
             if (q != hipSuccess && q != hipErrorNotReady)
@@ -255,11 +486,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
     const size_t n = (size_t)1 << nv;
     if (!sc2_setup(n)) return -1;
     Sc2State& S = g_sc2;
-    S.fold_ms = 0;
-    S.fold_bytes = 0;
-    S.fold_launches = 0;
-    S.tail_ms = 0;
-    S.tail_launches = 0;
+    for (int c = 0; c < SC_NCLASS; ++c) S.ms[c] = S.bytes[c] = 0, S.launches[c] = 0;
 
     fp2* partials = S.partials.as<fp2>();
     V2* A = S.work.as<V2>();             // f: n/2, then g: n/2
@@ -271,89 +498,165 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
     fs.item_str("c2", 2);   // put(["c2", seed])
     fs.item_int(seed);
 
-    const V2* cf = reinterpret_cast<const V2*>(d_f);
-    const V2* cg = reinterpret_cast<const V2*>(d_g);
-    size_t m = n;   // length of the current table T_k
-    u64 rr = 0, ri = 0;
-    for (int k = 0; k < nv; ++k) {
-        unsigned nb;
-        bool published = false;
-        const unsigned seq = ++S.seq;
-        if (k == 0) {
-            nb = blocks_for((m / 2 + 1) / 2);
-            k_sc2_first<<<nb, kBlock, 0, st>>>(cf, cg, m / 2, partials);
-        } else if (m / 2 <= kTailQuads) {
-            const size_t quads = m / 2;
-            V2* of = (k & 1) ? A : B;
-            V2* og = of + ((k & 1) ? capA : capB);
-            nb = 1;
-            if (S.timed) TP_HIP_OK(hipEventRecord(S.ev[0], st));
-            k_sc2_tail_round<<<1, kTailThreads, 0, st>>>(cf, cg, quads, rr, ri, of, og, S.dev_vals,
-                                                         S.dev_seq, seq);
-            if (S.timed) TP_HIP_OK(hipEventRecord(S.ev[1], st));
-            cf = of;
-            cg = og;
-            published = true;
-        } else {
-            const size_t quads = m / 2;   // input T_{k-1} has length 2m
-            V2* of = (k & 1) ? A : B;
-            V2* og = of + ((k & 1) ? capA : capB);
-            nb = blocks_for(quads);
-            if (S.timed) TP_HIP_OK(hipEventRecord(S.ev[0], st));
-            k_sc2_fold_round<<<nb, kBlock, 0, st>>>(cf, cg, quads, rr, ri, of, og, partials);
-            if (S.timed) TP_HIP_OK(hipEventRecord(S.ev[1], st));
-            cf = of;
-            cg = og;
-            S.fold_bytes += 48.0 * (double)(2 * m);
-            S.fold_launches += 1;
-        }
-        TP_HIP_OK(hipGetLastError());
-        if (!published) {
-            k_sc2_publish<<<1, kBlock, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
-            TP_HIP_OK(hipGetLastError());
-        }
-        if (!wait_seq(st, seq)) return -1;
-        if (S.timed && k > 0) {
-            TP_HIP_OK(hipEventSynchronize(S.ev[1]));
-            float ms = 0;
-            TP_HIP_OK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
-            if (published) {
-                S.tail_ms += ms;
-                S.tail_launches += 1;
-            } else {
-                S.fold_ms += ms;
-            }
-        }
-        const fp2 E0 = S.host_vals[0], E1 = S.host_vals[1], EA = S.host_vals[2];
+    // Records round k's polynomial from its sums and returns r_k.
+    auto round_done = [&](int k, fp2 E0, fp2 E1, fp2 EA) {
         const fp2 a = EA, c = E0, b = sub2(sub2(E1, EA), E0);
         polys[6 * k + 0] = a.re; polys[6 * k + 1] = a.im;
         polys[6 * k + 2] = b.re; polys[6 * k + 3] = b.im;
         polys[6 * k + 4] = c.re; polys[6 * k + 5] = c.im;
+        fp2 r;
         if (fixed) {
-            rr = fixed[2 * k];
-            ri = fixed[2 * k + 1];
+            r = mk2(fixed[2 * k], fixed[2 * k + 1]);
         } else {
             fs.item_fp2(a);   // put([a, b, c])
             fs.item_fp2(b);
             fs.item_fp2(c);
             char lab[24];
             const uint32_t ln = label_num(lab, "c2-", k);
-            const fp2 r = fs.rand_ext(lab, ln);
-            rr = r.re;
-            ri = r.im;
+            r = fs.rand_ext(lab, ln);
         }
-        chals[2 * k] = rr;
-        chals[2 * k + 1] = ri;
+        chals[2 * k] = r.re;
+        chals[2 * k + 1] = r.im;
+        return r;
+    };
+    // Launch bracket: events around one launch of class c (timing on).
+    auto t_begin = [&]() -> bool {
+        if (S.timed) TP_HIP_OK(hipEventRecord(S.ev[0], st));
+        return true;
+    };
+    auto t_end = [&](int c, double bytes) -> bool {
+        if (S.timed) TP_HIP_OK(hipEventRecord(S.ev[1], st));
+        S.bytes[c] += bytes;
+        S.launches[c] += 1;
+        return true;
+    };
+    auto t_collect = [&](int c) -> bool {   // after the round's wait
+        if (!S.timed) return true;
+        TP_HIP_OK(hipEventSynchronize(S.ev[1]));
+        float ms = 0;
+        TP_HIP_OK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
+        S.ms[c] += ms;
+        return true;
+    };
+
+    const V2* cf = reinterpret_cast<const V2*>(d_f);
+    const V2* cg = reinterpret_cast<const V2*>(d_g);
+    size_t m;       // length of T_k at the top of the loop below
+    int k0;         // first round the loop handles
+    fp2 r;          // challenge of the round before k0
+    if (nv >= 3) {
+        // rounds 0 and 1 from one pass, round 2 folds the input by r_0, r_1
+        unsigned seq = ++S.seq;
+        unsigned nb = blocks_for(n / 4);
+        if (!t_begin()) return -1;
+        k_sc2_first01<<<nb, kBlock, 0, st>>>(cf, cg, n / 4, partials);
+        if (!t_end(SC_FIRST, 32.0 * (double)n)) return -1;
+        TP_HIP_OK(hipGetLastError());
+        k_publish_sum<9><<<1, kBlock, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
+        TP_HIP_OK(hipGetLastError());
+        if (!wait_seq(st, seq) || !t_collect(SC_FIRST)) return -1;
+        fp2 v[9];
+        for (int i = 0; i < 9; ++i) v[i] = S.host_vals[i];
+        const fp2 r0 = round_done(0, add2(v[0], v[3]), add2(v[1], v[4]), add2(v[2], v[5]));
+        // the round-1 quadratics at r_0: c0 + r (c1 - c0 - c2) + r^2 c2
+        auto at_r0 = [&](fp2 c0, fp2 c1, fp2 c2) {
+            return add2(c0, mul2(r0, add2(sub2(sub2(c1, c0), c2), mul2(r0, c2))));
+        };
+        const fp2 r1 = round_done(1, at_r0(v[0], v[1], v[2]), at_r0(v[3], v[4], v[5]),
+                                  at_r0(v[6], v[7], v[8]));
+        seq = ++S.seq;
+        V2* of = B;
+        V2* og = B + capB;
+        nb = blocks_for(n / 8);
+        if (!t_begin()) return -1;
+        k_sc2_fold01_round<<<nb, kBlock, 0, st>>>(cf, cg, n / 8, r0.re, r0.im, r1.re, r1.im, of, og,
+                                                  partials);
+        if (!t_end(SC_FOLD01, 40.0 * (double)n)) return -1;
+        TP_HIP_OK(hipGetLastError());
+        k_publish_sum<3><<<1, kBlock, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
+        TP_HIP_OK(hipGetLastError());
+        if (!wait_seq(st, seq) || !t_collect(SC_FOLD01)) return -1;
+        r = round_done(2, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
+        cf = of;
+        cg = og;
+        k0 = 3;
+        m = n / 8;
+    } else {
+        const unsigned seq = ++S.seq;
+        const unsigned nb = blocks_for((n / 2 + 1) / 2);
+        if (!t_begin()) return -1;
+        k_sc2_first<<<nb, kBlock, 0, st>>>(cf, cg, n / 2, partials);
+        if (!t_end(SC_FIRST, 32.0 * (double)n)) return -1;
+        TP_HIP_OK(hipGetLastError());
+        k_publish_sum<3><<<1, kBlock, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
+        TP_HIP_OK(hipGetLastError());
+        if (!wait_seq(st, seq) || !t_collect(SC_FIRST)) return -1;
+        r = round_done(0, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
+        k0 = 1;
+        m = n / 2;
+    }
+    for (int k = k0;; ++k) {
+        if (2 * m <= kTailIn) {
+            // rounds k..nv-1 and the final fold in one workgroup; the host
+            // answers each round's challenge through the mapped answer slot
+            const int nr = nv - k;
+            const unsigned seq0 = S.seq + 1;
+            S.seq += (unsigned)nr + 1;
+            if (!t_begin()) return -1;
+            k_sc2_tail_host<<<1, kTailThreads, kTailLds, st>>>(
+                cf, cg, (unsigned)(2 * m), k, nv, r.re, r.im, S.dev_vals, S.dev_seq, seq0,
+                S.dev_answer, S.dev_answer_seq, S.spin_ticks, S.timed ? S.dev_trace : nullptr);
+            if (!t_end(SC_TAIL, 32.0 * (double)(2 * m))) return -1;
+            TP_HIP_OK(hipGetLastError());
+            volatile u64* ans = S.host_answer;
+            volatile unsigned* ans_seq = S.host_answer_seq;
+            double host_us = 0;
+            for (int i = 0; i < nr; ++i) {
+                if (!wait_seq(st, seq0 + i)) return -1;
+                const auto h0 = std::chrono::steady_clock::now();
+                const fp2 rk = round_done(k + i, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
+                ans[0] = rk.re;
+                ans[1] = rk.im;
+                std::atomic_thread_fence(std::memory_order_release);
+                *ans_seq = seq0 + i;
+                host_us += std::chrono::duration<double, std::micro>(
+                               std::chrono::steady_clock::now() - h0).count();
+            }
+            if (!wait_seq(st, seq0 + nr) || !t_collect(SC_TAIL)) return -1;
+            if (S.timed && nr > 0) {
+                const volatile long long* tr = S.host_trace;
+                double work = 0, wait = 0;
+                for (int i = 0; i < nr; ++i) {
+                    work += (double)(tr[3 * i + 1] - tr[3 * i]);
+                    wait += (double)(tr[3 * i + 2] - tr[3 * i + 1]);
+                    if (i < 40) S.tail_round_us[i] = (double)(tr[3 * i + 1] - tr[3 * i]) / S.clock_mhz;
+                }
+                S.tail_rounds = nr;
+                S.tail_work_us = work / S.clock_mhz / nr;
+                S.tail_wait_us = wait / S.clock_mhz / nr;
+                S.tail_host_us = host_us / nr;
+            }
+            final4[0] = S.host_vals[0].re; final4[1] = S.host_vals[0].im;
+            final4[2] = S.host_vals[1].re; final4[3] = S.host_vals[1].im;
+            return 0;
+        }
+        const unsigned seq = ++S.seq;
+        const size_t quads = m / 2;   // input T_{k-1} has length 2m
+        V2* of = (k & 1) ? A : B;
+        V2* og = of + ((k & 1) ? capA : capB);
+        const unsigned nb = blocks_for(quads);
+        if (!t_begin()) return -1;
+        k_sc2_fold_round<<<nb, kBlock, 0, st>>>(cf, cg, quads, r.re, r.im, of, og, partials);
+        if (!t_end(SC_FOLD, 48.0 * (double)(2 * m))) return -1;
+        TP_HIP_OK(hipGetLastError());
+        k_publish_sum<3><<<1, kBlock, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
+        TP_HIP_OK(hipGetLastError());
+        if (!wait_seq(st, seq) || !t_collect(SC_FOLD)) return -1;
+        cf = of;
+        cg = og;
+        r = round_done(k, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
         m /= 2;
     }
-    // T_{nv-1} has two entries: fold them with r_{nv-1}.
-    const unsigned seq = ++S.seq;
-    k_sc2_final<<<1, 64, 0, st>>>(cf, cg, rr, ri, S.dev_vals, S.dev_seq, seq);
-    TP_HIP_OK(hipGetLastError());
-    if (!wait_seq(st, seq)) return -1;
-    final4[0] = S.host_vals[0].re; final4[1] = S.host_vals[0].im;
-    final4[2] = S.host_vals[1].re; final4[3] = S.host_vals[1].im;
-    return 0;
 }
 
 // Enables per-launch HIP-event timing of the fused fold+round kernel.
@@ -364,17 +667,47 @@ void tp_sc2_set_timing(int on) { g_sc2.timed = on != 0; }
 // their count.
 void tp_sc2_stats(double* fold_ms, double* fold_bytes, int* fold_launches)
 {
-    *fold_ms = g_sc2.fold_ms;
-    *fold_bytes = g_sc2.fold_bytes;
-    *fold_launches = g_sc2.fold_launches;
+    *fold_ms = g_sc2.ms[SC_FOLD];
+    *fold_bytes = g_sc2.bytes[SC_FOLD];
+    *fold_launches = g_sc2.launches[SC_FOLD];
 }
 
 // Summed event time and count of the one-workgroup tail rounds of the last
 // tp_sc2_prove (timing on).
 void tp_sc2_tail_stats(double* tail_ms, int* tail_launches)
 {
-    *tail_ms = g_sc2.tail_ms;
-    *tail_launches = g_sc2.tail_launches;
+    *tail_ms = g_sc2.ms[SC_TAIL];
+    *tail_launches = g_sc2.launches[SC_TAIL];
+}
+
+// Per-round latency of the one-workgroup tail in the last timed
+// tp_sc2_prove: out[0] rounds, out[1] mean device time from a round's start
+// to its sums being ready (us), out[2] mean time the workgroup waited for the
+// host's challenge (us), out[3] mean host time from seeing the sums to
+// writing the challenge (transcript, us), out[4 + i] the device time of
+// round i of the tail (us, while 4 + i < n).  n >= 4.
+void tp_sc2_tail_profile(double* out, int n)
+{
+    if (n < 4) return;
+    out[0] = g_sc2.tail_rounds;
+    out[1] = g_sc2.tail_work_us;
+    out[2] = g_sc2.tail_wait_us;
+    out[3] = g_sc2.tail_host_us;
+    for (int i = 0; i < g_sc2.tail_rounds && i < 40 && 4 + i < n; ++i) out[4 + i] = g_sc2.tail_round_us[i];
+}
+
+// Per-class statistics of the last tp_sc2_prove (timing on): for each class
+// (0 the first pass over the input, 1 the round-2 fold from the input, 2 the
+// streaming fold+round launches, 3 the one-workgroup tail rounds) the summed
+// event ms, the bytes the launches read and write, and the launch count:
+// out[3 c + 0..2].  n >= 12.
+void tp_sc2_class_stats(double* out, int n)
+{
+    for (int c = 0; c < SC_NCLASS && 3 * c + 2 < n; ++c) {
+        out[3 * c] = g_sc2.ms[c];
+        out[3 * c + 1] = g_sc2.bytes[c];
+        out[3 * c + 2] = g_sc2.launches[c];
+    }
 }
 
 }  // extern "C"

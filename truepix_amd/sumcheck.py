@@ -96,3 +96,30 @@ def tail_stats():
     ms, n = ctypes.c_double(), ctypes.c_int()
     nb.LIB.tp_sc2_tail_stats(ctypes.byref(ms), ctypes.byref(n))
     return ms.value, n.value
+
+
+CLASSES = ("first", "fold01", "fold", "tail")
+
+
+def class_stats():
+    """Per launch class of the last prove (timing on): {"first" | "fold01" |
+    "fold" | "tail": {"ms", "bytes", "launches"}}.  "first" is the pass over
+    the input giving rounds 0 and 1, "fold01" round 2 folding the input by
+    r_0 and r_1, "fold" the streaming fold+round launches, "tail" the
+    one-workgroup rounds."""
+    out = (ctypes.c_double * 12)()
+    nb.LIB.tp_sc2_class_stats(out, 12)
+    return {c: {"ms": out[3 * i], "bytes": out[3 * i + 1], "launches": int(out[3 * i + 2])}
+            for i, c in enumerate(CLASSES)}
+
+
+def tail_profile():
+    """Per-round latency of the one-workgroup tail in the last timed prove:
+    {"rounds", "device_us" (a round's fold and sums), "wait_us" (waiting for
+    the host's challenge), "host_us" (the host's transcript step),
+    "round_device_us" (per round)}."""
+    out = (ctypes.c_double * 44)()
+    nb.LIB.tp_sc2_tail_profile(out, 44)
+    n = int(out[0])
+    return {"rounds": n, "device_us": round(out[1], 3), "wait_us": round(out[2], 3),
+            "host_us": round(out[3], 3), "round_device_us": [round(out[4 + i], 2) for i in range(n)]}
