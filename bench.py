@@ -7,11 +7,11 @@ from tables already resident in HBM.  Prints ONE JSON line:
   value      whole-job fold HBM throughput: the algorithmic bytes of a prove,
              48 (2^25 - 2) B (round r reads 2 x 2^(24-r) x 16 B and writes
              half of that, whatever the schedule), / wall time per step;
-  roofline   the launch class with the most device time (the first pass over
-             the input, the round-2 fold from the input, or the streaming
-             fold+round launches): the bytes its launches read and write / its
-             summed HIP-event time (events on the stream it runs on), against
-             8 TB/s;
+  roofline   the launch class moving the most HBM bytes per prove (of the
+             first pass over the input, the round-2 fold from the input and the
+             streaming fold+round launches): the bytes its launches read and
+             write / its summed HIP-event time (events on the stream it runs
+             on), against 8 TB/s; every class is listed beside it;
   cpu_baseline  the CPU oracle (oracle/fp2_oracle.c, a port) proving the same
              2 x 2^24 workload single-threaded on this host, repeated for at
              least 10 s, in GB/s of the same kind;
@@ -33,7 +33,8 @@ PEAK_HBM_GBS = 8000.0
 NV = 24
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sc2_fold_round_traffic.json")
+def traffic_file(kernel: str) -> str:
+    return os.path.join(ROOT, "profiles", "%s_traffic.json" % kernel.replace("k_", "", 1))
 
 
 def measured_traffic(kernel: str):
@@ -41,11 +42,11 @@ def measured_traffic(kernel: str):
     FETCH_SIZE / WRITE_SIZE summary (each scaled by the calibration copy), or
     None when the summary covers another kernel."""
     try:
-        with open(TRAFFIC_FILE) as fh:
+        with open(traffic_file(kernel)) as fh:
             rec = json.load(fh)
     except (OSError, ValueError):
         return None
-    if rec.get("kernel", "k_sc2_fold_round") != kernel:
+    if rec.get("kernel") != kernel:
         return None
     return rec.get("bytes_per_launch")
 
@@ -58,7 +59,7 @@ def alg_bytes(nv: int) -> float:
 
 
 SC2_KERNELS = {"first": "k_sc2_first01", "fold01": "k_sc2_fold01_round",
-               "fold": "k_sc2_fold_round", "tail": "k_sc2_tail_host"}
+               "fold": "k_sc2_fold_round", "tail": "k_sc2_tail_host", "mid": "k_sc2_mid"}
 
 
 def bench_rs_encode(steps: int = 5, cols: int = 64, lc: int = 20, lo: int = 22):
@@ -519,6 +520,7 @@ def main(argv=None):
             for key in v:
                 acc[c][key] += v[key]
     tail_prof = sc.tail_profile()
+    mid_prof = sc.mid_profile()
     sc.set_timing(False)
     classes = {}
     for c, v in acc.items():
@@ -530,7 +532,10 @@ def main(argv=None):
                           "achieved_gbs": round(v["bytes"] / (v["ms"] / 1e3) / 1e9, 1)}
     if "tail" in classes:
         classes["tail"]["per_round"] = tail_prof
-    dom = max((c for c in classes if c != "tail"), key=lambda c: acc[c]["ms"])
+    if "mid" in classes:
+        classes["mid"]["per_round"] = mid_prof
+    # the roofline is the HBM one: the class moving the most bytes per prove
+    dom = max((c for c in classes if c not in ("tail", "mid")), key=lambda c: acc[c]["bytes"])
     achieved = acc[dom]["bytes"] / (acc[dom]["ms"] / 1e3) / 1e9
 
     line = {
@@ -560,7 +565,9 @@ def main(argv=None):
                      "classes": classes,
                      "note": "first: rounds 0 and 1 from one read of the input; fold01: round 2 "
                              "folds the input by r_0 and r_1 into T_2; fold: streaming "
-                             "fold+round launches; tail: one workgroup proving every round "
+                             "fold+round launches; mid: 128 workgroups, each folding its own "
+                             "slice of the tables in LDS, the host adding their partials and "
+                             "answering each challenge; tail: one workgroup proving every round "
                              "whose input has <= 8192 entries per table, the host answering "
                              "each challenge through mapped memory (latency-bound)"},
         "check": {"round0_poly": [int(x) for x in proof.polys[0][:2]]},

@@ -118,11 +118,16 @@ void tp_sc2_stats(double* fold_ms, double* fold_bytes, int* fold_launches);
 /* Summed event milliseconds and count of the one-workgroup tail rounds
  * (fold + round sum + publish in one launch) of the last tp_sc2_prove. */
 void tp_sc2_tail_stats(double* tail_ms, int* tail_launches);
+/* The prove schedule's sizes: workgroups of the sliced mid rounds, and the
+ * largest input (entries per table) of the one-workgroup tail.  Streaming
+ * launches run while the input exceeds mid_groups * tail_in. */
+void tp_sc2_schedule(int* mid_groups, int* tail_in);
 /* Per-class statistics of the last tp_sc2_prove (timing on): for class c in
  * 0 = the first pass over the input (rounds 0 and 1), 1 = round 2 folding
  * the input by r_0 and r_1, 2 = the streaming fold+round launches, 3 = the
- * one-workgroup tail rounds: out[3c] summed event ms, out[3c+1] bytes the
- * launches read and write, out[3c+2] launch count.  n >= 12. */
+ * one-workgroup tail rounds, 4 = the rounds on per-workgroup slices:
+ * out[3c] summed event ms, out[3c+1] bytes the launches read and write,
+ * out[3c+2] launch count.  n >= 15. */
 void tp_sc2_class_stats(double* out, int n);
 /* Per-round latency of the one-workgroup tail of the last timed
  * tp_sc2_prove: out[0] rounds, out[1] device us per round until its sums are
@@ -130,6 +135,12 @@ void tp_sc2_class_stats(double* out, int n);
  * us per round (transcript), out[4 + i] device us of tail round i (while
  * 4 + i < n).  n >= 4. */
 void tp_sc2_tail_profile(double* out, int n);
+/* Per-round latency of the sliced mid rounds of the last timed tp_sc2_prove:
+ * out[0] rounds, out[1] us from the last workgroup's partials to workgroup 0
+ * holding the challenge, out[2] us of the relay to the last workgroup,
+ * out[3] us from a challenge to the next partials, out[4] us between the
+ * first and the last workgroup's partials.  n >= 5. */
+void tp_sc2_mid_profile(double* out, int n);
 
 /* ---- Reed-Solomon encoding: OZ/src/infrastructure/RS_polynomial.cpp:46-130
  *      (fast_fourier_transform, with Orion's root of unity,

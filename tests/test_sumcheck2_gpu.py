@@ -70,25 +70,34 @@ def test_sc2_full_size(bridge, oracle, torch_cuda):
 
 
 def test_sc2_launch_classes(bridge, torch_cuda):
-    """Rounds 0 and 1 share one pass over the input and round 2 folds the
-    input straight into T_2: per class the bytes the launches move."""
-    nv = 16
-    rng = np.random.default_rng(1)
-    f, g = rand_fp2(rng, 1 << nv), rand_fp2(rng, 1 << nv)
-    sc.set_timing(True)
-    try:
-        sc.prove_deg2_host(f, g, seed=1)
-        cls = sc.class_stats()
-    finally:
-        sc.set_timing(False)
-    n = 1 << nv
-    assert cls["first"]["launches"] == 1 and cls["first"]["bytes"] == 32 * n
-    assert cls["fold01"]["launches"] == 1 and cls["fold01"]["bytes"] == 40 * n
-    # rounds 3.. : streaming launches while the input T_{k-1} has more than
-    # 8192 entries, then one workgroup proves the rest from its input
-    big = [k for k in range(3, nv) if (n >> (k - 1)) > 8192]
-    assert cls["fold"]["launches"] == len(big)
-    assert cls["fold"]["bytes"] == sum(48 * (n >> (k - 1)) for k in big)
-    k0 = big[-1] + 1 if big else 3
-    assert cls["tail"]["launches"] == 1 and cls["tail"]["bytes"] == 32 * (n >> (k0 - 1))
-    assert all(c["ms"] > 0 for c in cls.values() if c["launches"])
+    """Rounds 0 and 1 share one pass over the input, round 2 folds the input
+    straight into T_2, the streaming launches run while the input exceeds
+    mid_groups x tail_in entries, the sliced launch takes the rounds down to
+    2 x mid_groups entries, and one workgroup proves the rest: per class the
+    launches and the bytes they move."""
+    groups, tail_in = sc.schedule()
+    for nv in (16, 22):
+        rng = np.random.default_rng(nv)
+        f, g = rand_fp2(rng, 1 << nv), rand_fp2(rng, 1 << nv)
+        sc.set_timing(True)
+        try:
+            sc.prove_deg2_host(f, g, seed=1)
+            cls = sc.class_stats()
+        finally:
+            sc.set_timing(False)
+        n = 1 << nv
+        assert cls["first"]["launches"] == 1 and cls["first"]["bytes"] == 32 * n
+        assert cls["fold01"]["launches"] == 1 and cls["fold01"]["bytes"] == 40 * n
+        big = [k for k in range(3, nv) if (n >> (k - 1)) > groups * tail_in]
+        assert cls["fold"]["launches"] == len(big)
+        assert cls["fold"]["bytes"] == sum(48 * (n >> (k - 1)) for k in big)
+        k_mid = big[-1] + 1 if big else 3
+        if (n >> (k_mid - 1)) > tail_in:
+            assert cls["mid"]["launches"] == 1
+            assert cls["mid"]["bytes"] == 32 * (n >> (k_mid - 1)) + 64 * groups
+            tail_bytes = 32 * 2 * groups
+        else:
+            assert cls["mid"]["launches"] == 0
+            tail_bytes = 32 * (n >> (k_mid - 1))
+        assert cls["tail"]["launches"] == 1 and cls["tail"]["bytes"] == tail_bytes
+        assert all(c["ms"] > 0 for c in cls.values() if c["launches"])

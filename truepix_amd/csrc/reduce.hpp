@@ -44,12 +44,27 @@ __device__ __forceinline__ void block_sum(fp2 (&v)[N])
         for (int k = 0; k < N; ++k) sh[k][wid] = v[k];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (nw <= 4) {
+        if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < N; ++k) {
-            fp2 s = sh[k][0];
-            for (int w = 1; w < nw; ++w) s = add2(s, sh[k][w]);
-            v[k] = s;
+            for (int k = 0; k < N; ++k) {
+                fp2 s = sh[k][0];
+                for (int w = 1; w < nw; ++w) s = add2(s, sh[k][w]);
+                v[k] = s;
+            }
+        }
+    } else if (wid == 0) {
+        // up to 16 wave totals: lanes 0..15 of wave 0, four xor-shuffle levels
+#pragma unroll
+        for (int k = 0; k < N; ++k) v[k] = lane < nw ? sh[k][lane] : zero2();
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                const u64 r = __shfl_xor(v[k].re, off, 64);
+                const u64 i = __shfl_xor(v[k].im, off, 64);
+                v[k] = add2(v[k], mk2(r, i));
+            }
         }
     }
 }
@@ -105,6 +120,22 @@ template <int N>
 __global__ void __launch_bounds__(kBlock) k_publish_sum(const fp2* partials, unsigned n_blocks,
                                                         fp2* host_vals, volatile unsigned* host_seq,
                                                         unsigned seq)
+{
+    fp2 acc[N];
+    sum_partials<N>(partials, n_blocks, acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) host_vals[k] = acc[k];
+        __threadfence_system();
+        *host_seq = seq;
+    }
+}
+
+// k_publish_sum with 1024 threads: for many partials (a few per thread).
+template <int N>
+__global__ void __launch_bounds__(1024) k_publish_wide(const fp2* partials, unsigned n_blocks,
+                                                       fp2* host_vals, volatile unsigned* host_seq,
+                                                       unsigned seq)
 {
     fp2 acc[N];
     sum_partials<N>(partials, n_blocks, acc);

@@ -37,6 +37,7 @@
 #include "runtime.hpp"
 #include "transcript.hpp"
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstring>
@@ -250,6 +251,256 @@ __device__ __forceinline__ void tail_fold(const V2* sf, const V2* sg, unsigned q
     g1 = fold2(ldv(sg + 4 * q + 2), ldv(sg + 4 * q + 3), r);
 }
 
+// Rounds on a table too large for one workgroup but small enough that the
+// launches, not the bytes, set the time: kMidGroups workgroups (launched
+// cooperatively, so all are resident) each keep a contiguous slice of the
+// tables in LDS.  Variables bind LSB-first, so a slice of S entries folds
+// into its own S/2 entries and its pairs stay inside it for log2(S) - 1
+// rounds.  Per round every workgroup writes its three partial sums to its six
+// words of `parts` (device memory) as agent-scope stores, each value (61 bits)
+// tagged with the round mod 8 in its top three bits; wave 0 of workgroup 0
+// gathers them (lane l waits for workgroup l's words to carry the round's tag),
+// adds them, and publishes the total to the host like the one-workgroup tail
+// (values, system fence, sequence number).  The words start as all ones (value
+// bits p, never canonical), so round 0 cannot match a stale word, and later
+// rounds find the previous round's tag until the new word lands: no fence and
+// no extra launch.  The host runs the transcript and answers through the
+// answer slot; workgroup 0 relays the challenge to the others through two
+// tagged words the same way.  Every wait is bounded by the wall clock; an
+// expired wait publishes kSeqAbort and relays p, and every workgroup leaves.
+constexpr int kMidGroups = 64;
+
+// End of a mid round, run by wave 0 of every workgroup: publish this
+// workgroup's sums (lane 0), and in workgroup 0 gather every workgroup's sums,
+// publish the total to the host and, unless `last`, wait for the host's
+// challenge and relay it; the others wait for the relay.  Returns 0 when a
+// wait expired (every workgroup then leaves); the challenge goes to rv.
+__device__ __forceinline__ int mid_exchange(const fp2 (&e)[3], int i, bool last, unsigned b,
+                                            int lane, u64* parts, fp2* host_vals,
+                                            unsigned* host_seq, unsigned seq0, const u64* answer,
+                                            unsigned* answer_seq, long long spin_ticks, u64* relay,
+                                            long long* trace, u64 (&rv)[2])
+{
+    const u64 tag = (u64)(i & 7) << 61;   // round mod 8 over the 61 value bits
+    if (lane == 0) {
+        if (trace) trace[((size_t)i * kMidGroups + b) * 2] = wall_clock64();
+        u64* part = parts + (size_t)b * 6;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            __hip_atomic_store(part + 2 * j, e[j].re | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(part + 2 * j + 1, e[j].im | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    int ok = 1;
+    const long long t0 = wall_clock64();
+    if (b == 0) {
+        // gather: lane l waits for workgroup l's six words of this round
+        fp2 v[3] = {zero2(), zero2(), zero2()};
+        if (lane < kMidGroups) {
+            const u64* part = parts + (size_t)lane * 6;
+            for (;;) {
+                u64 w[6];
+                bool all = true;
+#pragma unroll
+                for (int j = 0; j < 6; ++j) {
+                    w[j] = __hip_atomic_load(part + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    all = all && (w[j] & ~P) == tag && (w[j] & P) != P;
+                }
+                if (all) {
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) v[j] = mk2(w[2 * j] & P, w[2 * j + 1] & P);
+                    break;
+                }
+                if (wall_clock64() - t0 > spin_ticks) {
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        ok = __all(ok);
+        wave_sum<3>(v);
+        u64 re = P, im = P;   // p: the wait expired
+        if (lane == 0) {
+            if (ok) {
+                host_vals[0] = v[0];
+                host_vals[1] = v[1];
+                host_vals[2] = v[2];
+                __threadfence_system();
+                __hip_atomic_store(host_seq, seq0 + (unsigned)i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                if (!last) {
+                    const unsigned want = seq0 + (unsigned)i;
+                    while (__hip_atomic_load(answer_seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != want) {
+                        if (wall_clock64() - t0 > spin_ticks) {
+                            ok = 0;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    if (ok) {
+                        re = __hip_atomic_load(answer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                        im = __hip_atomic_load(answer + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    }
+                }
+            }
+            if (!ok) __hip_atomic_store(host_seq, kSeqAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (!last || !ok) {
+                // relay to the other workgroups (p under the tag: leave)
+                __hip_atomic_store(relay, re | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(relay + 1, im | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (trace) trace[((size_t)i * kMidGroups) * 2 + 1] = wall_clock64();
+        }
+        ok = __shfl(ok, 0, 64);
+        rv[0] = __shfl(re, 0, 64);
+        rv[1] = __shfl(im, 0, 64);
+        return ok;
+    }
+    u64 re = 0, im = 0;
+    if (lane == 0 && !last) {
+        // wait for workgroup 0's relay
+        for (;;) {
+            const u64 x = __hip_atomic_load(relay, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const u64 y = __hip_atomic_load(relay + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((x & ~P) == tag && (y & ~P) == tag) {
+                re = x & P;
+                im = y & P;
+                if (re == P || im == P) ok = 0;
+                break;
+            }
+            if (wall_clock64() - t0 > spin_ticks) {
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (trace) trace[((size_t)i * kMidGroups + b) * 2 + 1] = wall_clock64();
+    }
+    ok = __shfl(ok, 0, 64);
+    rv[0] = __shfl(re, 0, 64);
+    rv[1] = __shfl(im, 0, 64);
+    return ok;
+}
+
+__global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
+    const V2* __restrict__ fin, const V2* __restrict__ gin, unsigned slice_in, int n_rounds,
+    u64 rr, u64 ri, u64* parts, fp2* host_vals, unsigned* host_seq, unsigned seq0,
+    const u64* answer, unsigned* answer_seq, long long spin_ticks, V2* __restrict__ fout,
+    V2* __restrict__ gout, u64* relay, long long* trace)
+{
+    extern __shared__ V2 tl[];
+    __shared__ u64 s_r[2];
+    __shared__ int s_ok;
+    __shared__ fp2 s_part[3][kTailThreads / 64];
+    const unsigned half = slice_in / 2;
+    V2* tf = tl;
+    V2* tg = tl + half;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const unsigned b = blockIdx.x;
+    const V2* sf0 = fin + (size_t)b * slice_in;
+    const V2* sg0 = gin + (size_t)b * slice_in;
+    fp2 r = mk2(rr, ri);
+    unsigned len = slice_in;
+    int i = 0;
+    // Rounds with more than one wave's quads per slice: the whole workgroup.
+    for (; i < n_rounds && len / 4 > 64; ++i) {
+        const unsigned nq = len / 4;
+        LazyProdSum z[3];
+        lazy_zero(z);
+        if (i == 0) {
+            for (unsigned q = t; q < nq; q += kTailThreads) {
+                fp2 f0, f1, g0, g1;
+                tail_fold(sf0, sg0, q, r, f0, f1, g0, g1);
+                stv(tf + 2 * q, f0);
+                stv(tf + 2 * q + 1, f1);
+                stv(tg + 2 * q, g0);
+                stv(tg + 2 * q + 1, g1);
+                accum_pair(z, f0, f1, g0, g1);
+            }
+        } else {
+            const unsigned q = t;   // len <= slice_in / 2 <= 4 kTailThreads
+            fp2 f0, f1, g0, g1;
+            if (q < nq) {
+                tail_fold(tf, tg, q, r, f0, f1, g0, g1);
+                accum_pair(z, f0, f1, g0, g1);
+            }
+            __syncthreads();   // every read of T_{k-1} is done before T_k overwrites it
+            if (q < nq) {
+                stv(tf + 2 * q, f0);
+                stv(tf + 2 * q + 1, f1);
+                stv(tg + 2 * q, g0);
+                stv(tg + 2 * q + 1, g1);
+            }
+        }
+        const int na = (int)((nq + 63) / 64) < kTailThreads / 64 ? (int)((nq + 63) / 64)
+                                                                 : kTailThreads / 64;
+        fp2 e[3];
+        if (wid < na) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) e[j] = z[j].value();
+            wave_sum<3>(e);
+            if (lane == 0) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) s_part[j][wid] = e[j];
+            }
+        }
+        __syncthreads();
+        const bool last = i == n_rounds - 1;
+        if (wid == 0) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) e[j] = lane < na ? s_part[j][lane] : zero2();
+            wave_sum<3>(e);
+            u64 rv[2];
+            const int ok = mid_exchange(e, i, last, b, lane, parts, host_vals, host_seq, seq0, answer,
+                                        answer_seq, spin_ticks, relay, trace, rv);
+            if (lane == 0) {
+                s_ok = ok;
+                s_r[0] = rv[0];
+                s_r[1] = rv[1];
+            }
+        }
+        __syncthreads();   // r_k and T_k visible to every thread
+        if (!s_ok) return;
+        if (!last) r = mk2(s_r[0], s_r[1]);
+        len /= 2;
+    }
+    // The rest in wave 0 alone (its LDS reads precede its writes in program
+    // order; r goes lane to lane), then the slice's last table goes out.
+    if (wid != 0) return;
+    for (; i < n_rounds; ++i) {
+        const unsigned nq = len / 4;
+        LazyProdSum z[3];
+        lazy_zero(z);
+        const unsigned q = lane;
+        const V2* sf = i == 0 ? sf0 : tf;
+        const V2* sg = i == 0 ? sg0 : tg;
+        if (q < nq) {
+            fp2 f0, f1, g0, g1;
+            tail_fold(sf, sg, q, r, f0, f1, g0, g1);
+            stv(tf + 2 * q, f0);
+            stv(tf + 2 * q + 1, f1);
+            stv(tg + 2 * q, g0);
+            stv(tg + 2 * q + 1, g1);
+            accum_pair(z, f0, f1, g0, g1);
+        }
+        fp2 e[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) e[j] = z[j].value();
+        wave_sum<3>(e);
+        const bool last = i == n_rounds - 1;
+        u64 rv[2];
+        if (!mid_exchange(e, i, last, b, lane, parts, host_vals, host_seq, seq0, answer, answer_seq,
+                          spin_ticks, relay, trace, rv))
+            return;
+        if (!last) r = mk2(rv[0], rv[1]);
+        len /= 2;
+    }
+    for (unsigned j = lane; j < len; j += 64) {
+        fout[(size_t)b * len + j] = tf[j];
+        gout[(size_t)b * len + j] = tg[j];
+    }
+}
+
 __global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
     const V2* __restrict__ fin, const V2* __restrict__ gin, unsigned len_in, int k0, int nv,
     u64 rr, u64 ri, fp2* host_vals, unsigned* host_seq, unsigned seq0, const u64* answer,
@@ -258,6 +509,7 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
     extern __shared__ V2 tl[];
     __shared__ u64 s_r[2];
     __shared__ int s_ok;
+    __shared__ fp2 s_part[3][kTailThreads / 64];
     V2* tf = tl;
     V2* tg = tl + kTailIn / 2;
     const int t = threadIdx.x;
@@ -298,10 +550,26 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
                 stv(tg + 2 * q + 1, g1);
             }
         }
+        // waves with quads reduce theirs; wave 0 reduces the wave totals
+        const int lane = t & 63, wid = t >> 6;
+        const int na = (int)((nq + 63) / 64) < kTailThreads / 64 ? (int)((nq + 63) / 64)
+                                                                 : kTailThreads / 64;
         fp2 e[3];
+        if (wid < na) {
 #pragma unroll
-        for (int i = 0; i < 3; ++i) e[i] = z[i].value();
-        block_sum<3>(e);
+            for (int i = 0; i < 3; ++i) e[i] = z[i].value();
+            wave_sum<3>(e);
+            if (lane == 0) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) s_part[i][wid] = e[i];
+            }
+        }
+        __syncthreads();
+        if (wid == 0) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) e[i] = lane < na ? s_part[i][lane] : zero2();
+            wave_sum<3>(e);
+        }
         if (t == 0) {
             u64 rv[2] = {0, 0};
             s_ok = tail_exchange(e, seq0 + (unsigned)(k - k0), host_vals, host_seq, answer,
@@ -359,8 +627,14 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
 }
 
 
+// Rounds whose fold has at most this many workgroups write their partials
+// straight into mapped host memory and the host adds them once the launch is
+// done, instead of a one-workgroup publish launch (which costs its own
+// duration plus a dependent-launch gap of ~6 us per round).
+constexpr unsigned kHostSumBlocks = 256;
+
 // Event-timed launch classes of one prove (timing on).
-enum Sc2Class { SC_FIRST, SC_FOLD01, SC_FOLD, SC_TAIL, SC_NCLASS };
+enum Sc2Class { SC_FIRST, SC_FOLD01, SC_FOLD, SC_TAIL, SC_MID, SC_NCLASS };
 
 struct Sc2State {
     DevBuf work;                 // T_k ping-pong: A (n/2) and B (n/4), for f and g
@@ -382,12 +656,21 @@ struct Sc2State {
     int tail_rounds = 0;         // tail profile of the last timed prove
     double tail_work_us = 0, tail_wait_us = 0, tail_host_us = 0;
     double tail_round_us[40] = {};   // device us of each tail round
+    int mid_rounds = 0;          // mid profile of the last timed prove
+    double mid_host_us = 0, mid_relay_us = 0, mid_work_us = 0, mid_spread_us = 0;
     unsigned seq = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t ev_round = nullptr;   // end of a host-summed round
+    fp2* host_parts = nullptr;       // mapped: partials of host-summed rounds
+    fp2* dev_parts = nullptr;
     bool timed = false;          // per-launch event timing
-    double ms[SC_NCLASS] = {0, 0, 0, 0};
-    double bytes[SC_NCLASS] = {0, 0, 0, 0};
-    int launches[SC_NCLASS] = {0, 0, 0, 0};
+    double ms[SC_NCLASS] = {};
+    double bytes[SC_NCLASS] = {};
+    int launches[SC_NCLASS] = {};
+    DevBuf mid_parts;            // the mid rounds' tagged partials, [kMidGroups][6]
+    DevBuf relay;                // the mid rounds' challenge relay (2 words)
+    long long* host_mtrace = nullptr;   // mapped: [40][kMidGroups][2] stamps (timing on)
+    long long* dev_mtrace = nullptr;
 };
 Sc2State g_sc2;
 
@@ -415,6 +698,15 @@ bool sc2_setup(size_t n)
         S.dev_trace = reinterpret_cast<long long*>(dc + 1024);
         TP_HIP_OK(hipFuncSetAttribute((const void*)k_sc2_tail_host,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLds));
+        TP_HIP_OK(hipFuncSetAttribute((const void*)k_sc2_mid,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLds));
+        void* tp_ = nullptr;
+        TP_HIP_OK(hipHostMalloc(&tp_, 40 * kMidGroups * 2 * sizeof(long long),
+                                hipHostMallocMapped | hipHostMallocCoherent));
+        S.host_mtrace = static_cast<long long*>(tp_);
+        void* dtp = nullptr;
+        TP_HIP_OK(hipHostGetDevicePointer(&dtp, tp_, 0));
+        S.dev_mtrace = static_cast<long long*>(dtp);
         int khz = 0, dev = 0;
         TP_HIP_OK(hipGetDevice(&dev));
         TP_HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
@@ -424,9 +716,19 @@ bool sc2_setup(size_t n)
     if (!S.ev[0]) {
         TP_HIP_OK(hipEventCreate(&S.ev[0]));
         TP_HIP_OK(hipEventCreate(&S.ev[1]));
+        TP_HIP_OK(hipEventCreateWithFlags(&S.ev_round, hipEventDisableTiming));
+        void* hp = nullptr;
+        TP_HIP_OK(hipHostMalloc(&hp, kHostSumBlocks * 3 * sizeof(fp2),
+                                hipHostMallocMapped | hipHostMallocCoherent));
+        S.host_parts = static_cast<fp2*>(hp);
+        void* dp = nullptr;
+        TP_HIP_OK(hipHostGetDevicePointer(&dp, hp, 0));
+        S.dev_parts = static_cast<fp2*>(dp);
     }
     if (!S.work.ensure(2 * (n / 2 + n / 4 + 2) * sizeof(V2))) return false;
     if (!S.partials.ensure(2048 * 9 * sizeof(fp2))) return false;
+    if (!S.relay.ensure(2 * sizeof(u64))) return false;
+    if (!S.mid_parts.ensure(kMidGroups * 6 * sizeof(u64))) return false;
     return true;
 }
 
@@ -460,6 +762,22 @@ bool wait_seq(hipStream_t st, unsigned seq)
     std::atomic_thread_fence(std::memory_order_acquire);
     return true;
 }
+
+// Waits until the launches before `ev` on its stream are done.
+bool wait_event(hipEvent_t ev)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    for (long spins = 1;; ++spins) {
+        const hipError_t q = hipEventQuery(ev);
+        if (q == hipSuccess) break;
+        if (q != hipErrorNotReady) return fail("sumcheck stream error: %s", hipGetErrorString(q));
+        if (spins % 4096 == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(30))
+            return fail("sumcheck: timed out waiting for a round");
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return true;
+}
+
 
 }  // namespace
 
@@ -552,7 +870,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         k_sc2_first01<<<nb, kBlock, 0, st>>>(cf, cg, n / 4, partials);
         if (!t_end(SC_FIRST, 32.0 * (double)n)) return -1;
         TP_HIP_OK(hipGetLastError());
-        k_publish_sum<9><<<1, kBlock, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
+        k_publish_wide<9><<<1, 1024, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
         TP_HIP_OK(hipGetLastError());
         if (!wait_seq(st, seq) || !t_collect(SC_FIRST)) return -1;
         fp2 v[9];
@@ -573,7 +891,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
                                                   partials);
         if (!t_end(SC_FOLD01, 40.0 * (double)n)) return -1;
         TP_HIP_OK(hipGetLastError());
-        k_publish_sum<3><<<1, kBlock, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
+        k_publish_wide<3><<<1, 1024, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
         TP_HIP_OK(hipGetLastError());
         if (!wait_seq(st, seq) || !t_collect(SC_FOLD01)) return -1;
         r = round_done(2, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
@@ -588,7 +906,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         k_sc2_first<<<nb, kBlock, 0, st>>>(cf, cg, n / 2, partials);
         if (!t_end(SC_FIRST, 32.0 * (double)n)) return -1;
         TP_HIP_OK(hipGetLastError());
-        k_publish_sum<3><<<1, kBlock, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
+        k_publish_wide<3><<<1, 1024, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
         TP_HIP_OK(hipGetLastError());
         if (!wait_seq(st, seq) || !t_collect(SC_FIRST)) return -1;
         r = round_done(0, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
@@ -640,21 +958,120 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             final4[2] = S.host_vals[1].re; final4[3] = S.host_vals[1].im;
             return 0;
         }
+        if (2 * m <= (size_t)kMidGroups * kTailIn) {
+            // rounds k .. k + nr - 1 on slices in kMidGroups workgroups; their
+            // last table (2 entries per workgroup) goes to the tail
+            const unsigned slice = (unsigned)(2 * m / kMidGroups);   // >= 128 here
+            int nr = 0;
+            while ((slice >> (nr + 1)) >= 2) ++nr;   // log2(slice) - 1
+            V2* of = (cf >= A && cf < A + 2 * capA) ? B : A;
+            V2* og = of + (of == A ? capA : capB);
+            const unsigned seq0 = S.seq + 1;
+            S.seq += (unsigned)nr;
+            const V2* a_f = cf;
+            const V2* a_g = cg;
+            unsigned a_slice = slice;
+            int a_nr = nr;
+            u64 a_rr = r.re, a_ri = r.im;
+            u64* a_parts = S.mid_parts.as<u64>();
+            fp2* a_vals = S.dev_vals;
+            unsigned* a_seq = S.dev_seq;
+            unsigned a_seq0 = seq0;
+            const u64* a_ans = S.dev_answer;
+            unsigned* a_ans_seq = S.dev_answer_seq;
+            long long a_spin = S.spin_ticks;
+            V2* a_of = of;
+            V2* a_og = og;
+            u64* a_relay = S.relay.as<u64>();
+            long long* a_trace = S.timed && nr <= 40 ? S.dev_mtrace : nullptr;
+            void* args[] = {&a_f, &a_g, &a_slice, &a_nr, &a_rr, &a_ri, &a_parts, &a_vals, &a_seq,
+                            &a_seq0, &a_ans, &a_ans_seq, &a_spin, &a_of, &a_og, &a_relay, &a_trace};
+            // all-ones words: value bits p under tag 7, never a round's word
+            TP_HIP_OK(hipMemsetAsync(a_relay, 0xff, 2 * sizeof(u64), st));
+            TP_HIP_OK(hipMemsetAsync(a_parts, 0xff, kMidGroups * 6 * sizeof(u64), st));
+            if (!t_begin()) return -1;
+            TP_HIP_OK(hipLaunchCooperativeKernel((const void*)k_sc2_mid, dim3(kMidGroups),
+                                                 dim3(kTailThreads), args,
+                                                 (unsigned)(slice * sizeof(V2)), st));
+            if (!t_end(SC_MID, 32.0 * (double)(2 * m) + 64.0 * kMidGroups)) return -1;
+            volatile u64* ans = S.host_answer;
+            volatile unsigned* ans_seq = S.host_answer_seq;
+            for (int i = 0; i < nr; ++i) {
+                if (!wait_seq(st, seq0 + i)) return -1;
+                r = round_done(k + i, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
+                if (i + 1 < nr) {
+                    ans[0] = r.re;
+                    ans[1] = r.im;
+                    std::atomic_thread_fence(std::memory_order_release);
+                    *ans_seq = seq0 + i;
+                }
+            }
+            TP_HIP_OK(hipEventRecord(S.ev_round, st));
+            if (!wait_event(S.ev_round) || !t_collect(SC_MID)) return -1;
+            if (S.timed && nr > 1 && nr <= 40) {
+                // per round: out = partials written, in = challenge held
+                const volatile long long* tr = S.host_mtrace;
+                auto out_t = [&](int i, int b) { return (double)tr[((size_t)i * kMidGroups + b) * 2]; };
+                auto in_t = [&](int i, int b) { return (double)tr[((size_t)i * kMidGroups + b) * 2 + 1]; };
+                double host = 0, relay = 0, work = 0, spread = 0;
+                for (int i = 0; i + 1 < nr; ++i) {
+                    double last_out = out_t(i, 0), first_out = out_t(i, 0), last_in = in_t(i, 0);
+                    double w = 0;
+                    for (int b = 0; b < kMidGroups; ++b) {
+                        last_out = std::max(last_out, out_t(i, b));
+                        first_out = std::min(first_out, out_t(i, b));
+                        last_in = std::max(last_in, in_t(i, b));
+                        w += out_t(i + 1, b) - in_t(i, b);
+                    }
+                    host += in_t(i, 0) - last_out;      // last partials -> challenge at workgroup 0
+                    relay += last_in - in_t(i, 0);      // workgroup 0 -> the last to hold it
+                    work += w / kMidGroups;             // challenge -> next partials
+                    spread += last_out - first_out;     // partials: first to last
+                }
+                const double d = S.clock_mhz * (nr - 1);
+                S.mid_rounds = nr;
+                S.mid_host_us = host / d;
+                S.mid_relay_us = relay / d;
+                S.mid_work_us = work / d;
+                S.mid_spread_us = spread / d;
+            }
+            cf = of;
+            cg = og;
+            k += nr - 1;          // the loop's ++k moves to the first round after these
+            m = kMidGroups;       // T_k for that round has kMidGroups entries per table
+            continue;
+        }
         const unsigned seq = ++S.seq;
         const size_t quads = m / 2;   // input T_{k-1} has length 2m
         V2* of = (k & 1) ? A : B;
         V2* og = of + ((k & 1) ? capA : capB);
         const unsigned nb = blocks_for(quads);
+        const bool host_sum = nb <= kHostSumBlocks;
         if (!t_begin()) return -1;
-        k_sc2_fold_round<<<nb, kBlock, 0, st>>>(cf, cg, quads, r.re, r.im, of, og, partials);
+        k_sc2_fold_round<<<nb, kBlock, 0, st>>>(cf, cg, quads, r.re, r.im, of, og,
+                                                host_sum ? S.dev_parts : partials);
         if (!t_end(SC_FOLD, 48.0 * (double)(2 * m))) return -1;
         TP_HIP_OK(hipGetLastError());
-        k_publish_sum<3><<<1, kBlock, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
-        TP_HIP_OK(hipGetLastError());
-        if (!wait_seq(st, seq) || !t_collect(SC_FOLD)) return -1;
+        fp2 E[3];
+        if (host_sum) {
+            TP_HIP_OK(hipEventRecord(S.ev_round, st));
+            if (!wait_event(S.ev_round) || !t_collect(SC_FOLD)) return -1;
+            const volatile fp2* hp = S.host_parts;
+            E[0] = E[1] = E[2] = zero2();
+            for (unsigned b = 0; b < nb; ++b)
+                for (int i = 0; i < 3; ++i)
+                    E[i] = add2(E[i], mk2(hp[3 * b + i].re, hp[3 * b + i].im));
+        } else {
+            k_publish_wide<3><<<1, 1024, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
+            TP_HIP_OK(hipGetLastError());
+            if (!wait_seq(st, seq) || !t_collect(SC_FOLD)) return -1;
+            E[0] = S.host_vals[0];
+            E[1] = S.host_vals[1];
+            E[2] = S.host_vals[2];
+        }
         cf = of;
         cg = og;
-        r = round_done(k, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
+        r = round_done(k, E[0], E[1], E[2]);
         m /= 2;
     }
 }
@@ -696,11 +1113,36 @@ void tp_sc2_tail_profile(double* out, int n)
     for (int i = 0; i < g_sc2.tail_rounds && i < 40 && 4 + i < n; ++i) out[4 + i] = g_sc2.tail_round_us[i];
 }
 
+// Per-round latency of the sliced mid rounds in the last timed tp_sc2_prove
+// (means over the rounds that wait for a challenge): out[0] rounds, out[1] us
+// from the last workgroup's partials to workgroup 0 holding the challenge
+// (host sum, transcript, answer), out[2] us until the last workgroup holds it
+// (the relay), out[3] us from a workgroup's challenge to its next partials,
+// out[4] us between the first and the last workgroup's partials.  n >= 5.
+void tp_sc2_mid_profile(double* out, int n)
+{
+    if (n < 5) return;
+    out[0] = g_sc2.mid_rounds;
+    out[1] = g_sc2.mid_host_us;
+    out[2] = g_sc2.mid_relay_us;
+    out[3] = g_sc2.mid_work_us;
+    out[4] = g_sc2.mid_spread_us;
+}
+
+// The schedule's sizes: the workgroups of the sliced mid rounds and the
+// largest input (entries per table) the one-workgroup tail takes.  The
+// streaming launches run while the input exceeds mid_groups * tail_in.
+void tp_sc2_schedule(int* mid_groups, int* tail_in)
+{
+    *mid_groups = kMidGroups;
+    *tail_in = (int)kTailIn;
+}
+
 // Per-class statistics of the last tp_sc2_prove (timing on): for each class
 // (0 the first pass over the input, 1 the round-2 fold from the input, 2 the
-// streaming fold+round launches, 3 the one-workgroup tail rounds) the summed
-// event ms, the bytes the launches read and write, and the launch count:
-// out[3 c + 0..2].  n >= 12.
+// streaming fold+round launches, 3 the one-workgroup tail rounds, 4 the
+// sliced mid rounds) the summed event ms, the bytes the launches read and
+// write, and the launch count: out[3 c + 0..2].  n >= 15.
 void tp_sc2_class_stats(double* out, int n)
 {
     for (int c = 0; c < SC_NCLASS && 3 * c + 2 < n; ++c) {

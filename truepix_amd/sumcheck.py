@@ -98,17 +98,17 @@ def tail_stats():
     return ms.value, n.value
 
 
-CLASSES = ("first", "fold01", "fold", "tail")
+CLASSES = ("first", "fold01", "fold", "tail", "mid")
 
 
 def class_stats():
     """Per launch class of the last prove (timing on): {"first" | "fold01" |
-    "fold" | "tail": {"ms", "bytes", "launches"}}.  "first" is the pass over
-    the input giving rounds 0 and 1, "fold01" round 2 folding the input by
-    r_0 and r_1, "fold" the streaming fold+round launches, "tail" the
-    one-workgroup rounds."""
-    out = (ctypes.c_double * 12)()
-    nb.LIB.tp_sc2_class_stats(out, 12)
+    "fold" | "tail" | "mid": {"ms", "bytes", "launches"}}.  "first" is the
+    pass over the input giving rounds 0 and 1, "fold01" round 2 folding the
+    input by r_0 and r_1, "fold" the streaming fold+round launches, "tail" the
+    one-workgroup rounds, "mid" the rounds on per-workgroup slices."""
+    out = (ctypes.c_double * 15)()
+    nb.LIB.tp_sc2_class_stats(out, 15)
     return {c: {"ms": out[3 * i], "bytes": out[3 * i + 1], "launches": int(out[3 * i + 2])}
             for i, c in enumerate(CLASSES)}
 
@@ -123,3 +123,22 @@ def tail_profile():
     n = int(out[0])
     return {"rounds": n, "device_us": round(out[1], 3), "wait_us": round(out[2], 3),
             "host_us": round(out[3], 3), "round_device_us": [round(out[4 + i], 2) for i in range(n)]}
+
+
+def mid_profile():
+    """Per-round latency of the sliced mid rounds in the last timed prove:
+    {"rounds", "host_us" (last partials to the challenge at workgroup 0),
+    "relay_us" (to the last workgroup), "device_us" (challenge to the next
+    partials), "spread_us" (first to last workgroup's partials)}."""
+    out = (ctypes.c_double * 5)()
+    nb.LIB.tp_sc2_mid_profile(out, 5)
+    return {"rounds": int(out[0]), "host_us": round(out[1], 3), "relay_us": round(out[2], 3),
+            "device_us": round(out[3], 3), "spread_us": round(out[4], 3)}
+
+
+def schedule():
+    """(mid_groups, tail_in): workgroups of the sliced mid rounds and the
+    largest input (entries per table) of the one-workgroup tail."""
+    a, b = ctypes.c_int(), ctypes.c_int()
+    nb.LIB.tp_sc2_schedule(ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
