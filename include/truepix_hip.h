@@ -118,6 +118,13 @@ void tp_sc2_stats(double* fold_ms, double* fold_bytes, int* fold_launches);
 /* Summed event milliseconds and count of the one-workgroup tail rounds
  * (fold + round sum + publish in one launch) of the last tp_sc2_prove. */
 void tp_sc2_tail_stats(double* tail_ms, int* tail_launches);
+/* Bound of the sliced mid launch's arrival gate in wall-clock ticks (default
+ * 20 ms; negative restores it): the launch needs all its workgroups resident
+ * at once and otherwise gives way to streaming launches for those rounds.  0
+ * makes it give way unless every workgroup has already arrived (tests). */
+void tp_sc2_set_gate_ticks(long long ticks);
+/* Mid launches of the last tp_sc2_prove that gave way at the gate. */
+int tp_sc2_mid_fallbacks(void);
 /* The prove schedule's sizes: workgroups of the sliced mid rounds, and the
  * largest input (entries per table) of the one-workgroup tail.  Streaming
  * launches run while the input exceeds mid_groups * tail_in. */

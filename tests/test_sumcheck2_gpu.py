@@ -101,3 +101,25 @@ def test_sc2_launch_classes(bridge, torch_cuda):
             tail_bytes = 32 * (n >> (k_mid - 1))
         assert cls["tail"]["launches"] == 1 and cls["tail"]["bytes"] == tail_bytes
         assert all(c["ms"] > 0 for c in cls.values() if c["launches"])
+
+
+def test_sc2_mid_gate_fallback(bridge, oracle, torch_cuda):
+    """The sliced mid launch needs all its workgroups resident; when its
+    arrival gate closes before they are (forced here with a zero bound), the
+    rounds go to streaming launches and the proof is unchanged."""
+    nv = 20
+    rng = np.random.default_rng(77)
+    f, g = rand_fp2(rng, 1 << nv), rand_fp2(rng, 1 << nv)
+    want = oracle.sc2_prove(f, g, transcript_chal(77), nthreads=8)
+    sc.set_gate_ticks(0)
+    try:
+        got = sc.prove_deg2_host(f, g, seed=77)
+        fallbacks = sc.mid_fallbacks()
+    finally:
+        sc.set_gate_ticks(-1)
+    assert np.array_equal(got.polys, want[0])
+    assert np.array_equal(got.chals, want[1])
+    assert np.array_equal(got.final, want[2])
+    assert fallbacks in (0, 1)
+    got2 = sc.prove_deg2_host(f, g, seed=77)
+    assert sc.mid_fallbacks() == 0 and np.array_equal(got2.polys, want[0])

@@ -27,10 +27,22 @@
 // polynomials; the second pass folds the input by r_0 and r_1 straight into
 // T_2 (read 32 n, write 8 n), so T_1 is never written or read.
 //
-// Round trip per round: fused launch -> one-workgroup finalize launch that
-// writes the sums and a sequence number straight into mapped pinned host
-// memory -> the host polls the sequence number, runs the transcript, and
-// launches the next round with the challenge as a kernel argument.
+// Every challenge is a SHA-256 draw of the host transcript, so each round
+// goes to the host and back.  The schedule, by the size of the round's input:
+//   - above kMidGroups * kTailIn entries (streaming rounds): a fused launch,
+//     a one-workgroup publish launch writing the sums and a sequence number
+//     straight into mapped pinned host memory, the host polling it, running
+//     the transcript and launching the next round with the challenge as a
+//     kernel argument;
+//   - down to 2 kMidGroups entries (k_sc2_mid): one launch of kMidGroups
+//     workgroups, each folding its own slice in LDS, the sums gathered on the
+//     device and the challenges coming back through mapped memory;
+//   - the rest (k_sc2_tail_host): one workgroup, the same exchange.
+// A launch plus its dependent-launch gap costs ~6 us here, more than a
+// small round's work; the last two phases pay it once for all their rounds.
+// A one-lane SHA-256 on the device was measured at 4.8 us per compression
+// (~26 us per transcript round), against ~3.4 us for the host round trip, so
+// the transcript stays on the host.
 #include "fp61.hpp"
 #include "host_sha.hpp"
 #include "reduce.hpp"
@@ -125,7 +137,13 @@ __device__ __forceinline__ void accum_quad01(LazyProdSum (&z)[9], const V2* fp, 
     z[8].mac(sub2(dd, d), sub2(cc, c));
 }
 
-__global__ void __launch_bounds__(256) k_sc2_first01(const V2* __restrict__ f,
+// 128 VGPRs hold four waves per SIMD, so one 16-wave workgroup per CU (256
+// workgroups of 1024 threads) is the same occupancy as more, smaller ones,
+// with an eighth of the partial sums for the publish launch to add.
+constexpr int kFirst01Threads = 1024;
+constexpr unsigned kFirst01Groups = 256;
+
+__global__ void __launch_bounds__(kFirst01Threads) k_sc2_first01(const V2* __restrict__ f,
                                                      const V2* __restrict__ g, size_t n_quads,
                                                      fp2* partials)
 {
@@ -138,35 +156,45 @@ __global__ void __launch_bounds__(256) k_sc2_first01(const V2* __restrict__ f,
 }
 
 // Round 2 from the input: fold T_0 by r_0 and r_1 into T_2 and sum the
-// round-2 pairs.  Oct o = inputs 8o..8o+7 -> outputs 2o, 2o+1.
-__device__ __forceinline__ void fold_oct(const V2* p, fp2 r0, fp2 r1, fp2& o0, fp2& o1)
+// round-2 pairs.  Lane j owns output j (inputs 4j..4j+3, 64 contiguous bytes
+// per table, as the streaming rounds read) and stores it (16 B, contiguous
+// across the wave); output pairs (2p, 2p + 1) sit in adjacent lanes, and the
+// even lane takes its neighbour's outputs by a shuffle for the pair sums.
+// With a lane owning eight inputs instead, one load instruction touched 64
+// lines 128 B apart and the L2 dropped lines before their other loads came:
+// 1.23x the algorithmic reads.
+__device__ __forceinline__ fp2 fold_quad01(const V2* p, fp2 r0, fp2 r1)
 {
     const fp2 a0 = fold2(ldv(p), ldv(p + 1), r0), a1 = fold2(ldv(p + 2), ldv(p + 3), r0);
-    const fp2 a2 = fold2(ldv(p + 4), ldv(p + 5), r0), a3 = fold2(ldv(p + 6), ldv(p + 7), r0);
-    o0 = fold2(a0, a1, r1);
-    o1 = fold2(a2, a3, r1);
+    return fold2(a0, a1, r1);
+}
+
+__device__ __forceinline__ fp2 shfl_xor1(fp2 x)
+{
+    return mk2(__shfl_xor(x.re, 1, 64), __shfl_xor(x.im, 1, 64));
 }
 
 __global__ void __launch_bounds__(256) k_sc2_fold01_round(const V2* __restrict__ fin,
                                                           const V2* __restrict__ gin,
-                                                          size_t n_octs, u64 r0r, u64 r0i,
+                                                          size_t n_out, u64 r0r, u64 r0i,
                                                           u64 r1r, u64 r1i,
                                                           V2* __restrict__ fout,
                                                           V2* __restrict__ gout, fp2* partials)
 {
     const fp2 r0 = mk2(r0r, r0i), r1 = mk2(r1r, r1i);
+    const bool even = (threadIdx.x & 1) == 0;
     LazyProdSum z[3];
     lazy_zero(z);
-    for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < n_octs;
-         o += (size_t)gridDim.x * blockDim.x) {
-        fp2 f0, f1, g0, g1;
-        fold_oct(fin + 8 * o, r0, r1, f0, f1);
-        fold_oct(gin + 8 * o, r0, r1, g0, g1);
-        stv(fout + 2 * o, f0);
-        stv(fout + 2 * o + 1, f1);
-        stv(gout + 2 * o, g0);
-        stv(gout + 2 * o + 1, g1);
-        accum_pair(z, f0, f1, g0, g1);
+    // n_out is even and the stride a multiple of 64, so both lanes of a pair
+    // run every iteration together
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < n_out;
+         j += (size_t)gridDim.x * blockDim.x) {
+        const fp2 f = fold_quad01(fin + 4 * j, r0, r1);
+        const fp2 g = fold_quad01(gin + 4 * j, r0, r1);
+        stv(fout + j, f);
+        stv(gout + j, g);
+        const fp2 fo = shfl_xor1(f), go = shfl_xor1(g);
+        if (even) accum_pair(z, f, fo, g, go);
     }
     lazy_partial<3>(z, partials);
 }
@@ -252,9 +280,9 @@ __device__ __forceinline__ void tail_fold(const V2* sf, const V2* sg, unsigned q
 }
 
 // Rounds on a table too large for one workgroup but small enough that the
-// launches, not the bytes, set the time: kMidGroups workgroups (launched
-// cooperatively, so all are resident) each keep a contiguous slice of the
-// tables in LDS.  Variables bind LSB-first, so a slice of S entries folds
+// launches, not the bytes, set the time: kMidGroups workgroups, all resident
+// at once (checked by an arrival gate at the start), each keep a contiguous
+// slice of the tables in LDS.  Variables bind LSB-first, so a slice of S entries folds
 // into its own S/2 entries and its pairs stay inside it for log2(S) - 1
 // rounds.  Per round every workgroup writes its three partial sums to its six
 // words of `parts` (device memory) as agent-scope stores, each value (61 bits)
@@ -269,6 +297,21 @@ __device__ __forceinline__ void tail_fold(const V2* sf, const V2* sg, unsigned q
 // tagged words the same way.  Every wait is bounded by the wall clock; an
 // expired wait publishes kSeqAbort and relays p, and every workgroup leaves.
 constexpr int kMidGroups = 64;
+constexpr unsigned kGatePending = 0, kGateGo = 1, kGateAbort = 2;
+constexpr unsigned kSeqGateAbort = 0xfffffffeu;   // the mid launch did not start whole
+
+// Fresh state for a mid launch: the tagged partial words and the relay all
+// ones (value bits p under tag 7: no round's word), the gate at no arrivals
+// and no decision.  One launch instead of three fills.
+__global__ void __launch_bounds__(256) k_sc2_mid_reset(u64* parts, u64* relay, unsigned* gate)
+{
+    for (int j = threadIdx.x; j < kMidGroups * 6; j += blockDim.x) parts[j] = ~0ull;
+    if (threadIdx.x == 0) {
+        relay[0] = relay[1] = ~0ull;
+        gate[0] = 0;
+        gate[1] = kGatePending;
+    }
+}
 
 // End of a mid round, run by wave 0 of every workgroup: publish this
 // workgroup's sums (lane 0), and in workgroup 0 gather every workgroup's sums,
@@ -386,7 +429,7 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
     const V2* __restrict__ fin, const V2* __restrict__ gin, unsigned slice_in, int n_rounds,
     u64 rr, u64 ri, u64* parts, fp2* host_vals, unsigned* host_seq, unsigned seq0,
     const u64* answer, unsigned* answer_seq, long long spin_ticks, V2* __restrict__ fout,
-    V2* __restrict__ gout, u64* relay, long long* trace)
+    V2* __restrict__ gout, u64* relay, long long* trace, unsigned* gate, long long gate_ticks)
 {
     extern __shared__ V2 tl[];
     __shared__ u64 s_r[2];
@@ -397,6 +440,49 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
     V2* tg = tl + half;
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     const unsigned b = blockIdx.x;
+    // Arrival gate: the rounds need every workgroup resident at once.  Each
+    // counts itself into gate[0]; workgroup 0 waits up to gate_ticks for all
+    // of them and sets gate[1] to kGateGo, or to kGateAbort when some are
+    // missing (a GPU shared with other work); a workgroup that sees no
+    // decision within 2 gate_ticks sets kGateAbort itself.  Both settings are
+    // compare-and-swaps from kGatePending, so there is one decision; on abort
+    // every workgroup leaves and the host proves these rounds with streaming
+    // launches instead.
+    if (t == 0) {
+        __hip_atomic_fetch_add(gate, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const long long t0 = wall_clock64();
+        unsigned d = kGatePending;
+        if (b == 0) {
+            while (__hip_atomic_load(gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gridDim.x &&
+                   wall_clock64() - t0 <= gate_ticks)
+                __builtin_amdgcn_s_sleep(1);
+            const unsigned want =
+                __hip_atomic_load(gate, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= gridDim.x
+                    ? kGateGo : kGateAbort;
+            unsigned e = kGatePending;   // on failure e holds the decision already made
+            d = __hip_atomic_compare_exchange_strong(gate + 1, &e, want, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                    ? want : e;
+        } else {
+            while ((d = __hip_atomic_load(gate + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ==
+                       kGatePending &&
+                   wall_clock64() - t0 <= 2 * gate_ticks)
+                __builtin_amdgcn_s_sleep(1);
+            if (d == kGatePending) {
+                unsigned e = kGatePending;
+                if (__hip_atomic_compare_exchange_strong(gate + 1, &e, kGateAbort, __ATOMIC_RELAXED,
+                                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    d = kGateAbort;
+                else
+                    d = e;
+            }
+        }
+        if (d != kGateGo && b == 0)
+            __hip_atomic_store(host_seq, kSeqGateAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        s_ok = d == kGateGo;
+    }
+    __syncthreads();
+    if (!s_ok) return;
     const V2* sf0 = fin + (size_t)b * slice_in;
     const V2* sg0 = gin + (size_t)b * slice_in;
     fp2 r = mk2(rr, ri);
@@ -627,12 +713,6 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
 }
 
 
-// Rounds whose fold has at most this many workgroups write their partials
-// straight into mapped host memory and the host adds them once the launch is
-// done, instead of a one-workgroup publish launch (which costs its own
-// duration plus a dependent-launch gap of ~6 us per round).
-constexpr unsigned kHostSumBlocks = 256;
-
 // Event-timed launch classes of one prove (timing on).
 enum Sc2Class { SC_FIRST, SC_FOLD01, SC_FOLD, SC_TAIL, SC_MID, SC_NCLASS };
 
@@ -660,15 +740,16 @@ struct Sc2State {
     double mid_host_us = 0, mid_relay_us = 0, mid_work_us = 0, mid_spread_us = 0;
     unsigned seq = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
-    hipEvent_t ev_round = nullptr;   // end of a host-summed round
-    fp2* host_parts = nullptr;       // mapped: partials of host-summed rounds
-    fp2* dev_parts = nullptr;
+    hipEvent_t ev_round = nullptr;   // end of a mid launch
     bool timed = false;          // per-launch event timing
     double ms[SC_NCLASS] = {};
     double bytes[SC_NCLASS] = {};
     int launches[SC_NCLASS] = {};
     DevBuf mid_parts;            // the mid rounds' tagged partials, [kMidGroups][6]
     DevBuf relay;                // the mid rounds' challenge relay (2 words)
+    DevBuf gate;                 // the mid launch's arrival gate (2 words)
+    long long gate_ticks = 0;    // its wait bound in wall-clock ticks
+    int mid_fallbacks = 0;       // mid launches that did not start whole (last prove)
     long long* host_mtrace = nullptr;   // mapped: [40][kMidGroups][2] stamps (timing on)
     long long* dev_mtrace = nullptr;
 };
@@ -712,23 +793,18 @@ bool sc2_setup(size_t n)
         TP_HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
         S.spin_ticks = (long long)(khz > 0 ? khz : 100000) * 1000LL * 10;   // 10 s
         S.clock_mhz = (khz > 0 ? khz : 100000) / 1000.0;
+        S.gate_ticks = (long long)(khz > 0 ? khz : 100000) * 20;   // 20 ms
     }
     if (!S.ev[0]) {
         TP_HIP_OK(hipEventCreate(&S.ev[0]));
         TP_HIP_OK(hipEventCreate(&S.ev[1]));
         TP_HIP_OK(hipEventCreateWithFlags(&S.ev_round, hipEventDisableTiming));
-        void* hp = nullptr;
-        TP_HIP_OK(hipHostMalloc(&hp, kHostSumBlocks * 3 * sizeof(fp2),
-                                hipHostMallocMapped | hipHostMallocCoherent));
-        S.host_parts = static_cast<fp2*>(hp);
-        void* dp = nullptr;
-        TP_HIP_OK(hipHostGetDevicePointer(&dp, hp, 0));
-        S.dev_parts = static_cast<fp2*>(dp);
     }
     if (!S.work.ensure(2 * (n / 2 + n / 4 + 2) * sizeof(V2))) return false;
     if (!S.partials.ensure(2048 * 9 * sizeof(fp2))) return false;
     if (!S.relay.ensure(2 * sizeof(u64))) return false;
     if (!S.mid_parts.ensure(kMidGroups * 6 * sizeof(u64))) return false;
+    if (!S.gate.ensure(2 * sizeof(unsigned))) return false;
     return true;
 }
 
@@ -761,6 +837,31 @@ bool wait_seq(hipStream_t st, unsigned seq)
     }
     std::atomic_thread_fence(std::memory_order_acquire);
     return true;
+}
+
+// The first round of a mid launch: 1 when sequence number seq is
+// published, 2 when the launch gave way at its arrival gate, 0 on failure.
+int wait_first_mid(hipStream_t st, unsigned seq)
+{
+    volatile unsigned* hs = g_sc2.host_seq;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (long spins = 1;; ++spins) {
+        const unsigned v = *hs;
+        if (v == seq) break;
+        if (v == kSeqGateAbort) return 2;
+        if (v == kSeqAbort) return fail("sumcheck: a mid-round workgroup gave up waiting"), 0;
+        if (spins % 4096 == 0) {
+            const hipError_t q = hipStreamQuery(st);
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return fail("sumcheck stream error: %s", hipGetErrorString(q)), 0;
+            if (q == hipSuccess && *hs != seq && *hs != kSeqGateAbort)
+                return fail("sumcheck: stream idle but round %u not published", seq), 0;
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(30))
+                return fail("sumcheck: timed out waiting for round %u", seq), 0;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return 1;
 }
 
 // Waits until the launches before `ev` on its stream are done.
@@ -865,12 +966,12 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
     if (nv >= 3) {
         // rounds 0 and 1 from one pass, round 2 folds the input by r_0, r_1
         unsigned seq = ++S.seq;
-        unsigned nb = blocks_for(n / 4);
+        unsigned nb = (unsigned)std::min<size_t>(kFirst01Groups, (n / 4 + kFirst01Threads - 1) / kFirst01Threads);
         if (!t_begin()) return -1;
-        k_sc2_first01<<<nb, kBlock, 0, st>>>(cf, cg, n / 4, partials);
+        k_sc2_first01<<<nb, kFirst01Threads, 0, st>>>(cf, cg, n / 4, partials);
         if (!t_end(SC_FIRST, 32.0 * (double)n)) return -1;
         TP_HIP_OK(hipGetLastError());
-        k_publish_wide<9><<<1, 1024, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
+        k_publish_sum<9><<<1, kBlock, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
         TP_HIP_OK(hipGetLastError());
         if (!wait_seq(st, seq) || !t_collect(SC_FIRST)) return -1;
         fp2 v[9];
@@ -885,9 +986,9 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         seq = ++S.seq;
         V2* of = B;
         V2* og = B + capB;
-        nb = blocks_for(n / 8);
+        nb = blocks_for(n / 4);
         if (!t_begin()) return -1;
-        k_sc2_fold01_round<<<nb, kBlock, 0, st>>>(cf, cg, n / 8, r0.re, r0.im, r1.re, r1.im, of, og,
+        k_sc2_fold01_round<<<nb, kBlock, 0, st>>>(cf, cg, n / 4, r0.re, r0.im, r1.re, r1.im, of, og,
                                                   partials);
         if (!t_end(SC_FOLD01, 40.0 * (double)n)) return -1;
         TP_HIP_OK(hipGetLastError());
@@ -913,6 +1014,8 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         k0 = 1;
         m = n / 2;
     }
+    bool use_mid = true;   // cleared when the mid launch did not start whole
+    S.mid_fallbacks = 0;
     for (int k = k0;; ++k) {
         if (2 * m <= kTailIn) {
             // rounds k..nv-1 and the final fold in one workgroup; the host
@@ -958,7 +1061,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             final4[2] = S.host_vals[1].re; final4[3] = S.host_vals[1].im;
             return 0;
         }
-        if (2 * m <= (size_t)kMidGroups * kTailIn) {
+        if (use_mid && 2 * m <= (size_t)kMidGroups * kTailIn) {
             // rounds k .. k + nr - 1 on slices in kMidGroups workgroups; their
             // last table (2 entries per workgroup) goes to the tail
             const unsigned slice = (unsigned)(2 * m / kMidGroups);   // >= 128 here
@@ -984,20 +1087,36 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             V2* a_og = og;
             u64* a_relay = S.relay.as<u64>();
             long long* a_trace = S.timed && nr <= 40 ? S.dev_mtrace : nullptr;
+            unsigned* a_gate = S.gate.as<unsigned>();
+            long long a_gate_ticks = S.gate_ticks;
             void* args[] = {&a_f, &a_g, &a_slice, &a_nr, &a_rr, &a_ri, &a_parts, &a_vals, &a_seq,
-                            &a_seq0, &a_ans, &a_ans_seq, &a_spin, &a_of, &a_og, &a_relay, &a_trace};
-            // all-ones words: value bits p under tag 7, never a round's word
-            TP_HIP_OK(hipMemsetAsync(a_relay, 0xff, 2 * sizeof(u64), st));
-            TP_HIP_OK(hipMemsetAsync(a_parts, 0xff, kMidGroups * 6 * sizeof(u64), st));
+                            &a_seq0, &a_ans, &a_ans_seq, &a_spin, &a_of, &a_og, &a_relay, &a_trace,
+                            &a_gate, &a_gate_ticks};
+            k_sc2_mid_reset<<<1, 256, 0, st>>>(a_parts, a_relay, a_gate);
+            TP_HIP_OK(hipGetLastError());
             if (!t_begin()) return -1;
-            TP_HIP_OK(hipLaunchCooperativeKernel((const void*)k_sc2_mid, dim3(kMidGroups),
-                                                 dim3(kTailThreads), args,
-                                                 (unsigned)(slice * sizeof(V2)), st));
+            TP_HIP_OK(hipLaunchKernel((const void*)k_sc2_mid, dim3(kMidGroups), dim3(kTailThreads),
+                                      args, (unsigned)(slice * sizeof(V2)), st));
             if (!t_end(SC_MID, 32.0 * (double)(2 * m) + 64.0 * kMidGroups)) return -1;
+            const int first = wait_first_mid(st, seq0);
+            if (first == 0) return -1;
+            if (first == 2) {
+                // not every workgroup was resident: these rounds go to the
+                // streaming launches (the tables were only read)
+                TP_HIP_OK(hipEventRecord(S.ev_round, st));
+                if (!wait_event(S.ev_round)) return -1;
+                S.launches[SC_MID] -= 1;
+                S.bytes[SC_MID] -= 32.0 * (double)(2 * m) + 64.0 * kMidGroups;
+                S.seq += 1;   // past the abort mark
+                S.mid_fallbacks += 1;
+                use_mid = false;
+                --k;          // the loop's ++k returns to this round
+                continue;
+            }
             volatile u64* ans = S.host_answer;
             volatile unsigned* ans_seq = S.host_answer_seq;
             for (int i = 0; i < nr; ++i) {
-                if (!wait_seq(st, seq0 + i)) return -1;
+                if (i > 0 && !wait_seq(st, seq0 + i)) return -1;
                 r = round_done(k + i, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
                 if (i + 1 < nr) {
                     ans[0] = r.re;
@@ -1046,32 +1165,16 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         V2* of = (k & 1) ? A : B;
         V2* og = of + ((k & 1) ? capA : capB);
         const unsigned nb = blocks_for(quads);
-        const bool host_sum = nb <= kHostSumBlocks;
         if (!t_begin()) return -1;
-        k_sc2_fold_round<<<nb, kBlock, 0, st>>>(cf, cg, quads, r.re, r.im, of, og,
-                                                host_sum ? S.dev_parts : partials);
+        k_sc2_fold_round<<<nb, kBlock, 0, st>>>(cf, cg, quads, r.re, r.im, of, og, partials);
         if (!t_end(SC_FOLD, 48.0 * (double)(2 * m))) return -1;
         TP_HIP_OK(hipGetLastError());
-        fp2 E[3];
-        if (host_sum) {
-            TP_HIP_OK(hipEventRecord(S.ev_round, st));
-            if (!wait_event(S.ev_round) || !t_collect(SC_FOLD)) return -1;
-            const volatile fp2* hp = S.host_parts;
-            E[0] = E[1] = E[2] = zero2();
-            for (unsigned b = 0; b < nb; ++b)
-                for (int i = 0; i < 3; ++i)
-                    E[i] = add2(E[i], mk2(hp[3 * b + i].re, hp[3 * b + i].im));
-        } else {
-            k_publish_wide<3><<<1, 1024, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
-            TP_HIP_OK(hipGetLastError());
-            if (!wait_seq(st, seq) || !t_collect(SC_FOLD)) return -1;
-            E[0] = S.host_vals[0];
-            E[1] = S.host_vals[1];
-            E[2] = S.host_vals[2];
-        }
+        k_publish_wide<3><<<1, 1024, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
+        TP_HIP_OK(hipGetLastError());
+        if (!wait_seq(st, seq) || !t_collect(SC_FOLD)) return -1;
         cf = of;
         cg = og;
-        r = round_done(k, E[0], E[1], E[2]);
+        r = round_done(k, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
         m /= 2;
     }
 }
@@ -1128,6 +1231,18 @@ void tp_sc2_mid_profile(double* out, int n)
     out[3] = g_sc2.mid_work_us;
     out[4] = g_sc2.mid_spread_us;
 }
+
+// Bound of the mid launch's arrival gate in wall-clock ticks (default 20 ms;
+// a negative value restores it).  0 makes the launch give way unless every
+// workgroup has arrived when workgroup 0 looks, for tests of the fallback.
+void tp_sc2_set_gate_ticks(long long ticks)
+{
+    if (!sc2_setup(2)) return;
+    g_sc2.gate_ticks = ticks >= 0 ? ticks : (long long)(g_sc2.clock_mhz * 1000.0) * 20;
+}
+
+// Mid launches of the last tp_sc2_prove that gave way at the arrival gate.
+int tp_sc2_mid_fallbacks(void) { return g_sc2.mid_fallbacks; }
 
 // The schedule's sizes: the workgroups of the sliced mid rounds and the
 // largest input (entries per table) the one-workgroup tail takes.  The

@@ -142,3 +142,14 @@ def schedule():
     a, b = ctypes.c_int(), ctypes.c_int()
     nb.LIB.tp_sc2_schedule(ctypes.byref(a), ctypes.byref(b))
     return a.value, b.value
+
+
+def set_gate_ticks(ticks: int) -> None:
+    """Bound of the mid launch's arrival gate in wall-clock ticks (negative:
+    the default 20 ms; 0: give way unless every workgroup already arrived)."""
+    nb.LIB.tp_sc2_set_gate_ticks(int(ticks))
+
+
+def mid_fallbacks() -> int:
+    """Mid launches of the last prove that gave way to streaming launches."""
+    return int(nb.LIB.tp_sc2_mid_fallbacks())
