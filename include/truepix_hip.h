@@ -356,6 +356,33 @@ int tp_fft_gkr_trace(int lg_size, uint64_t* polys, int cap_polys, int* n_polys, 
  * every launch is enqueued directly. */
 void tp_fft_gkr_set_graph(int on);
 
+/* ---- the frame's macroblock layout: VideoEdition/Frame_Block_Extraction.py
+ *      :67-94 (process_macroblocks) and :220-240 (one input.txt line per
+ *      macroblock), VideoEdition/Crop.py:79-85 (the crop bit appended),
+ *      VideoEdition/Encode.py:19-42 (reconstruct_frame).  The reference does
+ *      these steps in numpy on the host and hands the prover and the signer a
+ *      text file; these build the same rows on the device. */
+
+/* YUV 4:2:0 planes (device) -> copies x copy_bytes rows (device): Y 16x16,
+ * Cb 8x8, Cr 8x8 per macroblock in raster order, partial edge macroblocks
+ * dropped; copy_bytes 385 appends extra[copy] (the crop bit), 384 takes no
+ * extra.  Even width and height >= 16; strides >= plane widths.  Returns the
+ * copies ((width/16)*(height/16)) or -1. */
+long long tp_yuv_pack(const uint8_t* y, const uint8_t* cb, const uint8_t* cr, int width,
+                      int height, long long y_stride, long long c_stride, const uint8_t* extra,
+                      int copy_bytes, uint8_t* out, void* stream);
+/* copies x copy_stride rows (device, first 384 bytes used) -> planes
+ * (device), zeros outside the macroblock area.  0 / -1. */
+int tp_yuv_unpack(const uint8_t* mb, long long copy_stride, int width, int height,
+                  long long y_stride, long long c_stride, uint8_t* y, uint8_t* cb, uint8_t* cr,
+                  void* stream);
+/* The same from and to host buffers (tightly packed planes), staged through
+ * the device; synchronous. */
+long long tp_yuv_pack_host(const uint8_t* y, const uint8_t* cb, const uint8_t* cr, int width,
+                           int height, const uint8_t* extra, int copy_bytes, uint8_t* out);
+int tp_yuv_unpack_host(const uint8_t* mb, long long copy_stride, int width, int height, uint8_t* y,
+                       uint8_t* cb, uint8_t* cr);
+
 /* ---- host pieces of the frame prover, exported for their CPU tests ------- */
 
 /* A15, TP/libTruePix/fiatshamir.py:234-237: pickle.dumps(tree, 5) of a

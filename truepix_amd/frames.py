@@ -203,6 +203,15 @@ class FramePool:
             raise RuntimeError("tp_pool_submit failed: %s" % nb.last_error())
         return int(jid)
 
+    def submit_yuv(self, circuit: FrameCircuit, y, cb, cr, pid: int, vole, binding: list,
+                   extra=None, keep_outputs: bool = False) -> int:
+        """Queue one frame given as YUV 4:2:0 planes: its per-copy rows are
+        built on the device (``yuv.macroblocks``; ``extra`` appends a byte per
+        copy, e.g. ``yuv.crop_bits`` for the crop circuit), then as ``submit``."""
+        from truepix_amd import yuv
+        return self.submit(circuit, yuv.macroblocks(y, cb, cr, extra), pid, vole, binding,
+                           keep_outputs)
+
     def submit_many(self, specs) -> List[int]:
         """Queue frames at once, in order: specs are submit()'s argument
         tuples.  A batch driver admits them together."""

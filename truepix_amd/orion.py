@@ -50,6 +50,13 @@ def commit(inputs, seed: int, timing: bool = False):
     return roots + (ms.value,) if timing else roots
 
 
+def commit_yuv(y, cb, cr, seed: int, timing: bool = False):
+    """``commit`` of a frame given as YUV 4:2:0 planes, its per-copy rows
+    built on the device (``yuv.macroblocks``)."""
+    from truepix_amd import yuv
+    return commit(yuv.macroblocks(y, cb, cr), seed, timing)
+
+
 def manifest(inputs, seed: int, zk: bool = True) -> dict:
     """The orion_commit.json the reference's commit phase writes for these
     inputs and mask seed (truepix_gkr.h:428-455)."""
