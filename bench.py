@@ -346,6 +346,51 @@ def bench_served(n_jobs: int = 2000, rate: float = 200.0, seed: int = 5, configs
     return out
 
 
+def bench_yuv(width: int = 1920, height: int = 1080, reps: int = 200, cpu: bool = True):
+    """A 1080p frame's planes, resident on the device, packed into the
+    prover's 8040 rows of 384 bytes and written back (tp_yuv_pack /
+    tp_yuv_unpack), event-timed on the launch stream; bytes = read + write.
+    CPU baseline: the oracle's numpy restatement of the reference's host
+    packing (VideoEdition/Frame_Block_Extraction.py:67-94, 220-240)."""
+    import numpy as np
+    import torch
+    from truepix_amd import yuv
+    rng = np.random.default_rng(7)
+    y = rng.integers(0, 256, (height, width), dtype=np.uint8)
+    cb = rng.integers(0, 256, (height // 2, width // 2), dtype=np.uint8)
+    cr = rng.integers(0, 256, (height // 2, width // 2), dtype=np.uint8)
+    ty, tcb, tcr = (torch.from_numpy(a).cuda() for a in (y, cb, cr))
+    n = yuv.copies(width, height)
+    rows = torch.empty((n, 384), dtype=torch.uint8, device="cuda")
+    out = {}
+    for name, fn in (("pack", lambda: yuv.macroblocks_device(ty, tcb, tcr, out=rows)),
+                     ("unpack", lambda: yuv.frame_device(rows, width, height))):
+        for _ in range(5):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        nbytes = 2.0 * n * 384
+        out[name] = {"us_per_frame": round(us, 2), "gbs": round(nbytes / us / 1e3, 1)}
+    rec = {"frame": "%dx%d" % (width, height), "copies": n, "bytes_per_frame": 2 * n * 384, **out}
+    if cpu:
+        from oracle import yuv_oracle
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < 3.0:
+            yuv_oracle.pack(y, cb, cr)
+            k += 1
+        dt = (time.perf_counter() - t0) / k
+        rec["cpu_baseline"] = {"value": round(dt * 1e6, 1), "unit": "us per frame", "cores": 1,
+                               "kind": "port", "sample": "%d packs of the same frame" % k}
+    return rec
+
+
 def bench_orion_commit(copies: int = 1350, reps: int = 5, seed: int = 777, cpu: bool = True):
     """The signer's Orion commitment of one SD frame (1350 macroblocks: n = 2^20
     coefficients per polynomial, 128 columns of 8192, codewords of 14,090):
@@ -470,7 +515,8 @@ def main(argv=None):
                     help="comma list of batched pool labels to run (default: all), e.g. "
                          "lockstep-1x256")
     ap.add_argument("--skip", default="", help="comma list of side measurements to skip "
-                    "(gkr_frame, gkr_frame_python_driver, batched, served, fft_gkr, rs_encode)")
+                    "(gkr_frame, gkr_frame_python_driver, batched, served, fft_gkr, orion, yuv, "
+                    "rs_encode)")
     a = ap.parse_args(argv)
     a.skip = set(x for x in a.skip.split(",") if x)
     if a.gpus != 1:
@@ -599,6 +645,12 @@ def main(argv=None):
         except Exception as exc:
             line["orion_commit"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
         print("bench: orion_commit done", file=sys.stderr, flush=True)
+    if "yuv" not in a.skip:
+        try:
+            line["yuv_layout"] = bench_yuv(cpu=not a.no_cpu_baseline)
+        except Exception as exc:
+            line["yuv_layout"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
+        print("bench: yuv_layout done", file=sys.stderr, flush=True)
     if "rs_encode" not in a.skip:
         try:
             line["rs_encode"] = bench_rs_encode()
