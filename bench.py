@@ -432,6 +432,53 @@ def bench_orion_commit(copies: int = 1350, reps: int = 5, seed: int = 777, cpu: 
             out["speedup_vs_cpu"] = round(dt * 1e3 / out["ms_wall"], 1)
         except Exception as exc:  # the baseline must not hide the device result
             out["cpu_baseline"] = {"error": str(exc)}
+    out["open"] = bench_orion_open(x, seed, cpu=cpu)
+    return out
+
+
+def bench_orion_open(x, seed: int, reps: int = 5, n_queries: int = 1222, cpu: bool = True):
+    """An opening of the frame's masked polynomial (tensor_product_protocol
+    up to its Virgo proof): re-encode and hash as the commit does, combine the
+    128 codewords with r0, hash and tree the combined codeword, gather the
+    1222 queried columns and both trees' siblings, serialize the
+    OpeningProof; wall time including the proof's copy to the host.  CPU: the
+    reference's own code for the same (oracle/_ref), one core."""
+    import hashlib
+    import numpy as np
+    from truepix_amd import orion
+    copies = x.shape[0]
+    pc = 1 << max(0, (copies - 1).bit_length())
+    lg = (pc * 512).bit_length() - 1
+    rng = np.random.Generator(np.random.PCG64(11))
+    r = rng.integers(0, (1 << 61) - 1, size=(lg, 2), dtype=np.uint64)
+    proof, _ = orion.opening(x, seed, r, [0], use_group1=True)
+    cw = orion.parse_opening(proof)["cw_len"]
+    q = rng.integers(0, cw, size=n_queries, dtype=np.int64)
+    walls = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        proof, ans = orion.opening(x, seed, r, q, use_group1=True)
+        walls.append(time.perf_counter() - t0)
+    walls.sort()
+    out = {"workload": "Orion opening of one frame's masked polynomial: %d copies, %d queries"
+                       % (copies, n_queries),
+           "ms_wall": round(walls[len(walls) // 2] * 1e3, 3), "proof_bytes": len(proof),
+           "proof_sha256": hashlib.sha256(proof).hexdigest()}
+    if cpu:
+        try:
+            from oracle.oracle import RefOrion, Oracle
+            t0 = time.perf_counter()
+            if RefOrion.available():
+                ref, kind = RefOrion().open(x, seed, r, q, True), "reference"
+            else:
+                ref, kind = Oracle().orion_open(x, seed, r, q, True)[0], "port"
+            dt = time.perf_counter() - t0
+            out["cpu_baseline"] = {"ms": round(dt * 1e3, 1), "cores": 1, "kind": kind,
+                                   "sample": "one opening, the same frame, point and queries",
+                                   "proof_equal": ref == proof}
+            out["speedup_vs_cpu"] = round(dt * 1e3 / out["ms_wall"], 1)
+        except Exception as exc:
+            out["cpu_baseline"] = {"error": str(exc)}
     return out
 
 

@@ -327,6 +327,17 @@ void tp_pool_release(void* pool, int64_t job);
  * device time.  copies <= 65536.  0 / -1. */
 int tp_orion_commit(int copies, const uint8_t* inputs, uint64_t seed, char masked_root[65],
                     char mask_root[65], double* ms);
+/* OZ/src/VPD/linearPC.cpp:700-966 (tensor_product_protocol; r0, r1 formed
+ * from the point r as open_and_verify does, :996-1034): the opening of the
+ * committed masked polynomial (use_group1) or the mask at r (n_r = lg n
+ * challenges, (re, im) each) at the code positions col_q[n_queries] the
+ * transcript drew, serialized in the reference's OpeningProof version-2
+ * layout (OZ/include/truepix/orion_proof.h:1-22) into *proof (free with
+ * tp_free); the claimed value in answer.  The Virgo code-switching proof that
+ * follows in the reference is not part of it.  0 / -1. */
+int tp_orion_open(int copies, const uint8_t* inputs, uint64_t seed, int use_group1,
+                  const uint64_t* r, int n_r, const int64_t* col_q, int n_queries,
+                  uint8_t** proof, size_t* proof_len, uint64_t answer[2]);
 /* The expander code alone (encode, OZ/include/linear_code/linear_code_encode.h:35-110,
  * graphs from glibc's initial rand() state as expander_init draws them):
  * n_cols <= 256 columns of n <= 65536 Fp2, src [n_cols][n] and dst

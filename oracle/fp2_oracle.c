@@ -1047,3 +1047,161 @@ int or_orion_commit(int copies, const unsigned char* inputs, u64 seed, char mask
     if (cw_len_out) *cw_len_out = cw;
     return 0;
 }
+
+/* The opening of one committed polynomial (linearPC.cpp:700-966
+ * tensor_product_protocol with r0, r1 from the point r as open_and_verify
+ * forms them, :996-1034; the masked polynomial when use_group1, else the
+ * mask), serialized as OpeningProof version 2 (orion_proof.h:1-22,
+ * orion_proof.cpp:69-100) into a malloc'd buffer.  0 / -1. */
+static void ob_put(unsigned char** o, size_t* n, size_t* cap, const void* p, size_t k)
+{
+    if (*n + k > *cap) {
+        *cap = (*n + k) * 2;
+        *o = (unsigned char*)realloc(*o, *cap);
+    }
+    memcpy(*o + *n, p, k);
+    *n += k;
+}
+static void ob_fe(unsigned char** o, size_t* n, size_t* cap, fp2 x)
+{
+    unsigned char b[16];
+    put_le64(b, x.im);
+    put_le64(b + 8, x.re);
+    ob_put(o, n, cap, b, 16);
+}
+static void ob_siblings(unsigned char** o, size_t* n, size_t* cap, const unsigned char* tree,
+                        long long leaves, const long long* q, int nq)
+{
+    char* seen = (char*)calloc((size_t)(2 * leaves), 1);
+    uint32_t* idx = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(nq * 64 + 1));
+    u64 cnt = 0;
+    for (int i = 0; i < nq; ++i)
+        for (long long v = q[i] + leaves; v != 1; v /= 2)
+            if (!seen[v ^ 1]) {
+                seen[v ^ 1] = 1;
+                idx[cnt++] = (uint32_t)(v ^ 1);
+            }
+    ob_put(o, n, cap, &cnt, 8);
+    for (u64 k = 0; k < cnt; ++k) {
+        ob_put(o, n, cap, &idx[k], 4);
+        ob_put(o, n, cap, tree + (size_t)idx[k] * 32, 32);
+    }
+    free(seen);
+    free(idx);
+}
+
+int or_orion_open(int copies, const unsigned char* inputs, u64 seed, int use_group1,
+                  const u64* r, int n_r, const long long* col_q, int nq, unsigned char** proof,
+                  size_t* proof_len, u64 answer[2])
+{
+    if (copies < 1) return -1;
+    long long pc = 1;
+    while (pc < copies) pc <<= 1;
+    const long long n = pc * 512, col = n / X_COLS, rows = 2 * col;
+    int lg = 0;
+    while ((1LL << lg) < n) ++lg;
+    if (n_r != lg) return -1;
+    grand rg;
+    grand_init(&rg, 1);
+    xg_init(&rg, col, 0);
+    const long long cw = x_len(col, 0);
+    fp2* poly = (fp2*)calloc((size_t)n, sizeof(fp2));
+    u64 state = seed;
+    for (long long c = 0; c < copies; ++c)
+        for (int w = 0; w < 384; ++w) {
+            state += 0x9E3779B97F4A7C15ULL;
+            const u64 m = smix(state) % P61;
+            poly[c * 512 + w] = use_group1 ? mk(fadd(inputs[c * 384 + w], m), 0) : mk(m, 0);
+        }
+    if (use_group1)
+        for (long long c = 0; c < copies; ++c) poly[c * 512 + 384] = mk(128, 0);
+    fp2* cwb = (fp2*)malloc(sizeof(fp2) * (size_t)X_COLS * cw);
+    for (int c = 0; c < X_COLS; ++c) x_encode(poly + c * col, col, 0, cwb + (long long)c * cw);
+    unsigned char* tcom = (unsigned char*)malloc((size_t)(2 * rows) * 32);
+    unsigned char* tcmb = (unsigned char*)malloc((size_t)(2 * rows) * 32);
+    for (long long rr = 0; rr < rows; ++rr) {
+        unsigned char msg[64];
+        memset(msg, 0, 32);
+        for (int j = 0; j < X_COLS / 2; ++j) {
+            for (int h = 0; h < 2; ++h) {
+                const fp2 e = rr < cw ? cwb[(long long)(2 * j + h) * cw + rr] : mk(0, 0);
+                put_le64(msg + 32 + 16 * h, e.im);
+                put_le64(msg + 40 + 16 * h, e.re);
+            }
+            or_sha256(msg, 64, msg);
+        }
+        memcpy(tcom + (rows + rr) * 32, msg, 32);
+    }
+    for (long long v = rows - 1; v >= 1; --v) or_sha256(tcom + 2 * v * 32, 64, tcom + v * 32);
+    /* r0 over r[0..7), r1 over the rest, the first challenge on the top bit */
+    const int lc = 7;
+    fp2* r0 = (fp2*)malloc(sizeof(fp2) * X_COLS);
+    fp2* r1 = (fp2*)malloc(sizeof(fp2) * (size_t)col);
+    for (long long i = 0; i < X_COLS; ++i) {
+        fp2 v = mk(1, 0);
+        for (int d = 0; d < lc; ++d) {
+            const fp2 x = mk(r[2 * d], r[2 * d + 1]);
+            v = mul2(v, ((i >> (lc - 1 - d)) & 1) ? x : sub2(mk(1, 0), x));
+        }
+        r0[i] = v;
+    }
+    for (long long t = 0; t < col; ++t) {
+        fp2 v = mk(1, 0);
+        for (int d = 0; d < lg - lc; ++d) {
+            const fp2 x = mk(r[2 * (lc + d)], r[2 * (lc + d) + 1]);
+            v = mul2(v, ((t >> (lg - lc - 1 - d)) & 1) ? x : sub2(mk(1, 0), x));
+        }
+        r1[t] = v;
+    }
+    fp2* cc = (fp2*)calloc((size_t)cw, sizeof(fp2));
+    fp2* cm = (fp2*)calloc((size_t)col, sizeof(fp2));
+    for (int j = 0; j < X_COLS; ++j) {
+        for (long long t = 0; t < cw; ++t) cc[t] = add2(cc[t], mul2(r0[j], cwb[(long long)j * cw + t]));
+        for (long long t = 0; t < col; ++t) cm[t] = add2(cm[t], mul2(r0[j], poly[j * col + t]));
+    }
+    fp2 ans = mk(0, 0);
+    for (long long t = 0; t < col; ++t) ans = add2(ans, mul2(r1[t], cm[t]));
+    for (long long rr = 0; rr < rows; ++rr) {
+        unsigned char msg[64];
+        memset(msg, 0, 64);
+        const fp2 e = rr < cw ? cc[rr] : mk(0, 0);
+        put_le64(msg, e.im);
+        put_le64(msg + 8, e.re);
+        or_sha256(msg, 64, tcmb + (rows + rr) * 32);
+    }
+    for (long long v = rows - 1; v >= 1; --v) or_sha256(tcmb + 2 * v * 32, 64, tcmb + v * 32);
+    unsigned char* o = NULL;
+    size_t on = 0, cap = 0;
+    const uint32_t version = 2, group = use_group1 ? 1 : 0, qn = (uint32_t)nq;
+    const u64 hdr[4] = {(u64)n, (u64)X_COLS, (u64)col, (u64)cw};
+    ob_put(&o, &on, &cap, "ORIONPV1", 8);
+    ob_put(&o, &on, &cap, &version, 4);
+    ob_put(&o, &on, &cap, &group, 4);
+    ob_put(&o, &on, &cap, hdr, 32);
+    ob_put(&o, &on, &cap, &qn, 4);
+    ob_fe(&o, &on, &cap, ans);
+    ob_put(&o, &on, &cap, tcom + 32, 32);
+    ob_put(&o, &on, &cap, tcmb + 32, 32);
+    for (int i = 0; i < nq; ++i) ob_put(&o, &on, &cap, &col_q[i], 8);
+    for (int i = 0; i < nq; ++i)
+        for (int j = 0; j < X_COLS; ++j) ob_fe(&o, &on, &cap, cwb[(long long)j * cw + col_q[i]]);
+    for (long long t = 0; t < cw; ++t) ob_fe(&o, &on, &cap, cc[t]);
+    for (long long t = 0; t < col; ++t) ob_fe(&o, &on, &cap, cm[t]);
+    ob_siblings(&o, &on, &cap, tcom, rows, col_q, nq);
+    ob_siblings(&o, &on, &cap, tcmb, rows, col_q, nq);
+    *proof = o;
+    *proof_len = on;
+    answer[0] = ans.re;
+    answer[1] = ans.im;
+    free(poly);
+    free(cwb);
+    free(tcom);
+    free(tcmb);
+    free(r0);
+    free(r1);
+    free(cc);
+    free(cm);
+    return 0;
+}
+
+void or_free(void* p) { free(p); }

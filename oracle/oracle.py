@@ -203,14 +203,49 @@ def _oracle_sha256(self, data: bytes) -> bytes:
     return out.raw
 
 
+_OPEN_ARGS = [_INT, ctypes.c_char_p, _U64, _INT, _U64P, _INT, ctypes.POINTER(ctypes.c_longlong), _INT,
+              ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(_SZ)]
+
+
+def _orion_open(fn, free, inputs, seed, use_group1, r, col_q, with_answer):
+    """OpeningProof bytes (and the answer when the function returns it)."""
+    arr = np.ascontiguousarray(inputs, dtype=np.uint8)
+    rr = np.ascontiguousarray(np.asarray(r, dtype=np.uint64).reshape(-1))
+    q = np.ascontiguousarray(col_q, dtype=np.int64)
+    buf = ctypes.c_void_p()
+    n = _SZ()
+    extra = []
+    ans = np.zeros(2, np.uint64)
+    if with_answer:
+        extra = [_p(ans)]
+    rc = fn(int(arr.shape[0]), arr.tobytes(), int(seed) & (2 ** 64 - 1), 1 if use_group1 else 0,
+            _p(rr), rr.size // 2, q.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), q.size,
+            ctypes.byref(buf), ctypes.byref(n), *extra)
+    if rc != 0:
+        raise ValueError("orion open rejected")
+    data = ctypes.string_at(buf.value, n.value)
+    free(buf)
+    return (data, (int(ans[0]), int(ans[1]))) if with_answer else data
+
+
+def _oracle_orion_open(self, inputs, seed: int, r, col_q, use_group1: bool = True):
+    """(OpeningProof bytes, answer) of one committed polynomial."""
+    if not hasattr(self, "_oro"):
+        self._oro = _bind(self.lib, "or_orion_open", _OPEN_ARGS + [_U64P], _INT)
+        self._orf = _bind(self.lib, "or_free", [ctypes.c_void_p])
+    return _orion_open(self._oro, self._orf, inputs, seed, use_group1, r, col_q, True)
+
+
+Oracle.orion_open = _oracle_orion_open
 Oracle.orion_commit = _oracle_orion_commit
 Oracle.orion_encode = _oracle_orion_encode
 Oracle.sha256 = _oracle_sha256
 
 
 class RefOrion:
-    """The signer's commitment computed with the reference's own layout, mask,
-    expander code, row hash and Merkle tree (oracle/ref_harness_orion.cpp)."""
+    """The signer's commitment and an opening computed with the reference's
+    own layout, mask, expander code, row hash, Merkle tree, siblings and
+    proof serializer (oracle/ref_harness_orion.cpp)."""
 
     PATH = os.path.join(HERE, "_ref", "liborion_ref.so")
 
@@ -225,6 +260,13 @@ class RefOrion:
 
     def commit(self, inputs, seed: int):
         return _orion_commit(self.c, inputs, seed)
+
+    def open(self, inputs, seed: int, r, col_q, use_group1: bool = True) -> bytes:
+        if not hasattr(self, "o"):
+            lib = ctypes.CDLL(self.PATH)
+            self.o = _bind(lib, "ref_orion_open", _OPEN_ARGS, _INT)
+            self.f = _bind(lib, "ref_free", [ctypes.c_void_p])
+        return _orion_open(self.o, self.f, inputs, seed, use_group1, r, col_q, False)
 
     def encode(self, col):
         return _orion_encode(self.e, col)

@@ -5,7 +5,9 @@ and encode of OZ/include/linear_code/, the row hash and create_tree of
 OZ/src/infrastructure/merkle_tree.cpp over flo-shani's SHA-256, compiled in
 place by `make -C oracle ref` and driven in commit_pair's order by
 oracle/ref_harness_orion.cpp): the masked and mask roots and the codeword
-length for seeded inputs, and SHA-256 digests of single-column codewords.
+length for seeded inputs, SHA-256 digests of single-column codewords, and
+SHA-256 digests of opening proofs (OpeningProof bytes from orion_proof.cpp's
+serializer) at seeded points and queries.
 Run where /root/reference exists:
 
     make -C oracle ref && python tests/golden/make_golden_orion.py
@@ -27,6 +29,11 @@ sys.path.insert(0, ROOT)
 COMMIT_CASES = [(1, 5, 1), (3, 7, 2), (5, 11, 3), (16, 2024, 4), (33, 99, 5), (300, 12345, 6),
                 (1350, 777, 7)]
 ENCODE_SIZES = [5, 13, 14, 31, 64, 100, 1024, 8192]
+# openings (tensor_product_protocol, with orion_proof.cpp's siblings and
+# serializer): (copies, mask seed, input seed, use_group1, point seed,
+# queries); the point and the queries from PCG64(point seed)
+OPEN_CASES = [(1, 5, 1, 1, 21, 8), (3, 7, 2, 0, 22, 16), (16, 2024, 4, 1, 23, 64),
+              (300, 12345, 6, 0, 24, 200), (1350, 777, 7, 1, 25, 1222)]
 
 
 def inputs_for(copies, seed):
@@ -37,6 +44,19 @@ def inputs_for(copies, seed):
 def column_for(n, seed):
     rng = np.random.Generator(np.random.PCG64(seed))
     return rng.integers(0, (1 << 61) - 1, size=(n, 2), dtype=np.uint64)
+
+
+def open_args(copies, point_seed, nq, cw_len):
+    """The point r (lg n challenges, (re, im)) and nq code positions."""
+    pc = 1
+    while pc < copies:
+        pc <<= 1
+    lg = (pc * 512).bit_length() - 1
+    rng = np.random.Generator(np.random.PCG64(point_seed))
+    r = rng.integers(0, (1 << 61) - 1, size=(lg, 2), dtype=np.uint64)
+    q = rng.integers(0, cw_len, size=nq, dtype=np.int64)
+    q[:2] = (0, cw_len - 1)
+    return r, q
 
 
 def main():
@@ -58,9 +78,19 @@ def main():
         out["encode"].append({"n": n, "cw_len": int(cwd.shape[0]),
                               "sha256": hashlib.sha256(cwd.tobytes()).hexdigest(),
                               "head": [int(x) for x in cwd[:4].reshape(-1)]})
+    out["open"] = []
+    cw_of = {c["copies"]: c["cw_len"] for c in out["commit"]}
+    for copies, seed, iseed, group, pseed, nq in OPEN_CASES:
+        r, q = open_args(copies, pseed, nq, cw_of[copies])
+        proof = ref.open(inputs_for(copies, iseed), seed, r, q, use_group1=bool(group))
+        out["open"].append({"copies": copies, "seed": seed, "input_seed": iseed,
+                            "use_group1": group, "point_seed": pseed, "queries": nq,
+                            "proof_len": len(proof),
+                            "proof_sha256": hashlib.sha256(proof).hexdigest()})
     with open(os.path.join(HERE, "orion_ref.json"), "w") as fh:
         json.dump(out, fh, indent=1)
-    print("wrote", len(out["commit"]), "commit cases,", len(out["encode"]), "encode cases")
+    print("wrote", len(out["commit"]), "commit cases,", len(out["encode"]), "encode cases,",
+          len(out["open"]), "openings")
 
 
 if __name__ == "__main__":

@@ -109,3 +109,98 @@ def test_commit_matches_live_reference(orc):
     for n in (17, 250):
         col = column_for(n, 7 * n)
         assert np.array_equal(orc.orion_encode(col), ref.encode(col)), n
+
+
+# ------------------------------------------------------------------ opening
+
+def open_args(copies, point_seed, nq, cw_len):
+    pc = 1
+    while pc < copies:
+        pc <<= 1
+    lg = (pc * 512).bit_length() - 1
+    rng = np.random.Generator(np.random.PCG64(point_seed))
+    r = rng.integers(0, P, size=(lg, 2), dtype=np.uint64)
+    q = rng.integers(0, cw_len, size=nq, dtype=np.int64)
+    q[:2] = (0, cw_len - 1)
+    return r, q
+
+
+def test_open_matches_reference_proofs(orc, gold):
+    cw_of = {c["copies"]: c["cw_len"] for c in gold["commit"]}
+    for case in gold["open"]:
+        r, q = open_args(case["copies"], case["point_seed"], case["queries"], cw_of[case["copies"]])
+        proof, _ = orc.orion_open(inputs_for(case["copies"], case["input_seed"]), case["seed"], r, q,
+                                  use_group1=bool(case["use_group1"]))
+        assert len(proof) == case["proof_len"], case
+        assert hashlib.sha256(proof).hexdigest() == case["proof_sha256"], case
+
+
+def _eq_msb(r, k):
+    """eq weights over k bits, r[0] on the top bit (Python big ints)."""
+    out = [(1, 0)]
+    for d in range(k):
+        x = (int(r[d][0]), int(r[d][1]))
+        nx = ((1 - x[0]) % P, (-x[1]) % P)
+        # each new challenge takes the lowest bit, pushing r[0] up to the top
+        out = [w for v in out for w in (_fp2_mul(v, nx), _fp2_mul(v, x))]
+    return out
+
+
+def _walk(leaf, pos, leaves, sib):
+    node, v = leaf, pos + leaves
+    while v != 1:
+        s = sib[v ^ 1]
+        node = hashlib.sha256((node + s) if v % 2 == 0 else (s + node)).digest()
+        v //= 2
+    return node
+
+
+def _fe(x):
+    return int(x[1]).to_bytes(8, "little") + int(x[0]).to_bytes(8, "little")   # (imag, real)
+
+
+def test_open_proof_checks_out(orc):
+    """What the reference verifier checks of an opening, with independent
+    code: the combined codeword is the code of the combined message (the code
+    is linear and systematic), each queried column's r0-combination is the
+    combined codeword there, and both Merkle paths lead to the roots; the
+    opened root is the commitment's."""
+    from truepix_amd.orion import parse_opening
+    for copies, seed, group in ((2, 3, True), (9, 41, False)):
+        x = inputs_for(copies, copies)
+        _, _, cw = orc.orion_commit(x, seed)
+        r, q = open_args(copies, 100 + copies, 12, cw)
+        proof, ans = orc.orion_open(x, seed, r, q, use_group1=group)
+        o = parse_opening(proof)
+        assert o["answer"] == ans and o["cw_len"] == cw and o["col_q"] == list(q)
+        cm, cc = o["combined_message"], o["combined_codeword"]
+        assert np.array_equal(orc.orion_encode(cm), cc)
+        r0 = _eq_msb(r, 7)
+        rows = 2 * o["msg_len"]
+        for i, pos in enumerate(q):
+            col = o["queried_cols"][i]
+            s = (0, 0)
+            for j in range(128):
+                t = _fp2_mul(r0[j], (int(col[j][0]), int(col[j][1])))
+                s = ((s[0] + t[0]) % P, (s[1] + t[1]) % P)
+            assert s == (int(cc[pos][0]), int(cc[pos][1])), (copies, i)
+            h = bytes(32)
+            for j in range(64):
+                h = hashlib.sha256(h + _fe(col[2 * j]) + _fe(col[2 * j + 1])).digest()
+            assert _walk(h, int(pos), rows, o["com_siblings"]) == o["com_root"]
+            leaf = hashlib.sha256(_fe(cc[pos]) + bytes(48)).digest()
+            assert _walk(leaf, int(pos), rows, o["combined_siblings"]) == o["combined_root"]
+        # the opened tree is the committed one: its root prints as the manifest's
+        m, k, _ = orc.orion_commit(x, seed)
+        words = [int.from_bytes(o["com_root"][a:a + 8], "little") for a in (8, 0, 24, 16)]
+        assert "".join("%016x" % w for w in words) == (m if group else k)
+
+
+@pytest.mark.skipif(not RefOrion.available(), reason="reference build absent (make -C oracle ref)")
+def test_open_matches_live_reference(orc):
+    ref = RefOrion()
+    for copies, seed, group in ((4, 17, True), (40, 2 ** 62 + 9, False)):
+        x = inputs_for(copies, seed % 1000)
+        _, _, cw = orc.orion_commit(x, seed)
+        r, q = open_args(copies, seed % 997, 33, cw)
+        assert orc.orion_open(x, seed, r, q, use_group1=group)[0] == ref.open(x, seed, r, q, group)

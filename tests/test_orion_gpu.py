@@ -70,3 +70,37 @@ def test_manifest_binds_the_roots(bridge):
     assert (man["masked_root"], man["mask_root"]) == (m, k)
     assert man["n"] == 64 * 512 and man["padded_copies"] == 64
     assert commit_binding(man)[-2:] == [m, k]
+
+
+@pytest.mark.parametrize("copies,seed,group,nq", [(1, 5, True, 8), (3, 7, False, 16), (16, 2024, True, 64),
+                                                  (300, 12345, False, 200), (1350, 777, True, 1222),
+                                                  (3600, 31, False, 300)])
+def test_open_matches_oracle(bridge, oracle, copies, seed, group, nq):
+    """The device opening (tp_orion_open) gives the oracle's OpeningProof
+    bytes and answer; the oracle's bytes are pinned to the reference's own
+    serializer (tests/test_orion_cpu.py)."""
+    x = inputs_for(copies, copies + 3)
+    _, _, cw = oracle.orion_commit(x, seed)
+    pc = 1
+    while pc < copies:
+        pc <<= 1
+    lg = (pc * 512).bit_length() - 1
+    rng = np.random.Generator(np.random.PCG64(copies))
+    r = rng.integers(0, (1 << 61) - 1, size=(lg, 2), dtype=np.uint64)
+    q = rng.integers(0, cw, size=nq, dtype=np.int64)
+    q[:3] = (0, cw - 1, 0)   # both ends, and a repeated position
+    got, ans = orion.opening(x, seed, r, q, use_group1=group)
+    want, want_ans = oracle.orion_open(x, seed, r, q, use_group1=group)
+    assert ans == want_ans
+    assert got == want
+    assert orion.parse_opening(got)["answer"] == ans
+
+
+def test_open_rejects_bad_points_and_queries(bridge, oracle):
+    x = inputs_for(4, 1)
+    _, _, cw = oracle.orion_commit(x, 3)
+    r = np.ones((11, 2), np.uint64)
+    with pytest.raises(RuntimeError):
+        orion.opening(x, 3, r[:10], [0])          # lg n = 11 challenges needed
+    with pytest.raises(RuntimeError):
+        orion.opening(x, 3, r, [cw])              # past the codeword

@@ -50,6 +50,88 @@ def commit(inputs, seed: int, timing: bool = False):
     return roots + (ms.value,) if timing else roots
 
 
+def opening(inputs, seed: int, r, col_q, use_group1: bool = True):
+    """The opening of the committed masked polynomial (use_group1) or the mask
+    at the point r (lg n challenges, each (re, im)), at the code positions
+    col_q the transcript drew: (proof bytes in the reference's OpeningProof
+    version-2 layout, the claimed value (re, im)).  OZ/src/VPD/linearPC.cpp:
+    700-966 (tensor_product_protocol) up to its Virgo code-switching proof,
+    which is not part of it."""
+    nb.require()
+    arr = np.ascontiguousarray(inputs)
+    if arr.ndim != 2 or arr.shape[1] != COPY_SIZE:
+        raise ValueError("inputs must be [copies][%d]" % COPY_SIZE)
+    data = np.ascontiguousarray(arr, dtype=np.uint8)
+    rr = np.ascontiguousarray(np.asarray(r, dtype=np.uint64).reshape(-1))
+    q = np.ascontiguousarray(col_q, dtype=np.int64)
+    buf = ctypes.c_void_p()
+    n = ctypes.c_size_t()
+    ans = np.zeros(2, np.uint64)
+    rc = nb.LIB.tp_orion_open(int(arr.shape[0]), data.tobytes(), int(seed) & (2 ** 64 - 1),
+                              1 if use_group1 else 0, rr.ctypes.data_as(_U64P), rr.size // 2,
+                              q.ctypes.data, q.size, ctypes.byref(buf), ctypes.byref(n),
+                              ans.ctypes.data_as(_U64P))
+    if rc != 0:
+        raise RuntimeError("tp_orion_open failed: %s" % nb.last_error())
+    proof = ctypes.string_at(buf.value, n.value)
+    nb.LIB.tp_free(buf)
+    return proof, (int(ans[0]), int(ans[1]))
+
+
+def parse_opening(proof: bytes) -> dict:
+    """The fields of an OpeningProof version-2 byte string
+    (OZ/include/truepix/orion_proof.h:1-22); field elements as (re, im),
+    digests as bytes."""
+    import struct
+    o = 0
+
+    def take(fmt):
+        nonlocal o
+        v = struct.unpack_from("<" + fmt, proof, o)
+        o += struct.calcsize("<" + fmt)
+        return v
+
+    def fes(k):
+        nonlocal o
+        a = np.frombuffer(proof, dtype="<u8", count=2 * k, offset=o).reshape(k, 2)
+        o += 16 * k
+        return a[:, ::-1].copy()   # (imag, real) on disk
+
+    if proof[:8] != b"ORIONPV1":
+        raise ValueError("not an OpeningProof")
+    o = 8
+    version, group = take("II")
+    n, column_size, msg_len, cw_len = take("QQQQ")
+    (nq,) = take("I")
+    answer = tuple(int(x) for x in fes(1)[0])
+    com_root, comb_root = proof[o:o + 32], proof[o + 32:o + 64]
+    o += 64
+    col_q = list(take("%dq" % nq)) if nq else []
+    queried = fes(nq * column_size).reshape(nq, column_size, 2)
+    combined_codeword = fes(cw_len)
+    combined_message = fes(msg_len)
+
+    def siblings():
+        nonlocal o
+        (k,) = take("Q")
+        out = {}
+        for _ in range(k):
+            (i,) = take("I")
+            out[i] = proof[o:o + 32]
+            o += 32
+        return out
+
+    com_sib = siblings()
+    comb_sib = siblings()
+    if o != len(proof):
+        raise ValueError("trailing bytes in the OpeningProof")
+    return {"version": version, "use_group1": group, "N": n, "column_size": column_size,
+            "msg_len": msg_len, "cw_len": cw_len, "answer": answer, "com_root": com_root,
+            "combined_root": comb_root, "col_q": col_q, "queried_cols": queried,
+            "combined_codeword": combined_codeword, "combined_message": combined_message,
+            "com_siblings": com_sib, "combined_siblings": comb_sib}
+
+
 def commit_yuv(y, cb, cr, seed: int, timing: bool = False):
     """``commit`` of a frame given as YUV 4:2:0 planes, its per-copy rows
     built on the device (``yuv.macroblocks``)."""
