@@ -338,6 +338,13 @@ int tp_orion_commit(int copies, const uint8_t* inputs, uint64_t seed, char maske
 int tp_orion_open(int copies, const uint8_t* inputs, uint64_t seed, int use_group1,
                   const uint64_t* r, int n_r, const int64_t* col_q, int n_queries,
                   uint8_t** proof, size_t* proof_len, uint64_t answer[2]);
+/* tp_orion_open with the code positions drawn as the reference's NIZK mode
+ * draws them (OZ/src/truepix/orion_fs.cpp: "tensor-open", the opened root, N,
+ * the group flag, r0 and r1, then 1222 positions from the opened group;
+ * linearPC.cpp:711-758).  0 / -1. */
+int tp_orion_open_nizk(int copies, const uint8_t* inputs, uint64_t seed, int use_group1,
+                       const uint64_t* r, int n_r, uint8_t** proof, size_t* proof_len,
+                       uint64_t answer[2]);
 /* The expander code alone (encode, OZ/include/linear_code/linear_code_encode.h:35-110,
  * graphs from glibc's initial rand() state as expander_init draws them):
  * n_cols <= 256 columns of n <= 65536 Fp2, src [n_cols][n] and dst

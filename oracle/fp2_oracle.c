@@ -42,6 +42,7 @@
 #include <stddef.h>
 #include <stdint.h>
 #include <stdlib.h>
+#include <math.h>
 #include <string.h>
 
 typedef unsigned long long u64;
@@ -1202,6 +1203,95 @@ int or_orion_open(int copies, const unsigned char* inputs, u64 seed, int use_gro
     free(cc);
     free(cm);
     return 0;
+}
+
+/* The Orion NIZK transcript (OZ/src/truepix/orion_fs.cpp:24-113) and the
+ * opening's query draw (linearPC.cpp:711-758): reset ("truepix/orion-fs/v1"),
+ * "tensor-open", the opened root, N, the group flag, r0 then r1 (real, imag
+ * as separate absorbs), then query_count = (int)(-128 / log2(0.93)) = 1222
+ * positions from the opened group (even positions for the masked
+ * polynomial, odd for the mask). */
+typedef struct { unsigned char state[32], pool[32]; int used; } ofs;
+static void ofs_hash64(const unsigned char b[64], unsigned char out[32]) { or_sha256(b, 64, out); }
+static void ofs_absorb(ofs* f, const void* data, size_t n)
+{
+    const unsigned char* p = (const unsigned char*)data;
+    while (n > 0) {
+        unsigned char b[64];
+        memset(b, 0, 64);
+        memcpy(b, f->state, 32);
+        const size_t take = n < 32 ? n : 32;
+        memcpy(b + 32, p, take);
+        ofs_hash64(b, f->state);
+        p += take;
+        n -= take;
+    }
+    f->used = 32;
+}
+static void ofs_reset(ofs* f)
+{
+    memset(f->state, 0, 32);
+    ofs_absorb(f, "truepix/orion-fs/v1", 19);
+}
+static void ofs_u64(ofs* f, u64 x)
+{
+    unsigned char b[8];
+    put_le64(b, x);
+    ofs_absorb(f, b, 8);
+}
+static u64 ofs_squeeze(ofs* f)
+{
+    if (f->used + 8 > 32) {
+        unsigned char b[64];
+        memset(b, 0, 64);
+        memcpy(b, f->state, 32);
+        b[32] = 1;
+        ofs_hash64(b, f->pool);
+        b[32] = 2;
+        ofs_hash64(b, f->state);
+        f->used = 0;
+    }
+    u64 v = 0;
+    for (int k = 7; k >= 0; --k) v = (v << 8) | f->pool[f->used + k];
+    f->used += 8;
+    return v;
+}
+
+/* The positions or_orion_open_nizk opens: writes query_count positions to q
+ * (capacity >= 1222) given the opened root (digest bytes), N, the group, r. */
+int or_orion_draw_queries(const unsigned char root[32], long long n, int use_group1, long long cw,
+                          const u64* r, int n_r, long long* q)
+{
+    ofs f;
+    ofs_reset(&f);
+    ofs_absorb(&f, "tensor-open", 11);
+    ofs_absorb(&f, root, 32);
+    ofs_u64(&f, (u64)n);
+    ofs_u64(&f, use_group1 ? 1 : 0);
+    const int lc = 7;
+    const long long col = n / X_COLS;
+    for (long long i = 0; i < X_COLS; ++i) {
+        fp2 v = mk(1, 0);
+        for (int d = 0; d < lc; ++d) {
+            const fp2 x = mk(r[2 * d], r[2 * d + 1]);
+            v = mul2(v, ((i >> (lc - 1 - d)) & 1) ? x : sub2(mk(1, 0), x));
+        }
+        ofs_u64(&f, v.re);
+        ofs_u64(&f, v.im);
+    }
+    for (long long t = 0; t < col; ++t) {
+        fp2 v = mk(1, 0);
+        for (int d = 0; d < n_r - lc; ++d) {
+            const fp2 x = mk(r[2 * (lc + d)], r[2 * (lc + d) + 1]);
+            v = mul2(v, ((t >> (n_r - lc - 1 - d)) & 1) ? x : sub2(mk(1, 0), x));
+        }
+        ofs_u64(&f, v.re);
+        ofs_u64(&f, v.im);
+    }
+    const int nq = (int)(-128.0 / log2(1.0 - 0.07));
+    const long long g = use_group1 ? (cw + 1) / 2 : cw / 2;
+    for (int i = 0; i < nq; ++i) q[i] = 2 * (long long)(ofs_squeeze(&f) % (u64)g) + (use_group1 ? 0 : 1);
+    return nq;
 }
 
 void or_free(void* p) { free(p); }

@@ -236,7 +236,28 @@ def _oracle_orion_open(self, inputs, seed: int, r, col_q, use_group1: bool = Tru
     return _orion_open(self._oro, self._orf, inputs, seed, use_group1, r, col_q, True)
 
 
+_DRAW_ARGS = [ctypes.c_char_p, ctypes.c_longlong, _INT, ctypes.c_longlong, _U64P, _INT,
+              ctypes.POINTER(ctypes.c_longlong)]
+
+
+def _draw(fn, root: bytes, n: int, use_group1: bool, cw_len: int, r):
+    rr = np.ascontiguousarray(np.asarray(r, dtype=np.uint64).reshape(-1))
+    q = np.zeros(4096, np.int64)
+    k = fn(root, n, 1 if use_group1 else 0, cw_len, _p(rr), rr.size // 2,
+           q.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
+    return [int(x) for x in q[:k]]
+
+
+def _oracle_orion_draw_queries(self, root: bytes, n: int, use_group1: bool, cw_len: int, r):
+    """The code positions the opening's NIZK transcript draws (the opened
+    root as digest bytes, N, the group, the point r)."""
+    if not hasattr(self, "_ord"):
+        self._ord = _bind(self.lib, "or_orion_draw_queries", _DRAW_ARGS, _INT)
+    return _draw(self._ord, root, n, use_group1, cw_len, r)
+
+
 Oracle.orion_open = _oracle_orion_open
+Oracle.orion_draw_queries = _oracle_orion_draw_queries
 Oracle.orion_commit = _oracle_orion_commit
 Oracle.orion_encode = _oracle_orion_encode
 Oracle.sha256 = _oracle_sha256
@@ -267,6 +288,11 @@ class RefOrion:
             self.o = _bind(lib, "ref_orion_open", _OPEN_ARGS, _INT)
             self.f = _bind(lib, "ref_free", [ctypes.c_void_p])
         return _orion_open(self.o, self.f, inputs, seed, use_group1, r, col_q, False)
+
+    def draw_queries(self, root: bytes, n: int, use_group1: bool, cw_len: int, r):
+        if not hasattr(self, "d"):
+            self.d = _bind(ctypes.CDLL(self.PATH), "ref_orion_draw_queries", _DRAW_ARGS, _INT)
+        return _draw(self.d, root, n, use_group1, cw_len, r)
 
     def encode(self, col):
         return _orion_encode(self.e, col)

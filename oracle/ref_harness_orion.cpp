@@ -21,9 +21,11 @@
 
 #include "infrastructure/merkle_tree.h"
 #include "linear_code/linear_code_encode.h"
+#include "truepix/orion_fs.h"
 #include "truepix/orion_proof.h"
 #include "truepix/truepix_gkr.h"
 
+#include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <iterator>
@@ -217,6 +219,43 @@ int ref_orion_open(int copies, const uint8_t* inputs, uint64_t seed, int use_gro
     memcpy(*proof, bytes.data(), bytes.size());
     *proof_len = bytes.size();
     return 0;
+}
+
+// The queries tensor_product_protocol draws in NIZK mode (linearPC.cpp:711-758)
+// from the reference's own transcript (orion_fs.cpp's global_fs), for the
+// opened root (digest bytes), N, the group and the point r; q gets
+// query_count positions.  Returns query_count.
+int ref_orion_draw_queries(const uint8_t root[32], long long n, int use_group1, long long cw_len,
+                           const uint64_t* r, int n_r, long long* q)
+{
+    std::vector<prime_field::field_element> rv(n_r);
+    for (int d = 0; d < n_r; ++d) {
+        rv[d].real = r[2 * d];
+        rv[d].img = r[2 * d + 1];
+    }
+    int lc = 0;
+    while ((1 << lc) < column_size) ++lc;
+    const long long msg_len = n / column_size;
+    std::vector<prime_field::field_element> r0(column_size), r1(msg_len);
+    ref_dfs(r0.data(), rv.data(), column_size, 0, prime_field::field_element(1));
+    ref_dfs(r1.data(), rv.data() + lc, msg_len, 0, prime_field::field_element(1));
+    __hhash_digest com;
+    memcpy(&com, root, 32);
+    truepix_orion::fs_reset_global();
+    auto& fs = truepix_orion::global_fs();
+    fs.absorb_label("tensor-open");
+    fs.absorb_digest(com);
+    fs.absorb_u64(static_cast<uint64_t>(n));
+    fs.absorb_u64(use_group1 ? 1ULL : 0ULL);
+    for (long long i = 0; i < column_size; ++i) fs.absorb_field(r0[i]);
+    for (long long i = 0; i < msg_len; ++i) fs.absorb_field(r1[i]);
+    const int query_count = -128 / (log2(1 - target_distance));
+    std::vector<int> group;
+    for (long long i = 0; i < cw_len; ++i)
+        if ((i % 2 == 0) == (use_group1 != 0)) group.push_back((int)i);
+    for (int i = 0; i < query_count; ++i)
+        q[i] = group[truepix_orion::orion_challenge_index((int)group.size())];
+    return query_count;
 }
 
 void ref_free(void* p) { free(p); }

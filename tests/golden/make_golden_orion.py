@@ -59,6 +59,16 @@ def open_args(copies, point_seed, nq, cw_len):
     return r, q
 
 
+def root_bytes(hex_root: str) -> bytes:
+    """A digest's bytes from its root_hex form (its u64 words at byte offsets
+    8, 0, 24, 16, each printed as 16 hex digits)."""
+    words = [int(hex_root[16 * i:16 * i + 16], 16) for i in range(4)]
+    out = bytearray(32)
+    for w, off in zip(words, (8, 0, 24, 16)):
+        out[off:off + 8] = w.to_bytes(8, "little")
+    return bytes(out)
+
+
 def main():
     from oracle.oracle import RefOrion
     if not RefOrion.available():
@@ -87,6 +97,19 @@ def main():
                             "use_group1": group, "point_seed": pseed, "queries": nq,
                             "proof_len": len(proof),
                             "proof_sha256": hashlib.sha256(proof).hexdigest()})
+    # the NIZK query draw (orion_fs.cpp's transcript) for the openings' roots
+    out["draw"] = []
+    for copies, seed, iseed, group, pseed, nq in OPEN_CASES:
+        r, _ = open_args(copies, pseed, nq, cw_of[copies])
+        m, k, cw = ref.commit(inputs_for(copies, iseed), seed)
+        root = root_bytes(m if group else k)
+        pc = 1
+        while pc < copies:
+            pc <<= 1
+        q = ref.draw_queries(root, pc * 512, bool(group), cw, r)
+        out["draw"].append({"copies": copies, "seed": seed, "input_seed": iseed,
+                            "use_group1": group, "point_seed": pseed, "queries": len(q),
+                            "head": q[:8], "sha256": hashlib.sha256(np.asarray(q, "<i8").tobytes()).hexdigest()})
     with open(os.path.join(HERE, "orion_ref.json"), "w") as fh:
         json.dump(out, fh, indent=1)
     print("wrote", len(out["commit"]), "commit cases,", len(out["encode"]), "encode cases,",

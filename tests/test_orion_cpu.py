@@ -204,3 +204,42 @@ def test_open_matches_live_reference(orc):
         _, _, cw = orc.orion_commit(x, seed)
         r, q = open_args(copies, seed % 997, 33, cw)
         assert orc.orion_open(x, seed, r, q, use_group1=group)[0] == ref.open(x, seed, r, q, group)
+
+
+def root_bytes(hex_root):
+    """A digest's bytes from its root_hex form (u64 words at byte offsets 8, 0,
+    24, 16)."""
+    words = [int(hex_root[16 * i:16 * i + 16], 16) for i in range(4)]
+    out = bytearray(32)
+    for w, off in zip(words, (8, 0, 24, 16)):
+        out[off:off + 8] = w.to_bytes(8, "little")
+    return bytes(out)
+
+
+def test_query_draw_matches_reference(orc, gold):
+    """The opening's NIZK query draw (orion_fs.cpp's transcript) against the
+    positions the reference's own transcript drew."""
+    cw_of = {c["copies"]: c["cw_len"] for c in gold["commit"]}
+    for case in gold["draw"]:
+        copies = case["copies"]
+        r, _ = open_args(copies, case["point_seed"], 2, cw_of[copies])
+        m, k, cw = orc.orion_commit(inputs_for(copies, case["input_seed"]), case["seed"])
+        pc = 1 << max(0, (copies - 1).bit_length())
+        q = orc.orion_draw_queries(root_bytes(m if case["use_group1"] else k), pc * 512,
+                                   bool(case["use_group1"]), cw, r)
+        assert len(q) == case["queries"] == 1222 and q[:8] == case["head"], copies
+        assert hashlib.sha256(np.asarray(q, "<i8").tobytes()).hexdigest() == case["sha256"], copies
+        assert all(x % 2 == (0 if case["use_group1"] else 1) and 0 <= x < cw for x in q)
+
+
+@pytest.mark.skipif(not RefOrion.available(), reason="reference build absent (make -C oracle ref)")
+def test_query_draw_matches_live_reference(orc):
+    ref = RefOrion()
+    for copies, seed, group in ((2, 8, True), (77, 5, False)):
+        x = inputs_for(copies, 3)
+        m, k, cw = orc.orion_commit(x, seed)
+        pc = 1 << max(0, (copies - 1).bit_length())
+        r, _ = open_args(copies, copies, 2, cw)
+        root = root_bytes(m if group else k)
+        assert orc.orion_draw_queries(root, pc * 512, group, cw, r) == \
+            ref.draw_queries(root, pc * 512, group, cw, r)

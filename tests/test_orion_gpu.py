@@ -104,3 +104,28 @@ def test_open_rejects_bad_points_and_queries(bridge, oracle):
         orion.opening(x, 3, r[:10], [0])          # lg n = 11 challenges needed
     with pytest.raises(RuntimeError):
         orion.opening(x, 3, r, [cw])              # past the codeword
+
+
+def _root_bytes(hex_root):
+    words = [int(hex_root[16 * i:16 * i + 16], 16) for i in range(4)]
+    out = bytearray(32)
+    for w, off in zip(words, (8, 0, 24, 16)):
+        out[off:off + 8] = w.to_bytes(8, "little")
+    return bytes(out)
+
+
+@pytest.mark.parametrize("copies,seed,group", [(1, 5, True), (16, 2024, False), (1350, 777, True)])
+def test_open_nizk_draws_the_reference_queries(bridge, oracle, copies, seed, group):
+    """tp_orion_open_nizk draws the positions the reference's NIZK transcript
+    draws (the oracle's draw is pinned to it) and opens them."""
+    x = inputs_for(copies, copies + 5)
+    m, k, cw = oracle.orion_commit(x, seed)
+    pc = 1 << max(0, (copies - 1).bit_length())
+    lg = (pc * 512).bit_length() - 1
+    r = np.random.Generator(np.random.PCG64(copies + 1)).integers(0, (1 << 61) - 1, size=(lg, 2),
+                                                                  dtype=np.uint64)
+    q = oracle.orion_draw_queries(_root_bytes(m if group else k), pc * 512, group, cw, r)
+    got, ans = orion.opening_nizk(x, seed, r, use_group1=group)
+    want, want_ans = oracle.orion_open(x, seed, r, q, use_group1=group)
+    assert orion.parse_opening(got)["col_q"] == q
+    assert (got, ans) == (want, want_ans)

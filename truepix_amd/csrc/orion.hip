@@ -40,10 +40,12 @@
 //     eleven levels.
 #include "fp61.hpp"
 #include "glibc_rand.hpp"
+#include "host_sha.hpp"
 #include "reduce.hpp"
 #include "runtime.hpp"
 #include "../../include/truepix_hip.h"
 
+#include <math.h>
 #include <stdint.h>
 #include <string.h>
 
@@ -601,6 +603,71 @@ bool commit_device(int copies, const uint8_t* inputs, uint64_t seed, hipStream_t
     return true;
 }
 
+// The Orion NIZK transcript (OZ/src/truepix/orion_fs.cpp): a 32-byte state,
+// each absorbed piece taken 32 bytes at a time as state = SHA-256(state |
+// piece bytes, zero-padded to 32); squeezes come from a 32-byte pool,
+// SHA-256(state | 0x01 | zeros), the state moving to SHA-256(state | 0x02 |
+// zeros) at each refill; an absorb empties the pool.
+struct OrionFs {
+    uint8_t state[32], pool[32];
+    int used = 32;
+    static void hash64(const uint8_t b[64], uint8_t out[32])
+    {
+        HostSha256 h;
+        h.init();
+        h.update(b, 64);
+        h.peek(out);
+    }
+    void reset()
+    {
+        memset(state, 0, 32);
+        absorb("truepix/orion-fs/v1", 19);
+        used = 32;
+    }
+    void absorb(const void* data, size_t n)
+    {
+        const uint8_t* p = static_cast<const uint8_t*>(data);
+        while (n > 0) {
+            uint8_t block[64] = {0};
+            memcpy(block, state, 32);
+            const size_t take = n < 32 ? n : 32;
+            memcpy(block + 32, p, take);
+            hash64(block, state);
+            p += take;
+            n -= take;
+        }
+        used = 32;
+    }
+    void absorb_u64(uint64_t x)
+    {
+        uint8_t b[8];
+        for (int k = 0; k < 8; ++k) b[k] = (uint8_t)(x >> (8 * k));
+        absorb(b, 8);
+    }
+    void absorb_field(u64 re, u64 im)   // real then imag, each its own absorb
+    {
+        absorb_u64(re);
+        absorb_u64(im);
+    }
+    uint64_t squeeze_u64()
+    {
+        if (used + 8 > 32) {
+            uint8_t block[64] = {0};
+            memcpy(block, state, 32);
+            block[32] = 0x01;
+            hash64(block, pool);
+            block[32] = 0x02;
+            hash64(block, state);
+            used = 0;
+        }
+        uint64_t v = 0;
+        for (int k = 7; k >= 0; --k) v = (v << 8) | pool[used + k];
+        used += 8;
+        return v;
+    }
+    int index(int bound) { return bound <= 0 ? 0 : (int)(squeeze_u64() % (uint64_t)bound); }
+};
+
 // Little-endian appends in the opening proof's layout.
 template <class T>
 void put_le(std::vector<uint8_t>& o, T v)
@@ -740,14 +807,14 @@ int tp_orion_commit(int copies, const uint8_t* inputs, uint64_t seed, char maske
 // OZ/include/truepix/orion_proof.h:1-22) to *proof (free with tp_free) and
 // the claimed value to answer.  The Virgo code-switching proof that follows
 // in the reference is not part of it.  0 / -1.
-int tp_orion_open(int copies, const uint8_t* inputs, uint64_t seed, int use_group1,
-                  const uint64_t* r, int n_r, const int64_t* col_q, int n_queries,
-                  uint8_t** proof, size_t* proof_len, uint64_t answer[2])
+static int open_impl(int copies, const uint8_t* inputs, uint64_t seed, int use_group1,
+                     const uint64_t* r, int n_r, const int64_t* col_q_in, int n_queries, bool nizk,
+                     uint8_t** proof, size_t* proof_len, uint64_t answer[2])
 {
     clear_error();
     if (!device_ready()) return -1;
     if (copies < 1 || copies > (1 << 16) || !inputs || !r || !proof || !proof_len || !answer ||
-        n_queries < 0 || (n_queries && !col_q))
+        (!nizk && (n_queries < 0 || (n_queries && !col_q_in))))
         return fail("tp_orion_open: bad arguments (copies=%d)", copies), -1;
     long long pc = 1;
     while (pc < copies) pc <<= 1;
@@ -762,10 +829,8 @@ int tp_orion_open(int copies, const uint8_t* inputs, uint64_t seed, int use_grou
     long long rows = 0;
     if (!commit_device(copies, inputs, seed, st, P, rows)) return -1;
     const long long cw = P->cw;
-    for (int i = 0; i < n_queries; ++i)
-        if (col_q[i] < 0 || col_q[i] >= cw)
-            return fail("tp_orion_open: query %d at %lld outside the codeword (%lld)", i,
-                        (long long)col_q[i], cw), -1;
+    std::vector<int64_t> drawn;
+    const int64_t* col_q = col_q_in;
     // r0 over r[0..7), r1 over the rest: eq tensors, the first challenge on
     // the highest bit of the index (dfs in linearPC.cpp:996-1007)
     auto tensor = [&](const uint64_t* rr, int k, std::vector<fp2>& out) {
@@ -782,6 +847,36 @@ int tp_orion_open(int copies, const uint8_t* inputs, uint64_t seed, int use_grou
     while ((1 << lc) < kCols) ++lc;
     tensor(r, lc, r0);
     tensor(r + 2 * lc, lg - lc, r1);
+    if (nizk) {
+        // the opening's public inputs, then the queries from the opened
+        // group (even code positions for the masked polynomial, odd for the
+        // mask), as tensor_product_protocol draws them (linearPC.cpp:711-758)
+        uint32_t root[8];
+        TP_HIP_OK(hipMemcpyAsync(root, g_tree.as<uint32_t>() + ((size_t)(use_group1 ? 0 : 1) * 2 * rows + 1) * 8,
+                                 32, hipMemcpyDeviceToHost, st));
+        TP_HIP_OK(hipStreamSynchronize(st));
+        uint8_t rb[32];
+        for (int k = 0; k < 8; ++k)
+            for (int b = 0; b < 4; ++b) rb[4 * k + b] = (uint8_t)(root[k] >> (24 - 8 * b));
+        OrionFs fs;
+        fs.reset();
+        fs.absorb("tensor-open", 11);
+        fs.absorb(rb, 32);
+        fs.absorb_u64((uint64_t)n);
+        fs.absorb_u64(use_group1 ? 1 : 0);
+        for (const fp2& x : r0) fs.absorb_field(x.re, x.im);
+        for (const fp2& x : r1) fs.absorb_field(x.re, x.im);
+        n_queries = (int)(-128.0 / log2(1.0 - 0.07));   // the reference's query_count
+        const long long gsize = use_group1 ? (cw + 1) / 2 : cw / 2;
+        drawn.resize((size_t)n_queries);
+        for (int i = 0; i < n_queries; ++i)
+            drawn[(size_t)i] = 2LL * fs.index((int)gsize) + (use_group1 ? 0 : 1);
+        col_q = drawn.data();
+    }
+    for (int i = 0; i < n_queries; ++i)
+        if (col_q[i] < 0 || col_q[i] >= cw)
+            return fail("tp_orion_open: query %d at %lld outside the codeword (%lld)", i,
+                        (long long)col_q[i], cw), -1;
     const int off = use_group1 ? 0 : kCols;
     const size_t nq = (size_t)n_queries;
     if (!g_open.ensure(((size_t)kCols + (size_t)msg + (size_t)cw + 1 + nq * kCols) * sizeof(V2) +
@@ -850,6 +945,26 @@ int tp_orion_open(int copies, const uint8_t* inputs, uint64_t seed, int use_grou
     *proof = buf;
     *proof_len = out.size();
     return 0;
+}
+
+int tp_orion_open(int copies, const uint8_t* inputs, uint64_t seed, int use_group1,
+                  const uint64_t* r, int n_r, const int64_t* col_q, int n_queries,
+                  uint8_t** proof, size_t* proof_len, uint64_t answer[2])
+{
+    return open_impl(copies, inputs, seed, use_group1, r, n_r, col_q, n_queries, false, proof,
+                     proof_len, answer);
+}
+
+// tp_orion_open with the queries drawn as the reference's NIZK mode draws
+// them (OZ/src/truepix/orion_fs.cpp: the transcript reset, the label
+// "tensor-open", the opened root, N, the group flag, r0 and r1, then 1222
+// positions from the opened group; linearPC.cpp:711-758).  0 / -1.
+int tp_orion_open_nizk(int copies, const uint8_t* inputs, uint64_t seed, int use_group1,
+                       const uint64_t* r, int n_r, uint8_t** proof, size_t* proof_len,
+                       uint64_t answer[2])
+{
+    return open_impl(copies, inputs, seed, use_group1, r, n_r, nullptr, 0, true, proof, proof_len,
+                     answer);
 }
 
 // The expander code alone: n_cols <= 256 columns of n Fp2 each, src

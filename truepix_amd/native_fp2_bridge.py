@@ -123,6 +123,9 @@ def bind(lib) -> None:
     _sig(lib, "tp_orion_open", [_I32, ctypes.c_char_p, ctypes.c_uint64, _I32, _U64P, _I32, _VP, _I32,
                                 ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t),
                                 _U64P], _I32)
+    _sig(lib, "tp_orion_open_nizk", [_I32, ctypes.c_char_p, ctypes.c_uint64, _I32, _U64P, _I32,
+                                     ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t),
+                                     _U64P], _I32)
     _LL = ctypes.c_longlong
     _sig(lib, "tp_yuv_pack", [_VP, _VP, _VP, _I32, _I32, _LL, _LL, _VP, _I32, _VP, _VP], _LL)
     _sig(lib, "tp_yuv_unpack", [_VP, _LL, _I32, _I32, _LL, _LL, _VP, _VP, _VP, _VP], _I32)

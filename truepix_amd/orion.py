@@ -78,6 +78,30 @@ def opening(inputs, seed: int, r, col_q, use_group1: bool = True):
     return proof, (int(ans[0]), int(ans[1]))
 
 
+def opening_nizk(inputs, seed: int, r, use_group1: bool = True):
+    """``opening`` with the code positions drawn as the reference's NIZK mode
+    draws them (OZ/src/truepix/orion_fs.cpp's transcript over the label
+    "tensor-open", the opened root, N, the group flag, r0 and r1; 1222
+    positions from the opened group, linearPC.cpp:711-758)."""
+    nb.require()
+    arr = np.ascontiguousarray(inputs)
+    if arr.ndim != 2 or arr.shape[1] != COPY_SIZE:
+        raise ValueError("inputs must be [copies][%d]" % COPY_SIZE)
+    data = np.ascontiguousarray(arr, dtype=np.uint8)
+    rr = np.ascontiguousarray(np.asarray(r, dtype=np.uint64).reshape(-1))
+    buf = ctypes.c_void_p()
+    n = ctypes.c_size_t()
+    ans = np.zeros(2, np.uint64)
+    rc = nb.LIB.tp_orion_open_nizk(int(arr.shape[0]), data.tobytes(), int(seed) & (2 ** 64 - 1),
+                                   1 if use_group1 else 0, rr.ctypes.data_as(_U64P), rr.size // 2,
+                                   ctypes.byref(buf), ctypes.byref(n), ans.ctypes.data_as(_U64P))
+    if rc != 0:
+        raise RuntimeError("tp_orion_open_nizk failed: %s" % nb.last_error())
+    proof = ctypes.string_at(buf.value, n.value)
+    nb.LIB.tp_free(buf)
+    return proof, (int(ans[0]), int(ans[1]))
+
+
 def parse_opening(proof: bytes) -> dict:
     """The fields of an OpeningProof version-2 byte string
     (OZ/include/truepix/orion_proof.h:1-22); field elements as (re, im),
