@@ -548,6 +548,35 @@ def cpu_baseline(nv_sample: int = 24, nthreads: int = 1, min_seconds: float = 10
                       "(oracle/fp2_oracle.c, -O3)" % (reps, nv_sample, nv_sample, dt)}
 
 
+def hbm_ceiling(reps: int = 5):
+    """HBM rates plain kernels reach on this GPU, beside the published peak:
+    a 256 MiB -> 256 MiB copy (read + write) and a sum over 512 MiB of int64
+    (read), median of `reps` launches timed with events."""
+    import torch
+    n = 1 << 26                                    # 2^26 int64 = 512 MiB
+    x = torch.ones(n, dtype=torch.int64, device="cuda")
+    y = torch.empty(n // 2, dtype=torch.int64, device="cuda")
+
+    def timed(fn):
+        ts = []
+        for _ in range(reps + 1):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        return sorted(ts[1:])[reps // 2]
+    t_copy = timed(lambda: y.copy_(x[: n // 2]))
+    t_read = timed(lambda: x.sum())
+    out = {"copy_gbs": round(2 * (n // 2) * 8 / t_copy / 1e9, 1),
+           "read_gbs": round(n * 8 / t_read / 1e9, 1),
+           "how": "torch copy_ of 256 MiB (read + write) and sum of 512 MiB int64, median of %d" % reps}
+    del x, y
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -656,13 +685,15 @@ def main(argv=None):
                      "launches_per_prove": classes[dom]["launches_per_prove"],
                      "alg_bytes_per_launch": classes[dom]["bytes_per_launch"],
                      "classes": classes,
+                     "ceiling": hbm_ceiling(),
                      "note": "first: rounds 0 and 1 from one read of the input; fold01: round 2 "
                              "folds the input by r_0 and r_1 into T_2; fold: streaming "
-                             "fold+round launches; mid: 128 workgroups, each folding its own "
-                             "slice of the tables in LDS, the host adding their partials and "
-                             "answering each challenge; tail: one workgroup proving every round "
-                             "whose input has <= 8192 entries per table, the host answering "
-                             "each challenge through mapped memory (latency-bound)"},
+                             "fold+round launches; mid: 64 workgroups, each folding its own "
+                             "slice of the tables in LDS, workgroup 0 gathering their sums for "
+                             "the host, which answers each challenge through mapped memory; "
+                             "tail: one workgroup proving the last rounds, the host answering "
+                             "each challenge through mapped memory (latency-bound); ceiling: "
+                             "what plain torch kernels reach on this GPU now"},
         "check": {"round0_poly": [int(x) for x in proof.polys[0][:2]]},
     }
     def batched():

@@ -1,0 +1,98 @@
+// CPU check of the device field arithmetic in truepix_amd/csrc/fp61.hpp
+// (built with plain g++, so the integer code the GPU runs is exercised as
+// written): mul31 against 128-bit products, including the worst-case
+// operands that set its no-overflow bounds, and LazyProdSum against an
+// exact Fp2 dot product.  Prints each failure; exit status 1 on any.
+#include "fp61.hpp"
+#include <cstdio>
+#include <vector>
+
+using namespace tp;
+typedef unsigned __int128 u128;
+
+static int fails = 0;
+#define CHECK(c, ...) do { if (!(c)) { if (++fails <= 20) { std::printf(__VA_ARGS__); std::printf("\n"); } } } while (0)
+
+static u64 modp(u128 x) { return (u64)(x % P); }
+
+static u64 rng_state = 0x9e3779b97f4a7c15ull;
+static u64 rnd() {
+    rng_state ^= rng_state << 13; rng_state ^= rng_state >> 7; rng_state ^= rng_state << 17;
+    return rng_state;
+}
+
+// The terms of mul31 summed exactly, to see that the u64 sum does not wrap.
+static u128 mul31_exact(u64 a, u64 b, u64 c) {
+    const u128 al = a & 0x7fffffffu, ah = a >> 31, bl = b & 0x7fffffffu, bh = b >> 31;
+    const u128 m = ah * bl + al * bh;
+    return 2 * ah * bh + (m >> 30) + (m & 0x3fffffffu) * ((u128)1 << 31) + al * bl + c;
+}
+
+static void check_mul31(u64 a, u64 b, u64 c) {
+    const u64 s = mul31(sp31(a), sp31(b), c);
+    const u128 t = mul31_exact(a, b, c);
+    CHECK(t < ((u128)1 << 64), "mul31 wraps: a=%llx b=%llx c=%llx", (unsigned long long)a,
+          (unsigned long long)b, (unsigned long long)c);
+    CHECK((u128)s == t, "mul31 differs from its terms: a=%llx b=%llx", (unsigned long long)a,
+          (unsigned long long)b);
+    CHECK(modp(s) == modp((u128)a * b + c), "mul31 not a b + c mod p: a=%llx b=%llx c=%llx",
+          (unsigned long long)a, (unsigned long long)b, (unsigned long long)c);
+    if (a < (1ull << 61) && b < (1ull << 61))
+        CHECK(t - c < (u128)4 * P, "canonical product not below 4p: a=%llx b=%llx",
+              (unsigned long long)a, (unsigned long long)b);
+}
+
+int main() {
+    const u64 two61 = 1ull << 61, two62 = 1ull << 62;
+    std::vector<u64> ops = {0, 1, 2, 0x7fffffffull, 0x80000000ull, 0xffffffffull, 1ull << 32,
+                            (1ull << 60) - 1, P - 1, P, two61, two61 + 1, 2 * P - 1, 2 * P,
+                            two62 - 2, two62 - 1, (two62 - 1) & ~0x7fffffffull,
+                            (two62 - 1) ^ (1ull << 30), 0x3fffffff80000000ull, 0x3fffffff7fffffffull};
+    for (int i = 0; i < 40; ++i) ops.push_back(rnd() >> 2);          // < 2^62
+    for (int i = 0; i < 40; ++i) ops.push_back(rnd() % P);           // canonical
+    const u64 cs[] = {0, 1, P - 1, two61 + 6, two61 + (1ull << 31)};
+    for (u64 a : ops)
+        for (u64 b : ops)
+            for (u64 c : cs) check_mul31(a, b, c);
+    for (int i = 0; i < 200000; ++i) check_mul31(rnd() >> 2, rnd() >> 2, rnd() % (two61 + (1ull << 31) + 1));
+
+    // fold1: any u64 to at most 2^61 + 6, congruent
+    const u64 xs[] = {0, P, two61, ~0ull, ~0ull - 1, 7ull << 61, (7ull << 61) | P};
+    for (u64 x : xs) {
+        CHECK(fold1(x) <= two61 + 6, "fold1 bound: %llx", (unsigned long long)x);
+        CHECK(modp(fold1(x)) == modp(x), "fold1 value: %llx", (unsigned long long)x);
+    }
+
+    // LazyProdSum over canonical Fp2 vectors, extremes included, against the
+    // exact dot product (i^2 = -1)
+    for (int trial = 0; trial < 400; ++trial) {
+        const int n = 1 + (int)(rnd() % 300);
+        LazyProdSum z;
+        z.zero();
+        u64 re = 0, im = 0;
+        for (int j = 0; j < n; ++j) {
+            fp2 x, y;
+            const int kind = (int)(rnd() % 4);
+            x.re = kind == 0 ? P - 1 : rnd() % P;
+            x.im = kind == 1 ? P - 1 : rnd() % P;
+            y.re = kind == 2 ? P - 1 : rnd() % P;
+            y.im = kind == 0 ? P - 1 : rnd() % P;
+            z.mac(x, y);
+            re = modp((u128)re + (u128)x.re * y.re + (u128)(P - x.im) * y.im);
+            im = modp((u128)im + (u128)x.re * y.im + (u128)x.im * y.re);
+        }
+        const fp2 v = z.value();
+        CHECK(v.re == re && v.im == im, "LazyProdSum trial %d (n=%d) differs", trial, n);
+    }
+    {   // every operand p - 1: the largest Karatsuba sums, many terms
+        LazyProdSum z;
+        z.zero();
+        const fp2 x = mk2(P - 1, P - 1);
+        for (int j = 0; j < 1000; ++j) z.mac(x, x);
+        // (-1 - i)^2 = 2i, times 1000
+        const fp2 v = z.value();
+        CHECK(v.re == 0 && v.im == 2000, "LazyProdSum at p - 1 differs");
+    }
+    std::printf("field check: %d failures\n", fails);
+    return fails ? 1 : 0;
+}

@@ -8,10 +8,14 @@
 // low limb and a high limb of at most 30 bits, and the three partial products
 // are folded with 2^61 = 1 and 2^64 = 8 (mod p), so no 128-bit value is formed.
 #pragma once
-#include <hip/hip_runtime.h>
 #include <stdint.h>
-
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
 #define TP_HD __host__ __device__ __forceinline__
+#else
+// plain C++ (the CPU check of the arithmetic below builds this header alone)
+#define TP_HD inline
+#endif
 
 namespace tp {
 
@@ -91,18 +95,50 @@ TP_HD fp2 fold2(fp2 a, fp2 b, fp2 r)
     return mk2(fold61(a.re + ac + (2 * P - bd)), fold61(a.im + k + (4 * P - ac - bd)));
 }
 
+// Operand of mul31: a < 2^62 as a = h 2^31 + l with l, h < 2^31.
+struct Sp31 { u32 l, h; };
+TP_HD Sp31 sp31(u64 a)
+{
+    Sp31 s;
+    s.l = (u32)a & 0x7fffffffu;
+    s.h = (u32)(a >> 31);
+    return s;
+}
+
+// a b + c (mod p) as an unreduced u64, for a, b < 2^62 and c <= 2^61 + 2^31.
+// With 2^62 = 2 and 2^61 = 1 (mod p):
+//   a b = 2 h_a h_b + m 2^31 + l_a l_b,  m = h_a l_b + l_a h_b < 2^63,
+//   m 2^31 = (m >> 30) + (m mod 2^30) 2^31  (mod p),
+// 32 x 32 -> 64 multiply-adds and no fold.  Term by term the sum before
+// c is below 7 2^61 - 2^32 (l l < 2^62, 2 h h < 2^63, m >> 30 < 2^33,
+// (m mod 2^30) 2^31 < 2^61), so with c it stays below 2^64; for canonical
+// a, b (< 2^61, so h < 2^30) it is below 4p.
+TP_HD u64 mul31(Sp31 a, Sp31 b, u64 c)
+{
+    const u64 m = (u64)a.h * b.l + (u64)a.l * b.h;
+    // 2 h_a < 2^32 (h_a < 2^31), so the doubling stays in 32 bits, and the
+    // shift by 31 is a multiply-add by 2^31: six multiply-adds in all
+    const u64 x = (u64)(2u * a.h) * b.h + (m >> 30);
+    const u64 y = (u64)((u32)m & 0x3fffffffu) * 0x80000000u + x;
+    return (u64)a.l * b.l + y + c;
+}
+
+// One fold of any u64: at most 2^61 + 6.
+TP_HD u64 fold1(u64 x) { return (x & P) + (x >> 61); }
+
 // Sum of Fp2 products kept lazily: the Karatsuba products ac, bd and
-// k = (a + b)(c + d) summed separately, each sum folded once per add so it
-// stays below 2^61 + 8; value() reduces once at the end.
+// k = (a + b)(c + d) summed separately, each product added unreduced (mul31)
+// and the sum folded once per add, so it stays at most 2^61 + 6; value()
+// reduces once at the end.  Operands are canonical (< 2^61), so the
+// Karatsuba sums are below 2^62.
 struct LazyProdSum {
     u64 ac, bd, k;
     TP_HD void zero() { ac = bd = k = 0; }
-    TP_HD static u64 acc(u64 s, u64 t) { s += t; return (s & P) + (s >> 61); }
     TP_HD void mac(fp2 x, fp2 y)
     {
-        ac = acc(ac, fmul_lazy(x.re, y.re));
-        bd = acc(bd, fmul_lazy(x.im, y.im));
-        k = acc(k, fmul_lazy(x.re + x.im, y.re + y.im));
+        ac = fold1(mul31(sp31(x.re), sp31(y.re), ac));
+        bd = fold1(mul31(sp31(x.im), sp31(y.im), bd));
+        k = fold1(mul31(sp31(x.re + x.im), sp31(y.re + y.im), k));
     }
     TP_HD fp2 value() const
     {

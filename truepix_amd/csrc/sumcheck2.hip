@@ -121,10 +121,12 @@ __global__ void __launch_bounds__(256) k_sc2_first(const V2* __restrict__ f,
 // Rounds 0 and 1 in one pass (see the header): per quad the 9 sums S00, S11,
 // SAA, S22, S33, SBB, X0, X1, X2 (9 products per quad against 6 for round 0
 // alone), each kept lazily.
-__device__ __forceinline__ void accum_quad01(LazyProdSum (&z)[9], const V2* fp, const V2* gp)
+__device__ __forceinline__ void accum_quad01(LazyProdSum (&z)[9], const V2 (&f)[4], const V2 (&g)[4])
 {
-    const fp2 f0 = ldv(fp), f1 = ldv(fp + 1), f2 = ldv(fp + 2), f3 = ldv(fp + 3);
-    const fp2 g0 = ldv(gp), g1 = ldv(gp + 1), g2 = ldv(gp + 2), g3 = ldv(gp + 3);
+    const fp2 f0 = mk2(f[0].re, f[0].im), f1 = mk2(f[1].re, f[1].im);
+    const fp2 f2 = mk2(f[2].re, f[2].im), f3 = mk2(f[3].re, f[3].im);
+    const fp2 g0 = mk2(g[0].re, g[0].im), g1 = mk2(g[1].re, g[1].im);
+    const fp2 g2 = mk2(g[2].re, g[2].im), g3 = mk2(g[3].re, g[3].im);
     const fp2 d = sub2(f1, f0), dd = sub2(f3, f2), c = sub2(g1, g0), cc = sub2(g3, g2);
     z[0].mac(f0, g0);
     z[1].mac(f1, g1);
@@ -137,9 +139,12 @@ __device__ __forceinline__ void accum_quad01(LazyProdSum (&z)[9], const V2* fp, 
     z[8].mac(sub2(dd, d), sub2(cc, c));
 }
 
-// 128 VGPRs hold four waves per SIMD, so one 16-wave workgroup per CU (256
+// 116 VGPRs hold four waves per SIMD, so one 16-wave workgroup per CU (256
 // workgroups of 1024 threads) is the same occupancy as more, smaller ones,
-// with an eighth of the partial sums for the publish launch to add.
+// with an eighth of the partial sums for the publish launch to add.  A
+// prefetch of the next quad (768 threads, 165 VGPRs) and other schedules of
+// the nine products measured the same: at four waves per SIMD the pass
+// overlaps its loads with its products only in part.
 constexpr int kFirst01Threads = 1024;
 constexpr unsigned kFirst01Groups = 256;
 
@@ -150,8 +155,15 @@ __global__ void __launch_bounds__(kFirst01Threads) k_sc2_first01(const V2* __res
     LazyProdSum z[9];
     lazy_zero(z);
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n_quads; q += stride)
-        accum_quad01(z, f + 4 * q, g + 4 * q);
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n_quads; q += stride) {
+        V2 cf[4], cg[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            cf[k] = f[4 * q + k];
+            cg[k] = g[4 * q + k];
+        }
+        accum_quad01(z, cf, cg);
+    }
     lazy_partial<9>(z, partials);
 }
 
@@ -186,32 +198,41 @@ __global__ void __launch_bounds__(256) k_sc2_fold01_round(const V2* __restrict__
     LazyProdSum z[3];
     lazy_zero(z);
     // n_out is even and the stride a multiple of 64, so both lanes of a pair
-    // run every iteration together
-    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < n_out;
-         j += (size_t)gridDim.x * blockDim.x) {
+    // run every iteration together.  The sweep runs from the top down: the
+    // first pass read the tables bottom up, so their top part is still in
+    // the memory-side cache.  Lane order is reversed with it, so the even
+    // lane holds output 2p + 1 and its neighbour 2p.
+    for (size_t j0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j0 < n_out;
+         j0 += (size_t)gridDim.x * blockDim.x) {
+        const size_t j = n_out - 1 - j0;
         const fp2 f = fold_quad01(fin + 4 * j, r0, r1);
         const fp2 g = fold_quad01(gin + 4 * j, r0, r1);
         stv(fout + j, f);
         stv(gout + j, g);
         const fp2 fo = shfl_xor1(f), go = shfl_xor1(g);
-        if (even) accum_pair(z, f, fo, g, go);
+        if (even) accum_pair(z, fo, f, go, g);
     }
     lazy_partial<3>(z, partials);
 }
 
 // Round k >= 1: fold T_{k-1} (length 4 * n_quads) with r into T_k and sum
-// the round-k pairs.  Quad q = inputs 4q..4q+3 -> outputs 2q, 2q+1.
+// the round-k pairs.  Quad q = inputs 4q..4q+3 -> outputs 2q, 2q+1.  With
+// `desc` the sweep runs from the top quad down, so a pass starts where the
+// previous pass (which ran the other way) ended: there its input is still
+// in the memory-side cache.
 __global__ void __launch_bounds__(256) k_sc2_fold_round(const V2* __restrict__ fin,
                                                         const V2* __restrict__ gin,
                                                         size_t n_quads, u64 rr, u64 ri,
                                                         V2* __restrict__ fout,
-                                                        V2* __restrict__ gout, fp2* partials)
+                                                        V2* __restrict__ gout, fp2* partials,
+                                                        int desc)
 {
     const fp2 r = mk2(rr, ri);
     LazyProdSum z[3];
     lazy_zero(z);
-    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n_quads;
-         q += (size_t)gridDim.x * blockDim.x) {
+    for (size_t q0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q0 < n_quads;
+         q0 += (size_t)gridDim.x * blockDim.x) {
+        const size_t q = desc ? n_quads - 1 - q0 : q0;
         const V2* fp = fin + 4 * q;
         const V2* gp = gin + 4 * q;
         const fp2 fa = ldv(fp), fb = ldv(fp + 1), fc = ldv(fp + 2), fd = ldv(fp + 3);
@@ -1166,7 +1187,9 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         V2* og = of + ((k & 1) ? capA : capB);
         const unsigned nb = blocks_for(quads);
         if (!t_begin()) return -1;
-        k_sc2_fold_round<<<nb, kBlock, 0, st>>>(cf, cg, quads, r.re, r.im, of, og, partials);
+        // round 2 ran top down, so round 3 runs bottom up, round 4 top down, ...
+        k_sc2_fold_round<<<nb, kBlock, 0, st>>>(cf, cg, quads, r.re, r.im, of, og, partials,
+                                                (k & 1) == 0);
         if (!t_end(SC_FOLD, 48.0 * (double)(2 * m))) return -1;
         TP_HIP_OK(hipGetLastError());
         k_publish_wide<3><<<1, 1024, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
