@@ -65,6 +65,114 @@ struct alignas(16) V2 { u64 re, im; };
 __device__ __forceinline__ fp2 ldv(const V2* p) { const V2 v = *p; return mk2(v.re, v.im); }
 __device__ __forceinline__ void stv(V2* p, fp2 x) { V2 v; v.re = x.re; v.im = x.im; *p = v; }
 
+constexpr unsigned kSeqAbort = 0xffffffffu;   // a kernel's bounded wait expired
+
+// Arguments of a launch that publishes its own round sums (ticket_publish).
+struct TicketPub {
+    u64* slots;          // [gridDim.x][N][2] tagged words; all ones between launches
+    unsigned* ticket;    // 0 between launches
+    fp2* host_vals;      // mapped: the N totals
+    unsigned* host_seq;  // mapped: then seq, or kSeqAbort if a wait expired
+    unsigned seq;
+    long long spin_ticks;
+};
+
+constexpr u64 kSlotEmpty = ~0ull;   // value bits p under tag 7: no round's word
+
+// End of a streaming launch, in place of a separate publish launch (which
+// cost its own launch and a dependent-launch gap per round).  Each
+// workgroup's N sums go to its slot as tagged words, value | (seq mod 8) << 61
+// (values are canonical, below 2^61), and thread 0 takes a ticket.  The
+// workgroup taking the last ticket adds every slot once each word carries
+// this launch's tag (spinning, bounded by the wall clock, on any still in
+// flight), clears the slots to all ones and the ticket for the next launch
+// (so no word of an earlier round, whatever its tag, is ever read as this
+// one's), and publishes the totals like k_publish_*: values, then seq.
+template <int N>
+__device__ __forceinline__ void ticket_publish(fp2 (&v)[N], const TicketPub& a)
+{
+    __shared__ int s_last;
+    __shared__ fp2 s_g[N][1024 / N];
+    __shared__ fp2 s_tot[N];
+    block_sum<N>(v);
+    const unsigned nb = gridDim.x, t = threadIdx.x;
+    const u64 tag = (u64)(a.seq & 7) << 61;
+    if (t == 0) {
+        u64* p = a.slots + (size_t)blockIdx.x * 2 * N;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            __hip_atomic_store(p + 2 * k, v[k].re | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(p + 2 * k + 1, v[k].im | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_last = __hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                 nb - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    // the last workgroup: thread t adds sum k = t mod N over slots g, g + G, ...
+    const unsigned G = blockDim.x / N, k = t % N, g = t / N;
+    const long long t0 = wall_clock64();
+    int ok = 1;
+    fp2 acc = zero2();
+    auto word = [&](unsigned j, int c) -> u64 {   // slot j, coordinate c of sum k
+        return j < nb ? __hip_atomic_load(a.slots + ((size_t)j * N + k) * 2 + c, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT)
+                      : tag;                       // past the last slot: 0 under this tag
+    };
+    if (g < G) {
+        // eight slots' words in flight at once: one round of loads covers
+        // every launch here (nb <= 8 G for all of them)
+        for (unsigned j0 = g; j0 < nb; j0 += 8 * G) {
+            u64 w[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) w[i] = word(j0 + (i >> 1) * G, i & 1);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                while (ok && ((w[i] & ~P) != tag || (w[i] & P) == P)) {
+                    if (wall_clock64() - t0 > a.spin_ticks) {
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    w[i] = word(j0 + (i >> 1) * G, i & 1);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) acc = add2(acc, mk2(w[2 * i] & P, w[2 * i + 1] & P));
+        }
+    }
+    ok = __syncthreads_and(ok);   // every word is read before the slots are cleared
+    for (unsigned w = t; w < nb * 2 * N; w += blockDim.x)
+        __hip_atomic_store(a.slots + w, kSlotEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (g < G) s_g[k][g] = acc;
+    __syncthreads();
+    const unsigned wid = t >> 6, lane = t & 63;
+    if (wid < (unsigned)N) {   // wave k adds the G partials of sum k
+        fp2 sv[1] = {zero2()};
+        for (unsigned x = lane; x < G; x += 64) sv[0] = add2(sv[0], s_g[wid][x]);
+        wave_sum<1>(sv);
+        if (lane == 0) s_tot[wid] = sv[0];
+    }
+    __syncthreads();
+    if (t == 0) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) a.host_vals[q] = s_tot[q];
+        __threadfence_system();
+        __hip_atomic_store(a.host_seq, ok ? a.seq : kSeqAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// The totals of N lazy sums published by ticket_publish.
+template <int N>
+__device__ __forceinline__ void lazy_publish(const LazyProdSum (&z)[N], const TicketPub& a)
+{
+    fp2 e[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) e[k] = z[k].value();
+    ticket_publish<N>(e, a);
+}
+
 // Round sums of one pair of T_k, lazily: E0 += f0 g0, E1 += f1 g1,
 // EA += (f1 - f0)(g1 - g0).
 __device__ __forceinline__ void accum_pair(LazyProdSum (&z)[3], fp2 f0, fp2 f1, fp2 g0, fp2 g1)
@@ -150,7 +258,7 @@ constexpr unsigned kFirst01Groups = 256;
 
 __global__ void __launch_bounds__(kFirst01Threads) k_sc2_first01(const V2* __restrict__ f,
                                                      const V2* __restrict__ g, size_t n_quads,
-                                                     fp2* partials)
+                                                     TicketPub pub)
 {
     LazyProdSum z[9];
     lazy_zero(z);
@@ -164,7 +272,7 @@ __global__ void __launch_bounds__(kFirst01Threads) k_sc2_first01(const V2* __res
         }
         accum_quad01(z, cf, cg);
     }
-    lazy_partial<9>(z, partials);
+    lazy_publish<9>(z, pub);
 }
 
 // Round 2 from the input: fold T_0 by r_0 and r_1 into T_2 and sum the
@@ -186,12 +294,19 @@ __device__ __forceinline__ fp2 shfl_xor1(fp2 x)
     return mk2(__shfl_xor(x.re, 1, 64), __shfl_xor(x.im, 1, 64));
 }
 
-__global__ void __launch_bounds__(256) k_sc2_fold01_round(const V2* __restrict__ fin,
+// The streaming launches run 512-thread workgroups, at most kFoldGroups of
+// them (as many as are resident at once), so the last workgroup gathers at
+// most that many slots.
+constexpr int kFoldThreads = 512;
+constexpr unsigned kFold01Groups = 1024;   // 62 VGPRs: 4 workgroups per CU
+constexpr unsigned kFoldGroups = 768;      // 74 VGPRs: 3 workgroups per CU
+
+__global__ void __launch_bounds__(kFoldThreads) k_sc2_fold01_round(const V2* __restrict__ fin,
                                                           const V2* __restrict__ gin,
                                                           size_t n_out, u64 r0r, u64 r0i,
                                                           u64 r1r, u64 r1i,
                                                           V2* __restrict__ fout,
-                                                          V2* __restrict__ gout, fp2* partials)
+                                                          V2* __restrict__ gout, TicketPub pub)
 {
     const fp2 r0 = mk2(r0r, r0i), r1 = mk2(r1r, r1i);
     const bool even = (threadIdx.x & 1) == 0;
@@ -212,7 +327,7 @@ __global__ void __launch_bounds__(256) k_sc2_fold01_round(const V2* __restrict__
         const fp2 fo = shfl_xor1(f), go = shfl_xor1(g);
         if (even) accum_pair(z, fo, f, go, g);
     }
-    lazy_partial<3>(z, partials);
+    lazy_publish<3>(z, pub);
 }
 
 // Round k >= 1: fold T_{k-1} (length 4 * n_quads) with r into T_k and sum
@@ -220,11 +335,11 @@ __global__ void __launch_bounds__(256) k_sc2_fold01_round(const V2* __restrict__
 // `desc` the sweep runs from the top quad down, so a pass starts where the
 // previous pass (which ran the other way) ended: there its input is still
 // in the memory-side cache.
-__global__ void __launch_bounds__(256) k_sc2_fold_round(const V2* __restrict__ fin,
+__global__ void __launch_bounds__(kFoldThreads) k_sc2_fold_round(const V2* __restrict__ fin,
                                                         const V2* __restrict__ gin,
                                                         size_t n_quads, u64 rr, u64 ri,
                                                         V2* __restrict__ fout,
-                                                        V2* __restrict__ gout, fp2* partials,
+                                                        V2* __restrict__ gout, TicketPub pub,
                                                         int desc)
 {
     const fp2 r = mk2(rr, ri);
@@ -245,7 +360,7 @@ __global__ void __launch_bounds__(256) k_sc2_fold_round(const V2* __restrict__ f
         stv(gout + 2 * q + 1, g1);
         accum_pair(z, f0, f1, g0, g1);
     }
-    lazy_partial<3>(z, partials);
+    lazy_publish<3>(z, pub);
 }
 
 // The last rounds in one workgroup, the host answering each round's
@@ -261,7 +376,6 @@ __global__ void __launch_bounds__(256) k_sc2_fold_round(const V2* __restrict__ f
 constexpr int kTailThreads = 1024;
 constexpr unsigned kTailIn = 8192;
 constexpr size_t kTailLds = (size_t)kTailIn * sizeof(V2);   // both tables, T_{k0}
-constexpr unsigned kSeqAbort = 0xffffffffu;
 
 // Lane 0 publishes a round's sums under sequence number `want` and waits for
 // the host's answer; returns 0 (and publishes kSeqAbort) if the wait expires.
@@ -769,12 +883,22 @@ struct Sc2State {
     DevBuf mid_parts;            // the mid rounds' tagged partials, [kMidGroups][6]
     DevBuf relay;                // the mid rounds' challenge relay (2 words)
     DevBuf gate;                 // the mid launch's arrival gate (2 words)
+    DevBuf slots;                // the streaming launches' tagged sums (ticket_publish)
+    DevBuf ticket;               // their ticket counter
+    bool slots_clean = false;    // slots all ones and ticket 0 (the last prove ended cleanly)
     long long gate_ticks = 0;    // its wait bound in wall-clock ticks
     int mid_fallbacks = 0;       // mid launches that did not start whole (last prove)
     long long* host_mtrace = nullptr;   // mapped: [40][kMidGroups][2] stamps (timing on)
     long long* dev_mtrace = nullptr;
 };
 Sc2State g_sc2;
+
+// Tagged words of the streaming launches: 256 first-pass workgroups x 9
+// sums or kFold01Groups x 3, two words each (room for more).
+constexpr size_t kSlotWords = 2048 * 9 * 2;
+static_assert(kFirst01Groups * 9 * 2 <= kSlotWords && kFold01Groups * 3 * 2 <= kSlotWords &&
+                  kFoldGroups * 3 * 2 <= kSlotWords,
+              "ticket slots");
 
 bool sc2_setup(size_t n)
 {
@@ -826,6 +950,8 @@ bool sc2_setup(size_t n)
     if (!S.relay.ensure(2 * sizeof(u64))) return false;
     if (!S.mid_parts.ensure(kMidGroups * 6 * sizeof(u64))) return false;
     if (!S.gate.ensure(2 * sizeof(unsigned))) return false;
+    if (!S.slots.ensure(kSlotWords * sizeof(u64))) return false;
+    if (!S.ticket.ensure(256)) return false;
     return true;
 }
 
@@ -845,7 +971,7 @@ bool wait_seq(hipStream_t st, unsigned seq)
     const auto t0 = std::chrono::steady_clock::now();
     long spins = 0;
     while (*hs != seq) {
-        if (*hs == kSeqAbort) return fail("sumcheck: the tail kernel gave up waiting for round %u", seq);
+        if (*hs == kSeqAbort) return fail("sumcheck: a kernel gave up waiting (round %u)", seq);
         if (++spins % 4096 == 0) {
             const hipError_t q = hipStreamQuery(st);
             if (q != hipSuccess && q != hipErrorNotReady)
@@ -927,6 +1053,22 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
     if (!sc2_setup(n)) return -1;
     Sc2State& S = g_sc2;
     for (int c = 0; c < SC_NCLASS; ++c) S.ms[c] = S.bytes[c] = 0, S.launches[c] = 0;
+    if (!S.slots_clean) {
+        // first prove, or the last one stopped part way: fresh slots and ticket
+        TP_HIP_OK(hipMemsetAsync(S.slots.ptr, 0xff, kSlotWords * sizeof(u64), st));
+        TP_HIP_OK(hipMemsetAsync(S.ticket.ptr, 0, 256, st));
+    }
+    S.slots_clean = false;   // until this prove returns 0
+    auto pub_for = [&](unsigned seq) {
+        TicketPub a;
+        a.slots = S.slots.as<u64>();
+        a.ticket = S.ticket.as<unsigned>();
+        a.host_vals = S.dev_vals;
+        a.host_seq = S.dev_seq;
+        a.seq = seq;
+        a.spin_ticks = S.spin_ticks;
+        return a;
+    };
 
     fp2* partials = S.partials.as<fp2>();
     V2* A = S.work.as<V2>();             // f: n/2, then g: n/2
@@ -989,10 +1131,8 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         unsigned seq = ++S.seq;
         unsigned nb = (unsigned)std::min<size_t>(kFirst01Groups, (n / 4 + kFirst01Threads - 1) / kFirst01Threads);
         if (!t_begin()) return -1;
-        k_sc2_first01<<<nb, kFirst01Threads, 0, st>>>(cf, cg, n / 4, partials);
+        k_sc2_first01<<<nb, kFirst01Threads, 0, st>>>(cf, cg, n / 4, pub_for(seq));
         if (!t_end(SC_FIRST, 32.0 * (double)n)) return -1;
-        TP_HIP_OK(hipGetLastError());
-        k_publish_sum<9><<<1, kBlock, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
         TP_HIP_OK(hipGetLastError());
         if (!wait_seq(st, seq) || !t_collect(SC_FIRST)) return -1;
         fp2 v[9];
@@ -1007,13 +1147,11 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         seq = ++S.seq;
         V2* of = B;
         V2* og = B + capB;
-        nb = blocks_for(n / 4);
+        nb = (unsigned)std::min<size_t>(kFold01Groups, (n / 4 + kFoldThreads - 1) / kFoldThreads);
         if (!t_begin()) return -1;
-        k_sc2_fold01_round<<<nb, kBlock, 0, st>>>(cf, cg, n / 4, r0.re, r0.im, r1.re, r1.im, of, og,
-                                                  partials);
+        k_sc2_fold01_round<<<nb, kFoldThreads, 0, st>>>(cf, cg, n / 4, r0.re, r0.im, r1.re, r1.im,
+                                                        of, og, pub_for(seq));
         if (!t_end(SC_FOLD01, 40.0 * (double)n)) return -1;
-        TP_HIP_OK(hipGetLastError());
-        k_publish_wide<3><<<1, 1024, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
         TP_HIP_OK(hipGetLastError());
         if (!wait_seq(st, seq) || !t_collect(SC_FOLD01)) return -1;
         r = round_done(2, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
@@ -1080,6 +1218,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             }
             final4[0] = S.host_vals[0].re; final4[1] = S.host_vals[0].im;
             final4[2] = S.host_vals[1].re; final4[3] = S.host_vals[1].im;
+            S.slots_clean = true;   // every streaming launch cleared its slots and ticket
             return 0;
         }
         if (use_mid && 2 * m <= (size_t)kMidGroups * kTailIn) {
@@ -1185,14 +1324,13 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         const size_t quads = m / 2;   // input T_{k-1} has length 2m
         V2* of = (k & 1) ? A : B;
         V2* og = of + ((k & 1) ? capA : capB);
-        const unsigned nb = blocks_for(quads);
+        const unsigned nb =
+            (unsigned)std::min<size_t>(kFoldGroups, (quads + kFoldThreads - 1) / kFoldThreads);
         if (!t_begin()) return -1;
         // round 2 ran top down, so round 3 runs bottom up, round 4 top down, ...
-        k_sc2_fold_round<<<nb, kBlock, 0, st>>>(cf, cg, quads, r.re, r.im, of, og, partials,
-                                                (k & 1) == 0);
+        k_sc2_fold_round<<<nb, kFoldThreads, 0, st>>>(cf, cg, quads, r.re, r.im, of, og,
+                                                      pub_for(seq), (k & 1) == 0);
         if (!t_end(SC_FOLD, 48.0 * (double)(2 * m))) return -1;
-        TP_HIP_OK(hipGetLastError());
-        k_publish_wide<3><<<1, 1024, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
         TP_HIP_OK(hipGetLastError());
         if (!wait_seq(st, seq) || !t_collect(SC_FOLD)) return -1;
         cf = of;
