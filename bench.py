@@ -688,7 +688,7 @@ def main(argv=None):
                      "ceiling": hbm_ceiling(),
                      "note": "first: rounds 0 and 1 from one read of the input; fold01: round 2 "
                              "folds the input by r_0 and r_1 into T_2; fold: streaming "
-                             "fold+round launches; mid: 64 workgroups, each folding its own "
+                             "fold+round launches; mid: 128 workgroups, each folding its own "
                              "slice of the tables in LDS, workgroup 0 gathering their sums for "
                              "the host, which answers each challenge through mapped memory; "
                              "tail: one workgroup proving the last rounds, the host answering "

@@ -125,10 +125,12 @@ void tp_sc2_tail_stats(double* tail_ms, int* tail_launches);
 void tp_sc2_set_gate_ticks(long long ticks);
 /* Mid launches of the last tp_sc2_prove that gave way at the gate. */
 int tp_sc2_mid_fallbacks(void);
-/* The prove schedule's sizes: workgroups of the sliced mid rounds, and the
- * largest input (entries per table) of the one-workgroup tail.  Streaming
- * launches run while the input exceeds mid_groups * tail_in. */
-void tp_sc2_schedule(int* mid_groups, int* tail_in);
+/* The prove schedule's sizes: workgroups of the sliced mid rounds, the
+ * largest input (entries per table) of the one-workgroup tail, and the input
+ * the mid rounds hand to the tail.  Streaming launches run while the input
+ * exceeds mid_groups * tail_in; the mid rounds run from there while it
+ * exceeds tail_in, down to mid_end entries. */
+void tp_sc2_schedule(int* mid_groups, int* tail_in, int* mid_end);
 /* Per-class statistics of the last tp_sc2_prove (timing on): for class c in
  * 0 = the first pass over the input (rounds 0 and 1), 1 = round 2 folding
  * the input by r_0 and r_1, 2 = the streaming fold+round launches, 3 = the

@@ -73,9 +73,9 @@ def test_sc2_launch_classes(bridge, torch_cuda):
     """Rounds 0 and 1 share one pass over the input, round 2 folds the input
     straight into T_2, the streaming launches run while the input exceeds
     mid_groups x tail_in entries, the sliced launch takes the rounds down to
-    2 x mid_groups entries, and one workgroup proves the rest: per class the
-    launches and the bytes they move."""
-    groups, tail_in = sc.schedule()
+    mid_end entries (when the input exceeds tail_in), and one workgroup proves
+    the rest: per class the launches and the bytes they move."""
+    groups, tail_in, mid_end = sc.schedule()
     for nv in (16, 22):
         rng = np.random.default_rng(nv)
         f, g = rand_fp2(rng, 1 << nv), rand_fp2(rng, 1 << nv)
@@ -95,7 +95,7 @@ def test_sc2_launch_classes(bridge, torch_cuda):
         if (n >> (k_mid - 1)) > tail_in:
             assert cls["mid"]["launches"] == 1
             assert cls["mid"]["bytes"] == 32 * (n >> (k_mid - 1)) + 64 * groups
-            tail_bytes = 32 * 2 * groups
+            tail_bytes = 32 * mid_end
         else:
             assert cls["mid"]["launches"] == 0
             tail_bytes = 32 * (n >> (k_mid - 1))

@@ -29,15 +29,16 @@
 //
 // Every challenge is a SHA-256 draw of the host transcript, so each round
 // goes to the host and back.  The schedule, by the size of the round's input:
-//   - above kMidGroups * kTailIn entries (streaming rounds): a fused launch,
-//     a one-workgroup publish launch writing the sums and a sequence number
-//     straight into mapped pinned host memory, the host polling it, running
-//     the transcript and launching the next round with the challenge as a
-//     kernel argument;
-//   - down to 2 kMidGroups entries (k_sc2_mid): one launch of kMidGroups
+//   - above kMidGroups * kTailIn entries (streaming rounds): a fused launch
+//     whose last workgroup writes the sums and a sequence number straight
+//     into mapped pinned host memory (ticket_publish), the host polling it,
+//     running the transcript and launching the next round with the challenge
+//     as a kernel argument;
+//   - down to kMidGroups * kMidEnd entries (k_sc2_mid): one launch of kMidGroups
 //     workgroups, each folding its own slice in LDS, the sums gathered on the
 //     device and the challenges coming back through mapped memory;
-//   - the rest (k_sc2_tail_host): one workgroup, the same exchange.
+//   - the rest (k_sc2_tail_host): one workgroup, the same exchange without
+//     the gather, about half the cost of a mid round.
 // A launch plus its dependent-launch gap costs ~6 us here, more than a
 // small round's work; the last two phases pay it once for all their rounds.
 // A one-lane SHA-256 on the device was measured at 4.8 us per compression
@@ -422,7 +423,8 @@ __device__ __forceinline__ void tail_fold(const V2* sf, const V2* sg, unsigned q
 // rounds.  Per round every workgroup writes its three partial sums to its six
 // words of `parts` (device memory) as agent-scope stores, each value (61 bits)
 // tagged with the round mod 8 in its top three bits; wave 0 of workgroup 0
-// gathers them (lane l waits for workgroup l's words to carry the round's tag),
+// gathers them (lane l waits for the words of workgroups l, l + 64, ... to
+// carry the round's tag),
 // adds them, and publishes the total to the host like the one-workgroup tail
 // (values, system fence, sequence number).  The words start as all ones (value
 // bits p, never canonical), so round 0 cannot match a stale word, and later
@@ -431,8 +433,16 @@ __device__ __forceinline__ void tail_fold(const V2* sf, const V2* sg, unsigned q
 // answer slot; workgroup 0 relays the challenge to the others through two
 // tagged words the same way.  Every wait is bounded by the wall clock; an
 // expired wait publishes kSeqAbort and relays p, and every workgroup leaves.
-constexpr int kMidGroups = 64;
+// 128 measured 1% faster a prove than 64: one streaming launch fewer (2^20
+// entries go to the slices), for a dearer gather (two workgroups per lane)
+constexpr int kMidGroups = 128;
 constexpr unsigned kGatePending = 0, kGateGo = 1, kGateAbort = 2;
+// The mid launch's last table per workgroup: kMidGroups of them make the
+// tail's input.  Measured per round, the tail's rounds cost less than a mid
+// round (about 12.5 us: the gather, the host, the relay) from 2048 entries
+// down, and more above (one workgroup folding 8192 entries took 14 us).
+constexpr unsigned kMidEnd = 2048 / kMidGroups;
+static_assert(kMidEnd >= 2 && kMidEnd * kMidGroups <= kTailIn, "mid hand-over");
 constexpr unsigned kSeqGateAbort = 0xfffffffeu;   // the mid launch did not start whole
 
 // Fresh state for a mid launch: the tagged partial words and the relay all
@@ -472,29 +482,38 @@ __device__ __forceinline__ int mid_exchange(const fp2 (&e)[3], int i, bool last,
     int ok = 1;
     const long long t0 = wall_clock64();
     if (b == 0) {
-        // gather: lane l waits for workgroup l's six words of this round
+        // gather: lane l waits for the six words of this round of workgroups
+        // l, l + 64, ... (every word's first load in flight at once)
+        constexpr int kPer = (kMidGroups + 63) / 64;
         fp2 v[3] = {zero2(), zero2(), zero2()};
-        if (lane < kMidGroups) {
-            const u64* part = parts + (size_t)lane * 6;
+        u64 w[kPer][6];
+        auto load6 = [&](int q) {
+            const unsigned gw = lane + 64u * q;
+#pragma unroll
+            for (int j = 0; j < 6; ++j)
+                w[q][j] = gw < (unsigned)kMidGroups
+                              ? __hip_atomic_load(parts + (size_t)gw * 6 + j, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : tag;   // no such workgroup: 0 under this round's tag
+        };
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) load6(q);
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
             for (;;) {
-                u64 w[6];
                 bool all = true;
 #pragma unroll
-                for (int j = 0; j < 6; ++j) {
-                    w[j] = __hip_atomic_load(part + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    all = all && (w[j] & ~P) == tag && (w[j] & P) != P;
-                }
-                if (all) {
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) v[j] = mk2(w[2 * j] & P, w[2 * j + 1] & P);
-                    break;
-                }
+                for (int j = 0; j < 6; ++j) all = all && (w[q][j] & ~P) == tag && (w[q][j] & P) != P;
+                if (all || !ok) break;
                 if (wall_clock64() - t0 > spin_ticks) {
                     ok = 0;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
+                load6(q);
             }
+#pragma unroll
+            for (int j = 0; j < 3; ++j) v[j] = add2(v[j], mk2(w[q][2 * j] & P, w[q][2 * j + 1] & P));
         }
         ok = __all(ok);
         wave_sum<3>(v);
@@ -1222,11 +1241,11 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             return 0;
         }
         if (use_mid && 2 * m <= (size_t)kMidGroups * kTailIn) {
-            // rounds k .. k + nr - 1 on slices in kMidGroups workgroups; their
-            // last table (2 entries per workgroup) goes to the tail
-            const unsigned slice = (unsigned)(2 * m / kMidGroups);   // >= 128 here
+            // rounds k .. k + nr - 1 on slices in kMidGroups workgroups, down
+            // to kMidEnd entries per workgroup; that table goes to the tail
+            const unsigned slice = (unsigned)(2 * m / kMidGroups);   // > kMidEnd here
             int nr = 0;
-            while ((slice >> (nr + 1)) >= 2) ++nr;   // log2(slice) - 1
+            while ((slice >> nr) > kMidEnd) ++nr;
             V2* of = (cf >= A && cf < A + 2 * capA) ? B : A;
             V2* og = of + (of == A ? capA : capB);
             const unsigned seq0 = S.seq + 1;
@@ -1317,7 +1336,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             cf = of;
             cg = og;
             k += nr - 1;          // the loop's ++k moves to the first round after these
-            m = kMidGroups;       // T_k for that round has kMidGroups entries per table
+            m = (size_t)kMidGroups * kMidEnd / 2;   // its input has kMidGroups x kMidEnd entries
             continue;
         }
         const unsigned seq = ++S.seq;
@@ -1405,13 +1424,15 @@ void tp_sc2_set_gate_ticks(long long ticks)
 // Mid launches of the last tp_sc2_prove that gave way at the arrival gate.
 int tp_sc2_mid_fallbacks(void) { return g_sc2.mid_fallbacks; }
 
-// The schedule's sizes: the workgroups of the sliced mid rounds and the
-// largest input (entries per table) the one-workgroup tail takes.  The
-// streaming launches run while the input exceeds mid_groups * tail_in.
-void tp_sc2_schedule(int* mid_groups, int* tail_in)
+// The schedule's sizes: the workgroups of the sliced mid rounds, the
+// largest input (entries per table) the one-workgroup tail takes, and the
+// input the mid rounds hand it.  The streaming launches run while the input
+// exceeds mid_groups * tail_in, the mid rounds while it exceeds tail_in.
+void tp_sc2_schedule(int* mid_groups, int* tail_in, int* mid_end)
 {
     *mid_groups = kMidGroups;
     *tail_in = (int)kTailIn;
+    *mid_end = (int)(kMidGroups * kMidEnd);
 }
 
 // Per-class statistics of the last tp_sc2_prove (timing on): for each class

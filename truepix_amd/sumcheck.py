@@ -137,11 +137,12 @@ def mid_profile():
 
 
 def schedule():
-    """(mid_groups, tail_in): workgroups of the sliced mid rounds and the
-    largest input (entries per table) of the one-workgroup tail."""
-    a, b = ctypes.c_int(), ctypes.c_int()
-    nb.LIB.tp_sc2_schedule(ctypes.byref(a), ctypes.byref(b))
-    return a.value, b.value
+    """(mid_groups, tail_in, mid_end): workgroups of the sliced mid rounds,
+    the largest input (entries per table) of the one-workgroup tail, and the
+    input the mid rounds hand to the tail."""
+    a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    nb.LIB.tp_sc2_schedule(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    return a.value, b.value, c.value
 
 
 def set_gate_ticks(ticks: int) -> None:
