@@ -3,7 +3,7 @@
 //
 // Fp2 addition of canonical values is exact and associative, so every
 // reduction order yields the same bits; the order here is chosen for CDNA4:
-// xor-shuffles inside each wave64, one LDS slot per wave, one partial per
+// DPP lane moves inside each wave64, one LDS slot per wave, one partial per
 // workgroup, then a one-workgroup finalize launch.  Cross-workgroup
 // visibility comes from the kernel boundary: an in-kernel "last workgroup"
 // scheme needs agent-scope release/acquire fences, which on gfx950 write back
@@ -15,18 +15,72 @@
 
 namespace tp {
 
+// One DPP step of a wave64 sum: every lane of the rows in ROW_MASK adds the
+// value CTRL selects (a lane move inside the VALU, no LDS round trip); lanes
+// of the other rows add 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ u64 dpp_u64(u64 x)
+{
+    u32 lo, hi;
+    if (ROW_MASK == 0xf) {   // every lane has a source: no old value to keep
+        lo = (u32)__builtin_amdgcn_mov_dpp((int)(u32)x, CTRL, 0xf, 0xf, false);
+        hi = (u32)__builtin_amdgcn_mov_dpp((int)(u32)(x >> 32), CTRL, 0xf, 0xf, false);
+    } else {
+        lo = (u32)__builtin_amdgcn_update_dpp(0, (int)(u32)x, CTRL, ROW_MASK, 0xf, false);
+        hi = (u32)__builtin_amdgcn_update_dpp(0, (int)(u32)(x >> 32), CTRL, ROW_MASK, 0xf, false);
+    }
+    return ((u64)hi << 32) | lo;
+}
+
+template <int CTRL, int ROW_MASK, int W>
+__device__ __forceinline__ void dpp_add_step(u64 (&w)[W])
+{
+#pragma unroll
+    for (int k = 0; k < W; ++k) w[k] += dpp_u64<CTRL, ROW_MASK>(w[k]);
+}
+
+template <int W>
+__device__ __forceinline__ void fold_words(u64 (&w)[W])
+{
+#pragma unroll
+    for (int k = 0; k < W; ++k) w[k] = fold1(w[k]);
+}
+
+// Sum over the 64 lanes of a wave (all of them active), the total in every
+// lane.  DPP steps: xor 1 and xor 2 inside each quad (quad_perm), the
+// 8-lane and 16-lane mirrors inside each row (after which every lane of a
+// row holds the row's sum), then row_bcast:15 into rows 1 and 3 and
+// row_bcast:31 into rows 2 and 3, leaving the total in lane 63, which
+// readlane hands to every lane.  The words add as plain u64 with a fold
+// after every second step: from canonical inputs a sum of four values of at
+// most 2^61 + 6 stays below 2^64, and one reduction at the end gives the
+// canonical total, the same bits as any order of canonical additions.
 template <int N>
 __device__ __forceinline__ void wave_sum(fp2 (&v)[N])
 {
+    u64 w[2 * N];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-            const u64 r = __shfl_xor(v[k].re, off, 64);
-            const u64 i = __shfl_xor(v[k].im, off, 64);
-            v[k] = add2(v[k], mk2(r, i));
-        }
+    for (int k = 0; k < N; ++k) {
+        w[2 * k] = v[k].re;
+        w[2 * k + 1] = v[k].im;
     }
+    dpp_add_step<0xb1, 0xf>(w);    // quad_perm [1,0,3,2]
+    dpp_add_step<0x4e, 0xf>(w);    // quad_perm [2,3,0,1]
+    fold_words(w);
+    dpp_add_step<0x141, 0xf>(w);   // row_half_mirror
+    dpp_add_step<0x140, 0xf>(w);   // row_mirror
+    fold_words(w);
+    dpp_add_step<0x142, 0xa>(w);   // row_bcast:15, rows 1 and 3
+    dpp_add_step<0x143, 0xc>(w);   // row_bcast:31, rows 2 and 3
+#pragma unroll
+    for (int k = 0; k < 2 * N; ++k) {
+        const u64 c = canon(fold1(w[k]));
+        const u32 lo = (u32)__builtin_amdgcn_readlane((int)(u32)c, 63);
+        const u32 hi = (u32)__builtin_amdgcn_readlane((int)(u32)(c >> 32), 63);
+        w[k] = ((u64)hi << 32) | lo;
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = mk2(w[2 * k], w[2 * k + 1]);
 }
 
 // Sums v over the workgroup (blockDim.x a multiple of 64, at most 1024); the
