@@ -378,11 +378,38 @@ constexpr int kTailThreads = 1024;
 constexpr unsigned kTailIn = 8192;
 constexpr size_t kTailLds = (size_t)kTailIn * sizeof(V2);   // both tables, T_{k0}
 
+// The host's answer to the round published as `want`, from the mapped answer
+// slot: the host writes both words with (want mod 8) << 61 above the
+// canonical value, so one read of the slot is both the check and the value
+// (a separate sequence word cost a second PCIe round trip).  A word of
+// another round, or the empty mark (all ones: value bits p), does not match.
+// Once read the slot is set back to empty; the system fence before this
+// thread's next publish orders that before the host can answer again, so a
+// stale answer never passes for a later round with the same tag.  Returns 0
+// if the wait expires.
+__device__ __forceinline__ int take_answer(u64* answer, unsigned want, long long t0,
+                                           long long spin_ticks, u64 (&rv)[2])
+{
+    const u64 tag = (u64)(want & 7) << 61;
+    for (;;) {
+        const u64 a0 = __hip_atomic_load(answer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const u64 a1 = __hip_atomic_load(answer + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if ((a0 & ~P) == tag && (a1 & ~P) == tag && (a0 & P) != P && (a1 & P) != P) {
+            rv[0] = a0 & P;
+            rv[1] = a1 & P;
+            __hip_atomic_store(answer, kSlotEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(answer + 1, kSlotEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            return 1;
+        }
+        if (wall_clock64() - t0 > spin_ticks) return 0;
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
 // Lane 0 publishes a round's sums under sequence number `want` and waits for
 // the host's answer; returns 0 (and publishes kSeqAbort) if the wait expires.
 __device__ __forceinline__ int tail_exchange(const fp2 (&e)[3], unsigned want, fp2* host_vals,
-                                             unsigned* host_seq, const u64* answer,
-                                             unsigned* answer_seq, long long spin_ticks,
+                                             unsigned* host_seq, u64* answer, long long spin_ticks,
                                              long long* trace, int i, u64 (&rv)[2])
 {
     if (trace) trace[3 * i + 1] = wall_clock64();
@@ -391,17 +418,11 @@ __device__ __forceinline__ int tail_exchange(const fp2 (&e)[3], unsigned want, f
     host_vals[2] = e[2];
     __threadfence_system();
     __hip_atomic_store(host_seq, want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    const long long t0 = wall_clock64();
-    while (__hip_atomic_load(answer_seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != want) {
-        if (wall_clock64() - t0 > spin_ticks) {
-            __hip_atomic_store(host_seq, kSeqAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            return 0;
-        }
-        __builtin_amdgcn_s_sleep(1);
+    if (!take_answer(answer, want, wall_clock64(), spin_ticks, rv)) {
+        __hip_atomic_store(host_seq, kSeqAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        return 0;
     }
     if (trace) trace[3 * i + 2] = wall_clock64();
-    rv[0] = __hip_atomic_load(answer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    rv[1] = __hip_atomic_load(answer + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     return 1;
 }
 
@@ -465,9 +486,9 @@ __global__ void __launch_bounds__(256) k_sc2_mid_reset(u64* parts, u64* relay, u
 // wait expired (every workgroup then leaves); the challenge goes to rv.
 __device__ __forceinline__ int mid_exchange(const fp2 (&e)[3], int i, bool last, unsigned b,
                                             int lane, u64* parts, fp2* host_vals,
-                                            unsigned* host_seq, unsigned seq0, const u64* answer,
-                                            unsigned* answer_seq, long long spin_ticks, u64* relay,
-                                            long long* trace, u64 (&rv)[2])
+                                            unsigned* host_seq, unsigned seq0, u64* answer,
+                                            long long spin_ticks, u64* relay, long long* trace,
+                                            u64 (&rv)[2])
 {
     const u64 tag = (u64)(i & 7) << 61;   // round mod 8 over the 61 value bits
     if (lane == 0) {
@@ -526,17 +547,11 @@ __device__ __forceinline__ int mid_exchange(const fp2 (&e)[3], int i, bool last,
                 __threadfence_system();
                 __hip_atomic_store(host_seq, seq0 + (unsigned)i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 if (!last) {
-                    const unsigned want = seq0 + (unsigned)i;
-                    while (__hip_atomic_load(answer_seq, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) != want) {
-                        if (wall_clock64() - t0 > spin_ticks) {
-                            ok = 0;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                    }
+                    u64 a[2];
+                    ok = take_answer(answer, seq0 + (unsigned)i, t0, spin_ticks, a);
                     if (ok) {
-                        re = __hip_atomic_load(answer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                        im = __hip_atomic_load(answer + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                        re = a[0];
+                        im = a[1];
                     }
                 }
             }
@@ -582,7 +597,7 @@ __device__ __forceinline__ int mid_exchange(const fp2 (&e)[3], int i, bool last,
 __global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
     const V2* __restrict__ fin, const V2* __restrict__ gin, unsigned slice_in, int n_rounds,
     u64 rr, u64 ri, u64* parts, fp2* host_vals, unsigned* host_seq, unsigned seq0,
-    const u64* answer, unsigned* answer_seq, long long spin_ticks, V2* __restrict__ fout,
+    u64* answer, long long spin_ticks, V2* __restrict__ fout,
     V2* __restrict__ gout, u64* relay, long long* trace, unsigned* gate, long long gate_ticks)
 {
     extern __shared__ V2 tl[];
@@ -692,7 +707,7 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
             wave_sum<3>(e);
             u64 rv[2];
             const int ok = mid_exchange(e, i, last, b, lane, parts, host_vals, host_seq, seq0, answer,
-                                        answer_seq, spin_ticks, relay, trace, rv);
+                                        spin_ticks, relay, trace, rv);
             if (lane == 0) {
                 s_ok = ok;
                 s_r[0] = rv[0];
@@ -729,8 +744,8 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
         wave_sum<3>(e);
         const bool last = i == n_rounds - 1;
         u64 rv[2];
-        if (!mid_exchange(e, i, last, b, lane, parts, host_vals, host_seq, seq0, answer, answer_seq,
-                          spin_ticks, relay, trace, rv))
+        if (!mid_exchange(e, i, last, b, lane, parts, host_vals, host_seq, seq0, answer, spin_ticks,
+                          relay, trace, rv))
             return;
         if (!last) r = mk2(rv[0], rv[1]);
         len /= 2;
@@ -743,8 +758,8 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
 
 __global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
     const V2* __restrict__ fin, const V2* __restrict__ gin, unsigned len_in, int k0, int nv,
-    u64 rr, u64 ri, fp2* host_vals, unsigned* host_seq, unsigned seq0, const u64* answer,
-    unsigned* answer_seq, long long spin_ticks, long long* trace)
+    u64 rr, u64 ri, fp2* host_vals, unsigned* host_seq, unsigned seq0, u64* answer,
+    long long spin_ticks, long long* trace)
 {
     extern __shared__ V2 tl[];
     __shared__ u64 s_r[2];
@@ -813,7 +828,7 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
         if (t == 0) {
             u64 rv[2] = {0, 0};
             s_ok = tail_exchange(e, seq0 + (unsigned)(k - k0), host_vals, host_seq, answer,
-                                 answer_seq, spin_ticks, trace, k - k0, rv);
+                                 spin_ticks, trace, k - k0, rv);
             s_r[0] = rv[0];
             s_r[1] = rv[1];
         }
@@ -851,7 +866,7 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
         u64 rv[2] = {0, 0};
         int ok = 1;
         if (t == 0)
-            ok = tail_exchange(e, seq0 + (unsigned)(k - k0), host_vals, host_seq, answer, answer_seq,
+            ok = tail_exchange(e, seq0 + (unsigned)(k - k0), host_vals, host_seq, answer,
                                spin_ticks, trace, k - k0, rv);
         if (!__shfl(ok, 0, 64)) return;
         r = mk2(__shfl(rv[0], 0, 64), __shfl(rv[1], 0, 64));
@@ -877,12 +892,10 @@ struct Sc2State {
     // the tail's answer (a challenge) at 512 and its sequence number at 528
     fp2* host_vals = nullptr;
     unsigned* host_seq = nullptr;
-    u64* host_answer = nullptr;
-    unsigned* host_answer_seq = nullptr;
+    u64* host_answer = nullptr;         // mapped answer slot: 2 tagged words
     fp2* dev_vals = nullptr;     // device aliases of the mapped memory
     unsigned* dev_seq = nullptr;
     u64* dev_answer = nullptr;
-    unsigned* dev_answer_seq = nullptr;
     long long spin_ticks = 0;    // the tail's wait bound in wall-clock ticks
     double clock_mhz = 100;      // wall-clock rate
     long long* host_trace = nullptr;   // mapped: 3 stamps per tail round (timing on)
@@ -929,17 +942,15 @@ bool sc2_setup(size_t n)
         S.host_vals = reinterpret_cast<fp2*>(hc);
         S.host_seq = reinterpret_cast<unsigned*>(hc + 256);
         S.host_answer = reinterpret_cast<u64*>(hc + 512);
-        S.host_answer_seq = reinterpret_cast<unsigned*>(hc + 528);
         S.host_trace = reinterpret_cast<long long*>(hc + 1024);   // 3 x 40 stamps
         *S.host_seq = 0;
-        *S.host_answer_seq = 0;
+        S.host_answer[0] = S.host_answer[1] = kSlotEmpty;
         void* dv = nullptr;
         TP_HIP_OK(hipHostGetDevicePointer(&dv, hv, 0));
         char* dc = static_cast<char*>(dv);
         S.dev_vals = reinterpret_cast<fp2*>(dc);
         S.dev_seq = reinterpret_cast<unsigned*>(dc + 256);
         S.dev_answer = reinterpret_cast<u64*>(dc + 512);
-        S.dev_answer_seq = reinterpret_cast<unsigned*>(dc + 528);
         S.dev_trace = reinterpret_cast<long long*>(dc + 1024);
         TP_HIP_OK(hipFuncSetAttribute((const void*)k_sc2_tail_host,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLds));
@@ -1078,6 +1089,10 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         TP_HIP_OK(hipMemsetAsync(S.ticket.ptr, 0, 256, st));
     }
     S.slots_clean = false;   // until this prove returns 0
+    {
+        volatile u64* ans = S.host_answer;   // no answer pending
+        ans[0] = ans[1] = kSlotEmpty;
+    }
     auto pub_for = [&](unsigned seq) {
         TicketPub a;
         a.slots = S.slots.as<u64>();
@@ -1204,20 +1219,18 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             if (!t_begin()) return -1;
             k_sc2_tail_host<<<1, kTailThreads, kTailLds, st>>>(
                 cf, cg, (unsigned)(2 * m), k, nv, r.re, r.im, S.dev_vals, S.dev_seq, seq0,
-                S.dev_answer, S.dev_answer_seq, S.spin_ticks, S.timed ? S.dev_trace : nullptr);
+                S.dev_answer, S.spin_ticks, S.timed ? S.dev_trace : nullptr);
             if (!t_end(SC_TAIL, 32.0 * (double)(2 * m))) return -1;
             TP_HIP_OK(hipGetLastError());
             volatile u64* ans = S.host_answer;
-            volatile unsigned* ans_seq = S.host_answer_seq;
             double host_us = 0;
             for (int i = 0; i < nr; ++i) {
                 if (!wait_seq(st, seq0 + i)) return -1;
                 const auto h0 = std::chrono::steady_clock::now();
                 const fp2 rk = round_done(k + i, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
-                ans[0] = rk.re;
-                ans[1] = rk.im;
-                std::atomic_thread_fence(std::memory_order_release);
-                *ans_seq = seq0 + i;
+                const u64 tag = (u64)((seq0 + i) & 7) << 61;   // marks the words as this round's
+                ans[0] = rk.re | tag;
+                ans[1] = rk.im | tag;
                 host_us += std::chrono::duration<double, std::micro>(
                                std::chrono::steady_clock::now() - h0).count();
             }
@@ -1259,8 +1272,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             fp2* a_vals = S.dev_vals;
             unsigned* a_seq = S.dev_seq;
             unsigned a_seq0 = seq0;
-            const u64* a_ans = S.dev_answer;
-            unsigned* a_ans_seq = S.dev_answer_seq;
+            u64* a_ans = S.dev_answer;
             long long a_spin = S.spin_ticks;
             V2* a_of = of;
             V2* a_og = og;
@@ -1269,7 +1281,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             unsigned* a_gate = S.gate.as<unsigned>();
             long long a_gate_ticks = S.gate_ticks;
             void* args[] = {&a_f, &a_g, &a_slice, &a_nr, &a_rr, &a_ri, &a_parts, &a_vals, &a_seq,
-                            &a_seq0, &a_ans, &a_ans_seq, &a_spin, &a_of, &a_og, &a_relay, &a_trace,
+                            &a_seq0, &a_ans, &a_spin, &a_of, &a_og, &a_relay, &a_trace,
                             &a_gate, &a_gate_ticks};
             k_sc2_mid_reset<<<1, 256, 0, st>>>(a_parts, a_relay, a_gate);
             TP_HIP_OK(hipGetLastError());
@@ -1293,15 +1305,13 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
                 continue;
             }
             volatile u64* ans = S.host_answer;
-            volatile unsigned* ans_seq = S.host_answer_seq;
             for (int i = 0; i < nr; ++i) {
                 if (i > 0 && !wait_seq(st, seq0 + i)) return -1;
                 r = round_done(k + i, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
                 if (i + 1 < nr) {
-                    ans[0] = r.re;
-                    ans[1] = r.im;
-                    std::atomic_thread_fence(std::memory_order_release);
-                    *ans_seq = seq0 + i;
+                    const u64 tag = (u64)((seq0 + i) & 7) << 61;
+                    ans[0] = r.re | tag;
+                    ans[1] = r.im | tag;
                 }
             }
             TP_HIP_OK(hipEventRecord(S.ev_round, st));
