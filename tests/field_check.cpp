@@ -1,8 +1,9 @@
 // CPU check of the device field arithmetic in truepix_amd/csrc/fp61.hpp
 // (built with plain g++, so the integer code the GPU runs is exercised as
 // written): mul31 against 128-bit products, including the worst-case
-// operands that set its no-overflow bounds, and LazyProdSum against an
-// exact Fp2 dot product.  Prints each failure; exit status 1 on any.
+// operands that set its no-overflow bounds, LazyProdSum against an exact Fp2
+// dot product, and the sumcheck fold against a + r (b - a).  Prints each
+// failure; exit status 1 on any.
 #include "fp61.hpp"
 #include <cstdio>
 #include <vector>
@@ -92,6 +93,30 @@ int main() {
         // (-1 - i)^2 = 2i, times 1000
         const fp2 v = z.value();
         CHECK(v.re == 0 && v.im == 2000, "LazyProdSum at p - 1 differs");
+    }
+    // fold2 = a + r (b - a) against the exact value, extremes included
+    {
+        const u64 ext[] = {0, 1, 2, P - 1, P - 2, 0x7fffffffull, 0x80000000ull, (1ull << 60) - 1,
+                           1ull << 60, 0x1fffffff80000000ull, 0x1fffffff7fffffffull};
+        auto check_fold = [&](fp2 a, fp2 b, fp2 r) {
+            const fp2 got = fold2(a, b, r);
+            // a + r (b - a), i^2 = -1
+            const u64 dre = modp((u128)b.re + P - a.re), dim = modp((u128)b.im + P - a.im);
+            const u64 re = modp((u128)a.re + (u128)r.re * dre + (u128)(P - r.im) * dim);
+            const u64 im = modp((u128)a.im + (u128)r.re * dim + (u128)r.im * dre);
+            CHECK(got.re == re && got.im == im, "fold2 differs: a=(%llx,%llx) b=(%llx,%llx) r=(%llx,%llx)",
+                  (unsigned long long)a.re, (unsigned long long)a.im, (unsigned long long)b.re,
+                  (unsigned long long)b.im, (unsigned long long)r.re, (unsigned long long)r.im);
+        };
+        for (u64 x : ext)
+            for (u64 y : ext)
+                for (u64 w : ext) {
+                    check_fold(mk2(x, y), mk2(y, w), mk2(w, x));
+                    check_fold(mk2(w, w), mk2(x, x), mk2(y, y));
+                    check_fold(mk2(0, x), mk2(P - 1, y), mk2(w, P - 1));
+                }
+        for (int i = 0; i < 200000; ++i)
+            check_fold(mk2(rnd() % P, rnd() % P), mk2(rnd() % P, rnd() % P), mk2(rnd() % P, rnd() % P));
     }
     std::printf("field check: %d failures\n", fails);
     return fails ? 1 : 0;
