@@ -105,15 +105,18 @@ int tp_device_count(void);
  * put([a, b, c]) and draws r_k = rand_ext("c2-<k>"), unless `fixed` (2*nv
  * limbs) is given.  Outputs: polys (nv x 6 limbs, (a, b, c) of
  * s_k(X) = a X^2 + b X + c as in the reference's quadratic_poly), chals
- * (nv x 2), final4 = f~(r), g~(r).  stream: hipStream_t or NULL.  0 / -1. */
+ * (nv x 2), final4 = f~(r), g~(r).  stream: hipStream_t or NULL.  0 / -1.
+ * One prove at a time per process: the mapped answer slot, the launches'
+ * tagged sum slots and the work tables are the library's own, shared by
+ * every call. */
 int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed,
                  const uint64_t* fixed, uint64_t* polys, uint64_t* chals, uint64_t* final4,
                  void* stream);
 /* Per-launch HIP-event timing of the fused fold+round kernel (off by default). */
 void tp_sc2_set_timing(int on);
 /* Summed event milliseconds, algorithmic bytes and launch count of the
- * streaming fused fold+round kernel in the last tp_sc2_prove (rounds whose
- * input exceeds 1024 entries). */
+ * streaming fused fold+round launches in the last tp_sc2_prove (the rounds
+ * before the sliced ones; see tp_sc2_schedule). */
 void tp_sc2_stats(double* fold_ms, double* fold_bytes, int* fold_launches);
 /* Summed event milliseconds and count of the one-workgroup tail rounds
  * (fold + round sum + publish in one launch) of the last tp_sc2_prove. */
