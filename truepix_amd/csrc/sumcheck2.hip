@@ -30,8 +30,8 @@
 // Every challenge is a SHA-256 draw of the host transcript, so each round
 // goes to the host and back.  The schedule, by the size of the round's input:
 //   - above kMidGroups * kTailIn entries (streaming rounds): a fused launch
-//     whose last workgroup writes the sums and a sequence number straight
-//     into mapped pinned host memory (ticket_publish), the host polling it,
+//     whose last workgroup writes the sums, tagged with the round, straight
+//     into mapped pinned host memory (ticket_publish), the host polling them,
 //     running the transcript and launching the next round with the challenge
 //     as a kernel argument;
 //   - down to kMidGroups * kMidEnd entries (k_sc2_mid): one launch of kMidGroups
@@ -80,6 +80,24 @@ struct TicketPub {
 
 constexpr u64 kSlotEmpty = ~0ull;   // value bits p under tag 7: no round's word
 
+// A round's sums to the host's mapped slot: every word carries (seq mod 8)
+// << 61 above its canonical value, so the host checks each word itself and
+// no fence has to order the values before a sequence word (a system fence
+// cost about 0.4 us a round).  The host empties the slot (all ones) once it
+// has read it, before it answers or launches again, so a stale word never
+// passes for a later round.  Aborts still go to the sequence word.
+template <int N>
+__device__ __forceinline__ void publish_tagged(fp2* host_vals, const fp2 (&v)[N], unsigned seq)
+{
+    const u64 tag = (u64)(seq & 7) << 61;
+    u64* w = reinterpret_cast<u64*>(host_vals);
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        __hip_atomic_store(w + 2 * k, v[k].re | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(w + 2 * k + 1, v[k].im | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 // End of a streaming launch, in place of a separate publish launch (which
 // cost its own launch and a dependent-launch gap per round).  Each
 // workgroup's N sums go to its slot as tagged words, value | (seq mod 8) << 61
@@ -88,7 +106,7 @@ constexpr u64 kSlotEmpty = ~0ull;   // value bits p under tag 7: no round's word
 // this launch's tag (spinning, bounded by the wall clock, on any still in
 // flight), clears the slots to all ones and the ticket for the next launch
 // (so no word of an earlier round, whatever its tag, is ever read as this
-// one's), and publishes the totals like k_publish_*: values, then seq.
+// one's), and publishes the totals as tagged words (publish_tagged).
 template <int N>
 __device__ __forceinline__ void ticket_publish(fp2 (&v)[N], const TicketPub& a)
 {
@@ -156,11 +174,14 @@ __device__ __forceinline__ void ticket_publish(fp2 (&v)[N], const TicketPub& a)
     }
     __syncthreads();
     if (t == 0) {
-#pragma unroll
-        for (int q = 0; q < N; ++q) a.host_vals[q] = s_tot[q];
-        __threadfence_system();
-        __hip_atomic_store(a.host_seq, ok ? a.seq : kSeqAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        fp2 tot[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) tot[q] = s_tot[q];
+        if (ok)
+            publish_tagged<N>(a.host_vals, tot, a.seq);
+        else
+            __hip_atomic_store(a.host_seq, kSeqAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -369,9 +390,9 @@ __global__ void __launch_bounds__(kFoldThreads) k_sc2_fold_round(const V2* __res
 // (len_in <= kTailIn entries per table) in global memory and r_{k0-1}: the
 // workgroup folds it into LDS and proves rounds k0..nv-1 there (T_k in place:
 // f at [0, kTailIn/2), g after it), then folds the last pair.  Per round,
-// lane 0 publishes (E0, E1, EA) and the sequence number seq0 + i like the
-// other launches, then waits for the host to write the challenge and the
-// same number into the answer slot; the final values go out as seq0 + nr.
+// lane 0 publishes (E0, E1, EA) tagged with seq0 + i like the other
+// launches, then waits for the host to write the challenge, tagged the same,
+// into the answer slot; the final values go out tagged with seq0 + nr.
 // The wait is bounded by the wall clock (spin_ticks): on expiry the kernel
 // publishes kSeqAbort and exits, so it ends whatever the host does.
 constexpr int kTailThreads = 1024;
@@ -383,11 +404,11 @@ constexpr size_t kTailLds = (size_t)kTailIn * sizeof(V2);   // both tables, T_{k
 // canonical value, so one read of the slot is both the check and the value
 // (a separate sequence word cost a second PCIe round trip).  A word of
 // another round, or the empty mark (all ones: value bits p), does not match.
-// Once read the slot is set back to empty; the system fence before this
-// thread's next publish orders that before the host can answer again, so a
-// stale answer never passes for a later round with the same tag.  Returns 0
+// The host empties the slot when a prove starts and each answer overwrites
+// the one before; answered rounds in a prove are never 8 sequence numbers
+// apart with no answer between, so a stale answer never matches.  Returns 0
 // if the wait expires.
-__device__ __forceinline__ int take_answer(u64* answer, unsigned want, long long t0,
+__device__ __forceinline__ int take_answer(const u64* answer, unsigned want, long long t0,
                                            long long spin_ticks, u64 (&rv)[2])
 {
     const u64 tag = (u64)(want & 7) << 61;
@@ -397,8 +418,6 @@ __device__ __forceinline__ int take_answer(u64* answer, unsigned want, long long
         if ((a0 & ~P) == tag && (a1 & ~P) == tag && (a0 & P) != P && (a1 & P) != P) {
             rv[0] = a0 & P;
             rv[1] = a1 & P;
-            __hip_atomic_store(answer, kSlotEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            __hip_atomic_store(answer + 1, kSlotEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             return 1;
         }
         if (wall_clock64() - t0 > spin_ticks) return 0;
@@ -413,11 +432,7 @@ __device__ __forceinline__ int tail_exchange(const fp2 (&e)[3], unsigned want, f
                                              long long* trace, int i, u64 (&rv)[2])
 {
     if (trace) trace[3 * i + 1] = wall_clock64();
-    host_vals[0] = e[0];
-    host_vals[1] = e[1];
-    host_vals[2] = e[2];
-    __threadfence_system();
-    __hip_atomic_store(host_seq, want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    publish_tagged<3>(host_vals, e, want);
     if (!take_answer(answer, want, wall_clock64(), spin_ticks, rv)) {
         __hip_atomic_store(host_seq, kSeqAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         return 0;
@@ -447,7 +462,7 @@ __device__ __forceinline__ void tail_fold(const V2* sf, const V2* sg, unsigned q
 // gathers them (lane l waits for the words of workgroups l, l + 64, ... to
 // carry the round's tag),
 // adds them, and publishes the total to the host like the one-workgroup tail
-// (values, system fence, sequence number).  The words start as all ones (value
+// (tagged words, publish_tagged).  The words start as all ones (value
 // bits p, never canonical), so round 0 cannot match a stale word, and later
 // rounds find the previous round's tag until the new word lands: no fence and
 // no extra launch.  The host runs the transcript and answers through the
@@ -541,11 +556,7 @@ __device__ __forceinline__ int mid_exchange(const fp2 (&e)[3], int i, bool last,
         u64 re = P, im = P;   // p: the wait expired
         if (lane == 0) {
             if (ok) {
-                host_vals[0] = v[0];
-                host_vals[1] = v[1];
-                host_vals[2] = v[2];
-                __threadfence_system();
-                __hip_atomic_store(host_seq, seq0 + (unsigned)i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                publish_tagged<3>(host_vals, v, seq0 + (unsigned)i);
                 if (!last) {
                     u64 a[2];
                     ok = take_answer(answer, seq0 + (unsigned)i, t0, spin_ticks, a);
@@ -874,10 +885,8 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
     }
     if (t == 0) {
         // T_{nv-1}: two entries (in LDS, or still the input when no round ran here)
-        host_vals[0] = fold2(ldv(sf), ldv(sf + 1), r);
-        host_vals[1] = fold2(ldv(sg), ldv(sg + 1), r);
-        __threadfence_system();
-        __hip_atomic_store(host_seq, seq0 + (unsigned)nr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const fp2 fin[2] = {fold2(ldv(sf), ldv(sf + 1), r), fold2(ldv(sg), ldv(sg + 1), r)};
+        publish_tagged<2>(host_vals, fin, seq0 + (unsigned)nr);
     }
 }
 
@@ -888,8 +897,9 @@ enum Sc2Class { SC_FIRST, SC_FOLD01, SC_FOLD, SC_TAIL, SC_MID, SC_NCLASS };
 struct Sc2State {
     DevBuf work;                 // T_k ping-pong: A (n/2) and B (n/4), for f and g
     DevBuf partials;             // up to 9 fp2 per workgroup
-    // mapped pinned memory: up to 16 values at 0, the sequence number at 256,
-    // the tail's answer (a challenge) at 512 and its sequence number at 528
+    // mapped pinned memory: up to 16 published values at 0 (tagged words), the
+    // sequence word (abort marks; k_publish_* launches) at 256, the answer (a
+    // challenge, two tagged words) at 512
     fp2* host_vals = nullptr;
     unsigned* host_seq = nullptr;
     u64* host_answer = nullptr;         // mapped answer slot: 2 tagged words
@@ -915,6 +925,7 @@ struct Sc2State {
     DevBuf mid_parts;            // the mid rounds' tagged partials, [kMidGroups][6]
     DevBuf relay;                // the mid rounds' challenge relay (2 words)
     DevBuf gate;                 // the mid launch's arrival gate (2 words)
+    fp2 pub[16];                 // the last published round's values, read out of the slot
     DevBuf slots;                // the streaming launches' tagged sums (ticket_publish)
     DevBuf ticket;               // their ticket counter
     bool slots_clean = false;    // slots all ones and ticket 0 (the last prove ended cleanly)
@@ -1016,29 +1027,52 @@ bool wait_seq(hipStream_t st, unsigned seq)
     return true;
 }
 
-// The first round of a mid launch: 1 when sequence number seq is
-// published, 2 when the launch gave way at its arrival gate, 0 on failure.
-int wait_first_mid(hipStream_t st, unsigned seq)
+// Waits until the device has published round `seq` as n tagged Fp2 values
+// (publish_tagged), copies them to pub[] and empties the slot.  Returns 1;
+// 2 when `gate` and the mid launch gave way at its arrival gate instead; 0
+// on failure (a kernel's abort mark, a stream error, the stream idle without
+// the values, or 30 s).
+int wait_pub(hipStream_t st, unsigned seq, int n, bool gate)
 {
-    volatile unsigned* hs = g_sc2.host_seq;
+    Sc2State& S = g_sc2;
+    volatile u64* w = reinterpret_cast<volatile u64*>(S.host_vals);
+    volatile unsigned* hs = S.host_seq;
+    const u64 tag = (u64)(seq & 7) << 61;
+    auto ready = [&]() {
+        for (int i = 0; i < 2 * n; ++i) {
+            const u64 x = w[i];
+            if ((x & ~P) != tag || (x & P) == P) return false;
+        }
+        return true;
+    };
     const auto t0 = std::chrono::steady_clock::now();
-    for (long spins = 1;; ++spins) {
+    for (long spins = 1; !ready(); ++spins) {
         const unsigned v = *hs;
-        if (v == seq) break;
-        if (v == kSeqGateAbort) return 2;
-        if (v == kSeqAbort) return fail("sumcheck: a mid-round workgroup gave up waiting"), 0;
+        if (gate && v == kSeqGateAbort) return 2;
+        if (v == kSeqAbort) return fail("sumcheck: a kernel gave up waiting (round %u)", seq), 0;
         if (spins % 4096 == 0) {
             const hipError_t q = hipStreamQuery(st);
             if (q != hipSuccess && q != hipErrorNotReady)
                 return fail("sumcheck stream error: %s", hipGetErrorString(q)), 0;
-            if (q == hipSuccess && *hs != seq && *hs != kSeqGateAbort)
+            if (q == hipSuccess && !ready() && !(gate && *hs == kSeqGateAbort))
                 return fail("sumcheck: stream idle but round %u not published", seq), 0;
             if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(30))
                 return fail("sumcheck: timed out waiting for round %u", seq), 0;
         }
     }
-    std::atomic_thread_fence(std::memory_order_acquire);
+    for (int i = 0; i < n; ++i) S.pub[i] = mk2(w[2 * i] & P, w[2 * i + 1] & P);
+    for (int i = 0; i < 2 * n; ++i) w[i] = kSlotEmpty;
     return 1;
+}
+
+// The n values a k_publish_* launch left under sequence number seq (read
+// after wait_seq), copied to pub[]; the slot is emptied like wait_pub's.
+void take_seq_vals(int n)
+{
+    Sc2State& S = g_sc2;
+    volatile u64* w = reinterpret_cast<volatile u64*>(S.host_vals);
+    for (int i = 0; i < n; ++i) S.pub[i] = mk2(w[2 * i], w[2 * i + 1]);
+    for (int i = 0; i < 2 * n; ++i) w[i] = kSlotEmpty;
 }
 
 // Waits until the launches before `ev` on its stream are done.
@@ -1090,8 +1124,12 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
     }
     S.slots_clean = false;   // until this prove returns 0
     {
-        volatile u64* ans = S.host_answer;   // no answer pending
+        // no answer pending, no values published, no abort mark
+        volatile u64* ans = S.host_answer;
         ans[0] = ans[1] = kSlotEmpty;
+        volatile u64* w = reinterpret_cast<volatile u64*>(S.host_vals);
+        for (int i = 0; i < 32; ++i) w[i] = kSlotEmpty;
+        *reinterpret_cast<volatile unsigned*>(S.host_seq) = 0;
     }
     auto pub_for = [&](unsigned seq) {
         TicketPub a;
@@ -1168,9 +1206,9 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         k_sc2_first01<<<nb, kFirst01Threads, 0, st>>>(cf, cg, n / 4, pub_for(seq));
         if (!t_end(SC_FIRST, 32.0 * (double)n)) return -1;
         TP_HIP_OK(hipGetLastError());
-        if (!wait_seq(st, seq) || !t_collect(SC_FIRST)) return -1;
+        if (wait_pub(st, seq, 9, false) != 1 || !t_collect(SC_FIRST)) return -1;
         fp2 v[9];
-        for (int i = 0; i < 9; ++i) v[i] = S.host_vals[i];
+        for (int i = 0; i < 9; ++i) v[i] = S.pub[i];
         const fp2 r0 = round_done(0, add2(v[0], v[3]), add2(v[1], v[4]), add2(v[2], v[5]));
         // the round-1 quadratics at r_0: c0 + r (c1 - c0 - c2) + r^2 c2
         auto at_r0 = [&](fp2 c0, fp2 c1, fp2 c2) {
@@ -1187,8 +1225,8 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
                                                         of, og, pub_for(seq));
         if (!t_end(SC_FOLD01, 40.0 * (double)n)) return -1;
         TP_HIP_OK(hipGetLastError());
-        if (!wait_seq(st, seq) || !t_collect(SC_FOLD01)) return -1;
-        r = round_done(2, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
+        if (wait_pub(st, seq, 3, false) != 1 || !t_collect(SC_FOLD01)) return -1;
+        r = round_done(2, S.pub[0], S.pub[1], S.pub[2]);
         cf = of;
         cg = og;
         k0 = 3;
@@ -1203,7 +1241,8 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         k_publish_wide<3><<<1, 1024, 0, st>>>(partials, nb, S.dev_vals, S.dev_seq, seq);
         TP_HIP_OK(hipGetLastError());
         if (!wait_seq(st, seq) || !t_collect(SC_FIRST)) return -1;
-        r = round_done(0, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
+        take_seq_vals(3);
+        r = round_done(0, S.pub[0], S.pub[1], S.pub[2]);
         k0 = 1;
         m = n / 2;
     }
@@ -1225,16 +1264,16 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             volatile u64* ans = S.host_answer;
             double host_us = 0;
             for (int i = 0; i < nr; ++i) {
-                if (!wait_seq(st, seq0 + i)) return -1;
+                if (wait_pub(st, seq0 + i, 3, false) != 1) return -1;
                 const auto h0 = std::chrono::steady_clock::now();
-                const fp2 rk = round_done(k + i, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
+                const fp2 rk = round_done(k + i, S.pub[0], S.pub[1], S.pub[2]);
                 const u64 tag = (u64)((seq0 + i) & 7) << 61;   // marks the words as this round's
                 ans[0] = rk.re | tag;
                 ans[1] = rk.im | tag;
                 host_us += std::chrono::duration<double, std::micro>(
                                std::chrono::steady_clock::now() - h0).count();
             }
-            if (!wait_seq(st, seq0 + nr) || !t_collect(SC_TAIL)) return -1;
+            if (wait_pub(st, seq0 + nr, 2, false) != 1 || !t_collect(SC_TAIL)) return -1;
             if (S.timed && nr > 0) {
                 const volatile long long* tr = S.host_trace;
                 double work = 0, wait = 0;
@@ -1248,8 +1287,8 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
                 S.tail_wait_us = wait / S.clock_mhz / nr;
                 S.tail_host_us = host_us / nr;
             }
-            final4[0] = S.host_vals[0].re; final4[1] = S.host_vals[0].im;
-            final4[2] = S.host_vals[1].re; final4[3] = S.host_vals[1].im;
+            final4[0] = S.pub[0].re; final4[1] = S.pub[0].im;
+            final4[2] = S.pub[1].re; final4[3] = S.pub[1].im;
             S.slots_clean = true;   // every streaming launch cleared its slots and ticket
             return 0;
         }
@@ -1289,7 +1328,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             TP_HIP_OK(hipLaunchKernel((const void*)k_sc2_mid, dim3(kMidGroups), dim3(kTailThreads),
                                       args, (unsigned)(slice * sizeof(V2)), st));
             if (!t_end(SC_MID, 32.0 * (double)(2 * m) + 64.0 * kMidGroups)) return -1;
-            const int first = wait_first_mid(st, seq0);
+            const int first = wait_pub(st, seq0, 3, true);
             if (first == 0) return -1;
             if (first == 2) {
                 // not every workgroup was resident: these rounds go to the
@@ -1306,8 +1345,8 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             }
             volatile u64* ans = S.host_answer;
             for (int i = 0; i < nr; ++i) {
-                if (i > 0 && !wait_seq(st, seq0 + i)) return -1;
-                r = round_done(k + i, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
+                if (i > 0 && wait_pub(st, seq0 + i, 3, false) != 1) return -1;
+                r = round_done(k + i, S.pub[0], S.pub[1], S.pub[2]);
                 if (i + 1 < nr) {
                     const u64 tag = (u64)((seq0 + i) & 7) << 61;
                     ans[0] = r.re | tag;
@@ -1361,10 +1400,10 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
                                                       pub_for(seq), (k & 1) == 0);
         if (!t_end(SC_FOLD, 48.0 * (double)(2 * m))) return -1;
         TP_HIP_OK(hipGetLastError());
-        if (!wait_seq(st, seq) || !t_collect(SC_FOLD)) return -1;
+        if (wait_pub(st, seq, 3, false) != 1 || !t_collect(SC_FOLD)) return -1;
         cf = of;
         cg = og;
-        r = round_done(k, S.host_vals[0], S.host_vals[1], S.host_vals[2]);
+        r = round_done(k, S.pub[0], S.pub[1], S.pub[2]);
         m /= 2;
     }
 }
