@@ -126,6 +126,11 @@ void tp_sc2_tail_stats(double* tail_ms, int* tail_launches);
  * at once and otherwise gives way to streaming launches for those rounds.  0
  * makes it give way unless every workgroup has already arrived (tests). */
 void tp_sc2_set_gate_ticks(long long ticks);
+/* Bound of every device-side wait of a prove in wall-clock ticks (default
+ * 10 s; negative restores it).  A small bound makes a prove fail with every
+ * kernel leaving on its own (tests of the error path); the next prove starts
+ * from fresh slots. */
+void tp_sc2_set_spin_ticks(long long ticks);
 /* Mid launches of the last tp_sc2_prove that gave way at the gate. */
 int tp_sc2_mid_fallbacks(void);
 /* The prove schedule's sizes: workgroups of the sliced mid rounds, the

@@ -103,6 +103,27 @@ def test_sc2_launch_classes(bridge, torch_cuda):
         assert all(c["ms"] > 0 for c in cls.values() if c["launches"])
 
 
+def test_sc2_abort_then_recover(bridge, oracle, torch_cuda):
+    """A device wait that expires (forced with a zero bound) makes the prove
+    fail with an error, every kernel leaving on its own; the next prove on
+    the same state starts from fresh sum slots, ticket and answer slot and
+    matches the oracle."""
+    nv = 20
+    rng = np.random.default_rng(5)
+    f, g = rand_fp2(rng, 1 << nv), rand_fp2(rng, 1 << nv)
+    want = oracle.sc2_prove(f, g, transcript_chal(5), nthreads=8)
+    sc.set_spin_ticks(0)
+    try:
+        with pytest.raises(RuntimeError, match="gave up"):
+            sc.prove_deg2_host(f, g, seed=5)
+    finally:
+        sc.set_spin_ticks(-1)
+    got = sc.prove_deg2_host(f, g, seed=5)
+    assert np.array_equal(got.polys, want[0])
+    assert np.array_equal(got.chals, want[1])
+    assert np.array_equal(got.final, want[2])
+
+
 def test_sc2_mid_gate_fallback(bridge, oracle, torch_cuda):
     """The sliced mid launch needs all its workgroups resident; when its
     arrival gate closes before they are (forced here with a zero bound), the

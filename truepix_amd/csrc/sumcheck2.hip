@@ -1473,6 +1473,16 @@ void tp_sc2_set_gate_ticks(long long ticks)
 // Mid launches of the last tp_sc2_prove that gave way at the arrival gate.
 int tp_sc2_mid_fallbacks(void) { return g_sc2.mid_fallbacks; }
 
+// Bound of every device-side wait of a prove (the tail's and mid launch's
+// answers, the mid relay, the streaming gathers) in wall-clock ticks; a
+// negative value restores the default 10 s.  A small bound makes a prove
+// fail with every kernel leaving on its own, for tests of the error path.
+void tp_sc2_set_spin_ticks(long long ticks)
+{
+    if (!sc2_setup(2)) return;
+    g_sc2.spin_ticks = ticks >= 0 ? ticks : (long long)(g_sc2.clock_mhz * 1000.0) * 1000LL * 10;
+}
+
 // The schedule's sizes: the workgroups of the sliced mid rounds, the
 // largest input (entries per table) the one-workgroup tail takes, and the
 // input the mid rounds hand it.  The streaming launches run while the input

@@ -107,6 +107,7 @@ def bind(lib) -> None:
     _sig(lib, "tp_sc2_mid_profile", [ctypes.POINTER(ctypes.c_double), _I32])
     _sig(lib, "tp_sc2_schedule", [_I32P, _I32P, _I32P])
     _sig(lib, "tp_sc2_set_gate_ticks", [ctypes.c_longlong])
+    _sig(lib, "tp_sc2_set_spin_ticks", [ctypes.c_longlong])
     _sig(lib, "tp_sc2_mid_fallbacks", [], _I32)
     _U8PP = ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8))
     _sig(lib, "tp_pickle_tree", [ctypes.c_char_p, _SIZE, _U8PP, ctypes.POINTER(_SIZE)], _I32)

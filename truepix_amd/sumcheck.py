@@ -151,6 +151,13 @@ def set_gate_ticks(ticks: int) -> None:
     nb.LIB.tp_sc2_set_gate_ticks(int(ticks))
 
 
+def set_spin_ticks(ticks: int) -> None:
+    """Bound of every device-side wait of a prove in wall-clock ticks
+    (negative: the default 10 s); a small bound makes a prove fail with
+    every kernel leaving on its own."""
+    nb.LIB.tp_sc2_set_spin_ticks(int(ticks))
+
+
 def mid_fallbacks() -> int:
     """Mid launches of the last prove that gave way to streaming launches."""
     return int(nb.LIB.tp_sc2_mid_fallbacks())
