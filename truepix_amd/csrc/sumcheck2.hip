@@ -765,6 +765,20 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
         fout[(size_t)b * len + j] = tf[j];
         gout[(size_t)b * len + j] = tg[j];
     }
+    if (b == 0) {
+        // every round done: workgroup 0 has gathered every workgroup's last
+        // sums, so all of them are past the gate and every relay.  It leaves
+        // the words as k_sc2_mid_reset would, and the next mid launch needs
+        // no reset launch of its own.
+        for (int j = lane; j < kMidGroups * 6; j += 64)
+            __hip_atomic_store(parts + j, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) {
+            __hip_atomic_store(relay, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(relay + 1, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(gate, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(gate + 1, kGatePending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
 }
 
 __global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
@@ -929,6 +943,7 @@ struct Sc2State {
     DevBuf slots;                // the streaming launches' tagged sums (ticket_publish)
     DevBuf ticket;               // their ticket counter
     bool slots_clean = false;    // slots all ones and ticket 0 (the last prove ended cleanly)
+    bool mid_clean = false;      // mid words reset by the last mid launch, which finished
     long long gate_ticks = 0;    // its wait bound in wall-clock ticks
     int mid_fallbacks = 0;       // mid launches that did not start whole (last prove)
     long long* host_mtrace = nullptr;   // mapped: [40][kMidGroups][2] stamps (timing on)
@@ -1322,8 +1337,11 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             void* args[] = {&a_f, &a_g, &a_slice, &a_nr, &a_rr, &a_ri, &a_parts, &a_vals, &a_seq,
                             &a_seq0, &a_ans, &a_spin, &a_of, &a_og, &a_relay, &a_trace,
                             &a_gate, &a_gate_ticks};
-            k_sc2_mid_reset<<<1, 256, 0, st>>>(a_parts, a_relay, a_gate);
-            TP_HIP_OK(hipGetLastError());
+            if (!S.mid_clean) {   // first mid launch, or the last one did not finish
+                k_sc2_mid_reset<<<1, 256, 0, st>>>(a_parts, a_relay, a_gate);
+                TP_HIP_OK(hipGetLastError());
+            }
+            S.mid_clean = false;
             if (!t_begin()) return -1;
             TP_HIP_OK(hipLaunchKernel((const void*)k_sc2_mid, dim3(kMidGroups), dim3(kTailThreads),
                                       args, (unsigned)(slice * sizeof(V2)), st));
@@ -1355,6 +1373,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             }
             TP_HIP_OK(hipEventRecord(S.ev_round, st));
             if (!wait_event(S.ev_round) || !t_collect(SC_MID)) return -1;
+            S.mid_clean = true;   // the launch finished every round and reset its words
             if (S.timed && nr > 1 && nr <= 40) {
                 // per round: out = partials written, in = challenge held
                 const volatile long long* tr = S.host_mtrace;
