@@ -425,8 +425,9 @@ __device__ __forceinline__ int take_answer(const u64* answer, unsigned want, lon
     }
 }
 
-// Lane 0 publishes a round's sums under sequence number `want` and waits for
-// the host's answer; returns 0 (and publishes kSeqAbort) if the wait expires.
+// Lane 0 publishes a round's sums tagged with sequence number `want` and
+// waits for the host's answer; returns 0 (and publishes kSeqAbort) if the
+// wait expires.
 __device__ __forceinline__ int tail_exchange(const fp2 (&e)[3], unsigned want, fp2* host_vals,
                                              unsigned* host_seq, u64* answer, long long spin_ticks,
                                              long long* trace, int i, u64 (&rv)[2])
@@ -1019,8 +1020,9 @@ unsigned blocks_for(size_t items)
     return (unsigned)(b ? b : 1);
 }
 
-// Waits until the device has published sequence number `seq`; checks the
-// stream for errors while waiting and gives up after a generous timeout.
+// Waits until a k_publish_* launch (round 0 of tables under 8 entries) has
+// published sequence number `seq`; checks the stream for errors while waiting
+// and gives up after a generous timeout.
 bool wait_seq(hipStream_t st, unsigned seq)
 {
     volatile unsigned* hs = g_sc2.host_seq;
