@@ -199,7 +199,7 @@ BATCHED_CONFIGS = [("stream-1", dict(n_workers=1)), ("stream-8", dict(n_workers=
                    ("lockstep-2x128", dict(n_workers=2, batched=True, max_batch=128))]
 
 
-def bench_batched(n_frames: int = 256, seed: int = 4, configs=None):
+def bench_batched(n_frames: int = 256, seed: int = 4, variants=None):
     """C4: 256 frames, circuit uniform over gray/crop/mask/invert, size uniform
     over 64x64 / 240p / 480p / 720p; all queued at once.  Pools: stream workers
     (W host threads, one HIP stream each, one frame at a time; W = 1 is the
@@ -208,8 +208,8 @@ def bench_batched(n_frames: int = 256, seed: int = 4, configs=None):
     across the batch)."""
     from truepix_amd import frames
     import numpy as np
-    if configs is None:
-        configs = BATCHED_CONFIGS
+    if variants is None:
+        variants = BATCHED_CONFIGS
     fac = _FrameFactory(seed)
     rng = np.random.Generator(np.random.PCG64(seed + 1))
     sizes = ["64x64", "240p", "480p", "720p"]
@@ -219,7 +219,7 @@ def bench_batched(n_frames: int = 256, seed: int = 4, configs=None):
                        % (n_frames, "/".join(CIRCUITS), "/".join(str(FRAME_SIZES[s]) for s in sizes)),
            "total_copies": int(sum(n for _, n in plan)), "by_config": {}}
     ref_proofs = None
-    for label, kw in configs:
+    for label, kw in variants:
         with frames.FramePool(**kw) as pool:
             if kw.get("batched"):
                 # sizes every driver's arena and lets the schedule tuner settle
@@ -265,7 +265,7 @@ def bench_batched(n_frames: int = 256, seed: int = 4, configs=None):
         out["by_config"][label] = rec
         print("bench: batched %s %.1f frames/s" % (label, rec["frames_per_s"]), file=sys.stderr,
               flush=True)
-    base = out["by_config"][configs[0][0]]["frames_per_s"]
+    base = out["by_config"][variants[0][0]]["frames_per_s"]
     best = max(out["by_config"], key=lambda k: out["by_config"][k]["frames_per_s"])
     out["best_config"] = best
     out["frames_per_s"] = out["by_config"][best]["frames_per_s"]
@@ -273,7 +273,7 @@ def bench_batched(n_frames: int = 256, seed: int = 4, configs=None):
     return out
 
 
-def bench_served(n_jobs: int = 2000, rate: float = 200.0, seed: int = 5, configs=None):
+def bench_served(n_jobs: int = 2000, rate: float = 200.0, seed: int = 5, variants=None):
     """C5: Poisson arrivals (rate jobs/s, PCG64(seed) exponential gaps) of
     gray/crop/mask/invert jobs over 240p / 480p / 720p / 1080p frames; each job
     is submitted at its arrival time and admitted as soon as a prover frees up
@@ -283,8 +283,8 @@ def bench_served(n_jobs: int = 2000, rate: float = 200.0, seed: int = 5, configs
     alone on an idle prover."""
     from truepix_amd import frames
     import numpy as np
-    if configs is None:
-        configs = [("stream-8", dict(n_workers=8)),
+    if variants is None:
+        variants = [("stream-8", dict(n_workers=8)),
                    ("lockstep-2x64", dict(n_workers=2, batched=True, max_batch=64))]
     fac = _FrameFactory(seed)
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -314,7 +314,7 @@ def bench_served(n_jobs: int = 2000, rate: float = 200.0, seed: int = 5, configs
            "plain_p50_ms": round(float(np.percentile(plain_lat, 50)) * 1e3, 3),
            "plain_p99_ms": round(float(np.percentile(plain_lat, 99)) * 1e3, 3),
            "by_config": {}}
-    for label, kw in configs:
+    for label, kw in variants:
         with frames.FramePool(**kw) as pool:
             for k in range(16):                                # warm every prover's buffers
                 pool.result(pool.submit(*fac.job("gray", FRAME_SIZES["1080p"], 90000 + k)))
@@ -700,7 +700,7 @@ def main(argv=None):
         if not a.batched_configs:
             return bench_batched()
         want = a.batched_configs.split(",")
-        return bench_batched(configs=[c for c in BATCHED_CONFIGS if c[0] in want])
+        return bench_batched(variants=[c for c in BATCHED_CONFIGS if c[0] in want])
 
     for key, fn in (("gkr_frame", bench_frames_plain), ("gkr_frame_python_driver", bench_gkr_frame),
                     ("batched", batched), ("served", bench_served)):
