@@ -13,10 +13,13 @@
 //          computed as S cosets (u = s + S q2): twist by w_{S M2}^(s j2), then
 //          an M2-point DFT over j2 giving q2;
 //   pass B (per row u):    result[u + S M2 q1] = sum_{j1<M1} Y[u][j1] w_{M1}^(j1 q1).
-// Each pass stages 8 rows of at most 1024 points in LDS (8 x 1025 x 16 B) and
+// Each pass stages 4 rows of at most 1024 points in LDS (4 x 1025 x 16 B) and
 // runs radix-8 Stockham stages (after one radix-2 or radix-4 stage when the
-// length is not a power of 8); the 8-row tiles make every
-// global load and store a 128-byte segment (8 consecutive Fp2).  Columns are
+// length is not a power of 8); the 4-row tiles make every strided global
+// load and store a 64-byte segment (4 consecutive Fp2), and two 512-thread
+// workgroups share a CU, so one loads or stores its tile while the other
+// computes (one 8-row, 1024-thread tile per CU left each phase idle in turn:
+// 18% slower).  Columns are
 // processed a few at a time so the intermediate stays in the 256 MB MALL.
 // Roots of order 4 are +-i in Fp2, so multiplying by w4 is a coordinate swap
 // and one negation; roots of order 8 are 2^30 (+-1 +-i), so multiplying by w8
@@ -35,8 +38,8 @@ struct alignas(16) V2 { u64 re, im; };
 __device__ __forceinline__ fp2 ldv(const V2* p) { const V2 v = *p; return mk2(v.re, v.im); }
 __device__ __forceinline__ void stv(V2* p, fp2 x) { V2 v; v.re = x.re; v.im = x.im; *p = v; }
 
-constexpr int kRows = 8;          // rows per LDS tile
-constexpr int kThreads = 1024;    // threads per NTT workgroup (16 waves: 4 per SIMD)
+constexpr int kRows = 4;          // rows per LDS tile (66 KB: two tiles per CU)
+constexpr int kThreads = 512;     // threads per NTT workgroup (8 waves; 16 a CU)
 constexpr int kMaxLog = 10;       // LDS DFT length up to 2^10
 constexpr int kRowStride = (1 << kMaxLog) + 1;   // +1 element pads LDS banks
 
