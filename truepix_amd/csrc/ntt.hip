@@ -19,7 +19,9 @@
 // load and store a 64-byte segment (4 consecutive Fp2), and two 512-thread
 // workgroups share a CU, so one loads or stores its tile while the other
 // computes (one 8-row, 1024-thread tile per CU left each phase idle in turn:
-// 18% slower).  Columns are
+// 18% slower).  Pass B's first DFT stage reads its rows from HBM and its last
+// stage writes the results there, so only the stages in between go through
+// LDS (5% faster than staging both ends).  Columns are
 // processed a few at a time so the intermediate stays in the 256 MB MALL.
 // Roots of order 4 are +-i in Fp2, so multiplying by w4 is a coordinate swap
 // and one negation; roots of order 8 are 2^30 (+-1 +-i), so multiplying by w8
@@ -78,16 +80,32 @@ __device__ __forceinline__ fp2 mul_w8(fp2 v, int s1, int s2)
     return mk2(rot30(re), rot30(im));
 }
 
+// Where a DFT's first stage reads and its last stage writes when they bypass
+// LDS: row r, element i at in[r 2^l + i] and out[r + i out_step].
+struct GlobalIO {
+    const V2* in;
+    V2* out;
+    size_t out_step;
+};
+
 // Forward DFT of `rows` rows of length 2^l held in LDS (row stride
 // kRowStride), Stockham auto-sort: stage with span Ns reads x[j + i L/R],
 // multiplies by w_{Ns R}^((j mod Ns) i), applies DFT_R and writes
 // y[(j / Ns) Ns R + (j mod Ns) + i Ns].  Values pass through registers with a
 // barrier between the read and write halves, so one LDS buffer suffices.
-__device__ void lds_dft(fp2* lds, int rows, int l, const Tw& tw)
+// kIn: the first stage reads the rows from io.in instead of LDS.  kOut (needs
+// l >= 3, so the last stage is radix 8): the last stage writes io.out instead
+// of LDS, with the row on the fastest lane bits so that `rows` consecutive
+// lanes store `rows` consecutive words; nothing is left in LDS.
+template <bool kIn, bool kOut>
+__device__ void lds_dft(fp2* lds, int rows, int log_rows, int l, const Tw& tw, GlobalIO io)
 {
     const int L = 1 << l;
     int Ns = 1;
     int log_ns = 0;
+    auto rd = [&](int row, int i) -> fp2 {
+        return kIn ? ldv(io.in + ((size_t)row << l) + i) : lds[row * kRowStride + i];
+    };
     // l = 3 q + rem: one radix-2 (rem 1) or radix-4 (rem 2) stage, then radix 8
     if (l % 3 == 1) {   // radix-2 stage (twiddles of a span-1 stage are all 1)
         const int nb = L >> 1, total = rows * nb;
@@ -98,9 +116,8 @@ __device__ void lds_dft(fp2* lds, int rows, int l, const Tw& tw)
             const int b = threadIdx.x + it * kThreads;
             if (b < total) {
                 const int row = b >> (l - 1), j = b & (nb - 1);
-                const fp2* x = lds + row * kRowStride;
-                v[it][0] = x[j];
-                v[it][1] = x[j + nb];
+                v[it][0] = rd(row, j);
+                v[it][1] = rd(row, j + nb);
             }
         }
         __syncthreads();
@@ -127,8 +144,8 @@ __device__ void lds_dft(fp2* lds, int rows, int l, const Tw& tw)
             const int b = threadIdx.x + it * kThreads;
             if (b < total) {
                 const int row = b >> lnb, j = b & (nb - 1);
-                const fp2* x = lds + row * kRowStride;
-                const fp2 a0 = x[j], a1 = x[j + nb], a2 = x[j + 2 * nb], a3 = x[j + 3 * nb];
+                const fp2 a0 = rd(row, j), a1 = rd(row, j + nb), a2 = rd(row, j + 2 * nb),
+                          a3 = rd(row, j + 3 * nb);
                 const fp2 s02 = add2(a0, a2), d02 = sub2(a0, a2);
                 const fp2 s13 = add2(a1, a3), d13 = mul_w4(sub2(a1, a3), tw.w4_is_i);
                 v[it][0] = add2(s02, s13);
@@ -159,16 +176,25 @@ __device__ void lds_dft(fp2* lds, int rows, int l, const Tw& tw)
         const int lnb = l - 3;
         const int tw_shift = kMaxLog - (log_ns + 3);   // w_{8 Ns}^t = w_1024^(t << shift)
         constexpr int IT = (kRows << (kMaxLog - 3)) / kThreads;
+        // the last stage (span L/8, so j < Ns and the output index is j + m Ns)
+        // stores straight to io.out when kOut
+        const bool last = kOut && (Ns << 3) == L;
         fp2 v[IT][8];
 #pragma unroll
         for (int it = 0; it < IT; ++it) {
             const int b = threadIdx.x + it * kThreads;
             if (b < total) {
-                const int row = b >> lnb, j = b & (nb - 1);
-                const fp2* x = lds + row * kRowStride;
+                const int row = last ? b & (rows - 1) : b >> lnb;
+                const int j = last ? b >> log_rows : b & (nb - 1);
                 fp2 a[8];
+                if (kIn && Ns == 1) {
 #pragma unroll
-                for (int m = 0; m < 8; ++m) a[m] = x[j + m * nb];
+                    for (int m = 0; m < 8; ++m) a[m] = rd(row, j + m * nb);
+                } else {
+                    const fp2* x = lds + row * kRowStride;
+#pragma unroll
+                    for (int m = 0; m < 8; ++m) a[m] = x[j + m * nb];
+                }
                 const int k = j & (Ns - 1);
                 if (k) {
 #pragma unroll
@@ -198,6 +224,19 @@ __device__ void lds_dft(fp2* lds, int rows, int l, const Tw& tw)
                     v[it][m + 4] = sub2(E[m], O[m]);
                 }
             }
+        }
+        if (last) {
+#pragma unroll
+            for (int it = 0; it < IT; ++it) {
+                const int b = threadIdx.x + it * kThreads;
+                if (b < total) {
+                    const int row = b & (rows - 1), j = b >> log_rows;
+                    V2* y = io.out + row + io.out_step * (size_t)j;
+#pragma unroll
+                    for (int m = 0; m < 8; ++m) stv(y + io.out_step * (size_t)(m * Ns), v[it][m]);
+                }
+            }
+            return;
         }
         __syncthreads();
 #pragma unroll
@@ -251,7 +290,7 @@ __global__ void __launch_bounds__(kThreads) k_ntt_pass_a(const V2* __restrict__ 
         }
     }
     __syncthreads();
-    lds_dft(lds, rows, b, tw);
+    lds_dft<false, false>(lds, rows, log_rows, b, tw, GlobalIO{nullptr, nullptr, 0});
     // twiddle w_N^(j1 u), u = s + S q2; store Y[u][j1]
     {
         const int r = threadIdx.x & (rows - 1);
@@ -284,13 +323,17 @@ __global__ void __launch_bounds__(kThreads) k_ntt_pass_b(const V2* __restrict__ 
     const size_t u0 = (size_t)blockIdx.x * rows;
     const V2* src = ybuf + blockIdx.y * y_stride;
     V2* dst = out + blockIdx.y * out_stride;
+    if (a >= 3) {   // the DFT's first stage loads the rows, its last stores them
+        lds_dft<true, true>(lds, rows, log_rows, a, tw, GlobalIO{src + u0 * M1, dst + u0, SM2});
+        return;
+    }
     const int total = rows * M1;
     for (int idx = threadIdx.x; idx < total; idx += kThreads) {
         const int r = idx >> a, j1 = idx & (M1 - 1);
         lds[r * kRowStride + j1] = ldv(src + (u0 + r) * M1 + j1);
     }
     __syncthreads();
-    lds_dft(lds, rows, a, tw);
+    lds_dft<false, false>(lds, rows, log_rows, a, tw, GlobalIO{nullptr, nullptr, 0});
     for (int idx = threadIdx.x; idx < total; idx += kThreads) {
         const int r = idx & (rows - 1), q1 = idx >> log_rows;
         stv(dst + (u0 + r) + SM2 * q1, lds[r * kRowStride + q1]);
