@@ -19,9 +19,10 @@
 // load and store a 64-byte segment (4 consecutive Fp2), and two 512-thread
 // workgroups share a CU, so one loads or stores its tile while the other
 // computes (one 8-row, 1024-thread tile per CU left each phase idle in turn:
-// 18% slower).  Pass B's first DFT stage reads its rows from HBM and its last
-// stage writes the results there, so only the stages in between go through
-// LDS (5% faster than staging both ends).  Columns are
+// 18% slower).  In both passes the DFT's first stage reads its rows from HBM
+// (pass A twisting them as it loads) and its last stage writes the results
+// there (pass A multiplying by w^(j1 u) as it stores), so only the stages in
+// between go through LDS (10% faster than staging both ends).  Columns are
 // processed a few at a time so the intermediate stays in the 256 MB MALL.
 // Roots of order 4 are +-i in Fp2, so multiplying by w4 is a coordinate swap
 // and one negation; roots of order 8 are 2^30 (+-1 +-i), so multiplying by w8
@@ -80,44 +81,92 @@ __device__ __forceinline__ fp2 mul_w8(fp2 v, int s1, int s2)
     return mk2(rot30(re), rot30(im));
 }
 
-// Where a DFT's first stage reads and its last stage writes when they bypass
-// LDS: row r, element i at in[r 2^l + i] and out[r + i out_step].
+// Where a DFT's first stage reads its rows and its last stage writes them.
+enum InMode {
+    kInLds,    // LDS
+    kInRows,   // row r, element i at in[r 2^l + i]
+    kInCols,   // row r, element i at in[r + i in_step], times w^(in_tw i)
+};
+enum OutMode {
+    kOutLds,      // LDS
+    kOutCols,     // row r, element i to out[r + i out_step]
+    kOutColsTw,   // the same, times w^((r0 + r)(out_e0 + out_e1 i))
+};
 struct GlobalIO {
     const V2* in;
+    size_t in_step;
+    u64 in_tw;
     V2* out;
     size_t out_step;
+    u64 r0, out_e0, out_e1;
+    u64 nmask;   // exponents are taken mod the order of w (a power of 2)
 };
 
-// Forward DFT of `rows` rows of length 2^l held in LDS (row stride
-// kRowStride), Stockham auto-sort: stage with span Ns reads x[j + i L/R],
-// multiplies by w_{Ns R}^((j mod Ns) i), applies DFT_R and writes
-// y[(j / Ns) Ns R + (j mod Ns) + i Ns].  Values pass through registers with a
+// Forward DFT of `rows` rows of length 2^l, Stockham auto-sort: stage with
+// span Ns reads x[j + i L/R], multiplies by w_{Ns R}^((j mod Ns) i), applies
+// DFT_R and writes y[(j / Ns) Ns R + (j mod Ns) + i Ns].  Between stages the
+// rows sit in LDS (row stride kRowStride); values pass through registers with a
 // barrier between the read and write halves, so one LDS buffer suffices.
-// kIn: the first stage reads the rows from io.in instead of LDS.  kOut (needs
-// l >= 3, so the last stage is radix 8): the last stage writes io.out instead
-// of LDS, with the row on the fastest lane bits so that `rows` consecutive
-// lanes store `rows` consecutive words; nothing is left in LDS.
-template <bool kIn, bool kOut>
-__device__ void lds_dft(fp2* lds, int rows, int log_rows, int l, const Tw& tw, GlobalIO io)
+// The first stage reads as kIn says, the last writes as kOut says (anything
+// but kOutLds needs l >= 3, so the last stage is radix 8; then nothing is left
+// in LDS).  Stages that touch rows interleaved in memory (kInCols, and the
+// last stage when it stores) put the row on the fastest lane bits, so `rows`
+// consecutive lanes move `rows` consecutive words.
+template <int kIn, int kOut>
+__device__ void lds_dft(fp2* lds, int rows, int log_rows, int l, const Tw& tw, const GlobalIO& io)
 {
     const int L = 1 << l;
     int Ns = 1;
     int log_ns = 0;
-    auto rd = [&](int row, int i) -> fp2 {
-        return kIn ? ldv(io.in + ((size_t)row << l) + i) : lds[row * kRowStride + i];
+    auto at = [&](int b, int lnb, int nb, bool row_fast, int& row, int& j) {
+        if (row_fast) {
+            row = b & (rows - 1);
+            j = b >> log_rows;
+        } else {
+            row = b >> lnb;
+            j = b & (nb - 1);
+        }
     };
+    // the first stage's R inputs x[j + m nb] of one row (twisted for kInCols:
+    // w^(in_tw j) times ratio^m, ratio = w^(in_tw nb))
+    auto first = [&](auto& a, int row, int j, int nb, fp2 ratio) {
+        constexpr int R = sizeof(a) / sizeof(a[0]);
+        if (kIn == kInLds) {
+#pragma unroll
+            for (int m = 0; m < R; ++m) a[m] = lds[row * kRowStride + j + m * nb];
+        } else if (kIn == kInRows) {
+#pragma unroll
+            for (int m = 0; m < R; ++m) a[m] = ldv(io.in + ((size_t)row << l) + j + m * nb);
+        } else {
+#pragma unroll
+            for (int m = 0; m < R; ++m) a[m] = ldv(io.in + row + io.in_step * (size_t)(j + m * nb));
+            if (io.in_tw) {
+                fp2 t = wpow(tw, (io.in_tw * (u64)j) & io.nmask);
+#pragma unroll
+                for (int m = 0; m < R; ++m) {
+                    a[m] = mul2(a[m], t);
+                    if (m + 1 < R) t = mul2(t, ratio);
+                }
+            }
+        }
+    };
+    auto in_ratio = [&](int nb) {
+        return (kIn == kInCols && io.in_tw) ? wpow(tw, (io.in_tw * (u64)nb) & io.nmask) : one2();
+    };
+    constexpr bool kColsIn = kIn == kInCols;
     // l = 3 q + rem: one radix-2 (rem 1) or radix-4 (rem 2) stage, then radix 8
     if (l % 3 == 1) {   // radix-2 stage (twiddles of a span-1 stage are all 1)
         const int nb = L >> 1, total = rows * nb;
         constexpr int IT = (kRows << (kMaxLog - 1)) / kThreads;
+        const fp2 ratio = in_ratio(nb);
         fp2 v[IT][2];
 #pragma unroll
         for (int it = 0; it < IT; ++it) {
             const int b = threadIdx.x + it * kThreads;
             if (b < total) {
-                const int row = b >> (l - 1), j = b & (nb - 1);
-                v[it][0] = rd(row, j);
-                v[it][1] = rd(row, j + nb);
+                int row, j;
+                at(b, l - 1, nb, kColsIn, row, j);
+                first(v[it], row, j, nb, ratio);
             }
         }
         __syncthreads();
@@ -125,7 +174,8 @@ __device__ void lds_dft(fp2* lds, int rows, int log_rows, int l, const Tw& tw, G
         for (int it = 0; it < IT; ++it) {
             const int b = threadIdx.x + it * kThreads;
             if (b < total) {
-                const int row = b >> (l - 1), j = b & (nb - 1);
+                int row, j;
+                at(b, l - 1, nb, kColsIn, row, j);
                 fp2* y = lds + row * kRowStride;
                 y[2 * j] = add2(v[it][0], v[it][1]);
                 y[2 * j + 1] = sub2(v[it][0], v[it][1]);
@@ -138,16 +188,18 @@ __device__ void lds_dft(fp2* lds, int rows, int log_rows, int l, const Tw& tw, G
         const int nb = L >> 2, total = rows * nb;
         const int lnb = l - 2;
         constexpr int IT = (kRows << (kMaxLog - 2)) / kThreads;
+        const fp2 ratio = in_ratio(nb);
         fp2 v[IT][4];
 #pragma unroll
         for (int it = 0; it < IT; ++it) {
             const int b = threadIdx.x + it * kThreads;
             if (b < total) {
-                const int row = b >> lnb, j = b & (nb - 1);
-                const fp2 a0 = rd(row, j), a1 = rd(row, j + nb), a2 = rd(row, j + 2 * nb),
-                          a3 = rd(row, j + 3 * nb);
-                const fp2 s02 = add2(a0, a2), d02 = sub2(a0, a2);
-                const fp2 s13 = add2(a1, a3), d13 = mul_w4(sub2(a1, a3), tw.w4_is_i);
+                int row, j;
+                at(b, lnb, nb, kColsIn, row, j);
+                fp2 a[4];
+                first(a, row, j, nb, ratio);
+                const fp2 s02 = add2(a[0], a[2]), d02 = sub2(a[0], a[2]);
+                const fp2 s13 = add2(a[1], a[3]), d13 = mul_w4(sub2(a[1], a[3]), tw.w4_is_i);
                 v[it][0] = add2(s02, s13);
                 v[it][1] = add2(d02, d13);
                 v[it][2] = sub2(s02, s13);
@@ -159,7 +211,8 @@ __device__ void lds_dft(fp2* lds, int rows, int log_rows, int l, const Tw& tw, G
         for (int it = 0; it < IT; ++it) {
             const int b = threadIdx.x + it * kThreads;
             if (b < total) {
-                const int row = b >> lnb, j = b & (nb - 1);
+                int row, j;
+                at(b, lnb, nb, kColsIn, row, j);
                 fp2* y = lds + row * kRowStride + 4 * j;
                 y[0] = v[it][0];
                 y[1] = v[it][1];
@@ -177,19 +230,21 @@ __device__ void lds_dft(fp2* lds, int rows, int log_rows, int l, const Tw& tw, G
         const int tw_shift = kMaxLog - (log_ns + 3);   // w_{8 Ns}^t = w_1024^(t << shift)
         constexpr int IT = (kRows << (kMaxLog - 3)) / kThreads;
         // the last stage (span L/8, so j < Ns and the output index is j + m Ns)
-        // stores straight to io.out when kOut
-        const bool last = kOut && (Ns << 3) == L;
+        // stores to io.out unless kOut is kOutLds
+        const bool last = kOut != kOutLds && (Ns << 3) == L;
+        const bool from_in = kIn != kInLds && Ns == 1;
+        const bool row_fast = last || (kColsIn && Ns == 1);
+        const fp2 ratio = from_in ? in_ratio(nb) : one2();
         fp2 v[IT][8];
 #pragma unroll
         for (int it = 0; it < IT; ++it) {
             const int b = threadIdx.x + it * kThreads;
             if (b < total) {
-                const int row = last ? b & (rows - 1) : b >> lnb;
-                const int j = last ? b >> log_rows : b & (nb - 1);
+                int row, j;
+                at(b, lnb, nb, row_fast, row, j);
                 fp2 a[8];
-                if (kIn && Ns == 1) {
-#pragma unroll
-                    for (int m = 0; m < 8; ++m) a[m] = rd(row, j + m * nb);
+                if (from_in) {
+                    first(a, row, j, nb, ratio);
                 } else {
                     const fp2* x = lds + row * kRowStride;
 #pragma unroll
@@ -232,8 +287,20 @@ __device__ void lds_dft(fp2* lds, int rows, int log_rows, int l, const Tw& tw, G
                 if (b < total) {
                     const int row = b & (rows - 1), j = b >> log_rows;
                     V2* y = io.out + row + io.out_step * (size_t)j;
+                    const u64 r = io.r0 + (u64)row;
+                    if (kOut == kOutColsTw && r) {
+                        // w^(r (e0 + e1 (j + m Ns))) = w^(r (e0 + e1 j)) (w^(r e1 Ns))^m
+                        fp2 t = wpow(tw, (r * (io.out_e0 + io.out_e1 * (u64)j)) & io.nmask);
+                        const fp2 step = wpow(tw, (r * io.out_e1 * (u64)Ns) & io.nmask);
 #pragma unroll
-                    for (int m = 0; m < 8; ++m) stv(y + io.out_step * (size_t)(m * Ns), v[it][m]);
+                        for (int m = 0; m < 8; ++m) {
+                            stv(y + io.out_step * (size_t)(m * Ns), mul2(v[it][m], t));
+                            if (m < 7) t = mul2(t, step);
+                        }
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < 8; ++m) stv(y + io.out_step * (size_t)(m * Ns), v[it][m]);
+                    }
                 }
             }
             return;
@@ -243,7 +310,8 @@ __device__ void lds_dft(fp2* lds, int rows, int log_rows, int l, const Tw& tw, G
         for (int it = 0; it < IT; ++it) {
             const int b = threadIdx.x + it * kThreads;
             if (b < total) {
-                const int row = b >> lnb, j = b & (nb - 1);
+                int row, j;
+                at(b, lnb, nb, row_fast, row, j);
                 const int k = j & (Ns - 1);
                 fp2* y = lds + row * kRowStride + ((j >> log_ns) << (log_ns + 3)) + k;
 #pragma unroll
@@ -270,6 +338,23 @@ __global__ void __launch_bounds__(kThreads) k_ntt_pass_a(const V2* __restrict__ 
     V2* dst = ybuf + blockIdx.z * y_stride;
     const int total = rows * M2;
     const u64 nmask = ((u64)1 << (a + b + log_s)) - 1;   // exponents mod N (w^N = 1)
+    if (b >= 3) {
+        // the DFT's first stage loads the rows (x[j1_0 + r + M1 j2] twisted by
+        // w_N^(M1 s j2)) and its last stage stores Y[u][j1] = v w_N^(j1 u),
+        // u = s + S q2, at ybuf[u M1 + j1]
+        GlobalIO io{};
+        io.in = src + j1_0;
+        io.in_step = (size_t)M1;
+        io.in_tw = ((u64)M1 * s) & nmask;
+        io.out = dst + (size_t)s * M1 + j1_0;
+        io.out_step = (size_t)S * M1;
+        io.r0 = (u64)j1_0;
+        io.out_e0 = (u64)s;
+        io.out_e1 = (u64)S;
+        io.nmask = nmask;
+        lds_dft<kInCols, kOutColsTw>(lds, rows, log_rows, b, tw, io);
+        return;
+    }
     // Each thread keeps one row r (kThreads is a multiple of rows) and steps
     // j2 (and q2) by kThreads / rows, so both twiddles advance by a constant
     // ratio: one multiply per element instead of two table loads and a multiply.
@@ -290,7 +375,7 @@ __global__ void __launch_bounds__(kThreads) k_ntt_pass_a(const V2* __restrict__ 
         }
     }
     __syncthreads();
-    lds_dft<false, false>(lds, rows, log_rows, b, tw, GlobalIO{nullptr, nullptr, 0});
+    lds_dft<kInLds, kOutLds>(lds, rows, log_rows, b, tw, GlobalIO{});
     // twiddle w_N^(j1 u), u = s + S q2; store Y[u][j1]
     {
         const int r = threadIdx.x & (rows - 1);
@@ -324,7 +409,11 @@ __global__ void __launch_bounds__(kThreads) k_ntt_pass_b(const V2* __restrict__ 
     const V2* src = ybuf + blockIdx.y * y_stride;
     V2* dst = out + blockIdx.y * out_stride;
     if (a >= 3) {   // the DFT's first stage loads the rows, its last stores them
-        lds_dft<true, true>(lds, rows, log_rows, a, tw, GlobalIO{src + u0 * M1, dst + u0, SM2});
+        GlobalIO io{};
+        io.in = src + u0 * M1;
+        io.out = dst + u0;
+        io.out_step = SM2;
+        lds_dft<kInRows, kOutCols>(lds, rows, log_rows, a, tw, io);
         return;
     }
     const int total = rows * M1;
@@ -333,7 +422,7 @@ __global__ void __launch_bounds__(kThreads) k_ntt_pass_b(const V2* __restrict__ 
         lds[r * kRowStride + j1] = ldv(src + (u0 + r) * M1 + j1);
     }
     __syncthreads();
-    lds_dft<false, false>(lds, rows, log_rows, a, tw, GlobalIO{nullptr, nullptr, 0});
+    lds_dft<kInLds, kOutLds>(lds, rows, log_rows, a, tw, GlobalIO{});
     for (int idx = threadIdx.x; idx < total; idx += kThreads) {
         const int r = idx & (rows - 1), q1 = idx >> log_rows;
         stv(dst + (u0 + r) + SM2 * q1, lds[r * kRowStride + q1]);
