@@ -13,7 +13,9 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("log_coef,log_order,cols", [
     (0, 0, 1), (0, 3, 2), (1, 1, 1), (2, 4, 3), (3, 3, 1), (5, 7, 2), (7, 7, 1),
-    (8, 10, 2), (10, 12, 2), (11, 13, 1), (12, 12, 1), (14, 16, 3), (17, 17, 1)])
+    (8, 10, 2), (10, 12, 2), (11, 13, 1), (12, 12, 1), (14, 16, 3), (17, 17, 1),
+    # rates 1/16 and 1/64: 16 and 64 cosets twisted in pass A's first stage
+    (10, 14, 2), (9, 15, 1)])
 def test_rs_encode_vs_oracle(bridge, oracle, torch_cuda, log_coef, log_order, cols):
     rng = np.random.default_rng(log_coef * 100 + log_order)
     coef = rand_fp2(rng, cols << log_coef)
