@@ -327,13 +327,26 @@ __device__ void lds_dft(fp2* lds, int rows, int log_rows, int l, const Tw& tw, c
 // Pass A.  grid = (M1 / rows, S, columns); rows consecutive j1 per tile.
 __global__ void __launch_bounds__(kThreads) k_ntt_pass_a(const V2* __restrict__ coef, size_t coef_stride,
                                                          V2* __restrict__ ybuf, size_t y_stride,
-                                                         int a, int b, int log_s, int log_rows, Tw tw)
+                                                         int a, int b, int log_s, int log_rows, Tw tw,
+                                                         int xcd_order)
 {
     extern __shared__ fp2 lds[];
     const int M1 = 1 << a, M2 = 1 << b, S = 1 << log_s;
     const int rows = 1 << log_rows;
-    const int s = blockIdx.y;
-    const int j1_0 = blockIdx.x * rows;
+    int s, j1_0;
+    if (xcd_order) {
+        // one grid dimension of tiles x cosets, workgroup ids dealt round-robin
+        // to the 8 XCDs: XCD x runs, in turn, the S cosets of tile 2(8q + x)
+        // and then of tile 2(8q + x) + 1, whose 64-byte segments share
+        // 128-byte lines, so those lines stay in that XCD's L2 (tiles % 16 == 0)
+        const int x = blockIdx.x & 7, k = blockIdx.x >> 3;
+        s = k & (S - 1);
+        const int h = (k >> log_s) & 1, q = k >> (log_s + 1);
+        j1_0 = (2 * (q * 8 + x) + h) * rows;
+    } else {
+        s = blockIdx.y;
+        j1_0 = blockIdx.x * rows;
+    }
     const V2* src = coef + blockIdx.z * coef_stride;
     V2* dst = ybuf + blockIdx.z * y_stride;
     const int total = rows * M2;
@@ -554,10 +567,13 @@ int tp_rs_encode(const uint64_t* d_coef, int n_cols, int log_coef, int log_order
     V2* out = reinterpret_cast<V2*>(d_out);
     for (int c0 = 0; c0 < n_cols; c0 += chunk) {
         const int nc = (n_cols - c0) < chunk ? (n_cols - c0) : chunk;
-        dim3 ga((unsigned)(M1 / rows_a), 1u << log_s, (unsigned)nc);
+        const unsigned tiles = (unsigned)(M1 / rows_a);
+        const int xcd_order = tiles % 16 == 0;
+        dim3 ga = xcd_order ? dim3(tiles << log_s, 1u, (unsigned)nc)
+                            : dim3(tiles, 1u << log_s, (unsigned)nc);
         k_ntt_pass_a<<<ga, kThreads, lds_bytes(rows_a), st>>>(coef + (size_t)c0 * M, M,
                                                                P_.ybuf.as<V2>(), N, a, b, log_s,
-                                                               lra, tw);
+                                                               lra, tw, xcd_order);
         TP_HIP_OK(hipGetLastError());
         dim3 gb((unsigned)(SM2 / rows_b), (unsigned)nc);
         k_ntt_pass_b<<<gb, kThreads, lds_bytes(rows_b), st>>>(P_.ybuf.as<V2>(), N,
