@@ -167,7 +167,10 @@ void tp_sc2_mid_profile(double* out, int n);
  * k at k*2^log_coef) -> 2^log_order natural-order evaluations per column
  * (column k at k*2^log_order): out[k][i] = sum_j coef[k][j] w^(i j).
  * cols_per_chunk: columns per pass pair (0 = 2).  log_coef <= 20,
- * log_coef <= log_order <= 30.  stream: hipStream_t or NULL.  0 / -1. */
+ * log_coef <= log_order <= 30.  stream: hipStream_t or NULL.  0 / -1.
+ * One set of twiddle tables and one intermediate serve every call: calls are
+ * serialized on the host, and a call on another stream than the previous
+ * call's waits for that stream before it enqueues its work. */
 int tp_rs_encode(const uint64_t* d_coef, int n_cols, int log_coef, int log_order,
                  uint64_t* d_out, int cols_per_chunk, void* stream);
 
@@ -230,7 +233,10 @@ int tp_mle_eval_base_dev(const uint64_t* d_inputs, int n_z, const uint64_t* z, u
  * gate the evaluation op and mux bit (as tp_eval_layer), the sumcheck type
  * index, in0 / in1; input_slot[w] (w < n_inputs) = the private input feeding
  * input wire w, or -1 for a constant wire whose value is input_const[w].
- * NULL on a bad shape.  Circuits are shared by any number of jobs. */
+ * NULL on a bad shape.  Circuits are shared by any number of jobs: each
+ * queued job holds a reference, so tp_circuit_destroy while jobs still use the
+ * circuit drops only the creator's reference and the last of those jobs frees
+ * it (when tp_pool_release or tp_pool_destroy forgets the job). */
 void* tp_circuit_create(int n_gates, const int* op, const int* muxbit, const int* type_idx,
                         const int* in0, const int* in1, int n_inputs, const int* input_slot,
                         const uint64_t* input_const);

@@ -62,3 +62,54 @@ def test_no_host_fallback_without_device():
     assert lib.native_fp2_cuda_available() == 0
     assert lib.native_fp2_init() == -1
     assert lib.native_fp2_early_claim(None, (ctypes.c_uint64 * 8)()) == -1
+
+
+def _integration_snippet():
+    """The `_load()` replacement block of INTEGRATION.md, as a maintainer would
+    paste it into TP/libTruePix/native_fp2_bridge.py."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    start = text.index("# TP/libTruePix/native_fp2_bridge.py, _load() replacement")
+    return text[start:text.index("```", start)]
+
+
+def test_integration_binding_raises_on_device_errors(monkeypatch):
+    """Runs the documented binding against the library: with no device, the
+    void entry points the reference's callers never check raise instead of
+    leaving their outputs unwritten; a call that needs no device does not."""
+    import numpy as np
+    from truepix_amd import native_fp2_bridge as nb
+    monkeypatch.setenv("TRUEPIX_HIP_LIB", LIB)
+    ns = {"os": os, "ctypes": ctypes, "_LIB": None, "_GPU_LIB": None, "AVAILABLE": False,
+          "GPU_ACTIVE": False, "GPU_FAIL_REASON": "",
+          # the reference's own binders set the CUDA library's argtypes; the
+          # drop-in symbols have the same signatures here
+          "_bind_batch": nb.bind, "_bind_early": lambda lib: None}
+    exec(compile(_integration_snippet(), "INTEGRATION.md", "exec"), ns)
+    ns["_load"]()
+    lib = ns["_HIP"]
+    assert lib is not None
+    u = lambda n: np.zeros(n, np.uint64)
+    p = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+    ir, ii, outr, outi = u(8), u(8), u(4), u(4)
+    lib.native_fp2_fold(p(ir), p(ii), 0, 1, 0, p(outr), p(outi))     # nothing to do: no error
+    z = u(41)
+    with pytest.raises(RuntimeError, match="n_z=41 too large"):
+        lib.native_fp2_compute_beta(p(z), p(z), 41, 1, 0, p(u(2)), p(u(2)))
+    if ns["AVAILABLE"]:
+        lib.native_fp2_fold(p(ir), p(ii), 4, 1, 0, p(outr), p(outi))
+        return
+    assert "HIP device unavailable" in ns["GPU_FAIL_REASON"] or "no HIP device" in ns["GPU_FAIL_REASON"]
+    calls = [
+        lambda: lib.native_fp2_fold(p(ir), p(ii), 4, 1, 0, p(outr), p(outi)),
+        lambda: lib.native_fp2_expand(p(ir), p(ii), 4, 2, p(u(8)), p(u(8))),
+        lambda: lib.native_fp2_compute_beta(p(z), p(z), 3, 1, 0, p(u(8)), p(u(8))),
+        lambda: lib.native_fp2_mle_eval_base(p(u(8)), p(z), p(z), 3, p(u(1)), p(u(1))),
+    ]
+    for call in calls:
+        with pytest.raises(RuntimeError, match="HIP device unavailable"):
+            call()
+    g = np.zeros(1, np.int32)
+    gi = g.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    with pytest.raises(RuntimeError, match="HIP device unavailable"):
+        lib.native_fp2_sumcheck_early(1, gi, gi, gi, p(u(1)), p(u(1)), 2, 1, *([p(u(2))] * 12),
+                                      p(u(8)))

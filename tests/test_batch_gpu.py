@@ -24,9 +24,11 @@ NAMES = ("gray", "crop", "mask", "invert")
 
 
 def golden():
+    """The golden frames of the four edit circuits (the multi-layer cases are
+    the Python-driven prover's)."""
     with open(os.path.join(GOLD, "golden_proofs.json")) as fh:
         idx = json.load(fh)
-    return idx["pid"], idx["cases"]
+    return idx["pid"], [e for e in idx["cases"] if e["circuit"] in NAMES]
 
 
 def check_gold(entry, res):
@@ -87,6 +89,36 @@ def test_batch_matches_reference_golden_proofs(circs):
     for e, r in res:
         check_gold(e, r)
     assert all(r.batch == len(specs) for _, r in res)     # one launch set
+
+
+def test_c4_batch_of_256_with_golden_frames_embedded(circs):
+    """A C4-sized lockstep batch: 256 frames of the four circuits at 64x64,
+    240p, 480p and 720p (16, 300, 1350, 3600 copies), with every golden frame
+    (5 to 8040 copies) among them.  Each golden frame's proof is the
+    reference's; every other frame's equals the same frame proved alone."""
+    gold = gold_specs(circs)
+    rng = np.random.Generator(np.random.PCG64(21))
+    sizes = [int(x) for x in rng.choice([16, 300, 1350, 3600], size=256 - len(gold))]
+    synth = synth_specs(circs, sizes, seed=22)
+    order = rng.permutation(256)
+    allspecs = [("gold", e, sp) for e, sp in gold] + [("synth", k, sp) for k, sp in enumerate(synth)]
+    allspecs = [allspecs[i] for i in order]
+    with frames.FramePool(4) as solo:
+        ids = [solo.submit(*sp) for kind, _tag, sp in allspecs if kind == "synth"]
+        want = iter([solo.result(j).proof for j in ids])
+    print("solo done", flush=True)
+    with frames.FramePool(1, batched=True, max_batch=256, host_threads=16) as pool:
+        ids = pool.submit_many([sp for _, _, sp in allspecs])
+        print("c4 batch submitted", len(ids), flush=True)
+        got = [pool.result(j) for j in ids]
+    n_gold = 0
+    for (kind, tag, _), r in zip(allspecs, got):
+        if kind == "gold":
+            check_gold(tag, r)
+            n_gold += 1
+        else:
+            assert r.proof == next(want), tag
+    assert n_gold == len(gold) and max(r.batch for r in got) == 256
 
 
 def test_batch_equals_stream_worker_on_mixed_shapes(circs):

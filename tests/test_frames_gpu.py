@@ -22,9 +22,10 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def cases():
+    """The golden frames of the four edit circuits."""
     with open(os.path.join(GOLD, "golden_proofs.json")) as fh:
         idx = json.load(fh)
-    return idx["pid"], idx["cases"]
+    return idx["pid"], [e for e in idx["cases"] if e["circuit"] in circuits.CIRCUITS]
 
 
 def gold_inputs(entry):
@@ -43,7 +44,8 @@ def check_against_gold(entry, proof, point, value):
     assert (value.real, value.imag) == tuple(entry["input_value"]), entry["name"]
 
 
-@pytest.mark.parametrize("name", ["gray16", "inv8", "crop12", "mask5", "gray1350"])
+@pytest.mark.parametrize("name", ["gray16", "inv8", "crop12", "mask5", "gray1350", "gray3600",
+                                  "inv3600", "crop8040", "mask8040"])
 def test_frame_prover_matches_reference(bridge, tmp_path, name):
     pid, entries = cases()
     entry = next(e for e in entries if e["name"] == name)
@@ -89,6 +91,35 @@ def test_frames_in_a_mix_equal_their_solo_proofs(bridge):
             assert res.t_submit <= res.t_start <= res.t_end
     for c in circs.values():
         c.close()
+
+
+def test_pool_keeps_a_dropped_circuit_alive(bridge):
+    """A frame queued with a circuit object the caller drops at once (and a
+    circuit closed while its frames are queued) still proves with the
+    reference's bytes: the pool and the library hold the circuit until the
+    job is released."""
+    import gc
+    pid, entries = cases()
+    vole = frames.vole_prover_arrays(os.path.join(GOLD, "vole_ext_prover_%d.json" % pid))
+    picked = [e for e in entries if e["name"] in ("gray1350", "crop12", "inv8")]
+    with frames.FramePool(1) as pool:
+        jobs = []
+        for entry in picked:
+            _, priv = gold_inputs(entry)
+            binding = commit_binding(read_commit_manifest(os.path.join(GOLD, entry["manifest"])))
+            c = frames.FrameCircuit(circuits.circuit(entry["circuit"]))
+            jobs.append((entry, pool.submit(c, priv, pid, vole, binding)))
+            c.close()                # closed by the caller while the frame is queued
+            del c
+        gc.collect()
+        for entry, jid in jobs:
+            res = pool.result(jid)
+            check_against_gold(entry, res.proof, res.input_point, res.input_value)
+        _, priv = gold_inputs(picked[0])
+        binding = commit_binding(read_commit_manifest(os.path.join(GOLD, picked[0]["manifest"])))
+        res = pool.result(pool.submit(frames.FrameCircuit(circuits.circuit(picked[0]["circuit"])),
+                                      priv, pid, vole, binding))
+        check_against_gold(picked[0], res.proof, res.input_point, res.input_value)
 
 
 def test_frame_errors_are_reported(bridge):

@@ -52,7 +52,8 @@ def write_outputs(tmp_path, prv, n_copies):
             fh.write(" ".join(str(v) for v in row) + "\n")
 
 
-@pytest.mark.parametrize("name", ["gray16", "inv8", "crop12", "mask5", "gray1350"])
+@pytest.mark.parametrize("name", ["gray16", "inv8", "crop12", "mask5", "gray1350", "inv3600",
+                                  "mask8040"])
 def test_proof_bytes_match_reference(bridge, torch_cuda, tmp_path, name):
     pid, entries = cases()
     entry = next(e for e in entries if e["name"] == name)

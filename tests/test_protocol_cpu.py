@@ -38,6 +38,33 @@ def test_restated_circuits_match_pws(name):
     assert len(parsed[0]) == want["n_inputs"] and len(parsed[1]) == want["n_gates"]
 
 
+# The packing TP/libTruePix/fp2_link.py:540-558 assigns to each PWS file
+# (literal constants of the reference source): copy_size, const_idx, const_val.
+REFERENCE_LAYOUTS = {"gray.pws": (384, 384, 128), "inv.pws": (384, 384, 255),
+                     "mask.pws": (384, -1, 0), "Crop.pws": (385, -1, 0)}
+
+
+@pytest.mark.parametrize("name,pws", [("gray", "gray.pws"), ("invert", "inv.pws"),
+                                      ("mask", "mask.pws"), ("crop", "Crop.pws")])
+def test_circuit_input_layouts_match_the_reference_packing(tmp_path, name, pws):
+    """The restated circuits put their private inputs and their constant wire
+    where the reference's commitment packing expects them, and an input.txt
+    line of a frame has copy_size values."""
+    from truepix_amd.fp2_link import infer_gkr_input_layout, input_layout, random_private_inputs
+    parsed = circuits.circuit(name)
+    want = REFERENCE_LAYOUTS[pws]
+    assert input_layout(parsed) == want
+    assert infer_gkr_input_layout("TP/pws/" + pws, str(tmp_path / "none.txt")) == want
+    priv = random_private_inputs(name, parsed, 3, seed=1)
+    with open(tmp_path / "input.txt", "w") as fh:
+        for row in priv.tolist():
+            fh.write(" ".join(map(str, row)) + "\n")
+    assert infer_gkr_input_layout(pws, str(tmp_path / "input.txt")) == want
+    # the constant sits right after the private samples, as the manifest says
+    if want[1] >= 0:
+        assert all(c is None for c in parsed[0][:want[1]])
+
+
 def test_pws_parser_subset():
     text = "P V0 = I0 E\nP V1 = I1 E\nP V2 = 7 E\nP V3 = 9 E\nP O4 = V0 * V2 E\nP O5 = V1 minus V0 E\n"
     parsed = circuits.parse_pws_text(text)

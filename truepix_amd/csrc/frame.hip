@@ -1138,14 +1138,9 @@ void* tp_circuit_create(int n_gates, const int* op, const int* muxbit, const int
     return C;
 }
 
-void tp_circuit_destroy(void* c)
-{
-    auto* C = static_cast<Circuit*>(c);
-    if (!C) return;
-    DevBuf* all[] = {&C->d_op, &C->d_mux, &C->d_type, &C->d_in0, &C->d_in1, &C->d_slot, &C->d_cval};
-    for (DevBuf* b : all) b->release();
-    delete C;
-}
+// Drops the creator's reference: the circuit is freed now, or by the last
+// queued or finished job that still holds it (tp_pool_release, tp_pool_destroy).
+void tp_circuit_destroy(void* c) { circuit_unref(static_cast<const Circuit*>(c)); }
 
 void* tp_pool_create(int n_workers)
 {
@@ -1216,10 +1211,31 @@ double tp_pool_now(void* pool)
 
 }  // extern "C"
 
+namespace tp {
+namespace frame {
+
+void circuit_ref(const Circuit* c)
+{
+    if (c) const_cast<Circuit*>(c)->refs.fetch_add(1, std::memory_order_relaxed);
+}
+
+void circuit_unref(const Circuit* c)
+{
+    if (!c) return;
+    auto* C = const_cast<Circuit*>(c);
+    if (C->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
+    DevBuf* all[] = {&C->d_op, &C->d_mux, &C->d_type, &C->d_in0, &C->d_in1, &C->d_slot, &C->d_cval};
+    for (DevBuf* b : all) b->release();
+    delete C;
+}
+
+}  // namespace frame
+}  // namespace tp
+
 namespace {
 
-// A job from its description (everything it reads is copied); null with the
-// error recorded on a bad shape.
+// A job from its description (everything it reads is copied, the circuit is
+// referenced); null with the error recorded on a bad shape.
 std::unique_ptr<Job> make_job(const tp_frame_job* d)
 {
     if (!d || !d->circuit) return fail("tp_pool_submit: null argument"), nullptr;
@@ -1229,6 +1245,7 @@ std::unique_ptr<Job> make_job(const tp_frame_job* d)
                     d->n_private, d->input_bytes), nullptr;
     if (d->n_vole < 0 || d->n_binding < 0) return fail("tp_pool_submit: bad counts"), nullptr;
     auto J = std::make_unique<Job>();
+    circuit_ref(C);
     J->circ = C;
     J->n_copies = d->n_copies;
     J->n_priv = d->n_private;

@@ -39,17 +39,29 @@ int clog2(long long v);
 double now_s(Clock::time_point t0);
 
 // A one-layer circuit resident on the device (gate arrays, input slots).
+// Reference-counted: tp_circuit_create holds one reference and every queued
+// job one more, so tp_circuit_destroy while jobs still use the circuit only
+// drops the creator's reference and the last job frees it.
 struct Circuit {
     int n_gates = 0, in_bits = 0, out_bits = 0, n_inputs = 0, n_priv = 0;
     std::vector<int> type, in0, in1;
     DevBuf d_op, d_mux, d_type, d_in0, d_in1, d_slot, d_cval;
+    std::atomic<int> refs{1};
 };
+
+// Takes / drops a reference; the last drop frees the device arrays.
+void circuit_ref(const Circuit* c);
+void circuit_unref(const Circuit* c);
 
 // One frame to prove: inputs, VOLE correlations, commitment binding, pid; and
 // its results.
 struct Job {
+    Job() = default;
+    Job(const Job&) = delete;
+    Job& operator=(const Job&) = delete;
+    ~Job() { circuit_unref(circ); }   // the job's reference on its circuit
     int64_t id = 0;
-    const Circuit* circ = nullptr;
+    const Circuit* circ = nullptr;     // referenced (circuit_ref) while the job lives
     int n_copies = 0, n_priv = 0, in_bytes = 1;
     std::vector<uint8_t> in8;
     std::vector<u64> in64;

@@ -31,6 +31,7 @@
 #include "runtime.hpp"
 
 #include <stdint.h>
+#include <mutex>
 
 using namespace tp;
 
@@ -532,6 +533,16 @@ int tp_rs_encode(const uint64_t* d_coef, int n_cols, int log_coef, int log_order
                         n_cols, log_coef, log_order), -1;
     if (!tp::device_ready()) return -1;
     hipStream_t st = stream_handle ? (hipStream_t)stream_handle : tp::stream();
+    // One plan (twiddle tables, intermediate) serves every call: calls are
+    // serialized on the host, and a call on another stream than the previous
+    // call's first waits for that stream, so no launch still in flight reads
+    // tables or an intermediate that this call rewrites.
+    static std::mutex mu;
+    static hipStream_t last = nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (last && last != st && hipStreamSynchronize(last) != hipSuccess)
+        return tp::fail("tp_rs_encode: waiting for the previous call's stream failed"), -1;
+    last = st;
     if (!plan_for(log_order, st)) return -1;
     Plan& P_ = g_plan;
     const int a = log_coef / 2, b = log_coef - a, log_s = log_order - log_coef;

@@ -58,7 +58,7 @@ class Fp2:
     @classmethod
     def deserialize(cls, data: Sequence[int], p: int = P) -> "Fp2":
         if len(data) != 2:
-            raise ValueError("Fp2.deserialize expects [real, imag]")
+            raise ValueError("a serialized Fp2 is a pair of ints, got %d items" % len(data))
         return cls(data[0], data[1], p)
 
     # pickling: state (real, imag, p), as in the proof format
@@ -126,14 +126,14 @@ class Fp2:
 
     def __mod__(self, q: int) -> "Fp2":
         if q != self.p:
-            raise ValueError("Fp2 % q is only defined for q == base prime")
+            raise ValueError("reducing an Fp2 element modulo %d: only its own prime is allowed" % q)
         return self
 
     def inverse(self) -> "Fp2":
         p = self.p
         n = (self.real * self.real + self.imag * self.imag) % p
         if n == 0:
-            raise ZeroDivisionError("inverse of zero in Fp2")
+            raise ZeroDivisionError("0 has no multiplicative inverse")
         ni = pow(n, p - 2, p)
         return _raw(self.real * ni % p, -self.imag * ni % p, p)
 
@@ -161,14 +161,11 @@ def _as_fp2(v, p: int = P) -> Fp2:
 
 
 def fold_val(in0, in1, val, p: int = P) -> Fp2:
-    """in0 + val * (in1 - in0) with ints or Fp2 operands."""
-    ar, ai = (in0.real, in0.imag) if type(in0) is Fp2 else (in0, 0)
-    br, bi = (in1.real, in1.imag) if type(in1) is Fp2 else (in1, 0)
-    dr, di = br - ar, bi - ai
-    if type(val) is Fp2:
-        vr, vi = val.real, val.imag
-        return _raw((ar + vr * dr - vi * di) % p, (ai + vr * di + vi * dr) % p, p)
-    return _raw((ar + val * dr) % p, (ai + val * di) % p, p)
+    """The multilinear fold of the pair (in0, in1) at val: the line through
+    in0 (at 0) and in1 (at 1) evaluated at val.  Operands are ints or Fp2;
+    the result is always a new Fp2."""
+    lo, hi, x = _as_fp2(in0, p), _as_fp2(in1, p), _as_fp2(val, p)
+    return _raw(lo.real, lo.imag, p) + (hi - lo) * x
 
 
 def serialize_point(point: Sequence[Fp2]) -> List[List[int]]:

@@ -117,6 +117,45 @@ def make_manifest(n_copies: int, copy_size: int = 384, const_idx: int = 384, con
             "masked_root": roots[0], "mask_root": roots[1] if zk else ""}
 
 
+# Input packing per PWS file name prefix: (copy_size, const_idx, const_val), as
+# TP/libTruePix/fp2_link.py:533-563 (infer_gkr_input_layout) assigns them; a
+# name with no entry keeps the gray packing.
+_PWS_LAYOUTS = (("inv", (384, 384, 255)), ("mask", (384, -1, 0)), ("crop", (385, -1, 0)))
+_DEFAULT_LAYOUT = (384, 384, 128)
+
+
+def infer_gkr_input_layout(pws_path: Optional[str] = None, input_path: str = "input.txt"):
+    """(copy_size, const_idx, const_val) of the private-input packing the Orion
+    commitment is made over, from the PWS file's name; an existing input.txt
+    fixes copy_size to the length of its first line (fp2_link.py:533-563)."""
+    layout = _DEFAULT_LAYOUT
+    if pws_path:
+        base = os.path.basename(pws_path).lower()
+        for prefix, lay in _PWS_LAYOUTS:
+            if base.startswith(prefix):
+                layout = lay
+                break
+    copy_size, const_idx, const_val = layout
+    if os.path.isfile(input_path):
+        with open(input_path) as fh:
+            first = fh.readline().split()
+        if first:
+            copy_size = len(first)
+    return copy_size, const_idx, const_val
+
+
+def input_layout(parsed: list):
+    """(copy_size, const_idx, const_val) read off a parsed circuit's input
+    layer: the private inputs per copy, and the wire index and value of its
+    one constant wire (-1, 0 without one)."""
+    layer = parsed[0]
+    consts = [(i, int(c)) for i, c in enumerate(layer) if c is not None]
+    if len(consts) > 1:
+        raise ValueError("the truepix packing has at most one constant wire (got %d)" % len(consts))
+    idx, val = consts[0] if consts else (-1, 0)
+    return sum(1 for c in layer if c is None), idx, val
+
+
 def write_manifest(path: str, manifest: dict) -> None:
     with open(path, "w") as fh:
         json.dump(manifest, fh, indent=2)

@@ -52,10 +52,11 @@ class FrameCircuit:
     pypws-shaped description (TP/libTruePix/parse_pws.py:8-64)."""
 
     def __init__(self, parsed, muxbits: Optional[Sequence[int]] = None):
-        from truepix_amd.gkr import parse_layers
+        from truepix_amd.gkr import parse_layers, refuse_mux
         input_layer, layers = parse_layers(parsed)
         if len(layers) != 1:
             raise ValueError("the frame prover takes one-layer circuits (got %d layers)" % len(layers))
+        refuse_mux(layers)
         lay = layers[0]
         muxlen = (int(lay.mux.max()) if lay.n_gates else 0) + 1
         bits = list(muxbits) if muxbits is not None else [0] * muxlen
@@ -153,6 +154,10 @@ class FramePool:
             raise RuntimeError("tp_pool_create failed: %s" % nb.last_error())
         self.n_workers = int(n_workers)
         self.batched = bool(batched)
+        # job id -> its FrameCircuit, so a caller's circuit object outlives its
+        # jobs even when the caller drops it (the library also holds a
+        # reference on the device circuit per job)
+        self._circuits = {}
 
     BATCH_PROFILE = ("batches", "frames", "stage_wall_ms", "launch_wall_ms", "head_wall_ms",
                      "head_wait_sum_ms", "head_cpu_sum_ms", "prefix_cpu_sum_ms", "rounds_wall_ms",
@@ -201,6 +206,7 @@ class FramePool:
         jid = self._lib.tp_pool_submit(self.handle, ctypes.byref(job))
         if jid <= 0:
             raise RuntimeError("tp_pool_submit failed: %s" % nb.last_error())
+        self._circuits[int(jid)] = circuit
         return int(jid)
 
     def submit_yuv(self, circuit: FrameCircuit, y, cb, cr, pid: int, vole, binding: list,
@@ -220,11 +226,15 @@ class FramePool:
         ids = (ctypes.c_int64 * max(1, len(built)))()
         if self._lib.tp_pool_submit_many(self.handle, arr, len(built), ids) != 0:
             raise RuntimeError("tp_pool_submit_many failed: %s" % nb.last_error())
+        for k, sp in enumerate(specs):
+            self._circuits[int(ids[k])] = sp[0]
         return [int(ids[k]) for k in range(len(built))]
 
     def _job(self, circuit: FrameCircuit, inputs, pid: int, vole, binding: list,
              keep_outputs: bool = False):
         """The C job description and the arrays it points into."""
+        if not getattr(circuit, "handle", None):
+            raise ValueError("the circuit is closed")
         arr = np.asarray(inputs)
         if arr.ndim != 2 or arr.shape[1] != circuit.n_private:
             raise ValueError("inputs must be [n_copies][%d]" % circuit.n_private)
@@ -284,13 +294,19 @@ class FramePool:
             phases_ms={k: ph[i] for i, k in enumerate(PHASES)},
             outputs=outs, batch=bsz.value, footprint=fp.value)
         if release:
-            lib.tp_pool_release(h, job)
+            self.release(job)
         return res
+
+    def release(self, job: int) -> None:
+        """Forget a finished job (its proof, outputs and circuit reference)."""
+        self._lib.tp_pool_release(self.handle, job)
+        self._circuits.pop(int(job), None)
 
     def close(self) -> None:
         if getattr(self, "handle", None):
             self._lib.tp_pool_destroy(self.handle)
             self.handle = None
+            self._circuits.clear()
 
     def __enter__(self):
         return self

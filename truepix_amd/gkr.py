@@ -424,6 +424,17 @@ def mlx_z2_factor(w3, z2):
     return acc
 
 
+def refuse_mux(layers) -> None:
+    """The NIZK provers take no MUX gates.  The reference's cannot prove one:
+    its CircuitProverNIZK.run reaches eval_mlext, which reads per-layer mux
+    bits (TP/libTruePix/circuitverifier.py:276) that the prover never sets, and
+    raises; no proof of a MUX circuit exists to match."""
+    for lay in layers:
+        if any(o == "MUX" for o in lay.ops):
+            raise ValueError("MUX gates cannot be proved by the NIZK prover (the reference's "
+                             "CircuitProverNIZK.run fails on them as well)")
+
+
 class _FromPWS(type):
     def from_pws(cls, parsed, nCopies):
         input_layer, layers = parse_layers(parsed)
@@ -522,6 +533,7 @@ class CircuitProverNIZK(_NIZKBase):
         self.fs.put(self.com.est_val(tag, chal))
 
     def run(self, inputs, muxbits=None):
+        refuse_mux(self.layers)
         import time as _time
         marks = [("start", _time.perf_counter())]
 
