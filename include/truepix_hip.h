@@ -225,6 +225,17 @@ int tp_wiring_mlext(int n_gates, const int* type_idx, const int* in0, const int*
 int tp_mle_eval_base_dev(const uint64_t* d_inputs, int n_z, const uint64_t* z, uint64_t out2[2],
                          void* stream);
 
+/* No counterpart in the reference: tp_mle_eval_base_dev for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and result; its
+ * partial sums and result live in the library's static device scratch
+ * (64 KiB per device, of which a call uses 256 + 16 * min(1024, 2^(n_z-10))
+ * bytes), so no call allocates or frees anything.  n_z in [1, 40]; larger
+ * inputs fail with an error.  Calls that overlap on one device share the
+ * scratch and race: serialize them.  0 / -1. */
+int tp_mle_eval_base_dev_noalloc(const uint64_t* d_inputs, int n_z, const uint64_t* z,
+                                 uint64_t out2[2], void* stream);
+
 /* ---- whole frames: CircuitProverNIZK.run (TP/libTruePix/circuitnizkvec_vole.py:265-501)
  *      for a one-layer circuit, as run_truepix_prover.run_truepix calls it
  *      (TP/run_truepix_prover.py:218-423), device-resident, batched and served --- */

@@ -415,4 +415,38 @@ int tp_mle_eval_base_dev(const uint64_t* d_inputs, int n_z, const uint64_t* z, u
     return 0;
 }
 
+// tp_mle_eval_base_dev with its partial sums and result in the library's
+// static device scratch (tp::scratch_ptr) instead of a library allocation:
+// no hipMalloc / hipFree on any call.  The same kernels; every word of the
+// scratch it reads is written by the same call first, so nothing needs
+// clearing.  Calls that overlap on one device share the scratch and race.
+int tp_mle_eval_base_dev_noalloc(const uint64_t* d_inputs, int n_z, const uint64_t* z,
+                                 uint64_t out2[2], void* stream_handle)
+{
+    tp::clear_error();
+    if (n_z < 1 || n_z > 40) return tp::fail("tp_mle_eval_base_dev_noalloc: n_z=%d", n_z), -1;
+    char* scr = static_cast<char*>(tp::scratch_ptr());
+    if (!scr) return -1;
+    hipStream_t st = stream_handle ? (hipStream_t)stream_handle : tp::stream();
+    PointArg pz;
+    memset(&pz, 0, sizeof(pz));
+    for (int j = 0; j < n_z; ++j) pz.z[j] = mk2(z[2 * j], z[2 * j + 1]);
+    const int lb = n_z < 10 ? n_z : 10;
+    const size_t rows = (size_t)1 << (n_z - lb);
+    const unsigned blocks = (unsigned)(rows < 1024 ? rows : 1024);
+    const size_t need = 256 + (size_t)blocks * 16;
+    if (need > tp::kScratchBytes)
+        return tp::fail("tp_mle_eval_base_dev_noalloc: needs %zu scratch bytes, has %zu", need,
+                        tp::kScratchBytes), -1;
+    fp2* result = reinterpret_cast<fp2*>(scr);
+    fp2* partials = reinterpret_cast<fp2*>(scr + 256);
+    kern::k_mle_base_dev<<<blocks, kBlock, 0, st>>>((const u64*)d_inputs, pz, n_z, partials);
+    TP_HIP_OK(hipGetLastError());
+    k_finalize_sum<1><<<1, kBlock, 0, st>>>(partials, blocks, result);
+    TP_HIP_OK(hipGetLastError());
+    TP_HIP_OK(hipMemcpyAsync(out2, result, 16, hipMemcpyDeviceToHost, st));
+    TP_HIP_OK(hipStreamSynchronize(st));
+    return 0;
+}
+
 }  // extern "C"

@@ -12,6 +12,13 @@ static std::once_flag g_once;
 static bool g_ok = false;
 static hipStream_t g_stream = nullptr;
 
+// The no-alloc scratch.  In host code `scratch` names a host-side stand-in,
+// not device memory: host code uses only the address looked up below.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void* g_scratch[kMaxDevices] = {nullptr};
+static char g_scratch_err[256] = {0};
+
 bool fail(const char* fmt, ...)
 {
     va_list ap;
@@ -37,7 +44,27 @@ static void init_once()
         fail("hipStreamCreate failed");
         return;
     }
+    // this device's copy of the scratch array (device 0 is current here)
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && ptr)
+        g_scratch[0] = ptr;
+    else
+        snprintf(g_scratch_err, sizeof(g_scratch_err),
+                 "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
     g_ok = true;
+}
+
+void* scratch_ptr()
+{
+    if (!device_ready()) return nullptr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices || !g_scratch[dev]) {
+        fail("no-alloc scratch unavailable on device %d%s%s", dev, g_scratch_err[0] ? ": " : "",
+             g_scratch_err);
+        return nullptr;
+    }
+    return g_scratch[dev];
 }
 
 bool device_ready()

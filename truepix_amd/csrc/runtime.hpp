@@ -23,6 +23,14 @@ bool tp_last_error_set();
 // TP/native_fp2/native_fp2.cu:88-93).  Returns false if no HIP device.
 bool device_ready();
 
+// Device scratch for the entry points that allocate nothing: a file-scope
+// __device__ array of kScratchBytes in runtime.hip, whose address on device d
+// is looked up once, at bring-up, with hipGetSymbolAddress.  Null when the
+// lookup failed (the no-alloc entry points then fail with that error).  Every
+// call on a device shares that device's copy: calls that overlap race.
+constexpr size_t kScratchBytes = 64 * 1024;
+void* scratch_ptr();
+
 // Library stream (default priority).  Every drop-in entry point enqueues on
 // it and synchronizes before returning, as the reference ABI is synchronous.
 hipStream_t stream();

@@ -99,6 +99,7 @@ def bind(lib) -> None:
     _sig(lib, "tp_wiring_mlext", [_I32, _I32P, _I32P, _I32P, _U64P, _U64P, _I32, _U64P, _U64P, _I32,
                                   _U64P, _U64P, _I32, _U64P], _I32)
     _sig(lib, "tp_mle_eval_base_dev", [_VP, _I32, _U64P, _U64P, _VP], _I32)
+    _sig(lib, "tp_mle_eval_base_dev_noalloc", [_VP, _I32, _U64P, _U64P, _VP], _I32)
     _sig(lib, "tp_sc2_stats", [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                _I32P])
     _sig(lib, "tp_sc2_tail_stats", [ctypes.POINTER(ctypes.c_double), _I32P])
