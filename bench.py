@@ -62,7 +62,25 @@ SC2_KERNELS = {"first": "k_sc2_first01", "fold01": "k_sc2_fold01_round",
                "fold": "k_sc2_fold_round", "tail": "k_sc2_tail_host", "mid": "k_sc2_mid"}
 
 
-def bench_rs_encode(steps: int = 5, cols: int = 64, lc: int = 20, lo: int = 22):
+# 32 x 32-bit multiply lane-ops of one Fp2 twiddle product (3 Karatsuba Fp
+# products of 4 limb products each), and MI355X's issue rate for them.
+MUL_LANE_OPS_PER_BUTTERFLY = 12
+INT_MUL_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+
+
+def rs_counters():
+    """Per-encode counter figures of the RS encode committed under profiles/
+    (rocprofv3 --pmc runs of this bench): VALU lane-instructions per
+    butterfly and HBM bytes per encode; None where absent."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "rs_encode_64x2e20_counters.json")) as fh:
+            rec = json.load(fh)
+        return rec.get("valu_lane_insts_per_butterfly"), rec.get("hbm_bytes_per_encode")
+    except (OSError, ValueError):
+        return None, None
+
+
+def bench_rs_encode(steps: int = 5, cols: int = 64, lc: int = 20, lo: int = 22, cpu: bool = True):
     """Reed-Solomon encode of 64 columns x 2^20 -> 2^22 (resident in HBM)."""
     import torch
     from truepix_amd import ntt
@@ -81,11 +99,82 @@ def bench_rs_encode(steps: int = 5, cols: int = 64, lc: int = 20, lo: int = 22):
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / steps
     alg = cols * ((16 << lc) + (16 << lo))
+    # the reference's network: replicate to 2^lo, then lc radix-2 stages of
+    # 2^(lo-1) butterflies, each one Fp2 twiddle product (RS_polynomial.cpp:46-130)
     bfly = cols * lc * (1 << (lo - 1))
-    del dc, do
-    return {"workload": "RS encode 64 x 2^20 -> 2^22 Fp2 (rate 1/4)", "ms": round(ms, 3),
-            "alg_GBps": round(alg / ms / 1e6, 1), "Gbutterflies_per_s": round(bfly / ms / 1e6, 2),
-            "alg_bytes": alg}
+    ops = bfly * MUL_LANE_OPS_PER_BUTTERFLY
+    coef_host = coef[: 2 << lc].copy()
+    del dc, do, coef
+    out = {"workload": "RS encode 64 x 2^20 -> 2^22 Fp2 (rate 1/4)", "ms": round(ms, 3),
+           "alg_GBps": round(alg / ms / 1e6, 1), "Gbutterflies_per_s": round(bfly / ms / 1e6, 2),
+           "alg_bytes": alg,
+           "roofline": {"bound": "valu", "achieved": round(ops / ms / 1e9, 3),
+                        "peak": INT_MUL_PEAK_TOPS, "unit": "T int32-multiply lane-ops/s",
+                        "frac": round(ops / ms / 1e9 / INT_MUL_PEAK_TOPS, 4),
+                        "traffic": rs_counters()[1],
+                        "hbm_frac": round(alg / ms / 1e6 / PEAK_HBM_GBS, 4),
+                        "valu_lane_insts_per_butterfly": rs_counters()[0],
+                        "note": "useful work = 12 32x32-bit multiply lane-ops per butterfly "
+                                "(one Fp2 twiddle product: 3 Fp products by Karatsuba, 4 limb "
+                                "products each) of the reference's radix-2 network; peak = "
+                                "256 CUs x 64 lanes x 2.4 GHz (the published 157.3 TFLOP/s "
+                                "fp32 counts packed FMAs: 4 flops per lane-cycle, the integer "
+                                "multiply-add has no packed form); traffic: HBM bytes of one "
+                                "encode (FETCH_SIZE + WRITE_SIZE, scaled), including the "
+                                "intermediate between the two passes; "
+                                "valu_lane_insts_per_butterfly: SQ_INSTS_VALU x 64 of both "
+                                "passes / butterflies (profiles/rs_encode_64x2e20_counters.json)"}}
+    if cpu:
+        out["cpu_baseline"] = cpu_rs_baseline(coef_host, lc, lo, ms, cols)
+    return out
+
+
+def cpu_rs_baseline(col, lc: int, lo: int, gpu_ms: float, cols: int, seconds: float = 6.0):
+    """One column of the same encode on the host: the reference's own
+    fast_fourier_transform (RS_polynomial.cpp compiled from source,
+    oracle/_ref) on one core, and the oracle's coset DFTs on every core of
+    this process's share, one column per thread."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle.oracle import Oracle, RefRS
+    col_bytes = (16 << lc) + (16 << lo)
+    rec = {}
+    if RefRS.available():
+        impl, kind, what = RefRS(), "reference", "oracle/_ref fast_fourier_transform (the reference's code)"
+    else:
+        impl, kind, what = Oracle(), "port", "oracle/fp2_oracle.c or_fft"
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < seconds / 2 or n == 0:
+        impl.fft(col, lc, lo)
+        n += 1
+    dt = (time.perf_counter() - t0) / n
+    rec.update({"value": round(col_bytes / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": kind,
+                "sample": "%d column(s) of 2^%d -> 2^%d, %s, %.2f s a column" % (n, lc, lo, what, dt),
+                "ms_per_64_columns": round(dt * cols * 1e3, 1),
+                "gpu_speedup": round(dt * cols * 1e3 / gpu_ms, 1)})
+    threads = host_threads()
+    orc = Oracle()
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda _: orc.fft(col, lc, lo), range(threads)))
+    dt = time.perf_counter() - t0
+    rec["all_cores"] = {"value": round(threads * col_bytes / dt / 1e9, 4), "unit": "GB/s",
+                        "cores": threads, "kind": "port",
+                        "sample": "%d columns at once, one per thread, oracle/fp2_oracle.c or_fft "
+                                  "(coset DFTs), %.2f s" % (threads, dt),
+                        "ms_per_64_columns": round(dt * cols / threads * 1e3, 1)}
+    return rec
+
+
+def host_threads() -> int:
+    """Host cores this process may use (its affinity, capped by OMP_NUM_THREADS)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, n)
 
 
 def bench_gkr_frame(frames: int = 5, circuit: str = "gray", n_copies: int = 1350):
@@ -188,6 +277,84 @@ def bench_frames_plain(frames_n: int = 12, circuit: str = "gray", n_copies: int 
             "frames_per_s": round(1.0 / med, 2), "ms_per_frame_median": round(med * 1e3, 3),
             "ms_per_frame_min": round(times[0] * 1e3, 3), "proof_bytes": len(res.proof),
             "phases_ms_median": ph}
+
+
+def cpu_frame_baseline(n_copies: int = 1350, seconds: float = 6.0, seed: int = 105):
+    """The arithmetic of one plain gray frame on the host: the reference's own
+    CPU library (TP/native_fp2/native_fp2.cpp compiled from source, oracle/_ref;
+    the oracle's restatement where that is absent) running the frame's early
+    session (the 11 claim + fold rounds over 512 wires x 2048 copies and 384
+    gates), its input MLE over 2^20 inputs, and CPython's SHA-256 over the
+    8 MiB of outputs the transcript absorbs.  The late rounds, the VOLE
+    commitments and the transcript framing (under 1 ms together on the
+    device path) are not counted, so this is a lower bound on the CPU's time
+    per frame."""
+    import ctypes
+    import hashlib
+    import numpy as np
+    from oracle.oracle import Oracle, RefNative, _p
+    from truepix_amd import circuits
+    from truepix_amd.fp2_link import full_input_rows, random_private_inputs
+    from truepix_amd.gkr import parse_layers
+    P = (1 << 61) - 1
+    rng = np.random.Generator(np.random.PCG64(seed))
+    parsed = circuits.circuit("gray")
+    _, (lay,) = parse_layers(parsed)
+    rows = full_input_rows(parsed, random_private_inputs("gray", parsed, n_copies, seed=seed))
+    cp = 1 << (n_copies - 1).bit_length()
+    nw = 1 << lay.in_bits
+    vr = np.zeros((nw, cp), np.uint64)
+    vr[:rows.shape[1], :n_copies] = rows.T.astype(np.uint64)
+    vi = np.zeros_like(vr)
+    gt, i0, i1 = (np.ascontiguousarray(a, np.int32) for a in (lay.type_idx([0]), lay.in0, lay.in1))
+    z = rng.integers(0, P, size=(4, lay.n_gates), dtype=np.int64).astype(np.uint64)
+    beta = rng.integers(0, P, size=(2, cp), dtype=np.int64).astype(np.uint64)
+    chal = rng.integers(0, P, size=(64, 2), dtype=np.int64).astype(np.uint64)
+    inputs = np.zeros(1 << 20, np.uint64)
+    inputs[: rows.size] = rows.reshape(-1).astype(np.uint64)
+    zin = rng.integers(0, P, size=(2, 20), dtype=np.int64).astype(np.uint64)
+    outputs = np.zeros(cp * 512, np.uint64).tobytes()
+    ref = RefNative.available()
+    impl = RefNative() if ref else None
+    orc = Oracle()
+    out8 = (ctypes.c_uint64 * 8)()
+    o2 = (ctypes.c_uint64 * 1)(), (ctypes.c_uint64 * 1)()
+
+    def one_frame():
+        hashlib.sha256(outputs).digest()
+        if ref:
+            h = impl.early_create(lay.n_gates, _p(gt), _p(i0), _p(i1), _p(z[0]), _p(z[1]), cp, nw,
+                                  _p(vr), _p(vi), _p(beta[0]), _p(beta[1]))
+            for r in range(cp.bit_length() - 1):
+                impl.early_claim(h, out8)
+                impl.early_fold(h, int(chal[r, 0]), int(chal[r, 1]))
+            impl.early_destroy(h)
+            impl.mle_eval_base(_p(inputs), _p(zin[0]), _p(zin[1]), 20, o2[0], o2[1])
+        else:
+            wr, wi, br_, bi_ = vr.copy(), vi.copy(), beta[0].copy(), beta[1].copy()
+            n = cp
+            for r in range(cp.bit_length() - 1):
+                orc.early_claim(lay.n_gates, _p(gt), _p(i0), _p(i1), _p(z[0]), _p(z[1]), n,
+                                _p(wr), _p(wi), _p(br_), _p(bi_), out8)
+                nr, ni = np.zeros(nw * n // 2, np.uint64), np.zeros(nw * n // 2, np.uint64)
+                nbr, nbi = np.zeros(n // 2, np.uint64), np.zeros(n // 2, np.uint64)
+                orc.early_fold(nw, n, _p(wr), _p(wi), _p(br_), _p(bi_), int(chal[r, 0]),
+                               int(chal[r, 1]), _p(nr), _p(ni), _p(nbr), _p(nbi))
+                wr, wi, br_, bi_, n = nr, ni, nbr, nbi, n // 2
+            orc.mle_eval_base(_p(inputs), _p(zin[0]), _p(zin[1]), 20, o2[0], o2[1])
+    t0 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t0 < seconds or k == 0:
+        one_frame()
+        k += 1
+    dt = (time.perf_counter() - t0) / k
+    return {"value": round(1.0 / dt, 3), "unit": "frames/s", "cores": 1,
+            "kind": "reference" if ref else "port", "ms_per_frame": round(dt * 1e3, 1),
+            "sample": "%d frame(s) of gray x %d: early session (11 claim + fold rounds over 512 x "
+                      "2048), input MLE over 2^20, SHA-256 of the 8 MiB outputs, by %s; late "
+                      "rounds, VOLE and transcript framing not counted (a lower bound)"
+                      % (k, n_copies, "the reference's native_fp2.cpp compiled from source "
+                         "(oracle/_ref)" if ref else "oracle/fp2_oracle.c")}
 
 
 # C4 pools: stream-W = W stream workers (stream-1 is the plain prover, one
@@ -307,6 +474,10 @@ def bench_served(n_jobs: int = 2000, rate: float = 200.0, seed: int = 5, variant
                     lat.append(r.t_end - r.t_submit)
                 plain[(c, n)] = sorted(lat)[1]
     plain_lat = np.array([plain[p] for p in plan])
+    # solo proofs of a sample of the jobs: each served proof must equal them
+    sample = list(range(0, n_jobs, max(1, n_jobs // 20)))
+    with frames.FramePool(1) as pool:
+        solo = {k: pool.result(pool.submit(*specs[k])).proof for k in sample}
     out = {"workload": "%d jobs, Poisson %.0f jobs/s, circuits uniform over %s, sizes uniform over "
                        "%s macroblocks" % (n_jobs, rate, "/".join(CIRCUITS),
                                            "/".join(str(FRAME_SIZES[s]) for s in sizes)),
@@ -332,7 +503,9 @@ def bench_served(n_jobs: int = 2000, rate: float = 200.0, seed: int = 5, variant
         rec = {"frames_per_s": round(n_jobs / span, 2),
                "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
                "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
-               "mean_batch": round(float(np.mean([r.batch for r in res])), 2)}
+               "mean_batch": round(float(np.mean([r.batch for r in res])), 2),
+               "proofs_equal_solo": "%d/%d" % (sum(res[k].proof == solo[k] for k in sample),
+                                               len(sample))}
         if mem is not None:
             rec["peak_admitted_GB"] = round(mem["peak"] / 1e9, 2)
             rec["budget_GB"] = round(mem["budget"] / 1e9, 1)
@@ -343,6 +516,69 @@ def bench_served(n_jobs: int = 2000, rate: float = 200.0, seed: int = 5, variant
     out["best_config"] = best
     for k in ("frames_per_s", "p50_ms", "p99_ms"):
         out[k] = out["by_config"][best][k]
+    return out
+
+
+def bench_served_sweep(rates=(200, 400, 800, 1600), n_jobs: int = 600, seed: int = 6,
+                       variants=None):
+    """C5 at rising Poisson rates: per pool and rate, the frames/s delivered
+    and the latency percentiles, and the highest rate whose p99 stays within
+    twice the plain p99 of the same job mix (each job alone on an idle
+    prover)."""
+    from truepix_amd import frames
+    import numpy as np
+    if variants is None:
+        variants = [("stream-16", dict(n_workers=16)),
+                    ("lockstep-2x64", dict(n_workers=2, batched=True, max_batch=64))]
+    fac = _FrameFactory(seed)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = ["240p", "480p", "720p", "1080p"]
+    plan = [(CIRCUITS[rng.integers(0, 4)], FRAME_SIZES[sizes[rng.integers(0, 4)]])
+            for _ in range(n_jobs)]
+    specs = [fac.job(c, n, k, variant=k % 2) for k, (c, n) in enumerate(plan)]
+    plain = {}
+    with frames.FramePool(1) as pool:
+        for key in sorted(set(plan)):
+            sp = fac.job(key[0], key[1], 99999)
+            pool.result(pool.submit(*sp))
+            lat = []
+            for _ in range(3):
+                r = pool.result(pool.submit(*sp))
+                lat.append(r.t_end - r.t_submit)
+            plain[key] = sorted(lat)[1]
+    plain_p99 = float(np.percentile([plain[p] for p in plan], 99)) * 1e3
+    out = {"workload": "%d jobs per rate, Poisson arrivals, circuits uniform over %s, sizes "
+                       "uniform over %s macroblocks" % (n_jobs, "/".join(CIRCUITS),
+                                                        "/".join(str(FRAME_SIZES[s]) for s in sizes)),
+           "plain_p99_ms": round(plain_p99, 3), "by_config": {}}
+    for label, kw in variants:
+        rows = {}
+        with frames.FramePool(**kw) as pool:
+            for k in range(16):
+                pool.result(pool.submit(*fac.job("gray", FRAME_SIZES["1080p"], 90000 + k)))
+            for rate in rates:
+                arrive = np.cumsum(np.random.Generator(np.random.PCG64(seed + rate))
+                                   .exponential(1.0 / rate, size=n_jobs))
+                t0 = time.perf_counter()
+                p0 = pool.now()
+                jobs = []
+                for k, sp in enumerate(specs):
+                    while time.perf_counter() - t0 < arrive[k]:
+                        pass
+                    jobs.append(pool.submit(*sp))
+                res = [pool.result(j) for j in jobs]
+                lat = np.array([r.t_end - r.t_submit for r in res]) * 1e3
+                span = max(r.t_end for r in res) - p0
+                rows[str(rate)] = {"frames_per_s": round(n_jobs / span, 1),
+                                   "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                                   "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+                print("bench: sweep %s %d/s p99 %.1f ms" % (label, rate, rows[str(rate)]["p99_ms"]),
+                      file=sys.stderr, flush=True)
+        ok = [int(r) for r, v in rows.items() if v["p99_ms"] <= 2 * plain_p99]
+        out["by_config"][label] = {
+            "by_rate": rows,
+            "max_rate_p99_within_2x_plain": max(ok) if ok else None,
+            "frames_per_s_at_that_rate": rows[str(max(ok))]["frames_per_s"] if ok else None}
     return out
 
 
@@ -544,8 +780,9 @@ def cpu_baseline(nv_sample: int = 24, nthreads: int = 1, min_seconds: float = 10
             break
     return {"value": round(reps * alg_bytes(nv_sample) / dt / 1e9, 3), "unit": "GB/s",
             "cores": nthreads, "kind": "port",
-            "sample": "%d full %d-round proves over 2 x 2^%d Fp2 tables, %.1f s single-threaded "
-                      "(oracle/fp2_oracle.c, -O3)" % (reps, nv_sample, nv_sample, dt)}
+            "sample": "%d full %d-round proves over 2 x 2^%d Fp2 tables, %.1f s on %d thread(s) "
+                      "(oracle/fp2_oracle.c, -O3, OpenMP)" % (reps, nv_sample, nv_sample, dt,
+                                                              nthreads)}
 
 
 def hbm_ceiling(reps: int = 5):
@@ -591,8 +828,8 @@ def main(argv=None):
                     help="comma list of batched pool labels to run (default: all), e.g. "
                          "lockstep-1x256")
     ap.add_argument("--skip", default="", help="comma list of side measurements to skip "
-                    "(gkr_frame, gkr_frame_python_driver, batched, served, fft_gkr, orion, yuv, "
-                    "rs_encode)")
+                    "(gkr_frame, gkr_frame_python_driver, batched, served, served_sweep, "
+                    "fft_gkr, orion, yuv, rs_encode)")
     a = ap.parse_args(argv)
     a.skip = set(x for x in a.skip.split(",") if x)
     if a.gpus != 1:
@@ -703,7 +940,8 @@ def main(argv=None):
         return bench_batched(variants=[c for c in BATCHED_CONFIGS if c[0] in want])
 
     for key, fn in (("gkr_frame", bench_frames_plain), ("gkr_frame_python_driver", bench_gkr_frame),
-                    ("batched", batched), ("served", bench_served)):
+                    ("batched", batched), ("served", bench_served),
+                    ("served_sweep", bench_served_sweep)):
         if key in a.skip:
             continue
         try:
@@ -711,6 +949,13 @@ def main(argv=None):
         except Exception as exc:
             line[key] = {"error": "%s: %s" % (type(exc).__name__, exc)}
         print("bench: %s done" % key, file=sys.stderr, flush=True)
+        if key == "gkr_frame" and not a.no_cpu_baseline and "error" not in line[key]:
+            try:
+                cb = cpu_frame_baseline()
+                cb["gpu_speedup"] = round(line[key]["frames_per_s"] / cb["value"], 1)
+                line[key]["cpu_baseline"] = cb
+            except Exception as exc:
+                line[key]["cpu_baseline"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
     if "fft_gkr" not in a.skip:
         try:
             line["fft_gkr"] = bench_fft_gkr(cpu=not a.no_cpu_baseline)
@@ -731,12 +976,14 @@ def main(argv=None):
         print("bench: yuv_layout done", file=sys.stderr, flush=True)
     if "rs_encode" not in a.skip:
         try:
-            line["rs_encode"] = bench_rs_encode()
+            line["rs_encode"] = bench_rs_encode(cpu=not a.no_cpu_baseline)
         except Exception as exc:
             line["rs_encode"] = {"error": str(exc)}
     if not a.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline()
+            n = host_threads()
+            line["cpu_baseline"]["all_cores"] = cpu_baseline(nthreads=n, min_seconds=4.0)
         except Exception as exc:  # the baseline must not hide the GPU result
             line["cpu_baseline"] = {"error": str(exc)}
     print(json.dumps(line))

@@ -23,3 +23,15 @@ def test_field_arithmetic(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failures" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_claim_terms_equal_the_gate_sum(tmp_path):
+    """The early claim's term table (csrc/claim_terms.hpp) gives the same four
+    claim values as the gate-by-gate sum, for layers over all 13 gate types."""
+    exe = str(tmp_path / "claim_terms_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "truepix_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "claim_terms_check.cpp"), "-o", exe], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 failures" in r.stdout

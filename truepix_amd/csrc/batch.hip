@@ -68,7 +68,7 @@ constexpr int kElemCap = 1024;   // workgroups per frame of an elementwise launc
 // workgroups per frame of the fold launches.
 constexpr long long kClaimCand[3] = {1 << 14, 1 << 16, 1 << 18};
 constexpr int kFoldCand[3] = {256, 1024, 4096};
-constexpr long long kTuneMinItems = 1 << 22;   // batches smaller than this are not trials
+constexpr long long kTuneMinItems = 1 << 21;   // batches with fewer claim terms are not trials
 constexpr int kMleCap = 256;     // workgroups per frame of the input MLE
 
 struct FDesc {
@@ -82,6 +82,10 @@ struct FDesc {
     unsigned* big;    // mapped flag: set when an output is above 255
     u64 *vr[2], *vi[2], *br[2], *bi[2];
     u64 *z1cr, *z1ci, *z1lr, *z1li, *othr, *othi;
+    // the early claim's terms (claim_terms.hpp) and their coefficients
+    const int *tt, *tin0, *tin1, *toff, *tsrc, *tneg;
+    int n_terms;
+    u64 *tzr, *tzi;
     fp2* partials;
     fp2* small;   // beta* (as two limbs), then line values at 4, wiring sums at 20
 };
@@ -190,8 +194,20 @@ __global__ void __launch_bounds__(kBlock) k_b_eq(const FDesc* __restrict__ D,
     else body::eq_table(zs, d.nout, d.z1cr, d.z1ci, d.n_gates, local - m.a);
 }
 
-// A6, early round rd for every frame that has it; a = gate chunks per pair,
-// b = gates per chunk (body::early_claim).
+// The claim terms' coefficients of every frame from its z1 values
+// (body::term_coeffs).
+__global__ void __launch_bounds__(kBlock) k_b_terms(const FDesc* __restrict__ D,
+                                                    const Item* __restrict__ it, int n)
+{
+    const int i = find_item(it, n, blockIdx.x);
+    const Item m = it[i];
+    const FDesc& d = D[m.f];
+    body::term_coeffs(d.z1cr, d.z1ci, d.toff, d.tsrc, d.tneg, d.n_terms, d.tzr, d.tzi,
+                      blockIdx.x - m.b0);
+}
+
+// A6, early round rd for every frame that has it; a = term chunks per pair,
+// b = terms per chunk (body::early_claim).
 __global__ void __launch_bounds__(kBlock) k_b_claim(const FDesc* __restrict__ D,
                                                     const Item* __restrict__ it, int n, int rd)
 {
@@ -200,7 +216,7 @@ __global__ void __launch_bounds__(kBlock) k_b_claim(const FDesc* __restrict__ D,
     const FDesc& d = D[m.f];
     const int cur = rd & 1;
     const int local = blockIdx.x - m.b0;
-    body::early_claim(d.n_gates, m.b, m.a, d.gt, d.in0, d.in1, d.z1cr, d.z1ci, d.cp >> rd,
+    body::early_claim(d.n_terms, m.b, m.a, d.tt, d.tin0, d.tin1, d.tzr, d.tzi, d.cp >> rd,
                       d.vr[cur], d.vi[cur], d.br[cur], d.bi[cur], d.partials, local, local);
 }
 
@@ -345,7 +361,7 @@ size_t al(size_t n) { return (n + 255) & ~(size_t)255; }
 // Early claim work of round rd with `items` (pair, chunk) items per frame.
 ClaimShape claim_of(const Shape& S, int rd, long long items)
 {
-    return claim_shape(S.n_gates, (S.cp >> rd) >> 1, items);
+    return claim_shape(S.n_terms, (S.cp >> rd) >> 1, items);
 }
 
 int mle_blocks(const Shape& S)
@@ -391,6 +407,8 @@ size_t tables_layout(const Shape& S, char* base, FDesc* d)
     char* z1li = take(g8);
     char* othr = take(g8);
     char* othi = take(g8);
+    char* tzr = take((size_t)S.n_terms * 8);
+    char* tzi = take((size_t)S.n_terms * 8);
     char* parts = take(partial_fp2(S) * sizeof(fp2));
     char* small = take(kSmall * sizeof(fp2));
     if (d) {
@@ -409,6 +427,8 @@ size_t tables_layout(const Shape& S, char* base, FDesc* d)
         d->z1li = (u64*)z1li;
         d->othr = (u64*)othr;
         d->othi = (u64*)othi;
+        d->tzr = (u64*)tzr;
+        d->tzi = (u64*)tzi;
         d->partials = (fp2*)parts;
         d->small = (fp2*)small;
     }
@@ -759,6 +779,9 @@ bool prove_batch(BatchCtx& B, std::vector<Job*>& jobs_in, TaskPool& tasks, doubl
         sc.add(L_eq, f, bb + bz, bb);
     }
     sc.end(L_eq);
+    Launch L_terms = sc.begin();
+    for (int f = 0; f < nf; ++f) sc.add(L_terms, f, (S[f].n_terms * 64 + kBlock - 1) / kBlock);
+    sc.end(L_terms);
     std::vector<Launch> L_claim(max_ncb), L_pub4(max_ncb), L_fold(max_ncb);
     double claim_terms = 0, fold_bytes = 0;
     for (int rd = 0; rd < max_ncb; ++rd) {
@@ -767,7 +790,7 @@ bool prove_batch(BatchCtx& B, std::vector<Job*>& jobs_in, TaskPool& tasks, doubl
             if (rd >= S[f].ncb) continue;
             const ClaimShape c = claim_of(S[f], rd, kn.claim_items);
             sc.add(L_claim[rd], f, (int)c.blocks, c.n_chunks, c.gpt);
-            claim_terms += (double)((S[f].cp >> rd) >> 1) * S[f].n_gates;
+            claim_terms += (double)((S[f].cp >> rd) >> 1) * S[f].n_terms;
         }
         sc.end(L_claim[rd]);
         L_pub4[rd] = sc.begin();
@@ -831,6 +854,13 @@ bool prove_batch(BatchCtx& B, std::vector<Job*>& jobs_in, TaskPool& tasks, doubl
         d.op = C.d_op.as<int>();
         d.mux = C.d_mux.as<int>();
         d.gt = C.d_type.as<int>();
+        d.tt = C.d_ttype.as<int>();
+        d.tin0 = C.d_tin0.as<int>();
+        d.tin1 = C.d_tin1.as<int>();
+        d.toff = C.d_toff.as<int>();
+        d.tsrc = C.d_tsrc.as<int>();
+        d.tneg = C.d_tneg.as<int>();
+        d.n_terms = C.n_terms;
         d.in0 = C.d_in0.as<int>();
         d.in1 = C.d_in1.as<int>();
         d.slot = C.d_slot.as<int>();
@@ -986,6 +1016,8 @@ bool prove_batch(BatchCtx& B, std::vector<Job*>& jobs_in, TaskPool& tasks, doubl
     // 6. eq tables (A5)
     if (!pull(SZ1, SER - SZ1)) return false;
     k_b_eq<<<L_eq.blocks, kBlock, 0, st>>>(dD, dI + L_eq.first, L_eq.n, dch, nf);
+    TP_HIP_OK(hipGetLastError());
+    k_b_terms<<<L_terms.blocks, kBlock, 0, st>>>(dD, dI + L_terms.first, L_terms.n);
     TP_HIP_OK(hipGetLastError());
     lap(2);
 

@@ -31,6 +31,7 @@
 #include "frame.hpp"
 #include "layer_bodies.hpp"
 #include "reduce.hpp"
+#include "relay.hpp"
 #include "../../include/truepix_hip.h"
 
 #include <stdint.h>
@@ -74,6 +75,48 @@ __global__ void k_fe_late_init(const u64* __restrict__ vr, const u64* __restrict
 {
     body::late_init(vr, vi, in1, n_gates, z1cr, z1ci, z1lr, z1li, othr, othi, br, bi, beta2,
                     blockIdx.x);
+}
+
+// The 2 nin late rounds of a frame in one resident workgroup (A9:
+// TP/libTruePix/layerprover.py:225-244,282-285, gateprover.py:102-176): each
+// round's three sums go to the host as tagged words (relay.hpp, sequence
+// number seq0 + round); the host's transcript answers with the round's
+// challenge in the answer slot, which the workgroup waits for (bounded by
+// spin_ticks) before the next round applies it; the w2 pass starts with
+// body::late_switch.  One launch instead of one per round plus a switch.
+__global__ void __launch_bounds__(kOneWg) k_fe_late_resident(
+    const int* __restrict__ gt, const int* __restrict__ in0, const int* __restrict__ in1,
+    u64* __restrict__ z1r, u64* __restrict__ z1i, u64* __restrict__ oth_r,
+    u64* __restrict__ oth_i, const u64* __restrict__ vr, const u64* __restrict__ vi, int nin,
+    int n_gates, const u64* __restrict__ beta, fp2* host_vals, unsigned* host_seq,
+    unsigned seq0, const u64* answer, long long spin_ticks)
+{
+    extern __shared__ fp2 lds[];
+    __shared__ fp2 cs[2 * kMaxNin];
+    __shared__ int s_ok;
+    for (int rd = 0; rd < 2 * nin; ++rd) {
+        if (rd == nin) {
+            body::late_switch(vr, vi, nin, cs, n_gates, oth_r, oth_i, lds);
+            __syncthreads();   // the fold buffers are reused below
+        }
+        fp2 out[3];
+        body::late_round(gt, in0, in1, z1r, z1i, oth_r, oth_i, vr, vi, cs, nin, n_gates, rd,
+                         rd > 0 ? 1 : 0, beta, lds, out);
+        if (threadIdx.x == 0) {
+            publish_tagged<3>(host_vals, out, seq0 + (unsigned)rd);
+            int ok = 1;
+            if (rd + 1 < 2 * nin) {
+                u64 a[2];
+                ok = take_answer(answer, seq0 + (unsigned)rd, wall_clock64(), spin_ticks, a);
+                if (ok) cs[rd] = mk2(a[0], a[1]);
+                else __hip_atomic_store(host_seq, kSeqAbort, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            s_ok = ok;
+        }
+        __syncthreads();   // cs[rd] (and every thread's LDS use of this round) settled
+        if (!s_ok) return;
+    }
 }
 
 // Copies n <= 64 device values to the mapped slot, then the sequence number.
@@ -217,12 +260,61 @@ double now_s(Clock::time_point t0)
     return std::chrono::duration<double>(Clock::now() - t0).count();
 }
 
+// Waits for the n Fp2 values a resident kernel publishes as tagged words
+// under sequence number seq (relay.hpp) in the slot's value words, copies
+// them to out and empties the words.  Fails if the kernel gave up waiting
+// (kSeqAbort), the stream reports an error or drains without publishing.
+bool wait_tagged(hipStream_t st, MappedSlot& slot, unsigned seq, int n, fp2* out, const char* what)
+{
+    volatile u64* w = reinterpret_cast<volatile u64*>(slot.vals);
+    const u64 tag = (u64)(seq & 7) << 61;
+    auto ready = [&]() {
+        for (int i = 0; i < 2 * n; ++i) {
+            const u64 x = w[i];
+            if ((x & ~P) != tag || (x & P) == P) return false;
+        }
+        return true;
+    };
+    const auto t0 = Clock::now();
+    for (long spins = 1; !ready(); ++spins) {
+        if (*slot.seq == kSeqAbort) return fail("%s: the device gave up waiting for a challenge", what);
+        if (spins % 4096 == 0) {
+            const hipError_t q = hipStreamQuery(st);
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return fail("%s: stream error %s", what, hipGetErrorString(q));
+            if (q == hipSuccess && !ready()) return fail("%s: result not published", what);
+            if (Clock::now() - t0 > std::chrono::seconds(30))
+                return fail("%s: timed out waiting for the device", what);
+        } else {
+            __builtin_ia32_pause();
+        }
+    }
+    for (int i = 0; i < n; ++i) out[i] = mk2(w[2 * i] & P, w[2 * i + 1] & P);
+    for (int i = 0; i < 2 * n; ++i) w[i] = kSlotEmpty;
+    return true;
+}
+
+// The resident late rounds' bound on each challenge wait: 10 s of the
+// device's wall clock.
+long long late_spin_ticks()
+{
+    static long long ticks = [] {
+        int khz = 0, dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess)
+            khz = 0;
+        return (long long)(khz > 0 ? khz : 100000) * 1000LL * 10;
+    }();
+    return ticks;
+}
+
 bool Shape::set(const Job& J)
 {
     const Circuit& C = *J.circ;
     nin = C.in_bits;
     nout = C.out_bits;
     n_gates = C.n_gates;
+    n_terms = C.n_terms;
     if (J.n_copies < 2) return fail("frame: nCopies must be at least 2");
     ncb = clog2(J.n_copies);
     cp = 1 << ncb;
@@ -695,6 +787,7 @@ struct Worker {
     MappedSlot slot;
     DevBuf raw_in, rows_in, rows_out, vr[2], vi[2], br[2], bi[2];
     DevBuf z1cr, z1ci, z1lr, z1li, othr, othi, partials, small;
+    DevBuf tzr, tzi;   // the claim terms' coefficients
     uint8_t* h_in = nullptr;
     size_t h_in_bytes = 0;
     u64* h_out = nullptr;
@@ -728,7 +821,7 @@ struct Worker {
     {
         DevBuf* all[] = {&raw_in, &rows_in, &rows_out, &vr[0], &vr[1], &vi[0], &vi[1], &br[0],
                          &br[1], &bi[0], &bi[1], &z1cr, &z1ci, &z1lr, &z1li, &othr, &othi,
-                         &partials, &small};
+                         &partials, &small, &tzr, &tzi};
         for (DevBuf* b : all) b->release();
         if (h_in) (void)hipHostFree(h_in);
         if (h_out) (void)hipHostFree(h_out);
@@ -815,19 +908,24 @@ bool prove_frame(Worker& W, Job& J)
     k_fe_eq<<<(C.n_gates + kBlock - 1) / kBlock, kBlock, 0, st>>>(z1a, nout, W.z1cr.as<u64>(),
                                                                  W.z1ci.as<u64>(), C.n_gates);
     TP_HIP_OK(hipGetLastError());
+    if (!W.tzr.ensure((size_t)C.n_terms * 8) || !W.tzi.ensure((size_t)C.n_terms * 8)) return false;
+    kern::k_term_coeffs<<<(C.n_terms * 64 + kBlock - 1) / kBlock, kBlock, 0, st>>>(
+        W.z1cr.as<u64>(), W.z1ci.as<u64>(), C.d_toff.as<int>(), C.d_tsrc.as<int>(),
+        C.d_tneg.as<int>(), C.n_terms, W.tzr.as<u64>(), W.tzi.as<u64>());
+    TP_HIP_OK(hipGetLastError());
     lap(2);
 
     // 4. the rounds (layer loop of circuitnizkvec_vole.py:318-372, one layer)
     const fp2* host = static_cast<const fp2*>(W.slot.vals);
     int cur = 0, nc = cp;
     for (int rd = 0; rd < ncb; ++rd) {                           // early rounds (A6, A7)
-        const ClaimShape cs = claim_shape(C.n_gates, nc >> 1, kClaimItems);
+        const ClaimShape cs = claim_shape(C.n_terms, nc >> 1, kClaimItems);
         const unsigned nblk = cs.blocks;
         if (!W.partials.ensure((size_t)nblk * 4 * sizeof(fp2))) return false;
         fp2* parts = W.partials.as<fp2>();
         kern::k_early_claim<<<nblk, kBlock, 0, st>>>(
-            C.n_gates, cs.gpt, cs.n_chunks, C.d_type.as<int>(), C.d_in0.as<int>(),
-            C.d_in1.as<int>(), W.z1cr.as<u64>(), W.z1ci.as<u64>(), nc, W.vr[cur].as<u64>(),
+            C.n_terms, cs.gpt, cs.n_chunks, C.d_ttype.as<int>(), C.d_tin0.as<int>(),
+            C.d_tin1.as<int>(), W.tzr.as<u64>(), W.tzi.as<u64>(), nc, W.vr[cur].as<u64>(),
             W.vi[cur].as<u64>(), W.br[cur].as<u64>(), W.bi[cur].as<u64>(), parts);
         const unsigned seq = W.slot.advance();
         k_publish_sum<4><<<1, kBlock, 0, st>>>(parts, nblk, static_cast<fp2*>(W.slot.dvals),
@@ -864,24 +962,45 @@ bool prove_frame(Worker& W, Job& J)
     ChalSet ch;
     memset(&ch, 0, sizeof(ch));
     const size_t lds = fold_lds(nin);
-    for (int k = 0; k < 2 * nin; ++k) {
-        if (k == nin) {
-            kern::k_late_switch<<<1, kOneWg, lds, st>>>(Vr, Vi, nin, ch, C.n_gates,
-                                                        W.othr.as<u64>(), W.othi.as<u64>());
-            TP_HIP_OK(hipGetLastError());
-        }
-        const unsigned seq = W.slot.advance();
-        kern::k_late_round<<<1, kOneWg, lds, st>>>(
+    {
+        // the late rounds in one resident workgroup, the host answering each
+        // round's challenge through the mapped answer slot (relay.hpp)
+        volatile u64* hw = reinterpret_cast<volatile u64*>(W.slot.vals);
+        for (int i = 0; i < 6; ++i) hw[i] = kSlotEmpty;
+        W.slot.ans[0] = W.slot.ans[1] = kSlotEmpty;
+        *W.slot.seq = 0;
+        const unsigned seq0 = W.slot.next + 1;
+        W.slot.next += 2 * nin;
+        k_fe_late_resident<<<1, kOneWg, lds, st>>>(
             C.d_type.as<int>(), C.d_in0.as<int>(), C.d_in1.as<int>(), W.z1lr.as<u64>(),
-            W.z1li.as<u64>(), W.othr.as<u64>(), W.othi.as<u64>(), Vr, Vi, ch, nin, C.n_gates, k,
-            k > 0 ? 1 : 0, beta2, static_cast<fp2*>(W.slot.dvals), W.slot.dseq, seq);
+            W.z1li.as<u64>(), W.othr.as<u64>(), W.othi.as<u64>(), Vr, Vi, nin, C.n_gates, beta2,
+            static_cast<fp2*>(W.slot.dvals), W.slot.dseq, seq0, W.slot.dans, late_spin_ticks());
         TP_HIP_OK(hipGetLastError());
-        if (!W.slot.wait(st, seq, "late round")) return false;
-        fp2 f[3];
-        for (int t = 0; t < 3; ++t) f[t] = host[t];
-        fp2 r;
-        if (!pr.late(k, f, r)) return false;
-        ch.c[k] = r;
+        auto answer = [&](int k, fp2 r) {
+            if (k + 1 >= 2 * nin) return;   // the last round's sums end the kernel
+            const u64 tag = (u64)((seq0 + (unsigned)k) & 7) << 61;
+            W.slot.ans[0] = r.re | tag;
+            W.slot.ans[1] = r.im | tag;
+        };
+        for (int k = 0; k < 2 * nin; ++k) {
+            fp2 f[3];
+            if (!wait_tagged(st, W.slot, seq0 + (unsigned)k, 3, f, "late round")) return false;
+            fp2 r;
+            if (!pr.late(k, f, r)) {
+                // answer the kernel's remaining rounds with zeros so that it
+                // ends now rather than at its wait bound, then report
+                const std::string err = tp_last_error();
+                answer(k, zero2());
+                for (int j = k + 1; j < 2 * nin; ++j) {
+                    if (!wait_tagged(st, W.slot, seq0 + (unsigned)j, 3, f, "late round")) break;
+                    answer(j, zero2());
+                }
+                (void)hipStreamSynchronize(st);
+                return fail("%s", err.c_str());
+            }
+            ch.c[k] = r;
+            answer(k, r);
+        }
     }
     lap(4);
 
@@ -1126,9 +1245,16 @@ void* tp_circuit_create(int n_gates, const int* op, const int* muxbit, const int
         return b.ensure(n) && hipMemcpyAsync(b.ptr, src, n, hipMemcpyHostToDevice, st) == hipSuccess;
     };
     const size_t gb = (size_t)n_gates * 4;
+    const ClaimTerms T = build_claim_terms(n_gates, type_idx, in0, in1);
+    C->n_terms = T.n_terms();
+    const size_t tb = (size_t)C->n_terms * 4;
     bool ok = up(C->d_op, op, gb) && up(C->d_mux, muxbit, gb) && up(C->d_type, type_idx, gb) &&
               up(C->d_in0, in0, gb) && up(C->d_in1, in1, gb) &&
               up(C->d_slot, slot.data(), (size_t)NW * 4) && up(C->d_cval, cval.data(), (size_t)NW * 8) &&
+              up(C->d_ttype, T.type.data(), tb) && up(C->d_tin0, T.in0.data(), tb) &&
+              up(C->d_tin1, T.in1.data(), tb) && up(C->d_toff, T.off.data(), tb + 4) &&
+              up(C->d_tsrc, T.src.data(), T.src.size() * 4) &&
+              up(C->d_tneg, T.neg.data(), T.neg.size() * 4) &&
               hipStreamSynchronize(st) == hipSuccess;
     if (!ok) {
         if (!tp_last_error_set()) fail("tp_circuit_create: upload failed");
@@ -1224,7 +1350,9 @@ void circuit_unref(const Circuit* c)
     if (!c) return;
     auto* C = const_cast<Circuit*>(c);
     if (C->refs.fetch_sub(1, std::memory_order_acq_rel) != 1) return;
-    DevBuf* all[] = {&C->d_op, &C->d_mux, &C->d_type, &C->d_in0, &C->d_in1, &C->d_slot, &C->d_cval};
+    DevBuf* all[] = {&C->d_op,    &C->d_mux,   &C->d_type,  &C->d_in0,  &C->d_in1,
+                     &C->d_slot,  &C->d_cval,  &C->d_ttype, &C->d_tin0, &C->d_tin1,
+                     &C->d_toff,  &C->d_tsrc,  &C->d_tneg};
     for (DevBuf* b : all) b->release();
     delete C;
 }

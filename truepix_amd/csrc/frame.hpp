@@ -9,6 +9,7 @@
 // same steps in the same order, so a frame's proof bytes do not depend on the
 // driver, the batch it was in or the frames beside it.
 #pragma once
+#include "claim_terms.hpp"
 #include "fp61.hpp"
 #include "host_sha.hpp"
 #include "kernels.hpp"
@@ -46,6 +47,10 @@ struct Circuit {
     int n_gates = 0, in_bits = 0, out_bits = 0, n_inputs = 0, n_priv = 0;
     std::vector<int> type, in0, in1;
     DevBuf d_op, d_mux, d_type, d_in0, d_in1, d_slot, d_cval;
+    // the early claim's terms (claim_terms.hpp): type, wires, and the CSR of
+    // their coefficients' z1 contributions
+    int n_terms = 0;
+    DevBuf d_ttype, d_tin0, d_tin1, d_toff, d_tsrc, d_tneg;
     std::atomic<int> refs{1};
 };
 
@@ -87,6 +92,7 @@ struct Job {
 // Sizes of one frame.
 struct Shape {
     int nin = 0, nout = 0, ncb = 0, cp = 0, NW = 0, NO = 0, n_z = 0, n_gates = 0;
+    int n_terms = 0;   // early-claim terms (claim_terms.hpp)
     size_t rows_n = 0, out_n = 0, in_bytes = 0;
     // false (with the error recorded) for frames this build does not take
     bool set(const Job& J);

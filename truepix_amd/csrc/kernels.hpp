@@ -55,12 +55,16 @@ __global__ void k_wiring_mlext(const int* type_idx, const int* in0, const int* i
 __global__ void k_mle_base_dev(const u64* in, PointArg z, int n_z, fp2* partials);
 
 // native_fp2.hip --------------------------------------------------------------
-// A6: early-round claim partials (4 accumulators per workgroup), 1-D grid of
-// ClaimShape::blocks workgroups.
+// A6: early-round claim partials (4 accumulators per workgroup) over the
+// layer's claim terms (claim_terms.hpp), 1-D grid of ClaimShape::blocks
+// workgroups.
 __global__ void k_early_claim(int n_gates, int gpt, int n_chunks, const int* gt, const int* in0,
                               const int* in1, const u64* z1r, const u64* z1i, int nc,
                               const u64* vr, const u64* vi, const u64* br, const u64* bi,
                               fp2* partials);
+// Claim-term coefficients from z1 (one wave per term, body::term_coeffs).
+__global__ void k_term_coeffs(const u64* z1r, const u64* z1i, const int* off, const int* src,
+                              const int* neg, int n_terms, u64* tzr, u64* tzi);
 // A7: fold every wire row and beta with r.
 __global__ void k_fold_rows(const u64* ir, const u64* ii, int n_wires, int log_out, u64 rr, u64 ri,
                             u64* orr, u64* oi, const u64* bir, const u64* bii, u64* bor, u64* boi);

@@ -8,6 +8,7 @@
 // arithmetic, so a frame's tables and sums do not depend on which driver ran
 // it.
 #pragma once
+#include "claim_terms.hpp"
 #include "fp61.hpp"
 #include "reduce.hpp"
 
@@ -175,15 +176,22 @@ __device__ __forceinline__ void eq_table(const fp2* z, int n, u64* outr, u64* ou
     outi[k] = b.im;
 }
 
-// A6: early-round claim.  Work items (pair p, gate chunk q) with p fastest:
-// item = bid * blockDim + tid < n_pairs * n_chunks, chunk q covering gates
-// [q gpt, min(n_gates, (q + 1) gpt)); small rounds (few pairs) thus still fill
-// whole workgroups.  The workgroup total of (F(0), F(1), F(-1), F(2)) goes to
-// partials[pidx] (TP/native_fp2/native_fp2.cu:232-281 semantics, SoA
-// [wire][copy] tables; nc a power of two).
-__device__ __forceinline__ void early_claim(int n_gates, int gpt, int n_chunks, const int* gt,
-                                            const int* in0, const int* in1, const u64* z1r,
-                                            const u64* z1i, int nc, const u64* vr, const u64* vi,
+// A6: early-round claim over a term table (claim_terms.hpp): a term is a
+// gate that is not affine in its inputs (type 0, 5, 6, 8-11, coefficient its
+// z1 value), an input wire (kTermWire, coefficient the z1-weighted sum of the
+// affine gates that read it, so the claim's affine part is one product per
+// wire instead of four per gate) or the affine gates' constant (kTermConst).
+// The affine part L(t) is linear in t, so only t = 0, 1 are formed and
+// L(-1) = 2 L(0) - L(1), L(2) = 2 L(1) - L(0).
+// Work items (pair p, term chunk q) with p fastest: item = bid * blockDim +
+// tid < n_pairs * n_chunks, chunk q covering terms [q gpt, min(n_terms,
+// (q + 1) gpt)); small rounds (few pairs) thus still fill whole workgroups.
+// The workgroup total of (F(0), F(1), F(-1), F(2)) goes to partials[pidx]
+// (TP/native_fp2/native_fp2.cu:232-281 semantics, SoA [wire][copy] tables;
+// nc a power of two).
+__device__ __forceinline__ void early_claim(int n_terms, int gpt, int n_chunks, const int* gt,
+                                            const int* in0, const int* in1, const u64* tzr,
+                                            const u64* tzi, int nc, const u64* vr, const u64* vi,
                                             const u64* br, const u64* bi, fp2* partials,
                                             size_t bid, size_t pidx)
 {
@@ -194,21 +202,35 @@ __device__ __forceinline__ void early_claim(int n_gates, int gpt, int n_chunks, 
     if (item < ((size_t)n_chunks << lp)) {
         const int p = (int)(item & (size_t)(n_pairs - 1));
         const int g0 = (int)(item >> lp) * gpt;
-        const int g1 = min(n_gates, g0 + gpt);
+        const int g1 = min(n_terms, g0 + gpt);
         const size_t c = 2 * (size_t)p;
         fp2 v0 = zero2(), v1 = zero2(), v2 = zero2(), v3 = zero2();
+        fp2 l0 = zero2(), l1 = zero2();
 #pragma unroll 2
         for (int g = g0; g < g1; ++g) {
             const int tpg = gt[g];
-            const size_t a = (size_t)in0[g] * nc + c, b = (size_t)in1[g] * nc + c;
-            const fp2 z = mk2(z1r[g], z1i[g]);
-            const fp2 a0 = mk2(vr[a], vi[a]), a1 = mk2(vr[a + 1], vi[a + 1]);
-            const fp2 b0 = mk2(vr[b], vi[b]), b1 = mk2(vr[b + 1], vi[b + 1]);
-            v0 = add2(v0, mul2(gatefn(tpg, a0, b0), z));
-            v1 = add2(v1, mul2(gatefn(tpg, a1, b1), z));
-            v2 = add2(v2, mul2(gatefn(tpg, sub2(dbl2(a0), a1), sub2(dbl2(b0), b1)), z));
-            v3 = add2(v3, mul2(gatefn(tpg, sub2(dbl2(a1), a0), sub2(dbl2(b1), b0)), z));
+            const fp2 z = mk2(tzr[g], tzi[g]);
+            const size_t a = (size_t)in0[g] * nc + c;
+            if (tpg == kTermWire) {
+                l0 = add2(l0, mul2(mk2(vr[a], vi[a]), z));
+                l1 = add2(l1, mul2(mk2(vr[a + 1], vi[a + 1]), z));
+            } else if (tpg == kTermConst) {
+                l0 = add2(l0, z);
+                l1 = add2(l1, z);
+            } else {
+                const size_t b = (size_t)in1[g] * nc + c;
+                const fp2 a0 = mk2(vr[a], vi[a]), a1 = mk2(vr[a + 1], vi[a + 1]);
+                const fp2 b0 = mk2(vr[b], vi[b]), b1 = mk2(vr[b + 1], vi[b + 1]);
+                v0 = add2(v0, mul2(gatefn(tpg, a0, b0), z));
+                v1 = add2(v1, mul2(gatefn(tpg, a1, b1), z));
+                v2 = add2(v2, mul2(gatefn(tpg, sub2(dbl2(a0), a1), sub2(dbl2(b0), b1)), z));
+                v3 = add2(v3, mul2(gatefn(tpg, sub2(dbl2(a1), a0), sub2(dbl2(b1), b0)), z));
+            }
         }
+        v0 = add2(v0, l0);
+        v1 = add2(v1, l1);
+        v2 = add2(v2, sub2(dbl2(l0), l1));
+        v3 = add2(v3, sub2(dbl2(l1), l0));
         const fp2 be0 = mk2(br[c], bi[c]), be1 = mk2(br[c + 1], bi[c + 1]);
         acc[0] = mul2(v0, be0);
         acc[1] = mul2(v1, be1);
@@ -216,6 +238,30 @@ __device__ __forceinline__ void early_claim(int n_gates, int gpt, int n_chunks, 
         acc[3] = mul2(v3, sub2(dbl2(be1), be0));
     }
     block_partial_at<4>(acc, partials, pidx);
+}
+
+// The claim terms' coefficients from the gates' z1 values: term k's is the
+// signed sum of z1 over its contributions [off[k], off[k + 1]) (gate src[e],
+// subtracted when neg[e]).  One wave per term (a wire read by many affine
+// gates has many contributions: the gray circuit's constant wire 128):
+// wave k = (bid * blockDim + tid) / 64, its lanes striding the contributions.
+__device__ __forceinline__ void term_coeffs(const u64* z1r, const u64* z1i, const int* off,
+                                            const int* src, const int* neg, int n_terms,
+                                            u64* tzr, u64* tzi, size_t bid)
+{
+    const size_t k = (bid * blockDim.x + threadIdx.x) >> 6;   // the same for the whole wave
+    const int lane = threadIdx.x & 63;
+    if (k >= (size_t)n_terms) return;
+    fp2 acc[1] = {zero2()};
+    for (int e = off[k] + lane; e < off[k + 1]; e += 64) {
+        const fp2 z = mk2(z1r[src[e]], z1i[src[e]]);
+        acc[0] = neg[e] ? sub2(acc[0], z) : add2(acc[0], z);
+    }
+    wave_sum<1>(acc);
+    if (lane == 0) {
+        tzr[k] = acc[0].re;
+        tzi[k] = acc[0].im;
+    }
 }
 
 // A7: fold every wire row (n_wires rows of 2^(log_out + 1) entries) and the

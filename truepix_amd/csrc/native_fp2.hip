@@ -11,6 +11,7 @@
 //    (per-workgroup partials, then a one-workgroup finalize launch);
 //  * beta and the input MLE are built from LDS-resident half tables in one
 //    launch instead of n_z doubling launches.
+#include "claim_terms.hpp"
 #include "fp61.hpp"
 #include "kernels.hpp"
 #include "layer_bodies.hpp"
@@ -26,9 +27,9 @@ namespace tp {
 namespace kern {
 
 // Early-round claim over resident [wire][nc] tables (session form,
-// TP/native_fp2/native_fp2_core.hpp:200-248).  Thread (pair p, gate chunk
-// blockIdx.y) sums z1[g] * gate_g at t = 0, 1, -1, 2 over its gates, scales by
-// beta_t(p) and joins the grid sum.  V(-1) = 2a - b and V(2) = 2b - a.
+// TP/native_fp2/native_fp2_core.hpp:200-248), over the layer's claim terms
+// (body::early_claim): thread (pair p, term chunk) sums its terms at
+// t = 0, 1, -1, 2, scales by beta_t(p) and joins the grid sum.
 __global__ void k_early_claim(int n_gates, int gpt, int n_chunks, const int* __restrict__ gt,
                               const int* __restrict__ in0, const int* __restrict__ in1,
                               const u64* __restrict__ z1r, const u64* __restrict__ z1i, int nc,
@@ -38,6 +39,16 @@ __global__ void k_early_claim(int n_gates, int gpt, int n_chunks, const int* __r
 {
     body::early_claim(n_gates, gpt, n_chunks, gt, in0, in1, z1r, z1i, nc, vr, vi, br, bi, partials,
                       blockIdx.x, blockIdx.x);
+}
+
+// Claim-term coefficients from a frame's z1 values (body::term_coeffs: one
+// wave per term, so n_terms * 64 threads).
+__global__ void k_term_coeffs(const u64* __restrict__ z1r, const u64* __restrict__ z1i,
+                              const int* __restrict__ off, const int* __restrict__ src,
+                              const int* __restrict__ neg, int n_terms, u64* __restrict__ tzr,
+                              u64* __restrict__ tzi)
+{
+    body::term_coeffs(z1r, z1i, off, src, neg, n_terms, tzr, tzi, blockIdx.x);
 }
 
 // Fold every wire row [n_wires][n_in] -> [n_wires][n_in/2] and beta in one
@@ -345,6 +356,33 @@ bool upload_gates(DevBuf& gt, DevBuf& i0, DevBuf& i1, DevBuf& zr, DevBuf& zi, in
            h2d(i1, in1, n_gates * 4) && h2d(zr, z1_r, n_gates * 8) && h2d(zi, z1_i, n_gates * 8);
 }
 
+// A session's claim terms (claim_terms.hpp) with their coefficients formed
+// here from the caller's z1 (canonical Fp2 sums); returns the term count, or
+// -1 with the error recorded.
+int upload_terms(DevBuf& gt, DevBuf& i0, DevBuf& i1, DevBuf& zr, DevBuf& zi, int n_gates,
+                 const int* gate_type, const int* in0, const int* in1, const u64* z1_r,
+                 const u64* z1_i)
+{
+    const ClaimTerms T = build_claim_terms(n_gates, gate_type, in0, in1);
+    const int n = T.n_terms();
+    std::vector<u64> tr(n), ti(n);
+    for (int k = 0; k < n; ++k) {
+        fp2 acc = zero2();
+        for (int e = T.off[k]; e < T.off[k + 1]; ++e) {
+            const fp2 z = mk2(z1_r[T.src[e]] % P, z1_i[T.src[e]] % P);
+            acc = T.neg[e] ? sub2(acc, z) : add2(acc, z);
+        }
+        tr[k] = acc.re;
+        ti[k] = acc.im;
+    }
+    if (!upload_gates(gt, i0, i1, zr, zi, n, T.type.data(), T.in0.data(), T.in1.data(), tr.data(),
+                      ti.data()))
+        return -1;
+    // the vectors above are host memory the async copies read: wait for them
+    if (hipStreamSynchronize(stream()) != hipSuccess) return fail("early_create: upload failed"), -1;
+    return n;
+}
+
 void store8(const fp2 res[4], u64 out8[8])
 {
     for (int t = 0; t < 4; ++t) { out8[2 * t] = res[t].re; out8[2 * t + 1] = res[t].im; }
@@ -394,6 +432,7 @@ bool gpu_sumcheck_early(int n_gates, const int* gate_type, const int* in0, const
 
 struct EarlySession {
     int n_gates = 0, n_wires = 0, n_copies = 0;
+    int n_terms = 0;                      // claim terms (claim_terms.hpp) in gt, i0, i1, z1r, z1i
     bool ping = false;                    // which of (val, val2) is current
     DevBuf gt, i0, i1, z1r, z1i;
     DevBuf val_r, val_i, val2_r, val2_i;
@@ -419,11 +458,11 @@ bool session_claim(EarlySession* s, u64 out8[8])
 {
     const int n_pairs = s->n_copies >> 1;
     if (n_pairs < 1) return fail("early_claim: session already folded to one copy");
-    const ClaimShape cs = claim_shape(s->n_gates, n_pairs, kClaimItems);
+    const ClaimShape cs = claim_shape(s->n_terms, n_pairs, kClaimItems);
     RedView rv;
     if (!red_view(s->red, cs.blocks, 4, rv)) return false;
     kern::k_early_claim<<<cs.blocks, kBlock, 0, stream()>>>(
-        s->n_gates, cs.gpt, cs.n_chunks, s->gt.as<int>(), s->i0.as<int>(), s->i1.as<int>(),
+        s->n_terms, cs.gpt, cs.n_chunks, s->gt.as<int>(), s->i0.as<int>(), s->i1.as<int>(),
         s->z1r.as<u64>(), s->z1i.as<u64>(), s->n_copies, s->cur_vr(), s->cur_vi(), s->cur_br(),
         s->cur_bi(), rv.partials);
     if (!launch_ok() || !finalize<4>(rv, cs.blocks)) return false;
@@ -486,8 +525,9 @@ EarlySession* session_create(int n_gates, const int* gate_type, const int* in0, 
     s->n_wires = n_wires;
     s->n_copies = n_copies;
     const size_t vn = (size_t)n_wires * n_copies;
-    bool ok = upload_gates(s->gt, s->i0, s->i1, s->z1r, s->z1i, n_gates, gate_type, in0, in1,
-                           z1_r, z1_i) &&
+    s->n_terms = upload_terms(s->gt, s->i0, s->i1, s->z1r, s->z1i, n_gates, gate_type, in0, in1,
+                              z1_r, z1_i);
+    bool ok = s->n_terms > 0 &&
               h2d(s->val_r, v_r, vn * 8) && h2d(s->val_i, v_i, vn * 8) &&
               s->val2_r.ensure(vn * 8 / 2) && s->val2_i.ensure(vn * 8 / 2) &&
               h2d(s->beta_r, b_r, n_copies * 8) && h2d(s->beta_i, b_i, n_copies * 8) &&
@@ -522,8 +562,9 @@ EarlySession* session_create_dev(int n_gates, const int* gate_type, const int* i
     s->n_wires = n_wires;
     s->n_copies = n_copies;
     const size_t vn = (size_t)n_wires * n_copies;
-    bool ok = upload_gates(s->gt, s->i0, s->i1, s->z1r, s->z1i, n_gates, gate_type, in0, in1,
-                           z1_r, z1_i) &&
+    s->n_terms = upload_terms(s->gt, s->i0, s->i1, s->z1r, s->z1i, n_gates, gate_type, in0, in1,
+                              z1_r, z1_i);
+    bool ok = s->n_terms > 0 &&
               s->val_r.ensure(vn * 8) && s->val_i.ensure(vn * 8) &&
               s->val2_r.ensure(vn * 8 / 2) && s->val2_i.ensure(vn * 8 / 2) &&
               h2d(s->beta_r, b_r, n_copies * 8) && h2d(s->beta_i, b_i, n_copies * 8) &&

@@ -110,6 +110,9 @@ bool MappedSlot::ready()
     *seq = 0;
     dvals = dv;
     dseq = reinterpret_cast<unsigned*>(static_cast<char*>(dv) + 1024);
+    ans = reinterpret_cast<volatile uint64_t*>(static_cast<char*>(hv) + 1536);
+    dans = reinterpret_cast<uint64_t*>(static_cast<char*>(dv) + 1536);
+    ans[0] = ans[1] = ~0ull;
     next = 0;
     return true;
 }
@@ -137,6 +140,8 @@ void MappedSlot::release()
     vals = dvals = nullptr;
     seq = nullptr;
     dseq = nullptr;
+    ans = nullptr;
+    dans = nullptr;
 }
 
 }  // namespace tp

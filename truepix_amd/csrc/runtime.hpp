@@ -52,6 +52,8 @@ struct MappedSlot {
     volatile unsigned* seq = nullptr;     // host view of the sequence number
     void* dvals = nullptr;                // device views of the same memory
     unsigned* dseq = nullptr;
+    volatile uint64_t* ans = nullptr;     // host view of a two-word answer slot
+    uint64_t* dans = nullptr;             // (relay.hpp), and its device view
     unsigned next = 0;
 
     bool ready();                         // allocates on first use

@@ -47,6 +47,7 @@
 #include "fp61.hpp"
 #include "host_sha.hpp"
 #include "reduce.hpp"
+#include "relay.hpp"
 #include "runtime.hpp"
 #include "transcript.hpp"
 
@@ -66,7 +67,6 @@ struct alignas(16) V2 { u64 re, im; };
 __device__ __forceinline__ fp2 ldv(const V2* p) { const V2 v = *p; return mk2(v.re, v.im); }
 __device__ __forceinline__ void stv(V2* p, fp2 x) { V2 v; v.re = x.re; v.im = x.im; *p = v; }
 
-constexpr unsigned kSeqAbort = 0xffffffffu;   // a kernel's bounded wait expired
 
 // Arguments of a launch that publishes its own round sums (ticket_publish).
 struct TicketPub {
@@ -77,26 +77,6 @@ struct TicketPub {
     unsigned seq;
     long long spin_ticks;
 };
-
-constexpr u64 kSlotEmpty = ~0ull;   // value bits p under tag 7: no round's word
-
-// A round's sums to the host's mapped slot: every word carries (seq mod 8)
-// << 61 above its canonical value, so the host checks each word itself and
-// no fence has to order the values before a sequence word (a system fence
-// cost about 0.4 us a round).  The host empties the slot (all ones) once it
-// has read it, before it answers or launches again, so a stale word never
-// passes for a later round.  Aborts still go to the sequence word.
-template <int N>
-__device__ __forceinline__ void publish_tagged(fp2* host_vals, const fp2 (&v)[N], unsigned seq)
-{
-    const u64 tag = (u64)(seq & 7) << 61;
-    u64* w = reinterpret_cast<u64*>(host_vals);
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        __hip_atomic_store(w + 2 * k, v[k].re | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        __hip_atomic_store(w + 2 * k + 1, v[k].im | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-}
 
 // End of a streaming launch, in place of a separate publish launch (which
 // cost its own launch and a dependent-launch gap per round).  Each
@@ -398,32 +378,6 @@ __global__ void __launch_bounds__(kFoldThreads) k_sc2_fold_round(const V2* __res
 constexpr int kTailThreads = 1024;
 constexpr unsigned kTailIn = 8192;
 constexpr size_t kTailLds = (size_t)kTailIn * sizeof(V2);   // both tables, T_{k0}
-
-// The host's answer to the round published as `want`, from the mapped answer
-// slot: the host writes both words with (want mod 8) << 61 above the
-// canonical value, so one read of the slot is both the check and the value
-// (a separate sequence word cost a second PCIe round trip).  A word of
-// another round, or the empty mark (all ones: value bits p), does not match.
-// The host empties the slot when a prove starts and each answer overwrites
-// the one before; answered rounds in a prove are never 8 sequence numbers
-// apart with no answer between, so a stale answer never matches.  Returns 0
-// if the wait expires.
-__device__ __forceinline__ int take_answer(const u64* answer, unsigned want, long long t0,
-                                           long long spin_ticks, u64 (&rv)[2])
-{
-    const u64 tag = (u64)(want & 7) << 61;
-    for (;;) {
-        const u64 a0 = __hip_atomic_load(answer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        const u64 a1 = __hip_atomic_load(answer + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        if ((a0 & ~P) == tag && (a1 & ~P) == tag && (a0 & P) != P && (a1 & P) != P) {
-            rv[0] = a0 & P;
-            rv[1] = a1 & P;
-            return 1;
-        }
-        if (wall_clock64() - t0 > spin_ticks) return 0;
-        __builtin_amdgcn_s_sleep(1);
-    }
-}
 
 // Lane 0 publishes a round's sums tagged with sequence number `want` and
 // waits for the host's answer; returns 0 (and publishes kSeqAbort) if the
