@@ -114,6 +114,13 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
                  void* stream);
 /* Per-launch HIP-event timing of the fused fold+round kernel (off by default). */
 void tp_sc2_set_timing(int on);
+
+/* 1: an untimed prove queues its launches down to the sliced mid launch at
+ * once, each taking its challenges from the mapped answer slot, so only the
+ * host's transcript steps sit between rounds; 0 (default, measured as fast):
+ * each launch is made after the transcript step before it (timed proves
+ * always are).  The proof bytes do not depend on it. */
+void tp_sc2_set_ahead(int on);
 /* Summed event milliseconds, algorithmic bytes and launch count of the
  * streaming fused fold+round launches in the last tp_sc2_prove (the rounds
  * before the sliced ones; see tp_sc2_schedule). */

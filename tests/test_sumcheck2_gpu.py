@@ -144,3 +144,23 @@ def test_sc2_mid_gate_fallback(bridge, oracle, torch_cuda):
     assert fallbacks in (0, 1)
     got2 = sc.prove_deg2_host(f, g, seed=77)
     assert sc.mid_fallbacks() == 0 and np.array_equal(got2.polys, want[0])
+
+
+@pytest.mark.parametrize("nv", [3, 12, 21, 23])
+def test_sc2_queued_and_serial_launches_agree(bridge, oracle, torch_cuda, nv):
+    """Launches queued ahead of their transcript steps (challenges through the
+    answer slot and the device relay) and launches made one after another
+    give the oracle's proof, over sizes with and without streaming rounds
+    and the sliced mid launch."""
+    rng = np.random.default_rng(100 + nv)
+    f, g = rand_fp2(rng, 1 << nv), rand_fp2(rng, 1 << nv)
+    want = oracle.sc2_prove(f, g, transcript_chal(9), nthreads=8)
+    try:
+        for ahead in (False, True, False, True):
+            sc.set_ahead(ahead)
+            got = sc.prove_deg2_host(f, g, seed=9)
+            assert np.array_equal(got.polys, want[0]), ahead
+            assert np.array_equal(got.chals, want[1]), ahead
+            assert np.array_equal(got.final, want[2]), ahead
+    finally:
+        sc.set_ahead(False)

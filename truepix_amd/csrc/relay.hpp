@@ -15,6 +15,7 @@
 namespace tp {
 
 constexpr unsigned kSeqAbort = 0xffffffffu;   // a kernel's bounded wait expired
+constexpr unsigned kSeqHostAbort = 0xfffffffdu;   // the host gave up: waiting launches leave
 constexpr u64 kSlotEmpty = ~0ull;   // value bits p under tag 7: no round's word
 
 // A round's sums to the host's mapped slot: every word carries (seq mod 8)

@@ -76,7 +76,71 @@ struct TicketPub {
     unsigned* host_seq;  // mapped: then seq, or kSeqAbort if a wait expired
     unsigned seq;
     long long spin_ticks;
+    // A launch queued ahead of its transcript step takes its n_ans challenge
+    // words from the mapped answer slot, tagged with sequence number `want`
+    // (launch_challenges); n_ans = 0: they are kernel arguments.
+    const u64* answer;
+    unsigned want;
+    int n_ans;
+    u64* relay;          // device: workgroup 0's copy of the answer for the others; all ones when idle
 };
+
+// The challenge words of a streaming launch: the arguments in w already, or,
+// for a launch queued ahead of its transcript step, n_ans words the host
+// writes into the mapped answer slot tagged (want mod 8) << 61 (relay.hpp).
+// Only workgroup 0 reads the host's slot (a read of host memory crosses
+// PCIe; a thousand workgroups polling it slowed the host's own accesses):
+// it copies the tagged words into the device relay, which the other
+// workgroups poll (agent scope).  The launch's last workgroup empties the
+// relay (ticket_publish), so the next launch starts from an idle one.  The
+// words are shared through LDS.  False in every thread if a wait expired or
+// the host marked the prove failed (kSeqHostAbort in the sequence word): the
+// launch then ends without taking a ticket or publishing.
+__device__ __forceinline__ bool launch_challenges(const TicketPub& a, u64 (&w)[4])
+{
+    if (a.n_ans == 0) return true;
+    __shared__ u64 s_w[4];
+    __shared__ int s_ok;
+    if (threadIdx.x == 0) {
+        const u64 tag = (u64)(a.want & 7) << 61;
+        const bool first = blockIdx.x == 0;
+        const u64* src = first ? a.answer : a.relay;
+        const long long t0 = wall_clock64();
+        int ok = 0;
+        for (unsigned it = 0;; ++it) {
+            u64 v[4];
+            bool all = true;
+            for (int i = 0; i < a.n_ans; ++i) {
+                v[i] = first ? __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)
+                             : __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                all = all && (v[i] & ~P) == tag && (v[i] & P) != P;
+            }
+            if (all) {
+                for (int i = 0; i < a.n_ans; ++i) {
+                    s_w[i] = v[i] & P;
+                    if (first)
+                        __hip_atomic_store(a.relay + i, v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                ok = 1;
+                break;
+            }
+            if ((first || (it & 255) == 255) &&
+                __hip_atomic_load(a.host_seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) ==
+                    kSeqHostAbort)
+                break;
+            if (wall_clock64() - t0 > a.spin_ticks) {
+                __hip_atomic_store(a.host_seq, kSeqAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    if (!s_ok) return false;
+    for (int i = 0; i < a.n_ans; ++i) w[i] = s_w[i];
+    return true;
+}
 
 // End of a streaming launch, in place of a separate publish launch (which
 // cost its own launch and a dependent-launch gap per round).  Each
@@ -143,14 +207,20 @@ __device__ __forceinline__ void ticket_publish(fp2 (&v)[N], const TicketPub& a)
     ok = __syncthreads_and(ok);   // every word is read before the slots are cleared
     for (unsigned w = t; w < nb * 2 * N; w += blockDim.x)
         __hip_atomic_store(a.slots + w, kSlotEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // every workgroup took its ticket after its challenges: the relay is idle
+    // again (the first pass of a prove, which waits for none, clears what an
+    // earlier prove may have left)
+    if (a.relay && t < 4)
+        __hip_atomic_store(a.relay + t, kSlotEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (g < G) s_g[k][g] = acc;
     __syncthreads();
-    const unsigned wid = t >> 6, lane = t & 63;
-    if (wid < (unsigned)N) {   // wave k adds the G partials of sum k
+    const unsigned wid = t >> 6, lane = t & 63, nwv = blockDim.x >> 6;
+    // wave w adds the G partials of sums w, w + nwv, ... (any workgroup size)
+    for (unsigned q = wid; q < (unsigned)N; q += nwv) {
         fp2 sv[1] = {zero2()};
-        for (unsigned x = lane; x < G; x += 64) sv[0] = add2(sv[0], s_g[wid][x]);
+        for (unsigned x = lane; x < G; x += 64) sv[0] = add2(sv[0], s_g[q][x]);
         wave_sum<1>(sv);
-        if (lane == 0) s_tot[wid] = sv[0];
+        if (lane == 0) s_tot[q] = sv[0];
     }
     __syncthreads();
     if (t == 0) {
@@ -310,7 +380,9 @@ __global__ void __launch_bounds__(kFoldThreads) k_sc2_fold01_round(const V2* __r
                                                           V2* __restrict__ fout,
                                                           V2* __restrict__ gout, TicketPub pub)
 {
-    const fp2 r0 = mk2(r0r, r0i), r1 = mk2(r1r, r1i);
+    u64 rw[4] = {r0r, r0i, r1r, r1i};
+    if (!launch_challenges(pub, rw)) return;
+    const fp2 r0 = mk2(rw[0], rw[1]), r1 = mk2(rw[2], rw[3]);
     const bool even = (threadIdx.x & 1) == 0;
     LazyProdSum z[3];
     lazy_zero(z);
@@ -344,7 +416,9 @@ __global__ void __launch_bounds__(kFoldThreads) k_sc2_fold_round(const V2* __res
                                                         V2* __restrict__ gout, TicketPub pub,
                                                         int desc)
 {
-    const fp2 r = mk2(rr, ri);
+    u64 rw[4] = {rr, ri, 0, 0};
+    if (!launch_challenges(pub, rw)) return;
+    const fp2 r = mk2(rw[0], rw[1]);
     LazyProdSum z[3];
     lazy_zero(z);
     for (size_t q0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q0 < n_quads;
@@ -564,7 +638,8 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
     const V2* __restrict__ fin, const V2* __restrict__ gin, unsigned slice_in, int n_rounds,
     u64 rr, u64 ri, u64* parts, fp2* host_vals, unsigned* host_seq, unsigned seq0,
     u64* answer, long long spin_ticks, V2* __restrict__ fout,
-    V2* __restrict__ gout, u64* relay, long long* trace, unsigned* gate, long long gate_ticks)
+    V2* __restrict__ gout, u64* relay, long long* trace, unsigned* gate, long long gate_ticks,
+    unsigned r_want, int r_from_slot, u64* arelay)
 {
     extern __shared__ V2 tl[];
     __shared__ u64 s_r[2];
@@ -618,6 +693,21 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
     }
     __syncthreads();
     if (!s_ok) return;
+    // queued ahead of its transcript step: the challenge of the round before
+    // the first comes through the answer slot, as a streaming launch's does
+    if (r_from_slot) {
+        TicketPub a;
+        a.host_seq = host_seq;
+        a.spin_ticks = spin_ticks;
+        a.answer = answer;
+        a.want = r_want;
+        a.n_ans = 2;
+        a.relay = arelay;
+        u64 rw[4] = {0, 0, 0, 0};
+        if (!launch_challenges(a, rw)) return;
+        rr = rw[0];
+        ri = rw[1];
+    }
     const V2* sf0 = fin + (size_t)b * slice_in;
     const V2* sg0 = gin + (size_t)b * slice_in;
     fp2 r = mk2(rr, ri);
@@ -727,6 +817,8 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
         // no reset launch of its own.
         for (int j = lane; j < kMidGroups * 6; j += 64)
             __hip_atomic_store(parts + j, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane < 4 && r_from_slot)   // every workgroup took the first challenge long ago
+            __hip_atomic_store(arelay + lane, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (lane == 0) {
             __hip_atomic_store(relay, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(relay + 1, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -893,6 +985,8 @@ struct Sc2State {
     int launches[SC_NCLASS] = {};
     DevBuf mid_parts;            // the mid rounds' tagged partials, [kMidGroups][6]
     DevBuf relay;                // the mid rounds' challenge relay (2 words)
+    DevBuf arelay;               // queued launches' answer relay (4 words, launch_challenges)
+    bool ahead = false;          // untimed proves queue their launches ahead (tp_sc2_set_ahead)
     DevBuf gate;                 // the mid launch's arrival gate (2 words)
     fp2 pub[16];                 // the last published round's values, read out of the slot
     DevBuf slots;                // the streaming launches' tagged sums (ticket_publish)
@@ -959,6 +1053,7 @@ bool sc2_setup(size_t n)
     if (!S.work.ensure(2 * (n / 2 + n / 4 + 2) * sizeof(V2))) return false;
     if (!S.partials.ensure(2048 * 9 * sizeof(fp2))) return false;
     if (!S.relay.ensure(2 * sizeof(u64))) return false;
+    if (!S.arelay.ensure(4 * sizeof(u64))) return false;
     if (!S.mid_parts.ensure(kMidGroups * 6 * sizeof(u64))) return false;
     if (!S.gate.ensure(2 * sizeof(unsigned))) return false;
     if (!S.slots.ensure(kSlotWords * sizeof(u64))) return false;
@@ -1092,17 +1187,18 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         // first prove, or the last one stopped part way: fresh slots and ticket
         TP_HIP_OK(hipMemsetAsync(S.slots.ptr, 0xff, kSlotWords * sizeof(u64), st));
         TP_HIP_OK(hipMemsetAsync(S.ticket.ptr, 0, 256, st));
+        TP_HIP_OK(hipMemsetAsync(S.arelay.ptr, 0xff, 4 * sizeof(u64), st));
     }
     S.slots_clean = false;   // until this prove returns 0
     {
         // no answer pending, no values published, no abort mark
         volatile u64* ans = S.host_answer;
-        ans[0] = ans[1] = kSlotEmpty;
+        ans[0] = ans[1] = ans[2] = ans[3] = kSlotEmpty;
         volatile u64* w = reinterpret_cast<volatile u64*>(S.host_vals);
         for (int i = 0; i < 32; ++i) w[i] = kSlotEmpty;
         *reinterpret_cast<volatile unsigned*>(S.host_seq) = 0;
     }
-    auto pub_for = [&](unsigned seq) {
+    auto pub_for = [&](unsigned seq, unsigned want = 0, int n_ans = 0) {
         TicketPub a;
         a.slots = S.slots.as<u64>();
         a.ticket = S.ticket.as<unsigned>();
@@ -1110,7 +1206,22 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         a.host_seq = S.dev_seq;
         a.seq = seq;
         a.spin_ticks = S.spin_ticks;
+        a.answer = S.dev_answer;
+        a.want = want;
+        a.n_ans = n_ans;
+        a.relay = S.arelay.as<u64>();
         return a;
+    };
+    // The host's answer for the launch waiting on sequence number `want`:
+    // n Fp2 challenges as tagged words (the tags written with the values, so
+    // a launch never takes a half-written answer for this one).
+    auto answer = [&](unsigned want, const fp2* rs, int n) {
+        volatile u64* ans = S.host_answer;
+        const u64 tag = (u64)(want & 7) << 61;
+        for (int i = 0; i < n; ++i) {
+            ans[2 * i] = rs[i].re | tag;
+            ans[2 * i + 1] = rs[i].im | tag;
+        }
     };
 
     fp2* partials = S.partials.as<fp2>();
@@ -1169,7 +1280,176 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
     size_t m;       // length of T_k at the top of the loop below
     int k0;         // first round the loop handles
     fp2 r;          // challenge of the round before k0
-    if (nv >= 3) {
+    bool use_mid = true;   // cleared when the mid launch did not start whole
+    S.mid_fallbacks = 0;
+    // With `ahead` set, an untimed prove queues every launch down to the mid
+    // launch at once, each taking its challenges from the answer slot, so
+    // only the transcript steps (not a launch each) sit between the rounds.
+    // Measured on MI355X it is no faster (0.571-0.574 against 0.568-0.572 ms
+    // a prove, the same runs): a launch behind the host's transcript step
+    // starts about as soon as a queued one sees its answer.  So by default
+    // each launch is made after the step before it; timed proves always are,
+    // so that each launch's events hold its kernel alone.
+    const bool ahead = !S.timed && S.ahead && nv >= 3;
+    // If the host stops while launches queued ahead still wait for their
+    // challenges, it marks the prove failed in the sequence word (they
+    // leave at once) and waits for the stream, so no launch of this prove
+    // outlives it to take a later prove's answers.
+    struct AheadGuard {
+        bool armed = false;
+        volatile unsigned* seq = nullptr;
+        hipStream_t st = nullptr;
+        ~AheadGuard()
+        {
+            if (!armed) return;
+            *seq = kSeqHostAbort;
+            (void)hipStreamSynchronize(st);
+        }
+    } guard;
+    guard.seq = S.host_seq;
+    guard.st = st;
+    if (ahead) {
+        // rounds 0 and 1 from one pass, round 2 folds the input by r_0, r_1,
+        // then the streaming rounds and the mid launch, all queued now
+        const unsigned seq_a = ++S.seq, seq_b = ++S.seq;
+        const unsigned nb1 = (unsigned)std::min<size_t>(
+            kFirst01Groups, (n / 4 + kFirst01Threads - 1) / kFirst01Threads);
+        k_sc2_first01<<<nb1, kFirst01Threads, 0, st>>>(cf, cg, n / 4, pub_for(seq_a));
+        TP_HIP_OK(hipGetLastError());
+        S.bytes[SC_FIRST] += 32.0 * (double)n;
+        S.launches[SC_FIRST] += 1;
+        const unsigned nb2 = (unsigned)std::min<size_t>(kFold01Groups, (n / 4 + kFoldThreads - 1) / kFoldThreads);
+        k_sc2_fold01_round<<<nb2, kFoldThreads, 0, st>>>(cf, cg, n / 4, 0, 0, 0, 0, B, B + capB,
+                                                         pub_for(seq_b, seq_a, 4));
+        TP_HIP_OK(hipGetLastError());
+        S.bytes[SC_FOLD01] += 40.0 * (double)n;
+        S.launches[SC_FOLD01] += 1;
+        struct Queued { int k; unsigned seq; };
+        std::vector<Queued> qs;
+        unsigned prev = seq_b;
+        const V2* qf = B;
+        const V2* qg = B + capB;
+        size_t qm = n / 8;
+        int qk = 3;
+        while (2 * qm > kTailIn && 2 * qm > (size_t)kMidGroups * kTailIn) {
+            const unsigned seq = ++S.seq;
+            const size_t quads = qm / 2;
+            V2* of = (qk & 1) ? A : B;
+            V2* og = of + ((qk & 1) ? capA : capB);
+            const unsigned nb =
+                (unsigned)std::min<size_t>(kFoldGroups, (quads + kFoldThreads - 1) / kFoldThreads);
+            k_sc2_fold_round<<<nb, kFoldThreads, 0, st>>>(qf, qg, quads, 0, 0, of, og,
+                                                          pub_for(seq, prev, 2), (qk & 1) == 0);
+            TP_HIP_OK(hipGetLastError());
+            S.bytes[SC_FOLD] += 48.0 * (double)(2 * qm);
+            S.launches[SC_FOLD] += 1;
+            qs.push_back({qk, seq});
+            prev = seq;
+            qf = of;
+            qg = og;
+            qm /= 2;
+            ++qk;
+        }
+        // the mid launch, if these sizes reach it (as the loop below decides)
+        const bool mid_q = 2 * qm > kTailIn && 2 * qm <= (size_t)kMidGroups * kTailIn;
+        unsigned mid_seq0 = 0;
+        int mid_nr = 0;
+        V2* mid_of = nullptr;
+        V2* mid_og = nullptr;
+        if (mid_q) {
+            const unsigned slice = (unsigned)(2 * qm / kMidGroups);
+            while ((slice >> mid_nr) > kMidEnd) ++mid_nr;
+            mid_of = (qf >= A && qf < A + 2 * capA) ? B : A;
+            mid_og = mid_of + (mid_of == A ? capA : capB);
+            mid_seq0 = S.seq + 1;
+            S.seq += (unsigned)mid_nr;
+            const V2* a_f = qf;
+            const V2* a_g = qg;
+            unsigned a_slice = slice;
+            int a_nr = mid_nr;
+            u64 a_rr = 0, a_ri = 0;
+            u64* a_parts = S.mid_parts.as<u64>();
+            fp2* a_vals = S.dev_vals;
+            unsigned* a_seq = S.dev_seq;
+            unsigned a_seq0 = mid_seq0;
+            u64* a_ans = S.dev_answer;
+            long long a_spin = S.spin_ticks;
+            V2* a_of = mid_of;
+            V2* a_og = mid_og;
+            u64* a_relay = S.relay.as<u64>();
+            long long* a_trace = nullptr;
+            unsigned* a_gate = S.gate.as<unsigned>();
+            long long a_gate_ticks = S.gate_ticks;
+            unsigned a_want = prev;
+            int a_from = 1;
+            u64* a_arelay = S.arelay.as<u64>();
+            void* args[] = {&a_f, &a_g, &a_slice, &a_nr, &a_rr, &a_ri, &a_parts, &a_vals, &a_seq,
+                            &a_seq0, &a_ans, &a_spin, &a_of, &a_og, &a_relay, &a_trace,
+                            &a_gate, &a_gate_ticks, &a_want, &a_from, &a_arelay};
+            if (!S.mid_clean) {   // first mid launch, or the last one did not finish
+                k_sc2_mid_reset<<<1, 256, 0, st>>>(a_parts, a_relay, a_gate);
+                TP_HIP_OK(hipGetLastError());
+            }
+            S.mid_clean = false;
+            TP_HIP_OK(hipLaunchKernel((const void*)k_sc2_mid, dim3(kMidGroups), dim3(kTailThreads),
+                                      args, (unsigned)(slice * sizeof(V2)), st));
+            S.bytes[SC_MID] += 32.0 * (double)(2 * qm) + 64.0 * kMidGroups;
+            S.launches[SC_MID] += 1;
+        }
+        guard.armed = true;
+        // the transcript steps, each answering the launch that waits on it
+        if (wait_pub(st, seq_a, 9, false) != 1) return -1;
+        fp2 v[9];
+        for (int i = 0; i < 9; ++i) v[i] = S.pub[i];
+        fp2 rs[2];
+        rs[0] = round_done(0, add2(v[0], v[3]), add2(v[1], v[4]), add2(v[2], v[5]));
+        auto at_r0 = [&](fp2 c0, fp2 c1, fp2 c2) {
+            return add2(c0, mul2(rs[0], add2(sub2(sub2(c1, c0), c2), mul2(rs[0], c2))));
+        };
+        rs[1] = round_done(1, at_r0(v[0], v[1], v[2]), at_r0(v[3], v[4], v[5]),
+                           at_r0(v[6], v[7], v[8]));
+        answer(seq_a, rs, 2);
+        if (wait_pub(st, seq_b, 3, false) != 1) return -1;
+        r = round_done(2, S.pub[0], S.pub[1], S.pub[2]);
+        answer(seq_b, &r, 1);
+        for (const Queued& q : qs) {
+            if (wait_pub(st, q.seq, 3, false) != 1) return -1;
+            r = round_done(q.k, S.pub[0], S.pub[1], S.pub[2]);
+            answer(q.seq, &r, 1);
+        }
+        cf = qf;
+        cg = qg;
+        m = qm;
+        k0 = qk;
+        if (mid_q) {
+            const int first = wait_pub(st, mid_seq0, 3, true);
+            if (first == 0) return -1;
+            if (first == 2) {
+                // not every workgroup was resident: the loop below proves
+                // these rounds with streaming launches (the tables were only read)
+                TP_HIP_OK(hipEventRecord(S.ev_round, st));
+                if (!wait_event(S.ev_round)) return -1;
+                S.launches[SC_MID] -= 1;
+                S.bytes[SC_MID] -= 32.0 * (double)(2 * qm) + 64.0 * kMidGroups;
+                S.seq += 1;   // past the abort mark
+                S.mid_fallbacks += 1;
+                use_mid = false;
+            } else {
+                for (int i = 0; i < mid_nr; ++i) {
+                    if (i > 0 && wait_pub(st, mid_seq0 + i, 3, false) != 1) return -1;
+                    r = round_done(qk + i, S.pub[0], S.pub[1], S.pub[2]);
+                    if (i + 1 < mid_nr) answer(mid_seq0 + i, &r, 1);
+                }
+                // every round is published; the launch resets its words as it ends
+                S.mid_clean = true;
+                cf = mid_of;
+                cg = mid_og;
+                k0 = qk + mid_nr;
+                m = (size_t)kMidGroups * kMidEnd / 2;
+            }
+        }
+        guard.armed = false;   // every queued launch has had its answers
+    } else if (nv >= 3) {
         // rounds 0 and 1 from one pass, round 2 folds the input by r_0, r_1
         unsigned seq = ++S.seq;
         unsigned nb = (unsigned)std::min<size_t>(kFirst01Groups, (n / 4 + kFirst01Threads - 1) / kFirst01Threads);
@@ -1217,8 +1497,6 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         k0 = 1;
         m = n / 2;
     }
-    bool use_mid = true;   // cleared when the mid launch did not start whole
-    S.mid_fallbacks = 0;
     for (int k = k0;; ++k) {
         if (2 * m <= kTailIn) {
             // rounds k..nv-1 and the final fold in one workgroup; the host
@@ -1290,9 +1568,12 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             long long* a_trace = S.timed && nr <= 40 ? S.dev_mtrace : nullptr;
             unsigned* a_gate = S.gate.as<unsigned>();
             long long a_gate_ticks = S.gate_ticks;
+            unsigned a_want = 0;
+            int a_from = 0;
+            u64* a_arelay = S.arelay.as<u64>();
             void* args[] = {&a_f, &a_g, &a_slice, &a_nr, &a_rr, &a_ri, &a_parts, &a_vals, &a_seq,
                             &a_seq0, &a_ans, &a_spin, &a_of, &a_og, &a_relay, &a_trace,
-                            &a_gate, &a_gate_ticks};
+                            &a_gate, &a_gate_ticks, &a_want, &a_from, &a_arelay};
             if (!S.mid_clean) {   // first mid launch, or the last one did not finish
                 k_sc2_mid_reset<<<1, 256, 0, st>>>(a_parts, a_relay, a_gate);
                 TP_HIP_OK(hipGetLastError());
@@ -1390,6 +1671,12 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
 
 // Enables per-launch HIP-event timing of the fused fold+round kernel.
 void tp_sc2_set_timing(int on) { g_sc2.timed = on != 0; }
+
+// 1: an untimed prove queues its launches down to the mid launch at once,
+// each taking its challenges from the answer slot; 0 (default): each launch
+// is made after the transcript step before it (as timed proves always are).
+// The proof is the same either way.
+void tp_sc2_set_ahead(int on) { g_sc2.ahead = on != 0; }
 
 // Statistics of the last tp_sc2_prove: summed event time and algorithmic
 // bytes of the streaming fused fold+round launches (k_sc2_fold_round), and

@@ -85,6 +85,12 @@ def set_timing(on: bool) -> None:
     nb.LIB.tp_sc2_set_timing(1 if on else 0)
 
 
+def set_ahead(on: bool) -> None:
+    """Queue an untimed prove's launches ahead of their transcript steps, or
+    make each after the step before it (default); same proof either way."""
+    nb.LIB.tp_sc2_set_ahead(1 if on else 0)
+
+
 def stats():
     ms, by, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
     nb.LIB.tp_sc2_stats(ctypes.byref(ms), ctypes.byref(by), ctypes.byref(n))
