@@ -669,6 +669,57 @@ def bench_orion_commit(copies: int = 1350, reps: int = 5, seed: int = 777, cpu: 
         except Exception as exc:  # the baseline must not hide the device result
             out["cpu_baseline"] = {"error": str(exc)}
     out["open"] = bench_orion_open(x, seed, cpu=cpu)
+    out["open_nizk"] = bench_orion_open_nizk(x, seed, cpu=cpu)
+    return out
+
+
+def bench_orion_open_nizk(x, seed: int, reps: int = 5, cpu: bool = True):
+    """The NIZK opening of the frame's masked polynomial with the Virgo
+    code-switching proof that closes it (linearPC.cpp:700-928): the opening
+    as above with the transcript's 1222 positions, then 1222 more positions,
+    the encoding circuit's layers proved on the device (two launches, every
+    layer at once) and the verifier's claim chain.  CPU: the reference's own
+    generate_circuit / zk_prover / zk_verifier checks for the same proof
+    (oracle/_ref), or the oracle's restatement, one core."""
+    import numpy as np
+    from truepix_amd import orion
+    copies = x.shape[0]
+    pc = 1 << max(0, (copies - 1).bit_length())
+    lg = (pc * 512).bit_length() - 1
+    r = np.random.Generator(np.random.PCG64(11)).integers(0, (1 << 61) - 1, size=(lg, 2),
+                                                         dtype=np.uint64)
+    orion.opening_nizk_virgo(x, seed, r, True)
+    walls, devs = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        proof, _, words = orion.opening_nizk_virgo(x, seed, r, True)
+        walls.append(time.perf_counter() - t0)
+        devs.append(orion.virgo_stats()[0])
+    walls.sort()
+    devs.sort()
+    layers = orion.virgo_stats()[2]
+    out = {"workload": "NIZK Orion opening + Virgo code-switching proof: %d copies, message %d, "
+                       "%d sumcheck layers" % (copies, pc * 512 // 128, layers),
+           "ms_wall": round(walls[len(walls) // 2] * 1e3, 3),
+           "virgo_ms_device": round(devs[len(devs) // 2], 4),
+           "virgo_verdict": int(words[-1]), "virgo_words": int(words.size)}
+    if cpu:
+        try:
+            from oracle.oracle import RefVirgo, Oracle
+            o = orion.parse_opening(proof)
+            args = (o["com_root"], pc * 512, True, r, o["combined_root"], o["combined_message"])
+            t0 = time.perf_counter()
+            if RefVirgo.available():
+                ref, kind = RefVirgo().transcript(*args), "reference"
+            else:
+                ref, kind = Oracle().orion_virgo(*args), "port"
+            dt = time.perf_counter() - t0
+            out["cpu_baseline"] = {"ms": round(dt * 1e3, 1), "cores": 1, "kind": kind,
+                                   "sample": "the code-switching proof of the same opening",
+                                   "transcript_equal": bool(np.array_equal(ref, words))}
+            out["virgo_speedup_vs_cpu"] = round(dt * 1e3 / out["virgo_ms_device"], 1)
+        except Exception as exc:
+            out["cpu_baseline"] = {"error": str(exc)}
     return out
 
 

@@ -375,10 +375,39 @@ int tp_orion_open(int copies, const uint8_t* inputs, uint64_t seed, int use_grou
 /* tp_orion_open with the code positions drawn as the reference's NIZK mode
  * draws them (OZ/src/truepix/orion_fs.cpp: "tensor-open", the opened root, N,
  * the group flag, r0 and r1, then 1222 positions from the opened group;
- * linearPC.cpp:711-758).  0 / -1. */
+ * linearPC.cpp:711-758), followed, as in the reference (linearPC.cpp:875-928),
+ * by the Virgo code-switching proof over the expander-encoding circuit: 1222
+ * more positions from the whole codeword, "combined-mt" and the combined
+ * root absorbed, then generate_circuit's circuit proved by the Virgo GKR
+ * prover (OZ/src/linear_gkr/prover.cpp) and checked by its verifier's
+ * per-layer checks (OZ/src/linear_gkr/verifier.cpp:1031-1211).  Its input
+ * layer is checked by evaluating the combined message, which the proof
+ * carries, instead of the reference's FRI-based VPD.  The proof is written
+ * only if every check passes (the reference writes it only then): -1 and
+ * tp_last_error() otherwise.  0 / -1. */
 int tp_orion_open_nizk(int copies, const uint8_t* inputs, uint64_t seed, int use_group1,
                        const uint64_t* r, int n_r, uint8_t** proof, size_t* proof_len,
                        uint64_t answer[2]);
+/* tp_orion_open_nizk that also writes the code-switching transcript: up to
+ * cap words into words, the full count into *n_words.  Layout (Fp2 as re, im;
+ * D layers, bits(j) of layer j, layers i = D-1 .. 1):
+ *   D, bits(0) .. bits(D-1), Q = 1222, the Q positions sorted, r_0 and r_1
+ *   of the output layer (bits(D-1) each), the output claim;
+ *   per layer i: r_u, r_v (bits(i-1) each; r_v zero at layer 1), the
+ *   bits(i-1) phase-1 polynomials (a, b, c of a X^2 + b X + c), the bits(i-1)
+ *   phase-2 polynomials, v_u, v_v, the relay, custom-combination and
+ *   direct-relay predicate values, the next alpha and beta;
+ *   the input-layer claim, the combined message's multilinear value at the
+ *   last r_u, and the verdict (1 / 0).
+ * The words are written when the proof does not check, too. */
+int tp_orion_open_nizk_virgo(int copies, const uint8_t* inputs, uint64_t seed, int use_group1,
+                             const uint64_t* r, int n_r, uint8_t** proof, size_t* proof_len,
+                             uint64_t answer[2], uint64_t* words, long long cap,
+                             long long* n_words);
+/* Device time (ms), launches and sumcheck layers of the last code-switching
+ * proof (no counterpart in the reference, which prints its prover time).
+ * Returns 0. */
+int tp_orion_virgo_stats(double* ms, int* launches, int* layers);
 /* The expander code alone (encode, OZ/include/linear_code/linear_code_encode.h:35-110,
  * graphs from glibc's initial rand() state as expander_init draws them):
  * n_cols <= 256 columns of n <= 65536 Fp2, src [n_cols][n] and dst

@@ -1257,17 +1257,17 @@ static u64 ofs_squeeze(ofs* f)
     return v;
 }
 
-/* The positions or_orion_open_nizk opens: writes query_count positions to q
- * (capacity >= 1222) given the opened root (digest bytes), N, the group, r. */
-int or_orion_draw_queries(const unsigned char root[32], long long n, int use_group1, long long cw,
-                          const u64* r, int n_r, long long* q)
+/* The opening's public inputs absorbed as tensor_product_protocol absorbs
+ * them: "tensor-open", the opened root, N, the group flag, r0 then r1 (eq
+ * weights with the first challenge on the top index bit). */
+static void ofs_open_prefix(ofs* f, const unsigned char root[32], long long n, int use_group1,
+                            const u64* r, int n_r)
 {
-    ofs f;
-    ofs_reset(&f);
-    ofs_absorb(&f, "tensor-open", 11);
-    ofs_absorb(&f, root, 32);
-    ofs_u64(&f, (u64)n);
-    ofs_u64(&f, use_group1 ? 1 : 0);
+    ofs_reset(f);
+    ofs_absorb(f, "tensor-open", 11);
+    ofs_absorb(f, root, 32);
+    ofs_u64(f, (u64)n);
+    ofs_u64(f, use_group1 ? 1 : 0);
     const int lc = 7;
     const long long col = n / X_COLS;
     for (long long i = 0; i < X_COLS; ++i) {
@@ -1276,8 +1276,8 @@ int or_orion_draw_queries(const unsigned char root[32], long long n, int use_gro
             const fp2 x = mk(r[2 * d], r[2 * d + 1]);
             v = mul2(v, ((i >> (lc - 1 - d)) & 1) ? x : sub2(mk(1, 0), x));
         }
-        ofs_u64(&f, v.re);
-        ofs_u64(&f, v.im);
+        ofs_u64(f, v.re);
+        ofs_u64(f, v.im);
     }
     for (long long t = 0; t < col; ++t) {
         fp2 v = mk(1, 0);
@@ -1285,13 +1285,538 @@ int or_orion_draw_queries(const unsigned char root[32], long long n, int use_gro
             const fp2 x = mk(r[2 * (lc + d)], r[2 * (lc + d) + 1]);
             v = mul2(v, ((t >> (n_r - lc - 1 - d)) & 1) ? x : sub2(mk(1, 0), x));
         }
-        ofs_u64(&f, v.re);
-        ofs_u64(&f, v.im);
+        ofs_u64(f, v.re);
+        ofs_u64(f, v.im);
     }
+}
+
+static const int X_QUERIES = 1222;   /* (int)(-128 / log2(1 - 0.07)) */
+
+/* The positions or_orion_open_nizk opens: writes query_count positions to q
+ * (capacity >= 1222) given the opened root (digest bytes), N, the group, r. */
+int or_orion_draw_queries(const unsigned char root[32], long long n, int use_group1, long long cw,
+                          const u64* r, int n_r, long long* q)
+{
+    ofs f;
+    ofs_open_prefix(&f, root, n, use_group1, r, n_r);
     const int nq = (int)(-128.0 / log2(1.0 - 0.07));
     const long long g = use_group1 ? (cw + 1) / 2 : cw / 2;
     for (int i = 0; i < nq; ++i) q[i] = 2 * (long long)(ofs_squeeze(&f) % (u64)g) + (use_group1 ? 0 : 1);
     return nq;
+}
+
+
+/* ------------------------------------------- Virgo code-switching proof --
+ * The proof that closes the NIZK opening (OZ/src/VPD/linearPC.cpp:875-905):
+ * the queried entries of the combined codeword are the code of the combined
+ * message, proved by the Virgo GKR prover over the expander-encoding circuit
+ * and checked by its verifier, both run by the opener.
+ *   - transcript: after the opening's absorbs and its 1222 group draws, 1222
+ *     positions q from [0, cw) (kept sorted), "combined-mt", the combined
+ *     root; every challenge after that is squeeze % p for re then im
+ *     (orion_fs.cpp:74-113, verifier.cpp:936-940);
+ *   - circuit (generate_circuit, linearPC.cpp:477-669): layer 0 the message
+ *     (input gates), layer 1 direct relays of it, then one layer per expander
+ *     step: relays of everything so far plus one custom linear combination
+ *     per right vertex of C[d] (going down) or D[d] (coming up), then the
+ *     relays of the queried positions; each layer's gate count comes from
+ *     prepare_gates_count's arithmetic, quirks kept, rounded up to 2^bits;
+ *   - evaluation gate by gate (prover.cpp:63-170);
+ *   - per layer i = top .. 1 (verifier.cpp:1031-1211): r_u, r_v drawn (r_v
+ *     zero at layer 1); phase 1 (prover.cpp:246-633) scatters
+ *     alpha eq(r_0, g) + beta eq(r_1, g) into add_mult_sum / addV_array by
+ *     gate type and runs bits(i-1) rounds of the quadratic sumcheck; phase 2
+ *     (prover.cpp:634-1006) scatters the gates' terms at v_u and runs
+ *     bits(i-1) more; v_u = V(r_u), v_v = V(r_v); the verifier's predicates
+ *     (verifier.cpp:349-753) and direct_relay (:754-766) close the layer;
+ *     alpha, beta drawn; the claim becomes alpha v_u + beta v_v (v_u at
+ *     layer 1);
+ *   - the input layer's claim is checked by evaluating the message's
+ *     multilinear extension at the last r_u (what the reference's FRI-based
+ *     VPD establishes).
+ * The output words follow ref_harness_virgo.cpp's layout. */
+typedef struct { fp2 a, b; } lpoly;          /* a X + b */
+typedef struct {
+    int bits;
+    long long n;            /* 2^bits gates */
+    int* ty;                /* 2 dummy, 3 input, 4 direct relay, 10 relay, 14 custom */
+    long long* u;
+    long long* off;         /* custom gates' sources: off[g] .. off[g + 1] */
+    long long* src;
+    fp2* w;
+    long long n_src, cap_src;
+} vlayer;
+
+typedef struct { vlayer* L; int depth; long long cnt[128]; int n_cnt; } vcirc;
+static int v_overflow;   /* a block of gates past its layer's 2^bits */
+/* A planted mistake for the tests of the verifier's checks: 1 a phase-1
+ * polynomial, 2 v_v, 3 the relay predicate (all of the output layer), 4 the
+ * input layer's value; 0 none. */
+static int v_plant;
+void or_orion_virgo_plant(int k) { v_plant = k; }
+
+static void vc_count(vcirc* c, long long key, long long val)
+{
+    c->cnt[key] = val;
+    if (key + 1 > c->n_cnt) c->n_cnt = (int)key + 1;
+}
+
+/* prepare_enc_count (linearPC.cpp:477-488): so_far excludes the input here */
+static void vc_prepare(vcirc* c, long long in, long long so_far, int dep, long long* od, long long* os)
+{
+    if (in <= X_SMALL) {
+        *od = dep;
+        *os = so_far;
+        return;
+    }
+    vc_count(c, dep + 1, so_far + in + XG_C[dep].R);
+    long long d2, s2;
+    vc_prepare(c, XG_C[dep].R, so_far + in, dep + 1, &d2, &s2);
+    vc_count(c, d2 + 1, s2 + XG_D[dep].R);
+    *od = d2 + 1;
+    *os = c->cnt[d2 + 1];
+}
+
+static int vc_log2(long long x)
+{
+    int b = 0;
+    while ((1LL << b) < x) ++b;
+    return b;
+}
+
+static void vl_alloc(vlayer* l, int bits)
+{
+    l->bits = bits;
+    l->n = 1LL << bits;
+    l->ty = (int*)malloc(sizeof(int) * (size_t)l->n);
+    l->u = (long long*)calloc((size_t)l->n, sizeof(long long));
+    l->off = (long long*)calloc((size_t)l->n + 1, sizeof(long long));
+    for (long long g = 0; g < l->n; ++g) l->ty[g] = 2;
+    l->n_src = 0;
+    l->cap_src = 1024;
+    l->src = (long long*)malloc(sizeof(long long) * (size_t)l->cap_src);
+    l->w = (fp2*)malloc(sizeof(fp2) * (size_t)l->cap_src);
+}
+
+static void vl_free(vlayer* l)
+{
+    free(l->ty);
+    free(l->u);
+    free(l->off);
+    free(l->src);
+    free(l->w);
+}
+
+/* custom gates at first .. first + R - 1 of layer l: the right vertices of
+ * graph g, sources offset by src_off (in the order r_neighbor lists them:
+ * by left vertex, then degree) */
+static void vl_custom(vlayer* l, long long first, const xgraph* g, long long src_off)
+{
+    if (first + g->R > l->n) {
+        v_overflow = 1;
+        return;
+    }
+    long long* cnt = (long long*)calloc((size_t)g->R + 1, sizeof(long long));
+    for (long long k = 0; k < g->L * g->deg; ++k) ++cnt[g->to[k] + 1];
+    for (long long t = 0; t < g->R; ++t) cnt[t + 1] += cnt[t];
+    const long long base = l->n_src;
+    l->n_src += g->L * g->deg;
+    if (l->n_src > l->cap_src) {
+        l->cap_src = l->n_src;
+        l->src = (long long*)realloc(l->src, sizeof(long long) * (size_t)l->cap_src);
+        l->w = (fp2*)realloc(l->w, sizeof(fp2) * (size_t)l->cap_src);
+    }
+    long long* fill = (long long*)malloc(sizeof(long long) * (size_t)(g->R + 1));
+    memcpy(fill, cnt, sizeof(long long) * (size_t)(g->R + 1));
+    for (long long i = 0; i < g->L; ++i)
+        for (int d = 0; d < g->deg; ++d) {
+            const long long k = i * g->deg + d;
+            const long long at = base + fill[g->to[k]]++;
+            l->src[at] = i + src_off;
+            l->w[at] = g->w[k];
+        }
+    for (long long t = 0; t < g->R; ++t) {
+        l->ty[first + t] = 14;
+        l->off[first + t] = base + cnt[t];
+    }
+    /* off[] of a custom gate starts its range; the next range starts at its end */
+    l->off[first + g->R] = base + cnt[g->R];
+    free(cnt);
+    free(fill);
+}
+
+static void vl_relays(vlayer* l, long long k)
+{
+    if (k > l->n) {
+        v_overflow = 1;
+        return;
+    }
+    for (long long g = 0; g < k; ++g) {
+        l->ty[g] = 10;
+        l->u[g] = g;
+    }
+}
+
+/* generate_enc_circuit (linearPC.cpp:514-578): so_far includes the input */
+static void vc_gen(vcirc* c, long long in, long long so_far, int rd, int in_depth, long long* od,
+                   long long* os)
+{
+    if (in <= X_SMALL) {
+        *od = in_depth;
+        *os = so_far;
+        return;
+    }
+    vlayer* l = &c->L[in_depth + 1];
+    vl_relays(l, so_far);
+    vl_custom(l, so_far, &XG_C[rd], so_far - in);
+    long long d2, s2;
+    vc_gen(c, XG_C[rd].R, so_far + XG_C[rd].R, rd + 1, in_depth + 1, &d2, &s2);
+    const long long fd = d2 + 1;
+    vlayer* m = &c->L[fd];
+    vl_relays(m, s2);
+    vl_custom(m, s2, &XG_D[rd], so_far);
+    *od = fd;
+    *os = s2 + XG_D[rd].R;
+}
+
+/* generate_circuit (linearPC.cpp:580-669) for the sorted queries q */
+static void vc_build(vcirc* c, long long N, const long long* q, int nq)
+{
+    memset(c->cnt, 0, sizeof(c->cnt));
+    c->n_cnt = 0;
+    vc_count(c, 0, N);
+    vc_count(c, 1, N + XG_C[0].R);
+    long long d, sz;
+    vc_prepare(c, XG_C[0].R, N, 1, &d, &sz);
+    vc_count(c, d + 1, N + sz + XG_D[0].R);
+    vc_count(c, d + 2, nq);
+    c->depth = c->n_cnt + 1;
+    c->L = (vlayer*)calloc((size_t)c->depth, sizeof(vlayer));
+    vl_alloc(&c->L[0], vc_log2(N));
+    for (int i = 0; i < c->n_cnt; ++i) vl_alloc(&c->L[i + 1], vc_log2(c->cnt[i]));
+    for (long long i = 0; i < N; ++i) {
+        c->L[0].ty[i] = 3;
+        c->L[1].ty[i] = 4;
+        c->L[1].u[i] = i;
+    }
+    vl_relays(&c->L[2], N);
+    vl_custom(&c->L[2], N, &XG_C[0], 0);
+    long long d2, s2;
+    vc_gen(c, XG_C[0].R, N + XG_C[0].R, 1, 2, &d2, &s2);
+    const long long fd = d2 + 1;
+    vl_relays(&c->L[fd], s2);
+    vl_custom(&c->L[fd], s2, &XG_D[0], N);
+    if (nq > c->L[fd + 1].n) v_overflow = 1;
+    for (int i = 0; i < nq && !v_overflow; ++i) {
+        c->L[fd + 1].ty[i] = 10;
+        c->L[fd + 1].u[i] = q[i];
+    }
+}
+
+static long long vl_src_end(const vlayer* l, long long g)
+{
+    /* the range of a custom gate ends where the next gate's begins; the last
+     * custom gate of a block has its end stored one past it */
+    return l->off[g + 1];
+}
+
+/* eq(r, x) over bits with bit k of x on r[k] (beta tables of prover.cpp:262-284) */
+static fp2* v_eq(const fp2* r, int bits, fp2 scale)
+{
+    const long long n = 1LL << bits;
+    fp2* t = (fp2*)malloc(sizeof(fp2) * (size_t)n);
+    t[0] = scale;
+    for (int k = 0; k < bits; ++k)
+        for (long long j = 0; j < (1LL << k); ++j) {
+            t[j | (1LL << k)] = mul2(t[j], r[k]);
+            t[j] = mul2(t[j], one_minus(r[k]));
+        }
+    return t;
+}
+
+/* one round of sumcheck_phase1_update / phase2_update (prover.cpp:576-633) */
+static void v_round(lpoly* amul, lpoly* addv, lpoly* vm, long long* total, fp2 prev, int bit, fp2 out[3])
+{
+    const long long h = *total >> 1;
+    for (long long i = 0; i < h; ++i) {
+        lpoly* arr[3] = {vm, addv, amul};
+        for (int k = 0; k < 3; ++k) {
+            lpoly z = arr[k][2 * i], o = arr[k][2 * i + 1], nw;
+            if (bit == 0) {
+                nw.b = z.b;
+                nw.a = sub2(o.b, nw.b);
+            } else {
+                nw.b = add2(mul2(z.a, prev), z.b);
+                nw.a = sub2(add2(mul2(o.a, prev), o.b), nw.b);
+            }
+            arr[k][i] = nw;
+        }
+    }
+    fp2 a = mk(0, 0), b = mk(0, 0), c = mk(0, 0);
+    for (long long i = 0; i < h; ++i) {
+        a = add2(a, mul2(amul[i].a, vm[i].a));
+        b = add2(b, add2(add2(mul2(amul[i].a, vm[i].b), mul2(amul[i].b, vm[i].a)), addv[i].a));
+        c = add2(c, add2(mul2(amul[i].b, vm[i].b), addv[i].b));
+    }
+    out[0] = a;
+    out[1] = b;
+    out[2] = c;
+    *total = h;
+}
+
+static fp2 v_qeval(const fp2 q[3], fp2 x) { return add2(mul2(add2(mul2(q[0], x), q[1]), x), q[2]); }
+
+/* V_res (prover.cpp:29-52): the multilinear value of vals at r, bit k on r[k] */
+static fp2 v_res(const fp2* vals, int bits, const fp2* r)
+{
+    long long n = 1LL << bits;
+    fp2* t = (fp2*)malloc(sizeof(fp2) * (size_t)n);
+    memcpy(t, vals, sizeof(fp2) * (size_t)n);
+    for (int k = 0; k < bits; ++k) {
+        for (long long j = 0; j < (n >> 1); ++j)
+            t[j] = add2(mul2(t[2 * j], one_minus(r[k])), mul2(t[2 * j + 1], r[k]));
+        n >>= 1;
+    }
+    const fp2 v = t[0];
+    free(t);
+    return v;
+}
+
+typedef struct { u64* w; long long cap, n; } vout;
+static void vo_put(vout* o, u64 x)
+{
+    if (o->n < o->cap) o->w[o->n] = x;
+    ++o->n;
+}
+static void vo_fe(vout* o, fp2 x)
+{
+    vo_put(o, x.re);
+    vo_put(o, x.im);
+}
+static fp2 v_squeeze_fe(ofs* f)
+{
+    const u64 re = ofs_squeeze(f) % P61;
+    return mk(re, ofs_squeeze(f) % P61);
+}
+static int cmp_ll(const void* a, const void* b)
+{
+    const long long x = *(const long long*)a, y = *(const long long*)b;
+    return x < y ? -1 : x > y;
+}
+
+long long or_orion_virgo(const unsigned char com_root[32], long long n, int use_group1, const u64* r,
+                         int n_r, const unsigned char combined_root[32], const u64* msg, long long N,
+                         u64* out, long long cap)
+{
+    if (n != N * X_COLS || N < 1 || (N & (N - 1))) return -1;
+    /* a fresh process's graphs: below the threshold expander_init draws none
+     * and generate_circuit reads C[0] and D[0] empty */
+    for (int k = 0; k < 64; ++k) {
+        xg_make(&XG_C[k], NULL, 0, 0, 0);
+        xg_make(&XG_D[k], NULL, 0, 0, 0);
+    }
+    grand rg;
+    grand_init(&rg, 1);
+    xg_init(&rg, N, 0);
+    const long long cw = x_len(N, 0);
+    ofs f;
+    ofs_open_prefix(&f, com_root, n, use_group1, r, n_r);
+    const long long g = use_group1 ? (cw + 1) / 2 : cw / 2;
+    for (int i = 0; i < X_QUERIES; ++i) (void)(ofs_squeeze(&f) % (u64)g);
+    long long* q = (long long*)malloc(sizeof(long long) * X_QUERIES);
+    for (int i = 0; i < X_QUERIES; ++i) q[i] = (long long)(ofs_squeeze(&f) % (u64)cw);
+    ofs_absorb(&f, "combined-mt", 11);
+    ofs_absorb(&f, combined_root, 32);
+    qsort(q, X_QUERIES, sizeof(long long), cmp_ll);
+
+    vcirc C;
+    v_overflow = 0;
+    vc_build(&C, N, q, X_QUERIES);
+    const int D = C.depth, top = D - 1;
+    if (v_overflow) {
+        for (int i = 0; i < D; ++i) vl_free(&C.L[i]);
+        free(C.L);
+        free(q);
+        return -1;
+    }
+    vout o = {out, cap, 0};
+    vo_put(&o, (u64)D);
+    for (int i = 0; i < D; ++i) vo_put(&o, (u64)C.L[i].bits);
+    vo_put(&o, (u64)X_QUERIES);
+    for (int i = 0; i < X_QUERIES; ++i) vo_put(&o, (u64)q[i]);
+
+    /* evaluate (prover.cpp:63-170) */
+    fp2** V = (fp2**)calloc((size_t)D, sizeof(fp2*));
+    V[0] = (fp2*)calloc((size_t)C.L[0].n, sizeof(fp2));
+    for (long long i = 0; i < N; ++i) V[0][i] = mk(msg[2 * i], msg[2 * i + 1]);
+    for (int i = 1; i < D; ++i) {
+        const vlayer* l = &C.L[i];
+        V[i] = (fp2*)calloc((size_t)l->n, sizeof(fp2));
+        for (long long gg = 0; gg < l->n; ++gg) {
+            if (l->ty[gg] == 4 || l->ty[gg] == 10) V[i][gg] = V[i - 1][l->u[gg]];
+            else if (l->ty[gg] == 14) {
+                fp2 acc = mk(0, 0);
+                for (long long e = l->off[gg]; e < vl_src_end(l, gg); ++e)
+                    acc = add2(acc, mul2(V[i - 1][l->src[e]], l->w[e]));
+                V[i][gg] = acc;
+            }
+        }
+    }
+
+    int ok = 1;
+    fp2 alpha = mk(1, 0), beta = mk(0, 0);
+    int rb = C.L[top].bits;
+    fp2* r0 = (fp2*)malloc(sizeof(fp2) * (size_t)rb);
+    fp2* r1 = (fp2*)malloc(sizeof(fp2) * (size_t)rb);
+    for (int k = 0; k < rb; ++k) r0[k] = v_squeeze_fe(&f);
+    for (int k = 0; k < rb; ++k) r1[k] = v_squeeze_fe(&f);
+    for (int k = 0; k < rb; ++k) vo_fe(&o, r0[k]);
+    for (int k = 0; k < rb; ++k) vo_fe(&o, r1[k]);
+    fp2 claim = mul2(alpha, v_res(V[top], rb, r0));
+    vo_fe(&o, claim);
+    for (int i = top; i >= 1; --i) {
+        const vlayer* l = &C.L[i];
+        const int bl = C.L[i - 1].bits;
+        const long long nu = 1LL << bl;
+        fp2* ru = (fp2*)malloc(sizeof(fp2) * (size_t)bl);
+        fp2* rv = (fp2*)malloc(sizeof(fp2) * (size_t)bl);
+        for (int k = 0; k < bl; ++k) ru[k] = v_squeeze_fe(&f);
+        for (int k = 0; k < bl; ++k) rv[k] = v_squeeze_fe(&f);
+        if (i == 1)
+            for (int k = 0; k < bl; ++k) rv[k] = mk(0, 0);
+        for (int k = 0; k < bl; ++k) vo_fe(&o, ru[k]);
+        for (int k = 0; k < bl; ++k) vo_fe(&o, rv[k]);
+        /* direct_relay (verifier.cpp:754-766), over layer i's bits */
+        fp2 dr0 = mk(0, 0), dr1 = mk(0, 0);
+        if (i == 1) {
+            dr0 = dr1 = mk(1, 0);
+            for (int k = 0; k < l->bits; ++k) {
+                dr0 = mul2(dr0, add2(sub2(sub2(mk(1, 0), r0[k]), ru[k]), mul2(mk(2, 0), mul2(r0[k], ru[k]))));
+                dr1 = mul2(dr1, add2(sub2(sub2(mk(1, 0), r1[k]), ru[k]), mul2(mk(2, 0), mul2(r1[k], ru[k]))));
+            }
+        }
+        const fp2 drv = add2(mul2(alpha, dr0), mul2(beta, dr1));
+        /* phase 1 */
+        fp2* bg0 = v_eq(r0, l->bits, alpha);
+        fp2* bg1 = v_eq(r1, l->bits, beta);
+        lpoly* amul = (lpoly*)calloc((size_t)nu, sizeof(lpoly));
+        lpoly* addv = (lpoly*)calloc((size_t)nu, sizeof(lpoly));
+        lpoly* vm = (lpoly*)calloc((size_t)nu, sizeof(lpoly));
+        for (long long x = 0; x < nu; ++x) vm[x].b = V[i - 1][x];
+        for (long long gg = 0; gg < l->n; ++gg) {
+            const int t = l->ty[gg];
+            if (t == 4) {
+                const long long u = l->u[gg];
+                amul[u].b = add2(amul[u].b, add2(bg0[u], bg1[u]));
+            } else if (t == 10) {
+                const long long u = l->u[gg];
+                amul[u].b = add2(amul[u].b, add2(bg0[gg], bg1[gg]));
+            } else if (t == 14) {
+                const fp2 tmp = add2(bg0[gg], bg1[gg]);
+                for (long long e = l->off[gg]; e < vl_src_end(l, gg); ++e)
+                    amul[l->src[e]].b = add2(amul[l->src[e]].b, mul2(l->w[e], tmp));
+            }
+        }
+        long long total = nu;
+        fp2 prev = mk(0, 0);
+        for (int k = 0; k < bl; ++k) {
+            fp2 pq[3];
+            v_round(amul, addv, vm, &total, prev, k, pq);
+            if (v_plant == 1 && i == top && k == 0) pq[2] = add2(pq[2], mk(1, 0));
+            vo_fe(&o, pq[0]);
+            vo_fe(&o, pq[1]);
+            vo_fe(&o, pq[2]);
+            prev = ru[k];
+            if (!eq2_(add2(v_qeval(pq, mk(0, 0)), v_qeval(pq, mk(1, 0))), claim)) ok = 0;
+            claim = v_qeval(pq, ru[k]);
+        }
+        /* phase 2 */
+        const fp2 v_u = add2(mul2(vm[0].a, prev), vm[0].b);
+        fp2* bu = v_eq(ru, bl, mk(1, 0));
+        memset(amul, 0, sizeof(lpoly) * (size_t)nu);
+        memset(addv, 0, sizeof(lpoly) * (size_t)nu);
+        for (long long x = 0; x < nu; ++x) {
+            vm[x].a = mk(0, 0);
+            vm[x].b = V[i - 1][x];
+        }
+        for (long long gg = 0; gg < l->n; ++gg) {
+            const int t = l->ty[gg];
+            if (t == 10) {
+                const fp2 tg = add2(bg0[gg], bg1[gg]);
+                addv[0].b = add2(addv[0].b, mul2(mul2(tg, bu[l->u[gg]]), v_u));
+            } else if (t == 14) {
+                const fp2 tgv = mul2(add2(bg0[gg], bg1[gg]), v_u);
+                for (long long e = l->off[gg]; e < vl_src_end(l, gg); ++e)
+                    addv[0].b = add2(addv[0].b, mul2(mul2(tgv, bu[l->src[e]]), l->w[e]));
+            }
+        }
+        total = nu;
+        prev = mk(0, 0);
+        for (int k = 0; k < bl; ++k) {
+            fp2 pq[3];
+            v_round(amul, addv, vm, &total, prev, k, pq);
+            vo_fe(&o, pq[0]);
+            vo_fe(&o, pq[1]);
+            vo_fe(&o, pq[2]);
+            prev = rv[k];
+            if (!eq2_(add2(add2(v_qeval(pq, mk(0, 0)), v_qeval(pq, mk(1, 0))), mul2(drv, v_u)), claim)) ok = 0;
+            claim = add2(v_qeval(pq, rv[k]), mul2(drv, v_u));
+        }
+        fp2 v_v = add2(mul2(vm[0].a, prev), vm[0].b);
+        if (v_plant == 2 && i == top) v_v = add2(v_v, mk(1, 0));
+        vo_fe(&o, v_u);
+        vo_fe(&o, v_v);
+        /* predicates (verifier.cpp:349-753): zero at layer 1 */
+        fp2 relay = mk(0, 0), cust = mk(0, 0);
+        if (i != 1) {
+            fp2* bv = v_eq(rv, bl, mk(1, 0));
+            for (long long gg = 0; gg < l->n; ++gg) {
+                const fp2 tg = add2(bg0[gg], bg1[gg]);
+                if (l->ty[gg] == 10) relay = add2(relay, mul2(tg, bu[l->u[gg]]));
+                else if (l->ty[gg] == 14)
+                    for (long long e = l->off[gg]; e < vl_src_end(l, gg); ++e)
+                        cust = add2(cust, mul2(mul2(mul2(tg, bv[0]), bu[l->src[e]]), l->w[e]));
+            }
+            relay = mul2(relay, bv[0]);
+            free(bv);
+            if (v_plant == 3 && i == top) relay = add2(relay, mk(1, 0));
+        }
+        vo_fe(&o, relay);
+        vo_fe(&o, cust);
+        vo_fe(&o, drv);
+        if (!eq2_(claim, add2(add2(mul2(relay, v_u), mul2(cust, v_u)), mul2(drv, v_u)))) ok = 0;
+        alpha = v_squeeze_fe(&f);
+        beta = v_squeeze_fe(&f);
+        vo_fe(&o, alpha);
+        vo_fe(&o, beta);
+        claim = i != 1 ? add2(mul2(alpha, v_u), mul2(beta, v_v)) : v_u;
+        free(bg0);
+        free(bg1);
+        free(bu);
+        free(amul);
+        free(addv);
+        free(vm);
+        free(r0);
+        free(r1);
+        r0 = ru;
+        r1 = rv;
+    }
+    vo_fe(&o, claim);
+    fp2 vin = v_res(V[0], C.L[0].bits, r0);
+    if (v_plant == 4) vin = add2(vin, mk(1, 0));
+    vo_fe(&o, vin);
+    if (!eq2_(vin, claim)) ok = 0;
+    vo_put(&o, ok ? 1 : 0);
+    free(r0);
+    free(r1);
+    for (int i = 0; i < D; ++i) {
+        free(V[i]);
+        vl_free(&C.L[i]);
+    }
+    free(V);
+    free(C.L);
+    free(q);
+    return o.n;
 }
 
 void or_free(void* p) { free(p); }

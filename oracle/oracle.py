@@ -263,6 +263,62 @@ Oracle.orion_encode = _oracle_orion_encode
 Oracle.sha256 = _oracle_sha256
 
 
+_VIRGO_ARGS = [ctypes.c_char_p, ctypes.c_longlong, _INT, _U64P, _INT, ctypes.c_char_p, _U64P,
+               ctypes.c_longlong, _U64P, ctypes.c_longlong]
+
+
+def _virgo(fn, com_root: bytes, n: int, use_group1: bool, r, combined_root: bytes, message):
+    """Words of the Virgo code-switching transcript (layout in
+    oracle/ref_harness_virgo.cpp)."""
+    rr = np.ascontiguousarray(np.asarray(r, dtype=np.uint64).reshape(-1))
+    msg = np.ascontiguousarray(np.asarray(message, dtype=np.uint64).reshape(-1))
+    cap = 1 << 16
+    while True:
+        out = np.zeros(cap, np.uint64)
+        k = fn(com_root, n, 1 if use_group1 else 0, _p(rr), rr.size // 2, combined_root, _p(msg),
+               msg.size // 2, _p(out), cap)
+        if k < 0:
+            raise ValueError("Virgo transcript refused its arguments")
+        if k <= cap:
+            return out[:k]
+        cap = k
+
+
+def _oracle_orion_virgo(self, com_root: bytes, n: int, use_group1: bool, r, combined_root: bytes,
+                        message):
+    if not hasattr(self, "_virgo_fn"):
+        self._virgo_fn = _bind(self.lib, "or_orion_virgo", _VIRGO_ARGS, ctypes.c_longlong)
+    return _virgo(self._virgo_fn, com_root, n, use_group1, r, combined_root, message)
+
+
+def _oracle_orion_virgo_plant(self, k: int):
+    """Plants a mistake in the next transcripts (0: none), for the tests of
+    the verifier's checks."""
+    _bind(self.lib, "or_orion_virgo_plant", [_INT])(k)
+
+
+Oracle.orion_virgo = _oracle_orion_virgo
+Oracle.orion_virgo_plant = _oracle_orion_virgo_plant
+
+
+class RefVirgo:
+    """The Virgo code-switching proof of the reference's NIZK opening, run by
+    the reference's own generate_circuit, zk_prover and zk_verifier pieces
+    (oracle/ref_harness_virgo.cpp)."""
+
+    PATH = os.path.join(HERE, "_ref", "libvirgo_ref.so")
+
+    @classmethod
+    def available(cls) -> bool:
+        return os.path.isfile(cls.PATH)
+
+    def __init__(self):
+        self.fn = _bind(ctypes.CDLL(self.PATH), "ref_orion_virgo", _VIRGO_ARGS, ctypes.c_longlong)
+
+    def transcript(self, com_root: bytes, n: int, use_group1: bool, r, combined_root: bytes, message):
+        return _virgo(self.fn, com_root, n, use_group1, r, combined_root, message)
+
+
 class RefOrion:
     """The signer's commitment and an opening computed with the reference's
     own layout, mask, expander code, row hash, Merkle tree, siblings and

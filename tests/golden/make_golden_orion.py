@@ -7,7 +7,9 @@ place by `make -C oracle ref` and driven in commit_pair's order by
 oracle/ref_harness_orion.cpp): the masked and mask roots and the codeword
 length for seeded inputs, SHA-256 digests of single-column codewords, and
 SHA-256 digests of opening proofs (OpeningProof bytes from orion_proof.cpp's
-serializer) at seeded points and queries.
+serializer) at seeded points and queries, and digests of the Virgo
+code-switching transcript that closes each NIZK opening (generate_circuit,
+zk_prover and zk_verifier's checks, oracle/ref_harness_virgo.cpp).
 Run where /root/reference exists:
 
     make -C oracle ref && python tests/golden/make_golden_orion.py
@@ -69,6 +71,20 @@ def root_bytes(hex_root: str) -> bytes:
     return bytes(out)
 
 
+def opening_fields(proof: bytes) -> dict:
+    """The opened and combined roots and the combined message of an
+    OpeningProof version-2 byte string (OZ/include/truepix/orion_proof.h)."""
+    import struct
+    n, cols, msg_len, cw_len = struct.unpack_from("<QQQQ", proof, 16)
+    (nq,) = struct.unpack_from("<I", proof, 48)
+    o = 52 + 16
+    com_root, comb_root = proof[o:o + 32], proof[o + 32:o + 64]
+    o += 64 + 8 * nq + 16 * nq * cols + 16 * cw_len
+    fe = np.frombuffer(proof, dtype="<u8", count=2 * msg_len, offset=o).reshape(msg_len, 2)
+    return {"com_root": com_root, "combined_root": comb_root,
+            "combined_message": fe[:, ::-1].copy()}   # (imag, real) on disk
+
+
 def main():
     from oracle.oracle import RefOrion
     if not RefOrion.available():
@@ -110,6 +126,30 @@ def main():
         out["draw"].append({"copies": copies, "seed": seed, "input_seed": iseed,
                             "use_group1": group, "point_seed": pseed, "queries": len(q),
                             "head": q[:8], "sha256": hashlib.sha256(np.asarray(q, "<i8").tobytes()).hexdigest()})
+    # the Virgo code-switching proof of each NIZK opening (queries drawn as
+    # above): the transcript words of oracle/ref_harness_virgo.cpp
+    from oracle.oracle import RefVirgo
+    virgo = RefVirgo()
+    out["virgo"] = []
+    for copies, seed, iseed, group, pseed, nq in OPEN_CASES:
+        r, _ = open_args(copies, pseed, nq, cw_of[copies])
+        m, k, cw = ref.commit(inputs_for(copies, iseed), seed)
+        pc = 1
+        while pc < copies:
+            pc <<= 1
+        q = ref.draw_queries(root_bytes(m if group else k), pc * 512, bool(group), cw, r)
+        proof = ref.open(inputs_for(copies, iseed), seed, r, np.asarray(q, np.int64),
+                         use_group1=bool(group))
+        f = opening_fields(proof)
+        words = virgo.transcript(f["com_root"], pc * 512, bool(group), r, f["combined_root"],
+                                 f["combined_message"])
+        depth = int(words[0])
+        out["virgo"].append({"copies": copies, "seed": seed, "input_seed": iseed,
+                             "use_group1": group, "point_seed": pseed, "words": int(words.size),
+                             "bits": [int(x) for x in words[1:1 + depth]],
+                             "verdict": int(words[-1]),
+                             "tail": [int(x) for x in words[-5:]],
+                             "sha256": hashlib.sha256(words.astype("<u8").tobytes()).hexdigest()})
     with open(os.path.join(HERE, "orion_ref.json"), "w") as fh:
         json.dump(out, fh, indent=1)
     print("wrote", len(out["commit"]), "commit cases,", len(out["encode"]), "encode cases,",

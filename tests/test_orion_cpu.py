@@ -13,7 +13,7 @@ import random
 import numpy as np
 import pytest
 
-from oracle.oracle import Oracle, RefOrion
+from oracle.oracle import Oracle, RefOrion, RefVirgo
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 P = (1 << 61) - 1
@@ -243,3 +243,94 @@ def test_query_draw_matches_live_reference(orc):
         root = root_bytes(m if group else k)
         assert orc.orion_draw_queries(root, pc * 512, group, cw, r) == \
             ref.draw_queries(root, pc * 512, group, cw, r)
+
+
+# ------------------------------------------------- code-switching proof
+
+def _virgo_case(orc, copies, seed, iseed, group, pseed):
+    """The oracle's NIZK opening of a seeded frame and its code-switching
+    transcript."""
+    from truepix_amd.orion import parse_opening
+    x = inputs_for(copies, iseed)
+    m, k, cw = orc.orion_commit(x, seed)
+    r, _ = open_args(copies, pseed, 2, cw)
+    pc = 1 << max(0, (copies - 1).bit_length())
+    q = orc.orion_draw_queries(root_bytes(m if group else k), pc * 512, group, cw, r)
+    proof, _ = orc.orion_open(x, seed, r, np.asarray(q, np.int64), use_group1=group)
+    o = parse_opening(proof)
+    args = (o["com_root"], pc * 512, group, r, o["combined_root"], o["combined_message"])
+    return orc.orion_virgo(*args), args
+
+
+def test_virgo_matches_reference_transcripts(orc, gold):
+    """The oracle's Virgo code-switching transcript (every challenge, round
+    polynomial, claim, predicate and the verdict) against the reference's
+    (generate_circuit, zk_prover stepped in zk_verifier::verify's order, the
+    verifier's predicates; tests/golden/orion_ref.json)."""
+    for case in gold["virgo"]:
+        words, _ = _virgo_case(orc, case["copies"], case["seed"], case["input_seed"],
+                               bool(case["use_group1"]), case["point_seed"])
+        assert words.size == case["words"] and words[-1] == case["verdict"] == 1, case["copies"]
+        assert [int(x) for x in words[1:1 + words[0]]] == case["bits"]
+        assert hashlib.sha256(words.astype("<u8").tobytes()).hexdigest() == case["sha256"], case
+
+
+@pytest.mark.skipif(not RefVirgo.available(), reason="reference build absent (make -C oracle ref)")
+def test_virgo_matches_live_reference(orc):
+    ref = RefVirgo()
+    for copies, seed, group in ((2, 4, False), (5, 8, True), (40, 2 ** 62 + 1, False), (520, 3, True)):
+        words, args = _virgo_case(orc, copies, seed, copies, group, copies + 1)
+        assert np.array_equal(words, ref.transcript(*args)), copies
+
+
+def test_virgo_transcript_shape_and_closed_forms(orc):
+    """What the algebra fixes about the transcript: the circuit of
+    generate_circuit (layer 1 as wide as the message, the output layer 2^11
+    for 1222 sorted positions inside the codeword), r_v zero at layer 1, and
+    for relays and linear combinations a phase 2 of (0, -S_j, S_j) with
+    S_{j+1} = S_j (1 - r_v[j]), S = 0 on the direct relays of layer 1."""
+    words, args = _virgo_case(orc, 16, 2024, 4, True, 23)
+    w = [int(x) for x in words]
+    D = w[0]
+    bits = w[1:1 + D]
+    msg_len = args[1] // 128
+    assert bits[0] == bits[1] == msg_len.bit_length() - 1 and bits[-1] == 11
+    pos = 1 + D
+    nq = w[pos]
+    q = w[pos + 1:pos + 1 + nq]
+    assert nq == 1222 and q == sorted(q)
+    pos += 1 + nq + 4 * bits[-1] + 2
+    for i in range(D - 1, 0, -1):
+        b = bits[i - 1]
+        ru = [tuple(w[pos + 2 * k:pos + 2 * k + 2]) for k in range(b)]
+        rv = [tuple(w[pos + 2 * b + 2 * k:pos + 2 * b + 2 * k + 2]) for k in range(b)]
+        pos += 4 * b
+        if i == 1:
+            assert all(x == (0, 0) for x in rv)
+        pos += 6 * b   # phase 1
+        p2 = [w[pos + 6 * k:pos + 6 * k + 6] for k in range(b)]
+        pos += 6 * b
+        s = (p2[0][4], p2[0][5])
+        for k in range(b):
+            a, bb, c = (p2[k][0], p2[k][1]), (p2[k][2], p2[k][3]), (p2[k][4], p2[k][5])
+            assert a == (0, 0) and c == s and bb == ((-s[0]) % P, (-s[1]) % P), (i, k)
+            s = _fp2_mul(s, ((1 - rv[k][0]) % P, (-rv[k][1]) % P))
+        if i == 1:
+            assert (p2[0][4], p2[0][5]) == (0, 0)
+        pos += 4 + 6 + 4   # v_u v_v, three predicates, alpha beta
+    assert pos + 5 == len(w) and w[-1] == 1
+
+
+def test_virgo_checks_catch_planted_mistakes(orc):
+    """The verifier's checks are live: a mistake planted in a round
+    polynomial, in v_v, in a predicate or in the input layer's value makes
+    the verdict 0."""
+    try:
+        for k in (1, 2, 3, 4):
+            orc.orion_virgo_plant(k)
+            words, _ = _virgo_case(orc, 16, 2024, 4, True, 23)
+            assert words[-1] == 0, k
+    finally:
+        orc.orion_virgo_plant(0)
+    words, _ = _virgo_case(orc, 16, 2024, 4, True, 23)
+    assert words[-1] == 1

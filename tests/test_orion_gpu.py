@@ -129,3 +129,54 @@ def test_open_nizk_draws_the_reference_queries(bridge, oracle, copies, seed, gro
     want, want_ans = oracle.orion_open(x, seed, r, q, use_group1=group)
     assert orion.parse_opening(got)["col_q"] == q
     assert (got, ans) == (want, want_ans)
+
+
+def _open_args(copies, point_seed, cw_len):
+    """The point of the golden openings (tests/golden/make_golden_orion.py)."""
+    pc = 1 << max(0, (copies - 1).bit_length())
+    lg = (pc * 512).bit_length() - 1
+    rng = np.random.Generator(np.random.PCG64(point_seed))
+    return rng.integers(0, P, size=(lg, 2), dtype=np.uint64)
+
+
+def test_open_nizk_virgo_matches_reference_transcripts(bridge, gold):
+    """The Virgo code-switching transcript of each golden NIZK opening (every
+    challenge, round polynomial, v_u, v_v, predicate and the verdict) hashes
+    to the words the reference's own generate_circuit, zk_prover and
+    zk_verifier checks produced (oracle/ref_harness_virgo.cpp)."""
+    cw_of = {c["copies"]: c["cw_len"] for c in gold["commit"]}
+    for case in gold["virgo"]:
+        copies = case["copies"]
+        r = _open_args(copies, case["point_seed"], cw_of[copies])
+        print("virgo golden", copies, case["use_group1"], flush=True)
+        proof, ans, words = orion.opening_nizk_virgo(inputs_for(copies, case["input_seed"]),
+                                                     case["seed"], r, bool(case["use_group1"]))
+        assert proof is not None and words[-1] == 1, copies
+        assert words.size == case["words"] and [int(x) for x in words[1:1 + words[0]]] == case["bits"]
+        assert hashlib.sha256(words.astype("<u8").tobytes()).hexdigest() == case["sha256"], copies
+        plain, plain_ans = orion.opening_nizk(inputs_for(copies, case["input_seed"]), case["seed"], r,
+                                              bool(case["use_group1"]))
+        assert (proof, ans) == (plain, plain_ans)
+
+
+@pytest.mark.parametrize("copies,seed,group", [(2, 9, False), (7, 31, True), (64, 5, False),
+                                               (700, 2 ** 61 + 3, True), (3600, 11, False),
+                                               (8040, 12, True)])
+def test_open_nizk_virgo_equals_oracle(bridge, oracle, copies, seed, group):
+    """The code-switching transcript equals the oracle's (pinned to the
+    reference) on fresh frames, both polynomials, up to the 8040-copy frame
+    (a 2^15-entry message, 2^17-gate layers)."""
+    x = inputs_for(copies, copies + 17)
+    pc = 1 << max(0, (copies - 1).bit_length())
+    lg = (pc * 512).bit_length() - 1
+    r = np.random.Generator(np.random.PCG64(copies + 3)).integers(0, P, size=(lg, 2), dtype=np.uint64)
+    print("virgo oracle", copies, group, flush=True)
+    proof, _, words = orion.opening_nizk_virgo(x, seed, r, group)
+    assert proof is not None and words[-1] == 1
+    o = orion.parse_opening(proof)
+    want = oracle.orion_virgo(o["com_root"], pc * 512, group, r, o["combined_root"],
+                              o["combined_message"])
+    assert words.size == want.size
+    assert np.array_equal(words, want)
+    ms, launches, layers = orion.virgo_stats()
+    assert launches == 2 and layers == int(words[0]) - 1 and ms > 0

@@ -43,12 +43,14 @@
 #include "host_sha.hpp"
 #include "reduce.hpp"
 #include "runtime.hpp"
+#include "virgo.hpp"
 #include "../../include/truepix_hip.h"
 
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <functional>
 #include <map>
 #include <memory>
@@ -404,7 +406,8 @@ __global__ void __launch_bounds__(256) k_or_cleaves(const V2* __restrict__ cc, l
 // -------------------------------------------------------------- plans ----
 
 // One column length's code: the graphs drawn as the reference draws them,
-// reversed into per-target edge lists, and the in-place row offsets.
+// reversed into per-target edge lists, and the in-place row offsets; the
+// forward edge lists too, for the code-switching proof's gathers.
 struct Plan {
     long long col = 0, cw = 0;
     struct Step {
@@ -413,6 +416,8 @@ struct Plan {
     };
     std::vector<Step> down, up;   // C steps by depth, D steps from the deepest up
     DevBuf rowptr, src, w;
+    DevBuf fto, fw;               // forward edges of every graph, by depth
+    virgo::Code code;             // the graphs as the code-switching proof reads them
 };
 
 struct Fwd {
@@ -489,23 +494,63 @@ bool build_plan(Plan& P, long long col)
     };
     P.down.clear();
     P.up.clear();
-    for (int d = 0; d < depth; ++d)
-        P.down.push_back({base[d], base[d + 1], C[d].R, reverse(C[d])});
-    for (int d = depth - 1; d >= 0; --d)
-        P.up.push_back({base[d + 1], base[d] + m[d] + len[d + 1], D[d].R, reverse(D[d])});
+    std::vector<int> rpC((size_t)depth), rpD((size_t)depth);
+    for (int d = 0; d < depth; ++d) {
+        rpC[(size_t)d] = reverse(C[d]);
+        P.down.push_back({base[d], base[d + 1], C[d].R, rpC[(size_t)d]});
+    }
+    for (int d = depth - 1; d >= 0; --d) {
+        rpD[(size_t)d] = reverse(D[d]);
+        P.up.push_back({base[d + 1], base[d] + m[d] + len[d + 1], D[d].R, rpD[(size_t)d]});
+    }
     if (rp.empty()) rp.push_back(0);
     if (src.empty()) {
         src.push_back(0);
         ww.push_back(V2{0, 0});
     }
+    // forward edges: C[0], D[0], C[1], D[1], ...
+    std::vector<int> fto;
+    std::vector<V2> fw;
+    std::vector<size_t> offC((size_t)depth), offD((size_t)depth);
+    for (int d = 0; d < depth; ++d)
+        for (int k = 0; k < 2; ++k) {
+            const Fwd& g = k ? D[d] : C[d];
+            (k ? offD : offC)[(size_t)d] = fto.size();
+            fto.insert(fto.end(), g.to.begin(), g.to.end());
+            for (const fp2& x : g.w) fw.push_back(V2{x.re, x.im});
+        }
+    if (fto.empty()) {
+        fto.push_back(0);
+        fw.push_back(V2{0, 0});
+    }
     hipStream_t st = stream();
     if (!P.rowptr.ensure(rp.size() * 4) || !P.src.ensure(src.size() * 4) ||
-        !P.w.ensure(ww.size() * sizeof(V2)))
+        !P.w.ensure(ww.size() * sizeof(V2)) || !P.fto.ensure(fto.size() * 4) ||
+        !P.fw.ensure(fw.size() * sizeof(V2)))
         return false;
     TP_HIP_OK(hipMemcpyAsync(P.rowptr.ptr, rp.data(), rp.size() * 4, hipMemcpyHostToDevice, st));
     TP_HIP_OK(hipMemcpyAsync(P.src.ptr, src.data(), src.size() * 4, hipMemcpyHostToDevice, st));
     TP_HIP_OK(hipMemcpyAsync(P.w.ptr, ww.data(), ww.size() * sizeof(V2), hipMemcpyHostToDevice, st));
+    TP_HIP_OK(hipMemcpyAsync(P.fto.ptr, fto.data(), fto.size() * 4, hipMemcpyHostToDevice, st));
+    TP_HIP_OK(hipMemcpyAsync(P.fw.ptr, fw.data(), fw.size() * sizeof(V2), hipMemcpyHostToDevice, st));
     TP_HIP_OK(hipStreamSynchronize(st));
+    P.code.n = col;
+    P.code.cw = P.cw;
+    P.code.C.assign((size_t)depth, virgo::Graph());
+    P.code.D.assign((size_t)depth, virgo::Graph());
+    for (int d = 0; d < depth; ++d)
+        for (int k = 0; k < 2; ++k) {
+            const Fwd& g = k ? D[d] : C[d];
+            virgo::Graph& v = (k ? P.code.D : P.code.C)[(size_t)d];
+            v.L = g.L;
+            v.R = g.R;
+            v.deg = g.deg;
+            v.to = P.fto.as<int>() + (k ? offD : offC)[(size_t)d];
+            v.w = P.fw.as<fp2>() + (k ? offD : offC)[(size_t)d];
+            v.rowptr = P.rowptr.as<int>() + (k ? rpD : rpC)[(size_t)d];
+            v.src = P.src.as<int>();
+            v.rw = P.w.as<fp2>();
+        }
     return true;
 }
 
@@ -807,9 +852,12 @@ int tp_orion_commit(int copies, const uint8_t* inputs, uint64_t seed, char maske
 // OZ/include/truepix/orion_proof.h:1-22) to *proof (free with tp_free) and
 // the claimed value to answer.  The Virgo code-switching proof that follows
 // in the reference is not part of it.  0 / -1.
+virgo::Stats g_virgo_stats;
+
 static int open_impl(int copies, const uint8_t* inputs, uint64_t seed, int use_group1,
                      const uint64_t* r, int n_r, const int64_t* col_q_in, int n_queries, bool nizk,
-                     uint8_t** proof, size_t* proof_len, uint64_t answer[2])
+                     uint8_t** proof, size_t* proof_len, uint64_t answer[2],
+                     std::vector<u64>* virgo_words = nullptr)
 {
     clear_error();
     if (!device_ready()) return -1;
@@ -830,7 +878,9 @@ static int open_impl(int copies, const uint8_t* inputs, uint64_t seed, int use_g
     if (!commit_device(copies, inputs, seed, st, P, rows)) return -1;
     const long long cw = P->cw;
     std::vector<int64_t> drawn;
+    std::vector<long long> vq;   // the code-switching proof's positions
     const int64_t* col_q = col_q_in;
+    OrionFs fs;
     // r0 over r[0..7), r1 over the rest: eq tensors, the first challenge on
     // the highest bit of the index (dfs in linearPC.cpp:996-1007)
     auto tensor = [&](const uint64_t* rr, int k, std::vector<fp2>& out) {
@@ -858,7 +908,6 @@ static int open_impl(int copies, const uint8_t* inputs, uint64_t seed, int use_g
         uint8_t rb[32];
         for (int k = 0; k < 8; ++k)
             for (int b = 0; b < 4; ++b) rb[4 * k + b] = (uint8_t)(root[k] >> (24 - 8 * b));
-        OrionFs fs;
         fs.reset();
         fs.absorb("tensor-open", 11);
         fs.absorb(rb, 32);
@@ -872,6 +921,10 @@ static int open_impl(int copies, const uint8_t* inputs, uint64_t seed, int use_g
         for (int i = 0; i < n_queries; ++i)
             drawn[(size_t)i] = 2LL * fs.index((int)gsize) + (use_group1 ? 0 : 1);
         col_q = drawn.data();
+        // then the code-switching proof's positions over the whole codeword
+        // (linearPC.cpp:879-881)
+        vq.resize((size_t)n_queries);
+        for (int i = 0; i < n_queries; ++i) vq[(size_t)i] = fs.index((int)cw);
     }
     for (int i = 0; i < n_queries; ++i)
         if (col_q[i] < 0 || col_q[i] >= cw)
@@ -936,6 +989,23 @@ static int open_impl(int copies, const uint8_t* inputs, uint64_t seed, int use_g
     answer[1] = ans.im;
     *proof = nullptr;
     *proof_len = 0;
+    if (nizk) {
+        // "combined-mt" and the combined root, then the Virgo proof that the
+        // queried entries are the code of the combined message; the proof is
+        // written only if it checks (linearPC.cpp:883-928)
+        uint8_t cb[32];
+        for (int k = 0; k < 8; ++k)
+            for (int b = 0; b < 4; ++b) cb[4 * k + b] = (uint8_t)(tcmb[8 + k] >> (24 - 8 * b));
+        fs.absorb("combined-mt", 11);
+        fs.absorb(cb, 32);
+        std::vector<u64> words;
+        bool accepted = false;
+        if (!virgo::prove(P->code, reinterpret_cast<const fp2*>(d_cc), vq,
+                          [&fs]() { return fs.squeeze_u64(); }, st, words, accepted, &g_virgo_stats))
+            return -1;
+        if (virgo_words) virgo_words->swap(words);
+        if (!accepted) return fail("tp_orion_open_nizk: the code-switching proof does not check"), -1;
+    }
     std::vector<uint8_t> out;
     serialize_opening(out, use_group1 != 0, n, msg, cw, ans, tcom, tcmb, rows, col_q, n_queries,
                       qc, cc);
@@ -965,6 +1035,32 @@ int tp_orion_open_nizk(int copies, const uint8_t* inputs, uint64_t seed, int use
 {
     return open_impl(copies, inputs, seed, use_group1, r, n_r, nullptr, 0, true, proof, proof_len,
                      answer);
+}
+
+// tp_orion_open_nizk, with the words of its code-switching transcript.
+int tp_orion_open_nizk_virgo(int copies, const uint8_t* inputs, uint64_t seed, int use_group1,
+                             const uint64_t* r, int n_r, uint8_t** proof, size_t* proof_len,
+                             uint64_t answer[2], uint64_t* words, long long cap, long long* n_words)
+{
+    if (cap < 0 || (cap && !words) || !n_words)
+        return fail("tp_orion_open_nizk_virgo: bad transcript buffer"), -1;
+    std::vector<u64> w;
+    const int rc = open_impl(copies, inputs, seed, use_group1, r, n_r, nullptr, 0, true, proof,
+                             proof_len, answer, &w);
+    *n_words = (long long)w.size();
+    if (cap && !w.empty())
+        memcpy(words, w.data(), (size_t)std::min<long long>(cap, (long long)w.size()) * sizeof(u64));
+    return rc;
+}
+
+// Device time (ms), launches and sumcheck layers of the last code-switching
+// proof.
+int tp_orion_virgo_stats(double* ms, int* launches, int* layers)
+{
+    if (ms) *ms = g_virgo_stats.ms;
+    if (launches) *launches = g_virgo_stats.launches;
+    if (layers) *layers = g_virgo_stats.layers;
+    return 0;
 }
 
 // The expander code alone: n_cols <= 256 columns of n Fp2 each, src

@@ -130,6 +130,11 @@ def bind(lib) -> None:
                                      ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t),
                                      _U64P], _I32)
     _LL = ctypes.c_longlong
+    _sig(lib, "tp_orion_open_nizk_virgo", [_I32, ctypes.c_char_p, ctypes.c_uint64, _I32, _U64P, _I32,
+                                           ctypes.POINTER(ctypes.c_void_p),
+                                           ctypes.POINTER(ctypes.c_size_t), _U64P, _U64P, _LL,
+                                           ctypes.POINTER(_LL)], _I32)
+    _sig(lib, "tp_orion_virgo_stats", [ctypes.POINTER(_D), _I32P, _I32P], _I32)
     _sig(lib, "tp_yuv_pack", [_VP, _VP, _VP, _I32, _I32, _LL, _LL, _VP, _I32, _VP, _VP], _LL)
     _sig(lib, "tp_yuv_unpack", [_VP, _LL, _I32, _I32, _LL, _LL, _VP, _VP, _VP, _VP], _I32)
     _sig(lib, "tp_yuv_pack_host", [_VP, _VP, _VP, _I32, _I32, _VP, _I32, _VP], _LL)

@@ -102,6 +102,48 @@ def opening_nizk(inputs, seed: int, r, use_group1: bool = True):
     return proof, (int(ans[0]), int(ans[1]))
 
 
+def opening_nizk_virgo(inputs, seed: int, r, use_group1: bool = True):
+    """``opening_nizk`` and the words of its Virgo code-switching transcript
+    (layout in include/truepix_hip.h, tp_orion_open_nizk_virgo).  Returns
+    (proof or None, answer, words): proof is None when the code-switching
+    proof does not check, as the reference then writes no proof."""
+    nb.require()
+    arr = np.ascontiguousarray(inputs)
+    if arr.ndim != 2 or arr.shape[1] != COPY_SIZE:
+        raise ValueError("inputs must be [copies][%d]" % COPY_SIZE)
+    data = np.ascontiguousarray(arr, dtype=np.uint8)
+    rr = np.ascontiguousarray(np.asarray(r, dtype=np.uint64).reshape(-1))
+    cap = 1 << 15
+    while True:
+        buf = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        ans = np.zeros(2, np.uint64)
+        words = np.zeros(cap, np.uint64)
+        nw = ctypes.c_longlong()
+        rc = nb.LIB.tp_orion_open_nizk_virgo(
+            int(arr.shape[0]), data.tobytes(), int(seed) & (2 ** 64 - 1), 1 if use_group1 else 0,
+            rr.ctypes.data_as(_U64P), rr.size // 2, ctypes.byref(buf), ctypes.byref(n),
+            ans.ctypes.data_as(_U64P), words.ctypes.data_as(_U64P), cap, ctypes.byref(nw))
+        if buf.value:
+            proof = ctypes.string_at(buf.value, n.value)
+            nb.LIB.tp_free(buf)
+        else:
+            proof = None
+        if nw.value > cap:
+            cap = nw.value
+            continue
+        if rc != 0 and (nw.value == 0 or words[nw.value - 1] != 0):
+            raise RuntimeError("tp_orion_open_nizk_virgo failed: %s" % nb.last_error())
+        return (proof if rc == 0 else None), (int(ans[0]), int(ans[1])), words[:nw.value].copy()
+
+
+def virgo_stats():
+    """(device ms, launches, sumcheck layers) of the last code-switching proof."""
+    ms, la, ly = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+    nb.LIB.tp_orion_virgo_stats(ctypes.byref(ms), ctypes.byref(la), ctypes.byref(ly))
+    return ms.value, la.value, ly.value
+
+
 def parse_opening(proof: bytes) -> dict:
     """The fields of an OpeningProof version-2 byte string
     (OZ/include/truepix/orion_proof.h:1-22); field elements as (re, im),
