@@ -494,21 +494,18 @@ bool build_plan(Plan& P, long long col)
     };
     P.down.clear();
     P.up.clear();
-    std::vector<int> rpC((size_t)depth), rpD((size_t)depth);
-    for (int d = 0; d < depth; ++d) {
-        rpC[(size_t)d] = reverse(C[d]);
-        P.down.push_back({base[d], base[d + 1], C[d].R, rpC[(size_t)d]});
-    }
-    for (int d = depth - 1; d >= 0; --d) {
-        rpD[(size_t)d] = reverse(D[d]);
-        P.up.push_back({base[d + 1], base[d] + m[d] + len[d + 1], D[d].R, rpD[(size_t)d]});
-    }
+    for (int d = 0; d < depth; ++d)
+        P.down.push_back({base[d], base[d + 1], C[d].R, reverse(C[d])});
+    for (int d = depth - 1; d >= 0; --d)
+        P.up.push_back({base[d + 1], base[d] + m[d] + len[d + 1], D[d].R, reverse(D[d])});
     if (rp.empty()) rp.push_back(0);
     if (src.empty()) {
         src.push_back(0);
         ww.push_back(V2{0, 0});
     }
-    // forward edges: C[0], D[0], C[1], D[1], ...
+    // forward edges, degree-major (edge slot e of every left vertex, then
+    // slot e + 1, ...), so consecutive left vertices read consecutive words:
+    // C[0], D[0], C[1], D[1], ...
     std::vector<int> fto;
     std::vector<V2> fw;
     std::vector<size_t> offC((size_t)depth), offD((size_t)depth);
@@ -516,8 +513,12 @@ bool build_plan(Plan& P, long long col)
         for (int k = 0; k < 2; ++k) {
             const Fwd& g = k ? D[d] : C[d];
             (k ? offD : offC)[(size_t)d] = fto.size();
-            fto.insert(fto.end(), g.to.begin(), g.to.end());
-            for (const fp2& x : g.w) fw.push_back(V2{x.re, x.im});
+            for (int e = 0; e < g.deg; ++e)
+                for (long long i = 0; i < g.L; ++i) {
+                    const size_t at = (size_t)i * g.deg + e;
+                    fto.push_back(g.to[at]);
+                    fw.push_back(V2{g.w[at].re, g.w[at].im});
+                }
         }
     if (fto.empty()) {
         fto.push_back(0);
@@ -547,9 +548,6 @@ bool build_plan(Plan& P, long long col)
             v.deg = g.deg;
             v.to = P.fto.as<int>() + (k ? offD : offC)[(size_t)d];
             v.w = P.fw.as<fp2>() + (k ? offD : offC)[(size_t)d];
-            v.rowptr = P.rowptr.as<int>() + (k ? rpD : rpC)[(size_t)d];
-            v.src = P.src.as<int>();
-            v.rw = P.w.as<fp2>();
         }
     return true;
 }

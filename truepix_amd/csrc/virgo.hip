@@ -27,17 +27,21 @@
 // proved at once.
 //
 // The launches.
-//   k_vg_pre (256 threads per workgroup, one job each): the A table of
-//     every layer (a gather over the forward graph, the beta weights from
-//     half-size eq tables in LDS); the verifier's sums (verifier.cpp:349-753)
-//     over every layer's gates through the reversed graph: relay and
-//     custom-combination predicates, the output claim V~_top(r_0) over the
-//     queried entries, the input layer's value at the last r_u.
-//   k_vg_layer (1024 threads): one workgroup per layer runs the phase-1
-//     rounds (fold by the known challenge, round sums reduced in the
-//     workgroup, tables ping-ponged through global scratch), V~(r_v) and
-//     the phase-2 polynomials; further workgroups reduce the verifier's
-//     partial sums.
+//   k_vg_pre (256 threads per workgroup, one job each):
+//     - A jobs: 256 entries of a layer's A table (a gather over the forward
+//       graph, the beta weights from half-size eq tables in LDS) beside the
+//       same entries of V; the first (up to) eight rounds of that layer's
+//       phase 1 run on them at once, six inside each wave (partner values
+//       by lane shuffles), two more over the four wave results; the round
+//       sums and the folded pair go out;
+//     - check jobs: the verifier's predicates (verifier.cpp:349-753) as flat
+//       sums over relays, graph edges or queried positions;
+//     - dot jobs: V~(r_v) of every layer, the output claim V~_top(r_0) over
+//       the queried entries, the input layer's value at the last r_u;
+//   k_vg_layer (1024 threads): one workgroup per layer adds the round sums
+//     of its A jobs and runs the remaining rounds on their folded pairs in
+//     LDS, then the phase-2 polynomials; further workgroups reduce the
+//     check and dot partial sums.
 // The host then walks the verifier's claim chain.
 #include <algorithm>
 #include <cstring>
@@ -52,10 +56,18 @@ namespace virgo {
 namespace {
 
 constexpr int kSmall = 13;          // distance_threshold (OZ/include/linear_code/parameter.h)
-constexpr int kMaxBits = 20;        // widest layer handled: 2^20 gates
+constexpr int kMaxBits = 18;        // widest layer handled: 2^18 gates (a message of 2^16)
 constexpr int kKindDirect = 0, kKindCode = 1, kKindQuery = 2;
-constexpr int kRoleA = 0, kRoleCheck = 1, kRoleOut = 2, kRoleIn = 3;
-constexpr int kPreThreads = 256, kLayerThreads = 1024;
+constexpr int kRoleA = 0, kRoleCheck = 1, kRoleDot = 2;
+constexpr int kDotV = 0, kDotOut = 1, kDotIn = 2;
+constexpr int kPreThreads = 256;    // k_vg_pre
+constexpr int kThreads = 1024;      // k_vg_layer
+constexpr int kPerThread = 4;       // entries / items per thread and step
+constexpr int kJobRounds = 10;      // phase-1 rounds inside an A job (1024 entries)
+constexpr int kThreadRounds = 2;    // of them inside each thread
+constexpr int kWaveRounds = 6;      // then across each wave's lanes
+constexpr int kCheckItems = 2048, kDotItems = 2048;
+constexpr int kCheckRelays = 0, kCheckEdges = 1, kCheckQueries = 2;
 
 struct VL {                 // sumcheck layer i: the gates of layer i over layer i-1
     int bits_g, bits_u, kind, ch;   // ch: r_0, r_1 (bits_g each), r_u, r_v (bits_u each), alpha, beta
@@ -66,16 +78,15 @@ struct VL {                 // sumcheck layer i: the gates of layer i over layer
     int deg, res;           // graph degree; results offset (Fp2)
     const int* to;
     const fp2* w;
-    const int* rowptr;
-    const int* src;
-    const fp2* rw;
-    long long a_off, s_off; // A table (2^bits_u) and scratch (2^(bits_u+1)) offsets
+    int r1, nwa;            // rounds inside each A job, A jobs
+    long long pf_off;       // A jobs' round sums: nwa x r1 x 3 Fp2
+    long long f_off;        // A jobs' folded pairs: nwa x (A, V)
 };
 
 struct Job {                // one workgroup of k_vg_pre
     int layer, role;
-    long long begin, end;
-    int slot, pad;          // partial-sum slot (check roles)
+    long long begin, end;   // entries, items or positions
+    int slot, aux;          // A job index in its layer / partial slot; dot kind
 };
 
 struct Group {              // one reduction workgroup of k_vg_layer
@@ -115,7 +126,7 @@ struct Beta {               // alpha eq(r_0, g) + beta eq(r_1, g)
     }
 };
 
-// Builds eq(r, .) over nb bits into lds (2 * 2^ceil(nb/2) entries) at *used.
+// Builds eq(r, .) over nb bits into lds (2^(nb/2) + 2^(nb - nb/2) entries).
 __device__ Eq make_eq(fp2* lds, int& used, const fp2* r, int nb, fp2 scale)
 {
     Eq e;
@@ -143,118 +154,274 @@ __device__ Beta make_beta(fp2* lds, int& used, const VL& l, const fp2* ch)
     return b;
 }
 
-// First position of q (sorted, n entries) holding a value >= u.
-__device__ int lower_bound_q(const long long* q, int n, long long u)
+__device__ __forceinline__ fp2 shfl_xor2(fp2 x, int s)
 {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (q[mid] < u) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
+    return mk2(__shfl_xor(x.re, s, 64), __shfl_xor(x.im, s, 64));
 }
 
-__device__ __forceinline__ fp2 ld(const fp2* p, long long i) { return p[i]; }
+// Phase-1 rounds on the entries held by the lanes of one wave (entry k in
+// lane k, A in a, V in v): round j pairs lanes 2^(j+1) m and 2^(j+1) m + 2^j
+// (entries 2i, 2i+1 of the table folded by the earlier rounds), adds
+// (dA dV, dA V0 + A0 dV, A0 V0) over the pairs and folds by ru[j] into the
+// lower lane.  Lane 0 stores round j's sums at sums + 3 j.
+__device__ void wave_rounds(fp2& a, fp2& v, const fp2* ru, int n, fp2* sums)
+{
+    const int lane = threadIdx.x & 63;
+    for (int j = 0; j < n; ++j) {
+        const int s = 1 << j;
+        const fp2 a1 = shfl_xor2(a, s), v1 = shfl_xor2(v, s);
+        LazyProdSum x, y, z;
+        x.zero();
+        y.zero();
+        z.zero();
+        if ((lane & (2 * s - 1)) == 0) {
+            const fp2 da = sub2(a1, a), dv = sub2(v1, v);
+            x.mac(da, dv);
+            y.mac(da, v);
+            y.mac(a, dv);
+            z.mac(a, v);
+            const fp2 r = ru[j];
+            a = fold2(a, a1, r);
+            v = fold2(v, v1, r);
+        }
+        fp2 pr[3] = {x.value(), y.value(), z.value()};
+        wave_sum<3>(pr);
+        if (lane == 0) {
+            sums[3 * j] = pr[0];
+            sums[3 * j + 1] = pr[1];
+            sums[3 * j + 2] = pr[2];
+        }
+    }
+}
+
+// Entry u of the A table of a code layer: beta(u) if gate u relays it, plus
+// w beta(first + t) over its edges to combinations t (degree-major edge
+// lists: a wave's lanes read consecutive words; five edges' loads at a time).
+__device__ fp2 a_code(const VL& l, const Beta& b, long long u)
+{
+    fp2 a = u < l.P ? b.at(u) : zero2();
+    if (u >= l.src_off && u < l.src_off + l.L) {
+        const long long i = u - l.src_off;
+        LazyProdSum s;
+        s.zero();
+        for (int d0 = 0; d0 < l.deg; d0 += 5) {
+            int to[5];
+            fp2 w[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+                if (d0 + k < l.deg) {
+                    to[k] = l.to[(d0 + k) * l.L + i];
+                    w[k] = l.w[(d0 + k) * l.L + i];
+                }
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+                if (d0 + k < l.deg) s.mac(w[k], b.at(l.first + to[k]));
+        }
+        a = add2(a, s.value());
+    }
+    return a;
+}
 
 __global__ void __launch_bounds__(kPreThreads) k_vg_pre(const VL* __restrict__ layers,
-                                                        const Job* __restrict__ jobs,
-                                                        const fp2* __restrict__ ch,
-                                                        const fp2* __restrict__ cc,
-                                                        const long long* __restrict__ q, int nq,
-                                                        fp2* __restrict__ A, fp2* __restrict__ part)
+                                                     const Job* __restrict__ jobs,
+                                                     const fp2* __restrict__ ch,
+                                                     const fp2* __restrict__ cc,
+                                                     const long long* __restrict__ q, int nq,
+                                                     fp2* __restrict__ pf, fp2* __restrict__ F,
+                                                     fp2* __restrict__ part)
 {
     extern __shared__ fp2 lds[];
+    __shared__ fp2 sw[kPreThreads / 64][3 * (kThreadRounds + kWaveRounds)];   // the waves' round sums
+    __shared__ fp2 sf[kPreThreads / 64][2];                  // the waves' folded pairs
     const Job jb = jobs[blockIdx.x];
     const VL& l = layers[jb.layer];
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     int used = 0;
     if (jb.role == kRoleA) {
         const Beta b = make_beta(lds, used, l, ch);
+        const long long u0 = jb.begin, B = jb.end - jb.begin;
+        fp2* sa = lds + used;   // the queried positions' A entries in this range
+        if (l.kind == kKindQuery) {
+            for (int k = t; k < B; k += blockDim.x) sa[k] = zero2();
+        }
         __syncthreads();
-        fp2* out = A + l.a_off;
-        for (long long u = jb.begin + threadIdx.x; u < jb.end; u += blockDim.x) {
-            fp2 a = zero2();
-            if (l.kind == kKindDirect) {
-                a = b.at(u);
-            } else if (l.kind == kKindCode) {
-                if (u < l.P) a = b.at(u);
-                if (u >= l.src_off && u < l.src_off + l.L) {
-                    const long long e0 = (u - l.src_off) * l.deg;
-                    LazyProdSum s;
-                    s.zero();
-                    for (int d = 0; d < l.deg; ++d) s.mac(l.w[e0 + d], b.at(l.first + l.to[e0 + d]));
-                    a = add2(a, s.value());
-                }
-            } else {
-                for (int k = lower_bound_q(q, nq, u); k < nq && q[k] == u; ++k) a = add2(a, b.at(k));
+        if (l.kind == kKindQuery) {
+            // sorted positions: the first of each run adds the run's betas
+            for (int k = t; k < nq; k += blockDim.x) {
+                const long long x = q[k];
+                if (x < u0 || x >= u0 + B || (k > 0 && q[k - 1] == x)) continue;
+                fp2 s = zero2();
+                for (int kk = k; kk < nq && q[kk] == x; ++kk) s = add2(s, b.at(kk));
+                sa[x - u0] = s;
             }
-            out[u] = a;
+            __syncthreads();
+        }
+        // entries 4t .. 4t+3 of the job in this thread
+        fp2 a[kPerThread], v[kPerThread];
+#pragma unroll
+        for (int m = 0; m < kPerThread; ++m) {
+            const long long k = (long long)kPerThread * t + m, u = u0 + k;
+            a[m] = v[m] = zero2();
+            if (k < B) {
+                if (u < l.vlen) v[m] = cc[u];
+                if (l.kind == kKindDirect) a[m] = b.at(u);
+                else if (l.kind == kKindCode) a[m] = a_code(l, b, u);
+                else a[m] = sa[k];
+            }
+        }
+        const fp2* ru = ch + l.ch + 2 * l.bits_g;
+        const int r1 = l.r1;
+        // rounds 0 and 1 inside the thread
+        const int nt = r1 < kThreadRounds ? r1 : kThreadRounds;
+        for (int j = 0; j < nt; ++j) {
+            const int s = 1 << j;
+            LazyProdSum x, y, z;
+            x.zero();
+            y.zero();
+            z.zero();
+            const fp2 r = ru[j];
+#pragma unroll
+            for (int m = 0; m < kPerThread; m += 2) {
+                if (m % (2 * s)) continue;
+                const fp2 da = sub2(a[m + s], a[m]), dv = sub2(v[m + s], v[m]);
+                x.mac(da, dv);
+                y.mac(da, v[m]);
+                y.mac(a[m], dv);
+                z.mac(a[m], v[m]);
+                a[m] = fold2(a[m], a[m + s], r);
+                v[m] = fold2(v[m], v[m + s], r);
+            }
+            fp2 pr[3] = {x.value(), y.value(), z.value()};
+            wave_sum<3>(pr);
+            if (lane == 0) {
+                sw[wid][3 * j] = pr[0];
+                sw[wid][3 * j + 1] = pr[1];
+                sw[wid][3 * j + 2] = pr[2];
+            }
+        }
+        // rounds 2 .. 7 across each wave's lanes
+        const int nwr = r1 - nt < kWaveRounds ? r1 - nt : kWaveRounds;
+        wave_rounds(a[0], v[0], ru + nt, nwr, sw[wid] + 3 * nt);
+        if (lane == 0) {
+            sf[wid][0] = a[0];
+            sf[wid][1] = v[0];
+        }
+        __syncthreads();
+        if (wid == 0) {
+            const int nw = (int)((B + 64 * kPerThread - 1) / (64 * kPerThread));
+            const int nr = nt + nwr;
+            fp2* out = pf + l.pf_off + (long long)jb.slot * r1 * 3;
+            for (int j = 0; j < nr; ++j) {
+                fp2 acc[3] = {zero2(), zero2(), zero2()};
+                if (lane < nw) {
+                    acc[0] = sw[lane][3 * j];
+                    acc[1] = sw[lane][3 * j + 1];
+                    acc[2] = sw[lane][3 * j + 2];
+                }
+                wave_sum<3>(acc);
+                if (lane == 0) {
+                    out[3 * j] = acc[0];
+                    out[3 * j + 1] = acc[1];
+                    out[3 * j + 2] = acc[2];
+                }
+            }
+            // rounds 8 and 9 over the waves' folded pairs, in wave 0's lanes
+            fp2 fa_ = lane < nw ? sf[lane][0] : zero2();
+            fp2 fv_ = lane < nw ? sf[lane][1] : zero2();
+            if (r1 > nr) wave_rounds(fa_, fv_, ru + nr, r1 - nr, out + 3 * nr);
+            if (lane == 0) {
+                F[l.f_off + 2LL * jb.slot] = fa_;
+                F[l.f_off + 2LL * jb.slot + 1] = fv_;
+            }
         }
         return;
     }
     fp2 acc[2] = {zero2(), zero2()};
     if (jb.role == kRoleCheck) {
-        // relay: sum beta(g) eq(r_u, u_g); custom: sum beta(g) sum_e w_e eq(r_u, src_e)
+        // relays: beta(g) eq(r_u, u_g); edges: beta(first + to) w eq(r_u, src);
+        // four items per thread and step, their loads first
         const Beta b = make_beta(lds, used, l, ch);
         const Eq eu = make_eq(lds, used, ch + l.ch + 2 * l.bits_g, l.bits_u, one2());
         __syncthreads();
-        LazyProdSum relay, cust;
-        relay.zero();
-        cust.zero();
-        for (long long g = jb.begin + threadIdx.x; g < jb.end; g += blockDim.x) {
-            if (l.kind == kKindQuery) {
-                relay.mac(b.at(g), eu.at(q[g]));
-            } else if (g < l.P) {
-                relay.mac(b.at(g), eu.at(g));
+        LazyProdSum s;
+        s.zero();
+        for (long long x0 = jb.begin + kPerThread * t; x0 < jb.end; x0 += kPerThread * blockDim.x) {
+            if (jb.aux == kCheckEdges) {
+                int to[kPerThread];
+                fp2 w[kPerThread];
+#pragma unroll
+                for (int m = 0; m < kPerThread; ++m)
+                    if (x0 + m < jb.end) {
+                        to[m] = l.to[x0 + m];
+                        w[m] = l.w[x0 + m];
+                    }
+#pragma unroll
+                for (int m = 0; m < kPerThread; ++m)
+                    if (x0 + m < jb.end)   // slot e / L of left vertex e mod L
+                        s.mac(b.at(l.first + to[m]), mul2(w[m], eu.at(l.src_off + (x0 + m) % l.L)));
+            } else if (jb.aux == kCheckQueries) {
+                long long qq[kPerThread];
+#pragma unroll
+                for (int m = 0; m < kPerThread; ++m)
+                    if (x0 + m < jb.end) qq[m] = q[x0 + m];
+#pragma unroll
+                for (int m = 0; m < kPerThread; ++m)
+                    if (x0 + m < jb.end) s.mac(b.at(x0 + m), eu.at(qq[m]));
             } else {
-                const long long t = g - l.first;
-                LazyProdSum e;
-                e.zero();
-                for (int k = l.rowptr[t]; k < l.rowptr[t + 1]; ++k) e.mac(l.rw[k], eu.at(l.src[k] + l.src_off));
-                cust.mac(b.at(g), e.value());
+#pragma unroll
+                for (int m = 0; m < kPerThread; ++m)
+                    if (x0 + m < jb.end) s.mac(b.at(x0 + m), eu.at(x0 + m));
             }
         }
-        acc[0] = relay.value();
-        acc[1] = cust.value();
+        acc[jb.aux == kCheckEdges ? 1 : 0] = s.value();
     } else {
-        // the output claim V~_top(r_0) over the queried entries, or the input
-        // layer's value V~_0(r_u of layer 1)
-        const fp2* r = jb.role == kRoleOut ? ch + l.ch : ch + l.ch + 2 * l.bits_g;
-        const int nb = jb.role == kRoleOut ? l.bits_g : l.bits_u;
+        // V~(r_v) of a layer, the output claim V~_top(r_0) over the queried
+        // entries, or the input layer's value V~_0(r_u of layer 1)
+        const fp2* r = jb.aux == kDotOut ? ch + l.ch
+                     : jb.aux == kDotV ? ch + l.ch + 2 * l.bits_g + l.bits_u
+                                       : ch + l.ch + 2 * l.bits_g;
+        const int nb = jb.aux == kDotOut ? l.bits_g : l.bits_u;
         const Eq e = make_eq(lds, used, r, nb, one2());
         __syncthreads();
         LazyProdSum s;
         s.zero();
-        for (long long x = jb.begin + threadIdx.x; x < jb.end; x += blockDim.x)
-            s.mac(e.at(x), ld(cc, jb.role == kRoleOut ? q[x] : x));
+        for (long long x0 = jb.begin + kPerThread * t; x0 < jb.end; x0 += kPerThread * blockDim.x) {
+            fp2 val[kPerThread];
+#pragma unroll
+            for (int m = 0; m < kPerThread; ++m)
+                if (x0 + m < jb.end) val[m] = cc[jb.aux == kDotOut ? q[x0 + m] : x0 + m];
+#pragma unroll
+            for (int m = 0; m < kPerThread; ++m)
+                if (x0 + m < jb.end) s.mac(e.at(x0 + m), val[m]);
+        }
         acc[0] = s.value();
     }
     block_sum<2>(acc);
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         part[2 * jb.slot] = acc[0];
         part[2 * jb.slot + 1] = acc[1];
     }
 }
 
-__global__ void __launch_bounds__(kLayerThreads) k_vg_layer(const VL* __restrict__ layers, int n_layers,
-                                                            const Group* __restrict__ groups,
-                                                            const fp2* __restrict__ ch,
-                                                            const fp2* __restrict__ cc,
-                                                            const fp2* __restrict__ A,
-                                                            fp2* __restrict__ S,
-                                                            const fp2* __restrict__ part,
-                                                            fp2* __restrict__ res)
+__global__ void __launch_bounds__(kThreads) k_vg_layer(const VL* __restrict__ layers, int n_layers,
+                                                       const Group* __restrict__ groups,
+                                                       const fp2* __restrict__ ch,
+                                                       const fp2* __restrict__ pf,
+                                                       const fp2* __restrict__ F,
+                                                       const fp2* __restrict__ part,
+                                                       fp2* __restrict__ res)
 {
-    extern __shared__ fp2 lds[];
+    __shared__ fp2 fa0[1024], fv0[1024], fa1[512], fv1[512];   // ping-pong: halves shrink
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     if ((int)blockIdx.x >= n_layers) {
         const Group gr = groups[blockIdx.x - n_layers];
         fp2 acc[2] = {zero2(), zero2()};
-        for (int k = threadIdx.x; k < gr.count; k += blockDim.x) {
+        for (int k = t; k < gr.count; k += blockDim.x) {
             acc[0] = add2(acc[0], part[2 * (gr.slot0 + k)]);
             acc[1] = add2(acc[1], part[2 * (gr.slot0 + k) + 1]);
         }
         block_sum<2>(acc);
-        if (threadIdx.x == 0) {
+        if (t == 0) {
             res[gr.res] = acc[0];
             res[gr.res + 1] = acc[1];
         }
@@ -264,74 +431,80 @@ __global__ void __launch_bounds__(kLayerThreads) k_vg_layer(const VL* __restrict
     const fp2* ru = ch + l.ch + 2 * l.bits_g;
     const fp2* rv = ru + l.bits_u;
     fp2* out = res + l.res;
-    long long n = 1LL << l.bits_u;
-    const long long h0 = n >> 1;
-    fp2* buf[2][2] = {{S + l.s_off, S + l.s_off + h0}, {S + l.s_off + 2 * h0, S + l.s_off + 3 * h0}};
-    const fp2* srcA = A + l.a_off;
-    const fp2* srcV = cc;
-    const long long vlen = l.vlen;
-    // phase 1: round j pairs entries 2i, 2i+1 of the tables folded by
-    // r_u[0 .. j-1] (prover.cpp:576-633); the fold by r_u[j] is made now
-    for (int j = 0; j < l.bits_u; ++j) {
-        const long long half = n >> 1;
-        fp2* dA = buf[j & 1][0];
-        fp2* dV = buf[j & 1][1];
-        const fp2 r = ru[j];
-        LazyProdSum sa, sb, sc;
-        sa.zero();
-        sb.zero();
-        sc.zero();
-        for (long long i = threadIdx.x; i < half; i += blockDim.x) {
-            const fp2 a0 = srcA[2 * i], a1 = srcA[2 * i + 1];
-            fp2 v0, v1;
-            if (j == 0) {
-                v0 = 2 * i < vlen ? cc[2 * i] : zero2();
-                v1 = 2 * i + 1 < vlen ? cc[2 * i + 1] : zero2();
-            } else {
-                v0 = srcV[2 * i];
-                v1 = srcV[2 * i + 1];
-            }
-            const fp2 da = sub2(a1, a0), dv = sub2(v1, v0);
-            sa.mac(da, dv);
-            sb.mac(da, v0);
-            sb.mac(a0, dv);
-            sc.mac(a0, v0);
-            dA[i] = fold2(a0, a1, r);
-            dV[i] = fold2(v0, v1, r);
+    const int r1 = l.r1, nwa = l.nwa;
+    // rounds 0 .. r1-1: wave j adds the A jobs' sums of round j
+    if (wid < r1) {
+        fp2 acc[3] = {zero2(), zero2(), zero2()};
+        for (int k = lane; k < nwa; k += 64) {
+            const fp2* p = pf + l.pf_off + ((long long)k * r1 + wid) * 3;
+            acc[0] = add2(acc[0], p[0]);
+            acc[1] = add2(acc[1], p[1]);
+            acc[2] = add2(acc[2], p[2]);
         }
-        fp2 poly[3] = {sa.value(), sb.value(), sc.value()};
-        block_sum<3>(poly);
-        if (threadIdx.x == 0) {
-            out[3 * j] = poly[0];
-            out[3 * j + 1] = poly[1];
-            out[3 * j + 2] = poly[2];
+        wave_sum<3>(acc);
+        if (lane == 0) {
+            out[3 * wid] = acc[0];
+            out[3 * wid + 1] = acc[1];
+            out[3 * wid + 2] = acc[2];
         }
-        srcA = dA;
-        srcV = dV;
-        n = half;
     }
-    // V~(r_v) over the layer below, the eq weights from LDS
-    int used = 0;
-    const Eq ev = make_eq(lds, used, rv, l.bits_u, one2());
+    // the rest over the jobs' folded pairs (nwa <= 2^(kMaxBits - kJobRounds)
+    // = 1024 of them)
+    for (int k = t; k < nwa; k += blockDim.x) {
+        fa0[k] = F[l.f_off + 2LL * k];
+        fv0[k] = F[l.f_off + 2LL * k + 1];
+    }
     __syncthreads();
-    LazyProdSum sv;
-    sv.zero();
-    for (long long x = threadIdx.x; x < vlen; x += blockDim.x) sv.mac(ev.at(x), cc[x]);
-    fp2 vv[1] = {sv.value()};
-    block_sum<1>(vv);
-    if (threadIdx.x == 0) {
-        const int b = l.bits_u;
-        const fp2 af = srcA[0], vu = srcV[0];
-        out[3 * b] = vu;
-        out[3 * b + 1] = vv[0];
-        out[3 * b + 2] = af;
-        // phase 2 (prover.cpp:634-1006): everything sits at v = 0
-        fp2 s = l.kind == kKindDirect ? zero2() : mul2(vu, af);
-        for (int j = 0; j < b; ++j) {
-            out[3 * b + 3 + 3 * j] = zero2();
-            out[3 * b + 3 + 3 * j + 1] = sub2(zero2(), s);
-            out[3 * b + 3 + 3 * j + 2] = s;
-            s = mul2(s, sub2(one2(), rv[j]));
+    int n = nwa, j = r1, cur = 0;
+    while (n > 64) {
+        const int half = n >> 1;
+        const fp2* sa = cur ? fa1 : fa0;
+        const fp2* sv = cur ? fv1 : fv0;
+        fp2* da_ = cur ? fa0 : fa1;
+        fp2* dv_ = cur ? fv0 : fv1;
+        LazyProdSum x, y, z;
+        x.zero();
+        y.zero();
+        z.zero();
+        if (t < half) {
+            const fp2 a0 = sa[2 * t], a1 = sa[2 * t + 1];
+            const fp2 v0 = sv[2 * t], v1 = sv[2 * t + 1];
+            const fp2 da = sub2(a1, a0), dv = sub2(v1, v0);
+            x.mac(da, dv);
+            y.mac(da, v0);
+            y.mac(a0, dv);
+            z.mac(a0, v0);
+            da_[t] = fold2(a0, a1, ru[j]);
+            dv_[t] = fold2(v0, v1, ru[j]);
+        }
+        fp2 pr[3] = {x.value(), y.value(), z.value()};
+        block_sum<3>(pr);
+        if (t == 0) {
+            out[3 * j] = pr[0];
+            out[3 * j + 1] = pr[1];
+            out[3 * j + 2] = pr[2];
+        }
+        __syncthreads();
+        cur ^= 1;
+        n = half;
+        ++j;
+    }
+    if (wid == 0) {
+        fp2 a = lane < n ? (cur ? fa1 : fa0)[lane] : zero2();
+        fp2 v = lane < n ? (cur ? fv1 : fv0)[lane] : zero2();
+        wave_rounds(a, v, ru + j, l.bits_u - j, out + 3 * j);
+        if (lane == 0) {
+            const int b = l.bits_u;
+            out[3 * b] = v;       // v_u
+            out[3 * b + 1] = a;   // A~(r_u)
+            // phase 2 (prover.cpp:634-1006): everything sits at v = 0
+            fp2 s = l.kind == kKindDirect ? zero2() : mul2(v, a);
+            for (int k = 0; k < b; ++k) {
+                out[3 * b + 2 + 3 * k] = zero2();
+                out[3 * b + 2 + 3 * k + 1] = sub2(zero2(), s);
+                out[3 * b + 2 + 3 * k + 2] = s;
+                s = mul2(s, sub2(one2(), rv[k]));
+            }
         }
     }
 }
@@ -349,7 +522,6 @@ struct HLayer {             // host view of layer j of the circuit
     const Graph* g = nullptr;
 };
 
-// Host Fp2 helpers for the verifier's claim chain.
 fp2 qeval(const fp2* p, fp2 x) { return add2(mul2(add2(mul2(p[0], x), p[1]), x), p[2]); }
 
 DevBuf g_buf;
@@ -471,15 +643,20 @@ bool prove(const Code& code, const fp2* d_cc, std::vector<long long> q,
     std::vector<fp2> ch;
     std::vector<Job> jobs;
     std::vector<Group> groups;
-    long long a_tot = 0, s_tot = 0;
+    long long pf_tot = 0, f_tot = 0;
     int res_tot = 0, slots = 0;
-    auto add_jobs = [&](int layer, int role, long long n, long long chunk) {
-        Group gr{slots, 0, 0, 0};
-        for (long long b = 0; b < n; b += chunk) {
-            jobs.push_back(Job{layer, role, b, std::min(n, b + chunk), slots++, 0});
-            ++gr.count;
-        }
-        return gr;
+    // jobs over [0, n) for each (aux, n) segment, summed by one group
+    auto add_jobs = [&](int layer, int role, std::initializer_list<std::pair<int, long long>> segs,
+                        long long chunk) {
+        Group gr{slots, 0, res_tot, 0};
+        for (const auto& sg : segs)
+            for (long long b = 0; b < sg.second; b += chunk) {
+                jobs.push_back(Job{layer, role, b, std::min(sg.second, b + chunk), slots++, sg.first});
+                ++gr.count;
+            }
+        res_tot += 2;
+        groups.push_back(gr);
+        return gr.res;
     };
     for (int k = 0; k < nl; ++k) {
         const int i = top - k;
@@ -508,56 +685,49 @@ bool prove(const Code& code, const fp2* d_cc, std::vector<long long> q,
             v.deg = h.g->deg;
             v.to = h.g->to;
             v.w = h.g->w;
-            v.rowptr = h.g->rowptr;
-            v.src = h.g->src;
-            v.rw = h.g->rw;
         }
-        v.a_off = a_tot;
-        a_tot += 1LL << v.bits_u;
-        v.s_off = s_tot;
-        s_tot += 2LL << v.bits_u;
+        v.r1 = std::min(kJobRounds, v.bits_u);
+        v.nwa = 1 << (v.bits_u - v.r1);
+        v.pf_off = pf_tot;
+        pf_tot += 3LL * v.nwa * v.r1;
+        v.f_off = f_tot;
+        f_tot += 2LL * v.nwa;
         v.res = res_tot;
-        res_tot += 6 * v.bits_u + 3;
-        add_jobs(k, kRoleA, 1LL << v.bits_u, 1024);
+        res_tot += 6 * v.bits_u + 2;
+        const long long B = 1LL << v.r1;
+        for (int w = 0; w < v.nwa; ++w) jobs.push_back(Job{k, kRoleA, w * B, (w + 1) * B, w, 0});
     }
-    // the verifier's sums: per layer above 1, the output claim, the input value
-    std::vector<int> chk_res((size_t)nl, -1);
+    // the verifier's predicates (layers above 1) and the dot products: V~(r_v)
+    // of every layer, the output claim, the input value
+    std::vector<int> chk_res((size_t)nl, -1), vv_res((size_t)nl, -1);
     for (int k = 0; k < nl; ++k) {
         const int i = top - k;
-        if (i == 1) continue;
-        const HLayer& h = H[(size_t)i];
-        Group gr = add_jobs(k, kRoleCheck, h.kind == kKindQuery ? nq : h.P + h.R, 512);
-        gr.res = res_tot;
-        chk_res[(size_t)k] = res_tot;
-        res_tot += 2;
-        groups.push_back(gr);
+        const VL& v = vl[(size_t)k];
+        if (i != 1 && v.kind == kKindQuery)
+            chk_res[(size_t)k] = add_jobs(k, kRoleCheck, {{kCheckQueries, (long long)nq}}, kCheckItems);
+        else if (i != 1)
+            chk_res[(size_t)k] = add_jobs(k, kRoleCheck, {{kCheckRelays, v.P}, {kCheckEdges, v.L * v.deg}},
+                                          kCheckItems);
+        vv_res[(size_t)k] = add_jobs(k, kRoleDot, {{kDotV, v.vlen}}, kDotItems);
     }
-    Group go_out = add_jobs(0, kRoleOut, nq, 1024);
-    go_out.res = res_tot;
-    const int out_res = res_tot;
-    res_tot += 2;
-    groups.push_back(go_out);
-    Group go_in = add_jobs(nl - 1, kRoleIn, N, 4096);
-    go_in.res = res_tot;
-    const int in_res = res_tot;
-    res_tot += 2;
-    groups.push_back(go_in);
+    const int out_res = add_jobs(0, kRoleDot, {{kDotOut, (long long)nq}}, kDotItems);
+    const int in_res = add_jobs(nl - 1, kRoleDot, {{kDotIn, N}}, kDotItems);
 
-    // one device arena: layers, jobs, groups, challenges, queries, A, scratch,
-    // partials, results
+    // one device arena: layers, jobs, groups, challenges, queries, round sums,
+    // folded pairs, partial sums, results
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t o_vl = 0, o_job = o_vl + al(vl.size() * sizeof(VL)),
                  o_grp = o_job + al(jobs.size() * sizeof(Job)),
                  o_ch = o_grp + al(groups.size() * sizeof(Group)),
                  o_q = o_ch + al(ch.size() * sizeof(fp2)),
-                 o_A = o_q + al(std::max(1, nq) * sizeof(long long)),
-                 o_S = o_A + al((size_t)a_tot * sizeof(fp2)),
-                 o_part = o_S + al((size_t)s_tot * sizeof(fp2)),
+                 o_pf = o_q + al(std::max(1, nq) * sizeof(long long)),
+                 o_F = o_pf + al((size_t)pf_tot * sizeof(fp2)),
+                 o_part = o_F + al((size_t)f_tot * sizeof(fp2)),
                  o_res = o_part + al((size_t)2 * slots * sizeof(fp2)),
                  total = o_res + al((size_t)res_tot * sizeof(fp2));
     if (!g_buf.ensure(total)) return false;
     char* base = g_buf.as<char>();
-    std::vector<char> up(o_A);
+    std::vector<char> up(o_pf);
     memcpy(up.data() + o_vl, vl.data(), vl.size() * sizeof(VL));
     memcpy(up.data() + o_job, jobs.data(), jobs.size() * sizeof(Job));
     memcpy(up.data() + o_grp, groups.data(), groups.size() * sizeof(Group));
@@ -568,34 +738,40 @@ bool prove(const Code& code, const fp2* d_cc, std::vector<long long> q,
         TP_HIP_OK(hipEventCreate(&g_ev[1]));
     }
     TP_HIP_OK(hipMemcpyAsync(base, up.data(), up.size(), hipMemcpyHostToDevice, st));
-    int hmax = 0;
-    for (const VL& v : vl) hmax = std::max(hmax, std::max(v.bits_g, v.bits_u) - std::max(v.bits_g, v.bits_u) / 2);
-    const size_t lds_pre = (size_t)6 << hmax << 4, lds_layer = (size_t)2 << hmax << 4;
-    if (lds_pre > 64 * 1024)
+    // LDS of a job: beta (two eq tables) and eq(r_u) for a check job, beta
+    // and the queried positions' entries for an A job of the output layer
+    auto eq_need = [](int nb) { return (size_t)(1 << (nb / 2)) + (size_t)(1 << (nb - nb / 2)); };
+    size_t lds_ent = 0;
+    for (const VL& v : vl) {
+        const size_t beta = 2 * eq_need(v.bits_g);
+        lds_ent = std::max(lds_ent, beta + std::max(eq_need(v.bits_u), (size_t)(1 << v.r1)));
+    }
+    const size_t lds_pre = lds_ent * sizeof(fp2);
+    if (lds_pre > 48 * 1024)
         TP_HIP_OK(hipFuncSetAttribute((const void*)k_vg_pre, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds_pre));
     const VL* d_vl = reinterpret_cast<const VL*>(base + o_vl);
     const fp2* d_ch = reinterpret_cast<const fp2*>(base + o_ch);
     const long long* d_q = reinterpret_cast<const long long*>(base + o_q);
-    fp2* d_A = reinterpret_cast<fp2*>(base + o_A);
-    fp2* d_S = reinterpret_cast<fp2*>(base + o_S);
+    fp2* d_pf = reinterpret_cast<fp2*>(base + o_pf);
+    fp2* d_F = reinterpret_cast<fp2*>(base + o_F);
     fp2* d_part = reinterpret_cast<fp2*>(base + o_part);
     fp2* d_res = reinterpret_cast<fp2*>(base + o_res);
     TP_HIP_OK(hipEventRecord(g_ev[0], st));
     k_vg_pre<<<(unsigned)jobs.size(), kPreThreads, lds_pre, st>>>(
-        d_vl, reinterpret_cast<const Job*>(base + o_job), d_ch, d_cc, d_q, nq, d_A, d_part);
+        d_vl, reinterpret_cast<const Job*>(base + o_job), d_ch, d_cc, d_q, nq, d_pf, d_F, d_part);
     TP_HIP_OK(hipGetLastError());
-    k_vg_layer<<<(unsigned)(nl + groups.size()), kLayerThreads, lds_layer, st>>>(
-        d_vl, nl, reinterpret_cast<const Group*>(base + o_grp), d_ch, d_cc, d_A, d_S, d_part, d_res);
+    k_vg_layer<<<(unsigned)(nl + groups.size()), kThreads, 0, st>>>(
+        d_vl, nl, reinterpret_cast<const Group*>(base + o_grp), d_ch, d_pf, d_F, d_part, d_res);
     TP_HIP_OK(hipGetLastError());
     TP_HIP_OK(hipEventRecord(g_ev[1], st));
     std::vector<fp2> res((size_t)res_tot);
     TP_HIP_OK(hipMemcpyAsync(res.data(), d_res, res.size() * sizeof(fp2), hipMemcpyDeviceToHost, st));
     TP_HIP_OK(hipStreamSynchronize(st));
     if (stats) {
-        float t = 0;
-        TP_HIP_OK(hipEventElapsedTime(&t, g_ev[0], g_ev[1]));
-        stats->ms = t;
+        float ms = 0;
+        TP_HIP_OK(hipEventElapsedTime(&ms, g_ev[0], g_ev[1]));
+        stats->ms = ms;
         stats->launches = 2;
         stats->layers = nl;
     }
@@ -626,8 +802,8 @@ bool prove(const Code& code, const fp2* d_cc, std::vector<long long> q,
         for (const fp2& x : c.ru) fe(x);
         for (const fp2& x : c.rv) fe(x);
         const fp2* p1 = &res[(size_t)v.res];
-        const fp2 vu = p1[3 * b], vv = p1[3 * b + 1];
-        const fp2* p2 = p1 + 3 * b + 3;
+        const fp2 vu = p1[3 * b], vv = res[(size_t)vv_res[(size_t)k]];
+        const fp2* p2 = p1 + 3 * b + 2;
         // direct_relay (verifier.cpp:754-766) over layer 1's bits
         fp2 drv = zero2();
         if (i == 1) {

@@ -15,18 +15,14 @@
 namespace tp {
 namespace virgo {
 
-// One expander graph of the code, resident on the device: forward (left
-// vertex i has edges i*deg .. i*deg + deg - 1 to right vertices to[] with
-// weights w[]) and reversed (right vertex t has edges rowptr[t] ..
-// rowptr[t + 1] - 1 from left vertices src[] with weights rw[]).
+// One expander graph of the code, resident on the device, forward and
+// degree-major: edge slot e of left vertex i goes to right vertex
+// to[e * L + i] with weight w[e * L + i].
 struct Graph {
     long long L = 0, R = 0;
     int deg = 0;
     const int* to = nullptr;
     const fp2* w = nullptr;
-    const int* rowptr = nullptr;
-    const int* src = nullptr;
-    const fp2* rw = nullptr;
 };
 
 // The code of one message length n: C[d] and D[d] for every depth d whose
