@@ -282,6 +282,9 @@ typedef struct tp_frame_job {
     const uint64_t* binding_int;    /* read where kind == 0 */
     const char* const* binding_str; /* read where kind == 1 */
     int keep_outputs;               /* keep the circuit outputs for tp_pool_outputs */
+    int inputs_on_device;           /* 1: inputs is device memory (HIP), e.g. rows
+                                     * that tp_yuv_pack built; complete when the job
+                                     * is submitted, and copied device to device */
 } tp_frame_job;
 
 /* A pool of n_workers provers (host threads, each with its own HIP stream and

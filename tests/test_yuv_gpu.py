@@ -116,3 +116,42 @@ def test_signer_commit_from_planes(bridge):
     w, h = 720, 480
     y, cb, cr = planes(w, h, 8)
     assert orion.commit_yuv(y, cb, cr, seed=99) == orion.commit(yo.pack(y, cb, cr), seed=99)
+
+
+def test_device_rows_in_both_drivers_and_copied_at_submit(circs):
+    """Rows built on the device from planes go to the prover without leaving
+    the GPU (tp_frame_job.inputs_on_device), in a stream worker and inside a
+    lockstep batch mixed with host-row frames; each proof equals the proof of
+    the same rows given from the host, and overwriting the device rows right
+    after submit changes nothing (the library copied them)."""
+    import torch
+    w, h = 160, 96
+    specs, want = [], []
+    rng = np.random.default_rng(9)
+    vole = tuple(rng.integers(1, P, size=500, dtype=np.int64).astype(np.uint64) for _ in range(4))
+    with frames.FramePool(1) as pool:
+        for k in range(4):
+            y, cb, cr = planes(w, h, 20 + k)
+            rows = yo.pack(y, cb, cr)
+            binding = commit_binding(make_manifest(rows.shape[0], seed=10 + k))
+            want.append(pool.result(pool.submit(circs["gray"], rows, 900 + k, vole, binding)).proof)
+            if k % 2:
+                specs.append((circs["gray"], rows, 900 + k, vole, binding))
+            else:
+                d = yuv.macroblocks_device(torch.from_numpy(y).cuda(), torch.from_numpy(cb).cuda(),
+                                           torch.from_numpy(cr).cuda())
+                torch.cuda.synchronize()
+                specs.append((circs["gray"], d, 900 + k, vole, binding))
+        d = specs[0][1]
+        jid = pool.submit(*specs[0])
+        d.fill_(0)                      # after submit: the job holds its own copy
+        torch.cuda.synchronize()
+        assert pool.result(jid).proof == want[0]
+    with frames.FramePool(1, batched=True) as bp:
+        got = [bp.result(j).proof for j in bp.submit_many(specs)]
+    assert got[1:] == want[1:]
+    y, cb, cr = planes(w, h, 20)
+    binding = commit_binding(make_manifest(yo.pack(y, cb, cr).shape[0], seed=10))
+    with frames.FramePool(1, batched=True) as bp:
+        tplanes = [torch.from_numpy(a).cuda() for a in (y, cb, cr)]
+        assert bp.result(bp.submit_yuv(circs["gray"], *tplanes, 900, vole, binding)).proof == want[0]

@@ -905,7 +905,7 @@ bool prove_batch(BatchCtx& B, std::vector<Job*>& jobs_in, TaskPool& tasks, doubl
     auto t_stage = Clock::now();
     tasks.parallel_for(nf, [&](int f) {
         const Job& J = *jobs[f];
-        if (S[f].in_bytes)
+        if (S[f].in_bytes && !J.dev_in.ptr)
             memcpy(B.h_in + in_off[f],
                    J.in_bytes == 1 ? (const void*)J.in8.data() : (const void*)J.in64.data(),
                    S[f].in_bytes);
@@ -913,6 +913,12 @@ bool prove_batch(BatchCtx& B, std::vector<Job*>& jobs_in, TaskPool& tasks, doubl
     prof[BP_STAGE_WALL] += ms_since(t_stage);
     auto t_launch = Clock::now();
     TP_HIP_OK(hipMemcpyAsync(base, B.h_in, in_total, hipMemcpyHostToDevice, st));
+    // frames submitted on the device: their inputs device to device, after
+    // the staged host inputs (which hold stale bytes in their places)
+    for (int f = 0; f < nf; ++f)
+        if (jobs[f]->dev_in.ptr && S[f].in_bytes)
+            TP_HIP_OK(hipMemcpyAsync(base + in_off[f], jobs[f]->dev_in.ptr, S[f].in_bytes,
+                                     hipMemcpyDeviceToDevice, st));
     TP_HIP_OK(hipMemcpyAsync(base + o_desc, B.h_stage, desc_bytes + item_bytes,
                              hipMemcpyHostToDevice, st));
     k_b_rows<<<L_rows.blocks, kBlock, 0, st>>>(dD, dI + L_rows.first, L_rows.n);

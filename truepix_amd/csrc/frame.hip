@@ -851,15 +851,18 @@ bool prove_frame(Worker& W, Job& J)
     // 1. inputs -> rows, evaluate the layer, outputs to the host (A3, A4)
     const size_t in_bytes = S.in_bytes, rows_n = S.rows_n, out_n = S.out_n;
     if (!W.host_in(in_bytes ? in_bytes : 8) || !W.host_out(out_n * 8)) return false;
-    memcpy(W.h_in, J.in_bytes == 1 ? (const void*)J.in8.data() : (const void*)J.in64.data(), in_bytes);
     if (!W.raw_in.ensure(in_bytes ? in_bytes : 8) || !W.rows_in.ensure(rows_n * 8) ||
         !W.rows_out.ensure(out_n * 8))
         return false;
-    if (in_bytes)
+    // inputs submitted on the device are read where they are
+    const void* raw = J.dev_in.ptr ? J.dev_in.ptr : W.raw_in.ptr;
+    if (!J.dev_in.ptr && in_bytes) {
+        memcpy(W.h_in, J.in_bytes == 1 ? (const void*)J.in8.data() : (const void*)J.in64.data(), in_bytes);
         TP_HIP_OK(hipMemcpyAsync(W.raw_in.ptr, W.h_in, in_bytes, hipMemcpyHostToDevice, st));
+    }
     k_fe_rows<<<grid_for(rows_n, kBlock, 8192), kBlock, 0, st>>>(
-        J.in_bytes == 1 ? W.raw_in.as<uint8_t>() : nullptr,
-        J.in_bytes == 1 ? nullptr : W.raw_in.as<u64>(), J.n_priv, J.n_copies, cp, nin,
+        J.in_bytes == 1 ? static_cast<const uint8_t*>(raw) : nullptr,
+        J.in_bytes == 1 ? nullptr : static_cast<const u64*>(raw), J.n_priv, J.n_copies, cp, nin,
         C.d_slot.as<int>(), C.d_cval.as<u64>(), W.rows_in.as<u64>());
     TP_HIP_OK(hipGetLastError());
     kern::k_eval_layer<<<grid_for(out_n, kBlock, 8192), kBlock, 0, st>>>(
@@ -1382,7 +1385,20 @@ std::unique_ptr<Job> make_job(const tp_frame_job* d)
     if (!S.set(*J)) return nullptr;
     J->footprint = S.footprint();
     const size_t n = (size_t)d->n_copies * d->n_private;
-    if (d->input_bytes == 1) {
+    if (d->inputs_on_device) {
+        // device to device into the job's own buffer, so the caller may free
+        // or reuse its memory as soon as submit returns
+        const size_t bytes = n * (size_t)d->input_bytes;
+        if (!d->inputs && bytes) return fail("tp_pool_submit: null device inputs"), nullptr;
+        if (!J->dev_in.ensure(bytes ? bytes : 8)) return nullptr;
+        if (bytes) {
+            hipStream_t st = stream();
+            if (hipMemcpyAsync(J->dev_in.ptr, d->inputs, bytes, hipMemcpyDeviceToDevice, st) !=
+                    hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                return fail("tp_pool_submit: copy of the device inputs failed"), nullptr;
+        }
+    } else if (d->input_bytes == 1) {
         const auto* s = static_cast<const uint8_t*>(d->inputs);
         J->in8.assign(s, s + n);
     } else {

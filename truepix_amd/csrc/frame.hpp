@@ -64,12 +64,17 @@ struct Job {
     Job() = default;
     Job(const Job&) = delete;
     Job& operator=(const Job&) = delete;
-    ~Job() { circuit_unref(circ); }   // the job's reference on its circuit
+    ~Job()
+    {
+        circuit_unref(circ);   // the job's reference on its circuit
+        dev_in.release();
+    }
     int64_t id = 0;
     const Circuit* circ = nullptr;     // referenced (circuit_ref) while the job lives
     int n_copies = 0, n_priv = 0, in_bytes = 1;
     std::vector<uint8_t> in8;
     std::vector<u64> in64;
+    DevBuf dev_in;                     // the inputs, when they were submitted on the device
     u64 pid = 0;
     std::vector<u64> ur, ui, vr, vi;
     std::vector<int> bkind;

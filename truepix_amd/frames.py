@@ -211,12 +211,24 @@ class FramePool:
 
     def submit_yuv(self, circuit: FrameCircuit, y, cb, cr, pid: int, vole, binding: list,
                    extra=None, keep_outputs: bool = False) -> int:
-        """Queue one frame given as YUV 4:2:0 planes: its per-copy rows are
-        built on the device (``yuv.macroblocks``; ``extra`` appends a byte per
-        copy, e.g. ``yuv.crop_bits`` for the crop circuit), then as ``submit``."""
+        """Queue one frame given as YUV 4:2:0 planes (uint8 numpy arrays, or
+        torch tensors already on the GPU): its per-copy rows are built on the
+        device (``yuv.macroblocks_device``; ``extra`` appends a byte per copy,
+        e.g. ``yuv.crop_bits`` for the crop circuit) and handed to the prover
+        there, so a frame crosses PCIe once, as planes."""
+        import torch
         from truepix_amd import yuv
-        return self.submit(circuit, yuv.macroblocks(y, cb, cr, extra), pid, vole, binding,
-                           keep_outputs)
+
+        def dev(a):
+            if isinstance(a, torch.Tensor):
+                return a if a.is_cuda else a.cuda()
+            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint8)).cuda()
+
+        ex = None if extra is None else dev(np.asarray(extra, dtype=np.uint8).reshape(-1)
+                                          if not isinstance(extra, torch.Tensor) else extra)
+        rows = yuv.macroblocks_device(dev(y), dev(cb), dev(cr), ex)
+        torch.cuda.current_stream().synchronize()   # the rows are complete before submit
+        return self.submit(circuit, rows, pid, vole, binding, keep_outputs)
 
     def submit_many(self, specs) -> List[int]:
         """Queue frames at once, in order: specs are submit()'s argument
@@ -235,24 +247,36 @@ class FramePool:
         """The C job description and the arrays it points into."""
         if not getattr(circuit, "handle", None):
             raise ValueError("the circuit is closed")
-        arr = np.asarray(inputs)
-        if arr.ndim != 2 or arr.shape[1] != circuit.n_private:
-            raise ValueError("inputs must be [n_copies][%d]" % circuit.n_private)
-        if arr.dtype == np.uint8 or (arr.size and arr.min() >= 0 and arr.max() <= 255):
-            data = np.ascontiguousarray(arr, dtype=np.uint8)
-            nbytes = 1
+        on_device = type(inputs).__module__.startswith("torch") and getattr(inputs, "is_cuda", False)
+        if on_device:
+            # rows resident on the GPU (uint8, contiguous, complete): the
+            # library copies them device to device at submit
+            import torch
+            if inputs.dtype != torch.uint8 or inputs.dim() != 2 or \
+                    inputs.shape[1] != circuit.n_private or not inputs.is_contiguous():
+                raise ValueError("device inputs must be a contiguous uint8 [n_copies][%d] tensor"
+                                 % circuit.n_private)
+            data, nbytes, n_copies, ptr = inputs, 1, int(inputs.shape[0]), inputs.data_ptr()
         else:
-            data = np.ascontiguousarray(np.asarray(arr, dtype=object) % P, dtype=np.uint64)
-            nbytes = 8
+            arr = np.asarray(inputs)
+            if arr.ndim != 2 or arr.shape[1] != circuit.n_private:
+                raise ValueError("inputs must be [n_copies][%d]" % circuit.n_private)
+            if arr.dtype == np.uint8 or (arr.size and arr.min() >= 0 and arr.max() <= 255):
+                data = np.ascontiguousarray(arr, dtype=np.uint8)
+                nbytes = 1
+            else:
+                data = np.ascontiguousarray(np.asarray(arr, dtype=object) % P, dtype=np.uint64)
+                nbytes = 8
+            n_copies, ptr = arr.shape[0], data.ctypes.data
         ur, ui, vr, vi = (np.ascontiguousarray(a, np.uint64) for a in vole)
         kinds = np.array([1 if isinstance(b, str) else 0 for b in binding], np.int32)
         ints = np.array([0 if isinstance(b, str) else int(b) for b in binding], np.uint64)
         strs = (ctypes.c_char_p * max(1, len(binding)))(
             *[b.encode("utf-8") if isinstance(b, str) else None for b in binding])
-        job = nb.FrameJob(circuit.handle, arr.shape[0], circuit.n_private, nbytes,
-                        data.ctypes.data, int(pid), ur.size, _p(ur, _U64P), _p(ui, _U64P),
-                        _p(vr, _U64P), _p(vi, _U64P), len(binding), _p(kinds, _I32P),
-                        _p(ints, _U64P), strs, 1 if keep_outputs else 0)
+        job = nb.FrameJob(circuit.handle, n_copies, circuit.n_private, nbytes, ptr, int(pid),
+                          ur.size, _p(ur, _U64P), _p(ui, _U64P), _p(vr, _U64P), _p(vi, _U64P),
+                          len(binding), _p(kinds, _I32P), _p(ints, _U64P), strs,
+                          1 if keep_outputs else 0, 1 if on_device else 0)
         return job, (data, ur, ui, vr, vi, kinds, ints, strs)
 
     def wait(self, job: int, timeout: Optional[float] = None) -> bool:

@@ -59,6 +59,7 @@ class FrameJob(ctypes.Structure):
         ("vole_u_re", _U64P), ("vole_u_im", _U64P), ("vole_v_re", _U64P), ("vole_v_im", _U64P),
         ("n_binding", _I32), ("binding_kind", _I32P), ("binding_int", _U64P),
         ("binding_str", ctypes.POINTER(ctypes.c_char_p)), ("keep_outputs", _I32),
+        ("inputs_on_device", _I32),
     ]
 
 
