@@ -654,12 +654,20 @@ bool commit_device(int copies, const uint8_t* inputs, uint64_t seed, hipStream_t
 struct OrionFs {
     uint8_t state[32], pool[32];
     int used = 32;
+    // SHA-256 of one 64-byte message: the message block and the constant
+    // padding block (0x80, zeros, bit length 512) in one compression call
     static void hash64(const uint8_t b[64], uint8_t out[32])
     {
-        HostSha256 h;
-        h.init();
-        h.update(b, 64);
-        h.peek(out);
+        alignas(16) uint8_t m[128];
+        memcpy(m, b, 64);
+        memset(m + 64, 0, 64);
+        m[64] = 0x80;
+        m[126] = 0x02;   // 512 bits, big-endian
+        uint32_t st[8] = {0x6a09e667u, 0xbb67ae85u, 0x3c6ef372u, 0xa54ff53au,
+                          0x510e527fu, 0x9b05688cu, 0x1f83d9abu, 0x5be0cd19u};
+        sha256_blocks(st, m, 2);
+        for (int k = 0; k < 8; ++k)
+            for (int j = 0; j < 4; ++j) out[4 * k + j] = (uint8_t)(st[k] >> (24 - 8 * j));
     }
     void reset()
     {
