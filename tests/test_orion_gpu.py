@@ -161,11 +161,11 @@ def test_open_nizk_virgo_matches_reference_transcripts(bridge, gold):
 
 @pytest.mark.parametrize("copies,seed,group", [(2, 9, False), (7, 31, True), (64, 5, False),
                                                (700, 2 ** 61 + 3, True), (3600, 11, False),
-                                               (8040, 12, True)])
+                                               (8040, 12, True), (20000, 13, False)])
 def test_open_nizk_virgo_equals_oracle(bridge, oracle, copies, seed, group):
     """The code-switching transcript equals the oracle's (pinned to the
-    reference) on fresh frames, both polynomials, up to the 8040-copy frame
-    (a 2^15-entry message, 2^17-gate layers)."""
+    reference) on fresh frames, both polynomials, up to a 20000-copy frame
+    (a 2^17-entry message, 2^19-gate layers: eq half-tables of 2^10 in LDS)."""
     x = inputs_for(copies, copies + 17)
     pc = 1 << max(0, (copies - 1).bit_length())
     lg = (pc * 512).bit_length() - 1

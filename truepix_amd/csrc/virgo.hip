@@ -56,7 +56,7 @@ namespace virgo {
 namespace {
 
 constexpr int kSmall = 13;          // distance_threshold (OZ/include/linear_code/parameter.h)
-constexpr int kMaxBits = 18;        // widest layer handled: 2^18 gates (a message of 2^16)
+constexpr int kMaxBits = 20;        // widest layer handled: 2^20 gates (a message of 2^18, 65536 copies)
 constexpr int kKindDirect = 0, kKindCode = 1, kKindQuery = 2;
 constexpr int kRoleA = 0, kRoleCheck = 1, kRoleDot = 2;
 constexpr int kDotV = 0, kDotOut = 1, kDotIn = 2;
