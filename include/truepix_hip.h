@@ -358,7 +358,15 @@ void tp_pool_release(void* pool, int64_t job);
  *      (OZ/src/VPD/linearPC.cpp:253-459) as OZ/examples/multivariate_PC_test_commit.cpp:265-331
  *      runs it (layout and mask of OZ/include/truepix/truepix_gkr.h) ------------ */
 
-/* inputs: [copies][384] sample values (the lines of input.txt); seed: the
+/* The packing of a copy that every later commit and opening uses
+ * (OZ/include/truepix/truepix_gkr.h:107-150, make_layout, where the
+ * reference's programs read it from TRUEPIX_COPY_SIZE, TRUEPIX_CONST_IDX and
+ * TRUEPIX_CONST_VAL): copy_size samples (1..512), the constant const_val at
+ * wire const_idx (-1: none; outside the samples), zeros up to 512.  The
+ * circuits use gray 384/384/128 (the default), invert 384/384/255, mask
+ * 384/-1/0 and crop 385/-1/0 (TP/libTruePix/fp2_link.py:533-563).  0 / -1. */
+int tp_orion_set_layout(int copy_size, int const_idx, int const_val);
+/* inputs: [copies][copy_size] sample values (the lines of input.txt); seed: the
  * mask seed (orion_mask_seed.txt).  Writes the masked and mask Merkle roots
  * as orion_commit.json carries them (64 hex digits + NUL); ms (optional):
  * device time.  copies <= 65536.  0 / -1. */

@@ -996,7 +996,23 @@ static void root_hex(const unsigned char d[32], char out[65])
     out[64] = 0;
 }
 
-/* inputs [copies][384]; 0 / -1 */
+/* The packing of a copy (truepix_gkr.h:107-150, from TRUEPIX_COPY_SIZE,
+ * TRUEPIX_CONST_IDX, TRUEPIX_CONST_VAL in the reference's programs): copy_size
+ * samples, the constant const_val at wire const_idx (none when -1), zeros up
+ * to 512; the mask covers the samples only.  Default: the gray circuit's
+ * 384, 384, 128. */
+static int X_CS = 384, X_CI = 384, X_CV = 128;
+int or_orion_set_layout(int copy_size, int const_idx, int const_val)
+{
+    if (copy_size < 1 || copy_size > 512 || const_idx >= 512 || (const_idx >= 0 && const_idx < copy_size))
+        return -1;
+    X_CS = copy_size;
+    X_CI = const_idx;
+    X_CV = const_val;
+    return 0;
+}
+
+/* inputs [copies][copy_size]; 0 / -1 */
 int or_orion_commit(int copies, const unsigned char* inputs, u64 seed, char masked_root[65],
                     char mask_root[65], long long* cw_len_out)
 {
@@ -1013,13 +1029,14 @@ int or_orion_commit(int copies, const unsigned char* inputs, u64 seed, char mask
     poly[1] = (fp2*)calloc((size_t)n, sizeof(fp2));   /* mask */
     u64 state = seed;
     for (long long c = 0; c < copies; ++c)
-        for (int w = 0; w < 384; ++w) {
+        for (int w = 0; w < X_CS; ++w) {
             state += 0x9E3779B97F4A7C15ULL;
             const u64 m = smix(state) % P61;
             poly[1][c * 512 + w] = mk(m, 0);
-            poly[0][c * 512 + w] = mk(fadd(inputs[c * 384 + w], m), 0);
+            poly[0][c * 512 + w] = mk(fadd(inputs[c * X_CS + w], m), 0);
         }
-    for (long long c = 0; c < copies; ++c) poly[0][c * 512 + 384] = mk(128, 0);
+    if (X_CI >= 0)
+        for (long long c = 0; c < copies; ++c) poly[0][c * 512 + X_CI] = mk((u64)X_CV, 0);
     fp2* cwb = (fp2*)malloc(sizeof(fp2) * (size_t)X_COLS * cw);
     unsigned char* tree = (unsigned char*)malloc((size_t)(2 * rows) * 32);
     char* roots[2] = {masked_root, mask_root};
@@ -1109,13 +1126,13 @@ int or_orion_open(int copies, const unsigned char* inputs, u64 seed, int use_gro
     fp2* poly = (fp2*)calloc((size_t)n, sizeof(fp2));
     u64 state = seed;
     for (long long c = 0; c < copies; ++c)
-        for (int w = 0; w < 384; ++w) {
+        for (int w = 0; w < X_CS; ++w) {
             state += 0x9E3779B97F4A7C15ULL;
             const u64 m = smix(state) % P61;
-            poly[c * 512 + w] = use_group1 ? mk(fadd(inputs[c * 384 + w], m), 0) : mk(m, 0);
+            poly[c * 512 + w] = use_group1 ? mk(fadd(inputs[c * X_CS + w], m), 0) : mk(m, 0);
         }
-    if (use_group1)
-        for (long long c = 0; c < copies; ++c) poly[c * 512 + 384] = mk(128, 0);
+    if (use_group1 && X_CI >= 0)
+        for (long long c = 0; c < copies; ++c) poly[c * 512 + X_CI] = mk((u64)X_CV, 0);
     fp2* cwb = (fp2*)malloc(sizeof(fp2) * (size_t)X_COLS * cw);
     for (int c = 0; c < X_COLS; ++c) x_encode(poly + c * col, col, 0, cwb + (long long)c * cw);
     unsigned char* tcom = (unsigned char*)malloc((size_t)(2 * rows) * 32);

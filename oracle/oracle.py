@@ -181,6 +181,14 @@ _COMMIT_ARGS = [_INT, ctypes.c_char_p, _U64, ctypes.c_char_p, ctypes.c_char_p,
                 ctypes.POINTER(ctypes.c_longlong)]
 
 
+def _oracle_orion_set_layout(self, copy_size: int, const_idx: int, const_val: int):
+    """The packing of a copy for every later commit and opening (default
+    384, 384, 128)."""
+    if _bind(self.lib, "or_orion_set_layout", [_INT, _INT, _INT], _INT)(copy_size, const_idx,
+                                                                      const_val) != 0:
+        raise ValueError("not a packing: %d %d %d" % (copy_size, const_idx, const_val))
+
+
 def _oracle_orion_commit(self, inputs, seed: int):
     """(masked_root, mask_root, cw_len) of the signer's commitment."""
     if not hasattr(self, "_orc"):
@@ -259,6 +267,7 @@ def _oracle_orion_draw_queries(self, root: bytes, n: int, use_group1: bool, cw_l
 Oracle.orion_open = _oracle_orion_open
 Oracle.orion_draw_queries = _oracle_orion_draw_queries
 Oracle.orion_commit = _oracle_orion_commit
+Oracle.orion_set_layout = _oracle_orion_set_layout
 Oracle.orion_encode = _oracle_orion_encode
 Oracle.sha256 = _oracle_sha256
 
@@ -334,6 +343,14 @@ class RefOrion:
         lib = ctypes.CDLL(self.PATH)
         self.c = _bind(lib, "ref_orion_commit", _COMMIT_ARGS, _INT)
         self.e = _bind(lib, "ref_orion_encode", [ctypes.c_longlong, _U64P, _U64P], ctypes.c_longlong)
+
+    @staticmethod
+    def set_layout(copy_size: int, const_idx: int, const_val: int):
+        """The packing, through the environment variables the reference's
+        make_layout reads (truepix_gkr.h:107-150) at every call."""
+        os.environ["TRUEPIX_COPY_SIZE"] = str(copy_size)
+        os.environ["TRUEPIX_CONST_IDX"] = str(const_idx)
+        os.environ["TRUEPIX_CONST_VAL"] = str(const_val)
 
     def commit(self, inputs, seed: int):
         return _orion_commit(self.c, inputs, seed)

@@ -36,10 +36,17 @@ ENCODE_SIZES = [5, 13, 14, 31, 64, 100, 1024, 8192]
 # queries); the point and the queries from PCG64(point seed)
 OPEN_CASES = [(1, 5, 1, 1, 21, 8), (3, 7, 2, 0, 22, 16), (16, 2024, 4, 1, 23, 64),
               (300, 12345, 6, 0, 24, 200), (1350, 777, 7, 1, 25, 1222)]
+# the other circuits' packings (copy_size, const_idx, const_val), as
+# TP/libTruePix/fp2_link.py:533-563 assigns them: commit roots, the NIZK
+# opening's proof bytes and its code-switching transcript, per (name, packing,
+# copies, mask seed, input seed, use_group1, point seed)
+LAYOUT_CASES = [("inv", (384, 384, 255), 16, 31, 8, 1, 41),
+                ("mask", (384, -1, 0), 40, 32, 9, 0, 42),
+                ("crop", (385, -1, 0), 300, 33, 10, 1, 43)]
 
 
-def inputs_for(copies, seed):
-    return np.random.Generator(np.random.PCG64(seed)).integers(0, 256, size=(copies, 384),
+def inputs_for(copies, seed, copy_size=384):
+    return np.random.Generator(np.random.PCG64(seed)).integers(0, 256, size=(copies, copy_size),
                                                                dtype=np.uint8)
 
 
@@ -150,6 +157,30 @@ def main():
                              "verdict": int(words[-1]),
                              "tail": [int(x) for x in words[-5:]],
                              "sha256": hashlib.sha256(words.astype("<u8").tobytes()).hexdigest()})
+    # the other packings, through the environment the reference reads
+    from oracle.oracle import RefOrion as _R
+    out["layouts"] = []
+    for name, lay, copies, seed, iseed, group, pseed in LAYOUT_CASES:
+        _R.set_layout(*lay)
+        try:
+            x = inputs_for(copies, iseed, lay[0])
+            m, k, cw = ref.commit(x, seed)
+            r, _ = open_args(copies, pseed, 2, cw)
+            pc = 1 << max(0, (copies - 1).bit_length())
+            q = ref.draw_queries(root_bytes(m if group else k), pc * 512, bool(group), cw, r)
+            proof = ref.open(x, seed, r, np.asarray(q, np.int64), use_group1=bool(group))
+            f = opening_fields(proof)
+            words = virgo.transcript(f["com_root"], pc * 512, bool(group), r, f["combined_root"],
+                                     f["combined_message"])
+        finally:
+            _R.set_layout(384, 384, 128)
+        out["layouts"].append({"name": name, "layout": list(lay), "copies": copies, "seed": seed,
+                               "input_seed": iseed, "use_group1": group, "point_seed": pseed,
+                               "masked_root": m, "mask_root": k, "cw_len": cw,
+                               "proof_len": len(proof),
+                               "proof_sha256": hashlib.sha256(proof).hexdigest(),
+                               "virgo_words": int(words.size), "virgo_verdict": int(words[-1]),
+                               "virgo_sha256": hashlib.sha256(words.astype("<u8").tobytes()).hexdigest()})
     with open(os.path.join(HERE, "orion_ref.json"), "w") as fh:
         json.dump(out, fh, indent=1)
     print("wrote", len(out["commit"]), "commit cases,", len(out["encode"]), "encode cases,",

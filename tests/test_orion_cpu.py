@@ -334,3 +334,74 @@ def test_virgo_checks_catch_planted_mistakes(orc):
         orc.orion_virgo_plant(0)
     words, _ = _virgo_case(orc, 16, 2024, 4, True, 23)
     assert words[-1] == 1
+
+
+# ------------------------------------------------ the circuits' packings
+
+def _layout_inputs(copies, seed, copy_size):
+    return np.random.Generator(np.random.PCG64(seed)).integers(0, 256, size=(copies, copy_size),
+                                                               dtype=np.uint8)
+
+
+def test_packings_match_reference(orc, gold):
+    """The invert, mask and crop circuits' packings (constant 255, no
+    constant, 385 samples): commitment roots, NIZK opening bytes and the
+    code-switching transcript against the reference's (orion_ref.json)."""
+    from truepix_amd.orion import parse_opening
+    try:
+        for case in gold["layouts"]:
+            orc.orion_set_layout(*case["layout"])
+            x = _layout_inputs(case["copies"], case["input_seed"], case["layout"][0])
+            m, k, cw = orc.orion_commit(x, case["seed"])
+            assert (m, k, cw) == (case["masked_root"], case["mask_root"], case["cw_len"]), case["name"]
+            r, _ = open_args(case["copies"], case["point_seed"], 2, cw)
+            group = bool(case["use_group1"])
+            pc = 1 << max(0, (case["copies"] - 1).bit_length())
+            q = orc.orion_draw_queries(root_bytes(m if group else k), pc * 512, group, cw, r)
+            proof, _ = orc.orion_open(x, case["seed"], r, np.asarray(q, np.int64), use_group1=group)
+            assert hashlib.sha256(proof).hexdigest() == case["proof_sha256"], case["name"]
+            o = parse_opening(proof)
+            words = orc.orion_virgo(o["com_root"], pc * 512, group, r, o["combined_root"],
+                                    o["combined_message"])
+            assert hashlib.sha256(words.astype("<u8").tobytes()).hexdigest() == case["virgo_sha256"]
+    finally:
+        orc.orion_set_layout(384, 384, 128)
+
+
+def test_packing_changes_only_what_it_should(orc):
+    """A constant changes only the masked root (the mask covers samples
+    only); dropping the constant changes it again; the crop packing's 385th
+    sample enters the mask stream, so both roots change."""
+    x = _layout_inputs(9, 3, 385)
+    try:
+        orc.orion_set_layout(384, 384, 128)
+        g = orc.orion_commit(x[:, :384], 7)
+        orc.orion_set_layout(384, 384, 255)
+        i = orc.orion_commit(x[:, :384], 7)
+        orc.orion_set_layout(384, -1, 0)
+        n = orc.orion_commit(x[:, :384], 7)
+        orc.orion_set_layout(385, -1, 0)
+        c = orc.orion_commit(x, 7)
+        assert g[1] == i[1] == n[1] and len({g[0], i[0], n[0]}) == 3
+        assert c[0] not in (g[0], n[0]) and c[1] != g[1]
+        with pytest.raises(ValueError):
+            orc.orion_set_layout(385, 384, 128)   # the constant inside the samples
+    finally:
+        orc.orion_set_layout(384, 384, 128)
+
+
+@pytest.mark.skipif(not RefOrion.available(), reason="reference build absent (make -C oracle ref)")
+def test_packings_match_live_reference(orc):
+    ref = RefOrion()
+    try:
+        for lay, copies, seed in (((384, 384, 255), 5, 2), ((384, -1, 0), 33, 4), ((385, -1, 0), 70, 6)):
+            orc.orion_set_layout(*lay)
+            RefOrion.set_layout(*lay)
+            x = _layout_inputs(copies, seed, lay[0])
+            assert orc.orion_commit(x, seed) == ref.commit(x, seed), lay
+            _, _, cw = ref.commit(x, seed)
+            r, q = open_args(copies, seed, 20, cw)
+            assert orc.orion_open(x, seed, r, q, use_group1=True)[0] == ref.open(x, seed, r, q, True)
+    finally:
+        orc.orion_set_layout(384, 384, 128)
+        RefOrion.set_layout(384, 384, 128)

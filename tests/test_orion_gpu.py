@@ -180,3 +180,28 @@ def test_open_nizk_virgo_equals_oracle(bridge, oracle, copies, seed, group):
     assert np.array_equal(words, want)
     ms, launches, layers = orion.virgo_stats()
     assert launches == 2 and layers == int(words[0]) - 1 and ms > 0
+
+
+def test_packings_match_reference(bridge, gold):
+    """The invert, mask and crop circuits' packings on the device: roots, NIZK
+    opening bytes and code-switching transcript equal the reference's
+    (orion_ref.json); the gray packing is restored after."""
+    try:
+        for case in gold["layouts"]:
+            lay = tuple(case["layout"])
+            x = np.random.Generator(np.random.PCG64(case["input_seed"])).integers(
+                0, 256, size=(case["copies"], lay[0]), dtype=np.uint8)
+            print("packing", case["name"], lay, case["copies"], flush=True)
+            assert orion.commit(x, case["seed"], layout=lay) == (case["masked_root"], case["mask_root"])
+            r = _open_args(case["copies"], case["point_seed"], case["cw_len"])
+            proof, _, words = orion.opening_nizk_virgo(x, case["seed"], r, bool(case["use_group1"]),
+                                                       layout=lay)
+            assert proof is not None and len(proof) == case["proof_len"]
+            assert hashlib.sha256(proof).hexdigest() == case["proof_sha256"], case["name"]
+            assert hashlib.sha256(words.astype("<u8").tobytes()).hexdigest() == case["virgo_sha256"]
+            man = orion.manifest(x, case["seed"], layout=lay)
+            assert (man["copy_size"], man["const_idx"], man["const_val"]) == lay
+    finally:
+        orion.commit(inputs_for(2, 1), 1)   # back to the gray packing
+    with pytest.raises(ValueError, match="packing"):   # the constant inside the samples
+        orion.commit(np.zeros((2, 385), np.uint8), 1, layout=(385, 384, 128))

@@ -5,10 +5,12 @@
 // transcript absorbs as the commitment binding.
 //
 // What is committed (OZ/include/truepix/truepix_gkr.h:85-386):
-//   n = next_pow2(copies) * 512 coefficients, copy-major: the copy's 384
-//   samples, the constant 128 at wire 384, zero elsewhere; the mask's real
-//   part is splitmix64(seed) % p at the k-th sample position, in order; the
-//   masked polynomial is coefficients + mask.  Each polynomial is split into
+//   n = next_pow2(copies) * 512 coefficients, copy-major: the copy's
+//   copy_size samples, a constant at wire const_idx (none when -1), zero
+//   elsewhere (the circuit's packing: gray 384 samples and 128 at 384,
+//   invert 255 at 384, mask 384 samples and no constant, crop 385 samples and
+//   no constant); the mask's real part is splitmix64(seed) % p at the k-th
+//   sample position, in order; the masked polynomial is coefficients + mask.  Each polynomial is split into
 //   128 columns of col = n / 128 coefficients (column c = flat indices
 //   c col .. c col + col - 1).
 // Encoding (OZ/include/linear_code/expanders.h:19-60,
@@ -164,8 +166,8 @@ __device__ __forceinline__ uint32_t be32(uint32_t x) { return __builtin_bswap32(
 // ------------------------------------------------------------- kernels ----
 
 // Both polynomials as rows of 256 lanes (rows 0 .. col - 1).
-__global__ void k_or_fill(const uint8_t* __restrict__ in, int copies, u64 seed, long long col,
-                          V2* __restrict__ X)
+__global__ void k_or_fill(const uint8_t* __restrict__ in, int copies, int copy_size, int const_idx,
+                          u64 const_val, u64 seed, long long col, V2* __restrict__ X)
 {
     const size_t total = (size_t)col * kLanes;
     for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
@@ -177,16 +179,16 @@ __global__ void k_or_fill(const uint8_t* __restrict__ in, int copies, u64 seed, 
         const int wire = (int)(flat & 511);
         u64 re = 0;
         if (copy < (size_t)copies) {
-            if (wire < 384) {
-                const u64 rank = copy * 384 + wire;
+            if (wire < copy_size) {
+                const u64 rank = copy * copy_size + wire;
                 u64 z = seed + (rank + 1) * kGolden;
                 z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
                 z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
                 z ^= z >> 31;
                 const u64 mask = fold61(z);
-                re = lane < kCols ? fadd((u64)in[copy * 384 + wire], mask) : mask;
-            } else if (wire == 384 && lane < kCols) {
-                re = 128;
+                re = lane < kCols ? fadd((u64)in[copy * copy_size + wire], mask) : mask;
+            } else if (wire == const_idx && lane < kCols) {
+                re = const_val;
             }
         }
         V2 v;
@@ -603,6 +605,14 @@ bool encode_all(const Plan& P, hipStream_t st)
 
 DevBuf g_open, g_ctree;   // the opening's vectors and the combined codeword's tree
 
+// The packing of a copy (truepix_gkr.h:107-150; the reference's programs
+// read it from TRUEPIX_COPY_SIZE, TRUEPIX_CONST_IDX, TRUEPIX_CONST_VAL).
+struct Packing {
+    int copy_size = 384, const_idx = 384;
+    u64 const_val = 128;
+};
+Packing g_pack;
+
 // Fills, encodes and hashes both polynomials of a frame and builds both
 // trees (g_X, g_tree); the plan and the row count come back.
 bool commit_device(int copies, const uint8_t* inputs, uint64_t seed, hipStream_t st,
@@ -613,13 +623,14 @@ bool commit_device(int copies, const uint8_t* inputs, uint64_t seed, hipStream_t
     const long long n = pc * 512, col = n / kCols, rows = 2 * col;
     Plan* P = plan_for(col);
     if (!P) return false;
-    const size_t in_bytes = (size_t)copies * 384;
+    const size_t in_bytes = (size_t)copies * g_pack.copy_size;
     if (!g_in.ensure(in_bytes) || !g_X.ensure((size_t)P->cw * kLanes * sizeof(V2)) ||
         !g_tree.ensure((size_t)2 * 2 * rows * 8 * 4))
         return false;
     TP_HIP_OK(hipMemcpyAsync(g_in.ptr, inputs, in_bytes, hipMemcpyHostToDevice, st));
     k_or_fill<<<grid_for((size_t)col * kLanes, kBlock, 8192), kBlock, 0, st>>>(
-        g_in.as<uint8_t>(), copies, seed, col, g_X.as<V2>());
+        g_in.as<uint8_t>(), copies, g_pack.copy_size, g_pack.const_idx, g_pack.const_val, seed, col,
+        g_X.as<V2>());
     TP_HIP_OK(hipGetLastError());
     if (!encode_all(*P, st)) return false;
     const long long span = P->cw < rows ? P->cw + 1 : P->cw;
@@ -815,7 +826,26 @@ void hex_root(const uint32_t st[8], char out[65])
 
 extern "C" {
 
-// The signer's Orion commitment of one frame: inputs [copies][384] sample
+// The packing every later commit and opening uses: copy_size samples per
+// copy (1..512), the constant const_val at wire const_idx (-1: none; outside
+// the samples, below 512).  Validated as make_layout does
+// (truepix_gkr.h:107-150).  0 / -1.
+int tp_orion_set_layout(int copy_size, int const_idx, int const_val)
+{
+    clear_error();
+    if (copy_size < 1 || copy_size > 512 || const_idx < -1 || const_idx >= 512 ||
+        (const_idx >= 0 && const_idx < copy_size) || const_val < 0)
+        return fail("tp_orion_set_layout: copy_size=%d const_idx=%d const_val=%d is not a packing "
+                    "(samples within 512, the constant outside them)",
+                    copy_size, const_idx, const_val), -1;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_pack.copy_size = copy_size;
+    g_pack.const_idx = const_idx;
+    g_pack.const_val = (u64)const_val;
+    return 0;
+}
+
+// The signer's Orion commitment of one frame: inputs [copies][copy_size] sample
 // values (the lines of input.txt), the mask seed; writes the masked and mask
 // Merkle roots as orion_commit.json carries them (64 hex digits + NUL) and,
 // if ms is not null, the device time in milliseconds.  0 / -1.

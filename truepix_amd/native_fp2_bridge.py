@@ -124,6 +124,7 @@ def bind(lib) -> None:
     _sig(lib, "tp_orion_commit", [_I32, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_char_p,
                                   ctypes.c_char_p, ctypes.POINTER(_D)], _I32)
     _sig(lib, "tp_orion_encode", [ctypes.c_longlong, _I32, _U64P, _U64P], ctypes.c_longlong)
+    _sig(lib, "tp_orion_set_layout", [_I32, _I32, _I32], _I32)
     _sig(lib, "tp_orion_open", [_I32, ctypes.c_char_p, ctypes.c_uint64, _I32, _U64P, _I32, _VP, _I32,
                                 ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t),
                                 _U64P], _I32)

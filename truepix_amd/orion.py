@@ -2,7 +2,9 @@
 
 The reference's commit phase is a program, ``multivariate_PC_test_commit``
 (OZ/examples/multivariate_PC_test_commit.cpp:265-331): it packs the frame's
-input.txt into the copy-major-384-const128-pad512 layout, masks it with a
+input.txt into the copy-major pad-512 layout with the circuit's packing
+(``layout``: copy_size samples and a constant wire; gray 384/384/128,
+invert 384/384/255, mask 384/-1/0, crop 385/-1/0), masks it with a
 seeded mask (OZ/include/truepix/truepix_gkr.h:286-386), commits the masked and
 the mask polynomials with commit_pair (OZ/src/VPD/linearPC.cpp:253-459: the
 expander code per column, SHA-256 row chains, a Merkle tree over the rows) and
@@ -28,29 +30,42 @@ COPY_SIZE = 384
 _U64P = ctypes.POINTER(ctypes.c_uint64)
 
 
-def commit(inputs, seed: int, timing: bool = False):
-    """(masked_root, mask_root) hex strings of the signer's commitment of
-    inputs [copies][384] (sample values 0..255) with the mask seed; with
-    timing, also the device milliseconds."""
-    nb.require()
+GRAY_LAYOUT = (384, 384, 128)   # (copy_size, const_idx, const_val): the gray circuit's packing
+
+
+def _samples(inputs, layout):
+    """The frame's samples as bytes, with the library set to the packing
+    (copy_size samples per copy, the constant const_val at wire const_idx or
+    none when -1; truepix_gkr.h:107-150).  The packing is process-wide in the
+    library, as the reference's programs take it from their environment."""
+    copy_size, const_idx, const_val = (int(x) for x in layout)
     arr = np.ascontiguousarray(inputs)
-    if arr.ndim != 2 or arr.shape[1] != COPY_SIZE:
-        raise ValueError("inputs must be [copies][%d]" % COPY_SIZE)
+    if arr.ndim != 2 or arr.shape[1] != copy_size:
+        raise ValueError("inputs must be [copies][%d]" % copy_size)
     if arr.size and (arr.min() < 0 or arr.max() > 255):
         raise ValueError("sample values must be in 0..255")
-    data = np.ascontiguousarray(arr, dtype=np.uint8)
+    if nb.LIB.tp_orion_set_layout(copy_size, const_idx, const_val) != 0:
+        raise ValueError(nb.last_error())
+    return int(arr.shape[0]), np.ascontiguousarray(arr, dtype=np.uint8).tobytes()
+
+
+def commit(inputs, seed: int, timing: bool = False, layout=GRAY_LAYOUT):
+    """(masked_root, mask_root) hex strings of the signer's commitment of
+    inputs [copies][copy_size] (sample values 0..255) with the mask seed and
+    the circuit's packing; with timing, also the device milliseconds."""
+    nb.require()
+    copies, data = _samples(inputs, layout)
     a = ctypes.create_string_buffer(65)
     b = ctypes.create_string_buffer(65)
     ms = ctypes.c_double()
-    rc = nb.LIB.tp_orion_commit(int(arr.shape[0]), data.tobytes(), int(seed) & (2 ** 64 - 1), a, b,
-                                ctypes.byref(ms))
+    rc = nb.LIB.tp_orion_commit(copies, data, int(seed) & (2 ** 64 - 1), a, b, ctypes.byref(ms))
     if rc != 0:
         raise RuntimeError("tp_orion_commit failed: %s" % nb.last_error())
     roots = (a.value.decode(), b.value.decode())
     return roots + (ms.value,) if timing else roots
 
 
-def opening(inputs, seed: int, r, col_q, use_group1: bool = True):
+def opening(inputs, seed: int, r, col_q, use_group1: bool = True, layout=GRAY_LAYOUT):
     """The opening of the committed masked polynomial (use_group1) or the mask
     at the point r (lg n challenges, each (re, im)), at the code positions
     col_q the transcript drew: (proof bytes in the reference's OpeningProof
@@ -58,16 +73,13 @@ def opening(inputs, seed: int, r, col_q, use_group1: bool = True):
     700-966 (tensor_product_protocol) up to its Virgo code-switching proof,
     which is not part of it."""
     nb.require()
-    arr = np.ascontiguousarray(inputs)
-    if arr.ndim != 2 or arr.shape[1] != COPY_SIZE:
-        raise ValueError("inputs must be [copies][%d]" % COPY_SIZE)
-    data = np.ascontiguousarray(arr, dtype=np.uint8)
+    copies, data = _samples(inputs, layout)
     rr = np.ascontiguousarray(np.asarray(r, dtype=np.uint64).reshape(-1))
     q = np.ascontiguousarray(col_q, dtype=np.int64)
     buf = ctypes.c_void_p()
     n = ctypes.c_size_t()
     ans = np.zeros(2, np.uint64)
-    rc = nb.LIB.tp_orion_open(int(arr.shape[0]), data.tobytes(), int(seed) & (2 ** 64 - 1),
+    rc = nb.LIB.tp_orion_open(copies, data, int(seed) & (2 ** 64 - 1),
                               1 if use_group1 else 0, rr.ctypes.data_as(_U64P), rr.size // 2,
                               q.ctypes.data, q.size, ctypes.byref(buf), ctypes.byref(n),
                               ans.ctypes.data_as(_U64P))
@@ -78,21 +90,18 @@ def opening(inputs, seed: int, r, col_q, use_group1: bool = True):
     return proof, (int(ans[0]), int(ans[1]))
 
 
-def opening_nizk(inputs, seed: int, r, use_group1: bool = True):
+def opening_nizk(inputs, seed: int, r, use_group1: bool = True, layout=GRAY_LAYOUT):
     """``opening`` with the code positions drawn as the reference's NIZK mode
     draws them (OZ/src/truepix/orion_fs.cpp's transcript over the label
     "tensor-open", the opened root, N, the group flag, r0 and r1; 1222
     positions from the opened group, linearPC.cpp:711-758)."""
     nb.require()
-    arr = np.ascontiguousarray(inputs)
-    if arr.ndim != 2 or arr.shape[1] != COPY_SIZE:
-        raise ValueError("inputs must be [copies][%d]" % COPY_SIZE)
-    data = np.ascontiguousarray(arr, dtype=np.uint8)
+    copies, data = _samples(inputs, layout)
     rr = np.ascontiguousarray(np.asarray(r, dtype=np.uint64).reshape(-1))
     buf = ctypes.c_void_p()
     n = ctypes.c_size_t()
     ans = np.zeros(2, np.uint64)
-    rc = nb.LIB.tp_orion_open_nizk(int(arr.shape[0]), data.tobytes(), int(seed) & (2 ** 64 - 1),
+    rc = nb.LIB.tp_orion_open_nizk(copies, data, int(seed) & (2 ** 64 - 1),
                                    1 if use_group1 else 0, rr.ctypes.data_as(_U64P), rr.size // 2,
                                    ctypes.byref(buf), ctypes.byref(n), ans.ctypes.data_as(_U64P))
     if rc != 0:
@@ -102,16 +111,13 @@ def opening_nizk(inputs, seed: int, r, use_group1: bool = True):
     return proof, (int(ans[0]), int(ans[1]))
 
 
-def opening_nizk_virgo(inputs, seed: int, r, use_group1: bool = True):
+def opening_nizk_virgo(inputs, seed: int, r, use_group1: bool = True, layout=GRAY_LAYOUT):
     """``opening_nizk`` and the words of its Virgo code-switching transcript
     (layout in include/truepix_hip.h, tp_orion_open_nizk_virgo).  Returns
     (proof or None, answer, words): proof is None when the code-switching
     proof does not check, as the reference then writes no proof."""
     nb.require()
-    arr = np.ascontiguousarray(inputs)
-    if arr.ndim != 2 or arr.shape[1] != COPY_SIZE:
-        raise ValueError("inputs must be [copies][%d]" % COPY_SIZE)
-    data = np.ascontiguousarray(arr, dtype=np.uint8)
+    copies, data = _samples(inputs, layout)
     rr = np.ascontiguousarray(np.asarray(r, dtype=np.uint64).reshape(-1))
     cap = 1 << 15
     while True:
@@ -121,7 +127,7 @@ def opening_nizk_virgo(inputs, seed: int, r, use_group1: bool = True):
         words = np.zeros(cap, np.uint64)
         nw = ctypes.c_longlong()
         rc = nb.LIB.tp_orion_open_nizk_virgo(
-            int(arr.shape[0]), data.tobytes(), int(seed) & (2 ** 64 - 1), 1 if use_group1 else 0,
+            copies, data, int(seed) & (2 ** 64 - 1), 1 if use_group1 else 0,
             rr.ctypes.data_as(_U64P), rr.size // 2, ctypes.byref(buf), ctypes.byref(n),
             ans.ctypes.data_as(_U64P), words.ctypes.data_as(_U64P), cap, ctypes.byref(nw))
         if buf.value:
@@ -198,23 +204,25 @@ def parse_opening(proof: bytes) -> dict:
             "com_siblings": com_sib, "combined_siblings": comb_sib}
 
 
-def commit_yuv(y, cb, cr, seed: int, timing: bool = False):
+def commit_yuv(y, cb, cr, seed: int, timing: bool = False, extra=None, layout=GRAY_LAYOUT):
     """``commit`` of a frame given as YUV 4:2:0 planes, its per-copy rows
-    built on the device (``yuv.macroblocks``)."""
+    built on the device (``yuv.macroblocks``; ``extra`` appends the crop
+    circuit's byte per copy, with its 385-sample packing)."""
     from truepix_amd import yuv
-    return commit(yuv.macroblocks(y, cb, cr), seed, timing)
+    return commit(yuv.macroblocks(y, cb, cr, extra), seed, timing, layout)
 
 
-def manifest(inputs, seed: int, zk: bool = True) -> dict:
+def manifest(inputs, seed: int, zk: bool = True, layout=GRAY_LAYOUT) -> dict:
     """The orion_commit.json the reference's commit phase writes for these
-    inputs and mask seed (truepix_gkr.h:428-455)."""
+    inputs, mask seed and packing (truepix_gkr.h:428-455)."""
     copies = int(np.asarray(inputs).shape[0])
-    masked_root, mask_root = commit(inputs, seed)
+    masked_root, mask_root = commit(inputs, seed, layout=layout)
     padded = 1 << max(0, (copies - 1).bit_length())
     n = padded * 512
+    copy_size, const_idx, const_val = (int(x) for x in layout)
     return {"protocol": PROTOCOL_ID, "layout": LAYOUT_ID, "zk": zk, "copies": copies,
-            "padded_copies": padded, "copy_size": COPY_SIZE, "padded_copy_size": 512,
-            "const_idx": 384, "const_val": 128, "n": n, "lg_n": n.bit_length() - 1,
+            "padded_copies": padded, "copy_size": copy_size, "padded_copy_size": 512,
+            "const_idx": const_idx, "const_val": const_val, "n": n, "lg_n": n.bit_length() - 1,
             "masked_root": masked_root, "mask_root": mask_root if zk else ""}
 
 
