@@ -233,3 +233,20 @@ def test_schedule_tuner_settles_and_keeps_proofs(circs):
         assert pool.tuning()["claim_items"] == 1 << 18
         with pytest.raises(RuntimeError):
             pool.fix_schedule(1 << 20, 1024)
+
+
+def test_sixteen_stream_workers_grow_buffers_while_others_answer_resident_rounds(circs):
+    """Sixteen stream workers take frames of rising size, so workers grow
+    their buffers (frees that synchronize the device) while other workers'
+    resident late-round kernels wait on their host threads for challenges.
+    Every frame completes and equals the same frame proved alone."""
+    copies = [16] * 16 + [300] * 16 + [1350] * 24 + [3600] * 24 + [16, 3600] * 8
+    specs = synth_specs(circs, copies, seed=31)
+    with frames.FramePool(1) as solo:
+        want = [solo.result(solo.submit(*sp)).proof for sp in specs]
+    print("solo done", flush=True)
+    with frames.FramePool(16) as pool:
+        jobs = [pool.submit(*sp) for sp in specs]
+        got = [pool.result(j).proof for j in jobs]
+    for k, (a, b) in enumerate(zip(want, got)):
+        assert a == b, (k, copies[k])

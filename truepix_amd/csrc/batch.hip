@@ -552,7 +552,7 @@ struct BatchCtx {
     bool pinned(uint8_t*& p, size_t& have, size_t need)
     {
         if (need <= have && p) return true;
-        if (p) (void)hipHostFree(p);
+        free_host(p);
         p = nullptr;
         have = 0;
         TP_HIP_OK(hipHostMalloc((void**)&p, need ? need : 64));
@@ -562,7 +562,7 @@ struct BatchCtx {
     bool mapped(int nf)
     {
         if (nf <= map_frames && h_map) return true;
-        if (h_map) (void)hipHostFree(h_map);
+        free_host(h_map);
         h_map = d_map = nullptr;
         map_frames = 0;
         const size_t bytes = ((size_t)kSlots + kVals) * nf * sizeof(fp2) + (size_t)nf * 8;
@@ -681,10 +681,10 @@ void batch_ctx_destroy(BatchCtx* B)
     if (!B) return;
     if (B->st) (void)hipStreamSynchronize(B->st);
     B->arena.release();
-    if (B->h_in) (void)hipHostFree(B->h_in);
-    if (B->h_out8) (void)hipHostFree(B->h_out8);
-    if (B->h_stage) (void)hipHostFree(B->h_stage);
-    if (B->h_map) (void)hipHostFree(B->h_map);
+    free_host(B->h_in);
+    free_host(B->h_out8);
+    free_host(B->h_stage);
+    free_host(B->h_map);
     for (hipEvent_t e : B->ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : B->tev) (void)hipEventDestroy(e);
     if (B->st) (void)hipStreamDestroy(B->st);

@@ -263,8 +263,10 @@ double now_s(Clock::time_point t0)
 // Waits for the n Fp2 values a resident kernel publishes as tagged words
 // under sequence number seq (relay.hpp) in the slot's value words, copies
 // them to out and empties the words.  Fails if the kernel gave up waiting
-// (kSeqAbort), the stream reports an error or drains without publishing.
-bool wait_tagged(hipStream_t st, MappedSlot& slot, unsigned seq, int n, fp2* out, const char* what)
+// (kSeqAbort) or after 30 s.  It makes no runtime call: the kernel is waiting
+// for this thread, and a runtime call can queue behind another thread's
+// device-wide synchronization, which waits for the kernel.
+bool wait_tagged(MappedSlot& slot, unsigned seq, int n, fp2* out, const char* what)
 {
     volatile u64* w = reinterpret_cast<volatile u64*>(slot.vals);
     const u64 tag = (u64)(seq & 7) << 61;
@@ -279,10 +281,6 @@ bool wait_tagged(hipStream_t st, MappedSlot& slot, unsigned seq, int n, fp2* out
     for (long spins = 1; !ready(); ++spins) {
         if (*slot.seq == kSeqAbort) return fail("%s: the device gave up waiting for a challenge", what);
         if (spins % 4096 == 0) {
-            const hipError_t q = hipStreamQuery(st);
-            if (q != hipSuccess && q != hipErrorNotReady)
-                return fail("%s: stream error %s", what, hipGetErrorString(q));
-            if (q == hipSuccess && !ready()) return fail("%s: result not published", what);
             if (Clock::now() - t0 > std::chrono::seconds(30))
                 return fail("%s: timed out waiting for the device", what);
         } else {
@@ -802,7 +800,7 @@ struct Worker {
     bool host_in(size_t n)
     {
         if (n <= h_in_bytes) return true;
-        if (h_in) (void)hipHostFree(h_in);
+        free_host(h_in);
         h_in = nullptr;
         TP_HIP_OK(hipHostMalloc((void**)&h_in, n));
         h_in_bytes = n;
@@ -811,7 +809,7 @@ struct Worker {
     bool host_out(size_t n)
     {
         if (n <= h_out_bytes) return true;
-        if (h_out) (void)hipHostFree(h_out);
+        free_host(h_out);
         h_out = nullptr;
         TP_HIP_OK(hipHostMalloc((void**)&h_out, n));
         h_out_bytes = n;
@@ -823,8 +821,8 @@ struct Worker {
                          &br[1], &bi[0], &bi[1], &z1cr, &z1ci, &z1lr, &z1li, &othr, &othi,
                          &partials, &small, &tzr, &tzi};
         for (DevBuf* b : all) b->release();
-        if (h_in) (void)hipHostFree(h_in);
-        if (h_out) (void)hipHostFree(h_out);
+        free_host(h_in);
+        free_host(h_out);
         h_in = nullptr;
         h_out = nullptr;
         slot.release();
@@ -974,6 +972,7 @@ bool prove_frame(Worker& W, Job& J)
         *W.slot.seq = 0;
         const unsigned seq0 = W.slot.next + 1;
         W.slot.next += 2 * nin;
+        ResidentPhase resident;   // no frees from here to the last answer
         k_fe_late_resident<<<1, kOneWg, lds, st>>>(
             C.d_type.as<int>(), C.d_in0.as<int>(), C.d_in1.as<int>(), W.z1lr.as<u64>(),
             W.z1li.as<u64>(), W.othr.as<u64>(), W.othi.as<u64>(), Vr, Vi, nin, C.n_gates, beta2,
@@ -987,7 +986,7 @@ bool prove_frame(Worker& W, Job& J)
         };
         for (int k = 0; k < 2 * nin; ++k) {
             fp2 f[3];
-            if (!wait_tagged(st, W.slot, seq0 + (unsigned)k, 3, f, "late round")) return false;
+            if (!wait_tagged(W.slot, seq0 + (unsigned)k, 3, f, "late round")) return false;
             fp2 r;
             if (!pr.late(k, f, r)) {
                 // answer the kernel's remaining rounds with zeros so that it
@@ -995,7 +994,7 @@ bool prove_frame(Worker& W, Job& J)
                 const std::string err = tp_last_error();
                 answer(k, zero2());
                 for (int j = k + 1; j < 2 * nin; ++j) {
-                    if (!wait_tagged(st, W.slot, seq0 + (unsigned)j, 3, f, "late round")) break;
+                    if (!wait_tagged(W.slot, seq0 + (unsigned)j, 3, f, "late round")) break;
                     answer(j, zero2());
                 }
                 (void)hipStreamSynchronize(st);

@@ -1,6 +1,7 @@
 // runtime.hip - device bring-up, error state and buffers for libtruepix_hip.so.
 #include "runtime.hpp"
 
+#include <condition_variable>
 #include <mutex>
 #include <stdarg.h>
 #include <stdio.h>
@@ -76,13 +77,61 @@ bool device_ready()
 
 hipStream_t stream() { return g_stream; }
 
+// The gate between resident phases and frees: writer-preferring, so a free
+// is not starved by phases that keep overlapping.
+static std::mutex g_gate_mu;
+static std::condition_variable g_gate_cv;
+static int g_resident = 0;        // phases in flight
+static int g_free_waiting = 0;    // frees waiting for them to end
+static bool g_freeing = false;
+
+void resident_enter()
+{
+    std::unique_lock<std::mutex> lk(g_gate_mu);
+    g_gate_cv.wait(lk, [] { return !g_freeing && g_free_waiting == 0; });
+    ++g_resident;
+}
+
+void resident_leave()
+{
+    std::lock_guard<std::mutex> lk(g_gate_mu);
+    if (--g_resident == 0) g_gate_cv.notify_all();
+}
+
+template <typename F> static void outside_resident(F&& f)
+{
+    {
+        std::unique_lock<std::mutex> lk(g_gate_mu);
+        ++g_free_waiting;
+        g_gate_cv.wait(lk, [] { return g_resident == 0 && !g_freeing; });
+        --g_free_waiting;
+        g_freeing = true;
+    }
+    f();
+    {
+        std::lock_guard<std::mutex> lk(g_gate_mu);
+        g_freeing = false;
+    }
+    g_gate_cv.notify_all();
+}
+
+void free_device(void* p)
+{
+    if (p) outside_resident([p] { (void)hipFree(p); });
+}
+
+void free_host(void* p)
+{
+    if (p) outside_resident([p] { (void)hipHostFree(p); });
+}
+
 bool DevBuf::ensure(size_t need)
 {
     if (need <= bytes && ptr) return true;
     void* fresh = nullptr;
     if (hipMalloc(&fresh, need ? need : 16) != hipSuccess)
         return fail("hipMalloc(%zu) failed", need);
-    if (ptr) (void)hipFree(ptr);
+    free_device(ptr);
     ptr = fresh;
     bytes = need;
     return true;
@@ -90,7 +139,7 @@ bool DevBuf::ensure(size_t need)
 
 void DevBuf::release()
 {
-    if (ptr) (void)hipFree(ptr);
+    free_device(ptr);
     ptr = nullptr;
     bytes = 0;
 }
@@ -102,7 +151,7 @@ bool MappedSlot::ready()
     TP_HIP_OK(hipHostMalloc(&hv, 2048, hipHostMallocMapped | hipHostMallocCoherent));
     void* dv = nullptr;
     if (hipHostGetDevicePointer(&dv, hv, 0) != hipSuccess) {
-        (void)hipHostFree(hv);
+        free_host(hv);
         return fail("hipHostGetDevicePointer failed");
     }
     vals = hv;
@@ -136,7 +185,7 @@ bool MappedSlot::wait(hipStream_t st, unsigned s, const char* what)
 
 void MappedSlot::release()
 {
-    if (vals) (void)hipHostFree(vals);
+    free_host(vals);
     vals = dvals = nullptr;
     seq = nullptr;
     dseq = nullptr;

@@ -35,6 +35,27 @@ void* scratch_ptr();
 // it and synchronizes before returning, as the reference ABI is synchronous.
 hipStream_t stream();
 
+// Resident phases and device-wide frees.  A resident kernel (a workgroup that
+// takes each round's challenge from mapped host memory) ends only while its
+// host thread keeps answering.  hipFree and hipHostFree synchronize the whole
+// device, so one that starts while such a kernel runs waits for it; if the
+// answering thread then queues behind that free inside the runtime, neither
+// moves until the kernel's wait bound expires.  So a thread answering a
+// resident kernel makes no runtime calls between the launch and the last
+// answer, it holds a ResidentPhase over that span, and the library's frees
+// (free_device, free_host) wait until no phase is in flight; new phases wait
+// while a free is pending.  A thread must not free while it holds a phase.
+void resident_enter();
+void resident_leave();
+struct ResidentPhase {
+    ResidentPhase() { resident_enter(); }
+    ~ResidentPhase() { resident_leave(); }
+    ResidentPhase(const ResidentPhase&) = delete;
+    ResidentPhase& operator=(const ResidentPhase&) = delete;
+};
+void free_device(void* p);   // hipFree outside every resident phase
+void free_host(void* p);     // hipHostFree outside every resident phase
+
 // Grow-only device allocation (malloc on grow, never shrinks).
 struct DevBuf {
     void* ptr = nullptr;

@@ -1181,6 +1181,9 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
     hipStream_t st = stream_handle ? (hipStream_t)stream_handle : tp::stream();
     const size_t n = (size_t)1 << nv;
     if (!sc2_setup(n)) return -1;
+    // its mid and tail launches wait on this thread's answers: no frees until
+    // the prove returns
+    tp::ResidentPhase resident;
     Sc2State& S = g_sc2;
     for (int c = 0; c < SC_NCLASS; ++c) S.ms[c] = S.bytes[c] = 0, S.launches[c] = 0;
     if (!S.slots_clean) {
