@@ -2,8 +2,10 @@
 // (built with plain g++, so the integer code the GPU runs is exercised as
 // written): mul31 against 128-bit products, including the worst-case
 // operands that set its no-overflow bounds, LazyProdSum against an exact Fp2
-// dot product, and the sumcheck fold against a + r (b - a).  Prints each
-// failure; exit status 1 on any.
+// dot product, the sumcheck fold against a + r (b - a), the device's limb
+// multiply against 128-bit products, and the two-round quad fold against
+// two single folds and the exact polynomial.  Prints each failure; exit
+// status 1 on any.
 #include "fp61.hpp"
 #include <cstdio>
 #include <vector>
@@ -117,6 +119,58 @@ int main() {
                 }
         for (int i = 0; i < 200000; ++i)
             check_fold(mk2(rnd() % P, rnd() % P), mk2(rnd() % P, rnd() % P), mk2(rnd() % P, rnd() % P));
+    }
+    // the device multiply from 32-bit limbs: congruent, at most 2^61 + 6, for
+    // operands below 2^62 (extremes included)
+    {
+        auto check_limbs = [&](u64 a, u64 b) {
+            const u64 got = fmul_lazy_limbs(a, b);
+            CHECK(got <= two61 + 6, "fmul_lazy_limbs bound: a=%llx b=%llx", (unsigned long long)a,
+                  (unsigned long long)b);
+            CHECK(modp(got) == modp((u128)a * b), "fmul_lazy_limbs value: a=%llx b=%llx",
+                  (unsigned long long)a, (unsigned long long)b);
+        };
+        for (u64 a : ops)
+            for (u64 b : ops) check_limbs(a, b);
+        for (int i = 0; i < 200000; ++i) check_limbs(rnd() >> 2, rnd() >> 2);
+    }
+    // two rounds of the fold on a quad: equal to fold2(fold2(p0, p1, r0),
+    // fold2(p2, p3, r0), r1) and to the exact p0 + r0 d1 + r1 d2 + r0 r1 d3,
+    // with the host multiply and with the device's limb multiply
+    {
+        struct LimbMul {
+            u64 operator()(u64 a, u64 b) const { return fmul_lazy_limbs(a, b); }
+        };
+        auto exact = [](fp2 x, fp2 y) {   // x y mod p, i^2 = -1
+            return mk2(modp((u128)x.re * y.re + (u128)(P - x.im) * y.im),
+                       modp((u128)x.re * y.im + (u128)x.im * y.re));
+        };
+        auto check_quad = [&](fp2 p0, fp2 p1, fp2 p2, fp2 p3, fp2 r0, fp2 r1) {
+            const fp2 r01 = mul2(r0, r1);
+            const fp2 want = fold2(fold2(p0, p1, r0), fold2(p2, p3, r0), r1);
+            const fp2 d1 = sub2(p1, p0), d2 = sub2(p2, p0), d3 = sub2(sub2(p3, p2), d1);
+            const fp2 poly = add2(add2(p0, exact(r0, d1)), add2(exact(r1, d2), exact(r01, d3)));
+            const fp2 a = fold_quad(p0, p1, p2, p3, r0, r1, r01);
+            const fp2 b = fold_quad_with(p0, p1, p2, p3, r0, r1, r01, LimbMul());
+            CHECK(eq2(want, poly), "two single folds differ from the quad polynomial");
+            CHECK(eq2(a, want) && eq2(b, want),
+                  "fold_quad differs: p0=(%llx,%llx) r0=(%llx,%llx) r1=(%llx,%llx)",
+                  (unsigned long long)p0.re, (unsigned long long)p0.im, (unsigned long long)r0.re,
+                  (unsigned long long)r0.im, (unsigned long long)r1.re, (unsigned long long)r1.im);
+        };
+        const u64 ext[] = {0, 1, P - 1, P - 2, 0x7fffffffull, 0xffffffffull, 1ull << 60,
+                           0x1fffffff80000000ull, 0x1fffffffffffffffull - 1};
+        for (u64 x : ext)
+            for (u64 y : ext)
+                for (u64 w : ext) {
+                    check_quad(mk2(x, y), mk2(y, w), mk2(w, x), mk2(P - 1, 0), mk2(y, x), mk2(w, w));
+                    check_quad(mk2(0, 0), mk2(x, x), mk2(P - 1, P - 1), mk2(y, w), mk2(P - 1, P - 1),
+                               mk2(x, P - 1));
+                }
+        for (int i = 0; i < 100000; ++i) {
+            auto r = [] { return mk2(rnd() % P, rnd() % P); };
+            check_quad(r(), r(), r(), r(), r(), r());
+        }
     }
     std::printf("field check: %d failures\n", fails);
     return fails ? 1 : 0;

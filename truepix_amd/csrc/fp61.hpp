@@ -30,13 +30,11 @@ TP_HD u64 fsub(u64 a, u64 b) { return canon(a + P - b); }
 TP_HD u64 fneg(u64 a) { return a ? P - a : 0; }
 TP_HD u64 fred64(u64 x) { return canon((x & P) + (x >> 61)); }  // any u64
 
-// a * b mod p as a value in [0, 2^61 + 7] (congruent, not yet canonical);
+// a * b mod p from 32-bit limbs, as the device computes it (no 64 x 64
+// multiplier): a value in [0, 2^61 + 6], congruent, not yet canonical;
 // a, b < 2^62 (so a canonical value plus another canonical value is allowed).
-// Device code has no 64 x 64 multiplier: it splits the operands into 32-bit
-// limbs (below); host code uses the 128-bit product.
-TP_HD u64 fmul_lazy(u64 a, u64 b)
+TP_HD u64 fmul_lazy_limbs(u64 a, u64 b)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
     const u32 a0 = (u32)a, a1 = (u32)(a >> 32);   // a1 < 2^30
     const u32 b0 = (u32)b, b1 = (u32)(b >> 32);
     const u64 lo = (u64)a0 * b0;                  // < 2^64
@@ -46,15 +44,28 @@ TP_HD u64 fmul_lazy(u64 a, u64 b)
     //     = 8 hi + (mid >> 29) + ((mid mod 2^29) << 32) + (lo >> 61) + (lo mod 2^61)
     const u64 s = (lo & P) + (lo >> 61) + ((mid & ((1ULL << 29) - 1)) << 32)
                 + (mid >> 29) + (hi << 3);        // < 2^64
+    return (s & P) + (s >> 61);
+}
+
+// a * b mod p as a value in [0, 2^61 + 6] (congruent, not yet canonical);
+// a, b < 2^62.  Device code has no 64 x 64 multiplier and uses the limb form
+// above; host code uses the 128-bit product.
+TP_HD u64 fmul_lazy(u64 a, u64 b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return fmul_lazy_limbs(a, b);
 #else
     const unsigned __int128 t = (unsigned __int128)a * b;   // < 2^124
     const u64 s = ((u64)t & P) + (u64)(t >> 61);            // < 2^63 + 2^61
-#endif
     return (s & P) + (s >> 61);
+#endif
 }
 
 // Any u64 to its canonical residue (x mod 2^61 folded once, then < 2p).
 TP_HD u64 fold61(u64 x) { return canon((x & P) + (x >> 61)); }
+
+// One fold of any u64: at most 2^61 + 6.
+TP_HD u64 fold1(u64 x) { return (x & P) + (x >> 61); }
 
 // a, b < 2^62; result canonical.
 TP_HD u64 fmul(u64 a, u64 b) { return canon(fmul_lazy(a, b)); }
@@ -95,6 +106,34 @@ TP_HD fp2 fold2(fp2 a, fp2 b, fp2 r)
     return mk2(fold61(a.re + ac + (2 * P - bd)), fold61(a.im + k + (4 * P - ac - bd)));
 }
 
+// Two rounds of the fold on one quad: fold2(fold2(p0, p1, r0),
+// fold2(p2, p3, r0), r1) = p0 + r0 d1 + r1 d2 + r01 d3 with d1 = p1 - p0,
+// d2 = p2 - p0, d3 = p3 - p2 - p1 + p0 and r01 = r0 r1.  Three Karatsuba
+// products (MUL: a lazy multiply, each result at most 2^61 + 6) summed
+// unreduced and folded once, so one reduction per coordinate instead of
+// three: with A = fold1(sum ac), B = fold1(sum bd) (each at most 2^61 + 2),
+// re = p0 + A + (2p - B) < 2^63 + 3 and im = p0 + fold1(sum k) + (4p - A - B)
+// < 2^64.
+template <typename Mul>
+TP_HD fp2 fold_quad_with(fp2 p0, fp2 p1, fp2 p2, fp2 p3, fp2 r0, fp2 r1, fp2 r01, Mul mul)
+{
+    const fp2 d1 = sub2(p1, p0), d2 = sub2(p2, p0), d3 = sub2(sub2(p3, p2), d1);
+    const u64 ac = fold1(mul(r0.re, d1.re) + mul(r1.re, d2.re) + mul(r01.re, d3.re));
+    const u64 bd = fold1(mul(r0.im, d1.im) + mul(r1.im, d2.im) + mul(r01.im, d3.im));
+    const u64 k = fold1(mul(r0.re + r0.im, d1.re + d1.im) + mul(r1.re + r1.im, d2.re + d2.im) +
+                        mul(r01.re + r01.im, d3.re + d3.im));
+    return mk2(fold61(p0.re + ac + (2 * P - bd)), fold61(p0.im + k + (4 * P - ac - bd)));
+}
+
+struct LazyMul {
+    TP_HD u64 operator()(u64 a, u64 b) const { return fmul_lazy(a, b); }
+};
+
+TP_HD fp2 fold_quad(fp2 p0, fp2 p1, fp2 p2, fp2 p3, fp2 r0, fp2 r1, fp2 r01)
+{
+    return fold_quad_with(p0, p1, p2, p3, r0, r1, r01, LazyMul());
+}
+
 // Operand of mul31: a < 2^62 as a = h 2^31 + l with l, h < 2^31.
 struct Sp31 { u32 l, h; };
 TP_HD Sp31 sp31(u64 a)
@@ -123,8 +162,6 @@ TP_HD u64 mul31(Sp31 a, Sp31 b, u64 c)
     return (u64)a.l * b.l + y + c;
 }
 
-// One fold of any u64: at most 2^61 + 6.
-TP_HD u64 fold1(u64 x) { return (x & P) + (x >> 61); }
 
 // Sum of Fp2 products kept lazily: the Karatsuba products ac, bd and
 // k = (a + b)(c + d) summed separately, each product added unreduced (mul31)

@@ -68,6 +68,14 @@ __device__ __forceinline__ fp2 ldv(const V2* p) { const V2 v = *p; return mk2(v.
 __device__ __forceinline__ void stv(V2* p, fp2 x) { V2 v; v.re = x.re; v.im = x.im; *p = v; }
 
 
+// A value every lane holds alike, moved to scalar registers.
+__device__ __forceinline__ u64 uniform64(u64 x)
+{
+    return ((u64)(u32)__builtin_amdgcn_readfirstlane((int)(x >> 32)) << 32) |
+           (u32)__builtin_amdgcn_readfirstlane((int)(u32)x);
+}
+__device__ __forceinline__ fp2 uniform2(fp2 x) { return mk2(uniform64(x.re), uniform64(x.im)); }
+
 // Arguments of a launch that publishes its own round sums (ticket_publish).
 struct TicketPub {
     u64* slots;          // [gridDim.x][N][2] tagged words; all ones between launches
@@ -138,61 +146,51 @@ __device__ __forceinline__ bool launch_challenges(const TicketPub& a, u64 (&w)[4
     }
     __syncthreads();
     if (!s_ok) return false;
-    for (int i = 0; i < a.n_ans; ++i) w[i] = s_w[i];
+    // uniform values: scalar registers for the launch's products
+    for (int i = 0; i < a.n_ans; ++i) w[i] = uniform64(s_w[i]);
     return true;
 }
 
-// End of a streaming launch, in place of a separate publish launch (which
-// cost its own launch and a dependent-launch gap per round).  Each
-// workgroup's N sums go to its slot as tagged words, value | (seq mod 8) << 61
-// (values are canonical, below 2^61), and thread 0 takes a ticket.  The
-// workgroup taking the last ticket adds every slot once each word carries
-// this launch's tag (spinning, bounded by the wall clock, on any still in
-// flight), clears the slots to all ones and the ticket for the next launch
-// (so no word of an earlier round, whatever its tag, is ever read as this
-// one's), and publishes the totals as tagged words (publish_tagged).
+// Slots and tickets of the launches that publish their own sums: the last
+// workgroup of a launch gathers every workgroup's slot, so a launch runs at
+// most kMaxPubGroups workgroups.
+constexpr unsigned kMaxPubGroups = 1024;
+constexpr size_t kTicketBytes = 256;
+
+// Adds `count` slots of N tagged sums at `base` (slot j at base + 2 N j) in
+// this workgroup once every word carries `tag` (spinning, bounded by the
+// wall clock, on any still in flight), then clears them to all ones, so no
+// word of an earlier round, whatever its tag, is ever read as a later one's.
+// The totals land in s_tot; the return value (every thread) is 0 if a wait
+// expired.
 template <int N>
-__device__ __forceinline__ void ticket_publish(fp2 (&v)[N], const TicketPub& a)
+__device__ __forceinline__ int gather_slots(u64* base, unsigned count, u64 tag, long long spin_ticks,
+                                            fp2 (&s_tot)[N])
 {
-    __shared__ int s_last;
     __shared__ fp2 s_g[N][1024 / N];
-    __shared__ fp2 s_tot[N];
-    block_sum<N>(v);
-    const unsigned nb = gridDim.x, t = threadIdx.x;
-    const u64 tag = (u64)(a.seq & 7) << 61;
-    if (t == 0) {
-        u64* p = a.slots + (size_t)blockIdx.x * 2 * N;
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-            __hip_atomic_store(p + 2 * k, v[k].re | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(p + 2 * k + 1, v[k].im | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        s_last = __hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
-                 nb - 1;
-    }
-    __syncthreads();
-    if (!s_last) return;
-    // the last workgroup: thread t adds sum k = t mod N over slots g, g + G, ...
+    const unsigned t = threadIdx.x;
+    // thread t adds sum k = t mod N over slots g, g + G, ...
     const unsigned G = blockDim.x / N, k = t % N, g = t / N;
     const long long t0 = wall_clock64();
     int ok = 1;
     fp2 acc = zero2();
     auto word = [&](unsigned j, int c) -> u64 {   // slot j, coordinate c of sum k
-        return j < nb ? __hip_atomic_load(a.slots + ((size_t)j * N + k) * 2 + c, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT)
-                      : tag;                       // past the last slot: 0 under this tag
+        return j < count ? __hip_atomic_load(base + ((size_t)j * N + k) * 2 + c, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT)
+                         : tag;                    // past the last slot: 0 under this tag
     };
     if (g < G) {
-        // eight slots' words in flight at once: one round of loads covers
-        // every launch here (nb <= 8 G for all of them)
-        for (unsigned j0 = g; j0 < nb; j0 += 8 * G) {
-            u64 w[16];
+        // four slots' words in flight at once (eight held the launches'
+        // registers above their loops' needs): one or two rounds of loads
+        // cover every launch here (count <= 8 G)
+        for (unsigned j0 = g; j0 < count; j0 += 4 * G) {
+            u64 w[8];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) w[i] = word(j0 + (i >> 1) * G, i & 1);
+            for (int i = 0; i < 8; ++i) w[i] = word(j0 + (i >> 1) * G, i & 1);
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
+            for (int i = 0; i < 8; ++i) {
                 while (ok && ((w[i] & ~P) != tag || (w[i] & P) == P)) {
-                    if (wall_clock64() - t0 > a.spin_ticks) {
+                    if (wall_clock64() - t0 > spin_ticks) {
                         ok = 0;
                         break;
                     }
@@ -201,17 +199,12 @@ __device__ __forceinline__ void ticket_publish(fp2 (&v)[N], const TicketPub& a)
                 }
             }
 #pragma unroll
-            for (int i = 0; i < 8; ++i) acc = add2(acc, mk2(w[2 * i] & P, w[2 * i + 1] & P));
+            for (int i = 0; i < 4; ++i) acc = add2(acc, mk2(w[2 * i] & P, w[2 * i + 1] & P));
         }
     }
     ok = __syncthreads_and(ok);   // every word is read before the slots are cleared
-    for (unsigned w = t; w < nb * 2 * N; w += blockDim.x)
-        __hip_atomic_store(a.slots + w, kSlotEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    // every workgroup took its ticket after its challenges: the relay is idle
-    // again (the first pass of a prove, which waits for none, clears what an
-    // earlier prove may have left)
-    if (a.relay && t < 4)
-        __hip_atomic_store(a.relay + t, kSlotEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (unsigned w = t; w < count * 2 * N; w += blockDim.x)
+        __hip_atomic_store(base + w, kSlotEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (g < G) s_g[k][g] = acc;
     __syncthreads();
     const unsigned wid = t >> 6, lane = t & 63, nwv = blockDim.x >> 6;
@@ -223,6 +216,51 @@ __device__ __forceinline__ void ticket_publish(fp2 (&v)[N], const TicketPub& a)
         if (lane == 0) s_tot[q] = sv[0];
     }
     __syncthreads();
+    return ok;
+}
+
+// Thread 0 stores this workgroup's N sums to slot `slot` as tagged words,
+// value | (seq mod 8) << 61 (values are canonical, below 2^61), and takes a
+// ticket from *ticket; true (every thread) in the workgroup that took the
+// last of `of` tickets.
+template <int N>
+__device__ __forceinline__ bool slot_and_ticket(const fp2 (&v)[N], u64* slot, unsigned* ticket,
+                                                unsigned of, u64 tag)
+{
+    __shared__ int s_last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            __hip_atomic_store(slot + 2 * k, v[k].re | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(slot + 2 * k + 1, v[k].im | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == of - 1;
+    }
+    __syncthreads();
+    return s_last != 0;
+}
+
+// End of a streaming launch, in place of a separate publish launch (which
+// cost its own launch and a dependent-launch gap per round).  Each
+// workgroup's N sums go to its slot and it takes a ticket (slot_and_ticket);
+// the workgroup taking the last ticket adds the slots (gather_slots), resets
+// the ticket for the next launch and publishes the totals as tagged words
+// (publish_tagged).
+template <int N>
+__device__ __forceinline__ void ticket_publish(fp2 (&v)[N], const TicketPub& a)
+{
+    __shared__ fp2 s_tot[N];
+    block_sum<N>(v);
+    const unsigned nb = gridDim.x;
+    const u64 tag = (u64)(a.seq & 7) << 61;
+    if (!slot_and_ticket<N>(v, a.slots + (size_t)blockIdx.x * 2 * N, a.ticket, nb, tag)) return;
+    const int ok = gather_slots<N>(a.slots, nb, tag, a.spin_ticks, s_tot);
+    // every workgroup took its ticket after its challenges: the relay is idle
+    // again (the first pass of a prove, which waits for none, clears what an
+    // earlier prove may have left)
+    const unsigned t = threadIdx.x;
+    if (a.relay && t < 4)
+        __hip_atomic_store(a.relay + t, kSlotEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (t == 0) {
         __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         fp2 tot[N];
@@ -355,10 +393,9 @@ __global__ void __launch_bounds__(kFirst01Threads) k_sc2_first01(const V2* __res
 // With a lane owning eight inputs instead, one load instruction touched 64
 // lines 128 B apart and the L2 dropped lines before their other loads came:
 // 1.23x the algorithmic reads.
-__device__ __forceinline__ fp2 fold_quad01(const V2* p, fp2 r0, fp2 r1)
+__device__ __forceinline__ fp2 fold_quad01(const V2* p, fp2 r0, fp2 r1, fp2 r01)
 {
-    const fp2 a0 = fold2(ldv(p), ldv(p + 1), r0), a1 = fold2(ldv(p + 2), ldv(p + 3), r0);
-    return fold2(a0, a1, r1);
+    return fold_quad(ldv(p), ldv(p + 1), ldv(p + 2), ldv(p + 3), r0, r1, r01);
 }
 
 __device__ __forceinline__ fp2 shfl_xor1(fp2 x)
@@ -366,13 +403,20 @@ __device__ __forceinline__ fp2 shfl_xor1(fp2 x)
     return mk2(__shfl_xor(x.re, 1, 64), __shfl_xor(x.im, 1, 64));
 }
 
-// The streaming launches run 512-thread workgroups, at most kFoldGroups of
-// them (as many as are resident at once), so the last workgroup gathers at
-// most that many slots.
+// The streaming launches run 512-thread workgroups, at most kFold01Groups /
+// kFoldGroups of them (as many as are resident at once).  Grids of 2048 to
+// 8192 workgroups for round 2 measured the same or slower: with fewer
+// outputs per thread, each wave's closing reduction costs more per output.
 constexpr int kFoldThreads = 512;
-constexpr unsigned kFold01Groups = 1024;   // 62 VGPRs: 4 workgroups per CU
-constexpr unsigned kFoldGroups = 768;      // 74 VGPRs: 3 workgroups per CU
+constexpr unsigned kFold01Groups = 1024;   // 58 VGPRs: 4 workgroups per CU
+constexpr unsigned kFoldGroups = 768;      // 76 VGPRs: 3 workgroups per CU
 
+// The pair sums of T_2 from two products per lane: the lane holding output j
+// adds f_j g_j (E0 over the even outputs, E1 over the odd) and
+// f_j (g_j - g_j') with j' its pair neighbour, whose sum over both lanes of
+// a pair is f0 g0 + f1 g1 - f0 g1 - f1 g0 = (f1 - f0)(g1 - g0) (EA): every
+// lane busy, where one lane of each pair adding all three left its
+// neighbour idle.
 __global__ void __launch_bounds__(kFoldThreads) k_sc2_fold01_round(const V2* __restrict__ fin,
                                                           const V2* __restrict__ gin,
                                                           size_t n_out, u64 r0r, u64 r0i,
@@ -383,25 +427,29 @@ __global__ void __launch_bounds__(kFoldThreads) k_sc2_fold01_round(const V2* __r
     u64 rw[4] = {r0r, r0i, r1r, r1i};
     if (!launch_challenges(pub, rw)) return;
     const fp2 r0 = mk2(rw[0], rw[1]), r1 = mk2(rw[2], rw[3]);
-    const bool even = (threadIdx.x & 1) == 0;
-    LazyProdSum z[3];
-    lazy_zero(z);
+    const fp2 r01 = uniform2(mul2(r0, r1));
     // n_out is even and the stride a multiple of 64, so both lanes of a pair
     // run every iteration together.  The sweep runs from the top down: the
     // first pass read the tables bottom up, so their top part is still in
-    // the memory-side cache.  Lane order is reversed with it, so the even
-    // lane holds output 2p + 1 and its neighbour 2p.
+    // the memory-side cache.  Lane order is reversed with it, so even lanes
+    // hold odd outputs.
+    const bool odd_out = (threadIdx.x & 1) == 0;
+    LazyProdSum z[2];   // f g, f (g - g')
+    lazy_zero(z);
     for (size_t j0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j0 < n_out;
          j0 += (size_t)gridDim.x * blockDim.x) {
         const size_t j = n_out - 1 - j0;
-        const fp2 f = fold_quad01(fin + 4 * j, r0, r1);
-        const fp2 g = fold_quad01(gin + 4 * j, r0, r1);
+        const fp2 f = fold_quad01(fin + 4 * j, r0, r1, r01);
+        const fp2 g = fold_quad01(gin + 4 * j, r0, r1, r01);
         stv(fout + j, f);
         stv(gout + j, g);
-        const fp2 fo = shfl_xor1(f), go = shfl_xor1(g);
-        if (even) accum_pair(z, fo, f, go, g);
+        const fp2 go = shfl_xor1(g);
+        z[0].mac(f, g);
+        z[1].mac(f, sub2(g, go));
     }
-    lazy_publish<3>(z, pub);
+    const fp2 self = z[0].value();
+    fp2 e[3] = {odd_out ? zero2() : self, odd_out ? self : zero2(), z[1].value()};
+    ticket_publish<3>(e, pub);
 }
 
 // Round k >= 1: fold T_{k-1} (length 4 * n_quads) with r into T_k and sum
@@ -1001,10 +1049,10 @@ struct Sc2State {
 Sc2State g_sc2;
 
 // Tagged words of the streaming launches: 256 first-pass workgroups x 9
-// sums or kFold01Groups x 3, two words each (room for more).
-constexpr size_t kSlotWords = 2048 * 9 * 2;
-static_assert(kFirst01Groups * 9 * 2 <= kSlotWords && kFold01Groups * 3 * 2 <= kSlotWords &&
-                  kFoldGroups * 3 * 2 <= kSlotWords,
+// sums or up to kMaxPubGroups x 3, two words each.
+constexpr size_t kSlotWords = (size_t)kMaxPubGroups * 3 * 2;
+static_assert(kFirst01Groups * 9 * 2 <= kSlotWords && kFold01Groups <= kMaxPubGroups &&
+                  kFoldGroups <= kMaxPubGroups,
               "ticket slots");
 
 bool sc2_setup(size_t n)
@@ -1057,7 +1105,7 @@ bool sc2_setup(size_t n)
     if (!S.mid_parts.ensure(kMidGroups * 6 * sizeof(u64))) return false;
     if (!S.gate.ensure(2 * sizeof(unsigned))) return false;
     if (!S.slots.ensure(kSlotWords * sizeof(u64))) return false;
-    if (!S.ticket.ensure(256)) return false;
+    if (!S.ticket.ensure(kTicketBytes)) return false;
     return true;
 }
 
@@ -1189,7 +1237,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
     if (!S.slots_clean) {
         // first prove, or the last one stopped part way: fresh slots and ticket
         TP_HIP_OK(hipMemsetAsync(S.slots.ptr, 0xff, kSlotWords * sizeof(u64), st));
-        TP_HIP_OK(hipMemsetAsync(S.ticket.ptr, 0, 256, st));
+        TP_HIP_OK(hipMemsetAsync(S.ticket.ptr, 0, kTicketBytes, st));
         TP_HIP_OK(hipMemsetAsync(S.arelay.ptr, 0xff, 4 * sizeof(u64), st));
     }
     S.slots_clean = false;   // until this prove returns 0
