@@ -991,10 +991,11 @@ def main(argv=None):
                              "folds the input by r_0 and r_1 into T_2; fold: streaming "
                              "fold+round launches; mid: 128 workgroups, each folding its own "
                              "slice of the tables in LDS, workgroup 0 gathering their sums for "
-                             "the host, which answers each challenge through mapped memory; "
+                             "the host, which answers the challenges through mapped memory; "
                              "tail: one workgroup proving the last rounds, the host answering "
-                             "each challenge through mapped memory (latency-bound); ceiling: "
-                             "what plain torch kernels reach on this GPU now"},
+                             "through mapped memory (latency-bound); mid and tail take two "
+                             "rounds per host exchange (per_round figures are per exchange); "
+                             "ceiling: what plain torch kernels reach on this GPU now"},
         "check": {"round0_poly": [int(x) for x in proof.polys[0][:2]]},
     }
     def batched():

@@ -153,16 +153,18 @@ void tp_sc2_schedule(int* mid_groups, int* tail_in, int* mid_end);
  * out[3c] summed event ms, out[3c+1] bytes the launches read and write,
  * out[3c+2] launch count.  n >= 15. */
 void tp_sc2_class_stats(double* out, int n);
-/* Per-round latency of the one-workgroup tail of the last timed
- * tp_sc2_prove: out[0] rounds, out[1] device us per round until its sums are
- * ready, out[2] us the workgroup waited for the host's challenge, out[3] host
- * us per round (transcript), out[4 + i] device us of tail round i (while
+/* Per-exchange latency of the one-workgroup tail of the last timed
+ * tp_sc2_prove (an exchange: one host round trip for one or two rounds):
+ * out[0] exchanges, out[1] device us per exchange until its sums are ready,
+ * out[2] us the workgroup waited for the host's challenges, out[3] host us
+ * per exchange (transcript), out[4 + i] device us of exchange i (while
  * 4 + i < n).  n >= 4. */
 void tp_sc2_tail_profile(double* out, int n);
-/* Per-round latency of the sliced mid rounds of the last timed tp_sc2_prove:
- * out[0] rounds, out[1] us from the last workgroup's partials to workgroup 0
- * holding the challenge, out[2] us of the relay to the last workgroup,
- * out[3] us from a challenge to the next partials, out[4] us between the
+/* Per-exchange latency of the sliced mid rounds of the last timed
+ * tp_sc2_prove (an exchange: one host round trip for one or two rounds):
+ * out[0] exchanges, out[1] us from the last workgroup's partials to workgroup
+ * 0 holding the challenges, out[2] us of the relay to the last workgroup,
+ * out[3] us from the challenges to the next partials, out[4] us between the
  * first and the last workgroup's partials.  n >= 5. */
 void tp_sc2_mid_profile(double* out, int n);
 

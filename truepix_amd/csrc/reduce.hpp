@@ -83,6 +83,43 @@ __device__ __forceinline__ void wave_sum(fp2 (&v)[N])
     for (int k = 0; k < N; ++k) v[k] = mk2(w[2 * k], w[2 * k + 1]);
 }
 
+// wave_sum when only lanes 0 .. nlanes - 1 hold nonzero values (nlanes the
+// same in every lane): the DPP steps that would only add zeros are skipped.
+// After the quad steps lane 0 holds the sum of lanes 0-3, after the two
+// mirrors of lanes 0-15, after row_bcast:15 lane 31 holds lanes 0-31, and
+// after row_bcast:31 lane 63 holds all; the total goes to every lane.
+template <int N>
+__device__ __forceinline__ void wave_sum_upto(fp2 (&v)[N], unsigned nlanes)
+{
+    if (nlanes > 32) {
+        wave_sum<N>(v);
+        return;
+    }
+    u64 w[2 * N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        w[2 * k] = v[k].re;
+        w[2 * k + 1] = v[k].im;
+    }
+    if (nlanes > 1) dpp_add_step<0xb1, 0xf>(w);    // quad_perm [1,0,3,2]
+    if (nlanes > 2) dpp_add_step<0x4e, 0xf>(w);    // quad_perm [2,3,0,1]
+    fold_words(w);
+    if (nlanes > 4) dpp_add_step<0x141, 0xf>(w);   // row_half_mirror
+    if (nlanes > 8) dpp_add_step<0x140, 0xf>(w);   // row_mirror
+    fold_words(w);
+    const int src = nlanes > 16 ? 31 : 0;
+    if (nlanes > 16) dpp_add_step<0x142, 0xa>(w);  // row_bcast:15, rows 1 and 3
+#pragma unroll
+    for (int k = 0; k < 2 * N; ++k) {
+        const u64 c = canon(fold1(w[k]));
+        const u32 lo = (u32)__builtin_amdgcn_readlane((int)(u32)c, src);
+        const u32 hi = (u32)__builtin_amdgcn_readlane((int)(u32)(c >> 32), src);
+        w[k] = ((u64)hi << 32) | lo;
+    }
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = mk2(w[2 * k], w[2 * k + 1]);
+}
+
 // Sums v over the workgroup (blockDim.x a multiple of 64, at most 1024); the
 // total lands in thread 0.
 template <int N>

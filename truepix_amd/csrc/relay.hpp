@@ -37,29 +37,41 @@ __device__ __forceinline__ void publish_tagged(fp2* host_vals, const fp2 (&v)[N]
 }
 
 // The host's answer to the round published as `want`, from the mapped answer
-// slot: the host writes both words with (want mod 8) << 61 above the
-// canonical value, so one read of the slot is both the check and the value
-// (a separate sequence word cost a second PCIe round trip).  A word of
-// another round, or the empty mark (all ones: value bits p), does not match.
-// The host empties the slot when a prove starts and each answer overwrites
-// the one before; answered rounds in a prove are never 8 sequence numbers
-// apart with no answer between, so a stale answer never matches.  Returns 0
-// if the wait expires.
-__device__ __forceinline__ int take_answer(const u64* answer, unsigned want, long long t0,
-                                           long long spin_ticks, u64 (&rv)[2])
+// slot: the host writes the N words (N / 2 challenges) with (want mod 8) << 61
+// above the canonical value, so one read of the slot is both the check and
+// the value (a separate sequence word cost a second PCIe round trip).  A word
+// of another round, or the empty mark (all ones: value bits p), does not
+// match.  The host empties the slot when a prove starts and each answer
+// overwrites the one before; answered rounds in a prove are never 8 sequence
+// numbers apart with no answer between, so a stale answer never matches.
+// Returns 0 if the wait expires.
+template <int N>
+__device__ __forceinline__ int take_answer_n(const u64* answer, unsigned want, long long t0,
+                                             long long spin_ticks, u64 (&rv)[N])
 {
     const u64 tag = (u64)(want & 7) << 61;
     for (;;) {
-        const u64 a0 = __hip_atomic_load(answer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        const u64 a1 = __hip_atomic_load(answer + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        if ((a0 & ~P) == tag && (a1 & ~P) == tag && (a0 & P) != P && (a1 & P) != P) {
-            rv[0] = a0 & P;
-            rv[1] = a1 & P;
+        u64 a[N];
+        bool all = true;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            a[i] = __hip_atomic_load(answer + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            all = all && (a[i] & ~P) == tag && (a[i] & P) != P;
+        }
+        if (all) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) rv[i] = a[i] & P;
             return 1;
         }
         if (wall_clock64() - t0 > spin_ticks) return 0;
         __builtin_amdgcn_s_sleep(1);
     }
+}
+
+__device__ __forceinline__ int take_answer(const u64* answer, unsigned want, long long t0,
+                                           long long spin_ticks, u64 (&rv)[2])
+{
+    return take_answer_n<2>(answer, want, t0, spin_ticks, rv);
 }
 
 }  // namespace tp

@@ -38,9 +38,15 @@
 //     workgroups, each folding its own slice in LDS, the sums gathered on the
 //     device and the challenges coming back through mapped memory;
 //   - the rest (k_sc2_tail_host): one workgroup, the same exchange without
-//     the gather, about half the cost of a mid round.
+//     the gather.
 // A launch plus its dependent-launch gap costs ~6 us here, more than a
 // small round's work; the last two phases pay it once for all their rounds.
+// They also take two rounds per host exchange, as the first pass does: the
+// nine sums of T_k's quads give round k's sums and round k+1's as
+// quadratics in r_k, so the host draws r_k and r_{k+1} from one exchange and
+// the next exchange folds by both (fold_quad).  Their tables are small, so
+// the exchange is shaped for latency (exchange_work): each thread folds its
+// own entries, then wave s adds sum s alone.
 // A one-lane SHA-256 on the device was measured at 4.8 us per compression
 // (~26 us per transcript round), against ~3.4 us for the host round trip, so
 // the transcript stays on the host.
@@ -487,30 +493,125 @@ __global__ void __launch_bounds__(kFoldThreads) k_sc2_fold_round(const V2* __res
     lazy_publish<3>(z, pub);
 }
 
-// The last rounds in one workgroup, the host answering each round's
-// challenge through mapped memory.  Entered at round k0 with T_{k0-1}
-// (len_in <= kTailIn entries per table) in global memory and r_{k0-1}: the
-// workgroup folds it into LDS and proves rounds k0..nv-1 there (T_k in place:
-// f at [0, kTailIn/2), g after it), then folds the last pair.  Per round,
-// lane 0 publishes (E0, E1, EA) tagged with seq0 + i like the other
-// launches, then waits for the host to write the challenge, tagged the same,
-// into the answer slot; the final values go out tagged with seq0 + nr.
-// The wait is bounded by the wall clock (spin_ticks): on expiry the kernel
-// publishes kSeqAbort and exits, so it ends whatever the host does.
+// The last rounds in one workgroup, two rounds per exchange with the host.
+// Entered at round k0 with T_{k0-1} (len_in <= kTailIn entries per table) in
+// global memory and r_{k0-1}.  Each exchange applies the challenges not yet
+// applied (r_{k-1}, or r_{k-2} and r_{k-1}) to the table, which gives T_k (in
+// LDS, in place: f at [0, kTailIn/2), g after it), and lane 0 publishes,
+// tagged with seq0 + e like the other launches:
+//   - the nine sums of T_k's quads (accum9), from which the host draws r_k
+//     and r_{k+1} (round k+1's sums are quadratics in r_k, as after the
+//     first pass), or
+//   - for a last single round (an odd count), the three sums of T_k's one
+//     pair;
+// then waits for the host to write the challenges, tagged the same, into the
+// answer slot.  One exchange per two rounds: the host round trip and the
+// wave reductions are paid half as often.  After the last exchange the
+// kernel applies the last challenges and publishes the two final values
+// tagged with seq0 + ne (ne = tail_exchanges(nv - k0)).  Every wait is
+// bounded by the wall clock (spin_ticks): on expiry the kernel publishes
+// kSeqAbort and exits, so it ends whatever the host does.
 constexpr int kTailThreads = 1024;
 constexpr unsigned kTailIn = 8192;
 constexpr size_t kTailLds = (size_t)kTailIn * sizeof(V2);   // both tables, T_{k0}
 
-// Lane 0 publishes a round's sums tagged with sequence number `want` and
-// waits for the host's answer; returns 0 (and publishes kSeqAbort) if the
-// wait expires.
-__device__ __forceinline__ int tail_exchange(const fp2 (&e)[3], unsigned want, fp2* host_vals,
-                                             unsigned* host_seq, u64* answer, long long spin_ticks,
-                                             long long* trace, int i, u64 (&rv)[2])
+// Exchanges for nr rounds: pairs, then a single one if nr is odd.
+__host__ __device__ constexpr int tail_exchanges(int nr) { return (nr + 1) / 2; }
+
+// Entry 0 of the next table from the first 2^np entries at s.
+__device__ __forceinline__ fp2 fold_head(const V2* s, int np, fp2 ra, fp2 rb, fp2 rab)
 {
-    if (trace) trace[3 * i + 1] = wall_clock64();
-    publish_tagged<3>(host_vals, e, want);
-    if (!take_answer(answer, want, wall_clock64(), spin_ticks, rv)) {
+    return np == 1 ? fold2(ldv(s), ldv(s + 1), ra)
+                   : fold_quad(ldv(s), ldv(s + 1), ldv(s + 2), ldv(s + 3), ra, rb, rab);
+}
+
+// Operands of sum s (0..8) of quad q of T_k in LDS (see the file header:
+// S00, S11, SAA, S22, S33, SBB, X0, X1, X2).
+__device__ __forceinline__ fp2 quad_operand(const V2* t, unsigned q, int s)
+{
+    const V2* p = t + 4 * q;
+    switch (s) {
+    case 0: return ldv(p);
+    case 1: return ldv(p + 1);
+    case 2: return sub2(ldv(p + 1), ldv(p));
+    case 3: return ldv(p + 2);
+    case 4: return ldv(p + 3);
+    case 5: return sub2(ldv(p + 3), ldv(p + 2));
+    case 6: return sub2(ldv(p + 2), ldv(p));
+    case 7: return sub2(ldv(p + 3), ldv(p + 1));
+    default: return sub2(sub2(ldv(p + 3), ldv(p + 2)), sub2(ldv(p + 1), ldv(p)));
+    }
+}
+
+// Operands of sum s (0..2: E0, E1, EA) of pair j of T_k in LDS.
+__device__ __forceinline__ fp2 pair_operand(const V2* t, unsigned j, int s)
+{
+    const V2* p = t + 2 * j;
+    return s == 0 ? ldv(p) : s == 1 ? ldv(p + 1) : sub2(ldv(p + 1), ldv(p));
+}
+
+constexpr int kFoldPer = 4;   // entries of T_k a thread folds (kTailIn / 2 / kTailThreads)
+
+// The device work of one exchange on a workgroup's table, shaped for the
+// latency-bound sizes of the last rounds: T_k = T_{k-np} (len entries per
+// table at sf / sg: global memory, or LDS when src_lds) folded by the
+// challenges not yet applied (np = 1: ra; np = 2: ra, rb, rab = ra rb) into
+// tf / tg, each thread folding its own entries; then wave s < NSUM adds sum s
+// over T_k's quads (NSUM = 9: two rounds at once) or pairs (NSUM = 3: one
+// round), a product a lane per quad or pair, and reduces its own two words.
+// The sums land in s_sum; every thread of the workgroup calls it.
+template <int NSUM>
+__device__ __forceinline__ void exchange_work(const V2* sf, const V2* sg, bool src_lds,
+                                              unsigned len, int np, fp2 ra, fp2 rb, fp2 rab,
+                                              V2* tf, V2* tg, fp2* s_sum)
+{
+    const unsigned lk = len >> np;   // entries of T_k, at most kFoldPer * blockDim.x
+    const unsigned t = threadIdx.x, nt = blockDim.x;
+    fp2 of[kFoldPer], og[kFoldPer];
+#pragma unroll
+    for (int i = 0; i < kFoldPer; ++i) {
+        const unsigned j = t + i * nt;
+        if (j < lk) {
+            of[i] = fold_head(sf + ((size_t)j << np), np, ra, rb, rab);
+            og[i] = fold_head(sg + ((size_t)j << np), np, ra, rb, rab);
+        }
+    }
+    if (src_lds) __syncthreads();   // every read of T_{k-np} is done before T_k overwrites it
+#pragma unroll
+    for (int i = 0; i < kFoldPer; ++i) {
+        const unsigned j = t + i * nt;
+        if (j < lk) {
+            stv(tf + j, of[i]);
+            stv(tg + j, og[i]);
+        }
+    }
+    __syncthreads();
+    const int wid = (int)(t >> 6);
+    const unsigned lane = t & 63;
+    if (wid < NSUM) {
+        LazyProdSum z;
+        z.zero();
+        const unsigned nu = NSUM == 9 ? lk / 4 : lk / 2;
+        for (unsigned u = lane; u < nu; u += 64) {
+            if (NSUM == 9)
+                z.mac(quad_operand(tf, u, wid), quad_operand(tg, u, wid));
+            else
+                z.mac(pair_operand(tf, u, wid), pair_operand(tg, u, wid));
+        }
+        fp2 v[1] = {z.value()};
+        wave_sum_upto<1>(v, nu);
+        if (lane == 0) s_sum[wid] = v[0];
+    }
+    __syncthreads();
+}
+
+// Lane 0 waits for the host's NA answer words to the exchange published as
+// `want`; returns 0 (and publishes kSeqAbort) if the wait expires.
+template <int NA>
+__device__ __forceinline__ int tail_wait(unsigned want, unsigned* host_seq, u64* answer,
+                                         long long spin_ticks, long long* trace, int i, u64 (&rv)[NA])
+{
+    if (!take_answer_n<NA>(answer, want, wall_clock64(), spin_ticks, rv)) {
         __hip_atomic_store(host_seq, kSeqAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         return 0;
     }
@@ -533,153 +634,168 @@ __device__ __forceinline__ void tail_fold(const V2* sf, const V2* sg, unsigned q
 // at once (checked by an arrival gate at the start), each keep a contiguous
 // slice of the tables in LDS.  Variables bind LSB-first, so a slice of S entries folds
 // into its own S/2 entries and its pairs stay inside it for log2(S) - 1
-// rounds.  Per round every workgroup writes its three partial sums to its six
-// words of `parts` (device memory) as agent-scope stores, each value (61 bits)
-// tagged with the round mod 8 in its top three bits; wave 0 of workgroup 0
-// gathers them (lane l waits for the words of workgroups l, l + 64, ... to
-// carry the round's tag),
+// rounds.  Per exchange (one or two rounds, mid_exchange_rounds) every
+// workgroup writes its three or nine partial sums to its words of `parts`
+// (device memory) as agent-scope stores, each value (61 bits) tagged with the
+// exchange mod 8 in its top three bits; waves of workgroup 0 gather them
+// (wave s: sum s; lane l waits for the words of workgroups l, l + 64 to carry
+// the tag),
 // adds them, and publishes the total to the host like the one-workgroup tail
 // (tagged words, publish_tagged).  The words start as all ones (value
 // bits p, never canonical), so round 0 cannot match a stale word, and later
 // rounds find the previous round's tag until the new word lands: no fence and
 // no extra launch.  The host runs the transcript and answers through the
-// answer slot; workgroup 0 relays the challenge to the others through two
-// tagged words the same way.  Every wait is bounded by the wall clock; an
+// answer slot; workgroup 0 relays the challenges to the others through two or
+// four tagged words the same way.  Every wait is bounded by the wall clock; an
 // expired wait publishes kSeqAbort and relays p, and every workgroup leaves.
 // 128 measured 1% faster a prove than 64: one streaming launch fewer (2^20
 // entries go to the slices), for a dearer gather (two workgroups per lane)
 constexpr int kMidGroups = 128;
 constexpr unsigned kGatePending = 0, kGateGo = 1, kGateAbort = 2;
 // The mid launch's last table per workgroup: kMidGroups of them make the
-// tail's input.  Measured per round, the tail's rounds cost less than a mid
-// round (about 12.5 us: the gather, the host, the relay) from 2048 entries
-// down, and more above (one workgroup folding 8192 entries took 14 us).
+// tail's input.  With two rounds an exchange, handing 2048 entries to the
+// tail measured 150 us for the mid and tail rounds of a 2^24 prove, against
+// 161 us for 4096 and 164 us for 8192 (one workgroup folding the larger
+// tables costs more than the mid exchanges it saves).
 constexpr unsigned kMidEnd = 2048 / kMidGroups;
 static_assert(kMidEnd >= 2 && kMidEnd * kMidGroups <= kTailIn, "mid hand-over");
 constexpr unsigned kSeqGateAbort = 0xfffffffeu;   // the mid launch did not start whole
+constexpr int kMidWords = 9 * 2;   // a workgroup's tagged words per exchange (up to nine sums)
 
 // Fresh state for a mid launch: the tagged partial words and the relay all
 // ones (value bits p under tag 7: no round's word), the gate at no arrivals
 // and no decision.  One launch instead of three fills.
 __global__ void __launch_bounds__(256) k_sc2_mid_reset(u64* parts, u64* relay, unsigned* gate)
 {
-    for (int j = threadIdx.x; j < kMidGroups * 6; j += blockDim.x) parts[j] = ~0ull;
+    for (int j = threadIdx.x; j < kMidGroups * kMidWords; j += blockDim.x) parts[j] = ~0ull;
     if (threadIdx.x == 0) {
-        relay[0] = relay[1] = ~0ull;
+        relay[0] = relay[1] = relay[2] = relay[3] = ~0ull;
         gate[0] = 0;
         gate[1] = kGatePending;
     }
 }
 
-// End of a mid round, run by wave 0 of every workgroup: publish this
-// workgroup's sums (lane 0), and in workgroup 0 gather every workgroup's sums,
-// publish the total to the host and, unless `last`, wait for the host's
-// challenge and relay it; the others wait for the relay.  Returns 0 when a
-// wait expired (every workgroup then leaves); the challenge goes to rv.
-__device__ __forceinline__ int mid_exchange(const fp2 (&e)[3], int i, bool last, unsigned b,
-                                            int lane, u64* parts, fp2* host_vals,
-                                            unsigned* host_seq, unsigned seq0, u64* answer,
-                                            long long spin_ticks, u64* relay, long long* trace,
-                                            u64 (&rv)[2])
+// Exchanges of a mid launch of nr rounds: the last is a single round (its
+// challenge goes to the next launch as an argument), the first too when nr is
+// even, every other one two rounds (the nine quad sums, as in the tail).
+__host__ __device__ constexpr int mid_exchanges(int nr) { return (nr & 1) ? (nr + 1) / 2 : nr / 2 + 1; }
+__host__ __device__ inline int mid_exchange_rounds(int nr, int e)
 {
-    const u64 tag = (u64)(i & 7) << 61;   // round mod 8 over the 61 value bits
-    if (lane == 0) {
-        if (trace) trace[((size_t)i * kMidGroups + b) * 2] = wall_clock64();
-        u64* part = parts + (size_t)b * 6;
+    return (e == mid_exchanges(nr) - 1 || (e == 0 && !(nr & 1))) ? 1 : 2;
+}
+
+// End of a mid exchange, run by every thread of every workgroup with this
+// workgroup's N sums in s_sum: thread 0 stores them to its words of `parts`,
+// tagged with the exchange mod 8; in workgroup 0, wave s < N adds sum s over
+// every workgroup's words (lane l: workgroups l, l + 64, each word waited for
+// until it carries the tag) and thread 0 publishes the totals to the host
+// and, unless `last`, waits for the host's NA challenge words and relays
+// them; the other workgroups' thread 0 waits for the relay.  Returns 0 (in
+// every thread) when a wait expired: every workgroup then leaves.  The
+// challenges land in s_r.
+template <int N, int NA>
+__device__ __forceinline__ int mid_exchange(const fp2* s_sum, fp2* s_tot, int* s_flag, u64* s_r,
+                                            int e, bool last, unsigned b, u64* parts,
+                                            fp2* host_vals, unsigned* host_seq, unsigned seq0,
+                                            u64* answer, long long spin_ticks, u64* relay,
+                                            long long* trace)
+{
+    const unsigned t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const u64 tag = (u64)(e & 7) << 61;
+    if (t == 0) {
+        if (trace) trace[((size_t)e * kMidGroups + b) * 2] = wall_clock64();
+        u64* part = parts + (size_t)b * kMidWords;
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            __hip_atomic_store(part + 2 * j, e[j].re | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(part + 2 * j + 1, e[j].im | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int j = 0; j < N; ++j) {
+            __hip_atomic_store(part + 2 * j, s_sum[j].re | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(part + 2 * j + 1, s_sum[j].im | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        *s_flag = 1;
     }
-    int ok = 1;
     const long long t0 = wall_clock64();
     if (b == 0) {
-        // gather: lane l waits for the six words of this round of workgroups
-        // l, l + 64, ... (every word's first load in flight at once)
-        constexpr int kPer = (kMidGroups + 63) / 64;
-        fp2 v[3] = {zero2(), zero2(), zero2()};
-        u64 w[kPer][6];
-        auto load6 = [&](int q) {
-            const unsigned gw = lane + 64u * q;
+        __syncthreads();   // s_flag set before any wave can clear it
+        if (wid < (unsigned)N) {
+            int ok = 1;
+            fp2 acc = zero2();
+            for (unsigned g = lane; g < (unsigned)kMidGroups; g += 64) {
+                const u64* w = parts + (size_t)g * kMidWords + 2 * wid;
+                u64 x = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                u64 y = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                while (ok && ((x & ~P) != tag || (x & P) == P || (y & ~P) != tag || (y & P) == P)) {
+                    if (wall_clock64() - t0 > spin_ticks) {
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    x = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    y = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                acc = add2(acc, mk2(x & P, y & P));
+            }
+            fp2 v[1] = {acc};
+            wave_sum<1>(v);
+            if (lane == 0) s_tot[wid] = v[0];
+            if (!__all(ok) && lane == 0) *s_flag = 0;
+        }
+        __syncthreads();
+        if (t == 0) {
+            int ok = *s_flag;
+            u64 rv[NA];
 #pragma unroll
-            for (int j = 0; j < 6; ++j)
-                w[q][j] = gw < (unsigned)kMidGroups
-                              ? __hip_atomic_load(parts + (size_t)gw * 6 + j, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT)
-                              : tag;   // no such workgroup: 0 under this round's tag
-        };
+            for (int i = 0; i < NA; ++i) rv[i] = P;   // p: the wait expired
+            if (ok) {
+                fp2 v[N];
 #pragma unroll
-        for (int q = 0; q < kPer; ++q) load6(q);
+                for (int j = 0; j < N; ++j) v[j] = s_tot[j];
+                publish_tagged<N>(host_vals, v, seq0 + (unsigned)e);
+                if (!last) ok = take_answer_n<NA>(answer, seq0 + (unsigned)e, t0, spin_ticks, rv);
+            }
+            if (!ok) __hip_atomic_store(host_seq, kSeqAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (!last || !ok) {
+                // relay to the other workgroups (p under the tag: leave)
 #pragma unroll
-        for (int q = 0; q < kPer; ++q) {
+                for (int i = 0; i < NA; ++i)
+                    __hip_atomic_store(relay + i, (ok ? rv[i] : P) | tag, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (trace) trace[((size_t)e * kMidGroups) * 2 + 1] = wall_clock64();
+#pragma unroll
+            for (int i = 0; i < NA; ++i) s_r[i] = rv[i];
+            *s_flag = ok;
+        }
+    } else if (t == 0) {
+        int ok = 1;
+        if (!last) {
+            // wait for workgroup 0's relay
             for (;;) {
+                u64 x[NA];
                 bool all = true;
 #pragma unroll
-                for (int j = 0; j < 6; ++j) all = all && (w[q][j] & ~P) == tag && (w[q][j] & P) != P;
-                if (all || !ok) break;
+                for (int i = 0; i < NA; ++i) {
+                    x[i] = __hip_atomic_load(relay + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    all = all && (x[i] & ~P) == tag;
+                }
+                if (all) {
+#pragma unroll
+                    for (int i = 0; i < NA; ++i) {
+                        s_r[i] = x[i] & P;
+                        if (s_r[i] == P) ok = 0;
+                    }
+                    break;
+                }
                 if (wall_clock64() - t0 > spin_ticks) {
                     ok = 0;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
-                load6(q);
             }
-#pragma unroll
-            for (int j = 0; j < 3; ++j) v[j] = add2(v[j], mk2(w[q][2 * j] & P, w[q][2 * j + 1] & P));
+            if (trace) trace[((size_t)e * kMidGroups + b) * 2 + 1] = wall_clock64();
         }
-        ok = __all(ok);
-        wave_sum<3>(v);
-        u64 re = P, im = P;   // p: the wait expired
-        if (lane == 0) {
-            if (ok) {
-                publish_tagged<3>(host_vals, v, seq0 + (unsigned)i);
-                if (!last) {
-                    u64 a[2];
-                    ok = take_answer(answer, seq0 + (unsigned)i, t0, spin_ticks, a);
-                    if (ok) {
-                        re = a[0];
-                        im = a[1];
-                    }
-                }
-            }
-            if (!ok) __hip_atomic_store(host_seq, kSeqAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            if (!last || !ok) {
-                // relay to the other workgroups (p under the tag: leave)
-                __hip_atomic_store(relay, re | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(relay + 1, im | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            if (trace) trace[((size_t)i * kMidGroups) * 2 + 1] = wall_clock64();
-        }
-        ok = __shfl(ok, 0, 64);
-        rv[0] = __shfl(re, 0, 64);
-        rv[1] = __shfl(im, 0, 64);
-        return ok;
+        *s_flag = ok;
     }
-    u64 re = 0, im = 0;
-    if (lane == 0 && !last) {
-        // wait for workgroup 0's relay
-        for (;;) {
-            const u64 x = __hip_atomic_load(relay, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const u64 y = __hip_atomic_load(relay + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((x & ~P) == tag && (y & ~P) == tag) {
-                re = x & P;
-                im = y & P;
-                if (re == P || im == P) ok = 0;
-                break;
-            }
-            if (wall_clock64() - t0 > spin_ticks) {
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        if (trace) trace[((size_t)i * kMidGroups + b) * 2 + 1] = wall_clock64();
-    }
-    ok = __shfl(ok, 0, 64);
-    rv[0] = __shfl(re, 0, 64);
-    rv[1] = __shfl(im, 0, 64);
-    return ok;
+    __syncthreads();
+    return *s_flag;
 }
 
 __global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
@@ -690,13 +806,13 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
     unsigned r_want, int r_from_slot, u64* arelay)
 {
     extern __shared__ V2 tl[];
-    __shared__ u64 s_r[2];
+    __shared__ u64 s_r[4];
     __shared__ int s_ok;
-    __shared__ fp2 s_part[3][kTailThreads / 64];
+    __shared__ fp2 s_sum[9], s_tot[9];
     const unsigned half = slice_in / 2;
     V2* tf = tl;
     V2* tg = tl + half;
-    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int t = threadIdx.x, lane = t & 63;
     const unsigned b = blockIdx.x;
     // Arrival gate: the rounds need every workgroup resident at once.  Each
     // counts itself into gate[0]; workgroup 0 waits up to gate_ticks for all
@@ -756,120 +872,53 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
         rr = rw[0];
         ri = rw[1];
     }
-    const V2* sf0 = fin + (size_t)b * slice_in;
-    const V2* sg0 = gin + (size_t)b * slice_in;
-    fp2 r = mk2(rr, ri);
+    const V2* sf = fin + (size_t)b * slice_in;   // T_{k-np}
+    const V2* sg = gin + (size_t)b * slice_in;
+    fp2 ra = mk2(rr, ri), rb = zero2(), rab = zero2();
+    int np = 1;
     unsigned len = slice_in;
-    int i = 0;
-    // Rounds with more than one wave's quads per slice: the whole workgroup.
-    for (; i < n_rounds && len / 4 > 64; ++i) {
-        const unsigned nq = len / 4;
-        LazyProdSum z[3];
-        lazy_zero(z);
-        if (i == 0) {
-            for (unsigned q = t; q < nq; q += kTailThreads) {
-                fp2 f0, f1, g0, g1;
-                tail_fold(sf0, sg0, q, r, f0, f1, g0, g1);
-                stv(tf + 2 * q, f0);
-                stv(tf + 2 * q + 1, f1);
-                stv(tg + 2 * q, g0);
-                stv(tg + 2 * q + 1, g1);
-                accum_pair(z, f0, f1, g0, g1);
-            }
+    const int ne = mid_exchanges(n_rounds);
+    for (int e = 0; e < ne; ++e) {
+        const int nre = mid_exchange_rounds(n_rounds, e);
+        const bool last = e == ne - 1;
+        int ok;
+        if (nre == 2) {
+            exchange_work<9>(sf, sg, e > 0, len, np, ra, rb, rab, tf, tg, s_sum);
+            ok = mid_exchange<9, 4>(s_sum, s_tot, &s_ok, s_r, e, last, b, parts, host_vals,
+                                    host_seq, seq0, answer, spin_ticks, relay, trace);
         } else {
-            const unsigned q = t;   // len <= slice_in / 2 <= 4 kTailThreads
-            fp2 f0, f1, g0, g1;
-            if (q < nq) {
-                tail_fold(tf, tg, q, r, f0, f1, g0, g1);
-                accum_pair(z, f0, f1, g0, g1);
-            }
-            __syncthreads();   // every read of T_{k-1} is done before T_k overwrites it
-            if (q < nq) {
-                stv(tf + 2 * q, f0);
-                stv(tf + 2 * q + 1, f1);
-                stv(tg + 2 * q, g0);
-                stv(tg + 2 * q + 1, g1);
-            }
+            exchange_work<3>(sf, sg, e > 0, len, np, ra, rb, rab, tf, tg, s_sum);
+            ok = mid_exchange<3, 2>(s_sum, s_tot, &s_ok, s_r, e, last, b, parts, host_vals,
+                                    host_seq, seq0, answer, spin_ticks, relay, trace);
         }
-        const int na = (int)((nq + 63) / 64) < kTailThreads / 64 ? (int)((nq + 63) / 64)
-                                                                 : kTailThreads / 64;
-        fp2 e[3];
-        if (wid < na) {
-#pragma unroll
-            for (int j = 0; j < 3; ++j) e[j] = z[j].value();
-            wave_sum<3>(e);
-            if (lane == 0) {
-#pragma unroll
-                for (int j = 0; j < 3; ++j) s_part[j][wid] = e[j];
-            }
+        if (!ok) return;
+        len >>= np;   // the table at tf / tg is T_k now
+        np = nre;
+        sf = tf;
+        sg = tg;
+        if (!last) {
+            ra = uniform2(mk2(s_r[0], s_r[1]));
+            rb = uniform2(mk2(s_r[2], s_r[3]));
+            rab = uniform2(mul2(ra, rb));
         }
-        __syncthreads();
-        const bool last = i == n_rounds - 1;
-        if (wid == 0) {
-#pragma unroll
-            for (int j = 0; j < 3; ++j) e[j] = lane < na ? s_part[j][lane] : zero2();
-            wave_sum<3>(e);
-            u64 rv[2];
-            const int ok = mid_exchange(e, i, last, b, lane, parts, host_vals, host_seq, seq0, answer,
-                                        spin_ticks, relay, trace, rv);
-            if (lane == 0) {
-                s_ok = ok;
-                s_r[0] = rv[0];
-                s_r[1] = rv[1];
-            }
-        }
-        __syncthreads();   // r_k and T_k visible to every thread
-        if (!s_ok) return;
-        if (!last) r = mk2(s_r[0], s_r[1]);
-        len /= 2;
+        __syncthreads();   // s_r and s_sum are rewritten by the next exchange
     }
-    // The rest in wave 0 alone (its LDS reads precede its writes in program
-    // order; r goes lane to lane), then the slice's last table goes out.
-    if (wid != 0) return;
-    for (; i < n_rounds; ++i) {
-        const unsigned nq = len / 4;
-        LazyProdSum z[3];
-        lazy_zero(z);
-        const unsigned q = lane;
-        const V2* sf = i == 0 ? sf0 : tf;
-        const V2* sg = i == 0 ? sg0 : tg;
-        if (q < nq) {
-            fp2 f0, f1, g0, g1;
-            tail_fold(sf, sg, q, r, f0, f1, g0, g1);
-            stv(tf + 2 * q, f0);
-            stv(tf + 2 * q + 1, f1);
-            stv(tg + 2 * q, g0);
-            stv(tg + 2 * q + 1, g1);
-            accum_pair(z, f0, f1, g0, g1);
-        }
-        fp2 e[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) e[j] = z[j].value();
-        wave_sum<3>(e);
-        const bool last = i == n_rounds - 1;
-        u64 rv[2];
-        if (!mid_exchange(e, i, last, b, lane, parts, host_vals, host_seq, seq0, answer, spin_ticks,
-                          relay, trace, rv))
-            return;
-        if (!last) r = mk2(rv[0], rv[1]);
-        len /= 2;
-    }
-    for (unsigned j = lane; j < len; j += 64) {
+    // the slice's last table goes out (its round's challenge is the next launch's)
+    for (unsigned j = t; j < len; j += kTailThreads) {
         fout[(size_t)b * len + j] = tf[j];
         gout[(size_t)b * len + j] = tg[j];
     }
-    if (b == 0) {
-        // every round done: workgroup 0 has gathered every workgroup's last
+    if (b == 0 && t < 64) {
+        // every exchange done: workgroup 0 has gathered every workgroup's last
         // sums, so all of them are past the gate and every relay.  It leaves
         // the words as k_sc2_mid_reset would, and the next mid launch needs
         // no reset launch of its own.
-        for (int j = lane; j < kMidGroups * 6; j += 64)
+        for (int j = lane; j < kMidGroups * kMidWords; j += 64)
             __hip_atomic_store(parts + j, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (lane < 4 && r_from_slot)   // every workgroup took the first challenge long ago
             __hip_atomic_store(arelay + lane, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane < 4) __hip_atomic_store(relay + lane, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (lane == 0) {
-            __hip_atomic_store(relay, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(relay + 1, ~0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(gate, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(gate + 1, kGatePending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -882,120 +931,66 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
     long long spin_ticks, long long* trace)
 {
     extern __shared__ V2 tl[];
-    __shared__ u64 s_r[2];
+    __shared__ u64 s_r[4];
     __shared__ int s_ok;
-    __shared__ fp2 s_part[3][kTailThreads / 64];
+    __shared__ fp2 s_sum[9];
     V2* tf = tl;
     V2* tg = tl + kTailIn / 2;
     const int t = threadIdx.x;
-    const int nr = nv - k0;
-    fp2 r = mk2(rr, ri);
-    unsigned len = len_in;   // entries of the input table T_{k-1}
-    int k = k0;
-    // Rounds with more than one wave's quads: the whole workgroup.
-    for (; k < nv && len / 4 > 64; ++k) {
-        const unsigned nq = len / 4;
-        if (trace && t == 0) trace[3 * (k - k0)] = wall_clock64();
-        LazyProdSum z[3];
-        lazy_zero(z);
-        if (k == k0) {
-            // from global memory straight into LDS (up to two quads a thread)
-            for (unsigned q = t; q < nq; q += kTailThreads) {
-                fp2 f0, f1, g0, g1;
-                tail_fold(fin, gin, q, r, f0, f1, g0, g1);
-                stv(tf + 2 * q, f0);
-                stv(tf + 2 * q + 1, f1);
-                stv(tg + 2 * q, g0);
-                stv(tg + 2 * q + 1, g1);
-                accum_pair(z, f0, f1, g0, g1);
-            }
-        } else {
-            // in place in LDS: at most one quad a thread (len <= kTailIn / 2)
-            const unsigned q = t;
-            fp2 f0, f1, g0, g1;
-            if (q < nq) {
-                tail_fold(tf, tg, q, r, f0, f1, g0, g1);
-                accum_pair(z, f0, f1, g0, g1);
-            }
-            __syncthreads();   // every read of T_{k-1} is done before T_k overwrites it
-            if (q < nq) {
-                stv(tf + 2 * q, f0);
-                stv(tf + 2 * q + 1, f1);
-                stv(tg + 2 * q, g0);
-                stv(tg + 2 * q + 1, g1);
-            }
-        }
-        // waves with quads reduce theirs; wave 0 reduces the wave totals
-        const int lane = t & 63, wid = t >> 6;
-        const int na = (int)((nq + 63) / 64) < kTailThreads / 64 ? (int)((nq + 63) / 64)
-                                                                 : kTailThreads / 64;
-        fp2 e[3];
-        if (wid < na) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) e[i] = z[i].value();
-            wave_sum<3>(e);
-            if (lane == 0) {
-#pragma unroll
-                for (int i = 0; i < 3; ++i) s_part[i][wid] = e[i];
-            }
-        }
-        __syncthreads();
-        if (wid == 0) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) e[i] = lane < na ? s_part[i][lane] : zero2();
-            wave_sum<3>(e);
-        }
+    fp2 ra = mk2(rr, ri), rb = zero2(), rab = zero2();
+    int np = 1;                 // challenges not yet applied to the table at sf / sg
+    const V2* sf = fin;         // T_{k-np}
+    const V2* sg = gin;
+    unsigned len = len_in;      // its entries per table
+    for (int k = k0, e = 0; k < nv; ++e) {
+        if (trace && t == 0) trace[3 * e] = wall_clock64();
+        const bool pair = nv - k >= 2;
+        if (pair)
+            exchange_work<9>(sf, sg, sf != fin, len, np, ra, rb, rab, tf, tg, s_sum);
+        else
+            exchange_work<3>(sf, sg, sf != fin, len, np, ra, rb, rab, tf, tg, s_sum);
         if (t == 0) {
-            u64 rv[2] = {0, 0};
-            s_ok = tail_exchange(e, seq0 + (unsigned)(k - k0), host_vals, host_seq, answer,
-                                 spin_ticks, trace, k - k0, rv);
-            s_r[0] = rv[0];
-            s_r[1] = rv[1];
+            // lane 0 publishes the sums and waits for the challenges
+            if (trace) trace[3 * e + 1] = wall_clock64();
+            const unsigned want = seq0 + (unsigned)e;
+            u64 rv[4] = {0, 0, 0, 0};
+            if (pair) {
+                fp2 v[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) v[i] = s_sum[i];
+                publish_tagged<9>(host_vals, v, want);
+                s_ok = tail_wait<4>(want, host_seq, answer, spin_ticks, trace, e, rv);
+            } else {
+                const fp2 v[3] = {s_sum[0], s_sum[1], s_sum[2]};
+                publish_tagged<3>(host_vals, v, want);
+                u64 r2[2] = {0, 0};
+                s_ok = tail_wait<2>(want, host_seq, answer, spin_ticks, trace, e, r2);
+                rv[0] = r2[0];
+                rv[1] = r2[1];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) s_r[i] = rv[i];
         }
-        __syncthreads();   // r_k and T_k visible to every thread
+        __syncthreads();   // the challenges visible to every thread
         if (!s_ok) return;
-        r = mk2(s_r[0], s_r[1]);
-        len /= 2;
-    }
-    // The rest in wave 0 alone: its LDS reads of T_{k-1} precede its writes
-    // of T_k in program order, so no barrier is needed; r goes lane to lane.
-    if (t >= 64) return;
-    const V2* sf = k == k0 ? fin : tf;
-    const V2* sg = k == k0 ? gin : tg;
-    for (; k < nv; ++k) {
-        const unsigned nq = len / 4;
-        if (trace && t == 0) trace[3 * (k - k0)] = wall_clock64();
-        LazyProdSum z[3];
-        lazy_zero(z);
-        const unsigned q = t;
-        if (q < nq) {
-            fp2 f0, f1, g0, g1;
-            tail_fold(sf, sg, q, r, f0, f1, g0, g1);
-            stv(tf + 2 * q, f0);
-            stv(tf + 2 * q + 1, f1);
-            stv(tg + 2 * q, g0);
-            stv(tg + 2 * q + 1, g1);
-            accum_pair(z, f0, f1, g0, g1);
-        }
+        ra = uniform2(mk2(s_r[0], s_r[1]));
+        rb = uniform2(mk2(s_r[2], s_r[3]));
+        rab = uniform2(mul2(ra, rb));
+        len >>= np;   // the table at sf / sg is T_k now
+        np = pair ? 2 : 1;
+        k += np;
         sf = tf;
         sg = tg;
-        fp2 e[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) e[i] = z[i].value();
-        wave_sum<3>(e);
-        u64 rv[2] = {0, 0};
-        int ok = 1;
-        if (t == 0)
-            ok = tail_exchange(e, seq0 + (unsigned)(k - k0), host_vals, host_seq, answer,
-                               spin_ticks, trace, k - k0, rv);
-        if (!__shfl(ok, 0, 64)) return;
-        r = mk2(__shfl(rv[0], 0, 64), __shfl(rv[1], 0, 64));
-        len /= 2;
+        if (k == nv && t == 0) {
+            // T_{nv-np} (2^np entries) folded by the last challenges
+            const fp2 out[2] = {fold_head(sf, np, ra, rb, rab), fold_head(sg, np, ra, rb, rab)};
+            publish_tagged<2>(host_vals, out, seq0 + (unsigned)(e + 1));
+        }
     }
-    if (t == 0) {
-        // T_{nv-1}: two entries (in LDS, or still the input when no round ran here)
-        const fp2 fin[2] = {fold2(ldv(sf), ldv(sf + 1), r), fold2(ldv(sg), ldv(sg + 1), r)};
-        publish_tagged<2>(host_vals, fin, seq0 + (unsigned)nr);
+    if (k0 == nv && t == 0) {
+        // no round here: the input's two entries folded by r_{nv-1}
+        const fp2 out[2] = {fold_head(fin, 1, ra, rb, rab), fold_head(gin, 1, ra, rb, rab)};
+        publish_tagged<2>(host_vals, out, seq0);
     }
 }
 
@@ -1100,9 +1095,9 @@ bool sc2_setup(size_t n)
     }
     if (!S.work.ensure(2 * (n / 2 + n / 4 + 2) * sizeof(V2))) return false;
     if (!S.partials.ensure(2048 * 9 * sizeof(fp2))) return false;
-    if (!S.relay.ensure(2 * sizeof(u64))) return false;
+    if (!S.relay.ensure(4 * sizeof(u64))) return false;
     if (!S.arelay.ensure(4 * sizeof(u64))) return false;
-    if (!S.mid_parts.ensure(kMidGroups * 6 * sizeof(u64))) return false;
+    if (!S.mid_parts.ensure(kMidGroups * kMidWords * sizeof(u64))) return false;
     if (!S.gate.ensure(2 * sizeof(unsigned))) return false;
     if (!S.slots.ensure(kSlotWords * sizeof(u64))) return false;
     if (!S.ticket.ensure(kTicketBytes)) return false;
@@ -1187,6 +1182,14 @@ void take_seq_vals(int n)
     volatile u64* w = reinterpret_cast<volatile u64*>(S.host_vals);
     for (int i = 0; i < n; ++i) S.pub[i] = mk2(w[2 * i], w[2 * i + 1]);
     for (int i = 0; i < 2 * n; ++i) w[i] = kSlotEmpty;
+}
+
+// c0 + r (c1 - c0 - c2) + r^2 c2: a round's sum at challenge r of the round
+// before, from the three sums over the quads of that round's table whose
+// products give its coefficients (see the file header).
+fp2 quad_at(fp2 r, fp2 c0, fp2 c1, fp2 c2)
+{
+    return add2(c0, mul2(r, add2(sub2(sub2(c1, c0), c2), mul2(r, c2))));
 }
 
 // Waits until the launches before `ev` on its stream are done.
@@ -1306,6 +1309,38 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         chals[2 * k + 1] = r.im;
         return r;
     };
+
+    // The host side of a mid launch's exchanges (mid_exchange_rounds): from
+    // round k_first, sequence numbers seq0m + e; the first exchange's values
+    // are already in S.pub when first_done.  Every exchange but the last is
+    // answered (the last round's challenge goes to the next launch); r is the
+    // last round's challenge.  False on failure.
+    auto mid_host = [&](int k_first, unsigned seq0m, int nrm, bool first_done, fp2& r_out) -> bool {
+        const int ne = mid_exchanges(nrm);
+        int kk = k_first;
+        for (int e = 0; e < ne; ++e) {
+            const int nre = mid_exchange_rounds(nrm, e);
+            if ((e > 0 || !first_done) && wait_pub(st, seq0m + e, nre == 2 ? 9 : 3, false) != 1)
+                return false;
+            const bool last = e + 1 == ne;
+            if (nre == 2) {
+                fp2 v[9];
+                for (int i = 0; i < 9; ++i) v[i] = S.pub[i];
+                fp2 rs[2];
+                rs[0] = round_done(kk, add2(v[0], v[3]), add2(v[1], v[4]), add2(v[2], v[5]));
+                rs[1] = round_done(kk + 1, quad_at(rs[0], v[0], v[1], v[2]),
+                                   quad_at(rs[0], v[3], v[4], v[5]), quad_at(rs[0], v[6], v[7], v[8]));
+                if (!last) answer(seq0m + e, rs, 2);
+                r_out = rs[1];
+            } else {
+                r_out = round_done(kk, S.pub[0], S.pub[1], S.pub[2]);
+                if (!last) answer(seq0m + e, &r_out, 1);
+            }
+            kk += nre;
+        }
+        return true;
+    };
+
     // Launch bracket: events around one launch of class c (timing on).
     auto t_begin = [&]() -> bool {
         if (S.timed) TP_HIP_OK(hipEventRecord(S.ev[0], st));
@@ -1413,7 +1448,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             mid_of = (qf >= A && qf < A + 2 * capA) ? B : A;
             mid_og = mid_of + (mid_of == A ? capA : capB);
             mid_seq0 = S.seq + 1;
-            S.seq += (unsigned)mid_nr;
+            S.seq += (unsigned)mid_exchanges(mid_nr);
             const V2* a_f = qf;
             const V2* a_g = qg;
             unsigned a_slice = slice;
@@ -1454,11 +1489,8 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         for (int i = 0; i < 9; ++i) v[i] = S.pub[i];
         fp2 rs[2];
         rs[0] = round_done(0, add2(v[0], v[3]), add2(v[1], v[4]), add2(v[2], v[5]));
-        auto at_r0 = [&](fp2 c0, fp2 c1, fp2 c2) {
-            return add2(c0, mul2(rs[0], add2(sub2(sub2(c1, c0), c2), mul2(rs[0], c2))));
-        };
-        rs[1] = round_done(1, at_r0(v[0], v[1], v[2]), at_r0(v[3], v[4], v[5]),
-                           at_r0(v[6], v[7], v[8]));
+        rs[1] = round_done(1, quad_at(rs[0], v[0], v[1], v[2]), quad_at(rs[0], v[3], v[4], v[5]),
+                           quad_at(rs[0], v[6], v[7], v[8]));
         answer(seq_a, rs, 2);
         if (wait_pub(st, seq_b, 3, false) != 1) return -1;
         r = round_done(2, S.pub[0], S.pub[1], S.pub[2]);
@@ -1473,7 +1505,8 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         m = qm;
         k0 = qk;
         if (mid_q) {
-            const int first = wait_pub(st, mid_seq0, 3, true);
+            const int first =
+                wait_pub(st, mid_seq0, mid_exchange_rounds(mid_nr, 0) == 2 ? 9 : 3, true);
             if (first == 0) return -1;
             if (first == 2) {
                 // not every workgroup was resident: the loop below proves
@@ -1486,11 +1519,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
                 S.mid_fallbacks += 1;
                 use_mid = false;
             } else {
-                for (int i = 0; i < mid_nr; ++i) {
-                    if (i > 0 && wait_pub(st, mid_seq0 + i, 3, false) != 1) return -1;
-                    r = round_done(qk + i, S.pub[0], S.pub[1], S.pub[2]);
-                    if (i + 1 < mid_nr) answer(mid_seq0 + i, &r, 1);
-                }
+                if (!mid_host(qk, mid_seq0, mid_nr, true, r)) return -1;
                 // every round is published; the launch resets its words as it ends
                 S.mid_clean = true;
                 cf = mid_of;
@@ -1512,12 +1541,9 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         fp2 v[9];
         for (int i = 0; i < 9; ++i) v[i] = S.pub[i];
         const fp2 r0 = round_done(0, add2(v[0], v[3]), add2(v[1], v[4]), add2(v[2], v[5]));
-        // the round-1 quadratics at r_0: c0 + r (c1 - c0 - c2) + r^2 c2
-        auto at_r0 = [&](fp2 c0, fp2 c1, fp2 c2) {
-            return add2(c0, mul2(r0, add2(sub2(sub2(c1, c0), c2), mul2(r0, c2))));
-        };
-        const fp2 r1 = round_done(1, at_r0(v[0], v[1], v[2]), at_r0(v[3], v[4], v[5]),
-                                  at_r0(v[6], v[7], v[8]));
+        // the round-1 quadratics at r_0
+        const fp2 r1 = round_done(1, quad_at(r0, v[0], v[1], v[2]), quad_at(r0, v[3], v[4], v[5]),
+                                  quad_at(r0, v[6], v[7], v[8]));
         seq = ++S.seq;
         V2* of = B;
         V2* og = B + capB;
@@ -1550,11 +1576,12 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
     }
     for (int k = k0;; ++k) {
         if (2 * m <= kTailIn) {
-            // rounds k..nv-1 and the final fold in one workgroup; the host
-            // answers each round's challenge through the mapped answer slot
+            // rounds k..nv-1 and the final fold in one workgroup, two rounds
+            // per exchange; the host answers through the mapped answer slot
             const int nr = nv - k;
+            const int ne = tail_exchanges(nr);
             const unsigned seq0 = S.seq + 1;
-            S.seq += (unsigned)nr + 1;
+            S.seq += (unsigned)ne + 1;
             if (!t_begin()) return -1;
             k_sc2_tail_host<<<1, kTailThreads, kTailLds, st>>>(
                 cf, cg, (unsigned)(2 * m), k, nv, r.re, r.im, S.dev_vals, S.dev_seq, seq0,
@@ -1563,29 +1590,45 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             TP_HIP_OK(hipGetLastError());
             volatile u64* ans = S.host_answer;
             double host_us = 0;
-            for (int i = 0; i < nr; ++i) {
-                if (wait_pub(st, seq0 + i, 3, false) != 1) return -1;
+            for (int e = 0; e < ne; ++e) {
+                const int kk = k + 2 * e;
+                const bool pair = nv - kk >= 2;
+                if (wait_pub(st, seq0 + e, pair ? 9 : 3, false) != 1) return -1;
                 const auto h0 = std::chrono::steady_clock::now();
-                const fp2 rk = round_done(k + i, S.pub[0], S.pub[1], S.pub[2]);
-                const u64 tag = (u64)((seq0 + i) & 7) << 61;   // marks the words as this round's
-                ans[0] = rk.re | tag;
-                ans[1] = rk.im | tag;
+                const u64 tag = (u64)((seq0 + e) & 7) << 61;   // marks the words as this exchange's
+                if (pair) {
+                    // rounds kk and kk + 1 from the nine quad sums of T_kk
+                    fp2 v[9];
+                    for (int i = 0; i < 9; ++i) v[i] = S.pub[i];
+                    const fp2 ra = round_done(kk, add2(v[0], v[3]), add2(v[1], v[4]), add2(v[2], v[5]));
+                    const fp2 rb = round_done(kk + 1, quad_at(ra, v[0], v[1], v[2]),
+                                              quad_at(ra, v[3], v[4], v[5]),
+                                              quad_at(ra, v[6], v[7], v[8]));
+                    ans[0] = ra.re | tag;
+                    ans[1] = ra.im | tag;
+                    ans[2] = rb.re | tag;
+                    ans[3] = rb.im | tag;
+                } else {
+                    const fp2 rk = round_done(kk, S.pub[0], S.pub[1], S.pub[2]);
+                    ans[0] = rk.re | tag;
+                    ans[1] = rk.im | tag;
+                }
                 host_us += std::chrono::duration<double, std::micro>(
                                std::chrono::steady_clock::now() - h0).count();
             }
-            if (wait_pub(st, seq0 + nr, 2, false) != 1 || !t_collect(SC_TAIL)) return -1;
-            if (S.timed && nr > 0) {
+            if (wait_pub(st, seq0 + ne, 2, false) != 1 || !t_collect(SC_TAIL)) return -1;
+            if (S.timed && ne > 0) {
                 const volatile long long* tr = S.host_trace;
                 double work = 0, wait = 0;
-                for (int i = 0; i < nr; ++i) {
+                for (int i = 0; i < ne; ++i) {
                     work += (double)(tr[3 * i + 1] - tr[3 * i]);
                     wait += (double)(tr[3 * i + 2] - tr[3 * i + 1]);
                     if (i < 40) S.tail_round_us[i] = (double)(tr[3 * i + 1] - tr[3 * i]) / S.clock_mhz;
                 }
-                S.tail_rounds = nr;
-                S.tail_work_us = work / S.clock_mhz / nr;
-                S.tail_wait_us = wait / S.clock_mhz / nr;
-                S.tail_host_us = host_us / nr;
+                S.tail_rounds = ne;
+                S.tail_work_us = work / S.clock_mhz / ne;
+                S.tail_wait_us = wait / S.clock_mhz / ne;
+                S.tail_host_us = host_us / ne;
             }
             final4[0] = S.pub[0].re; final4[1] = S.pub[0].im;
             final4[2] = S.pub[1].re; final4[3] = S.pub[1].im;
@@ -1601,7 +1644,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             V2* of = (cf >= A && cf < A + 2 * capA) ? B : A;
             V2* og = of + (of == A ? capA : capB);
             const unsigned seq0 = S.seq + 1;
-            S.seq += (unsigned)nr;
+            S.seq += (unsigned)mid_exchanges(nr);
             const V2* a_f = cf;
             const V2* a_g = cg;
             unsigned a_slice = slice;
@@ -1616,7 +1659,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             V2* a_of = of;
             V2* a_og = og;
             u64* a_relay = S.relay.as<u64>();
-            long long* a_trace = S.timed && nr <= 40 ? S.dev_mtrace : nullptr;
+            long long* a_trace = S.timed && mid_exchanges(nr) <= 40 ? S.dev_mtrace : nullptr;
             unsigned* a_gate = S.gate.as<unsigned>();
             long long a_gate_ticks = S.gate_ticks;
             unsigned a_want = 0;
@@ -1634,7 +1677,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             TP_HIP_OK(hipLaunchKernel((const void*)k_sc2_mid, dim3(kMidGroups), dim3(kTailThreads),
                                       args, (unsigned)(slice * sizeof(V2)), st));
             if (!t_end(SC_MID, 32.0 * (double)(2 * m) + 64.0 * kMidGroups)) return -1;
-            const int first = wait_pub(st, seq0, 3, true);
+            const int first = wait_pub(st, seq0, mid_exchange_rounds(nr, 0) == 2 ? 9 : 3, true);
             if (first == 0) return -1;
             if (first == 2) {
                 // not every workgroup was resident: these rounds go to the
@@ -1649,16 +1692,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
                 --k;          // the loop's ++k returns to this round
                 continue;
             }
-            volatile u64* ans = S.host_answer;
-            for (int i = 0; i < nr; ++i) {
-                if (i > 0 && wait_pub(st, seq0 + i, 3, false) != 1) return -1;
-                r = round_done(k + i, S.pub[0], S.pub[1], S.pub[2]);
-                if (i + 1 < nr) {
-                    const u64 tag = (u64)((seq0 + i) & 7) << 61;
-                    ans[0] = r.re | tag;
-                    ans[1] = r.im | tag;
-                }
-            }
+            if (!mid_host(k, seq0, nr, true, r)) return -1;
             // Every round is published; the launch resets its words as it ends.
             // The next launches queue behind it on the stream, so the host
             // waits for its end only to read its timing (timing on).
@@ -1667,13 +1701,14 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
                 TP_HIP_OK(hipEventRecord(S.ev_round, st));
                 if (!wait_event(S.ev_round) || !t_collect(SC_MID)) return -1;
             }
-            if (S.timed && nr > 1 && nr <= 40) {
-                // per round: out = partials written, in = challenge held
+            const int ne = mid_exchanges(nr);
+            if (S.timed && ne > 1 && ne <= 40) {
+                // per exchange: out = partials written, in = challenges held
                 const volatile long long* tr = S.host_mtrace;
                 auto out_t = [&](int i, int b) { return (double)tr[((size_t)i * kMidGroups + b) * 2]; };
                 auto in_t = [&](int i, int b) { return (double)tr[((size_t)i * kMidGroups + b) * 2 + 1]; };
                 double host = 0, relay = 0, work = 0, spread = 0;
-                for (int i = 0; i + 1 < nr; ++i) {
+                for (int i = 0; i + 1 < ne; ++i) {
                     double last_out = out_t(i, 0), first_out = out_t(i, 0), last_in = in_t(i, 0);
                     double w = 0;
                     for (int b = 0; b < kMidGroups; ++b) {
@@ -1687,8 +1722,8 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
                     work += w / kMidGroups;             // challenge -> next partials
                     spread += last_out - first_out;     // partials: first to last
                 }
-                const double d = S.clock_mhz * (nr - 1);
-                S.mid_rounds = nr;
+                const double d = S.clock_mhz * (ne - 1);
+                S.mid_rounds = ne;
                 S.mid_host_us = host / d;
                 S.mid_relay_us = relay / d;
                 S.mid_work_us = work / d;

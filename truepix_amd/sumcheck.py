@@ -120,10 +120,11 @@ def class_stats():
 
 
 def tail_profile():
-    """Per-round latency of the one-workgroup tail in the last timed prove:
-    {"rounds", "device_us" (a round's fold and sums), "wait_us" (waiting for
-    the host's challenge), "host_us" (the host's transcript step),
-    "round_device_us" (per round)}."""
+    """Per-exchange latency of the one-workgroup tail in the last timed
+    prove (an exchange is one host round trip for one or two rounds):
+    {"rounds" (exchanges), "device_us" (an exchange's fold and sums),
+    "wait_us" (waiting for the host's challenges), "host_us" (the host's
+    transcript steps), "round_device_us" (per exchange)}."""
     out = (ctypes.c_double * 44)()
     nb.LIB.tp_sc2_tail_profile(out, 44)
     n = int(out[0])
@@ -132,10 +133,11 @@ def tail_profile():
 
 
 def mid_profile():
-    """Per-round latency of the sliced mid rounds in the last timed prove:
-    {"rounds", "host_us" (last partials to the challenge at workgroup 0),
-    "relay_us" (to the last workgroup), "device_us" (challenge to the next
-    partials), "spread_us" (first to last workgroup's partials)}."""
+    """Per-exchange latency of the sliced mid rounds in the last timed prove
+    (an exchange is one host round trip for one or two rounds):
+    {"rounds" (exchanges), "host_us" (last partials to the challenges at
+    workgroup 0), "relay_us" (to the last workgroup), "device_us" (challenges
+    to the next partials), "spread_us" (first to last workgroup's partials)}."""
     out = (ctypes.c_double * 5)()
     nb.LIB.tp_sc2_mid_profile(out, 5)
     return {"rounds": int(out[0]), "host_us": round(out[1], 3), "relay_us": round(out[2], 3),
