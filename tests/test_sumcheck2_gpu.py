@@ -164,3 +164,24 @@ def test_sc2_queued_and_serial_launches_agree(bridge, oracle, torch_cuda, nv):
             assert np.array_equal(got.final, want[2]), ahead
     finally:
         sc.set_ahead(False)
+
+
+@pytest.mark.parametrize("nv", [14, 18])
+def test_sc2_edge_values_matrix_core_pass(bridge, oracle, torch_cuda, nv):
+    """Tables large enough for the first pass on the int8 matrix cores, with
+    the extreme values (p - 1, 0, 1, bytes of 0x00 and 0xff, every byte at
+    its signed limits) mixed in: its integer tiles, byte offsets and column
+    sums must give the oracle's round polynomials exactly."""
+    P = (1 << 61) - 1
+    rng = np.random.default_rng(nv)
+    f, g = rand_fp2(rng, 1 << nv), rand_fp2(rng, 1 << nv)
+    specials = np.array([P - 1, 0, 1, 0x7f7f7f7f7f7f7f7f & P, 0x0080808080808080,
+                         0x1fffffffffffffff - 1, 0x00ff00ff00ff00ff], dtype=np.uint64)
+    for arr in (f, g):
+        idx = rng.integers(0, arr.size, size=arr.size // 3)
+        arr[idx] = specials[rng.integers(0, specials.size, size=idx.size)]
+    f[: 1 << 10] = P - 1
+    g[: 1 << 10] = P - 1
+    want = oracle.sc2_prove(f, g, transcript_chal(11))
+    got = sc.prove_deg2_host(f, g, seed=11)
+    assert np.array_equal(got.polys, want[0]) and np.array_equal(got.final, want[2])

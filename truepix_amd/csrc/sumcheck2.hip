@@ -279,6 +279,30 @@ __device__ __forceinline__ void ticket_publish(fp2 (&v)[N], const TicketPub& a)
     }
 }
 
+// ticket_publish for a workgroup whose N totals thread 0 holds already.
+template <int N>
+__device__ __forceinline__ void ticket_publish_wg(const fp2 (&v)[N], const TicketPub& a)
+{
+    __shared__ fp2 s_tot[N];
+    const unsigned nb = gridDim.x;
+    const u64 tag = (u64)(a.seq & 7) << 61;
+    if (!slot_and_ticket<N>(v, a.slots + (size_t)blockIdx.x * 2 * N, a.ticket, nb, tag)) return;
+    const int ok = gather_slots<N>(a.slots, nb, tag, a.spin_ticks, s_tot);
+    const unsigned t = threadIdx.x;
+    if (a.relay && t < 4)
+        __hip_atomic_store(a.relay + t, kSlotEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t == 0) {
+        __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        fp2 tot[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) tot[q] = s_tot[q];
+        if (ok)
+            publish_tagged<N>(a.host_vals, tot, a.seq);
+        else
+            __hip_atomic_store(a.host_seq, kSeqAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 // The totals of N lazy sums published by ticket_publish.
 template <int N>
 __device__ __forceinline__ void lazy_publish(const LazyProdSum (&z)[N], const TicketPub& a)
@@ -391,31 +415,252 @@ __global__ void __launch_bounds__(kFirst01Threads) k_sc2_first01(const V2* __res
     lazy_publish<9>(z, pub);
 }
 
-// Round 2 from the input: fold T_0 by r_0 and r_1 into T_2 and sum the
-// round-2 pairs.  Lane j owns output j (inputs 4j..4j+3, 64 contiguous bytes
-// per table, as the streaming rounds read) and stores it (16 B, contiguous
-// across the wave); output pairs (2p, 2p + 1) sit in adjacent lanes, and the
-// even lane takes its neighbour's outputs by a shuffle for the pair sums.
-// With a lane owning eight inputs instead, one load instruction touched 64
-// lines 128 B apart and the L2 dropped lines before their other loads came:
-// 1.23x the algorithmic reads.
-__device__ __forceinline__ fp2 fold_quad01(const V2* p, fp2 r0, fp2 r1, fp2 r01)
+// Rounds 0 and 1 by int8 matrix cores.  The nine sums are exact integer
+// bilinear forms over the quads before any reduction mod p, so they can be
+// taken on MFMA with the bytes of each value as its limbs:
+//   sum_q x_q y_q = sum_{i,j} 2^{8(i+j)} sum_q x_q[i] y_q[j]
+// (x_q[i]: byte i of the 64-bit word), and the inner sums over the quads are
+// the K dimension of v_mfma_i32_16x16x64_i8.  Per wave and chunk of 64
+// quads, for every pair (a, b) of quad entries (f_a, g_b, 16 bytes each: re
+// then im) one MFMA adds the 16 x 16 byte products summed over the chunk:
+//   A[i][k] = byte i of f_a of quad k,  B[k][j] = byte j of g_b of quad k.
+// MFMA operands hold 16 consecutive k of one row per lane, while a load
+// gives a lane 16 consecutive bytes of one quad; ds_read_b64_tr_b8 turns one
+// into the other: within 16 lanes, lane 8h + j receives byte j of the eight
+// 8-byte words that lanes 2b + h address (b = 0..7), so pointing lanes 2b and
+// 2b + 1 at the re and im words of one quad gives a lane its row for eight
+// quads per read.  A and B are read with the same lane pattern, so their k
+// pair up whatever the hardware's k order (checked on the device: C[i][j] =
+// sum over (lane group, byte) of A[16G + i][.] B[16G + j][.], lane l holding
+// C[4 (l / 16) + r][l % 16]).  The MFMA takes signed bytes, so every byte is
+// stored as u = b - 128 (b xor 0x80) and the offset restored at the end:
+//   sum_q X Y = sum_q X_u Y_u + 128 K1 (sum_q Y_u + sum_q X_u) + 128^2 K1^2 N
+// per pair of words, K1 = sum_{i<8} 2^{8i}, N the quads, X_u = sum_i u_i 2^{8i}
+// (sum_q X_u from the bytes' column sums, v_dot4 against ones).  The 16
+// integer tiles of a workgroup are added in LDS (|entry| < 2^31 for up to
+// 2^24 quads), weighted by 2^{8(i+j)} mod p, and combined into the nine
+// sums: S00 = P00, SAA = P11 - P10 - P01 + P00, ..., X2 = sum of s_a s_b P_ab
+// with s = (1, -1, -1, 1), P_ab = sum_q f_a g_b in Fp2.  Loads are coalesced
+// (a wave instruction covers 1 KB of the chunk) and non-temporal, and each
+// wave has the next chunk's loads in flight while it stages, transposes and
+// multiplies the current one.  About 100 VALU instructions per quad and
+// lane instead of the VALU pass's 690.  The same kernel with temporal loads
+// ran 129 us at 2 x 2^24 against 100 us non-temporal; round 2's pass after
+// it then ran 117 rather than 105 us (less of the tables left in the
+// memory-side cache), 217 against 234 us for the two.
+constexpr int kMfThreads = 768;                    // 12 waves: ~156 VGPRs with the prefetch
+constexpr int kMfWaves = kMfThreads / 64;
+constexpr unsigned kMfGroups = 256;
+constexpr int kMfStage = 8192;                     // LDS per wave: a chunk of f, then of g
+constexpr size_t kMfLds = (size_t)kMfWaves * kMfStage;
+constexpr size_t kMfMaxQuads = (size_t)1 << 24;    // int32 tile sums stay exact up to here
+typedef int mf_v4i __attribute__((ext_vector_type(4)));
+typedef int mf_v2i __attribute__((ext_vector_type(2)));
+typedef unsigned mf_v4u __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ mf_v2i lds_tr8(const uint8_t* p)
 {
-    return fold_quad(ldv(p), ldv(p + 1), ldv(p + 2), ldv(p + 3), r0, r1, r01);
+    return __builtin_amdgcn_ds_read_tr8_b64_v2i32((__attribute__((address_space(3))) mf_v2i*)p);
 }
 
+// A signed int32 as its residue mod p.
+__device__ __forceinline__ u64 mod_i32(int v) { return v < 0 ? P - (u64)(-(long long)v) : (u64)v; }
+
+__global__ void __launch_bounds__(kMfThreads) k_sc2_first01_mfma(const V2* __restrict__ f,
+                                                                const V2* __restrict__ g,
+                                                                size_t n_quads, TicketPub pub)
+{
+    extern __shared__ __attribute__((aligned(16))) uint8_t mf_lds[];
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    uint8_t* stage = mf_lds + wid * kMfStage;
+    mf_v4i acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = mf_v4i{0, 0, 0, 0};
+    int ucol[4] = {0, 0, 0, 0}, vcol[4] = {0, 0, 0, 0};   // column sums of the u bytes
+    int nq = 0;
+    // transposed-read addresses: lane 16 G + m reads word m & 1 of quads
+    // 16 G + m / 2 and 16 G + 8 + m / 2
+    const int grp = lane >> 4, m = lane & 15;
+    const int r1 = 64 * (16 * grp + (m >> 1)) + (m & 1) * 8, r2 = r1 + 64 * 8;
+    const size_t n_chunks = n_quads / 64;
+    const size_t cstride = (size_t)gridDim.x * kMfWaves;
+    uint4 fv[4], gv[4];
+    // load k covers bytes [1024 k, 1024 k + 1024) of the chunk: lane l its 16
+    // bytes at 1024 k + 16 l, staged at the same offset
+    auto load = [&](size_t c) {
+        const mf_v4u* pf = reinterpret_cast<const mf_v4u*>(f) + 256 * c + lane;
+        const mf_v4u* pg = reinterpret_cast<const mf_v4u*>(g) + 256 * c + lane;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const mf_v4u x = __builtin_nontemporal_load(pf + 64 * k);
+            const mf_v4u y = __builtin_nontemporal_load(pg + 64 * k);
+            fv[k] = make_uint4(x.x, x.y, x.z, x.w);
+            gv[k] = make_uint4(y.x, y.y, y.z, y.w);
+        }
+    };
+    size_t c = (size_t)blockIdx.x * kMfWaves + wid;
+    if (c < n_chunks) load(c);
+    for (; c < n_chunks; c += cstride) {
+        uint4* sf = reinterpret_cast<uint4*>(stage + 16 * lane);
+        uint4* sg = reinterpret_cast<uint4*>(stage + 4096 + 16 * lane);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint4 x = fv[k], y = gv[k];
+            sf[64 * k] = make_uint4(x.x ^ 0x80808080u, x.y ^ 0x80808080u, x.z ^ 0x80808080u, x.w ^ 0x80808080u);
+            sg[64 * k] = make_uint4(y.x ^ 0x80808080u, y.y ^ 0x80808080u, y.z ^ 0x80808080u, y.w ^ 0x80808080u);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (c + cstride < n_chunks) load(c + cstride);   // in flight while this chunk runs
+        __builtin_amdgcn_sched_barrier(0);
+        mf_v4i A[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const mf_v2i lo = lds_tr8(stage + r1 + 16 * a), hi = lds_tr8(stage + r2 + 16 * a);
+            A[a] = mf_v4i{lo.x, lo.y, hi.x, hi.y};
+#pragma unroll
+            for (int w = 0; w < 4; ++w) ucol[a] = __builtin_amdgcn_sdot4(A[a][w], 0x01010101, ucol[a], false);
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const mf_v2i lo = lds_tr8(stage + 4096 + r1 + 16 * b), hi = lds_tr8(stage + 4096 + r2 + 16 * b);
+            const mf_v4i Bb = mf_v4i{lo.x, lo.y, hi.x, hi.y};
+#pragma unroll
+            for (int w = 0; w < 4; ++w) vcol[b] = __builtin_amdgcn_sdot4(Bb[w], 0x01010101, vcol[b], false);
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+                acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[a], Bb, acc[a][b], 0, 0, 0);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // reads done before the next stage
+        nq += 1;
+    }
+    // ---- the workgroup's integer tiles and column sums, added in LDS
+    __syncthreads();
+    int* Ts = reinterpret_cast<int*>(mf_lds);      // [16 (a, b)][16 i][16 j]
+    int* Us = Ts + 4096;                           // [4 a][16 i]
+    int* Vs = Us + 64;                             // [4 b][16 j]
+    int* Ns = Vs + 64;                             // chunks
+    u64* Qs = reinterpret_cast<u64*>(Ns + 4);      // [16 (a, b)][4 word pairs]
+    u64* Ws = Qs + 64;                             // [4 a][2] weighted U, then [4 b][2] V
+    for (int i = t; i < 4096 + 64 + 64 + 4; i += kMfThreads) Ts[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                atomicAdd(&Ts[(a * 4 + b) * 256 + (4 * (lane >> 4) + r) * 16 + (lane & 15)], acc[a][b][r]);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        atomicAdd(&Us[a * 16 + (lane & 15)], ucol[a]);
+        atomicAdd(&Vs[a * 16 + (lane & 15)], vcol[a]);
+    }
+    if (lane == 0) atomicAdd(Ns, nq);
+    __syncthreads();
+    // a wave per tile (a, b); a row of 16 lanes per word pair (hf, hg); a lane
+    // adds 4 of the pair's 64 (i, j) entries weighted by 2^{8(i+j)} mod p
+    for (int tile = wid; tile < 16; tile += kMfWaves) {
+        const int qd = lane >> 4, hf = qd >> 1, hg = qd & 1, sub = lane & 15;
+        const int bi = sub >> 1, bj0 = (sub & 1) * 4;
+        u64 sacc = 0;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const int bj = bj0 + jj;
+            sacc += fmul_lazy(mod_i32(Ts[tile * 256 + (8 * hf + bi) * 16 + (8 * hg + bj)]),
+                              1ull << ((8 * (bi + bj)) % 61));
+        }
+        u64 w[1] = {fold1(sacc)};
+        dpp_add_step<0xb1, 0xf>(w);    // the 16 lanes of the row
+        dpp_add_step<0x4e, 0xf>(w);
+        fold_words(w);
+        dpp_add_step<0x141, 0xf>(w);
+        dpp_add_step<0x140, 0xf>(w);
+        if (sub == 0) Qs[tile * 4 + qd] = canon(fold1(w[0]));
+    }
+    if (t < 16) {
+        // t < 8: U of entry t / 2, word t % 2; then V: sum_i 2^{8i} U[8 h + i]
+        const int* src = (t < 8 ? Us : Vs) + ((t >> 1) & 3) * 16 + 8 * (t & 1);
+        u64 wsum = 0;
+        for (int i = 0; i < 8; ++i) wsum += fmul_lazy(mod_i32(src[i]), 1ull << (8 * i));
+        Ws[t] = fold61(wsum);
+    }
+    __syncthreads();
+    fp2 e[9];
+    if (t == 0) {
+        u64 k1 = 0;
+        for (int i = 0; i < 8; ++i) k1 = fadd(k1, 1ull << (8 * i));
+        const u64 c128 = fmul(128, k1);                                // 128 K1
+        const u64 cnn = fmul(fmul(c128, c128), (u64)*Ns * 64);         // 128^2 K1^2 N
+        fp2 Pm[4][4];
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) {
+                u64 Q[2][2];
+                for (int hf = 0; hf < 2; ++hf)
+                    for (int hg = 0; hg < 2; ++hg)
+                        Q[hf][hg] = fadd(fadd(Qs[(a * 4 + b) * 4 + hf * 2 + hg],
+                                              fmul(c128, fadd(Ws[a * 2 + hf], Ws[8 + b * 2 + hg]))),
+                                         cnn);
+                Pm[a][b] = mk2(fsub(Q[0][0], Q[1][1]), fadd(Q[0][1], Q[1][0]));
+            }
+        // sum of sa[a] sb[b] P_ab
+        auto comb = [&](const int (&sa)[4], const int (&sb)[4]) {
+            fp2 r = zero2();
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) {
+                    const int sg = sa[a] * sb[b];
+                    if (sg > 0) r = add2(r, Pm[a][b]);
+                    else if (sg < 0) r = sub2(r, Pm[a][b]);
+                }
+            return r;
+        };
+        const int e0[4] = {1, 0, 0, 0}, e1[4] = {0, 1, 0, 0}, e2[4] = {0, 0, 1, 0}, e3[4] = {0, 0, 0, 1};
+        const int dA[4] = {-1, 1, 0, 0}, dB[4] = {0, 0, -1, 1}, x0[4] = {-1, 0, 1, 0},
+                  x1[4] = {0, -1, 0, 1}, x2[4] = {1, -1, -1, 1};
+        e[0] = comb(e0, e0); e[1] = comb(e1, e1); e[2] = comb(dA, dA);
+        e[3] = comb(e2, e2); e[4] = comb(e3, e3); e[5] = comb(dB, dB);
+        e[6] = comb(x0, x0); e[7] = comb(x1, x1); e[8] = comb(x2, x2);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) e[i] = zero2();
+    }
+    ticket_publish_wg<9>(e, pub);
+}
+
+// Round 2 from the input: fold T_0 by r_0 and r_1 into T_2 and sum the
+// round-2 pairs.  A wave takes a chunk of 64 outputs (256 inputs, 4 KB per
+// table): it reads each table's chunk with four coalesced non-temporal
+// 16-byte loads per lane (1 KB a load instruction), stages it in its own
+// LDS slice and reads back the quad of the output its lane owns; outputs
+// are stored contiguous across the wave.  With each lane loading its own
+// 64 contiguous bytes instead, one load instruction touched 64 lines 64 B
+// apart and the kernel ran at 5.2 TB/s from a cold cache, against 5.7 TB/s
+// staged (which is also what a staged kernel without the arithmetic
+// reaches).  Output pairs (2p, 2p + 1) sit in adjacent lanes, and each
+// lane takes its neighbour's g by a shuffle for the pair sums.
+constexpr int kFold01Waves = 8;
 __device__ __forceinline__ fp2 shfl_xor1(fp2 x)
 {
     return mk2(__shfl_xor(x.re, 1, 64), __shfl_xor(x.im, 1, 64));
 }
 
 // The streaming launches run 512-thread workgroups, at most kFold01Groups /
-// kFoldGroups of them (as many as are resident at once).  Grids of 2048 to
-// 8192 workgroups for round 2 measured the same or slower: with fewer
-// outputs per thread, each wave's closing reduction costs more per output.
+// kFoldGroups of them.  Round 2 measured the same at 2048 workgroups (116.5
+// against 118.0 us) and the ticket publish takes at most kMaxPubGroups;
+// for the streaming rounds grids of 2048 to 8192 workgroups measured the
+// same or slower: with fewer outputs per thread, each wave's closing
+// reduction costs more per output.
 constexpr int kFoldThreads = 512;
-constexpr unsigned kFold01Groups = 1024;   // 58 VGPRs: 4 workgroups per CU
+constexpr unsigned kFold01Groups = 1024;   // 82 VGPRs, 32 KB LDS
 constexpr unsigned kFoldGroups = 768;      // 76 VGPRs: 3 workgroups per CU
+
+// Round 2's grid: a wave per 64-output chunk, at most kFold01Groups.
+inline unsigned fold01_groups(size_t n_out)
+{
+    const size_t chunks = (n_out + 63) / 64;
+    return (unsigned)std::min<size_t>(kFold01Groups, (chunks + kFold01Waves - 1) / kFold01Waves);
+}
 
 // The pair sums of T_2 from two products per lane: the lane holding output j
 // adds f_j g_j (E0 over the even outputs, E1 over the odd) and
@@ -430,30 +675,63 @@ __global__ void __launch_bounds__(kFoldThreads) k_sc2_fold01_round(const V2* __r
                                                           V2* __restrict__ fout,
                                                           V2* __restrict__ gout, TicketPub pub)
 {
+    static_assert(kFoldThreads == 64 * kFold01Waves, "one LDS slice per wave");
+    __shared__ __attribute__((aligned(16))) V2 stage[kFold01Waves][256];
     u64 rw[4] = {r0r, r0i, r1r, r1i};
     if (!launch_challenges(pub, rw)) return;
     const fp2 r0 = mk2(rw[0], rw[1]), r1 = mk2(rw[2], rw[3]);
     const fp2 r01 = uniform2(mul2(r0, r1));
-    // n_out is even and the stride a multiple of 64, so both lanes of a pair
-    // run every iteration together.  The sweep runs from the top down: the
-    // first pass read the tables bottom up, so their top part is still in
-    // the memory-side cache.  Lane order is reversed with it, so even lanes
-    // hold odd outputs.
-    const bool odd_out = (threadIdx.x & 1) == 0;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    mf_v4u* s_st = reinterpret_cast<mf_v4u*>(stage[wid]) + lane;   // this lane's staging stores
+    const V2* s_ld = stage[wid] + 4 * lane;                        // and its quad
+    // n_out is a power of two: either a multiple of 64 or one short chunk
+    // of an even length m.  In a short chunk the loads past its 4m inputs
+    // repeat its last one (in bounds), and only lanes below m, both lanes of
+    // each pair, store and add.  The sweep runs from the top chunk down: the
+    // first pass read the tables bottom up, so their top part is the
+    // likeliest to be still in the memory-side cache.
+    const size_t n_chunks = (n_out + 63) / 64;
+    const unsigned m = n_out < 64 ? (unsigned)n_out : 64u;
     LazyProdSum z[2];   // f g, f (g - g')
     lazy_zero(z);
-    for (size_t j0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j0 < n_out;
-         j0 += (size_t)gridDim.x * blockDim.x) {
-        const size_t j = n_out - 1 - j0;
-        const fp2 f = fold_quad01(fin + 4 * j, r0, r1, r01);
-        const fp2 g = fold_quad01(gin + 4 * j, r0, r1, r01);
-        stv(fout + j, f);
-        stv(gout + j, g);
+    for (size_t c0 = (size_t)blockIdx.x * kFold01Waves + wid; c0 < n_chunks;
+         c0 += (size_t)gridDim.x * kFold01Waves) {
+        const size_t c = n_chunks - 1 - c0;
+        const mf_v4u* pf = reinterpret_cast<const mf_v4u*>(fin + 256 * c);
+        const mf_v4u* pg = reinterpret_cast<const mf_v4u*>(gin + 256 * c);
+        mf_v4u xf[4], xg[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const unsigned i = min((unsigned)(lane + 64 * k), 4 * m - 1);
+            xf[k] = __builtin_nontemporal_load(pf + i);
+            xg[k] = __builtin_nontemporal_load(pg + i);
+        }
+        // one slice per wave: f staged and its quads read back, then g; the
+        // wave-scope fences keep the compiler from moving an LDS store
+        // across the reads of the data it replaces
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s_st[64 * k] = xf[k];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const fp2 f0 = ldv(s_ld), f1 = ldv(s_ld + 1), f2 = ldv(s_ld + 2), f3 = ldv(s_ld + 3);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s_st[64 * k] = xg[k];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const fp2 g0 = ldv(s_ld), g1 = ldv(s_ld + 1), g2 = ldv(s_ld + 2), g3 = ldv(s_ld + 3);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const fp2 f = fold_quad(f0, f1, f2, f3, r0, r1, r01);
+        const fp2 g = fold_quad(g0, g1, g2, g3, r0, r1, r01);
         const fp2 go = shfl_xor1(g);
-        z[0].mac(f, g);
-        z[1].mac(f, sub2(g, go));
+        if ((unsigned)lane < m) {
+            const size_t j = 64 * c + lane;
+            stv(fout + j, f);
+            stv(gout + j, g);
+            z[0].mac(f, g);
+            z[1].mac(f, sub2(g, go));
+        }
     }
     const fp2 self = z[0].value();
+    const bool odd_out = (lane & 1) != 0;
     fp2 e[3] = {odd_out ? zero2() : self, odd_out ? self : zero2(), z[1].value()};
     ticket_publish<3>(e, pub);
 }
@@ -1072,6 +1350,8 @@ bool sc2_setup(size_t n)
         S.dev_trace = reinterpret_cast<long long*>(dc + 1024);
         TP_HIP_OK(hipFuncSetAttribute((const void*)k_sc2_tail_host,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLds));
+        TP_HIP_OK(hipFuncSetAttribute((const void*)k_sc2_first01_mfma,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMfLds));
         TP_HIP_OK(hipFuncSetAttribute((const void*)k_sc2_mid,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLds));
         void* tp_ = nullptr;
@@ -1266,6 +1546,22 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         a.relay = S.arelay.as<u64>();
         return a;
     };
+    // Rounds 0 and 1 from one pass over the input: on the matrix cores when
+    // the quads come in whole chunks of 64 and their tile sums fit int32,
+    // else by the VALU pass.
+    auto launch_first01 = [&](unsigned seq) {
+        const V2* cf = reinterpret_cast<const V2*>(d_f);
+        const V2* cg = reinterpret_cast<const V2*>(d_g);
+        const size_t nq = n / 4;
+        if (nq % 64 == 0 && nq >= (size_t)64 * kMfWaves && nq <= kMfMaxQuads) {
+            const unsigned nbm = (unsigned)std::min<size_t>(kMfGroups, nq / 64 / kMfWaves);
+            k_sc2_first01_mfma<<<nbm, kMfThreads, kMfLds, st>>>(cf, cg, nq, pub_for(seq));
+        } else {
+            const unsigned nb1 = (unsigned)std::min<size_t>(
+                kFirst01Groups, (nq + kFirst01Threads - 1) / kFirst01Threads);
+            k_sc2_first01<<<nb1, kFirst01Threads, 0, st>>>(cf, cg, nq, pub_for(seq));
+        }
+    };
     // The host's answer for the launch waiting on sequence number `want`:
     // n Fp2 challenges as tagged words (the tags written with the values, so
     // a launch never takes a half-written answer for this one).
@@ -1398,13 +1694,11 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         // rounds 0 and 1 from one pass, round 2 folds the input by r_0, r_1,
         // then the streaming rounds and the mid launch, all queued now
         const unsigned seq_a = ++S.seq, seq_b = ++S.seq;
-        const unsigned nb1 = (unsigned)std::min<size_t>(
-            kFirst01Groups, (n / 4 + kFirst01Threads - 1) / kFirst01Threads);
-        k_sc2_first01<<<nb1, kFirst01Threads, 0, st>>>(cf, cg, n / 4, pub_for(seq_a));
+        launch_first01(seq_a);
         TP_HIP_OK(hipGetLastError());
         S.bytes[SC_FIRST] += 32.0 * (double)n;
         S.launches[SC_FIRST] += 1;
-        const unsigned nb2 = (unsigned)std::min<size_t>(kFold01Groups, (n / 4 + kFoldThreads - 1) / kFoldThreads);
+        const unsigned nb2 = fold01_groups(n / 4);
         k_sc2_fold01_round<<<nb2, kFoldThreads, 0, st>>>(cf, cg, n / 4, 0, 0, 0, 0, B, B + capB,
                                                          pub_for(seq_b, seq_a, 4));
         TP_HIP_OK(hipGetLastError());
@@ -1532,9 +1826,9 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
     } else if (nv >= 3) {
         // rounds 0 and 1 from one pass, round 2 folds the input by r_0, r_1
         unsigned seq = ++S.seq;
-        unsigned nb = (unsigned)std::min<size_t>(kFirst01Groups, (n / 4 + kFirst01Threads - 1) / kFirst01Threads);
+        unsigned nb = 0;
         if (!t_begin()) return -1;
-        k_sc2_first01<<<nb, kFirst01Threads, 0, st>>>(cf, cg, n / 4, pub_for(seq));
+        launch_first01(seq);
         if (!t_end(SC_FIRST, 32.0 * (double)n)) return -1;
         TP_HIP_OK(hipGetLastError());
         if (wait_pub(st, seq, 9, false) != 1 || !t_collect(SC_FIRST)) return -1;
@@ -1547,7 +1841,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         seq = ++S.seq;
         V2* of = B;
         V2* og = B + capB;
-        nb = (unsigned)std::min<size_t>(kFold01Groups, (n / 4 + kFoldThreads - 1) / kFoldThreads);
+        nb = fold01_groups(n / 4);
         if (!t_begin()) return -1;
         k_sc2_fold01_round<<<nb, kFoldThreads, 0, st>>>(cf, cg, n / 4, r0.re, r0.im, r1.re, r1.im,
                                                         of, og, pub_for(seq));
