@@ -58,7 +58,7 @@ def alg_bytes(nv: int) -> float:
     return 48.0 * ((1 << (nv + 1)) - 2)
 
 
-SC2_KERNELS = {"first": "k_sc2_first01_mfma", "fold01": "k_sc2_fold01_round",
+SC2_KERNELS = {"first": "k_sc2_first01_mfma", "fold01": "k_sc2_fold2_quads", "pair": "k_sc2_fold2_pairs_cached",
                "fold": "k_sc2_fold_round", "tail": "k_sc2_tail_host", "mid": "k_sc2_mid"}
 
 

@@ -148,10 +148,12 @@ int tp_sc2_mid_fallbacks(void);
 void tp_sc2_schedule(int* mid_groups, int* tail_in, int* mid_end);
 /* Per-class statistics of the last tp_sc2_prove (timing on): for class c in
  * 0 = the first pass over the input (rounds 0 and 1), 1 = round 2 folding
- * the input by r_0 and r_1, 2 = the streaming fold+round launches, 3 = the
- * one-workgroup tail rounds, 4 = the rounds on per-workgroup slices:
- * out[3c] summed event ms, out[3c+1] bytes the launches read and write,
- * out[3c+2] launch count.  n >= 15. */
+ * the input by r_0 and r_1 (with round 3's sums too on inputs above 2^22
+ * entries), 2 = the streaming fold+round launches, 3 = the one-workgroup
+ * tail rounds, 4 = the rounds on per-workgroup slices, 5 = the later passes
+ * folding by two challenges: out[3c] summed event ms, out[3c+1] bytes the
+ * launches read and write, out[3c+2] launch count.  Fills the classes that
+ * fit in n (all six for n >= 18). */
 void tp_sc2_class_stats(double* out, int n);
 /* Per-exchange latency of the one-workgroup tail of the last timed
  * tp_sc2_prove (an exchange: one host round trip for one or two rounds):

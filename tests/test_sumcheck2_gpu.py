@@ -70,13 +70,16 @@ def test_sc2_full_size(bridge, oracle, torch_cuda):
 
 
 def test_sc2_launch_classes(bridge, torch_cuda):
-    """Rounds 0 and 1 share one pass over the input, round 2 folds the input
-    straight into T_2, the streaming launches run while the input exceeds
-    mid_groups x tail_in entries, the sliced launch takes the rounds down to
-    mid_end entries (when the input exceeds tail_in), and one workgroup proves
-    the rest: per class the launches and the bytes they move."""
+    """Rounds 0 and 1 share one pass over the input; from round 2, passes fold
+    by two challenges (the first reads the input), each publishing the nine
+    quad sums of its output while that output exceeds mid_groups x tail_in
+    entries and the pair sums after that; streaming launches run while the
+    input still exceeds mid_groups x tail_in, the sliced launch takes the
+    rounds down to mid_end entries (when the input exceeds tail_in), and one
+    workgroup proves the rest: per class the launches and the bytes they
+    move."""
     groups, tail_in, mid_end = sc.schedule()
-    for nv in (16, 22):
+    for nv in (16, 22, 24, 25):
         rng = np.random.default_rng(nv)
         f, g = rand_fp2(rng, 1 << nv), rand_fp2(rng, 1 << nv)
         sc.set_timing(True)
@@ -87,20 +90,30 @@ def test_sc2_launch_classes(bridge, torch_cuda):
             sc.set_timing(False)
         n = 1 << nv
         assert cls["first"]["launches"] == 1 and cls["first"]["bytes"] == 32 * n
+        passes = [n]   # input entries of each two-round pass
+        while passes[-1] // 4 > groups * tail_in:
+            passes.append(passes[-1] // 4)
         assert cls["fold01"]["launches"] == 1 and cls["fold01"]["bytes"] == 40 * n
-        big = [k for k in range(3, nv) if (n >> (k - 1)) > groups * tail_in]
+        assert cls["pair"]["launches"] == len(passes) - 1
+        assert cls["pair"]["bytes"] == sum(40 * x for x in passes[1:])
+        length = passes[-1] // 4   # the streaming rounds' first input
+        big = []
+        while length > groups * tail_in:
+            big.append(length)
+            length //= 2
         assert cls["fold"]["launches"] == len(big)
-        assert cls["fold"]["bytes"] == sum(48 * (n >> (k - 1)) for k in big)
-        k_mid = big[-1] + 1 if big else 3
-        if (n >> (k_mid - 1)) > tail_in:
+        assert cls["fold"]["bytes"] == sum(48 * x for x in big)
+        if length > tail_in:
             assert cls["mid"]["launches"] == 1
-            assert cls["mid"]["bytes"] == 32 * (n >> (k_mid - 1)) + 64 * groups
+            assert cls["mid"]["bytes"] == 32 * length + 64 * groups
             tail_bytes = 32 * mid_end
         else:
             assert cls["mid"]["launches"] == 0
-            tail_bytes = 32 * (n >> (k_mid - 1))
+            tail_bytes = 32 * length
         assert cls["tail"]["launches"] == 1 and cls["tail"]["bytes"] == tail_bytes
         assert all(c["ms"] > 0 for c in cls.values() if c["launches"])
+        if nv == 24:   # the benchmark shape: one pass with quad sums, one with pair sums
+            assert len(passes) == 2 and not big and cls["mid"]["launches"] == 1
 
 
 def test_sc2_abort_then_recover(bridge, oracle, torch_cuda):
@@ -124,11 +137,12 @@ def test_sc2_abort_then_recover(bridge, oracle, torch_cuda):
     assert np.array_equal(got.final, want[2])
 
 
-def test_sc2_mid_gate_fallback(bridge, oracle, torch_cuda):
+@pytest.mark.parametrize("nv", [20, 24])
+def test_sc2_mid_gate_fallback(bridge, oracle, torch_cuda, nv):
     """The sliced mid launch needs all its workgroups resident; when its
     arrival gate closes before they are (forced here with a zero bound), the
-    rounds go to streaming launches and the proof is unchanged."""
-    nv = 20
+    rounds go to streaming launches and the proof is unchanged (at 2^24
+    after the two-round passes, reading the buffer the last pass wrote)."""
     rng = np.random.default_rng(77)
     f, g = rand_fp2(rng, 1 << nv), rand_fp2(rng, 1 << nv)
     want = oracle.sc2_prove(f, g, transcript_chal(77), nthreads=8)
@@ -185,3 +199,24 @@ def test_sc2_edge_values_matrix_core_pass(bridge, oracle, torch_cuda, nv):
     want = oracle.sc2_prove(f, g, transcript_chal(11))
     got = sc.prove_deg2_host(f, g, seed=11)
     assert np.array_equal(got.polys, want[0]) and np.array_equal(got.final, want[2])
+
+
+@pytest.mark.parametrize("nv", [23, 25])
+def test_sc2_two_round_passes(bridge, oracle, torch_cuda, nv):
+    """Inputs above 2^22 entries: round 2's pass publishes the nine quad sums
+    of T_2 (rounds 2 and 3) and the passes after it fold by two challenges,
+    with quad sums again while their output exceeds the sliced rounds' input
+    (2^25: 2^23 -> 2^21 entries) and pair sums last; extreme values mixed
+    in, bit-exact against the oracle."""
+    P = (1 << 61) - 1
+    rng = np.random.default_rng(300 + nv)
+    f, g = rand_fp2(rng, 1 << nv), rand_fp2(rng, 1 << nv)
+    specials = np.array([P - 1, 0, 1, 0x00ff00ff00ff00ff], dtype=np.uint64)
+    for arr in (f, g):
+        idx = rng.integers(0, arr.size, size=arr.size // 5)
+        arr[idx] = specials[rng.integers(0, specials.size, size=idx.size)]
+    want = oracle.sc2_prove(f, g, transcript_chal(nv), nthreads=8)
+    got = sc.prove_deg2_host(f, g, seed=nv)
+    assert np.array_equal(got.polys, want[0])
+    assert np.array_equal(got.chals, want[1])
+    assert np.array_equal(got.final, want[2])

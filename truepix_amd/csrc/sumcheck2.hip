@@ -628,21 +628,28 @@ __global__ void __launch_bounds__(kMfThreads) k_sc2_first01_mfma(const V2* __res
     ticket_publish_wg<9>(e, pub);
 }
 
-// Round 2 from the input: fold T_0 by r_0 and r_1 into T_2 and sum the
-// round-2 pairs.  A wave takes a chunk of 64 outputs (256 inputs, 4 KB per
-// table): it reads each table's chunk with four coalesced non-temporal
-// 16-byte loads per lane (1 KB a load instruction), stages it in its own
-// LDS slice and reads back the quad of the output its lane owns; outputs
-// are stored contiguous across the wave.  With each lane loading its own
-// 64 contiguous bytes instead, one load instruction touched 64 lines 64 B
-// apart and the kernel ran at 5.2 TB/s from a cold cache, against 5.7 TB/s
-// staged (which is also what a staged kernel without the arithmetic
-// reaches).  Output pairs (2p, 2p + 1) sit in adjacent lanes, and each
-// lane takes its neighbour's g by a shuffle for the pair sums.
+// Two rounds of folding in one pass: T_{k-2} (4 n_out entries per table)
+// folded by r_{k-2} and r_{k-1} into T_k, with T_k's round sums (round 2
+// from the input, and the later passes of the streaming rounds).  A wave
+// takes a chunk of 64 outputs (256 inputs, 4 KB per table): it reads each
+// table's chunk with four coalesced non-temporal 16-byte loads per lane
+// (1 KB a load instruction), stages it in its own LDS slice and reads back
+// the quad of the output its lane owns; outputs are stored contiguous
+// across the wave.  With each lane loading its own 64 contiguous bytes
+// instead, one load instruction touched 64 lines 64 B apart and the kernel
+// ran at 5.2 TB/s from a cold cache, against 5.7 TB/s staged (which is also
+// what a staged kernel without the arithmetic reaches).
 constexpr int kFold01Waves = 8;
 __device__ __forceinline__ fp2 shfl_xor1(fp2 x)
 {
     return mk2(__shfl_xor(x.re, 1, 64), __shfl_xor(x.im, 1, 64));
+}
+
+// x from lane (quad_perm CTRL) of this lane's group of four.
+template <int CTRL>
+__device__ __forceinline__ fp2 quad_pick(fp2 x)
+{
+    return mk2(dpp_u64<CTRL, 0xf>(x.re), dpp_u64<CTRL, 0xf>(x.im));
 }
 
 // The streaming launches run 512-thread workgroups, at most kFold01Groups /
@@ -652,30 +659,48 @@ __device__ __forceinline__ fp2 shfl_xor1(fp2 x)
 // same or slower: with fewer outputs per thread, each wave's closing
 // reduction costs more per output.
 constexpr int kFoldThreads = 512;
-constexpr unsigned kFold01Groups = 1024;   // 82 VGPRs, 32 KB LDS
+constexpr unsigned kFold01Groups = 1024;
+// The quad-sum pass publishes nine sums a workgroup: 512 workgroups, as many
+// as are resident at once (two per CU at 128 VGPRs).
+constexpr unsigned kFold2QuadGroups = 512;
 constexpr unsigned kFoldGroups = 768;      // 76 VGPRs: 3 workgroups per CU
 
-// Round 2's grid: a wave per 64-output chunk, at most kFold01Groups.
-inline unsigned fold01_groups(size_t n_out)
+// The two-round pass's grid: a wave per 64-output chunk, at most `cap`
+// workgroups (kFold01Groups with pair sums, kFold2QuadGroups with quad sums).
+inline unsigned fold01_groups(size_t n_out, unsigned cap = kFold01Groups)
 {
     const size_t chunks = (n_out + 63) / 64;
-    return (unsigned)std::min<size_t>(kFold01Groups, (chunks + kFold01Waves - 1) / kFold01Waves);
+    return (unsigned)std::min<size_t>(cap, (chunks + kFold01Waves - 1) / kFold01Waves);
 }
 
-// The pair sums of T_2 from two products per lane: the lane holding output j
-// adds f_j g_j (E0 over the even outputs, E1 over the odd) and
-// f_j (g_j - g_j') with j' its pair neighbour, whose sum over both lanes of
-// a pair is f0 g0 + f1 g1 - f0 g1 - f1 g0 = (f1 - f0)(g1 - g0) (EA): every
-// lane busy, where one lane of each pair adding all three left its
-// neighbour idle.
-__global__ void __launch_bounds__(kFoldThreads) k_sc2_fold01_round(const V2* __restrict__ fin,
-                                                          const V2* __restrict__ gin,
-                                                          size_t n_out, u64 r0r, u64 r0i,
-                                                          u64 r1r, u64 r1i,
-                                                          V2* __restrict__ fout,
-                                                          V2* __restrict__ gout, TicketPub pub)
+// The sums of T_k:
+// - NS = 3, round k's pair sums from two products per lane: the lane
+//   holding output j adds f_j g_j (E0 over the even outputs, E1 over the
+//   odd) and f_j (g_j - g_j') with j' its pair neighbour, whose sum over both
+//   lanes of a pair is f0 g0 + f1 g1 - f0 g1 - f1 g0 = (f1 - f0)(g1 - g0)
+//   (EA): every lane busy, where one lane of each pair adding all three left
+//   its neighbour idle.
+// - NS = 9, the nine quad sums (rounds k and k + 1, the second as
+//   quadratics in r_k), from three products per lane: the four lanes of a
+//   quad take their neighbours' values by DPP quad permutes; lane i of the
+//   quad adds f_i g_i (S00, S11, S22, S33), then one difference product
+//   (lane 0: SAA from p1 - p0, lane 1: X1 from p3 - p1, lane 2: X0 from
+//   p2 - p0, lane 3: SBB from p3 - p2), and lane 0 also X2 from
+//   (p3 - p2) - (p1 - p0) (the others add a product that is dropped).  Each
+//   wave's sums meet in lanes 0-3 of each 16-lane row by two row rotations,
+//   the workgroup's in LDS; needs n_out a multiple of 64.
+// The sweep runs over the chunks from the top down when `desc`, else from
+// the bottom up: a pass starts where the pass before it ended, whose reads
+// or writes there are the likeliest to be still in the memory-side cache.
+// NTL: non-temporal loads (an input read once, from HBM) or cached ones.
+template <int NS, bool NTL = true>
+__device__ __forceinline__ void fold2_body(const V2* __restrict__ fin, const V2* __restrict__ gin,
+                                           size_t n_out, u64 r0r, u64 r0i, u64 r1r, u64 r1i,
+                                           V2* __restrict__ fout, V2* __restrict__ gout,
+                                           const TicketPub& pub, int desc)
 {
     static_assert(kFoldThreads == 64 * kFold01Waves, "one LDS slice per wave");
+    static_assert(NS == 3 || NS == 9, "pair or quad sums");
     __shared__ __attribute__((aligned(16))) V2 stage[kFold01Waves][256];
     u64 rw[4] = {r0r, r0i, r1r, r1i};
     if (!launch_challenges(pub, rw)) return;
@@ -687,24 +712,28 @@ __global__ void __launch_bounds__(kFoldThreads) k_sc2_fold01_round(const V2* __r
     // n_out is a power of two: either a multiple of 64 or one short chunk
     // of an even length m.  In a short chunk the loads past its 4m inputs
     // repeat its last one (in bounds), and only lanes below m, both lanes of
-    // each pair, store and add.  The sweep runs from the top chunk down: the
-    // first pass read the tables bottom up, so their top part is the
-    // likeliest to be still in the memory-side cache.
+    // each pair, store and add.
     const size_t n_chunks = (n_out + 63) / 64;
     const unsigned m = n_out < 64 ? (unsigned)n_out : 64u;
-    LazyProdSum z[2];   // f g, f (g - g')
+    constexpr int NZ = NS == 9 ? 3 : 2;
+    LazyProdSum z[NZ];
     lazy_zero(z);
     for (size_t c0 = (size_t)blockIdx.x * kFold01Waves + wid; c0 < n_chunks;
          c0 += (size_t)gridDim.x * kFold01Waves) {
-        const size_t c = n_chunks - 1 - c0;
+        const size_t c = desc ? n_chunks - 1 - c0 : c0;
         const mf_v4u* pf = reinterpret_cast<const mf_v4u*>(fin + 256 * c);
         const mf_v4u* pg = reinterpret_cast<const mf_v4u*>(gin + 256 * c);
         mf_v4u xf[4], xg[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const unsigned i = min((unsigned)(lane + 64 * k), 4 * m - 1);
-            xf[k] = __builtin_nontemporal_load(pf + i);
-            xg[k] = __builtin_nontemporal_load(pg + i);
+            if (NTL) {
+                xf[k] = __builtin_nontemporal_load(pf + i);
+                xg[k] = __builtin_nontemporal_load(pg + i);
+            } else {
+                xf[k] = pf[i];
+                xg[k] = pg[i];
+            }
         }
         // one slice per wave: f staged and its quads read back, then g; the
         // wave-scope fences keep the compiler from moving an LDS store
@@ -721,19 +750,110 @@ __global__ void __launch_bounds__(kFoldThreads) k_sc2_fold01_round(const V2* __r
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const fp2 f = fold_quad(f0, f1, f2, f3, r0, r1, r01);
         const fp2 g = fold_quad(g0, g1, g2, g3, r0, r1, r01);
-        const fp2 go = shfl_xor1(g);
-        if ((unsigned)lane < m) {
+        if constexpr (NS == 3) {
+            const fp2 go = shfl_xor1(g);
+            if ((unsigned)lane < m) {
+                const size_t j = 64 * c + lane;
+                stv(fout + j, f);
+                stv(gout + j, g);
+                z[0].mac(f, g);
+                z[1].mac(f, sub2(g, go));
+            }
+        } else {
+            // difference operands: lane i of the quad takes p_hi(i) - p_lo(i)
+            // with hi = (1, 3, 2, 3) and lo = (0, 1, 0, 2); lane 0 then
+            // (p3 - p2) - (p1 - p0) from lane 3's difference and its own
+            const fp2 df = sub2(quad_pick<0xED>(f), quad_pick<0x84>(f));
+            const fp2 dg = sub2(quad_pick<0xED>(g), quad_pick<0x84>(g));
+            const fp2 xf2 = sub2(quad_pick<0xFF>(df), df);
+            const fp2 xg2 = sub2(quad_pick<0xFF>(dg), dg);
             const size_t j = 64 * c + lane;
             stv(fout + j, f);
             stv(gout + j, g);
             z[0].mac(f, g);
-            z[1].mac(f, sub2(g, go));
+            z[1].mac(df, dg);
+            z[2].mac(xf2, xg2);
         }
     }
-    const fp2 self = z[0].value();
-    const bool odd_out = (lane & 1) != 0;
-    fp2 e[3] = {odd_out ? zero2() : self, odd_out ? self : zero2(), z[1].value()};
-    ticket_publish<3>(e, pub);
+    if constexpr (NS == 3) {
+        const fp2 self = z[0].value();
+        const bool odd_out = (lane & 1) != 0;
+        fp2 e[3] = {odd_out ? zero2() : self, odd_out ? self : zero2(), z[1].value()};
+        ticket_publish<3>(e, pub);
+    } else {
+        // lane l of a row adds the four lanes l mod 4 of the row (row_ror 8,
+        // then 4); lanes 0-3 of each row store their three sums
+        __shared__ fp2 s_part[kFold01Waves * 4][4][3];
+        __shared__ fp2 s_tot9[9];
+        u64 w[6];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const fp2 v = z[q].value();
+            w[2 * q] = v.re;
+            w[2 * q + 1] = v.im;
+        }
+        dpp_add_step<0x128, 0xf>(w);   // row_ror:8
+        dpp_add_step<0x124, 0xf>(w);   // row_ror:4
+        if ((lane & 15) < 4) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                s_part[wid * 4 + (lane >> 4)][lane & 3][q] =
+                    mk2(canon(fold1(w[2 * q])), canon(fold1(w[2 * q + 1])));
+        }
+        __syncthreads();
+        if (threadIdx.x < 9) {
+            // sum s: (product, quad lane) = S00 (0, 0), S11 (0, 1), SAA (1, 0),
+            // S22 (0, 2), S33 (0, 3), SBB (1, 3), X0 (1, 2), X1 (1, 1), X2 (2, 0),
+            // packed as nibble s = 4 product + lane
+            const int sidx = threadIdx.x;
+            const int code = (int)((0x856732410ull >> (4 * sidx)) & 0xf);
+            fp2 acc = zero2();
+            for (int r = 0; r < kFold01Waves * 4; ++r) acc = add2(acc, s_part[r][code & 3][code >> 2]);
+            s_tot9[sidx] = acc;
+        }
+        __syncthreads();
+        fp2 e[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) e[q] = s_tot9[q];
+        ticket_publish_wg<9>(e, pub);
+    }
+}
+
+__global__ void __launch_bounds__(kFoldThreads) k_sc2_fold2_pairs(const V2* __restrict__ fin,
+                                                         const V2* __restrict__ gin,
+                                                         size_t n_out, u64 r0r, u64 r0i,
+                                                         u64 r1r, u64 r1i,
+                                                         V2* __restrict__ fout,
+                                                         V2* __restrict__ gout, TicketPub pub,
+                                                         int desc)
+{
+    fold2_body<3>(fin, gin, n_out, r0r, r0i, r1r, r1i, fout, gout, pub, desc);
+}
+
+// The same with plain (cached) loads, for an input the pass before has just
+// written: 43 us against 46 us for 2 x 2^22 entries (the memory-side cache
+// still holds much of it).
+__global__ void __launch_bounds__(kFoldThreads) k_sc2_fold2_pairs_cached(const V2* __restrict__ fin,
+                                                         const V2* __restrict__ gin,
+                                                         size_t n_out, u64 r0r, u64 r0i,
+                                                         u64 r1r, u64 r1i,
+                                                         V2* __restrict__ fout,
+                                                         V2* __restrict__ gout, TicketPub pub,
+                                                         int desc)
+{
+    fold2_body<3, false>(fin, gin, n_out, r0r, r0i, r1r, r1i, fout, gout, pub, desc);
+}
+
+__global__ void __launch_bounds__(kFoldThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_sc2_fold2_quads(const V2* __restrict__ fin,
+                                                         const V2* __restrict__ gin,
+                                                         size_t n_out, u64 r0r, u64 r0i,
+                                                         u64 r1r, u64 r1i,
+                                                         V2* __restrict__ fout,
+                                                         V2* __restrict__ gout, TicketPub pub,
+                                                         int desc)
+{
+    fold2_body<9>(fin, gin, n_out, r0r, r0i, r1r, r1i, fout, gout, pub, desc);
 }
 
 // Round k >= 1: fold T_{k-1} (length 4 * n_quads) with r into T_k and sum
@@ -1274,7 +1394,7 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
 
 
 // Event-timed launch classes of one prove (timing on).
-enum Sc2Class { SC_FIRST, SC_FOLD01, SC_FOLD, SC_TAIL, SC_MID, SC_NCLASS };
+enum Sc2Class { SC_FIRST, SC_FOLD01, SC_FOLD, SC_TAIL, SC_MID, SC_PAIR, SC_NCLASS };
 
 struct Sc2State {
     DevBuf work;                 // T_k ping-pong: A (n/2) and B (n/4), for f and g
@@ -1321,11 +1441,12 @@ struct Sc2State {
 };
 Sc2State g_sc2;
 
-// Tagged words of the streaming launches: 256 first-pass workgroups x 9
-// sums or up to kMaxPubGroups x 3, two words each.
-constexpr size_t kSlotWords = (size_t)kMaxPubGroups * 3 * 2;
-static_assert(kFirst01Groups * 9 * 2 <= kSlotWords && kFold01Groups <= kMaxPubGroups &&
-                  kFoldGroups <= kMaxPubGroups,
+// Tagged words of the streaming launches, two words a sum: up to
+// kMaxPubGroups workgroups x 3 sums, or kFold2QuadGroups x 9.
+constexpr size_t kSlotWords = (size_t)kFold2QuadGroups * 9 * 2;
+static_assert(kFirst01Groups * 9 * 2 <= kSlotWords && kMfGroups * 9 * 2 <= kSlotWords &&
+                  kMaxPubGroups * 3 * 2 <= kSlotWords && kFold01Groups <= kMaxPubGroups &&
+                  kFoldGroups <= kMaxPubGroups && kFold2QuadGroups <= kMaxPubGroups,
               "ticket slots");
 
 bool sc2_setup(size_t n)
@@ -1699,8 +1820,8 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         S.bytes[SC_FIRST] += 32.0 * (double)n;
         S.launches[SC_FIRST] += 1;
         const unsigned nb2 = fold01_groups(n / 4);
-        k_sc2_fold01_round<<<nb2, kFoldThreads, 0, st>>>(cf, cg, n / 4, 0, 0, 0, 0, B, B + capB,
-                                                         pub_for(seq_b, seq_a, 4));
+        k_sc2_fold2_pairs<<<nb2, kFoldThreads, 0, st>>>(cf, cg, n / 4, 0, 0, 0, 0, B, B + capB,
+                                                        pub_for(seq_b, seq_a, 4), 1);
         TP_HIP_OK(hipGetLastError());
         S.bytes[SC_FOLD01] += 40.0 * (double)n;
         S.launches[SC_FOLD01] += 1;
@@ -1838,21 +1959,57 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         // the round-1 quadratics at r_0
         const fp2 r1 = round_done(1, quad_at(r0, v[0], v[1], v[2]), quad_at(r0, v[3], v[4], v[5]),
                                   quad_at(r0, v[6], v[7], v[8]));
-        seq = ++S.seq;
-        V2* of = B;
-        V2* og = B + capB;
-        nb = fold01_groups(n / 4);
-        if (!t_begin()) return -1;
-        k_sc2_fold01_round<<<nb, kFoldThreads, 0, st>>>(cf, cg, n / 4, r0.re, r0.im, r1.re, r1.im,
-                                                        of, og, pub_for(seq));
-        if (!t_end(SC_FOLD01, 40.0 * (double)n)) return -1;
-        TP_HIP_OK(hipGetLastError());
-        if (wait_pub(st, seq, 3, false) != 1 || !t_collect(SC_FOLD01)) return -1;
-        r = round_done(2, S.pub[0], S.pub[1], S.pub[2]);
-        cf = of;
-        cg = og;
-        k0 = 3;
-        m = n / 8;
+        // From round 2, passes that fold by two challenges: T_{k-2} into T_k
+        // by r_{k-2}, r_{k-1}.  While T_k is larger than the sliced mid
+        // rounds take, the pass publishes T_k's nine quad sums (rounds k and
+        // k + 1) and the next pass folds by both; the last one publishes
+        // round k's pair sums and the rounds below go on from r_k.  Each
+        // pass writes the work buffer that does not hold its input and
+        // sweeps the other way from the pass before it.
+        fp2 ra = r0, rb = r1;
+        size_t len = n;   // entries per table of T_{k-2}
+        int cls = SC_FOLD01;
+        int desc = 1;     // the first pass read the input bottom up
+        for (int kp = 2;; kp += 2) {
+            const size_t out = len / 4;   // entries per table of T_kp
+            const bool quads = out > (size_t)kMidGroups * kTailIn && out % 64 == 0;
+            seq = ++S.seq;
+            V2* of = (cf >= A && cf < A + 2 * capA) ? B : A;
+            V2* og = of + (of == A ? capA : capB);
+            nb = fold01_groups(out, quads ? kFold2QuadGroups : kFold01Groups);
+            if ((size_t)nb * (quads ? 9 : 3) * 2 > kSlotWords) {
+                fail("sumcheck: %u workgroups exceed the ticket slots", nb);
+                return -1;
+            }
+            if (!t_begin()) return -1;
+            if (quads)
+                k_sc2_fold2_quads<<<nb, kFoldThreads, 0, st>>>(cf, cg, out, ra.re, ra.im, rb.re,
+                                                               rb.im, of, og, pub_for(seq), desc);
+            else if (cls == SC_PAIR)   // its input was just written: cached loads
+                k_sc2_fold2_pairs_cached<<<nb, kFoldThreads, 0, st>>>(
+                    cf, cg, out, ra.re, ra.im, rb.re, rb.im, of, og, pub_for(seq), desc);
+            else
+                k_sc2_fold2_pairs<<<nb, kFoldThreads, 0, st>>>(cf, cg, out, ra.re, ra.im, rb.re,
+                                                               rb.im, of, og, pub_for(seq), desc);
+            if (!t_end(cls, 40.0 * (double)len)) return -1;
+            TP_HIP_OK(hipGetLastError());
+            if (wait_pub(st, seq, quads ? 9 : 3, false) != 1 || !t_collect(cls)) return -1;
+            cf = of;
+            cg = og;
+            if (!quads) {
+                r = round_done(kp, S.pub[0], S.pub[1], S.pub[2]);
+                k0 = kp + 1;
+                m = out / 2;
+                break;
+            }
+            for (int i = 0; i < 9; ++i) v[i] = S.pub[i];
+            ra = round_done(kp, add2(v[0], v[3]), add2(v[1], v[4]), add2(v[2], v[5]));
+            rb = round_done(kp + 1, quad_at(ra, v[0], v[1], v[2]), quad_at(ra, v[3], v[4], v[5]),
+                            quad_at(ra, v[6], v[7], v[8]));
+            len = out;
+            cls = SC_PAIR;
+            desc ^= 1;
+        }
     } else {
         const unsigned seq = ++S.seq;
         const unsigned nb = blocks_for((n / 2 + 1) / 2);
@@ -2031,8 +2188,10 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         }
         const unsigned seq = ++S.seq;
         const size_t quads = m / 2;   // input T_{k-1} has length 2m
-        V2* of = (k & 1) ? A : B;
-        V2* og = of + ((k & 1) ? capA : capB);
+        // the work buffer not holding the input (A holds up to n/2 entries
+        // per table, B n/4: T_1 goes to A, and any later table fits either)
+        V2* of = (cf >= A && cf < A + 2 * capA) ? B : A;
+        V2* og = of + (of == A ? capA : capB);
         const unsigned nb =
             (unsigned)std::min<size_t>(kFoldGroups, (quads + kFoldThreads - 1) / kFoldThreads);
         if (!t_begin()) return -1;
@@ -2142,10 +2301,11 @@ void tp_sc2_schedule(int* mid_groups, int* tail_in, int* mid_end)
 }
 
 // Per-class statistics of the last tp_sc2_prove (timing on): for each class
-// (0 the first pass over the input, 1 the round-2 fold from the input, 2 the
-// streaming fold+round launches, 3 the one-workgroup tail rounds, 4 the
-// sliced mid rounds) the summed event ms, the bytes the launches read and
-// write, and the launch count: out[3 c + 0..2].  n >= 15.
+// (0 the first pass over the input, 1 the round-2 pass folding the input by
+// two challenges, 2 the streaming fold+round launches, 3 the one-workgroup
+// tail rounds, 4 the sliced mid rounds, 5 the later passes folding by two
+// challenges) the summed event ms, the bytes the launches read and write,
+// and the launch count: out[3 c + 0..2].  n >= 18.
 void tp_sc2_class_stats(double* out, int n)
 {
     for (int c = 0; c < SC_NCLASS && 3 * c + 2 < n; ++c) {

@@ -104,17 +104,19 @@ def tail_stats():
     return ms.value, n.value
 
 
-CLASSES = ("first", "fold01", "fold", "tail", "mid")
+CLASSES = ("first", "fold01", "fold", "tail", "mid", "pair")
 
 
 def class_stats():
     """Per launch class of the last prove (timing on): {"first" | "fold01" |
-    "fold" | "tail" | "mid": {"ms", "bytes", "launches"}}.  "first" is the
-    pass over the input giving rounds 0 and 1, "fold01" round 2 folding the
-    input by r_0 and r_1, "fold" the streaming fold+round launches, "tail" the
-    one-workgroup rounds, "mid" the rounds on per-workgroup slices."""
-    out = (ctypes.c_double * 15)()
-    nb.LIB.tp_sc2_class_stats(out, 15)
+    "fold" | "tail" | "mid" | "pair": {"ms", "bytes", "launches"}}.  "first"
+    is the pass over the input giving rounds 0 and 1, "fold01" round 2
+    folding the input by r_0 and r_1 (and giving round 3's sums too on inputs
+    above 2^22 entries), "fold" the streaming fold+round launches, "tail" the
+    one-workgroup rounds, "mid" the rounds on per-workgroup slices, "pair"
+    the later passes folding by two challenges."""
+    out = (ctypes.c_double * 18)()
+    nb.LIB.tp_sc2_class_stats(out, 18)
     return {c: {"ms": out[3 * i], "bytes": out[3 * i + 1], "launches": int(out[3 * i + 2])}
             for i, c in enumerate(CLASSES)}
 
