@@ -58,7 +58,7 @@ def alg_bytes(nv: int) -> float:
     return 48.0 * ((1 << (nv + 1)) - 2)
 
 
-SC2_KERNELS = {"first": "k_sc2_first01_mfma", "fold01": "k_sc2_fold2_quads", "pair": "k_sc2_fold2_pairs_cached",
+SC2_KERNELS = {"first": "k_sc2_blocks_mfma", "fold01": "k_sc2_fold16", "pair": "k_sc2_fold2_pairs_cached",
                "fold": "k_sc2_fold_round", "tail": "k_sc2_tail_host", "mid": "k_sc2_mid"}
 
 
@@ -987,8 +987,11 @@ def main(argv=None):
                      "alg_bytes_per_launch": classes[dom]["bytes_per_launch"],
                      "classes": classes,
                      "ceiling": hbm_ceiling(),
-                     "note": "first: rounds 0 and 1 from one read of the input; fold01: round 2 "
-                             "folds the input by r_0 and r_1 into T_2; fold: streaming "
+                     "note": "first: the 256 block products of the input on the int8 matrix "
+                             "cores (one read of the input) and their reduction, from which the "
+                             "host takes rounds 0 to 3; fold01: one pass folding the input by "
+                             "r_0 .. r_3 into T_4 with round 4's sums; pair: passes folding by "
+                             "two challenges (inputs outside the block pass); fold: streaming "
                              "fold+round launches; mid: 128 workgroups, each folding its own "
                              "slice of the tables in LDS, workgroup 0 gathering their sums for "
                              "the host, which answers the challenges through mapped memory; "

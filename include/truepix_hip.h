@@ -155,6 +155,13 @@ void tp_sc2_schedule(int* mid_groups, int* tail_in, int* mid_end);
  * launches read and write, out[3c+2] launch count.  Fills the classes that
  * fit in n (all six for n >= 18). */
 void tp_sc2_class_stats(double* out, int n);
+/* 1 (default): proves of 2^14 to 2^28 entries per table take rounds 0 to 3
+ * from the 256 block products of the input (one pass on the matrix cores)
+ * and fold the input into T_4 in one pass; 0: rounds 0 and 1 from the quad
+ * products, then passes folding by two challenges.  The same proof either
+ * way.  No counterpart in the reference (a schedule switch of this
+ * library's prover, for tests and measurement). */
+void tp_sc2_set_block_pass(int on);
 /* Per-exchange latency of the one-workgroup tail of the last timed
  * tp_sc2_prove (an exchange: one host round trip for one or two rounds):
  * out[0] exchanges, out[1] device us per exchange until its sums are ready,

@@ -4,8 +4,10 @@
 // operands that set its no-overflow bounds, LazyProdSum against an exact Fp2
 // dot product, the sumcheck fold against a + r (b - a), the device's limb
 // multiply against 128-bit products, and the two-round quad fold against
-// two single folds and the exact polynomial.  Prints each failure; exit
-// status 1 on any.
+// two single folds and the exact polynomial, and the first four sumcheck
+// rounds from 256 block products (blocks.hpp) against rounds taken directly
+// on folded tables.  Prints each failure; exit status 1 on any.
+#include "blocks.hpp"
 #include "fp61.hpp"
 #include <cstdio>
 #include <vector>
@@ -170,6 +172,69 @@ int main() {
         for (int i = 0; i < 100000; ++i) {
             auto r = [] { return mk2(rnd() % P, rnd() % P); };
             check_quad(r(), r(), r(), r(), r(), r());
+        }
+    }
+    {
+        // rounds 0..3 from the block products against the rounds on the
+        // tables themselves, folded entry by entry with exact products
+        auto xmul = [](fp2 x, fp2 y) {
+            const u64 re = modp((u128)x.re * y.re + (u128)(P - y.im) * x.im);
+            const u64 im = modp((u128)x.re * y.im + (u128)x.im * y.re);
+            return mk2(re, im);
+        };
+        auto xfold = [&](fp2 a, fp2 b, fp2 r) { return add2(a, xmul(r, sub2(b, a))); };
+        for (int trial = 0; trial < 40; ++trial) {
+            const int nblk = 1 + trial % 7;
+            const int n = 16 * nblk;
+            std::vector<fp2> f(n), g(n);
+            const u64 ext[] = {0, 1, P - 1, 0x7f7f7f7f7f7f7f7full & P, 0x0080808080808080ull};
+            for (int i = 0; i < n; ++i) {
+                auto v = [&]() { return trial % 3 == 0 ? ext[rnd() % 5] : rnd() % P; };
+                f[i] = mk2(v(), v());
+                g[i] = mk2(v(), v());
+            }
+            fp2 Pm[16][16];
+            for (int a = 0; a < 16; ++a)
+                for (int b = 0; b < 16; ++b) {
+                    fp2 s = zero2();
+                    for (int B = 0; B < nblk; ++B) s = add2(s, xmul(f[16 * B + a], g[16 * B + b]));
+                    Pm[a][b] = s;
+                }
+            // challenges: special values in some trials, else drawn
+            std::vector<fp2> chal(4);
+            for (int k = 0; k < 4; ++k) {
+                const u64 sv[] = {0, 1, P - 1};
+                chal[k] = trial % 4 == 1 ? mk2(sv[(k + trial) % 3], sv[k % 3]) : mk2(rnd() % P, rnd() % P);
+            }
+            fp2 got[4][3];
+            auto rd = [&](int k, fp2 E0, fp2 E1, fp2 EA) {
+                got[k][0] = E0;
+                got[k][1] = E1;
+                got[k][2] = EA;
+                return chal[k];
+            };
+            fp2 rr[4];
+            rounds_from_blocks(Pm, rd, rr);
+            std::vector<fp2> tf = f, tg = g;
+            for (int k = 0; k < 4; ++k) {
+                fp2 E0 = zero2(), E1 = zero2(), EA = zero2();
+                const size_t half = tf.size() / 2;
+                for (size_t j = 0; j < half; ++j) {
+                    E0 = add2(E0, xmul(tf[2 * j], tg[2 * j]));
+                    E1 = add2(E1, xmul(tf[2 * j + 1], tg[2 * j + 1]));
+                    EA = add2(EA, xmul(sub2(tf[2 * j + 1], tf[2 * j]), sub2(tg[2 * j + 1], tg[2 * j])));
+                }
+                CHECK(eq2(E0, got[k][0]) && eq2(E1, got[k][1]) && eq2(EA, got[k][2]),
+                      "block rounds: trial %d round %d differs", trial, k);
+                CHECK(eq2(rr[k], chal[k]), "block rounds: challenge %d not returned", k);
+                std::vector<fp2> nf(half), ng(half);
+                for (size_t j = 0; j < half; ++j) {
+                    nf[j] = xfold(tf[2 * j], tf[2 * j + 1], chal[k]);
+                    ng[j] = xfold(tg[2 * j], tg[2 * j + 1], chal[k]);
+                }
+                tf.swap(nf);
+                tg.swap(ng);
+            }
         }
     }
     std::printf("field check: %d failures\n", fails);

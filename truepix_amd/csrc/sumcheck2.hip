@@ -50,6 +50,7 @@
 // A one-lane SHA-256 on the device was measured at 4.8 us per compression
 // (~26 us per transcript round), against ~3.4 us for the host round trip, so
 // the transcript stays on the host.
+#include "blocks.hpp"
 #include "fp61.hpp"
 #include "host_sha.hpp"
 #include "reduce.hpp"
@@ -169,11 +170,20 @@ constexpr size_t kTicketBytes = 256;
 // word of an earlier round, whatever its tag, is ever read as a later one's.
 // The totals land in s_tot; the return value (every thread) is 0 if a wait
 // expired.
-template <int N>
+// With EXT the per-thread partials go to `ext` (LDS of at least 1024 Fp2
+// that the caller no longer uses) instead of a buffer of their own.
+template <int N, bool EXT = false>
 __device__ __forceinline__ int gather_slots(u64* base, unsigned count, u64 tag, long long spin_ticks,
-                                            fp2 (&s_tot)[N])
+                                            fp2 (&s_tot)[N], fp2* ext = nullptr)
 {
-    __shared__ fp2 s_g[N][1024 / N];
+    constexpr unsigned kRow = 1024 / N;
+    fp2* s_g;   // [N][kRow]
+    if constexpr (EXT) {
+        s_g = ext;
+    } else {
+        __shared__ fp2 own[N * kRow];
+        s_g = own;
+    }
     const unsigned t = threadIdx.x;
     // thread t adds sum k = t mod N over slots g, g + G, ...
     const unsigned G = blockDim.x / N, k = t % N, g = t / N;
@@ -211,13 +221,13 @@ __device__ __forceinline__ int gather_slots(u64* base, unsigned count, u64 tag, 
     ok = __syncthreads_and(ok);   // every word is read before the slots are cleared
     for (unsigned w = t; w < count * 2 * N; w += blockDim.x)
         __hip_atomic_store(base + w, kSlotEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (g < G) s_g[k][g] = acc;
+    if (g < G) s_g[k * kRow + g] = acc;
     __syncthreads();
     const unsigned wid = t >> 6, lane = t & 63, nwv = blockDim.x >> 6;
     // wave w adds the G partials of sums w, w + nwv, ... (any workgroup size)
     for (unsigned q = wid; q < (unsigned)N; q += nwv) {
         fp2 sv[1] = {zero2()};
-        for (unsigned x = lane; x < G; x += 64) sv[0] = add2(sv[0], s_g[q][x]);
+        for (unsigned x = lane; x < G; x += 64) sv[0] = add2(sv[0], s_g[q * kRow + x]);
         wave_sum<1>(sv);
         if (lane == 0) s_tot[q] = sv[0];
     }
@@ -252,15 +262,15 @@ __device__ __forceinline__ bool slot_and_ticket(const fp2 (&v)[N], u64* slot, un
 // the workgroup taking the last ticket adds the slots (gather_slots), resets
 // the ticket for the next launch and publishes the totals as tagged words
 // (publish_tagged).
-template <int N>
-__device__ __forceinline__ void ticket_publish(fp2 (&v)[N], const TicketPub& a)
+template <int N, bool EXT = false>
+__device__ __forceinline__ void ticket_publish(fp2 (&v)[N], const TicketPub& a, fp2* ext = nullptr)
 {
     __shared__ fp2 s_tot[N];
     block_sum<N>(v);
     const unsigned nb = gridDim.x;
     const u64 tag = (u64)(a.seq & 7) << 61;
     if (!slot_and_ticket<N>(v, a.slots + (size_t)blockIdx.x * 2 * N, a.ticket, nb, tag)) return;
-    const int ok = gather_slots<N>(a.slots, nb, tag, a.spin_ticks, s_tot);
+    const int ok = gather_slots<N, EXT>(a.slots, nb, tag, a.spin_ticks, s_tot, ext);
     // every workgroup took its ticket after its challenges: the relay is idle
     // again (the first pass of a prove, which waits for none, clears what an
     // earlier prove may have left)
@@ -856,6 +866,265 @@ k_sc2_fold2_quads(const V2* __restrict__ fin,
     fold2_body<9>(fin, gin, n_out, r0r, r0i, r1r, r1i, fout, gout, pub, desc);
 }
 
+// Rounds 0 to 3 from one pass over the input on the int8 matrix cores: the
+// 256 block products P[a][b] = sum over 16-entry blocks B of
+// f[16B + a] g[16B + b] (a, b < 16), from which the host takes the four
+// rounds (blocks.hpp).  The byte-limb method of k_sc2_first01_mfma, with a
+// block of 16 entries in place of a quad: per chunk of 64 blocks (16 KB a
+// table) one MFMA per pair (a, b) adds the 16 x 16 byte products summed
+// over the chunk.  The 256 pairs need 256 MFMAs a chunk, so the chunk is
+// staged once per workgroup and shared: 16 waves, wave w taking the pairs
+// with a in 4 (w / 4) .. + 3 and b in 4 (w % 4) .. + 3 (16 accumulator
+// tiles), every wave reading its operands from the same staged chunk.
+// Staging is double buffered (one barrier a chunk).  A
+// block takes 272 bytes of LDS (16 of padding), so the eight blocks a
+// transposed read touches sit on distinct banks.  Each workgroup writes its
+// 256 products to its slots; k_sc2_blocks_reduce adds them.
+constexpr int kBkThreads = 1024;
+constexpr int kBkWaves = kBkThreads / 64;
+constexpr unsigned kBkGroups = 256;
+constexpr int kBkStride = 272;
+constexpr int kBkTable = 64 * kBkStride;     // one table's chunk in LDS
+constexpr int kBkBuf = 2 * kBkTable;         // f and g
+constexpr size_t kBkLds = 2 * (size_t)kBkBuf;
+// |tile entry| <= (number of blocks) x 2^14 below 2^31: at most 131071
+// blocks a workgroup; the host keeps n / 16 <= 2^16 x kBkGroups
+constexpr size_t kBkMaxEntries = (size_t)16 * 65536 * kBkGroups;
+// Inputs from 2^14 entries (16 chunks) take the block pass.
+constexpr int kBlocksMinVars = 14;
+
+__global__ void __launch_bounds__(kBkThreads) k_sc2_blocks_mfma(const V2* __restrict__ f,
+                                                               const V2* __restrict__ g,
+                                                               size_t n_blocks, fp2* __restrict__ slots)
+{
+    extern __shared__ __attribute__((aligned(16))) uint8_t bk_lds[];
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int a0 = 4 * (wid >> 2), b0 = 4 * (wid & 3);
+    mf_v4i acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = mf_v4i{0, 0, 0, 0};
+    // column sums of the u bytes: wave w adds those of f entry w (its A
+    // operand wid % 4) and of g entry 4 (w % 4) + w / 4 (its B operand w / 4),
+    // so that every entry's are added once
+    int ucol = 0, vcol = 0;
+    const int ua = wid & 3, vb = wid >> 2;
+    // transposed-read addresses: lane 16 G + m reads word m & 1 of blocks
+    // 16 G + m / 2 and 16 G + 8 + m / 2
+    const int grp = lane >> 4, m = lane & 15;
+    const int r1 = kBkStride * (16 * grp + (m >> 1)) + (m & 1) * 8, r2 = r1 + kBkStride * 8;
+    // thread t stages entry t of the chunk's 1024: block t / 16, entry t % 16
+    const int st_off = kBkStride * (t >> 4) + 16 * (t & 15);
+    const size_t n_chunks = n_blocks / 64;
+    // one chunk of loads in flight while a chunk is staged and multiplied
+    // (two or three, in registers of a loop unrolled by as many, measured
+    // 125 and 143-147 us against 117-122 us at 2 x 2^24 entries)
+    mf_v4u fv = {0, 0, 0, 0}, gv = {0, 0, 0, 0};
+    const mf_v4u* pf = reinterpret_cast<const mf_v4u*>(f) + t;
+    const mf_v4u* pg = reinterpret_cast<const mf_v4u*>(g) + t;
+    if (blockIdx.x < n_chunks) {
+        fv = __builtin_nontemporal_load(pf + 1024 * (size_t)blockIdx.x);
+        gv = __builtin_nontemporal_load(pg + 1024 * (size_t)blockIdx.x);
+    }
+    unsigned nb = 0;
+    const mf_v4u x80 = {0x80808080u, 0x80808080u, 0x80808080u, 0x80808080u};
+    for (size_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        uint8_t* sb = bk_lds + ((nb & 1) ? kBkBuf : 0);
+        *reinterpret_cast<mf_v4u*>(sb + st_off) = fv ^ x80;
+        *reinterpret_cast<mf_v4u*>(sb + kBkTable + st_off) = gv ^ x80;
+        if (c + gridDim.x < n_chunks) {   // the next chunk, in flight while this one runs
+            fv = __builtin_nontemporal_load(pf + 1024 * (c + gridDim.x));
+            gv = __builtin_nontemporal_load(pg + 1024 * (c + gridDim.x));
+        }
+        __syncthreads();   // the chunk is staged; every read of the buffer two chunks back is done
+        mf_v4i A[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const mf_v2i lo = lds_tr8(sb + r1 + 16 * (a0 + a)), hi = lds_tr8(sb + r2 + 16 * (a0 + a));
+            A[a] = mf_v4i{lo.x, lo.y, hi.x, hi.y};
+            if (a == ua) {
+#pragma unroll
+                for (int w = 0; w < 4; ++w) ucol = __builtin_amdgcn_sdot4(A[a][w], 0x01010101, ucol, false);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const mf_v2i lo = lds_tr8(sb + kBkTable + r1 + 16 * (b0 + b));
+            const mf_v2i hi = lds_tr8(sb + kBkTable + r2 + 16 * (b0 + b));
+            const mf_v4i Bb = mf_v4i{lo.x, lo.y, hi.x, hi.y};
+            if (b == vb) {
+#pragma unroll
+                for (int w = 0; w < 4; ++w) vcol = __builtin_amdgcn_sdot4(Bb[w], 0x01010101, vcol, false);
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+                acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[a], Bb, acc[a][b], 0, 0, 0);
+        }
+        ++nb;
+    }
+    // ---- column sums in LDS (one wave of each row and column of the wave
+    // grid adds them), their weighted words, then each wave's 16 products
+    __syncthreads();
+    int* Us = reinterpret_cast<int*>(bk_lds);      // [16 a][16 i]
+    int* Vs = Us + 256;                            // [16 b][16 j]
+    u64* Ws = reinterpret_cast<u64*>(Vs + 256);    // [16 a][2] weighted U, then [16 b][2] V
+    fp2* Ps = reinterpret_cast<fp2*>(Ws + 64);     // [16 a][16 b]
+    if (t < 512) Us[t] = 0;
+    __syncthreads();
+    atomicAdd(&Us[(a0 + ua) * 16 + m], ucol);
+    atomicAdd(&Vs[(b0 + vb) * 16 + m], vcol);
+    __syncthreads();
+    if (t < 64) {
+        // t < 32: U of entry t / 2, word t % 2; then V: sum_i 2^{8i} U[8 h + i]
+        const int* src = (t < 32 ? Us : Vs) + ((t >> 1) & 15) * 16 + 8 * (t & 1);
+        u64 wsum = 0;
+        for (int i = 0; i < 8; ++i) wsum += fmul_lazy(mod_i32(src[i]), 1ull << (8 * i));
+        Ws[t] = fold61(wsum);
+    }
+    __syncthreads();
+    // lane l holds C[4 (l / 16) + r][l % 16]: word pair (hf, hg) = (l / 32,
+    // (l / 8) % 2), bytes (bi, bj) = ((4 (l / 16) + r) % 8, l % 8)
+    const u64 k1 = 0x0101010101010101ull;                           // sum_{i<8} 2^{8i}, below p
+    const u64 c128 = fmul(128, k1);                                 // 128 K1
+    const u64 cnn = fmul(fmul(c128, c128), fold61((u64)nb * 64));   // 128^2 K1^2 N
+    const int hf = lane >> 5, hg = (lane >> 3) & 1, bj = lane & 7;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            u64 w[1] = {0};
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int bi = (4 * (lane >> 4) + r) & 7;
+                w[0] += fmul_lazy(mod_i32(acc[a][b][r]), 1ull << ((8 * (bi + bj)) % 61));
+            }
+            w[0] = fold1(w[0]);
+            // the 16 lanes of each word pair: the 8 of a half row, then the
+            // other row of the pair
+            dpp_add_step<0xb1, 0xf>(w);    // quad_perm [1,0,3,2]
+            dpp_add_step<0x4e, 0xf>(w);    // quad_perm [2,3,0,1]
+            fold_words(w);
+            dpp_add_step<0x141, 0xf>(w);   // row_half_mirror
+            w[0] = fold1(w[0]);
+            w[0] += __shfl_xor(w[0], 16, 64);
+            const u64 q = fold61(w[0]);
+            // Q[hf][hg] with the byte offsets restored
+            const u64 qt = fadd(fadd(q, fmul(c128, fadd(Ws[(a0 + a) * 2 + hf], Ws[32 + (b0 + b) * 2 + hg]))), cnn);
+            const u64 q00 = __shfl(qt, 0, 64), q01 = __shfl(qt, 8, 64);
+            const u64 q10 = __shfl(qt, 32, 64), q11 = __shfl(qt, 40, 64);
+            if (lane == 0) Ps[(a0 + a) * 16 + b0 + b] = mk2(fsub(q00, q11), fadd(q01, q10));
+        }
+    __syncthreads();
+    if (t < 256) {
+        const fp2 v = Ps[t];
+        V2 x;
+        x.re = v.re;
+        x.im = v.im;
+        reinterpret_cast<V2*>(slots)[(size_t)blockIdx.x * 256 + t] = x;
+    }
+}
+
+// The 256 block products over the groups' slots: workgroup s adds product s
+// and publishes it to the host as two tagged words (value | (seq mod 8) << 61).
+__global__ void __launch_bounds__(256) k_sc2_blocks_reduce(const fp2* __restrict__ slots,
+                                                           unsigned groups, fp2* host_vals,
+                                                           unsigned seq)
+{
+    const unsigned s = blockIdx.x, t = threadIdx.x;
+    fp2 v[1] = {zero2()};
+    for (unsigned g = t; g < groups; g += blockDim.x) {
+        const V2 x = reinterpret_cast<const V2*>(slots)[(size_t)g * 256 + s];
+        v[0] = add2(v[0], mk2(x.re, x.im));
+    }
+    block_sum<1>(v);
+    if (t == 0) {
+        const u64 tag = (u64)(seq & 7) << 61;
+        u64* w = reinterpret_cast<u64*>(host_vals) + 2 * s;
+        __hip_atomic_store(w, v[0].re | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(w + 1, v[0].im | tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+// Rounds 0 to 3 applied in one pass: T_0 (16 n_out entries a table) folded
+// by r_0 .. r_3 into T_4, with round 4's pair sums (published as
+// k_sc2_fold2_pairs does).  A wave takes 64 outputs (1024 inputs, 16 KB a
+// table), one table at a time, in four 4 KB sub-chunks staged through its
+// LDS slice as in k_sc2_fold2_pairs: lane l folds quad l of sub-chunk s by
+// r_0, r_1 into T_2 entry 64 s + l; the four T_2 entries of each lane then
+// go through the slice again so that lane j reads entries 4j .. 4j + 3 and
+// folds them by r_2, r_3 into output j.  The sweep runs from the top down.
+// The pass is bound by its integer multiplies, not by HBM: per wave and 64
+// outputs (2048 input entries) its loop issues 3078 VALU instructions, 474
+// of them 64-bit multiply-adds, about 120 us of issue for 2 x 2^24 entries,
+// which is what it takes.  So these measured the same or slower: issuing
+// each sub-chunk's loads one step ahead (127 VGPRs: 127-132 us against
+// 121-123 us), 8-wave workgroups, and 4, 5 or 6 waves per SIMD (6 spills).
+// Workgroups of 4 waves; the last one's ticket gather reuses the slices.
+constexpr int kFold16Waves = 4;
+constexpr int kFold16Threads = 64 * kFold16Waves;
+inline unsigned fold16_groups(size_t n_out)
+{
+    return (unsigned)std::min<size_t>(kMaxPubGroups, (n_out / 64 + kFold16Waves - 1) / kFold16Waves);
+}
+
+__global__ void __launch_bounds__(kFold16Threads) __attribute__((amdgpu_waves_per_eu(5))) k_sc2_fold16(const V2* __restrict__ fin,
+                                                    const V2* __restrict__ gin, size_t n_out,
+                                                    u64 r0r, u64 r0i, u64 r1r, u64 r1i,
+                                                    u64 r2r, u64 r2i, u64 r3r, u64 r3i,
+                                                    V2* __restrict__ fout, V2* __restrict__ gout,
+                                                    TicketPub pub)
+{
+    // one 4 KB slice per wave; the last workgroup's ticket gather reuses them
+    static_assert(kFold16Waves * 256 >= 1024, "gather partials fit the slices");
+    __shared__ __attribute__((aligned(16))) V2 stage[kFold16Waves][256];
+    const fp2 r0 = uniform2(mk2(r0r, r0i)), r1 = uniform2(mk2(r1r, r1i));
+    const fp2 r2 = uniform2(mk2(r2r, r2i)), r3 = uniform2(mk2(r3r, r3i));
+    const fp2 r01 = uniform2(mul2(r0, r1)), r23 = uniform2(mul2(r2, r3));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    mf_v4u* s_st = reinterpret_cast<mf_v4u*>(stage[wid]) + lane;
+    const V2* s_ld = stage[wid] + 4 * lane;
+    const size_t n_super = n_out / 64;   // n_out is a multiple of 64
+    LazyProdSum z[2];   // f g, f (g - g')
+    lazy_zero(z);
+    for (size_t c0 = (size_t)blockIdx.x * kFold16Waves + wid; c0 < n_super;
+         c0 += (size_t)gridDim.x * kFold16Waves) {
+        const size_t c = n_super - 1 - c0;
+        fp2 out[2];
+#pragma unroll
+        for (int tab = 0; tab < 2; ++tab) {
+            const mf_v4u* src = reinterpret_cast<const mf_v4u*>((tab ? gin : fin) + 1024 * c) + lane;
+            fp2 t2[4];
+#pragma unroll
+            for (int sub = 0; sub < 4; ++sub) {
+                mf_v4u x[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) x[k] = __builtin_nontemporal_load(src + 256 * sub + 64 * k);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) s_st[64 * k] = x[k];
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                t2[sub] = fold_quad(ldv(s_ld), ldv(s_ld + 1), ldv(s_ld + 2), ldv(s_ld + 3), r0, r1, r01);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
+#pragma unroll
+            for (int sub = 0; sub < 4; ++sub) stv(stage[wid] + 64 * sub + lane, t2[sub]);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            out[tab] = fold_quad(ldv(s_ld), ldv(s_ld + 1), ldv(s_ld + 2), ldv(s_ld + 3), r2, r3, r23);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        }
+        const size_t j = 64 * c + lane;
+        stv(fout + j, out[0]);
+        stv(gout + j, out[1]);
+        const fp2 go = shfl_xor1(out[1]);
+        z[0].mac(out[0], out[1]);
+        z[1].mac(out[0], sub2(out[1], go));
+    }
+    const fp2 self = z[0].value();
+    const bool odd_out = (lane & 1) != 0;
+    fp2 e[3] = {odd_out ? zero2() : self, odd_out ? self : zero2(), z[1].value()};
+    ticket_publish<3, true>(e, pub, reinterpret_cast<fp2*>(&stage[0][0]));
+}
+
 // Round k >= 1: fold T_{k-1} (length 4 * n_quads) with r into T_k and sum
 // the round-k pairs.  Quad q = inputs 4q..4q+3 -> outputs 2q, 2q+1.  With
 // `desc` the sweep runs from the top quad down, so a pass starts where the
@@ -1428,6 +1697,7 @@ struct Sc2State {
     DevBuf relay;                // the mid rounds' challenge relay (2 words)
     DevBuf arelay;               // queued launches' answer relay (4 words, launch_challenges)
     bool ahead = false;          // untimed proves queue their launches ahead (tp_sc2_set_ahead)
+    bool blocks = true;          // rounds 0 to 3 from the block products (tp_sc2_set_block_pass)
     DevBuf gate;                 // the mid launch's arrival gate (2 words)
     fp2 pub[16];                 // the last published round's values, read out of the slot
     DevBuf slots;                // the streaming launches' tagged sums (ticket_publish)
@@ -1438,6 +1708,9 @@ struct Sc2State {
     int mid_fallbacks = 0;       // mid launches that did not start whole (last prove)
     long long* host_mtrace = nullptr;   // mapped: [40][kMidGroups][2] stamps (timing on)
     long long* dev_mtrace = nullptr;
+    DevBuf blk_slots;            // the block products of each k_sc2_blocks_mfma workgroup
+    fp2* host_blk = nullptr;     // mapped: the 256 block products as tagged words
+    fp2* dev_blk = nullptr;
 };
 Sc2State g_sc2;
 
@@ -1473,6 +1746,15 @@ bool sc2_setup(size_t n)
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLds));
         TP_HIP_OK(hipFuncSetAttribute((const void*)k_sc2_first01_mfma,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMfLds));
+        TP_HIP_OK(hipFuncSetAttribute((const void*)k_sc2_blocks_mfma,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBkLds));
+        void* hb = nullptr;
+        TP_HIP_OK(hipHostMalloc(&hb, 256 * sizeof(fp2), hipHostMallocMapped | hipHostMallocCoherent));
+        S.host_blk = static_cast<fp2*>(hb);
+        for (int i = 0; i < 512; ++i) reinterpret_cast<u64*>(hb)[i] = kSlotEmpty;
+        void* db = nullptr;
+        TP_HIP_OK(hipHostGetDevicePointer(&db, hb, 0));
+        S.dev_blk = static_cast<fp2*>(db);
         TP_HIP_OK(hipFuncSetAttribute((const void*)k_sc2_mid,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTailLds));
         void* tp_ = nullptr;
@@ -1502,6 +1784,7 @@ bool sc2_setup(size_t n)
     if (!S.gate.ensure(2 * sizeof(unsigned))) return false;
     if (!S.slots.ensure(kSlotWords * sizeof(u64))) return false;
     if (!S.ticket.ensure(kTicketBytes)) return false;
+    if (!S.blk_slots.ensure((size_t)kBkGroups * 256 * sizeof(fp2))) return false;
     return true;
 }
 
@@ -1575,6 +1858,48 @@ int wait_pub(hipStream_t st, unsigned seq, int n, bool gate)
     return 1;
 }
 
+// Waits until k_sc2_blocks_reduce has published the 256 block products as
+// sequence number `seq` (tagged words) and copies them to prod; the caller
+// empties the words (empty_blocks) once the next launch is queued.  Returns
+// false on a kernel abort mark, a stream error, the stream idle without
+// them, or 30 s.
+bool wait_blocks(hipStream_t st, unsigned seq, fp2 (&prod)[16][16])
+{
+    Sc2State& S = g_sc2;
+    volatile u64* w = reinterpret_cast<volatile u64*>(S.host_blk);
+    const u64 tag = (u64)(seq & 7) << 61;
+    int next = 0;   // words before this one already carry the tag
+    auto ready = [&]() {
+        for (; next < 512; ++next) {
+            const u64 x = w[next];
+            if ((x & ~P) != tag || (x & P) == P) return false;
+        }
+        return true;
+    };
+    const auto t0 = std::chrono::steady_clock::now();
+    for (long spins = 1; !ready(); ++spins) {
+        if (*S.host_seq == kSeqAbort) return fail("sumcheck: a kernel gave up waiting (round %u)", seq);
+        if (spins % 4096 == 0) {
+            const hipError_t q = hipStreamQuery(st);
+            if (q != hipSuccess && q != hipErrorNotReady)
+                return fail("sumcheck stream error: %s", hipGetErrorString(q));
+            if (q == hipSuccess && !ready())
+                return fail("sumcheck: stream idle but block products %u not published", seq);
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(30))
+                return fail("sumcheck: timed out waiting for block products %u", seq);
+        }
+    }
+    for (int i = 0; i < 256; ++i) prod[i / 16][i % 16] = mk2(w[2 * i] & P, w[2 * i + 1] & P);
+    return true;
+}
+
+// All ones in the block products' words (value bits p: no sequence's word).
+void empty_blocks()
+{
+    volatile u64* w = reinterpret_cast<volatile u64*>(g_sc2.host_blk);
+    for (int i = 0; i < 512; ++i) w[i] = kSlotEmpty;
+}
+
 // The n values a k_publish_* launch left under sequence number seq (read
 // after wait_seq), copied to pub[]; the slot is emptied like wait_pub's.
 void take_seq_vals(int n)
@@ -1643,6 +1968,7 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
         TP_HIP_OK(hipMemsetAsync(S.slots.ptr, 0xff, kSlotWords * sizeof(u64), st));
         TP_HIP_OK(hipMemsetAsync(S.ticket.ptr, 0, kTicketBytes, st));
         TP_HIP_OK(hipMemsetAsync(S.arelay.ptr, 0xff, 4 * sizeof(u64), st));
+        empty_blocks();
     }
     S.slots_clean = false;   // until this prove returns 0
     {
@@ -1944,6 +2270,38 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             }
         }
         guard.armed = false;   // every queued launch has had its answers
+    } else if (S.blocks && nv >= kBlocksMinVars && n % 1024 == 0 && n <= kBkMaxEntries) {
+        // rounds 0 to 3 from the 256 block products (one pass over the input
+        // and a 256-workgroup reduction), then one pass folds the input by
+        // r_0 .. r_3 into T_4 with round 4's sums
+        unsigned seq = ++S.seq;
+        const unsigned nbk = (unsigned)std::min<size_t>(kBkGroups, n / 1024);
+        if (!t_begin()) return -1;
+        k_sc2_blocks_mfma<<<nbk, kBkThreads, kBkLds, st>>>(cf, cg, n / 16, S.blk_slots.as<fp2>());
+        k_sc2_blocks_reduce<<<256, 256, 0, st>>>(S.blk_slots.as<fp2>(), nbk, S.dev_blk, seq);
+        if (!t_end(SC_FIRST, 32.0 * (double)n)) return -1;
+        TP_HIP_OK(hipGetLastError());
+        fp2 prod[16][16];
+        if (!wait_blocks(st, seq, prod) || !t_collect(SC_FIRST)) return -1;
+        fp2 rk[4];
+        rounds_from_blocks(prod, round_done, rk);
+        seq = ++S.seq;
+        V2* of = A;
+        V2* og = A + capA;
+        const size_t out = n / 16;
+        if (!t_begin()) return -1;
+        k_sc2_fold16<<<fold16_groups(out), kFold16Threads, 0, st>>>(
+            cf, cg, out, rk[0].re, rk[0].im, rk[1].re, rk[1].im, rk[2].re, rk[2].im, rk[3].re,
+            rk[3].im, of, og, pub_for(seq));
+        if (!t_end(SC_FOLD01, 34.0 * (double)n)) return -1;
+        TP_HIP_OK(hipGetLastError());
+        empty_blocks();   // while the fold runs
+        if (wait_pub(st, seq, 3, false) != 1 || !t_collect(SC_FOLD01)) return -1;
+        r = round_done(4, S.pub[0], S.pub[1], S.pub[2]);
+        cf = of;
+        cg = og;
+        k0 = 5;
+        m = out / 2;
     } else if (nv >= 3) {
         // rounds 0 and 1 from one pass, round 2 folds the input by r_0, r_1
         unsigned seq = ++S.seq;
@@ -2210,6 +2568,12 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
 
 // Enables per-launch HIP-event timing of the fused fold+round kernel.
 void tp_sc2_set_timing(int on) { g_sc2.timed = on != 0; }
+
+// 1 (default): proves of 2^14 to 2^28 entries take rounds 0 to 3 from the
+// block products and fold the input into T_4 in one pass; 0: rounds 0 and 1
+// from the quad products and the passes folding by two challenges.  The
+// proof is the same either way.
+void tp_sc2_set_block_pass(int on) { g_sc2.blocks = on != 0; }
 
 // 1: an untimed prove queues its launches down to the mid launch at once,
 // each taking its challenges from the answer slot; 0 (default): each launch

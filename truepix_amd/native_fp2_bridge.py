@@ -87,6 +87,7 @@ def bind(lib) -> None:
     _sig(lib, "tp_sc2_prove", [_VP, _VP, _I32, _U64, _U64P, _U64P, _U64P, _U64P, _VP], _I32)
     _sig(lib, "tp_sc2_set_timing", [_I32])
     _sig(lib, "tp_sc2_set_ahead", [_I32])
+    _sig(lib, "tp_sc2_set_block_pass", [_I32])
     _sig(lib, "tp_rs_encode", [_VP, _I32, _I32, _I32, _VP, _I32, _VP], _I32)
     _sig(lib, "tp_early_create_dev",
          [_I32, _I32P, _I32P, _I32P, _U64P, _U64P, _I32, _I32, _VP, _VP, _U64P, _U64P], _VP)

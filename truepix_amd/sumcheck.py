@@ -91,6 +91,14 @@ def set_ahead(on: bool) -> None:
     nb.LIB.tp_sc2_set_ahead(1 if on else 0)
 
 
+def set_block_pass(on: bool) -> None:
+    """Rounds 0 to 3 from the input's block products and one pass folding the
+    input into T_4 (default, for 2^14 to 2^28 entries), or rounds 0 and 1 from
+    its quad products and passes folding by two challenges; same proof either
+    way."""
+    nb.LIB.tp_sc2_set_block_pass(1 if on else 0)
+
+
 def stats():
     ms, by, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
     nb.LIB.tp_sc2_stats(ctypes.byref(ms), ctypes.byref(by), ctypes.byref(n))
