@@ -19,8 +19,24 @@
 
 namespace tp {
 
-// -x in Fp2.
-inline fp2 fsub2_neg(fp2 x) { return sub2(zero2(), x); }
+// sum_i x_i y_i in Fp2 for canonical x, y and n <= 16: the products summed
+// exactly in 128 bits (re = sum x.re y.re + x.im (p - y.im), im = sum x.re
+// y.im + x.im y.re; 2n terms below 2^122 each, so below 2^127) and reduced
+// once, instead of reducing every product.
+inline fp2 dot2(const fp2* x, const fp2* y, int n)
+{
+    typedef unsigned __int128 u128;
+    u128 re = 0, im = 0;
+    for (int i = 0; i < n; ++i) {
+        re += (u128)x[i].re * y[i].re + (u128)x[i].im * (P - y[i].im);
+        im += (u128)x[i].re * y[i].im + (u128)x[i].im * y[i].re;
+    }
+    auto red = [](u128 v) {
+        const u64 t = (u64)(v & P) + (u64)((v >> 61) & P) + (u64)(v >> 122);   // < 3p
+        return canon(canon(t >= 2 * P ? t - P : t));
+    };
+    return mk2(red(re), red(im));
+}
 
 // Rounds 0..3 from the block products: round_done(k, E0, E1, EA) records
 // round k and returns r_k; the four challenges land in r.  Per pair of T_k
@@ -28,7 +44,8 @@ inline fp2 fsub2_neg(fp2 x) { return sub2(zero2(), x); }
 // support S of 2 sup block entries (lo on the first sup, hi on the rest):
 //   Ylo[a] = sum_b lo_b P[a][b],  Yhi[a] = sum_b hi_b P[a][b]  (a in S),
 //   E0 += lo . Ylo,  E1 += hi . Yhi,  EA += (hi - lo) . (Yhi - Ylo),
-// 4 sup^2 + 4 sup products a pair, about 600 for the four rounds.
+// 4 sup^2 + 4 sup products a pair, about 600 for the four rounds, each dot
+// product (dot2) reduced once.
 template <typename RoundDone>
 void rounds_from_blocks(const fp2 (&P)[16][16], RoundDone round_done, fp2 (&r)[4])
 {
@@ -43,24 +60,16 @@ void rounds_from_blocks(const fp2 (&P)[16][16], RoundDone round_done, fp2 (&r)[4
             const int base = 2 * p * sup;
             const fp2* lo = c[2 * p];
             const fp2* hi = c[2 * p + 1];
-            fp2 ylo[16], yhi[16];
+            fp2 ylo[16], yhi[16], d[16], dy[16];
             for (int a = 0; a < 2 * sup; ++a) {
-                fp2 sl = zero2(), sh = zero2();
-                for (int b = 0; b < sup; ++b) {
-                    sl = add2(sl, mul2(lo[b], P[base + a][base + b]));
-                    sh = add2(sh, mul2(hi[b], P[base + a][base + sup + b]));
-                }
-                ylo[a] = sl;
-                yhi[a] = sh;
+                ylo[a] = dot2(lo, &P[base + a][base], sup);
+                yhi[a] = dot2(hi, &P[base + a][base + sup], sup);
+                d[a] = a < sup ? sub2(zero2(), lo[a]) : hi[a - sup];
+                dy[a] = sub2(yhi[a], ylo[a]);
             }
-            for (int a = 0; a < sup; ++a) {
-                E0 = add2(E0, mul2(lo[a], ylo[a]));
-                E1 = add2(E1, mul2(hi[a], yhi[sup + a]));
-            }
-            for (int a = 0; a < 2 * sup; ++a) {
-                const fp2 d = a < sup ? fsub2_neg(lo[a]) : hi[a - sup];
-                EA = add2(EA, mul2(d, sub2(yhi[a], ylo[a])));
-            }
+            E0 = add2(E0, dot2(lo, ylo, sup));
+            E1 = add2(E1, dot2(hi, yhi + sup, sup));
+            EA = add2(EA, dot2(d, dy, 2 * sup));
         }
         r[k] = round_done(k, E0, E1, EA);
         for (int p = 0; p < entries / 2; ++p) {
