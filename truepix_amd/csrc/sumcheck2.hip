@@ -650,6 +650,14 @@ __global__ void __launch_bounds__(kMfThreads) k_sc2_first01_mfma(const V2* __res
 // ran at 5.2 TB/s from a cold cache, against 5.7 TB/s staged (which is also
 // what a staged kernel without the arithmetic reaches).
 constexpr int kFold01Waves = 8;
+// A wave's staging slice holds 256 entries with one entry of padding after
+// every 16 (entry e at slice_pos(e)): a lane reading its quad back (64 B at
+// lane x 64 B) then meets no other lane of its cycle on a bank; unpadded,
+// lanes four apart did (11% of the T_0 -> T_4 pass's time in bank
+// conflicts, 0% padded, the pass's time the same: it waits on its
+// multiplies).  Entry lane + 64 k sits at slice_pos(lane) + 68 k.
+constexpr int kSlice = 272;
+__host__ __device__ constexpr int slice_pos(int e) { return e + (e >> 4); }
 __device__ __forceinline__ fp2 shfl_xor1(fp2 x)
 {
     return mk2(__shfl_xor(x.re, 1, 64), __shfl_xor(x.im, 1, 64));
@@ -711,14 +719,14 @@ __device__ __forceinline__ void fold2_body(const V2* __restrict__ fin, const V2*
 {
     static_assert(kFoldThreads == 64 * kFold01Waves, "one LDS slice per wave");
     static_assert(NS == 3 || NS == 9, "pair or quad sums");
-    __shared__ __attribute__((aligned(16))) V2 stage[kFold01Waves][256];
+    __shared__ __attribute__((aligned(16))) V2 stage[kFold01Waves][kSlice];
     u64 rw[4] = {r0r, r0i, r1r, r1i};
     if (!launch_challenges(pub, rw)) return;
     const fp2 r0 = mk2(rw[0], rw[1]), r1 = mk2(rw[2], rw[3]);
     const fp2 r01 = uniform2(mul2(r0, r1));
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    mf_v4u* s_st = reinterpret_cast<mf_v4u*>(stage[wid]) + lane;   // this lane's staging stores
-    const V2* s_ld = stage[wid] + 4 * lane;                        // and its quad
+    mf_v4u* s_st = reinterpret_cast<mf_v4u*>(stage[wid]) + slice_pos(lane);   // staging stores
+    const V2* s_ld = stage[wid] + slice_pos(4 * lane);                       // and its quad
     // n_out is a power of two: either a multiple of 64 or one short chunk
     // of an even length m.  In a short chunk the loads past its 4m inputs
     // repeat its last one (in bounds), and only lanes below m, both lanes of
@@ -749,12 +757,12 @@ __device__ __forceinline__ void fold2_body(const V2* __restrict__ fin, const V2*
         // wave-scope fences keep the compiler from moving an LDS store
         // across the reads of the data it replaces
 #pragma unroll
-        for (int k = 0; k < 4; ++k) s_st[64 * k] = xf[k];
+        for (int k = 0; k < 4; ++k) s_st[68 * k] = xf[k];
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const fp2 f0 = ldv(s_ld), f1 = ldv(s_ld + 1), f2 = ldv(s_ld + 2), f3 = ldv(s_ld + 3);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 #pragma unroll
-        for (int k = 0; k < 4; ++k) s_st[64 * k] = xg[k];
+        for (int k = 0; k < 4; ++k) s_st[68 * k] = xg[k];
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const fp2 g0 = ldv(s_ld), g1 = ldv(s_ld + 1), g2 = ldv(s_ld + 2), g3 = ldv(s_ld + 3);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -884,7 +892,15 @@ constexpr int kBkThreads = 1024;
 constexpr int kBkWaves = kBkThreads / 64;
 constexpr unsigned kBkGroups = 256;
 constexpr int kBkStride = 272;
-constexpr int kBkTable = 64 * kBkStride;     // one table's chunk in LDS
+// block b of a staged chunk sits at kBkStride b + kBkGroupPad (b / 16): the
+// extra 128 B puts the second 16-lane group of a transposed read (blocks
+// 16 G + ...) on the other half of the banks from the first.  Bank conflicts
+// went from 13% of the pass's time to 0%, its time the same: one chunk per
+// CU in flight (32 KB) at the 1.7 us a chunk takes is the 4.7 TB/s it
+// reaches, and a second chunk in flight only lengthened that latency.
+constexpr int kBkGroupPad = 128;
+__host__ __device__ constexpr int bk_block_off(int b) { return kBkStride * b + kBkGroupPad * (b >> 4); }
+constexpr int kBkTable = bk_block_off(64);   // one table's chunk in LDS
 constexpr int kBkBuf = 2 * kBkTable;         // f and g
 constexpr size_t kBkLds = 2 * (size_t)kBkBuf;
 // |tile entry| <= (number of blocks) x 2^14 below 2^31: at most 131071
@@ -913,9 +929,9 @@ __global__ void __launch_bounds__(kBkThreads) k_sc2_blocks_mfma(const V2* __rest
     // transposed-read addresses: lane 16 G + m reads word m & 1 of blocks
     // 16 G + m / 2 and 16 G + 8 + m / 2
     const int grp = lane >> 4, m = lane & 15;
-    const int r1 = kBkStride * (16 * grp + (m >> 1)) + (m & 1) * 8, r2 = r1 + kBkStride * 8;
+    const int r1 = bk_block_off(16 * grp + (m >> 1)) + (m & 1) * 8, r2 = r1 + kBkStride * 8;
     // thread t stages entry t of the chunk's 1024: block t / 16, entry t % 16
-    const int st_off = kBkStride * (t >> 4) + 16 * (t & 15);
+    const int st_off = bk_block_off(t >> 4) + 16 * (t & 15);
     const size_t n_chunks = n_blocks / 64;
     // one chunk of loads in flight while a chunk is staged and multiplied
     // (two or three, in registers of a loop unrolled by as many, measured
@@ -1078,14 +1094,14 @@ __global__ void __launch_bounds__(kFold16Threads) __attribute__((amdgpu_waves_pe
                                                     TicketPub pub)
 {
     // one 4 KB slice per wave; the last workgroup's ticket gather reuses them
-    static_assert(kFold16Waves * 256 >= 1024, "gather partials fit the slices");
-    __shared__ __attribute__((aligned(16))) V2 stage[kFold16Waves][256];
+    static_assert(kFold16Waves * kSlice >= 1024, "gather partials fit the slices");
+    __shared__ __attribute__((aligned(16))) V2 stage[kFold16Waves][kSlice];
     const fp2 r0 = uniform2(mk2(r0r, r0i)), r1 = uniform2(mk2(r1r, r1i));
     const fp2 r2 = uniform2(mk2(r2r, r2i)), r3 = uniform2(mk2(r3r, r3i));
     const fp2 r01 = uniform2(mul2(r0, r1)), r23 = uniform2(mul2(r2, r3));
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    mf_v4u* s_st = reinterpret_cast<mf_v4u*>(stage[wid]) + lane;
-    const V2* s_ld = stage[wid] + 4 * lane;
+    mf_v4u* s_st = reinterpret_cast<mf_v4u*>(stage[wid]) + slice_pos(lane);
+    const V2* s_ld = stage[wid] + slice_pos(4 * lane);
     const size_t n_super = n_out / 64;   // n_out is a multiple of 64
     LazyProdSum z[2];   // f g, f (g - g')
     lazy_zero(z);
@@ -1103,13 +1119,13 @@ __global__ void __launch_bounds__(kFold16Threads) __attribute__((amdgpu_waves_pe
 #pragma unroll
                 for (int k = 0; k < 4; ++k) x[k] = __builtin_nontemporal_load(src + 256 * sub + 64 * k);
 #pragma unroll
-                for (int k = 0; k < 4; ++k) s_st[64 * k] = x[k];
+                for (int k = 0; k < 4; ++k) s_st[68 * k] = x[k];
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 t2[sub] = fold_quad(ldv(s_ld), ldv(s_ld + 1), ldv(s_ld + 2), ldv(s_ld + 3), r0, r1, r01);
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
 #pragma unroll
-            for (int sub = 0; sub < 4; ++sub) stv(stage[wid] + 64 * sub + lane, t2[sub]);
+            for (int sub = 0; sub < 4; ++sub) stv(stage[wid] + slice_pos(lane) + 68 * sub, t2[sub]);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             out[tab] = fold_quad(ldv(s_ld), ldv(s_ld + 1), ldv(s_ld + 2), ldv(s_ld + 3), r2, r3, r23);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
