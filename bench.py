@@ -51,6 +51,24 @@ def measured_traffic(kernel: str):
     return rec.get("bytes_per_launch")
 
 
+def measured_valu(kernel: str):
+    """The committed VALU counter summary of `kernel` (rocprofv3 SQ_INSTS_VALU
+    and the kernel's loop instruction counts), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", "%s_valu.json" % kernel.replace("k_", "", 1))
+    try:
+        with open(path) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if rec.get("kernel") != kernel:
+        return None
+    keep = ("sq_insts_valu_per_launch", "valu_busy_at_4_cycles", "loop_valu_insts_per_wave_iteration",
+            "loop_mad_u64_u32_per_wave_iteration", "note")
+    out = {k: rec[k] for k in keep if k in rec}
+    out["source"] = os.path.relpath(path, ROOT)
+    return out
+
+
 def alg_bytes(nv: int) -> float:
     """Algorithmic HBM bytes of one nv-round prove (both tables, 16 B per Fp2):
     round r reads the 2 x 2^(nv-r) entries of its tables and writes the
@@ -981,6 +999,7 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": measured_traffic(SC2_KERNELS[dom]),
+                     "valu": measured_valu(SC2_KERNELS[dom]),
                      "kernel": SC2_KERNELS[dom],
                      "avg_launch_us": classes[dom]["avg_launch_us"],
                      "launches_per_prove": classes[dom]["launches_per_prove"],

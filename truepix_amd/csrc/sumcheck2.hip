@@ -1613,7 +1613,7 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_mid(
 __global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
     const V2* __restrict__ fin, const V2* __restrict__ gin, unsigned len_in, int k0, int nv,
     u64 rr, u64 ri, fp2* host_vals, unsigned* host_seq, unsigned seq0, u64* answer,
-    long long spin_ticks, long long* trace)
+    long long spin_ticks, long long* trace, unsigned r_want, int r_from_slot)
 {
     extern __shared__ V2 tl[];
     __shared__ u64 s_r[4];
@@ -1622,6 +1622,38 @@ __global__ void __launch_bounds__(kTailThreads) k_sc2_tail_host(
     V2* tf = tl;
     V2* tg = tl + kTailIn / 2;
     const int t = threadIdx.x;
+    if (r_from_slot) {
+        // queued before the transcript step of r = r_{k0-1}: the host's answer
+        // under sequence number r_want, unless the host gives up first
+        if (t == 0) {
+            const u64 tag = (u64)(r_want & 7) << 61;
+            const long long t0 = wall_clock64();
+            int ok = 0;
+            for (;;) {
+                const u64 a0 = __hip_atomic_load(answer, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                const u64 a1 = __hip_atomic_load(answer + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                if ((a0 & ~P) == tag && (a0 & P) != P && (a1 & ~P) == tag && (a1 & P) != P) {
+                    s_r[0] = a0 & P;
+                    s_r[1] = a1 & P;
+                    ok = 1;
+                    break;
+                }
+                if (__hip_atomic_load(host_seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == kSeqHostAbort)
+                    break;
+                if (wall_clock64() - t0 > spin_ticks) {
+                    __hip_atomic_store(host_seq, kSeqAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            s_ok = ok;
+        }
+        __syncthreads();
+        if (!s_ok) return;
+        rr = s_r[0];
+        ri = s_r[1];
+        __syncthreads();   // s_r and s_ok are rewritten by the exchanges
+    }
     fp2 ra = mk2(rr, ri), rb = zero2(), rab = zero2();
     int np = 1;                 // challenges not yet applied to the table at sf / sg
     const V2* sf = fin;         // T_{k-np}
@@ -2076,7 +2108,8 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
     // are already in S.pub when first_done.  Every exchange but the last is
     // answered (the last round's challenge goes to the next launch); r is the
     // last round's challenge.  False on failure.
-    auto mid_host = [&](int k_first, unsigned seq0m, int nrm, bool first_done, fp2& r_out) -> bool {
+    auto mid_host = [&](int k_first, unsigned seq0m, int nrm, bool first_done, fp2& r_out,
+                        bool answer_last = false) -> bool {
         const int ne = mid_exchanges(nrm);
         int kk = k_first;
         for (int e = 0; e < ne; ++e) {
@@ -2095,12 +2128,76 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
                 r_out = rs[1];
             } else {
                 r_out = round_done(kk, S.pub[0], S.pub[1], S.pub[2]);
-                if (!last) answer(seq0m + e, &r_out, 1);
+                if (!last || answer_last) answer(seq0m + e, &r_out, 1);
             }
             kk += nre;
         }
         return true;
     };
+
+    // The sliced mid launch over the input T_{k-1} (2m entries a table) with
+    // r = r_{k-1}, or, when from_slot, r taken from the answer slot under
+    // sequence number `want` (a launch queued before that transcript step).
+    struct MidLaunch { unsigned seq0 = 0; int nr = 0; V2* of = nullptr; V2* og = nullptr; unsigned slice = 0; };
+    auto launch_mid = [&](const V2* in_f, const V2* in_g, size_t m2, fp2 rr, unsigned want,
+                          int from_slot, MidLaunch& ml) -> bool {
+        ml.slice = (unsigned)(m2 / kMidGroups);   // > kMidEnd here
+        ml.nr = 0;
+        while ((ml.slice >> ml.nr) > kMidEnd) ++ml.nr;
+        ml.of = (in_f >= A && in_f < A + 2 * capA) ? B : A;
+        ml.og = ml.of + (ml.of == A ? capA : capB);
+        ml.seq0 = S.seq + 1;
+        S.seq += (unsigned)mid_exchanges(ml.nr);
+        const V2* a_f = in_f;
+        const V2* a_g = in_g;
+        unsigned a_slice = ml.slice;
+        int a_nr = ml.nr;
+        u64 a_rr = rr.re, a_ri = rr.im;
+        u64* a_parts = S.mid_parts.as<u64>();
+        fp2* a_vals = S.dev_vals;
+        unsigned* a_seq = S.dev_seq;
+        unsigned a_seq0 = ml.seq0;
+        u64* a_ans = S.dev_answer;
+        long long a_spin = S.spin_ticks;
+        V2* a_of = ml.of;
+        V2* a_og = ml.og;
+        u64* a_relay = S.relay.as<u64>();
+        long long* a_trace = S.timed && mid_exchanges(ml.nr) <= 40 ? S.dev_mtrace : nullptr;
+        unsigned* a_gate = S.gate.as<unsigned>();
+        long long a_gate_ticks = S.gate_ticks;
+        unsigned a_want = want;
+        int a_from = from_slot;
+        u64* a_arelay = S.arelay.as<u64>();
+        void* args[] = {&a_f, &a_g, &a_slice, &a_nr, &a_rr, &a_ri, &a_parts, &a_vals, &a_seq,
+                        &a_seq0, &a_ans, &a_spin, &a_of, &a_og, &a_relay, &a_trace,
+                        &a_gate, &a_gate_ticks, &a_want, &a_from, &a_arelay};
+        if (!S.mid_clean) {   // first mid launch, or the last one did not finish
+            k_sc2_mid_reset<<<1, 256, 0, st>>>(a_parts, a_relay, a_gate);
+            TP_HIP_OK(hipGetLastError());
+        }
+        S.mid_clean = false;
+        TP_HIP_OK(hipLaunchKernel((const void*)k_sc2_mid, dim3(kMidGroups), dim3(kTailThreads),
+                                  args, (unsigned)(ml.slice * sizeof(V2)), st));
+        return true;
+    };
+    // A mid launch queued behind the pass before it (untimed proves), for the
+    // loop's round mid_pre_k.
+    MidLaunch mid_pre;
+    int mid_pre_k = -1;
+    // The one-workgroup tail over T_{k-1} (2m entries) from round k, r =
+    // r_{k-1} or from the answer slot (from_slot: under sequence number want).
+    auto launch_tail = [&](const V2* in_f, const V2* in_g, size_t m2, int k, fp2 rr, unsigned seq0t,
+                           unsigned want, int from_slot) -> bool {
+        k_sc2_tail_host<<<1, kTailThreads, kTailLds, st>>>(
+            in_f, in_g, (unsigned)m2, k, nv, rr.re, rr.im, S.dev_vals, S.dev_seq, seq0t,
+            S.dev_answer, S.spin_ticks, S.timed ? S.dev_trace : nullptr, want, from_slot);
+        TP_HIP_OK(hipGetLastError());
+        return true;
+    };
+    // A tail launch queued behind the mid launch (untimed proves), for the
+    // loop's round tail_pre_k.
+    int tail_pre_k = -1;
+    unsigned tail_pre_seq0 = 0;
 
     // Launch bracket: events around one launch of class c (timing on).
     auto t_begin = [&]() -> bool {
@@ -2313,9 +2410,21 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             rk[3].im, of, og, pub_for(seq));
         if (!t_end(SC_FOLD01, 34.0 * (double)n)) return -1;
         TP_HIP_OK(hipGetLastError());
+        // an untimed prove queues the sliced mid launch (rounds from 5) now,
+        // taking r_4 from the answer slot: its launch latency then overlaps
+        // this pass and the round-4 transcript step (0.425-0.428 ms a prove
+        // against 0.436-0.447 ms launching it after that step)
+        const bool queue_mid = !S.timed && out > kTailIn && out <= (size_t)kMidGroups * kTailIn;
+        if (queue_mid) {
+            if (!launch_mid(of, og, out, zero2(), seq, 1, mid_pre)) return -1;
+            if (!t_end(SC_MID, 32.0 * (double)out + 64.0 * kMidGroups)) return -1;
+            mid_pre_k = 5;
+            guard.armed = true;   // it waits for r_4 until answered or aborted
+        }
         empty_blocks();   // while the fold runs
         if (wait_pub(st, seq, 3, false) != 1 || !t_collect(SC_FOLD01)) return -1;
         r = round_done(4, S.pub[0], S.pub[1], S.pub[2]);
+        if (queue_mid) answer(seq, &r, 1);
         cf = of;
         cg = og;
         k0 = 5;
@@ -2407,14 +2516,17 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             // per exchange; the host answers through the mapped answer slot
             const int nr = nv - k;
             const int ne = tail_exchanges(nr);
-            const unsigned seq0 = S.seq + 1;
-            S.seq += (unsigned)ne + 1;
-            if (!t_begin()) return -1;
-            k_sc2_tail_host<<<1, kTailThreads, kTailLds, st>>>(
-                cf, cg, (unsigned)(2 * m), k, nv, r.re, r.im, S.dev_vals, S.dev_seq, seq0,
-                S.dev_answer, S.spin_ticks, S.timed ? S.dev_trace : nullptr);
-            if (!t_end(SC_TAIL, 32.0 * (double)(2 * m))) return -1;
-            TP_HIP_OK(hipGetLastError());
+            unsigned seq0;
+            if (tail_pre_k == k) {
+                seq0 = tail_pre_seq0;   // queued already, taking r from the answer slot
+                tail_pre_k = -1;
+            } else {
+                seq0 = S.seq + 1;
+                S.seq += (unsigned)ne + 1;
+                if (!t_begin()) return -1;
+                if (!launch_tail(cf, cg, 2 * m, k, r, seq0, 0, 0)) return -1;
+                if (!t_end(SC_TAIL, 32.0 * (double)(2 * m))) return -1;
+            }
             volatile u64* ans = S.host_answer;
             double host_us = 0;
             for (int e = 0; e < ne; ++e) {
@@ -2460,57 +2572,38 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
             final4[0] = S.pub[0].re; final4[1] = S.pub[0].im;
             final4[2] = S.pub[1].re; final4[3] = S.pub[1].im;
             S.slots_clean = true;   // every streaming launch cleared its slots and ticket
+            guard.armed = false;    // a queued tail has had every answer
             return 0;
         }
         if (use_mid && 2 * m <= (size_t)kMidGroups * kTailIn) {
             // rounds k .. k + nr - 1 on slices in kMidGroups workgroups, down
             // to kMidEnd entries per workgroup; that table goes to the tail
-            const unsigned slice = (unsigned)(2 * m / kMidGroups);   // > kMidEnd here
-            int nr = 0;
-            while ((slice >> nr) > kMidEnd) ++nr;
-            V2* of = (cf >= A && cf < A + 2 * capA) ? B : A;
-            V2* og = of + (of == A ? capA : capB);
-            const unsigned seq0 = S.seq + 1;
-            S.seq += (unsigned)mid_exchanges(nr);
-            const V2* a_f = cf;
-            const V2* a_g = cg;
-            unsigned a_slice = slice;
-            int a_nr = nr;
-            u64 a_rr = r.re, a_ri = r.im;
-            u64* a_parts = S.mid_parts.as<u64>();
-            fp2* a_vals = S.dev_vals;
-            unsigned* a_seq = S.dev_seq;
-            unsigned a_seq0 = seq0;
-            u64* a_ans = S.dev_answer;
-            long long a_spin = S.spin_ticks;
-            V2* a_of = of;
-            V2* a_og = og;
-            u64* a_relay = S.relay.as<u64>();
-            long long* a_trace = S.timed && mid_exchanges(nr) <= 40 ? S.dev_mtrace : nullptr;
-            unsigned* a_gate = S.gate.as<unsigned>();
-            long long a_gate_ticks = S.gate_ticks;
-            unsigned a_want = 0;
-            int a_from = 0;
-            u64* a_arelay = S.arelay.as<u64>();
-            void* args[] = {&a_f, &a_g, &a_slice, &a_nr, &a_rr, &a_ri, &a_parts, &a_vals, &a_seq,
-                            &a_seq0, &a_ans, &a_spin, &a_of, &a_og, &a_relay, &a_trace,
-                            &a_gate, &a_gate_ticks, &a_want, &a_from, &a_arelay};
-            if (!S.mid_clean) {   // first mid launch, or the last one did not finish
-                k_sc2_mid_reset<<<1, 256, 0, st>>>(a_parts, a_relay, a_gate);
-                TP_HIP_OK(hipGetLastError());
+            MidLaunch ml;
+            if (mid_pre_k == k) {
+                ml = mid_pre;   // queued already, taking r from the answer slot
+                mid_pre_k = -1;
+            } else {
+                if (!t_begin()) return -1;
+                if (!launch_mid(cf, cg, 2 * m, r, 0, 0, ml)) return -1;
+                if (!t_end(SC_MID, 32.0 * (double)(2 * m) + 64.0 * kMidGroups)) return -1;
             }
-            S.mid_clean = false;
-            if (!t_begin()) return -1;
-            TP_HIP_OK(hipLaunchKernel((const void*)k_sc2_mid, dim3(kMidGroups), dim3(kTailThreads),
-                                      args, (unsigned)(slice * sizeof(V2)), st));
-            if (!t_end(SC_MID, 32.0 * (double)(2 * m) + 64.0 * kMidGroups)) return -1;
+            const unsigned seq0 = ml.seq0;
+            const int nr = ml.nr;
+            V2* of = ml.of;
+            V2* og = ml.og;
             const int first = wait_pub(st, seq0, mid_exchange_rounds(nr, 0) == 2 ? 9 : 3, true);
             if (first == 0) return -1;
             if (first == 2) {
                 // not every workgroup was resident: these rounds go to the
-                // streaming launches (the tables were only read)
+                // streaming launches (the tables were only read); an answer
+                // left for a queued launch goes (it was never taken)
                 TP_HIP_OK(hipEventRecord(S.ev_round, st));
                 if (!wait_event(S.ev_round)) return -1;
+                {
+                    volatile u64* ans = S.host_answer;
+                    ans[0] = ans[1] = ans[2] = ans[3] = kSlotEmpty;
+                }
+                guard.armed = false;
                 S.launches[SC_MID] -= 1;
                 S.bytes[SC_MID] -= 32.0 * (double)(2 * m) + 64.0 * kMidGroups;
                 S.seq += 1;   // past the abort mark
@@ -2519,7 +2612,25 @@ int tp_sc2_prove(const uint64_t* d_f, const uint64_t* d_g, int nv, uint64_t seed
                 --k;          // the loop's ++k returns to this round
                 continue;
             }
-            if (!mid_host(k, seq0, nr, true, r)) return -1;
+            // an untimed prove queues the tail now, behind the mid launch, to
+            // take the mid's last challenge from the answer slot (0.428-0.429 ms
+            // a prove against 0.433-0.439 ms launching it after that step)
+            const int tail_k = k + nr;
+            const bool queue_tail = !S.timed && (size_t)kMidGroups * kMidEnd <= kTailIn && tail_k <= nv &&
+                                    mid_exchange_rounds(nr, mid_exchanges(nr) - 1) == 1;
+            if (queue_tail) {
+                const int ne_t = tail_exchanges(nv - tail_k);
+                tail_pre_seq0 = S.seq + 1;
+                S.seq += (unsigned)ne_t + 1;
+                if (!launch_tail(of, og, (size_t)kMidGroups * kMidEnd, tail_k, zero2(), tail_pre_seq0,
+                                 seq0 + (unsigned)mid_exchanges(nr) - 1, 1))
+                    return -1;
+                if (!t_end(SC_TAIL, 32.0 * (double)kMidGroups * kMidEnd)) return -1;
+                tail_pre_k = tail_k;
+                guard.armed = true;   // it waits for the mid's last challenge
+            }
+            if (!mid_host(k, seq0, nr, true, r, queue_tail)) return -1;
+            if (!queue_tail) guard.armed = false;   // a queued mid launch has had its first challenge
             // Every round is published; the launch resets its words as it ends.
             // The next launches queue behind it on the stream, so the host
             // waits for its end only to read its timing (timing on).
