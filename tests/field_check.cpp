@@ -175,6 +175,21 @@ int main() {
         }
     }
     {
+        // three products summed with one reduction, against 128-bit sums,
+        // at the operand extremes and at random
+        auto check3 = [&](u64 a, u64 b, u64 c, u64 d, u64 e, u64 f) {
+            const u64 got = sum3_lazy(a, b, c, d, e, f);
+            const u128 want = ((u128)a * b + (u128)c * d + (u128)e * f) % P;
+            CHECK(got <= P + 6 && got % P == (u64)want, "sum3_lazy(%llx,%llx,%llx,%llx,%llx,%llx)",
+                  (unsigned long long)a, (unsigned long long)b, (unsigned long long)c,
+                  (unsigned long long)d, (unsigned long long)e, (unsigned long long)f);
+        };
+        const u64 ex[] = {0, 1, P - 1, P - 2, 0x7fffffffull, 0x80000000ull, (1ull << 60) - 1,
+                          0x1fffffff80000000ull, 0x1fffffff7fffffffull};
+        for (u64 a : ex) for (u64 b : ex) for (u64 c : ex) check3(a, b, c, b, a, c), check3(a, b, a, b, a, b);
+        for (int i = 0; i < 200000; ++i) check3(rnd() % P, rnd() % P, rnd() % P, rnd() % P, rnd() % P, rnd() % P);
+    }
+    {
         // rounds 0..3 from the block products against the rounds on the
         // tables themselves, folded entry by entry with exact products
         auto xmul = [](fp2 x, fp2 y) {

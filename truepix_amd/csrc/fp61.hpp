@@ -129,9 +129,38 @@ struct LazyMul {
     TP_HD u64 operator()(u64 a, u64 b) const { return fmul_lazy(a, b); }
 };
 
+// x0 y0 + x1 y1 + x2 y2 (mod p) as a value at most 2^61 + 6, for canonical
+// operands (below 2^61), with one reduction for the three products: each
+// operand split as h 2^31 + l (h < 2^30, l < 2^31), so
+//   x y = h_x h_y 2^62 + (h_x l_y + l_x h_y) 2^31 + l_x l_y
+// and, with 2^62 = 2 and m 2^31 = (m >> 30) + (m mod 2^30) 2^31 (mod p),
+// the three products' parts add up unreduced: L = sum l l < 3 2^62,
+// M = sum (h l + l h) < 6 2^61, H = sum h h < 3 2^60 (12 multiply-adds);
+// fold1(L) + 2 H + (M >> 30) + (M mod 2^30) 2^31 < 2^61 + 2 + 2^63 + 2^34 +
+// 2^61 < 2^64, folded once more.
+TP_HD u64 sum3_lazy(u64 x0, u64 y0, u64 x1, u64 y1, u64 x2, u64 y2)
+{
+    const u32 a0 = (u32)x0 & 0x7fffffffu, c0 = (u32)(x0 >> 31), b0 = (u32)y0 & 0x7fffffffu, d0 = (u32)(y0 >> 31);
+    const u32 a1 = (u32)x1 & 0x7fffffffu, c1 = (u32)(x1 >> 31), b1 = (u32)y1 & 0x7fffffffu, d1 = (u32)(y1 >> 31);
+    const u32 a2 = (u32)x2 & 0x7fffffffu, c2 = (u32)(x2 >> 31), b2 = (u32)y2 & 0x7fffffffu, d2 = (u32)(y2 >> 31);
+    const u64 L = (u64)a0 * b0 + (u64)a1 * b1 + (u64)a2 * b2;
+    const u64 M = (u64)c0 * b0 + (u64)a0 * d0 + (u64)c1 * b1 + (u64)a1 * d1 + (u64)c2 * b2 + (u64)a2 * d2;
+    const u64 H = (u64)c0 * d0 + (u64)c1 * d1 + (u64)c2 * d2;
+    return fold1(fold1(L) + 2 * H + (M >> 30) + ((M & ((1ULL << 30) - 1)) << 31));
+}
+
+// fold_quad_with's polynomial with each Karatsuba sum of three products
+// from sum3_lazy: the (a + b) operands of the third sum are reduced to
+// canonical first, so every operand is below 2^61.  Each of ac, bd and k is
+// at most 2^61 + 6, within the final bounds above.
 TP_HD fp2 fold_quad(fp2 p0, fp2 p1, fp2 p2, fp2 p3, fp2 r0, fp2 r1, fp2 r01)
 {
-    return fold_quad_with(p0, p1, p2, p3, r0, r1, r01, LazyMul());
+    const fp2 d1 = sub2(p1, p0), d2 = sub2(p2, p0), d3 = sub2(sub2(p3, p2), d1);
+    const u64 ac = sum3_lazy(r0.re, d1.re, r1.re, d2.re, r01.re, d3.re);
+    const u64 bd = sum3_lazy(r0.im, d1.im, r1.im, d2.im, r01.im, d3.im);
+    const u64 k = sum3_lazy(fadd(r0.re, r0.im), fadd(d1.re, d1.im), fadd(r1.re, r1.im),
+                            fadd(d2.re, d2.im), fadd(r01.re, r01.im), fadd(d3.re, d3.im));
+    return mk2(fold61(p0.re + ac + (2 * P - bd)), fold61(p0.im + k + (4 * P - ac - bd)));
 }
 
 // Operand of mul31: a < 2^62 as a = h 2^31 + l with l, h < 2^31.

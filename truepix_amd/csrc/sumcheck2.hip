@@ -1077,10 +1077,12 @@ __global__ void __launch_bounds__(256) k_sc2_blocks_reduce(const fp2* __restrict
 // r_0, r_1 into T_2 entry 64 s + l; the four T_2 entries of each lane then
 // go through the slice again so that lane j reads entries 4j .. 4j + 3 and
 // folds them by r_2, r_3 into output j.  The sweep runs from the top down.
-// The pass is bound by its integer multiplies, not by HBM: per wave and 64
-// outputs (2048 input entries) its loop issues 3078 VALU instructions, 474
-// of them 64-bit multiply-adds, about 120 us of issue for 2 x 2^24 entries,
-// which is what it takes.  So these measured the same or slower: issuing
+// The pass is bound by its integer arithmetic, not by HBM: per wave and 64
+// outputs (2048 input entries) its loop issued 3078 VALU instructions, 474
+// of them 64-bit multiply-adds, with a reduction per product; summing each
+// Karatsuba sum's three products before one reduction (sum3_lazy) took it to
+// 2327 and the pass from 119-122 to 112-118 us.  These measured the same or
+// slower (before that change): issuing
 // each sub-chunk's loads one step ahead (127 VGPRs: 127-132 us against
 // 121-123 us), 8-wave workgroups, and 4, 5 or 6 waves per SIMD (6 spills).
 // Workgroups of 4 waves; the last one's ticket gather reuses the slices.
