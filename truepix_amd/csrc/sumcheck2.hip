@@ -1081,8 +1081,9 @@ __global__ void __launch_bounds__(256) k_sc2_blocks_reduce(const fp2* __restrict
 // outputs (2048 input entries) its loop issued 3078 VALU instructions, 474
 // of them 64-bit multiply-adds, with a reduction per product; summing each
 // Karatsuba sum's three products before one reduction (sum3_lazy) took it to
-// 2327 and the pass from 119-122 to 112-118 us.  These measured the same or
-// slower (before that change): issuing
+// 2327 and the pass from 119-122 to 112-118 us (79 VGPRs, 6 waves a SIMD;
+// forced to 7 waves it spills and ran 118-120 against 115-117 us).  These
+// measured the same or slower (before that change): issuing
 // each sub-chunk's loads one step ahead (127 VGPRs: 127-132 us against
 // 121-123 us), 8-wave workgroups, and 4, 5 or 6 waves per SIMD (6 spills).
 // Workgroups of 4 waves; the last one's ticket gather reuses the slices.
