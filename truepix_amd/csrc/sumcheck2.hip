@@ -481,6 +481,13 @@ __device__ __forceinline__ mf_v2i lds_tr8(const uint8_t* p)
 // A signed int32 as its residue mod p.
 __device__ __forceinline__ u64 mod_i32(int v) { return v < 0 ? P - (u64)(-(long long)v) : (u64)v; }
 
+// x 2^e (mod p) for canonical x and 0 <= e <= 60, as a value below
+// 1.5 x 2^61: x 2^e = (x >> (61 - e)) 2^61 + ((x << e) mod 2^61), and
+// 2^61 = 1 (mod p) - a rotation of the 61-bit word, no multiply.  (In the
+// block pass's closing weights it replaced a multiply per tile entry; the
+// pass measured the same, its closing being a small part of it.)
+__device__ __forceinline__ u64 rot61(u64 x, int e) { return ((x << e) & P) + (x >> (61 - e)); }
+
 __global__ void __launch_bounds__(kMfThreads) k_sc2_first01_mfma(const V2* __restrict__ f,
                                                                 const V2* __restrict__ g,
                                                                 size_t n_quads, TicketPub pub)
@@ -996,29 +1003,29 @@ __global__ void __launch_bounds__(kBkThreads) k_sc2_blocks_mfma(const V2* __rest
     atomicAdd(&Us[(a0 + ua) * 16 + m], ucol);
     atomicAdd(&Vs[(b0 + vb) * 16 + m], vcol);
     __syncthreads();
+    const u64 k1 = 0x0101010101010101ull;                           // sum_{i<8} 2^{8i}, below p
+    const u64 c128 = fmul(128, k1);                                 // 128 K1
+    const u64 cnn = fmul(fmul(c128, c128), fold61((u64)nb * 64));   // 128^2 K1^2 N
     if (t < 64) {
-        // t < 32: U of entry t / 2, word t % 2; then V: sum_i 2^{8i} U[8 h + i]
+        // t < 32: U of entry t / 2, word t % 2; then V: 128 K1 sum_i 2^{8i} U[8 h + i]
         const int* src = (t < 32 ? Us : Vs) + ((t >> 1) & 15) * 16 + 8 * (t & 1);
         u64 wsum = 0;
-        for (int i = 0; i < 8; ++i) wsum += fmul_lazy(mod_i32(src[i]), 1ull << (8 * i));
-        Ws[t] = fold61(wsum);
+        for (int i = 0; i < 8; ++i) wsum = fold1(wsum + rot61(mod_i32(src[i]), 8 * i));   // < 2^64
+        Ws[t] = fmul(c128, fold61(wsum));
     }
     __syncthreads();
     // lane l holds C[4 (l / 16) + r][l % 16]: word pair (hf, hg) = (l / 32,
     // (l / 8) % 2), bytes (bi, bj) = ((4 (l / 16) + r) % 8, l % 8)
-    const u64 k1 = 0x0101010101010101ull;                           // sum_{i<8} 2^{8i}, below p
-    const u64 c128 = fmul(128, k1);                                 // 128 K1
-    const u64 cnn = fmul(fmul(c128, c128), fold61((u64)nb * 64));   // 128^2 K1^2 N
     const int hf = lane >> 5, hg = (lane >> 3) & 1, bj = lane & 7;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            u64 w[1] = {0};
+            u64 w[1] = {0};   // four rotations, each below 1.5 x 2^61
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int bi = (4 * (lane >> 4) + r) & 7;
-                w[0] += fmul_lazy(mod_i32(acc[a][b][r]), 1ull << ((8 * (bi + bj)) % 61));
+                w[0] += rot61(mod_i32(acc[a][b][r]), (8 * (bi + bj)) % 61);
             }
             w[0] = fold1(w[0]);
             // the 16 lanes of each word pair: the 8 of a half row, then the
@@ -1031,7 +1038,7 @@ __global__ void __launch_bounds__(kBkThreads) k_sc2_blocks_mfma(const V2* __rest
             w[0] += __shfl_xor(w[0], 16, 64);
             const u64 q = fold61(w[0]);
             // Q[hf][hg] with the byte offsets restored
-            const u64 qt = fadd(fadd(q, fmul(c128, fadd(Ws[(a0 + a) * 2 + hf], Ws[32 + (b0 + b) * 2 + hg]))), cnn);
+            const u64 qt = fadd(fadd(q, fadd(Ws[(a0 + a) * 2 + hf], Ws[32 + (b0 + b) * 2 + hg])), cnn);
             const u64 q00 = __shfl(qt, 0, 64), q01 = __shfl(qt, 8, 64);
             const u64 q10 = __shfl(qt, 32, 64), q11 = __shfl(qt, 40, 64);
             if (lane == 0) Ps[(a0 + a) * 16 + b0 + b] = mk2(fsub(q00, q11), fadd(q01, q10));
